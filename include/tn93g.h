@@ -1,0 +1,130 @@
+/* tn93g.h -- C ABI of the MI355X all-pairs TN93 engine (libtn93g.so).
+ *
+ * The engine computes, on one gfx950 GPU, exactly what tn93 v1.1.0's standard all-pairs mode computes: for every
+ * pair of aligned nucleotide sequences the TN93 distance (byte-identical to single-thread tn93 once printed with
+ * %g), the 200-bin histogram(s), the links at or below the threshold, and the per-row sums that tn93 reduces in row
+ * order.  Plain pointers and sizes only; no C++ or GPU types cross this boundary.
+ *
+ * It replaces the reference's inner boundary (paths relative to the reference repository):
+ *   Engine::prepare(const SeqSet&, long L, long n1, int mode, double rf, long overlap)   swifttn93.cpp:570
+ *   the per (row, 8-column block) kernels + phaseA + replay + tail8a/b + phaseC           swifttn93.cpp:2610-2806
+ *   ref::computeTN93(s1, s2, L, matchMode, NULL, min_overlap, histogram, 200, 200,
+ *                    count1, count2, d1, d2, 0.0)                                         refkernel.h:58-62
+ * The process interface above it (tn93's command line, argparse.cpp:138-226) is the CLI tn93g_cli.cpp.
+ *
+ * Conventions: every function returns 0 on success and a negative TN93G_E* code on failure; the library never
+ * exits.  One host thread drives a context.  Codes are the reference's ValidChars indices
+ * ("ACGTURYSWKMBDHVN?-" -> 0..17, refkernel.cpp:9), one byte per site.
+ */
+#ifndef TN93G_H
+#define TN93G_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TN93G_OK 0
+#define TN93G_E_ARG (-1)    /* bad argument */
+#define TN93G_E_HIP (-2)    /* HIP runtime error (message in tn93g_error) */
+#define TN93G_E_NOMEM (-3)  /* device or host allocation failed */
+#define TN93G_E_NODEV (-4)  /* no usable gfx950 device */
+
+/* ambiguity modes (refkernel.h:14-18) */
+#define TN93G_RESOLVE 0
+#define TN93G_AVERAGE 1
+#define TN93G_SKIP 2
+#define TN93G_GAPMM 3
+#define TN93G_SUBSET 4
+
+/* pair sets */
+#define TN93G_TRIANGLE 0  /* single file: pairs i < j over the set (swifttn93.cpp:2392)            */
+#define TN93G_CROSS 1     /* -s without -m: rows [0, n1) x columns [n1, n) (swifttn93.cpp:2321-2322) */
+#define TN93G_FST 2       /* -s -m: triangle over both files, three histograms (swifttn93.cpp:2680-2686) */
+
+/* backends of the clean/clean site counts */
+#define TN93G_BACKEND_AUTO 0
+#define TN93G_BACKEND_POPC 1 /* VALU popcount over bit planes */
+#define TN93G_BACKEND_MFMA 2 /* int8 one-hot contraction on MFMA, int32 accumulation */
+
+/* run flags */
+#define TN93G_FLAG_EXACT 1u      /* resolve every uncertain log on the host (libm) instead of bracketing it   */
+#define TN93G_FLAG_ALL_SLOW 2u   /* route every pair through the site-order replay kernel (tests)            */
+
+typedef struct tn93g_ctx tn93g_ctx;
+typedef struct tn93g_set tn93g_set;
+
+/* Opens device `device` (HIP ordinal).  Replaces nothing in the reference (which is CPU-only); it owns the stream,
+ * the device buffers reused across runs and the last error message. */
+int tn93g_open(int device, tn93g_ctx **out);
+void tn93g_close(tn93g_ctx *ctx);
+/* Last error message of the context (never NULL). */
+const char *tn93g_error(const tn93g_ctx *ctx);
+
+/* Copies n sequences of L codes (row-major, n*L bytes, values 0..17) to the device and packs them (bit planes,
+ * site-major codes, per-sequence ambiguity event lists).  Replaces Engine::prepare (swifttn93.cpp:570-1044).
+ * The host keeps ownership of `codes`.  weights: n copy numbers (swifttn93.cpp:324-333), or NULL for all 1. */
+int tn93g_set_upload(tn93g_ctx *ctx, const uint8_t *codes, int64_t n, int64_t L, const int64_t *weights,
+                     tn93g_set **out);
+void tn93g_set_free(tn93g_set *set);
+
+typedef struct {
+  int32_t mode;            /* TN93G_RESOLVE .. TN93G_SUBSET                                     */
+  int32_t kind;            /* TN93G_TRIANGLE, TN93G_CROSS or TN93G_FST                          */
+  int64_t n1;              /* sequences of the first file (CROSS, FST); ignored for TRIANGLE    */
+  uint32_t subset_mask;    /* SUBSET: bit c set when code c (4..16) is resolved (-a RY ...)     */
+  int32_t backend;         /* TN93G_BACKEND_*                                                   */
+  double resolve_fraction; /* -g (1.0 = off)                                                    */
+  int64_t min_overlap;     /* -l, as tn93 casts it to long                                      */
+  double threshold;        /* -t: links have min_distance <= d <= threshold                    */
+  double min_distance;     /* -w (default -1e-10)                                               */
+  uint32_t flags;          /* TN93G_FLAG_*                                                      */
+} tn93g_params;
+
+/* Result of one run.  All arrays are owned by the result.  Everything tn93 prints is here:
+ *   hist[b][k]           the histograms (b = 0 single set; FST: 0 file 1, 1 file 2, 2 between), u64 wrap-around;
+ *   links                (row, col, dist) in tn93's emission order (row-major, columns ascending); col is the
+ *                        global sequence index (the second file continues the numbering, as csvn prints it);
+ *   mean, weighted, max  tn93's row-order reductions (swifttn93.cpp:2839-2848) of the per-row sums;
+ *   found_links          sum of link weights; skipped = pairs with overlap <= min_overlap.
+ * Floating values that depend on a GPU logarithm are exact when `exact` is 1.  Otherwise each comes as an
+ * enclosure [lo, hi] that contains tn93's value (lo == hi when it is known exactly); link distances are then
+ * certified only to tn93's printed precision (%g, 6 digits: link_dist_lo/hi give the same string).  A caller
+ * that prints with another precision and finds lo and hi print differently calls tn93g_run again with
+ * TN93G_FLAG_EXACT. */
+typedef struct {
+  int32_t exact;
+  int32_t nrows;
+  uint64_t hist[3][200];
+  int64_t n_links;
+  int64_t *link_row, *link_col;
+  double *link_dist, *link_dist_lo, *link_dist_hi;
+  double mean_lo[3], mean_hi[3];
+  double weighted[3];
+  double max_lo[3], max_hi[3];
+  int64_t found_links;
+  int64_t skipped;
+  /* diagnostics */
+  int64_t n_replayed;   /* pairs that went through the site-order replay kernel        */
+  int64_t n_uncertain;  /* pairs whose distance is an enclosure (lo < hi)               */
+  int64_t n_host_exact; /* pairs whose logarithms were resolved on the host (EXACT)     */
+  double gpu_ms;        /* device time of the run's kernels (HIP events)                 */
+  double pair_kernel_ms;/* device time of the clean/clean counting + tail kernel         */
+} tn93g_result;
+
+/* Computes all pairs of `set` described by `params` (set must hold n1 + n2 sequences for CROSS/FST).
+ * Replaces the OpenMP tile loop of swifttn93's main (swifttn93.cpp:2339-2864).  *out is allocated by the library
+ * and released with tn93g_result_free. */
+int tn93g_run(tn93g_ctx *ctx, const tn93g_set *set, const tn93g_params *params, tn93g_result **out);
+void tn93g_result_free(tn93g_result *res);
+
+/* Library version string. */
+const char *tn93g_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TN93G_H */

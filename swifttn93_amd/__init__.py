@@ -1,0 +1,144 @@
+"""swifttn93_amd: all-pairs TN93 distances (tn93 v1.1.0 standard mode) on one MI355X.
+
+The product is the native library swifttn93_amd/lib/libtn93g.so (C ABI in include/tn93g.h) and the tn93-compatible
+CLI swifttn93_amd/bin/tn93g.  This module is a thin ctypes binding for tests and benchmarks; it has no compute
+fallback: without the built library every call raises.
+"""
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "lib" / "libtn93g.so"
+CLI_PATH = PKG / "bin" / "tn93g"
+
+RESOLVE, AVERAGE, SKIP, GAPMM, SUBSET = 0, 1, 2, 3, 4
+TRIANGLE, CROSS, FST = 0, 1, 2
+BACKEND_AUTO, BACKEND_POPC, BACKEND_MFMA = 0, 1, 2
+FLAG_EXACT, FLAG_ALL_SLOW = 1, 2
+
+MODES = {"resolve": RESOLVE, "average": AVERAGE, "skip": SKIP, "gapmm": GAPMM}
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("kind", ctypes.c_int32), ("n1", ctypes.c_int64),
+                ("subset_mask", ctypes.c_uint32), ("backend", ctypes.c_int32),
+                ("resolve_fraction", ctypes.c_double), ("min_overlap", ctypes.c_int64),
+                ("threshold", ctypes.c_double), ("min_distance", ctypes.c_double), ("flags", ctypes.c_uint32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("exact", ctypes.c_int32), ("nrows", ctypes.c_int32), ("hist", (ctypes.c_uint64 * 200) * 3),
+                ("n_links", ctypes.c_int64), ("link_row", ctypes.POINTER(ctypes.c_int64)),
+                ("link_col", ctypes.POINTER(ctypes.c_int64)), ("link_dist", ctypes.POINTER(ctypes.c_double)),
+                ("link_dist_lo", ctypes.POINTER(ctypes.c_double)), ("link_dist_hi", ctypes.POINTER(ctypes.c_double)),
+                ("mean_lo", ctypes.c_double * 3), ("mean_hi", ctypes.c_double * 3), ("weighted", ctypes.c_double * 3),
+                ("max_lo", ctypes.c_double * 3), ("max_hi", ctypes.c_double * 3), ("found_links", ctypes.c_int64),
+                ("skipped", ctypes.c_int64), ("n_replayed", ctypes.c_int64), ("n_uncertain", ctypes.c_int64),
+                ("n_host_exact", ctypes.c_int64), ("gpu_ms", ctypes.c_double), ("pair_kernel_ms", ctypes.c_double)]
+
+
+EXPORTS = ["tn93g_open", "tn93g_close", "tn93g_error", "tn93g_set_upload", "tn93g_set_free", "tn93g_run",
+           "tn93g_result_free", "tn93g_version"]
+
+_lib = None
+
+
+def load():
+    """Loads libtn93g.so (raises if it has not been built: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python3 -m swifttn93_amd.build`")
+    lib = ctypes.CDLL(str(LIB_PATH))
+    lib.tn93g_open.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    lib.tn93g_close.argtypes = [ctypes.c_void_p]
+    lib.tn93g_error.argtypes = [ctypes.c_void_p]
+    lib.tn93g_error.restype = ctypes.c_char_p
+    lib.tn93g_set_upload.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
+                                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
+    lib.tn93g_set_free.argtypes = [ctypes.c_void_p]
+    lib.tn93g_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Params),
+                              ctypes.POINTER(ctypes.POINTER(Result))]
+    lib.tn93g_result_free.argtypes = [ctypes.POINTER(Result)]
+    lib.tn93g_version.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+class Engine:
+    """One GPU context (tn93g_open)."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        self.ctx = ctypes.c_void_p()
+        rc = self.lib.tn93g_open(device, ctypes.byref(self.ctx))
+        if rc:
+            raise RuntimeError(f"tn93g_open failed ({rc}): no usable gfx950 device")
+
+    def _check(self, rc):
+        if rc:
+            raise RuntimeError(f"tn93g error {rc}: {self.lib.tn93g_error(self.ctx).decode()}")
+
+    def upload(self, codes, weights=None):
+        codes = np.ascontiguousarray(codes, dtype=np.uint8)
+        n, L = codes.shape
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.int64)
+        s = ctypes.c_void_p()
+        self._check(self.lib.tn93g_set_upload(self.ctx, codes.ctypes.data, n, L,
+                                              None if w is None else w.ctypes.data, ctypes.byref(s)))
+        return SeqSet(self, s, n, L)
+
+    def run(self, seqset, mode=RESOLVE, kind=TRIANGLE, n1=0, subset_mask=0, backend=BACKEND_AUTO,
+            resolve_fraction=1.0, min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0):
+        p = Params(mode, kind, n1, subset_mask, backend, resolve_fraction, min_overlap, threshold, min_distance, flags)
+        r = ctypes.POINTER(Result)()
+        self._check(self.lib.tn93g_run(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r)))
+        try:
+            return _to_py(r.contents)
+        finally:
+            self.lib.tn93g_result_free(r)
+
+    def close(self):
+        if self.ctx:
+            self.lib.tn93g_close(self.ctx)
+            self.ctx = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SeqSet:
+    def __init__(self, eng, handle, n, L):
+        self.eng, self.handle, self.n, self.L = eng, handle, n, L
+
+    def free(self):
+        if self.handle:
+            self.eng.lib.tn93g_set_free(self.handle)
+            self.handle = None
+
+
+def _arr(ptr, n, dt):
+    if n == 0:
+        return np.zeros(0, dtype=dt)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+
+
+def _to_py(r):
+    n = r.n_links
+    return {
+        "exact": bool(r.exact),
+        "hist": np.array([[r.hist[b][k] for k in range(200)] for b in range(3)], dtype=np.uint64),
+        "link_row": _arr(r.link_row, n, np.int64), "link_col": _arr(r.link_col, n, np.int64),
+        "link_dist": _arr(r.link_dist, n, np.float64), "link_lo": _arr(r.link_dist_lo, n, np.float64),
+        "link_hi": _arr(r.link_dist_hi, n, np.float64),
+        "mean_lo": list(r.mean_lo), "mean_hi": list(r.mean_hi), "weighted": list(r.weighted),
+        "max_lo": list(r.max_lo), "max_hi": list(r.max_hi), "found_links": r.found_links, "skipped": r.skipped,
+        "n_replayed": r.n_replayed, "n_uncertain": r.n_uncertain, "n_host_exact": r.n_host_exact,
+        "gpu_ms": r.gpu_ms, "pair_kernel_ms": r.pair_kernel_ms,
+    }

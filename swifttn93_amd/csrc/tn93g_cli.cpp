@@ -1,0 +1,616 @@
+// tn93g_cli.cpp -- tn93 v1.1.0-compatible command line over libtn93g (include/tn93g.h).
+//
+// Same command line (argparse.cpp:138-445), same reader (swifttn93.cpp:152-406), same distance file and JSON summary
+// (swifttn93.cpp:2222-2966) as single-thread tn93; the pair computation runs on the GPU through tn93g_run.
+// Printed values that depend on a GPU logarithm are certified from the engine's enclosures: every printed string
+// must be the same at both ends of the enclosure, otherwise the run is repeated with TN93G_FLAG_EXACT (logarithms
+// resolved with the host libm).
+#include <cctype>
+#include <cmath>
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <iomanip>
+#include <iostream>
+#include <fstream>
+#include <unistd.h>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/tn93g.h"
+
+namespace {
+
+enum { F_CSV = 0, F_CSVN = 1, F_HYPHY = 2 };
+constexpr int RESOLVE = 0, AVERAGE = 1, SKIP = 2, GAPMM = 3, SUBSET = 4;
+const char ALPHA[] = "ACGTURYSWKMBDHVN?-";
+
+// tn93 ends the process on every error; here errors unwind to main (or to the batch driver) as Exit
+struct Exit {
+  int code;
+};
+
+struct Args {
+  FILE *output = stdout, *input1 = stdin, *input2 = nullptr;
+  ~Args()
+  {
+    if (output && output != stdout) fclose(output);
+    if (input1 && input1 != stdin) fclose(input1);
+    if (input2 && input2 != stdin) fclose(input2);
+  }
+  double distance = 0.015, min_distance = -1.e-10, include_prob = 1.0, resolve_fraction = 1.0;
+  int ambig = RESOLVE, format = F_CSV;
+  unsigned long overlap = 100;
+  bool do_bootstrap = false, do_bootstrap_two_files = false, do_count = false, quiet = false, do_fst = false,
+       skip_header = false, report_self = false, hamming_skip = false, subquadratic = false;
+  char counts_in_name = ':', delimiter = ',';
+  const char *ambigs_to_resolve = nullptr;
+};
+
+const char USAGE[] =
+    "usage: tn93 [-h] [-v] [-o OUTPUT] [-t THRESHOLD] [-w MIN_THRESHOLD] [-a AMBIGS] [-g FRACTION] [-l OVERLAP] "
+    "[-d COUNTS_IN_NAME] [-f FORMAT] [-s SECOND_FASTA] [-b] [-r] [-c] [-n] [-0] [-q] [-H] [-S] [-D DELIMITER]"
+    "[FASTA]\n";
+
+[[noreturn]] void arg_error(const char *f, ...)  // argparse.cpp:118-128
+{
+  va_list ap;
+  fprintf(stderr, "%stn93: error: ", USAGE);
+  va_start(ap, f);
+  vfprintf(stderr, f, ap);
+  va_end(ap);
+  fprintf(stderr, "\n");
+  throw Exit{1};
+}
+
+const char *next_arg(int &i, int argc, const char **argv)
+{
+  if (++i == argc) arg_error("ran out of command line arguments");
+  return argv[i];
+}
+
+// tn93's grammar (argparse.cpp:138-226): options are told apart by their first character; the FASTA file must be
+// the last argument
+void parse_args(Args &A, int argc, const char **argv)
+{
+  for (int i = 1; i < argc; ++i) {
+    const char *arg = argv[i];
+    if (arg[0] == '-' && arg[1] == '-') {
+      if (!strcmp(arg + 2, "help")) {
+        fprintf(stderr, "%s\ncompute Tamura Nei 93 distances between aligned sequences (tn93g, gfx950)\n", USAGE);
+        throw Exit{1};
+      } else if (!strcmp(arg + 2, "version")) {
+        fprintf(stderr, "v1.1.0\n");
+        throw Exit{0};
+      } else if (!strcmp(arg + 2, "hamming-skip"))
+        A.hamming_skip = true;
+      else if (!strcmp(arg + 2, "subquadratic"))
+        A.subquadratic = true;
+      else
+        arg_error("unknown argument: %s", arg);
+      continue;
+    }
+    if (arg[0] != '-') {
+      if (i != argc - 1) arg_error("unknown argument: %s", arg);
+      A.input1 = strcmp(arg, "-") ? fopen(arg, "rb") : stdin;
+      if (!A.input1) arg_error("failed to open the INPUT file %s", arg);
+      if (A.input1 == A.input2) arg_error("input FASTA files must not both be stdin %s", arg);
+      continue;
+    }
+    const char o = arg[1];
+    if (!strcmp(arg + 1, "h")) {
+      fprintf(stderr, "%s\ncompute Tamura Nei 93 distances between aligned sequences (tn93g, gfx950)\n", USAGE);
+      throw Exit{1};
+    } else if (o == 'v') {
+      fprintf(stderr, "v1.1.0\n");
+      throw Exit{0};
+    } else if (o == 'o') {
+      const char *s = next_arg(i, argc, argv);
+      A.output = strcmp(s, "-") ? fopen(s, "wb") : stdout;
+      if (!A.output) arg_error("failed to open the OUTPUT file %s", s);
+    } else if (o == 't') {
+      const char *s = next_arg(i, argc, argv);
+      A.distance = atof(s);
+      if (A.distance < 0.0) arg_error("genetic distance threshold must be >= 0, had: %s", s);
+    } else if (o == 'w') {
+      const char *s = next_arg(i, argc, argv);
+      A.min_distance = atof(s);
+      if (A.min_distance < 0.0 || A.min_distance > 1.0)
+        arg_error("genetic minimum distance threshold must be >= 0, had: %s", s);
+    } else if (o == 'l') {
+      const char *s = next_arg(i, argc, argv);
+      A.overlap = (unsigned long)(long)atoi(s);
+      if (A.overlap == 0) arg_error("overlap must be positive, had: %s", s);
+    } else if (o == 'f') {
+      const char *s = next_arg(i, argc, argv);
+      if (!strcmp(s, "csv")) A.format = F_CSV;
+      else if (!strcmp(s, "csvn")) A.format = F_CSVN;
+      else if (!strcmp(s, "hyphy")) A.format = F_HYPHY;
+      else arg_error("invalid output format: %s", s);
+    } else if (o == 'a') {
+      const char *s = next_arg(i, argc, argv);
+      if (!strcmp(s, "resolve")) A.ambig = RESOLVE;
+      else if (!strcmp(s, "average")) A.ambig = AVERAGE;
+      else if (!strcmp(s, "skip")) A.ambig = SKIP;
+      else if (!strcmp(s, "gapmm")) A.ambig = GAPMM;
+      else {
+        A.ambig = SUBSET;
+        A.ambigs_to_resolve = s;
+      }
+    } else if (o == 's') {
+      const char *s = next_arg(i, argc, argv);
+      A.input2 = strcmp(s, "-") ? fopen(s, "rb") : stdin;
+      if (!A.input2) arg_error("failed to open the second INPUT file %s", s);
+      if (A.input1 == A.input2) arg_error("input FASTA files must not both be stdin %s", s);
+    } else if (o == 'd') {
+      const char *s = next_arg(i, argc, argv);
+      A.counts_in_name = s[0];
+      if (!isprint((unsigned char)A.counts_in_name))
+        arg_error("count separator must be a printable character, had: %s", s);
+    } else if (o == 'u') {
+      const char *s = next_arg(i, argc, argv);
+      A.include_prob = atof(s);
+      if (A.include_prob < 0.0 || A.include_prob > 1.0)
+        arg_error("sequence inclusion probability must be >= 0, had: %s", s);
+    } else if (o == 'D') {
+      const char *s = next_arg(i, argc, argv);
+      if (strlen(s) != 1) arg_error("invalid output format: %s", s);
+      A.delimiter = s[0];
+    } else if (o == 'b') A.do_bootstrap = true;
+    else if (o == 'r') A.do_bootstrap_two_files = true;
+    else if (o == 'c') A.do_count = true;
+    else if (o == 'n') A.skip_header = true;
+    else if (o == 'q') A.quiet = true;
+    else if (o == 'H') A.hamming_skip = true;
+    else if (o == 'S') A.subquadratic = true;
+    else if (o == 'm') A.do_fst = true;
+    else if (o == '0') A.report_self = true;
+    else if (o == 'g') {
+      const char *s = next_arg(i, argc, argv);
+      A.resolve_fraction = atof(s);
+      if (A.resolve_fraction < 0.0 || A.resolve_fraction > 1.0)
+        arg_error("resolve ambigous fraction must be >= 0, had: %s", s);
+    } else
+      arg_error("unknown argument: %s", arg);
+  }
+}
+
+struct Seqs {
+  std::vector<uint8_t> codes;
+  std::vector<std::string> names;
+  std::vector<int64_t> counts;
+  long N = 0;
+};
+
+signed char g_tab[256];
+
+void add_sequence(Seqs &S, size_t begin, long &firstLen)  // swifttn93.cpp:259-273
+{
+  long len = (long)(S.codes.size() - begin);
+  if (S.N == 0) {
+    firstLen = len;
+    if (firstLen <= 0) {
+      std::cerr << "First sequence length must be positive." << std::endl;
+      throw Exit{1};
+    }
+  } else if (len != firstLen) {
+    std::cerr << "All sequences must have the same length (" << firstLen << "), but sequence '" << S.names.back()
+              << "' had length " << len;
+    throw Exit{1};
+  }
+  S.N++;
+}
+
+// readFASTA (swifttn93.cpp:292-406)
+int read_fasta(FILE *F, Seqs &S, long &firstLen, char sep)
+{
+  std::vector<unsigned char> buf;
+  {
+    std::vector<unsigned char> tmp(1 << 20);
+    size_t r;
+    while ((r = fread(tmp.data(), 1, tmp.size(), F)) > 0) buf.insert(buf.end(), tmp.data(), tmp.data() + r);
+  }
+  S.codes.reserve(S.codes.size() + buf.size());
+  const unsigned char *p = buf.data(), *end = p + buf.size();
+  int state = 0;
+  size_t seq_begin = S.codes.size();
+  std::string name;
+  while (p < end) {
+    if (state == 0) {
+      while (p < end && *p != '>' && *p != '#') p++;
+      if (p == end) break;
+      p++;
+      state = 1;
+      name.clear();
+      continue;
+    }
+    if (state == 1) {
+      const unsigned char *q = p;
+      while (q < end && *q != '\n' && *q != '\r') q++;
+      name.append((const char *)p, (size_t)(q - p));
+      if (q == end) {
+        p = q;
+        break;
+      }
+      p = q + 1;
+      const long len = (long)name.size();
+      if (len <= 0) {
+        std::cerr << "Sequence names must be non-empty." << std::endl;
+        return 1;
+      }
+      unsigned long count = 1UL;
+      if (len >= 3) {
+        long k;
+        for (k = 2; k < len; k++)
+          if (name[(size_t)(len - k)] == sep) break;
+        if (k < len) count = (unsigned long)(long)atoi(name.c_str() + (len - k + 1));
+        if (count < 1) count = 1;
+      }
+      S.counts.push_back((int64_t)(long)count);
+      S.names.push_back(name);
+      state = 2;
+      seq_begin = S.codes.size();
+      continue;
+    }
+    while (p < end) {
+      const signed char v = g_tab[*p];
+      if (v < 0) break;
+      S.codes.push_back((uint8_t)v);
+      p++;
+    }
+    if (p == end) break;
+    const unsigned char c = *p++;
+    if (c == '>' || c == '#') {
+      add_sequence(S, seq_begin, firstLen);
+      state = 1;
+      name.clear();
+    }
+  }
+  if (state == 2) {
+    add_sequence(S, seq_begin, firstLen);
+    return 0;
+  }
+  std::cerr << "Unexpected end of file: state " << state << std::endl;
+  return 1;
+}
+
+int fmt_g(char *out, double d) { return snprintf(out, 64, "%g", d); }
+
+void dump_histogram(std::ostream &os, const char *tag, const uint64_t *h)  // swifttn93.cpp:60-74
+{
+  if (tag)
+    os << "\t\"Histogram " << tag << "\" : [";
+  else
+    os << "\t\"Histogram\" : [";
+  for (unsigned long k = 0; k < 200; k++) {
+    if (k) os << ',';
+    os << '[' << (k + 1.) / 200 << ',' << (unsigned long)h[k] << ']';
+  }
+  os << "]" << std::endl;
+}
+
+struct Summary {
+  unsigned long pairwise, n1, n2, N;
+  long found, skipped;
+  double weighted[3], max0, mean[3];
+  const uint64_t *hist;  // [3][200]
+};
+
+// The JSON summary (swifttn93.cpp:2914-2958), written to `os` with the stream's current precision.
+void write_json(std::ostream &os, const Summary &s, bool two_files, bool do_fst, uint64_t hist0_combined[200])
+{
+  os << "{" << std::endl;
+  os << '\t' << "\"Actual comparisons performed\" :" << s.pairwise - (unsigned long)s.skipped << ',' << std::endl;
+  os << "\t\"Comparisons accounting for copy numbers \" :" << (s.weighted[0] + s.weighted[1] + s.weighted[2]) << ','
+     << std::endl;
+  os << "\t\"Total comparisons possible\" : " << s.pairwise << ',' << std::endl;
+  os << "\t\"Links found\" : " << s.found << ',' << std::endl;
+  os << "\t\"Maximum distance\" : " << s.max0 << ',' << std::endl;
+  if (!two_files)
+    os << "\t\"Sequences\" : " << s.N << ',' << std::endl;
+  else {
+    os << "\t\"Sequences in first file\" : " << s.n1 << ',' << std::endl;
+    os << "\t\"Sequences in second file\" : " << s.n2 << ',' << std::endl;
+  }
+  if (do_fst) {
+    const char *keys[4] = {"File 1", "File 2", "Between", "Combined"};
+    const double *mean = s.mean, *wc = s.weighted;
+    for (int k = 0; k < 3; k++) os << "\t\"Mean distance " << keys[k] << "\" : " << mean[k] / wc[k] << ',' << std::endl;
+    double meta = (mean[0] + mean[1] + mean[2]) / (wc[0] + wc[1] + wc[2]),
+           intra = (mean[0] + mean[1]) / (wc[0] + wc[1]), pi_D = mean[2] / wc[2] - intra;
+    os << "\t\"Mean distance [intra] " << "\" : " << intra << ',' << std::endl;
+    os << "\t\"Mean distance [combined]" << "\" : " << meta << ',' << std::endl;
+    os << "\t\"F_ST\" : " << pi_D / (intra + pi_D) << ',' << std::endl;
+    for (int k = 0; k < 3; k++) {
+      dump_histogram(os, keys[k], s.hist + 200 * k);
+      os << ',' << std::endl;
+    }
+    for (int k = 0; k < 200; k++) hist0_combined[k] = s.hist[k] + s.hist[200 + k] + s.hist[400 + k];
+    dump_histogram(os, keys[3], hist0_combined);
+  } else {
+    os << "\t\"Mean distance\" : " << s.mean[0] / s.weighted[0] << ',' << std::endl;
+    dump_histogram(os, nullptr, s.hist);
+  }
+  os << '}' << std::endl;
+}
+
+tn93g_ctx *g_ctx = nullptr;  // opened on first use, shared by the cases of a batch
+
+struct SetGuard {
+  tn93g_set *s = nullptr;
+  ~SetGuard() { tn93g_set_free(s); }
+};
+struct ResGuard {
+  tn93g_result *r = nullptr;
+  ~ResGuard() { tn93g_result_free(r); }
+};
+
+int run(int argc, const char **argv)
+{
+  Args A;
+  parse_args(A, argc, argv);
+  if (A.do_bootstrap || A.do_bootstrap_two_files || A.include_prob < 1.) {
+    std::cerr << "swifttn93: -b, -r and -u (random resampling) are not supported" << std::endl;
+    return 1;
+  }
+  if (A.hamming_skip) {
+    std::cerr << "swifttn93: -H (Hamming early exit) is not supported" << std::endl;
+    return 1;
+  }
+  // validFlags[toupper(c)] and the subset of resolved codes (refkernel.cpp:56-103)
+  signed char valid[256];
+  for (int i = 0; i < 256; i++) valid[i] = -1;
+  for (int i = 0; ALPHA[i]; i++) valid[(unsigned char)ALPHA[i]] = (signed char)i;
+  for (int c = 0; c < 256; c++) g_tab[c] = valid[(unsigned char)toupper(c)];
+  uint32_t subset_mask = 0;
+  if (A.ambigs_to_resolve)
+    for (const char *q = A.ambigs_to_resolve; *q; q++) {
+      signed char v = valid[(unsigned char)toupper((unsigned char)*q)];
+      if (v > 3) subset_mask |= 1u << v;
+    }
+  Seqs S;
+  long firstLen = 0;
+  if (read_fasta(A.input1, S, firstLen, A.counts_in_name) == 1) return 1;
+  const unsigned long n1 = (unsigned long)S.N;
+  unsigned long n2 = 0;
+  if (A.input2) {
+    if (read_fasta(A.input2, S, firstLen, A.counts_in_name) == 1) return 1;
+    n2 = (unsigned long)S.N - n1;
+  }
+  const bool do_fst = A.input2 && A.do_fst;
+  const bool report_self = A.input2 == nullptr && A.report_self && !A.do_count;
+  const unsigned long N = (unsigned long)S.N;
+  const unsigned long pairwise = (A.input2 && !do_fst) ? n2 * n1 : (N - 1) * N / 2;
+  if (!A.quiet) {
+    if (A.input2 == nullptr)
+      std::cerr << "Read " << N << " sequences of length " << firstLen << std::endl
+                << "Will perform " << pairwise << " pairwise distance calculations";
+    else {
+      std::cerr << "Read " << n1 << " sequences from file 1 and " << n2 << " sequences from file 2 of length "
+                << firstLen << std::endl
+                << "Will perform " << pairwise << " pairwise distance calculations";
+      if (do_fst) std::cerr << ". Running in F_ST mode";
+    }
+  }
+  // ---- the GPU
+  int rc;
+  if (!g_ctx && (rc = tn93g_open(0, &g_ctx))) {
+    std::cerr << "tn93g: no usable GPU (error " << rc << ")" << std::endl;
+    return 1;
+  }
+  tn93g_ctx *ctx = g_ctx;
+  SetGuard sg;
+  rc = tn93g_set_upload(ctx, S.codes.data(), (int64_t)N, firstLen, S.counts.data(), &sg.s);
+  tn93g_set *set = sg.s;
+  if (rc) {
+    std::cerr << "tn93g: " << tn93g_error(ctx) << std::endl;
+    return 1;
+  }
+  tn93g_params P;
+  memset(&P, 0, sizeof P);
+  P.mode = A.ambig;
+  P.kind = (A.input2 && !do_fst) ? TN93G_CROSS : (do_fst ? TN93G_FST : TN93G_TRIANGLE);
+  P.n1 = (int64_t)n1;
+  P.subset_mask = subset_mask;
+  P.backend = TN93G_BACKEND_AUTO;
+  P.resolve_fraction = A.resolve_fraction;
+  P.min_overlap = (long)A.overlap;
+  P.threshold = A.distance;
+  P.min_distance = A.min_distance;
+  P.flags = do_fst ? TN93G_FLAG_EXACT : 0u;  // F_ST combines the means non-monotonically: resolve logs exactly
+  if (getenv("TN93G_FLAGS")) P.flags |= (uint32_t)strtoul(getenv("TN93G_FLAGS"), nullptr, 0);
+  ResGuard rg;
+  tn93g_result *&R = rg.r;
+  const int prec = (!A.quiet && A.output == stdout) ? 3 : 6;  // tn93's progress line leaves cerr at precision 3
+  uint64_t comb[200];
+  for (int attempt = 0;; attempt++) {
+    rc = tn93g_run(ctx, set, &P, &R);
+    if (rc) {
+      std::cerr << "tn93g: " << tn93g_error(ctx) << std::endl;
+      return 1;
+    }
+    if (R->exact || attempt > 0) break;
+    // certify: every printed link distance and the summary must be the same at both ends of the enclosures
+    bool ok = true;
+    char a[64], b[64];
+    for (int64_t k = 0; k < R->n_links && ok; k++)
+      if (R->link_dist_lo[k] != R->link_dist_hi[k]) {
+        fmt_g(a, R->link_dist_lo[k]);
+        fmt_g(b, R->link_dist_hi[k]);
+        ok = !strcmp(a, b);
+      }
+    if (ok) {
+      std::ostringstream s1, s2;
+      s1 << std::setprecision(prec);
+      s2 << std::setprecision(prec);
+      Summary lo{pairwise, n1, n2, N, R->found_links, R->skipped, {R->weighted[0], R->weighted[1], R->weighted[2]},
+                 R->max_lo[0], {R->mean_lo[0], R->mean_lo[1], R->mean_lo[2]}, &R->hist[0][0]};
+      Summary hi = lo;
+      hi.max0 = R->max_hi[0];
+      for (int k = 0; k < 3; k++) hi.mean[k] = R->mean_hi[k];
+      write_json(s1, lo, A.input2 != nullptr, do_fst, comb);
+      write_json(s2, hi, A.input2 != nullptr, do_fst, comb);
+      ok = s1.str() == s2.str();
+    }
+    if (ok) break;
+    tn93g_result_free(R);
+    R = nullptr;
+    P.flags |= TN93G_FLAG_EXACT;
+  }
+  if (getenv("TN93G_STATS"))
+    fprintf(stderr, "tn93g: gpu %.3f ms (pair kernel %.3f ms) links %lld replayed %lld uncertain %lld host-exact %lld\n",
+            R->gpu_ms, R->pair_kernel_ms, (long long)R->n_links, (long long)R->n_replayed,
+            (long long)R->n_uncertain, (long long)R->n_host_exact);
+  // ---- output (swifttn93.cpp:2307-2316, 2406-2434, 2547-2603, 2869-2897)
+  static char obuf[1 << 16];
+  setvbuf(A.output, obuf, _IOFBF, sizeof obuf);
+  if (A.format != F_HYPHY && !A.do_count && !A.skip_header) {
+    if (A.format == F_CSV)
+      fprintf(A.output, "ID1%cID2%cDistance\n", A.delimiter, A.delimiter);
+    else
+      fprintf(A.output, "Seq1%cSeq2%cDistance\n", A.delimiter, A.delimiter);
+  }
+  const bool emit = !A.do_count;
+  const bool cross = A.input2 && !do_fst;
+  const long upper = cross ? (long)n1 : (long)N;
+  std::string out;
+  out.reserve(1 << 20);
+  char t[64];
+  int64_t k = 0;
+  std::vector<std::vector<std::pair<long, double>>> hylinks(emit && A.format == F_HYPHY ? N : 0);
+  for (long i = 0; i < upper; i++) {
+    if (!cross && report_self) {
+      if (A.format == F_CSV) {
+        out += S.names[(size_t)i];
+        out += A.delimiter;
+        out += S.names[(size_t)i];
+        out += A.delimiter;
+        out += "0\n";
+      } else if (A.format == F_CSVN) {
+        out += std::to_string(i);
+        out += A.delimiter;
+        out += std::to_string(i);
+        out += A.delimiter;
+        out += "0\n";
+      }
+    }
+    for (; k < R->n_links && R->link_row[k] == i; k++) {
+      const long j = (long)R->link_col[k];
+      const double d = R->link_dist[k];
+      if (!emit) continue;
+      if (A.format == F_CSV) {
+        out += S.names[(size_t)i];
+        out += A.delimiter;
+        out += S.names[(size_t)j];
+        out += A.delimiter;
+        out.append(t, (size_t)fmt_g(t, d));
+        out += '\n';
+      } else if (A.format == F_CSVN) {
+        out += std::to_string(i);
+        out += A.delimiter;
+        out += std::to_string(j);
+        out += A.delimiter;
+        out.append(t, (size_t)fmt_g(t, d));
+        out += '\n';
+      } else {
+        hylinks[(size_t)i].push_back(std::make_pair(j, d));
+        hylinks[(size_t)j].push_back(std::make_pair(i, d));
+      }
+    }
+    if (out.size() > (1u << 20)) {
+      fwrite(out.data(), 1, out.size(), A.output);
+      out.clear();
+    }
+  }
+  fwrite(out.data(), 1, out.size(), A.output);
+  out.clear();
+  if (emit && A.format == F_HYPHY) {
+    std::string o = "{";
+    for (long r = 0; r < (long)N; r++) {
+      o += "\n{";
+      auto &lk = hylinks[(size_t)r];
+      size_t li = 0;
+      for (long c = 0; c < (long)N; c++) {
+        if (c) o += ',';
+        if (c == r)
+          o += '0';
+        else if (li < lk.size() && lk[li].first == c) {
+          o.append(t, (size_t)fmt_g(t, lk[li].second));
+          li++;
+        } else
+          o += "100";
+      }
+      o += '}';
+      if (o.size() > (1u << 20)) {
+        fwrite(o.data(), 1, o.size(), A.output);
+        o.clear();
+      }
+    }
+    o += "\n}";
+    fwrite(o.data(), 1, o.size(), A.output);
+  }
+  if (!A.quiet)
+    std::cerr << std::endl
+              << "Progress:" << std::setw(8) << 100. << "% (" << std::setw(8) << R->found_links << " links found, "
+              << std::setw(12) << std::setprecision(3) << (double)pairwise << " evals)";
+  std::cerr << std::endl;
+  std::ostream &os = A.output == stdout ? std::cerr : std::cout;
+  Summary s{pairwise, n1, n2, N, R->found_links, R->skipped, {R->weighted[0], R->weighted[1], R->weighted[2]},
+            R->max_lo[0], {R->mean_lo[0], R->mean_lo[1], R->mean_lo[2]}, &R->hist[0][0]};
+  write_json(os, s, A.input2 != nullptr, do_fst, comb);
+  if (A.do_count)
+    fprintf(A.output, "Found %ld links among %ld pairwise comparisons\n", (long)R->found_links,
+            (long)(pairwise - (unsigned long)R->skipped));
+  return 0;
+}
+
+int run_guarded(int argc, const char **argv)
+{
+  try {
+    return run(argc, argv);
+  } catch (const Exit &e) {
+    return e.code;
+  }
+}
+
+// Batch driver for tests: every line of FILE is one tn93 invocation, fields separated by 0x1f: the working
+// directory, then the arguments.  Each case runs in this process on one GPU context; its standard output goes to
+// DIR/stdout.bin and its exit code to DIR/rc.txt.
+int batch(const char *file)
+{
+  std::ifstream in(file, std::ios::binary);
+  std::string line;
+  char cwd[4096];
+  if (!getcwd(cwd, sizeof cwd)) return 2;
+  while (std::getline(in, line)) {
+    std::vector<std::string> f;
+    size_t a = 0;
+    for (size_t b; (b = line.find('\x1f', a)) != std::string::npos; a = b + 1) f.push_back(line.substr(a, b - a));
+    f.push_back(line.substr(a));
+    if (f.empty() || chdir(f[0].c_str())) return 2;
+    std::vector<const char *> av{"tn93g"};
+    for (size_t k = 1; k < f.size(); k++) av.push_back(f[k].c_str());
+    std::ostringstream cap;
+    std::streambuf *old = std::cout.rdbuf(cap.rdbuf());
+    std::cerr.precision(6);
+    int rc = run_guarded((int)av.size(), av.data());
+    std::cout.flush();
+    std::cout.rdbuf(old);
+    std::ofstream(f[0] + "/stdout.bin", std::ios::binary) << cap.str();
+    std::ofstream(f[0] + "/rc.txt") << rc << "\n";
+    if (chdir(cwd)) return 2;
+  }
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, const char **argv)
+{
+  int rc = (argc == 3 && !strcmp(argv[1], "--batch")) ? batch(argv[2]) : run_guarded(argc, argv);
+  if (g_ctx) tn93g_close(g_ctx);
+  return rc;
+}

@@ -1,0 +1,224 @@
+// tn93g_math.h -- the TN93 formula tail of tn93 (refkernel.cpp:752-815) on the GPU, with an own logarithm.
+//
+// tn93 evaluates log() with the host libm.  The GPU cannot call it, so the logarithm here is evaluated to about
+// 2^-70 relative error in double-double arithmetic (Tang's table-driven method: x = 2^e m, breakpoint F = 1 + j/256
+// near m, log x = e ln2 + log F + log1p((m-F)/F), with log F and 1/F from an mpmath table, tn93g_logtab.h).  The
+// value rounded to nearest is what libm returns unless the exact logarithm lies within `window` units in the last
+// place of a rounding midpoint, where libm's own approximation error (a few hundredths of an ulp) could round it
+// the other way.  Such a result is returned with `alt`, the neighbouring double on the other side of the midpoint:
+// libm's value is then one of {v, alt}.
+//
+// The distance is a monotone function of each logarithm (every IEEE operation rounds monotonically), so evaluating
+// the formula at the two extreme choices gives an enclosure [dlo, dhi] of tn93's distance.  Callers certify what is
+// printed from the enclosure and fall back to the host libm when that fails.
+//
+// This header is compiled for the device by hipcc (the product) and for the host by the tests (calibration of the
+// window against the running libm).  All code must be compiled with -ffp-contract=off: fma() is written out where
+// it is meant, every other expression is evaluated as written, left to right, exactly as tn93's C++ does.
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+#if defined(__HIPCC__)
+#define TN93G_FN __device__ static inline
+#define TN93G_TABLE_ATTR __device__ __constant__
+#define TN93G_FMA(a, b, c) __builtin_fma((a), (b), (c))
+#else
+#include <cmath>
+#define TN93G_FN static inline
+#define TN93G_TABLE_ATTR static
+#define TN93G_FMA(a, b, c) std::fma((a), (b), (c))
+#endif
+
+#include "tn93g_logtab.h"
+
+// Half-width of the uncertain band around a rounding midpoint, in ulps of the result.  libm's log (glibc >= 2.28)
+// rounds an internal approximation whose error is documented as below 0.02 ulp; the band is wider than that.
+#define TN93G_LOG_WINDOW 0.05
+
+TN93G_FN uint64_t tn93g_bits(double x)
+{
+  uint64_t u;
+  memcpy(&u, &x, 8);
+  return u;
+}
+TN93G_FN double tn93g_from_bits(uint64_t u)
+{
+  double x;
+  memcpy(&x, &u, 8);
+  return x;
+}
+
+struct tn93g_logres {
+  double v;    // log(x) rounded to nearest (from the double-double value)
+  double alt;  // the other candidate when uncertain, else == v
+  int unc;     // 1 when libm may return alt instead of v
+};
+
+// exact sum a + b = s + e
+TN93G_FN void tn93g_two_sum(double a, double b, double *s, double *e)
+{
+  double t = a + b;
+  double bb = t - a;
+  *e = (a - (t - bb)) + (b - bb);
+  *s = t;
+}
+
+// x > 0, finite.
+TN93G_FN tn93g_logres tn93g_log(double x, double window)
+{
+  tn93g_logres R;
+  uint64_t u = tn93g_bits(x);
+  int e = 0;
+  if (u < 0x0010000000000000ULL) {  // subnormal: scale into the normal range
+    x = x * 0x1p54;
+    u = tn93g_bits(x);
+    e = -54;
+  }
+  e += (int)(u >> 52) - 1023;
+  double m = tn93g_from_bits((u & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL);  // [1, 2)
+  int j = (int)((m - 1.0) * 256.0 + 0.5);                                           // 0..256
+  double F = 1.0 + (double)j * (1.0 / 256.0);
+  double uu = m - F;  // exact (|m - F| <= 2^-9, both in [1, 2])
+  const double lFh = tn93g_logtab[j][0], lFl = tn93g_logtab[j][1];
+  const double iFh = tn93g_logtab[j][2], iFl = tn93g_logtab[j][3];
+  // r = uu / F as double-double: uu * (iFh + iFl)
+  double rh = uu * iFh;
+  double rl = TN93G_FMA(uu, iFh, -rh) + uu * iFl;
+  double r0, r1;
+  tn93g_two_sum(rh, rl, &r0, &r1);  // |r| <= 2^-9
+  // log1p(r) = r - r^2/2 + r^3 P(r), P(r) = 1/3 - r/4 + r^2/5 - ...
+  double s0 = r0 * r0;
+  double s1 = TN93G_FMA(r0, r0, -s0) + 2.0 * r0 * r1;  // r^2 = s0 + s1
+  double p = -1.0 / 10.0;
+  p = p * r0 + 1.0 / 9.0;
+  p = p * r0 - 1.0 / 8.0;
+  p = p * r0 + 1.0 / 7.0;
+  p = p * r0 - 1.0 / 6.0;
+  p = p * r0 + 1.0 / 5.0;
+  p = p * r0 - 1.0 / 4.0;
+  p = p * r0 + 1.0 / 3.0;
+  double t3 = s0 * r0 * p;
+  // A = e*ln2 + log F (double-double, error-free product for e*ln2)
+  double ed = (double)e;
+  double a0 = ed * TN93G_LN2_HI;
+  double a1 = TN93G_FMA(ed, TN93G_LN2_HI, -a0) + ed * TN93G_LN2_LO;
+  double b0, b1;
+  tn93g_two_sum(a0, lFh, &b0, &b1);
+  b1 = b1 + (a1 + lFl);
+  // B = r - r^2/2 (double-double)
+  double c0, c1;
+  tn93g_two_sum(r0, -0.5 * s0, &c0, &c1);
+  c1 = c1 + (r1 - 0.5 * s1);
+  // total = b + c + t3
+  double d0, d1;
+  tn93g_two_sum(b0, c0, &d0, &d1);
+  d1 = d1 + (b1 + c1 + t3);
+  double v, lo;
+  tn93g_two_sum(d0, d1, &v, &lo);  // v = RN(total) except within ~2^-70 of a midpoint
+  R.v = v;
+  R.alt = v;
+  R.unc = 0;
+  if (lo != 0.0 && v != 0.0) {
+    // neighbour of v in the direction of lo, and the distance of the value from their midpoint in ulps
+    uint64_t vb = tn93g_bits(v);
+    bool up = (lo > 0) == (v > 0);  // moving away from zero?
+    double n = tn93g_from_bits(up ? vb + 1 : vb - 1);
+    double gap = n - v;  // exact (adjacent doubles)
+    if (gap < 0) gap = -gap;
+    double al = lo < 0 ? -lo : lo;
+    double dist = (0.5 * gap - al) / gap;  // 0 at the midpoint, 0.5 at v
+    if (dist < window) {
+      R.alt = n;
+      R.unc = 1;
+    }
+  }
+  return R;
+}
+
+// Inputs of the tail: tn93's nucFreq[4] and float_counts sums after the merge (refkernel.cpp:714-729).
+struct tn93g_tailin {
+  double nf[4];
+  double diag;  // fc00 + fc11 + fc22 + fc33 (left to right)
+  double ag;    // fc02 + fc20
+  double ct;    // fc13 + fc31
+};
+
+struct tn93g_tailout {
+  double d, dlo, dhi;  // tn93's raw distance (before the clamp at 0), and an enclosure of it
+  int unc;             // any logarithm uncertain
+};
+
+// The formula of computeTN93 (refkernel.cpp:752-803) and swifttn93's tn93_tail_raw (swifttn93.cpp:1954-2000),
+// operation for operation.  Logs come from tn93g_log; d uses the rounded-to-nearest values, dlo/dhi the extreme
+// candidates (d is non-increasing in each log whose coefficient is positive).
+TN93G_FN tn93g_tailout tn93g_tail(const tn93g_tailin &in, double window)
+{
+  tn93g_tailout o;
+  const double *nucFreq = in.nf;
+  double auxd, fY, fR, K1 = 0, K2 = 0, K3 = 0, AG, CT, ti, tv, tng2, nucF[4];
+  bool useK2P = false;
+  tng2 = 2. / (nucFreq[0] + nucFreq[1] + nucFreq[2] + nucFreq[3]);
+  auxd = 1. / (nucFreq[0] + nucFreq[1] + nucFreq[2] + nucFreq[3]);
+  for (int k = 0; k < 4; k++) nucF[k] = nucFreq[k] * auxd;
+  fR = nucF[0] + nucF[2];
+  fY = nucF[1] + nucF[3];
+  if (nucFreq[0] == 0 || nucFreq[1] == 0 || nucFreq[2] == 0 || nucFreq[3] == 0) {
+    useK2P = true;
+  } else {
+    K1 = 2. * nucF[0] * nucF[2] / fR;
+    K2 = 2. * nucF[1] * nucF[3] / fY;
+    K3 = 2. * (fR * fY - nucF[0] * nucF[2] * fY / fR - nucF[1] * nucF[3] * fR / fY);
+  }
+  AG = in.ag * tng2;
+  CT = in.ct * tng2;
+  tv = 1. - (in.diag * tng2 + AG + CT);
+  o.unc = 0;
+  if (useK2P) {
+    ti = AG + CT;
+    AG = 1. - 2. * ti - tv;
+    CT = 1. - 2. * tv;
+    if (AG > 0. && CT > 0.) {
+      tn93g_logres la = tn93g_log(AG, window), lc = tn93g_log(CT, window);
+      o.d = -0.5 * la.v - 0.25 * lc.v;
+      o.unc = la.unc | lc.unc;
+      if (o.unc) {
+        double ah = la.v > la.alt ? la.v : la.alt, al_ = la.v > la.alt ? la.alt : la.v;
+        double ch = lc.v > lc.alt ? lc.v : lc.alt, cl = lc.v > lc.alt ? lc.alt : lc.v;
+        o.dlo = -0.5 * ah - 0.25 * ch;
+        o.dhi = -0.5 * al_ - 0.25 * cl;
+      } else
+        o.dlo = o.dhi = o.d;
+    } else
+      o.d = o.dlo = o.dhi = 1000.0;
+  } else {
+    AG = 1. - AG / K1 - 0.5 * tv / fR;
+    CT = 1. - CT / K2 - 0.5 * tv / fY;
+    tv = 1. - 0.5 * tv / fY / fR;
+    if (AG > 0. && CT > 0. && tv > 0) {
+      tn93g_logres la = tn93g_log(AG, window), lc = tn93g_log(CT, window), lt = tn93g_log(tv, window);
+      o.d = -K1 * la.v - K2 * lc.v - K3 * lt.v;
+      o.unc = la.unc | lc.unc | lt.unc;
+      if (o.unc) {
+        double ah = la.v > la.alt ? la.v : la.alt, al_ = la.v > la.alt ? la.alt : la.v;
+        double ch = lc.v > lc.alt ? lc.v : lc.alt, cl = lc.v > lc.alt ? lc.alt : lc.v;
+        double th = lt.v > lt.alt ? lt.v : lt.alt, tl = lt.v > lt.alt ? lt.alt : lt.v;
+        // d is non-increasing in la, lc; in lt it is non-increasing when K3 > 0, non-decreasing when K3 < 0
+        o.dlo = -K1 * ah - K2 * ch - K3 * (K3 > 0 ? th : tl);
+        o.dhi = -K1 * al_ - K2 * cl - K3 * (K3 > 0 ? tl : th);
+      } else
+        o.dlo = o.dhi = o.d;
+    } else
+      o.d = o.dlo = o.dhi = 1000.0;
+  }
+  return o;
+}
+
+// histogram index with tn93's unsigned clamp (refkernel.cpp:805-813): floor(d*200) as a long, 199 when it is not
+// in [0, 200) as an unsigned value (negative, NaN or too large)
+TN93G_FN int tn93g_hist_index(double d)
+{
+  double f = floor(d * 200.0);
+  if (f >= 0.0 && f < 200.0) return (int)f;
+  return 199;
+}

@@ -1,0 +1,140 @@
+"""Deterministic synthetic alignments for the benchmark workloads (codes 0..17 = "ACGTURYSWKMBDHVN?-").
+
+Randomness comes from a counter-based splitmix64 stream (numpy uint64 arithmetic), so the bytes are identical on
+every machine and numpy version.  Workloads:
+
+  c0   500 x 1,000: uniform random root; 8 clusters on star trees, per-branch substitution rate 0.2-3 %;
+       0.5 % IUPAC codes (R, Y, K, M, S, W, N uniform); 2 % gaps in runs of 1-30.
+  c1   6,700 x 1,300 HIV-pol-shaped: 200 transmission clusters (sizes geometric, mean 15) at 0.1-1.5 %
+       intra-cluster divergence, 5-12 % between clusters (background sequences at the same spread); 1 % ambiguity
+       codes (R, Y, K, M, S, W, N, with B/D/H/V at 2 % of them so the site-order replay is exercised); 5 % of
+       sequences carry a leading or trailing gap run of 50-400 columns.
+  c1s  a smaller sample of the c1 generator (n sequences, default 600) for parity tests.
+"""
+import numpy as np
+
+ALPHABET = b"ACGTURYSWKMBDHVN?-"
+GAP = 17
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+class Stream:
+    """splitmix64 over a counter: value k of stream `seed` is mix(seed + (k + 1) * golden)."""
+
+    def __init__(self, seed):
+        self.seed = np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+        self.k = 0
+
+    def u64(self, n):
+        with np.errstate(over="ignore"):
+            c = np.arange(self.k + 1, self.k + 1 + n, dtype=np.uint64)
+            z = self.seed + c * np.uint64(0x9E3779B97F4A7C15)
+            z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+            z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+            z = z ^ (z >> np.uint64(31))
+        self.k += n
+        return z
+
+    def uniform(self, n):
+        return (self.u64(n) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+    def integers(self, hi, n):
+        return (self.uniform(n) * hi).astype(np.int64)
+
+
+def _mutate(rs, seq, rate):
+    """Substitutes each site with probability `rate` by one of the three other bases."""
+    L = seq.shape[-1]
+    hit = rs.uniform(L) < rate
+    shift = rs.integers(3, L) + 1
+    out = seq.copy()
+    out[hit] = (seq[hit] + shift[hit]) % 4
+    return out
+
+
+def _ambiguities(rs, codes, rate, pool):
+    n, L = codes.shape
+    hit = rs.uniform(n * L).reshape(n, L) < rate
+    pick = np.array(pool, dtype=np.uint8)[rs.integers(len(pool), n * L).reshape(n, L)]
+    codes[hit] = pick[hit]
+
+
+def _gap_runs(rs, codes, frac, lo, hi):
+    n, L = codes.shape
+    for i in range(n):
+        u = rs.uniform(3)
+        if u[0] < frac:
+            run = int(lo + u[1] * (hi - lo + 1))
+            run = min(run, L)
+            if u[2] < 0.5:
+                codes[i, :run] = GAP
+            else:
+                codes[i, L - run:] = GAP
+
+
+def c0(seed=1):
+    rs = Stream(seed)
+    N, L = 500, 1000
+    root = rs.integers(4, L).astype(np.uint8)
+    codes = np.empty((N, L), dtype=np.uint8)
+    centers = [_mutate(rs, root, 0.03) for _ in range(8)]
+    for i in range(N):
+        rate = 0.002 + rs.uniform(1)[0] * 0.028
+        codes[i] = _mutate(rs, centers[i % 8], rate)
+    _ambiguities(rs, codes, 0.005, [5, 6, 9, 10, 7, 8, 15])
+    # 2 % gaps in runs of 1-30
+    for i in range(N):
+        target = int(0.02 * L)
+        while target > 0:
+            u = rs.uniform(2)
+            run = min(1 + int(u[0] * 30), target)
+            p = int(u[1] * (L - run))
+            codes[i, p:p + run] = GAP
+            target -= run
+    return codes
+
+
+def c1(seed=2, n=6700, L=1300, clusters=200):
+    rs = Stream(seed)
+    root = rs.integers(4, L).astype(np.uint8)
+    codes = np.empty((n, L), dtype=np.uint8)
+    i = 0
+    # each clustered/background lineage sits at 2.5-6 % from the root, so lineages differ by about 5-12 %
+    while i < n:
+        center = _mutate(rs, root, 0.025 + 0.035 * rs.uniform(1)[0])
+        if clusters > 0:
+            clusters -= 1
+            size = 1 + int(np.floor(np.log(1.0 - rs.uniform(1)[0]) / np.log(1.0 - 1.0 / 15.0)))
+            for _ in range(min(size, n - i)):
+                codes[i] = _mutate(rs, center, 0.0005 + 0.007 * rs.uniform(1)[0])
+                i += 1
+        else:
+            codes[i] = center
+            i += 1
+    # shuffle rows so clusters are spread over the matrix
+    perm = np.argsort(rs.u64(n), kind="stable")
+    codes = codes[perm]
+    pool = [5] * 24 + [6] * 12 + [9] * 2 + [10] * 3 + [7] * 1 + [8] * 2 + [15] * 10 + [11, 12, 13, 14]
+    _ambiguities(rs, codes, 0.01, pool)
+    _gap_runs(rs, codes, 0.05, 50, 400)
+    return codes
+
+
+def c1s(n=600, seed=3):
+    return c1(seed=seed, n=n, L=1300, clusters=max(1, n * 200 // 6700))
+
+
+def names_for(n, prefix="s"):
+    return [f"{prefix}{i}" for i in range(n)]
+
+
+def write_fasta(path, codes, names=None):
+    codes = np.asarray(codes, dtype=np.uint8)
+    names = names or names_for(codes.shape[0])
+    lut = np.frombuffer(ALPHABET, dtype=np.uint8)
+    with open(path, "wb") as f:
+        for nm, row in zip(names, codes):
+            f.write(b">" + nm.encode() + b"\n" + lut[row].tobytes() + b"\n")
+
+
+WORKLOADS = {"c0": c0, "c1": c1, "c1s": c1s}

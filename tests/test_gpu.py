@@ -1,0 +1,191 @@
+"""GPU parity: the product (libtn93g.so through the tn93g CLI and the C ABI) against the golden fixtures, the
+published tn93 pins and the oracle, byte for byte.  Needs an MI355X: run with ``pytest -m gpu``."""
+import hashlib
+import json
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, REPO
+
+pytestmark = pytest.mark.gpu
+
+sys.path.insert(0, str(REPO))
+import swifttn93_amd as T  # noqa: E402
+from swifttn93_amd import gen  # noqa: E402
+
+SEP = "\x1f"
+
+
+def _cli():
+    if not T.CLI_PATH.exists() or not T.LIB_PATH.exists():
+        pytest.fail("the engine is not built (python3 -m swifttn93_amd.build)")
+    return T.CLI_PATH
+
+
+def run_batch(cases, tmp_path, timeout=600):
+    """cases: list of (dir, argv list without program name).  Runs them in one tn93g process."""
+    bf = tmp_path / "batch.txt"
+    bf.write_text("".join(SEP.join([str(d)] + list(a)) + "\n" for d, a in cases))
+    print(f"tn93g --batch: {len(cases)} cases", flush=True)
+    p = subprocess.run([str(_cli()), "--batch", str(bf)], capture_output=True, timeout=timeout)
+    assert p.returncode == 0, p.stderr[-2000:]
+    res = []
+    for d, _ in cases:
+        rc = int((d / "rc.txt").read_text())
+        out = (d / "out.txt").read_bytes() if (d / "out.txt").exists() else None
+        res.append((rc, out, (d / "stdout.bin").read_bytes()))
+    return res
+
+
+def test_fuzz_fixtures(fuzz_cases, tmp_path):
+    cases = []
+    for k, c in enumerate(fuzz_cases):
+        d = tmp_path / f"c{k}"
+        d.mkdir()
+        for key, v in c["files"].items():
+            (d / f"{key}.fa").write_bytes(v.encode("latin-1"))
+        argv = ["-q"] + [f"{f.strip('{}')}.fa" if f.startswith("{") else f for f in c["flags"]] + ["-o", "out.txt",
+                                                                                                   "in.fa"]
+        cases.append((d, argv))
+    res = run_batch(cases, tmp_path)
+    bad = []
+    for k, (c, (rc, out, so)) in enumerate(zip(fuzz_cases, res)):
+        if c["rc"] != 0:
+            if rc == 0:
+                bad.append((k, c["flags"], "expected failure"))
+            continue
+        exp = None if c["out"] is None else c["out"].encode("latin-1")
+        if rc != 0 or out != exp or so != c["stdout"].encode("latin-1"):
+            bad.append((k, c["flags"], rc, (out or b"")[:200], (exp or b"")[:200], so[:300]))
+    assert not bad, bad[:3]
+
+
+def _oracle_cli():
+    cli = REPO / "oracle" / "_build" / "tn93_oracle"
+    if not cli.exists():
+        subprocess.run(["make", "-s", "-C", str(REPO / "oracle"), "oracle"], check=True)
+    return cli
+
+
+CONFIGS = [
+    ("-a", "resolve", "-g", "0.05", "-l", "500"),
+    ("-a", "average", "-l", "500"),
+    ("-a", "skip", "-l", "500"),
+    ("-a", "gapmm", "-l", "500"),
+    ("-a", "RYN", "-g", "0.5", "-l", "500"),
+    ("-a", "resolve", "-l", "1250", "-0"),
+]
+
+
+@pytest.mark.parametrize("workload", ["c1s", "c0"])
+def test_synthetic_vs_oracle(workload, tmp_path):
+    """Every option set at -t 0.015 and at -t 1 (every pair written) on a seeded synthetic set."""
+    codes = gen.c1s(500) if workload == "c1s" else gen.c0()
+    fa = tmp_path / "in.fa"
+    gen.write_fasta(fa, codes)
+    cases, want = [], []
+    for ci, cfg in enumerate(CONFIGS):
+        for t in ["0.015", "1"]:
+            for fmt in (["csv", "csvn"] if t == "1" else ["csv"]):
+                d = tmp_path / f"k{ci}_{t}_{fmt}"
+                d.mkdir()
+                argv = ["-q", "-t", t, "-f", fmt, *cfg, "-o", "out.txt", str(fa)]
+                p = subprocess.run([str(_oracle_cli())] + argv[:-3] + ["-o", str(d / "want.txt"), str(fa)],
+                                   capture_output=True, check=True)
+                want.append(((d / "want.txt").read_bytes(), p.stdout))
+                cases.append((d, argv))
+    res = run_batch(cases, tmp_path)
+    for (d, argv), (rc, out, so), (wo, ws) in zip(cases, res):
+        assert rc == 0, argv
+        assert out == wo, (argv, len(out or b""), len(wo))
+        assert so == ws, (argv, so[:400], ws[:400])
+
+
+def test_two_files_and_fst(tmp_path):
+    codes = gen.c1s(300, seed=11)
+    a, b = tmp_path / "a.fa", tmp_path / "b.fa"
+    gen.write_fasta(a, codes[:120], [f"q{i}:{1 + i % 3}" for i in range(120)])
+    gen.write_fasta(b, codes[120:], [f"r{i}" for i in range(180)])
+    cases, want = [], []
+    for k, extra in enumerate([[], ["-m"], ["-f", "csvn"], ["-f", "hyphy"]]):
+        d = tmp_path / f"k{k}"
+        d.mkdir()
+        argv = ["-q", "-t", "0.05", "-s", str(b), *extra, "-o", "out.txt", str(a)]
+        p = subprocess.run([str(_oracle_cli())] + argv[:-3] + ["-o", str(d / "want.txt"), str(a)],
+                           capture_output=True, check=True)
+        want.append(((d / "want.txt").read_bytes(), p.stdout))
+        cases.append((d, argv))
+    for (d, argv), (rc, out, so), (wo, ws) in zip(cases, run_batch(cases, tmp_path), want):
+        assert rc == 0 and out == wo and so == ws, argv
+
+
+def test_network_pins(hiv8766, tmp_path):
+    """The ten published tn93 sha256 pins on the real 8,766-sequence HIV-1 pol set (38,416,995 pairs per run)."""
+    P = json.loads((GOLDEN / "real" / "network_pins.json").read_text())
+    cases = []
+    for pin in P["pins"]:
+        d = tmp_path / ("t" + pin["t_arg"])
+        d.mkdir()
+        flags = list(P["flags"])
+        cases.append((d, flags[:1] + ["-o", "out.txt", "-t", pin["t_arg"]] + flags[1:] + [str(hiv8766)]))
+    res = run_batch(cases, tmp_path)
+    for pin, (rc, out, so) in zip(P["pins"], res):
+        assert rc == 0
+        assert hashlib.sha256(out).hexdigest() == pin["ref_file_sha256"], pin["t_arg"]
+        assert hashlib.sha256(so).hexdigest() == pin["ref_stdout_sha256"], pin["t_arg"]
+
+
+@pytest.fixture(scope="module")
+def engine():
+    e = T.Engine(0)
+    yield e
+    e.close()
+
+
+def _same(r1, r2):
+    assert (r1["hist"] == r2["hist"]).all()
+    for k in ["link_row", "link_col"]:
+        assert (r1[k] == r2[k]).all()
+    assert r1["found_links"] == r2["found_links"] and r1["skipped"] == r2["skipped"]
+    assert r1["weighted"] == r2["weighted"]
+    # distances: the enclosures must overlap and print the same
+    for a, b in zip(r1["link_dist"], r2["link_dist"]):
+        assert "%g" % a == "%g" % b
+
+
+@pytest.mark.parametrize("mode", [T.RESOLVE, T.AVERAGE, T.SKIP, T.SUBSET])
+def test_fast_path_equals_replay(engine, mode):
+    """The popcount + event-table path and the site-order replay kernel agree on every pair (all written)."""
+    codes = gen.c1s(400, seed=5)
+    s = engine.upload(codes)
+    kw = dict(mode=mode, threshold=1.0, min_overlap=500, resolve_fraction=0.05 if mode == T.RESOLVE else 1.0,
+              subset_mask=(1 << 5) | (1 << 6) | (1 << 15))
+    fast = engine.run(s, **kw)
+    slow = engine.run(s, flags=T.FLAG_ALL_SLOW, **kw)
+    exact = engine.run(s, flags=T.FLAG_EXACT, **kw)
+    _same(fast, slow)
+    _same(fast, exact)
+    assert exact["exact"]
+    assert slow["n_replayed"] == 400 * 399 // 2
+    for r in (fast, slow):
+        assert (r["link_lo"] <= exact["link_dist"]).all() and (exact["link_dist"] <= r["link_hi"]).all()
+    s.free()
+
+
+def test_full_size_c1_vs_oracle(tmp_path):
+    """The benchmark workload itself (6,700 x 1,300, 22.4 M pairs) against the oracle, byte for byte."""
+    fa = tmp_path / "c1.fa"
+    gen.write_fasta(fa, gen.c1())
+    d = tmp_path / "run"
+    d.mkdir()
+    argv = ["-q", "-t", "0.015", "-a", "resolve", "-g", "0.05", "-l", "500", "-o", "out.txt", str(fa)]
+    p = subprocess.run([str(_oracle_cli())] + argv[:-3] + ["-o", str(tmp_path / "want.txt"), str(fa)],
+                       capture_output=True, check=True, timeout=600)
+    (rc, out, so), = run_batch([(d, argv)], tmp_path)
+    assert rc == 0
+    assert out == (tmp_path / "want.txt").read_bytes()
+    assert so == p.stdout
