@@ -111,7 +111,8 @@ typedef struct {
   int64_t n_uncertain;  /* pairs whose distance is an enclosure (lo < hi)               */
   int64_t n_host_exact; /* pairs whose logarithms were resolved on the host (EXACT)     */
   double gpu_ms;        /* device time of the run's kernels (HIP events)                 */
-  double pair_kernel_ms;/* device time of the clean/clean counting + tail kernel         */
+  double pair_kernel_ms;/* device time of the pair kernel (clean/clean counts + events)   */
+  double tail_kernel_ms;/* device time of the FP64 tail kernel                            */
 } tn93g_result;
 
 /* Computes all pairs of `set` described by `params` (set must hold n1 + n2 sequences for CROSS/FST).

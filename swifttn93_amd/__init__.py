@@ -36,7 +36,8 @@ class Result(ctypes.Structure):
                 ("mean_lo", ctypes.c_double * 3), ("mean_hi", ctypes.c_double * 3), ("weighted", ctypes.c_double * 3),
                 ("max_lo", ctypes.c_double * 3), ("max_hi", ctypes.c_double * 3), ("found_links", ctypes.c_int64),
                 ("skipped", ctypes.c_int64), ("n_replayed", ctypes.c_int64), ("n_uncertain", ctypes.c_int64),
-                ("n_host_exact", ctypes.c_int64), ("gpu_ms", ctypes.c_double), ("pair_kernel_ms", ctypes.c_double)]
+                ("n_host_exact", ctypes.c_int64), ("gpu_ms", ctypes.c_double), ("pair_kernel_ms", ctypes.c_double),
+                ("tail_kernel_ms", ctypes.c_double)]
 
 
 EXPORTS = ["tn93g_open", "tn93g_close", "tn93g_error", "tn93g_set_upload", "tn93g_set_free", "tn93g_run",
@@ -140,5 +141,5 @@ def _to_py(r):
         "mean_lo": list(r.mean_lo), "mean_hi": list(r.mean_hi), "weighted": list(r.weighted),
         "max_lo": list(r.max_lo), "max_hi": list(r.max_hi), "found_links": r.found_links, "skipped": r.skipped,
         "n_replayed": r.n_replayed, "n_uncertain": r.n_uncertain, "n_host_exact": r.n_host_exact,
-        "gpu_ms": r.gpu_ms, "pair_kernel_ms": r.pair_kernel_ms,
+        "gpu_ms": r.gpu_ms, "pair_kernel_ms": r.pair_kernel_ms, "tail_kernel_ms": r.tail_kernel_ms,
     }

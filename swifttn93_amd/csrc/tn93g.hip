@@ -5,7 +5,10 @@
 //   codes   [n][Ls]    u8   ValidChars codes, row-major, padded with gaps        (replay kernel, j-side lookups)
 //   codesT  [Ls][npad] u8   site-major copy                                     (i-side event lookups, coalesced)
 //   planes  [n][W]     uint4 per 32 sites: clean, C|T bit, G|T bit, T bit       (clean/clean counts, VALU popcount)
-//   onehot  [n][W64][4][64] u8 one-hot A,C,G,T per site, 64-site blocks      (clean/clean counts, int8 MFMA)
+//   ambp    [n][W]     u32 per 32 sites: ambiguity code (4..16)                 (event partners, from LDS)
+//   onehot  [npad/4][W64][4][16][16] u8 one-hot A,C,G,T per site in MFMA operand order: per group of 4 sequences
+//           and 64-site block, 1 KB = the A/B operand of one v_mfma_i32_16x16x64_i8 (lane l: row l%16 = 4*seq+base,
+//           16 sites of quarter l/16)                                      (clean/clean counts, int8 MFMA)
 //   ev_off  [n+1] i32, ev [] u32 (site << 5 | code) ambiguity events (codes 4..16) in site order
 // Per run: D [rows][cols] f64 (tn93's thisD of each pair), U [rows][cols] u16 (ulps below/above D that bound the
 // value when a logarithm was uncertain), the histogram, per-row link counts, replay and host-exact lists.
@@ -171,6 +174,7 @@ struct SetDev {
   int64_t n, L, Ls, npad, W, W64;
   uint8_t *codes, *codesT;
   uint4 *planes;
+  uint32_t *ambp;
   uint8_t *onehot;
   int32_t *ev_off;
   uint32_t *ev;
@@ -189,9 +193,29 @@ struct PatchRec {
   int32_t link;
 };
 
+// Per-row reductions of the fast path (no distance matrix): tn93 sums thisD*w of a row in column order, but only
+// prints the final mean to 6 digits, so the sum is taken in any order together with sum |thisD*w| and turned into
+// an enclosure of tn93's sequentially rounded value on the host (|fl(sum) - sum| <= gamma_n sum |t|).
+struct RowAcc {
+  double slo, shi, sabs;              // sum of thisD*w at the low / high end of the enclosure, sum of |thisD*w|
+  unsigned long long mlo, mhi;        // bit patterns of max thisD (non-negative doubles order like their bits)
+  long long wsum, skip, linkw;        // sum of w over counted pairs, skipped pairs, link weights
+};
+struct RowPart {
+  double slo, shi, sabs, mlo, mhi;
+  long long wsum, skip, linkw;
+};
+struct LinkRec {
+  int64_t i, j;
+  double d, lo, hi;
+};
+
+struct ChunkHdr;
+struct RunArgs;
 struct RunArgs {
   // set
   const uint4 *planes;
+  const uint32_t *ambp;
   const uint8_t *onehot;
   const uint8_t *codes, *codesT;
   const int32_t *ev_off;
@@ -208,15 +232,31 @@ struct RunArgs {
   // outputs
   double *D;
   uint16_t *U;
-  unsigned long long *hist;    // [3*200]
+  unsigned long long *hist;    // [NSTRIPE][3*200]: histogram replicas, block b adds into replica b % NSTRIPE
   int32_t *row_links;          // [nrows]
-  unsigned long long *slow_list;
-  unsigned int *slow_cnt;
+  unsigned long long *slow_list;  // [tile][TI*TJ] pairs for the site-order replay, per pair tile
+  unsigned int *slow_cnt;         // [tile]
   ExactRec *exact_list;
   unsigned int *exact_cnt;
   unsigned int exact_cap;
-  unsigned int *stats;         // [0] uncertain pairs, [1] undecided pairs (decision not certified)
+  unsigned int *stripes;       // [NSTRIPE][32]: counters spread over cache lines (ST_* fields), summed on the host
+  int fast;                    // 1: row reductions in RowAcc, links unordered in LinkRec; 0: D/U + k_rows
+  RowAcc *racc;                // [nrows]
+  LinkRec *links;              // [NSTRIPE][link_cap]: region s holds the links counted in stripe s
+  unsigned int link_cap;       // per region
+  // fast path: the pair kernel hands each wave-row (row i, 64 columns of a tile) to k_tail in a fixed slot
+  // (i - r0) * gx + column tile: `need` lanes carry the tail inputs, `skip` lanes had too small an overlap
+  struct ChunkHdr *chdr;
+  uint32_t *cdata;             // [slot][7][64]: nf0..nf3, diag, AG, CT (x16) per lane
+  int64_t gx;                  // column tiles
 };
+
+struct ChunkHdr {
+  unsigned long long need, skip;
+};
+
+constexpr int NSTRIPE = 64;
+enum { ST_UNCERTAIN = 0, ST_UNDECIDED = 1, ST_LINKS = 2, ST_EXACT = 3 };
 
 __device__ inline int fst_bin(const RunArgs &A, int64_t i, int64_t j)
 {
@@ -225,13 +265,41 @@ __device__ inline int fst_bin(const RunArgs &A, int64_t i, int64_t j)
   return 1;
 }
 
-// Everything after the counts of one pair: tests, tail, histogram, link count, D/U (refkernel.cpp:732-815 and the
-// phase C of swifttn93.cpp:2547-2603 up to the row sums, which k_rows does in column order).
+__device__ inline RowPart rowpart_zero() { return RowPart{0., 0., 0., -1., -1., 0, 0, 0}; }
+
+__device__ inline unsigned stripe_of_block() { return (blockIdx.x + blockIdx.y * 7u) % NSTRIPE; }
+
+// links go to the region of the block's stripe (one counter per region, so no single hot atomic)
+__device__ inline void append_link(const RunArgs &A, int64_t i, int64_t j, double x, double lo, double hi)
+{
+  const unsigned st = stripe_of_block();
+  const unsigned k = atomicAdd(&A.stripes[st * 32 + ST_LINKS], 1u);
+  if (k < A.link_cap) A.links[(size_t)st * A.link_cap + k] = LinkRec{i, j, x, lo, hi};
+}
+
+// block-local counters flushed once per block into its stripe
+struct BlockStats {
+  unsigned unc, undecided;
+};
+__device__ inline void flush_block(const RunArgs &A, BlockStats *bs, unsigned long long *lhist, int tid, int nthr)
+{
+  __syncthreads();
+  const unsigned st = stripe_of_block();
+  if (tid == 0) {
+    if (bs->unc) atomicAdd(&A.stripes[st * 32 + ST_UNCERTAIN], bs->unc);
+    if (bs->undecided) atomicAdd(&A.stripes[st * 32 + ST_UNDECIDED], bs->undecided);
+  }
+  if (lhist)
+    for (int k = tid; k < 600; k += nthr)
+      if (lhist[k]) atomicAdd(&A.hist[st * 600 + k], lhist[k]);
+}
+
+// Everything after the counts of one pair: tests, tail, histogram, link (refkernel.cpp:732-815 and the phase C of
+// swifttn93.cpp:2547-2603).  Fast path: the pair's share of its row's reductions in rp; otherwise D/U for k_rows.
 __device__ void finish_pair(const RunArgs &A, int64_t i, int64_t j, const tn93g_tailin &tin,
-                            unsigned long long *lhist)
+                            unsigned long long *lhist, BlockStats *bs, RowPart &rp)
 {
   tn93g_tailout t = tn93g_tail(tin, A.window);
-  const int64_t off = (i - A.r0) * A.ncols + (j - A.c0);
   if (A.exact && t.unc) {
     unsigned k = atomicAdd(A.exact_cnt, 1u);
     if (k < A.exact_cap) {
@@ -245,38 +313,97 @@ __device__ void finish_pair(const RunArgs &A, int64_t i, int64_t j, const tn93g_
   const int bin = fst_bin(A, i, j);
   bool decided = true;
   // histogram bin: floor(200 d) must agree over the enclosure (or clamp to 199 on the same side)
-  double flo = floor(t.dlo * 200.0), fhi = floor(t.dhi * 200.0);
+  const double flo = floor(t.dlo * 200.0), fhi = floor(t.dhi * 200.0);
   if (flo != fhi && !(flo >= 200.0 && fhi >= 200.0) && !(flo < 0.0 && fhi < 0.0)) decided = false;
   atomicAdd(&lhist[bin * 200 + tn93g_hist_index(t.d)], (unsigned long long)w);
-  double x = t.d <= 0. ? 0. : t.d;
-  double xlo = t.dlo <= 0. ? 0. : t.dlo, xhi = t.dhi <= 0. ? 0. : t.dhi;
-  bool in_lo = xlo >= A.mind && xlo <= A.thr, in_hi = xhi >= A.mind && xhi <= A.thr;
-  bool cert_in = xlo >= A.mind && xhi <= A.thr;
-  bool cert_out = xhi < A.mind || xlo > A.thr;
-  (void)in_lo;
-  (void)in_hi;
+  const double x = t.d <= 0. ? 0. : t.d;
+  const double xlo = t.dlo <= 0. ? 0. : t.dlo, xhi = t.dhi <= 0. ? 0. : t.dhi;
+  const bool cert_in = xlo >= A.mind && xhi <= A.thr;
+  const bool cert_out = xhi < A.mind || xlo > A.thr;
   if (!cert_in && !cert_out) decided = false;
-  if (x >= A.mind && x <= A.thr) atomicAdd(&A.row_links[i - A.r0], 1);
+  const bool link = x >= A.mind && x <= A.thr;
+  if (!(x == x)) decided = false;
+  if (A.fast) {
+    if (xlo != xhi) atomicAdd(&bs->unc, 1u);
+    if (!decided) atomicAdd(&bs->undecided, 1u);
+    if (link) append_link(A, i, j, x, xlo, xhi);
+    const double wd = (double)w;
+    double tl = xlo * wd, th = xhi * wd;
+    if (w < 0) {
+      const double q = tl;
+      tl = th;
+      th = q;
+    }
+    rp.slo += tl;
+    rp.shi += th;
+    rp.sabs += fmax(fabs(tl), fabs(th));
+    rp.mlo = fmax(rp.mlo, xlo);
+    rp.mhi = fmax(rp.mhi, xhi);
+    rp.wsum += w;
+    if (link) rp.linkw += w;
+    return;
+  }
+  const int64_t off = (i - A.r0) * A.ncols + (j - A.c0);
+  if (link) atomicAdd(&A.row_links[i - A.r0], 1);
   uint16_t u = 0;
   if (t.unc) {
     uint64_t b = tn93g_bits(x), bl = tn93g_bits(xlo), bh = tn93g_bits(xhi);
     uint64_t dn = b - bl, up = bh - b;
-    if (dn > 255 || up > 255 || !(x == x)) decided = false;
+    if (dn > 255 || up > 255) decided = false;
     u = (uint16_t)((dn > 255 ? 255 : dn) | (up > 255 ? 255 : up) << 8);
-    if (dn || up) atomicAdd(&A.stats[0], 1u);
+    if (dn || up) atomicAdd(&bs->unc, 1u);
   }
-  if (!decided) atomicAdd(&A.stats[1], 1u);
+  if (!decided) atomicAdd(&bs->undecided, 1u);
   A.D[off] = x;
   A.U[off] = u;
 }
 
 // overlap too small: tn93's thisD is -1 (a link only if the caller's range reaches -1)
-__device__ void skip_pair(const RunArgs &A, int64_t i, int64_t j)
+__device__ void skip_pair(const RunArgs &A, int64_t i, int64_t j, RowPart &rp)
 {
+  const bool link = -1.0 >= A.mind && -1.0 <= A.thr;
+  if (A.fast) {
+    rp.skip += 1;
+    if (link) {
+      append_link(A, i, j, -1.0, -1.0, -1.0);
+      rp.linkw += A.weights[i] * A.weights[j];
+    }
+    return;
+  }
   const int64_t off = (i - A.r0) * A.ncols + (j - A.c0);
   A.D[off] = -1.0;
   A.U[off] = 0;
-  if (-1.0 >= A.mind && -1.0 <= A.thr) atomicAdd(&A.row_links[i - A.r0], 1);
+  if (link) atomicAdd(&A.row_links[i - A.r0], 1);
+}
+
+// adds one row's part to its accumulator
+__device__ inline void rowacc_add(const RunArgs &A, int64_t i, const RowPart &rp)
+{
+  RowAcc *r = A.racc + (i - A.r0);
+  if (rp.slo != 0.) atomicAdd(&r->slo, rp.slo);
+  if (rp.shi != 0.) atomicAdd(&r->shi, rp.shi);
+  if (rp.sabs != 0.) atomicAdd(&r->sabs, rp.sabs);
+  if (rp.mlo > 0.) atomicMax(&r->mlo, (unsigned long long)tn93g_bits(rp.mlo));
+  if (rp.mhi > 0.) atomicMax(&r->mhi, (unsigned long long)tn93g_bits(rp.mhi));
+  if (rp.wsum) atomicAdd((unsigned long long *)&r->wsum, (unsigned long long)rp.wsum);
+  if (rp.skip) atomicAdd((unsigned long long *)&r->skip, (unsigned long long)rp.skip);
+  if (rp.linkw) atomicAdd((unsigned long long *)&r->linkw, (unsigned long long)rp.linkw);
+}
+
+__device__ inline double wave_sum_d(double x)
+{
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+__device__ inline double wave_max_d(double x)
+{
+  for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+  return x;
+}
+__device__ inline long long wave_sum_l(long long x)
+{
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
 }
 
 // ------------------------------------------------------------------------------------------ pack kernels
@@ -300,13 +427,13 @@ __global__ void k_transpose(const uint8_t *__restrict__ codes, uint8_t *__restri
 
 // planes: per 32 sites, bit k of .x = site clean, .y = clean and bit0 of the code (C, T), .z = clean and bit1
 // (G, T), .w = code T
-__global__ void k_planes(const uint8_t *__restrict__ codes, uint4 *__restrict__ planes, int64_t n, int64_t L,
-                         int64_t Ls, int64_t W)
+__global__ void k_planes(const uint8_t *__restrict__ codes, uint4 *__restrict__ planes, uint32_t *__restrict__ ambp,
+                         int64_t n, int64_t L, int64_t Ls, int64_t W)
 {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n * W) return;
   int64_t s = t / W, w = t % W;
-  uint32_t c = 0, h = 0, g = 0, tt = 0;
+  uint32_t c = 0, h = 0, g = 0, tt = 0, am = 0;
   for (int k = 0; k < 32; k++) {
     int64_t p = w * 32 + k;
     if (p >= L) break;
@@ -316,23 +443,25 @@ __global__ void k_planes(const uint8_t *__restrict__ codes, uint4 *__restrict__ 
       h |= (code & 1u) << k;
       g |= (code >> 1 & 1u) << k;
       tt |= (code == 3 ? 1u : 0u) << k;
+    } else if (code < (unsigned)GAPC) {
+      am |= 1u << k;
     }
   }
   planes[t] = make_uint4(c, h, g, tt);
+  ambp[t] = am;
 }
 
-// one-hot bytes for the MFMA backend: per sequence, per 64-site block, 4 rows (A, C, G, T) of 64 bytes
+// one-hot bytes for the MFMA backend, in operand order (see the layout above); sequences >= n are all zero
 __global__ void k_onehot(const uint8_t *__restrict__ codes, uint8_t *__restrict__ onehot, int64_t n, int64_t L,
-                         int64_t Ls, int64_t W64)
+                         int64_t Ls, int64_t W64, int64_t npad)
 {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one thread per (sequence, site)
-  int64_t total = n * W64 * 64;
-  if (t >= total) return;
+  if (t >= npad * W64 * 64) return;
   int64_t s = t / (W64 * 64), p = t % (W64 * 64);
-  unsigned code = p < L ? codes[s * Ls + p] : (unsigned)GAPC;
-  int64_t blk = p / 64, k = p % 64;
-  uint8_t *o = onehot + ((s * W64 + blk) * 4) * 64 + k;
-  for (int b = 0; b < 4; b++) o[b * 64] = code == (unsigned)b ? 1 : 0;
+  unsigned code = (s < n && p < L) ? codes[s * Ls + p] : (unsigned)GAPC;
+  int64_t g = s >> 2, sl = s & 3, blk = p >> 6, kg = (p >> 4) & 3, m = p & 15;
+  uint8_t *o = onehot + (((g * W64 + blk) * 4 + kg) * 16 + 4 * sl) * 16 + m;
+  for (int b = 0; b < 4; b++) o[b * 16] = code == (unsigned)b ? 1 : 0;
 }
 
 __global__ void k_evcount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls)
@@ -368,8 +497,9 @@ __global__ void k_evfill(const uint8_t *__restrict__ codes, const int32_t *__res
 }
 
 // ------------------------------------------------------------------------------------------ pair kernel
-// Tile: 32 rows x 64 columns, 256 threads = 4 waves; lane = column j, each wave owns 8 rows (uniform i).
-constexpr int TI = 32, TJ = 64, KW = 16, RPW = 8;
+// Tile: TI = 16 rows x 64 columns, 256 threads = 4 waves; lane = column j, each wave owns RPW = 4 rows (uniform i).
+constexpr int RPW = 4, TI = 4 * RPW, TJ = 64, KW = 16;
+constexpr int EVCAP = 2048;  // events of a tile's rows and columns held in LDS
 
 struct Cnt {
   uint32_t T, DG, TS, CT, B0, B1, BT;
@@ -418,13 +548,133 @@ __device__ inline void ev_flush(EvAcc &e, Tot &o)
   e.px = e.py = e.pz = e.pw = 0;
 }
 
-template <int BACKEND>
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// sum over the 4 lanes of a quad
+__device__ inline uint32_t quad_sum(uint32_t x)
+{
+  x += (uint32_t)__shfl_xor((int)x, 1, 64);
+  x += (uint32_t)__shfl_xor((int)x, 2, 64);
+  return x;
+}
+
+// LDS of the pair kernel.  Per chunk of KW words (popcount) or one 64-site block (MFMA): the bit planes of the tile's
+// rows and columns (both backends: the ambiguity events of the chunk look up their partner's code here), the MFMA
+// operands, and at the end (MFMA) the per-pair counts transposed into the lane = column layout of the epilogue.
+template <int KWORDS, bool MFMA>
+struct PairLds {
+  uint4 sI[KWORDS][TI];
+  uint4 sJ[KWORDS][TJ];
+  uint32_t sIa[KWORDS][TI];
+  union {
+    uint4 ops[(TI + TJ) / 4][64];  // one 64-site block: TI/4 row groups + 16 column groups, 1 KB each
+    uint32_t agg[7][TI][TJ];       // T, DG, TS, CT, B0, B1, BT per pair
+  } u;
+};
+template <int KWORDS>
+struct PairLds<KWORDS, false> {
+  uint4 sI[KWORDS][TI];
+  uint4 sJ[KWORDS][TJ];
+  uint32_t sIa[KWORDS][TI];
+};
+
+// Stages the planes of words [w0, w0 + kw) of the tile's sequences.
+template <class LDS>
+__device__ inline void stage_planes(const RunArgs &A, LDS &lds, int64_t i0, int64_t j0, int64_t w0, int kw, int tid)
+{
+  for (int k = tid; k < kw * (TI + TJ); k += 256) {
+    const int s = k / kw, w = k % kw;  // consecutive threads: consecutive words of one sequence
+    const int64_t seq = s < TI ? i0 + s : j0 + (s - TI);
+    uint4 v = make_uint4(0, 0, 0, 0);
+    uint32_t am = 0;
+    if (seq < A.n) {
+      v = A.planes[seq * A.W + w0 + w];
+      if (s < TI) am = A.ambp[seq * A.W + w0 + w];
+    }
+    if (s < TI) {
+      lds.sI[w][s] = v;
+      lds.sIa[w][s] = am;
+    } else
+      lds.sJ[w][s - TI] = v;
+  }
+}
+
+// The ambiguity events of sites [w0*32, w1*32) for the wave's 8 rows x the lane's column, from the staged planes.
+// Row events (site order, uniform per row): the column's clean base; a column that is not clean there is either a
+// gap (no contribution outside GAPMM) or ambiguous (handled with the column's events).  Column events (per lane):
+// the row's clean base, or its ambiguity code (both ambiguous, looked up in HBM: rare), or a gap.
+// Event lists of a tile: the rows' and the columns' lists are each one contiguous range of the CSR array; when
+// both fit they are read from the LDS copy in sEv (rows at [0, nr), columns at [nr, ...)), else from HBM.
+struct TileEv {
+  const uint32_t *s;
+  int er0, ec0, nr;
+  bool lds;
+  __device__ inline uint32_t row(const RunArgs &A, int e) const { return lds ? s[e - er0] : A.ev[e]; }
+  __device__ inline uint32_t col(const RunArgs &A, int e) const { return lds ? s[nr + e - ec0] : A.ev[e]; }
+};
+
+template <class LDS>
+__device__ inline void events_chunk(const RunArgs &A, const LDS &lds, const uint4 *sTab, const TileEv &te,
+                                    int64_t w0, int64_t w1, int wave, int lane, int64_t ib, int (&ptr)[RPW],
+                                    const int (&end)[RPW], int &pj, int pj_end, EvAcc (&acc)[RPW])
+{
+  const uint32_t lim = (uint32_t)(w1 * 32);
+#pragma unroll
+  for (int a = 0; a < RPW; a++) {
+    int e = ptr[a];
+    for (; e < end[a]; e++) {
+      const uint32_t ev = te.row(A, e);
+      const uint32_t p = ev >> 5;
+      if (p >= lim) break;
+      const uint4 q = lds.sJ[(p >> 5) - w0][lane];
+      const uint32_t bit = p & 31u;
+      if (q.x >> bit & 1u) {
+        const unsigned cb = (q.y >> bit & 1u) | (q.z >> bit & 1u) << 1;
+        ev_add(acc[a], sTab[(ev & 31u) * NCODE + cb]);
+      }
+    }
+    ptr[a] = e;
+  }
+  for (; pj < pj_end; pj++) {
+    const uint32_t ev = te.col(A, pj);
+    const uint32_t p = ev >> 5;
+    if (p >= lim) break;
+    const unsigned cb = ev & 31u;
+    const int w = (int)((p >> 5) - w0);
+    const uint32_t bit = p & 31u;
+#pragma unroll
+    for (int a = 0; a < RPW; a++) {
+      const uint4 q = lds.sI[w][wave * RPW + a];
+      unsigned ca;
+      if (q.x >> bit & 1u)
+        ca = (q.y >> bit & 1u) | (q.z >> bit & 1u) << 1;
+      else if (lds.sIa[w][wave * RPW + a] >> bit & 1u)
+        ca = A.codes[(ib + a) * A.Ls + p];
+      else
+        continue;  // gap
+      ev_add(acc[a], sTab[ca * NCODE + cb]);
+    }
+  }
+}
+
+template <int BACKEND, bool FAST>
 __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
 {
-  const int64_t i0 = A.r0 + (int64_t)blockIdx.y * TI, j0 = A.c0 + (int64_t)blockIdx.x * TJ;
-  if (A.triangle && j0 + TJ - 1 <= i0) return;  // no pair j > i in this tile
-  __shared__ uint4 sI[KW][TI];
-  __shared__ uint4 sJ[KW][TJ];
+  const int64_t c0a = A.c0 & ~(int64_t)63;  // column tiles start on a 64 boundary (whole 4-sequence groups)
+  const int64_t i0 = A.r0 + (int64_t)blockIdx.y * TI, j0 = c0a + (int64_t)blockIdx.x * TJ;
+  const int64_t tile = (int64_t)blockIdx.y * A.gx + blockIdx.x;
+  if (A.triangle && j0 + TJ - 1 <= i0) {  // no pair j > i in this tile
+    if (threadIdx.x == 0) A.slow_cnt[tile] = 0;
+    return;
+  }
+  constexpr bool MF = BACKEND == TN93G_BACKEND_MFMA;
+  __shared__ unsigned lslow;
+  __shared__ BlockStats bs;
+  if (threadIdx.x == 0) {
+    lslow = 0;
+    bs.unc = bs.undecided = 0;
+  }
+  __shared__ PairLds<MF ? 2 : KW, MF> lds;
   __shared__ uint4 sTab[NCODE * NCODE];
   __shared__ unsigned long long lhist[3 * 200];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -434,127 +684,253 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   const int64_t jl = j < A.n ? j : A.n - 1;
   const int64_t ib = i0 + wave * RPW;
   Cnt cnt[RPW];
-#pragma unroll
-  for (int a = 0; a < RPW; a++) cnt[a] = Cnt{0, 0, 0, 0, 0, 0, 0};
-  const bool do_fast = !A.gapmm && !A.all_slow;
-  if (do_fast) {
-    for (int64_t w0 = 0; w0 < A.W; w0 += KW) {
-      __syncthreads();
-      for (int k = tid; k < KW * (TI + TJ); k += 256) {
-        int s = k / KW, w = k % KW;  // consecutive threads: consecutive words of one sequence
-        int64_t seq = s < TI ? i0 + s : j0 + (s - TI);
-        uint4 v = make_uint4(0, 0, 0, 0);
-        if (seq < A.n && w0 + w < A.W) v = A.planes[seq * A.W + w0 + w];
-        if (s < TI)
-          sI[w][s] = v;
-        else
-          sJ[w][s - TI] = v;
-      }
-      __syncthreads();
-      const int kw = (int)min((int64_t)KW, A.W - w0);
-      for (int k = 0; k < kw; k++) {
-        const uint4 pj = sJ[k][lane];
-#pragma unroll
-        for (int a = 0; a < RPW; a++) count_word(sI[k][wave * RPW + a], pj, cnt[a]);
-      }
-    }
-  }
-  __syncthreads();
-  // totals x16 from the clean/clean counts
   Tot tot[RPW];
   EvAcc acc[RPW];
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
-    const Cnt &c = cnt[a];
-    uint32_t nf3 = c.BT, nf1 = c.B0 - c.BT, nf2 = c.B1 - c.BT, nf0 = 2 * c.T - c.B0 - c.B1 + c.BT;
-    tot[a].nf[0] = 16 * nf0;
-    tot[a].nf[1] = 16 * nf1;
-    tot[a].nf[2] = 16 * nf2;
-    tot[a].nf[3] = 16 * nf3;
-    tot[a].dg = 16 * c.DG;
-    tot[a].ag = 16 * (c.TS - c.CT);
-    tot[a].ct = 16 * c.CT;
-    tot[a].amb = 0;
+    cnt[a] = Cnt{0, 0, 0, 0, 0, 0, 0};
+    tot[a] = Tot{{0, 0, 0, 0}, 0, 0, 0, 0};
     acc[a] = EvAcc{0, 0, 0, 0, 0};
   }
-  if (do_fast && !A.skip_events) {
-    // events of each row i (uniform loop): the column's code at that site from the site-major copy
-#pragma unroll
-    for (int a = 0; a < RPW; a++) {
-      const int64_t i = ib + a;
-      if (i >= A.r1) break;
-      const int e0 = A.ev_off[i], e1 = A.ev_off[i + 1];
-      int since = 0;
-      for (int e = e0; e < e1; e++) {
-        const uint32_t ev = A.ev[e];
-        const int64_t p = ev >> 5;
-        const unsigned ca = ev & 31u;
-        const unsigned cb = A.codesT[p * A.npad + jl];
-        ev_add(acc[a], sTab[ca * NCODE + cb]);
-        if (++since == 2047) {
-          ev_flush(acc[a], tot[a]);
-          since = 0;
-        }
-      }
-      ev_flush(acc[a], tot[a]);
-    }
-    // events of the column j (per lane): the 8 rows' codes at that site in one 8-byte load; sites where the row
-    // is itself ambiguous were counted above
-    {
-      const int f0 = A.ev_off[jl], f1 = A.ev_off[jl + 1];
-      int since = 0;
-      for (int e = f0; e < f1; e++) {
-        const uint32_t ev = A.ev[e];
-        const int64_t p = ev >> 5;
-        const unsigned cb = ev & 31u;
-        const uint2 ci = *reinterpret_cast<const uint2 *>(A.codesT + p * A.npad + ib);
-#pragma unroll
-        for (int a = 0; a < RPW; a++) {
-          const unsigned ca = ((a < 4 ? ci.x : ci.y) >> (8 * (a & 3))) & 255u;
-          if (ca < 4 || ca >= (unsigned)GAPC) ev_add(acc[a], sTab[ca * NCODE + cb]);
-        }
-        if (++since == 2047) {
-#pragma unroll
-          for (int a = 0; a < RPW; a++) ev_flush(acc[a], tot[a]);
-          since = 0;
-        }
-      }
-#pragma unroll
-      for (int a = 0; a < RPW; a++) ev_flush(acc[a], tot[a]);
-    }
-  }
-  // per pair: tests, replay routing, tail
+  // event cursors: rows past the set get empty lists
+  int ptr[RPW], pend[RPW];
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
     const int64_t i = ib + a;
-    const bool valid = i < A.r1 && j < A.c1 && (!A.triangle || j > i);
-    if (!valid) continue;
-    if (!do_fast || (acc[a].fl >> 31)) {  // order-dependent: exact site-order replay
-      unsigned k = atomicAdd(A.slow_cnt, 1u);
-      A.slow_list[k] = (unsigned long long)i << 32 | (unsigned long long)j;
-      continue;
+    ptr[a] = i < A.r1 ? A.ev_off[i] : 0;
+    pend[a] = i < A.r1 ? A.ev_off[i + 1] : 0;
+  }
+  int pj = A.ev_off[jl], pj_end = A.ev_off[jl + 1];
+  const bool do_fast = !A.gapmm && !A.all_slow;
+  const bool do_events = do_fast && !A.skip_events;
+  // the tile's event lists into LDS (synchronised by the first staging barrier below)
+  __shared__ uint32_t sEv[EVCAP];
+  TileEv te;
+  {
+    const int64_t rl = i0 < A.r1 ? min(i0 + (int64_t)TI, A.r1) : i0;
+    const int64_t cl = min(j0 + (int64_t)TJ, A.n);
+    te.s = sEv;
+    te.er0 = i0 < A.r1 ? A.ev_off[i0] : 0;
+    te.nr = i0 < A.r1 ? A.ev_off[rl] - te.er0 : 0;
+    te.ec0 = A.ev_off[j0];
+    const int nc = A.ev_off[cl] - te.ec0;
+    te.lds = do_events && te.nr + nc <= EVCAP;
+    if (te.lds) {
+      for (int k = tid; k < te.nr; k += 256) sEv[k] = A.ev[te.er0 + k];
+      for (int k = tid; k < nc; k += 256) sEv[te.nr + k] = A.ev[te.ec0 + k];
     }
-    const Tot &o = tot[a];
-    tn93g_tailin tin;
-    for (int k = 0; k < 4; k++) tin.nf[k] = (double)o.nf[k] * 0.0625;
-    const double tng = (double)(o.nf[0] + o.nf[1] + o.nf[2] + o.nf[3]) * 0.03125;
-    if (tng <= (double)A.min_overlap) {  // refkernel.cpp:732-735
-      skip_pair(A, i, j);
-      continue;
+  }
+  if constexpr (!MF) {
+    if (do_fast)
+    for (int64_t w0 = 0; w0 < A.W; w0 += KW) {
+      const int kw = (int)min((int64_t)KW, A.W - w0);
+      __syncthreads();
+      stage_planes(A, lds, i0, j0, w0, kw, tid);
+      __syncthreads();
+      for (int k = 0; k < kw; k++) {
+        const uint4 pjv = lds.sJ[k][lane];
+#pragma unroll
+        for (int a = 0; a < RPW; a++) count_word(lds.sI[k][wave * RPW + a], pjv, cnt[a]);
+      }
+      if (do_events) {
+        events_chunk(A, lds, sTab, te, w0, w0 + kw, wave, lane, ib, ptr, pend, pj, pj_end, acc);
+#pragma unroll
+        for (int a = 0; a < RPW; a++) ev_flush(acc[a], tot[a]);  // <= 2 * 512 events per chunk: 16 bits suffice
+      }
     }
-    if (A.avg_fallback && tng * A.rf <= (double)o.amb) {  // refkernel.cpp:742-750: AVERAGE, via the replay
-      unsigned k = atomicAdd(A.slow_cnt, 1u);
-      A.slow_list[k] = (unsigned long long)i << 32 | (unsigned long long)j;
-      continue;
+  }
+  if constexpr (MF) {
+    if (do_fast) {
+    // wave w: all TI/4 row groups x column groups 4w..4w+3; one v_mfma_i32_16x16x64_i8 per (row group, column group)
+    // and 64-site block gives the 4x4 base table of 16 pairs in int32
+    v4i macc[TI / 4][4];
+#pragma unroll
+    for (int ig = 0; ig < TI / 4; ig++)
+#pragma unroll
+      for (int jg = 0; jg < 4; jg++) macc[ig][jg] = v4i{0, 0, 0, 0};
+    const uint4 *gI = reinterpret_cast<const uint4 *>(A.onehot) + (i0 >> 2) * A.W64 * 64;
+    const uint4 *gJ = reinterpret_cast<const uint4 *>(A.onehot) + (j0 >> 2) * A.W64 * 64;
+    for (int64_t blk = 0; blk < A.W64; blk++) {
+      const int kw = (int)min((int64_t)2, A.W - 2 * blk);
+      __syncthreads();
+      for (int k = tid; k < (TI + TJ) / 4 * 64; k += 256) {
+        const int g = k >> 6, q = k & 63;
+        lds.u.ops[g][q] = g < TI / 4 ? gI[((int64_t)g * A.W64 + blk) * 64 + q]
+                                     : gJ[((int64_t)(g - TI / 4) * A.W64 + blk) * 64 + q];
+      }
+      if (do_events) stage_planes(A, lds, i0, j0, 2 * blk, kw, tid);
+      __syncthreads();
+      v4i bop[4];
+#pragma unroll
+      for (int jg = 0; jg < 4; jg++) {
+        const uint4 v = lds.u.ops[TI / 4 + 4 * wave + jg][lane];
+        bop[jg] = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+      }
+#pragma unroll
+      for (int ig = 0; ig < TI / 4; ig++) {
+        const uint4 v = lds.u.ops[ig][lane];
+        const v4i aop = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+#pragma unroll
+        for (int jg = 0; jg < 4; jg++)
+          macc[ig][jg] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aop, bop[jg], macc[ig][jg], 0, 0, 0);
+      }
+      if (do_events) {
+        events_chunk(A, lds, sTab, te, 2 * blk, 2 * blk + kw, wave, lane, ib, ptr, pend, pj, pj_end, acc);
+#pragma unroll
+        for (int a = 0; a < RPW; a++) ev_flush(acc[a], tot[a]);
+      }
     }
-    tin.diag = (double)o.dg * 0.0625;
-    tin.ag = (double)o.ag * 0.0625;
-    tin.ct = (double)o.ct * 0.0625;
-    finish_pair(A, i, j, tin, lhist);
+    __syncthreads();
+    // lane l of tile (ig, jg) holds C[a][b] for a = 0..3 of the pair (row 4ig + l/16, column 4(4w+jg) + (l%16)/4)
+    // with b = l%4; reduce over the quad into the seven counts of the popcount backend
+    const int b = lane & 3;
+#pragma unroll
+    for (int ig = 0; ig < TI / 4; ig++)
+#pragma unroll
+      for (int jg = 0; jg < 4; jg++) {
+        const v4i v = macc[ig][jg];
+        const uint32_t v0 = (uint32_t)v.x, v1 = (uint32_t)v.y, v2 = (uint32_t)v.z, v3 = (uint32_t)v.w;
+        const uint32_t col = v0 + v1 + v2 + v3;
+        const uint32_t dg = b == 0 ? v0 : b == 1 ? v1 : b == 2 ? v2 : v3;
+        const uint32_t ct = b == 1 ? v3 : b == 3 ? v1 : 0u;
+        const uint32_t ts = ct + (b == 0 ? v2 : b == 2 ? v0 : 0u);
+        const uint32_t b0 = v1 + v3 + ((b & 1) ? col : 0u);
+        const uint32_t b1 = v2 + v3 + ((b >> 1) ? col : 0u);
+        const uint32_t bt = v3 + (b == 3 ? col : 0u);
+        const uint32_t q0 = quad_sum(col), q1 = quad_sum(dg), q2 = quad_sum(ts), q3 = quad_sum(ct);
+        const uint32_t q4 = quad_sum(b0), q5 = quad_sum(b1), q6 = quad_sum(bt);
+        const int il = 4 * ig + (lane >> 4), jloc = 4 * (4 * wave + jg) + ((lane & 15) >> 2);
+        // lane b of the quad stores fields b and b + 4
+        lds.u.agg[b][il][jloc] = b == 0 ? q0 : b == 1 ? q1 : b == 2 ? q2 : q3;
+        if (b < 3) lds.u.agg[b + 4][il][jloc] = b == 0 ? q4 : b == 1 ? q5 : q6;
+      }
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < RPW; a++) {
+      const int il = wave * RPW + a;
+      cnt[a] = Cnt{lds.u.agg[0][il][lane], lds.u.agg[1][il][lane], lds.u.agg[2][il][lane], lds.u.agg[3][il][lane],
+                   lds.u.agg[4][il][lane], lds.u.agg[5][il][lane], lds.u.agg[6][il][lane]};
+    }
+    }
   }
   __syncthreads();
-  for (int k = tid; k < 600; k += 256)
-    if (lhist[k]) atomicAdd(&A.hist[k], lhist[k]);
+  // totals x16 from the clean/clean counts, plus the events
+#pragma unroll
+  for (int a = 0; a < RPW; a++) {
+    const Cnt &c = cnt[a];
+    const uint32_t nf3 = c.BT, nf1 = c.B0 - c.BT, nf2 = c.B1 - c.BT, nf0 = 2 * c.T - c.B0 - c.B1 + c.BT;
+    tot[a].nf[0] += 16 * nf0;
+    tot[a].nf[1] += 16 * nf1;
+    tot[a].nf[2] += 16 * nf2;
+    tot[a].nf[3] += 16 * nf3;
+    tot[a].dg += 16 * c.DG;
+    tot[a].ag += 16 * (c.TS - c.CT);
+    tot[a].ct += 16 * c.CT;
+  }
+  // per pair: tests and routing.  Fast path: the tail inputs go to k_tail in one chunk per wave-row; exact path:
+  // the tail right here, into D/U
+#pragma unroll
+  for (int a = 0; a < RPW; a++) {
+    const int64_t i = ib + a;
+    const bool valid = i < A.r1 && j >= A.c0 && j < A.c1 && (!A.triangle || j > i);
+    int state = 0;  // 1: tail, 2: overlap too small
+    const Tot &o = tot[a];
+    if (valid) {
+      bool slow = !do_fast || (acc[a].fl >> 31);  // order-dependent: exact site-order replay
+      if (!slow) {
+        const double tng = (double)(o.nf[0] + o.nf[1] + o.nf[2] + o.nf[3]) * 0.03125;
+        if (tng <= (double)A.min_overlap)  // refkernel.cpp:732-735
+          state = 2;
+        else if (A.avg_fallback && tng * A.rf <= (double)o.amb)  // refkernel.cpp:742-750: AVERAGE, replayed
+          slow = true;
+        else
+          state = 1;
+      }
+      if (slow) {
+        const unsigned k = atomicAdd(&lslow, 1u);  // < TI * TJ
+        A.slow_list[tile * (TI * TJ) + k] = (unsigned long long)i << 32 | (unsigned long long)j;
+      }
+    }
+    if constexpr (FAST) {
+      const unsigned long long need = __ballot(state == 1), skp = __ballot(state == 2);
+      if (i < A.r1) {
+        const size_t slot = (size_t)(i - A.r0) * (size_t)A.gx + blockIdx.x;
+        if (lane == 0) A.chdr[slot] = ChunkHdr{need, skp};
+        if (state == 1) {
+          uint32_t *d = A.cdata + slot * (7 * 64) + lane;
+          d[0] = o.nf[0];
+          d[64] = o.nf[1];
+          d[128] = o.nf[2];
+          d[192] = o.nf[3];
+          d[256] = o.dg;
+          d[320] = o.ag;
+          d[384] = o.ct;
+        }
+      }
+    } else if (state) {
+      RowPart rp = rowpart_zero();  // unused: the exact path writes D/U
+      if (state == 2) {
+        skip_pair(A, i, j, rp);
+      } else {
+        tn93g_tailin tin;
+        for (int k = 0; k < 4; k++) tin.nf[k] = (double)o.nf[k] * 0.0625;
+        tin.diag = (double)o.dg * 0.0625;
+        tin.ag = (double)o.ag * 0.0625;
+        tin.ct = (double)o.ct * 0.0625;
+        finish_pair(A, i, j, tin, lhist, &bs, rp);
+      }
+    }
+  }
+  flush_block(A, &bs, FAST ? nullptr : lhist, tid, 256);
+  if (tid == 0) A.slow_cnt[tile] = lslow;
+}
+
+// The FP64 tail of the fast path: one wave per chunk (one row, 64 columns), high occupancy so the long dependent
+// chains of divisions and logarithms of many pairs overlap; the row's reductions are summed over the wave.
+__global__ __launch_bounds__(256) void k_tail(RunArgs A)
+{
+  __shared__ unsigned long long lhist[3 * 200];
+  __shared__ BlockStats bs;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  for (int k = tid; k < 600; k += 256) lhist[k] = 0;
+  if (tid == 0) bs.unc = bs.undecided = 0;
+  __syncthreads();
+  const int64_t nrows = A.r1 - A.r0, nslots = nrows * A.gx;
+  const int64_t c0a = A.c0 & ~(int64_t)63;
+  for (int64_t k = (int64_t)blockIdx.x * 4 + wave; k < nslots; k += (int64_t)gridDim.x * 4) {
+    const int64_t r = k / A.gx, bx = k % A.gx;
+    const int64_t i = A.r0 + r, j0 = c0a + bx * TJ;
+    if (A.triangle && j0 + TJ - 1 <= A.r0 + (r / TI) * TI) continue;  // the pair kernel skipped this tile
+    const ChunkHdr h = A.chdr[k];
+    if (!(h.need | h.skip)) continue;
+    const int64_t j = j0 + lane;
+    RowPart rp = rowpart_zero();
+    if (h.need >> lane & 1ull) {
+      const uint32_t *d = A.cdata + (size_t)k * (7 * 64) + lane;
+      tn93g_tailin tin;
+      tin.nf[0] = (double)d[0] * 0.0625;
+      tin.nf[1] = (double)d[64] * 0.0625;
+      tin.nf[2] = (double)d[128] * 0.0625;
+      tin.nf[3] = (double)d[192] * 0.0625;
+      tin.diag = (double)d[256] * 0.0625;
+      tin.ag = (double)d[320] * 0.0625;
+      tin.ct = (double)d[384] * 0.0625;
+      finish_pair(A, i, j, tin, lhist, &bs, rp);
+    } else if (h.skip >> lane & 1ull) {
+      skip_pair(A, i, j, rp);
+    }
+    RowPart q;
+    q.slo = wave_sum_d(rp.slo);
+    q.shi = wave_sum_d(rp.shi);
+    q.sabs = wave_sum_d(rp.sabs);
+    q.mlo = wave_max_d(rp.mlo);
+    q.mhi = wave_max_d(rp.mhi);
+    q.wsum = wave_sum_l(rp.wsum);
+    q.skip = wave_sum_l(rp.skip);
+    q.linkw = wave_sum_l(rp.linkw);
+    if (lane == 0) rowacc_add(A, i, q);
+  }
+  flush_block(A, &bs, lhist, tid, 256);
 }
 
 // ------------------------------------------------------------------------------------------ replay kernel
@@ -618,38 +994,126 @@ __device__ bool replay_scan(const RunArgs &A, int64_t i, int64_t j, const Replay
   return true;
 }
 
+// The clean/clean 4x4 integer table of a pair from the bit planes (integer_counts before any resolved ambiguity).
+__device__ void cc_table(const RunArgs &A, int64_t i, int64_t j, int (*icc)[RB], int t)
+{
+  uint32_t c[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) c[k] = 0;
+  const uint4 *Pi = A.planes + i * A.W, *Pj = A.planes + j * A.W;
+  for (int64_t w = 0; w < A.W; w++) {
+    const uint4 pi = Pi[w], pj = Pj[w];
+    const uint32_t M = pi.x & pj.x;
+    const uint32_t ai[4] = {M & ~(pi.y | pi.z), M & pi.y & ~pi.z, M & pi.z & ~pi.y, M & pi.w};
+    const uint32_t bj[4] = {~(pj.y | pj.z), pj.y & ~pj.z, pj.z & ~pj.y, pj.w};
+#pragma unroll
+    for (int x = 0; x < 4; x++)
+#pragma unroll
+      for (int y = 0; y < 4; y++) c[4 * x + y] += __popc(ai[x] & bj[y]);
+  }
+#pragma unroll
+  for (int k = 0; k < 16; k++) icc[k][t] = (int)c[k];
+}
+
+// computeTN93's ambiguity handling in site order (refkernel.cpp:384-729) for a pair whose contributions depend on
+// the order: only the sites where either sequence is ambiguous contribute to float_counts, so the two sorted event
+// lists are merged and each event applied in FP64 in increasing site order; then tn93's merge.
+__device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *__restrict__ tab,
+                              double (*fc)[RB], int (*ic)[RB], int (*icc)[RB], int t, tn93g_tailin &tin, double &tng,
+                              uint32_t &amb)
+{
+  for (int k = 0; k < 16; k++) {
+    fc[k][t] = 0.0;
+    ic[k][t] = icc[k][t];
+  }
+  amb = 0;
+  int a = A.ev_off[i], ae = A.ev_off[i + 1], b = A.ev_off[j], be = A.ev_off[j + 1];
+  const uint8_t *si = A.codes + i * A.Ls, *sj = A.codes + j * A.Ls;
+  while (a < ae || b < be) {
+    const uint32_t ua = a < ae ? A.ev[a] : 0xFFFFFFFFu, ub = b < be ? A.ev[b] : 0xFFFFFFFFu;
+    const uint32_t pa = ua >> 5, pb = ub >> 5;
+    unsigned c1, c2;
+    if (pa == pb) {
+      c1 = ua & 31u;
+      c2 = ub & 31u;
+      a++;
+      b++;
+    } else if (pa < pb) {
+      c1 = ua & 31u;
+      c2 = sj[pa];
+      a++;
+    } else {
+      c1 = si[pb];
+      c2 = ub & 31u;
+      b++;
+    }
+    const ReplayEnt E = tab[c1 * NCODE + c2];
+    uint32_t m = E.mask;
+    while (m) {
+      const int c = __builtin_ctz(m);
+      m &= m - 1;
+      fc[c][t] += E.v;
+    }
+    if (E.icd >= 0) ic[5 * E.icd][t]++;
+    amb += (uint32_t)E.amb;
+  }
+  double nf[4] = {0., 0., 0., 0.};
+  tng = 0.;
+  for (int x = 0; x < 4; x++)
+    for (int y = 0; y < 4; y++) {
+      double pc = (fc[4 * x + y][t] += (double)ic[4 * x + y][t]);
+      tng += pc;
+      nf[x] += pc;
+      nf[y] += pc;
+    }
+  for (int k = 0; k < 4; k++) tin.nf[k] = nf[k];
+  tin.diag = fc[0][t] + fc[5][t] + fc[10][t] + fc[15][t];
+  tin.ag = fc[2][t] + fc[8][t];
+  tin.ct = fc[7][t] + fc[13][t];
+}
+
+// one block per pair tile: the tile's pairs that need the site-order replay
 __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
 {
   __shared__ double fc[16][RB];
   __shared__ int ic[16][RB];
+  __shared__ int icc[16][RB];
   __shared__ unsigned long long lhist[3 * 200];
+  __shared__ BlockStats bs;
   const int t = threadIdx.x;
   for (int k = t; k < 600; k += RB) lhist[k] = 0;
+  if (t == 0) bs.unc = bs.undecided = 0;
   __syncthreads();
-  const unsigned cnt = *A.slow_cnt;
-  for (unsigned k = blockIdx.x * RB + t; k < cnt; k += gridDim.x * RB) {
-    const unsigned long long pr = A.slow_list[k];
+  const int64_t tile = (int64_t)blockIdx.y * A.gx + blockIdx.x;
+  const unsigned cnt = A.slow_cnt[tile];
+  for (unsigned k = t; k < cnt; k += RB) {
+    const unsigned long long pr = A.slow_list[tile * (TI * TJ) + k];
     const int64_t i = (int64_t)(pr >> 32), j = (int64_t)(pr & 0xffffffffull);
     tn93g_tailin tin;
     double tng;
     uint32_t amb;
-    replay_scan(A, i, j, A.rtab, fc, ic, t, tin, tng, amb);
-    if (tng <= (double)A.min_overlap) {
-      skip_pair(A, i, j);
-      continue;
+    if (A.gapmm) {  // gaps facing bases are events too: scan every site
+      replay_scan(A, i, j, A.rtab, fc, ic, t, tin, tng, amb);
+    } else {
+      cc_table(A, i, j, icc, t);
+      replay_events(A, i, j, A.rtab, fc, ic, icc, t, tin, tng, amb);
     }
-    if (A.avg_fallback && tng * A.rf <= (double)amb) {  // recompute in AVERAGE (refkernel.cpp:742-750)
-      replay_scan(A, i, j, A.rtab + NCODE * NCODE, fc, ic, t, tin, tng, amb);
-      if (tng <= (double)A.min_overlap) {
-        skip_pair(A, i, j);
-        continue;
-      }
+    RowPart rp = rowpart_zero();
+    bool skip = tng <= (double)A.min_overlap;
+    if (!skip && A.avg_fallback && tng * A.rf <= (double)amb) {  // recompute in AVERAGE (refkernel.cpp:742-750)
+      if (A.gapmm)
+        replay_scan(A, i, j, A.rtab + NCODE * NCODE, fc, ic, t, tin, tng, amb);
+      else
+        replay_events(A, i, j, A.rtab + NCODE * NCODE, fc, ic, icc, t, tin, tng, amb);
+      skip = tng <= (double)A.min_overlap;
     }
-    finish_pair(A, i, j, tin, lhist);
+    if (skip)
+      skip_pair(A, i, j, rp);
+    else
+      finish_pair(A, i, j, tin, lhist, &bs, rp);
+    if (A.fast) rowacc_add(A, i, rp);
   }
-  __syncthreads();
-  for (int k = t; k < 600; k += RB)
-    if (lhist[k]) atomicAdd(&A.hist[k], lhist[k]);
+  flush_block(A, &bs, lhist, t, RB);
 }
 
 // host-exact distances back into D (after TN93G_FLAG_EXACT)
@@ -705,12 +1169,18 @@ struct RowOut {
   const int64_t *link_off;
 };
 
-__global__ void k_rows(RunArgs A, RowOut O)
+constexpr int RT = 64;  // rows per block (one wave) and columns per tile of the row kernel
+
+__global__ __launch_bounds__(RT) void k_rows(RunArgs A, RowOut O)
 {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double sD[RT][RT + 1];
+  __shared__ uint16_t sU[RT][RT + 2];
+  __shared__ int64_t sW[RT];
+  const int lane = threadIdx.x;
   const int64_t nrows = A.r1 - A.r0;
-  if (r >= nrows) return;
-  const int64_t i = A.r0 + r;
+  const int64_t rb = (int64_t)blockIdx.x * RT, r = rb + lane;
+  const bool live = r < nrows;
+  const int64_t i = A.r0 + (live ? r : nrows - 1);
   const int64_t lower = A.triangle ? i + 1 : A.c0;
   const int64_t wi = A.weights[i];
   double slo[3] = {0., 0., 0.}, shi[3] = {0., 0., 0.}, ws[3] = {0., 0., 0.}, mlo[3] = {0., 0., 0.},
@@ -719,49 +1189,81 @@ __global__ void k_rows(RunArgs A, RowOut O)
     unsigned long long svs = (unsigned long long)((wi * (wi - 1)) >> 1);
     ws[(A.fst && i >= A.n1) ? 1 : 0] += (double)svs;
   }
-  int64_t skipped = 0, linkw = 0, k = O.link_off[r];
-  const int64_t kend = O.link_off[r + 1];
-  const double *Drow = A.D + r * A.ncols - A.c0;
-  const uint16_t *Urow = A.U + r * A.ncols - A.c0;
-  for (int64_t j = lower; j < A.c1; j++) {
-    const double x = Drow[j];
-    const int bin = fst_bin(A, i, j);
-    const int64_t w = wi * A.weights[j];
-    const uint16_t u = Urow[j];
-    const double lo = u ? tn93g_from_bits(tn93g_bits(x) - (u & 255u)) : x;
-    const double hi = u ? tn93g_from_bits(tn93g_bits(x) + (u >> 8)) : x;
-    if (x >= A.mind && x <= A.thr && k < kend) {
-      linkw += w;
-      O.link_row[k] = i;
-      O.link_col[k] = j;
-      O.link_d[k] = x;
-      O.link_lo[k] = lo;
-      O.link_hi[k] = hi;
-      k++;
+  int64_t skipped = 0, linkw = 0, k = O.link_off[live ? r : 0];
+  const int64_t kend = live ? O.link_off[r + 1] : k;
+  // column tiles: from the tile holding the block's first column to the end
+  const int64_t cfirst = A.triangle ? A.r0 + rb + 1 : A.c0;
+  const int64_t jt0 = cfirst & ~(int64_t)(RT - 1);
+  // tile loader: lane = column; rows rb .. rb + 63 of the tile
+  double nd[RT];
+  uint16_t nu[RT];
+  auto load = [&](int64_t jt) {
+    const int64_t col = jt + lane;
+    const bool cok = col >= A.c0 && col < A.c1;
+#pragma unroll
+    for (int rr = 0; rr < RT; rr++) {
+      const int64_t row = rb + rr;
+      const bool ok = cok && row < nrows;
+      const int64_t off = ok ? row * A.ncols + (col - A.c0) : 0;
+      nd[rr] = ok ? A.D[off] : 0.0;
+      nu[rr] = ok ? A.U[off] : (uint16_t)0;
     }
-    if (x <= -0.5) {
-      skipped++;
-    } else {
-      const double wd = (double)w;
-      double tl = lo * wd, th = hi * wd;
-      if (w < 0) {
-        double q = tl;
-        tl = th;
-        th = q;
+  };
+  if (jt0 < A.c1) load(jt0);
+  for (int64_t jt = jt0; jt < A.c1; jt += RT) {
+    __syncthreads();
+#pragma unroll
+    for (int rr = 0; rr < RT; rr++) {
+      sD[rr][lane] = nd[rr];
+      sU[rr][lane] = nu[rr];
+    }
+    sW[lane] = (jt + lane < A.c1) ? A.weights[jt + lane] : 0;
+    __syncthreads();
+    if (jt + RT < A.c1) load(jt + RT);  // next tile in flight while this one is reduced
+    if (!live) continue;
+    const int64_t jb = max(jt, lower), je = min(jt + RT, A.c1);
+    for (int64_t j = jb; j < je; j++) {
+      const int jj = (int)(j - jt);
+      const double x = sD[lane][jj];
+      const uint16_t u = sU[lane][jj];
+      const int bin = fst_bin(A, i, j);
+      const int64_t w = wi * sW[jj];
+      const double lo = u ? tn93g_from_bits(tn93g_bits(x) - (u & 255u)) : x;
+      const double hi = u ? tn93g_from_bits(tn93g_bits(x) + (u >> 8)) : x;
+      if (x >= A.mind && x <= A.thr && k < kend) {
+        linkw += w;
+        O.link_row[k] = i;
+        O.link_col[k] = j;
+        O.link_d[k] = x;
+        O.link_lo[k] = lo;
+        O.link_hi[k] = hi;
+        k++;
       }
-      slo[bin] += tl;
-      shi[bin] += th;
-      ws[bin] += wd;
-      if (lo > mlo[bin]) mlo[bin] = lo;
-      if (hi > mhi[bin]) mhi[bin] = hi;
+      if (x <= -0.5) {
+        skipped++;
+      } else {
+        const double wd = (double)w;
+        double tl = lo * wd, th = hi * wd;
+        if (w < 0) {
+          const double q = tl;
+          tl = th;
+          th = q;
+        }
+        slo[bin] += tl;
+        shi[bin] += th;
+        ws[bin] += wd;
+        if (lo > mlo[bin]) mlo[bin] = lo;
+        if (hi > mhi[bin]) mhi[bin] = hi;
+      }
     }
   }
-  for (int b = 0; b < 3; b++) {
-    O.sum_lo[b * nrows + r] = slo[b];
-    O.sum_hi[b * nrows + r] = shi[b];
-    O.wsum[b * nrows + r] = ws[b];
-    O.max_lo[b * nrows + r] = mlo[b];
-    O.max_hi[b * nrows + r] = mhi[b];
+  if (!live) return;
+  for (int bb = 0; bb < 3; bb++) {
+    O.sum_lo[bb * nrows + r] = slo[bb];
+    O.sum_hi[bb * nrows + r] = shi[bb];
+    O.wsum[bb * nrows + r] = ws[bb];
+    O.max_lo[bb * nrows + r] = mlo[bb];
+    O.max_hi[bb * nrows + r] = mhi[bb];
   }
   O.skipped[r] = skipped;
   O.linkw[r] = linkw;
@@ -783,7 +1285,15 @@ std::string fmt(const char *f, ...)
 struct tn93g_set {
   tn93g_ctx *ctx;
   SetDev d;
+  std::vector<int64_t> hw;  // copy numbers, host copy
 };
+
+static double tn93g_host_from_bits(unsigned long long u)
+{
+  double x;
+  memcpy(&x, &u, 8);
+  return x;
+}
 
 struct DevBuf {
   void *p = nullptr;
@@ -795,8 +1305,10 @@ struct tn93g_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4];
   std::string err;
-  DevBuf D, U, hist, row_links, link_off, slow_list, counters, exact_list, rowout, links, evtab, rtab, patch;
+  DevBuf D, U, hist, row_links, link_off, slow_list, slow_cnt, counters, exact_list, rowout, links, evtab, rtab, patch,
+      chdr, cdata;
   unsigned long long *pin_counters = nullptr;  // host pinned copy of the counters
+  int64_t link_cap_hint = 0;                   // link buffer size that sufficed last time
 
   int fail(int code, const std::string &m)
   {
@@ -855,7 +1367,8 @@ void tn93g_close(tn93g_ctx *c)
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (DevBuf *b : {&c->D, &c->U, &c->hist, &c->row_links, &c->link_off, &c->slow_list, &c->counters,
-                    &c->exact_list, &c->rowout, &c->links, &c->evtab, &c->rtab, &c->patch})
+                    &c->exact_list, &c->rowout, &c->links, &c->evtab, &c->rtab, &c->patch, &c->chdr, &c->cdata,
+                    &c->slow_cnt})
     if (b->p) (void)hipFree(b->p);
   for (auto &e : c->ev) (void)hipEventDestroy(e);
   if (c->pin_counters) (void)hipHostFree(c->pin_counters);
@@ -889,6 +1402,7 @@ int tn93g_set_upload(tn93g_ctx *c, const uint8_t *codes, int64_t n, int64_t L, c
   std::vector<int64_t> wts((size_t)nn, 1);
   if (weights)
     for (int64_t s = 0; s < n; s++) wts[(size_t)s] = weights[s];
+  S->hw.assign(wts.begin(), wts.begin() + n);
   auto fail = [&](int code, const std::string &m) {
     tn93g_set_free(S);
     return c->fail(code, m);
@@ -901,7 +1415,8 @@ int tn93g_set_upload(tn93g_ctx *c, const uint8_t *codes, int64_t n, int64_t L, c
   SCHK(hipMalloc(&d.codes, padded.size()));
   SCHK(hipMalloc(&d.codesT, (size_t)(d.Ls * d.npad)));
   SCHK(hipMalloc(&d.planes, (size_t)(nn * d.W) * sizeof(uint4)));
-  SCHK(hipMalloc(&d.onehot, (size_t)(nn * d.W64 * 256)));
+  SCHK(hipMalloc(&d.ambp, (size_t)(nn * d.W) * sizeof(uint32_t)));
+  SCHK(hipMalloc(&d.onehot, (size_t)(d.npad * d.W64 * 256)));
   SCHK(hipMalloc(&d.ev_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.weights, (size_t)nn * sizeof(int64_t)));
   SCHK(hipMemcpyAsync(d.codes, padded.data(), padded.size(), hipMemcpyHostToDevice, c->stream));
@@ -912,11 +1427,10 @@ int tn93g_set_upload(tn93g_ctx *c, const uint8_t *codes, int64_t n, int64_t L, c
     int64_t tot = n * d.W;
     if (tot > 0)
       hipLaunchKernelGGL(k_planes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.planes,
-                         n, L, d.Ls, d.W);
-    int64_t toh = n * d.W64 * 64;
-    if (toh > 0)
-      hipLaunchKernelGGL(k_onehot, dim3((unsigned)((toh + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.onehot,
-                         n, L, d.Ls, d.W64);
+                         d.ambp, n, L, d.Ls, d.W);
+    int64_t toh = d.npad * d.W64 * 64;
+    hipLaunchKernelGGL(k_onehot, dim3((unsigned)((toh + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.onehot, n,
+                       L, d.Ls, d.W64, d.npad);
   }
   int32_t *cnt_d = nullptr;
   SCHK(hipMalloc(&cnt_d, (size_t)nn * sizeof(int32_t)));
@@ -948,7 +1462,7 @@ void tn93g_set_free(tn93g_set *S)
   if (!S) return;
   (void)hipSetDevice(S->ctx->device);
   (void)hipStreamSynchronize(S->ctx->stream);
-  void *ps[] = {S->d.codes, S->d.codesT, S->d.planes, S->d.onehot, S->d.ev_off, S->d.ev, S->d.weights};
+  void *ps[] = {S->d.codes, S->d.codesT, S->d.planes, S->d.ambp, S->d.onehot, S->d.ev_off, S->d.ev, S->d.weights};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   delete S;
@@ -1016,6 +1530,7 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   RunArgs A;
   memset(&A, 0, sizeof A);
   A.planes = d.planes;
+  A.ambp = d.ambp;
   A.onehot = d.onehot;
   A.codes = d.codes;
   A.codesT = d.codesT;
@@ -1085,162 +1600,291 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   A.evtab = (const uint4 *)c->evtab.p;
   A.rtab = (const ReplayEnt *)c->rtab.p;
   // buffers
+  const bool fast = !A.exact && P->kind != TN93G_FST;  // F_ST combines the means non-monotonically: exact path
+  A.fast = fast ? 1 : 0;
   const size_t nbox = (size_t)std::max<int64_t>(npairs_box, 1);
-  if (ensure(c, c->D, nbox * 8) || ensure(c, c->U, nbox * 2) || ensure(c, c->hist, 600 * 8) ||
-      ensure(c, c->row_links, (size_t)std::max<int64_t>(nrows, 1) * 4) ||
+  const size_t nr1 = (size_t)std::max<int64_t>(nrows, 1);
+  const int64_t c0a = A.c0 & ~(int64_t)63;
+  const int64_t gx = std::max<int64_t>((A.c1 - c0a + TJ - 1) / TJ, 1), gy = std::max<int64_t>((nrows + TI - 1) / TI, 1);
+  const int64_t ntiles = gx * gy, nslots = fast ? (int64_t)nr1 * gx : 1;
+  if (gx >= (1ll << 31) || gy >= 65536) return c->fail(TN93G_E_ARG, "too many pair tiles in one run");
+  if ((!fast && (ensure(c, c->D, nbox * 8) || ensure(c, c->U, nbox * 2))) ||
+      ensure(c, c->hist, (size_t)NSTRIPE * 600 * 8) || ensure(c, c->row_links, nr1 * 4) ||
       ensure(c, c->link_off, (size_t)(nrows + 1) * 8) ||
-      ensure(c, c->slow_list, (size_t)std::max<int64_t>(npairs, 1) * 8) || ensure(c, c->counters, 64 * 8) ||
-      ensure(c, c->rowout, (size_t)std::max<int64_t>(nrows, 1) * (15 * 8 + 16)))
+      ensure(c, c->slow_list, (size_t)ntiles * TI * TJ * 8) || ensure(c, c->slow_cnt, (size_t)ntiles * 4) ||
+      ensure(c, c->counters, (size_t)NSTRIPE * 32 * 4) ||
+      ensure(c, c->rowout, nr1 * std::max<size_t>(15 * 8 + 16, sizeof(RowAcc))) ||
+      ensure(c, c->chdr, (size_t)nslots * sizeof(ChunkHdr)) ||
+      ensure(c, c->cdata, (size_t)nslots * 7 * 64 * sizeof(uint32_t)))
     return TN93G_E_NOMEM;
   const unsigned exact_cap = A.exact ? (unsigned)std::min<int64_t>(npairs, 1ll << 31) : 1u;
   if (ensure(c, c->exact_list, (size_t)exact_cap * sizeof(ExactRec))) return TN93G_E_NOMEM;
+  unsigned link_cap = 1;
+  if (fast) {
+    // links per stripe region; a run that overflows a region is repeated with larger regions
+    const int64_t want = std::max<int64_t>(c->link_cap_hint, 1 << 15);
+    link_cap = (unsigned)std::min<int64_t>(std::max<int64_t>(npairs, 1), want);
+    if (ensure(c, c->links, (size_t)link_cap * NSTRIPE * sizeof(LinkRec))) return TN93G_E_NOMEM;
+  }
   A.D = (double *)c->D.p;
   A.U = (uint16_t *)c->U.p;
   A.hist = (unsigned long long *)c->hist.p;
   A.row_links = (int32_t *)c->row_links.p;
   A.slow_list = (unsigned long long *)c->slow_list.p;
-  unsigned *ctr = (unsigned *)c->counters.p;
-  A.slow_cnt = ctr + 0;
-  A.exact_cnt = ctr + 1;
-  A.stats = ctr + 2;
+  A.slow_cnt = (unsigned *)c->slow_cnt.p;
+  A.stripes = (unsigned *)c->counters.p;
+  A.exact_cnt = A.stripes + ST_EXACT;
   A.exact_list = (ExactRec *)c->exact_list.p;
   A.exact_cap = exact_cap;
-  HIPCHK(c, hipMemsetAsync(c->hist.p, 0, 600 * 8, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->counters.p, 0, 64 * 8, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->row_links.p, 0, (size_t)std::max<int64_t>(nrows, 1) * 4, c->stream));
+  A.racc = (RowAcc *)c->rowout.p;
+  A.links = (LinkRec *)c->links.p;
+  A.link_cap = link_cap;
+  A.chdr = (ChunkHdr *)c->chdr.p;
+  A.cdata = (uint32_t *)c->cdata.p;
+  A.gx = gx;
+  HIPCHK(c, hipMemsetAsync(c->hist.p, 0, (size_t)NSTRIPE * 600 * 8, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->counters.p, 0, (size_t)NSTRIPE * 32 * 4, c->stream));
+  HIPCHK(c, hipMemsetAsync(c->row_links.p, 0, nr1 * 4, c->stream));
+  if (fast) HIPCHK(c, hipMemsetAsync(c->rowout.p, 0, nr1 * sizeof(RowAcc), c->stream));
   HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
   if (nrows > 0 && A.ncols > 0) {
-    dim3 g((unsigned)((A.ncols + TJ - 1) / TJ), (unsigned)((nrows + TI - 1) / TI));
-    hipLaunchKernelGGL(k_pairs<TN93G_BACKEND_POPC>, g, dim3(256), 0, c->stream, A);
+    dim3 g((unsigned)gx, (unsigned)gy);
+    const bool mf = P->backend == TN93G_BACKEND_MFMA;
+    if (mf && fast)
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, true>), g, dim3(256), 0, c->stream, A);
+    else if (mf)
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, false>), g, dim3(256), 0, c->stream, A);
+    else if (fast)
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, true>), g, dim3(256), 0, c->stream, A);
+    else
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, false>), g, dim3(256), 0, c->stream, A);
+  } else {
+    HIPCHK(c, hipMemsetAsync(c->slow_cnt.p, 0, (size_t)ntiles * 4, c->stream));
   }
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  hipLaunchKernelGGL(k_replay, dim3(2048), dim3(RB), 0, c->stream, A);
+  if (fast && nrows > 0 && A.ncols > 0) hipLaunchKernelGGL(k_tail, dim3(4096), dim3(256), 0, c->stream, A);
+  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
+  hipLaunchKernelGGL(k_replay, dim3((unsigned)gx, (unsigned)gy), dim3(RB), 0, c->stream, A);
   HIPCHK(c, hipGetLastError());
-  HIPCHK(c, hipMemcpyAsync(c->pin_counters, c->counters.p, 64, hipMemcpyDeviceToHost, c->stream));
+  std::vector<unsigned> stripes((size_t)NSTRIPE * 32), scnt((size_t)ntiles);
+  HIPCHK(c, hipMemcpyAsync(stripes.data(), c->counters.p, stripes.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(scnt.data(), c->slow_cnt.p, scnt.size() * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  const unsigned *hc = (const unsigned *)c->pin_counters;
-  const unsigned n_slow = hc[0], n_exact = hc[1], n_unc = hc[2], n_undecided = hc[3];
+  uint64_t n_slow = 0, n_unc = 0, n_undecided = 0, n_linkrec = 0;
+  unsigned max_region = 0;
+  for (unsigned v : scnt) n_slow += v;
+  for (int st = 0; st < NSTRIPE; st++) {
+    n_unc += stripes[(size_t)st * 32 + ST_UNCERTAIN];
+    n_undecided += stripes[(size_t)st * 32 + ST_UNDECIDED];
+    n_linkrec += stripes[(size_t)st * 32 + ST_LINKS];
+    max_region = std::max(max_region, stripes[(size_t)st * 32 + ST_LINKS]);
+  }
+  const unsigned n_exact = stripes[ST_EXACT];
   if (n_undecided && !A.exact) {
     // an output decision could not be certified from the enclosures: run again resolving logs on the host
     tn93g_params Q = *P;
     Q.flags |= TN93G_FLAG_EXACT;
     return tn93g_run(c, S, &Q, out);
   }
-  if (A.exact && n_exact) {
-    if (n_exact > exact_cap) return c->fail(TN93G_E_NOMEM, "host-exact list overflow");
-    std::vector<ExactRec> ex(n_exact);
-    HIPCHK(c, hipMemcpyAsync(ex.data(), c->exact_list.p, n_exact * sizeof(ExactRec), hipMemcpyDeviceToHost,
-                             c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::vector<PatchRec> pr(n_exact);
-    auto work = [&](size_t b, size_t e) {
-      for (size_t k = b; k < e; k++) {
-        double dd = host_tail(ex[k].tin);
-        double x = dd <= 0. ? 0. : dd;
-        pr[k].i = ex[k].i;
-        pr[k].j = ex[k].j;
-        pr[k].thisD = x;
-        pr[k].idx = host_hist_index(dd);
-        pr[k].link = x >= P->min_distance && x <= P->threshold;
-      }
-    };
-    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (n_exact < 4096) nth = 1;
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nth; t++) th.emplace_back(work, (size_t)n_exact * t / nth, (size_t)n_exact * (t + 1) / nth);
-    for (auto &t : th) t.join();
-    if (ensure(c, c->patch, pr.size() * sizeof(PatchRec))) return TN93G_E_NOMEM;
-    HIPCHK(c, hipMemcpyAsync(c->patch.p, pr.data(), pr.size() * sizeof(PatchRec), hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(k_patch, dim3((unsigned)((n_exact + 255) / 256)), dim3(256), 0, c->stream, A,
-                       (const PatchRec *)c->patch.p, (int64_t)n_exact);
+  if (fast && max_region > link_cap) {  // more links than a region holds: grow the regions and run again
+    c->link_cap_hint = (int64_t)max_region;
+    return tn93g_run(c, S, P, out);
   }
-  // link offsets, then the rows
-  int64_t *loff = (int64_t *)c->link_off.p;
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)c->row_links.p, loff, nrows);
-  int64_t n_links = 0;
-  HIPCHK(c, hipMemcpyAsync(&n_links, loff + nrows, 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (ensure(c, c->links, (size_t)std::max<int64_t>(n_links, 1) * 40)) return TN93G_E_NOMEM;
-  RowOut O;
-  {
-    double *base = (double *)c->rowout.p;
-    const size_t nr = (size_t)std::max<int64_t>(nrows, 1);
-    O.sum_lo = base;
-    O.sum_hi = base + 3 * nr;
-    O.wsum = base + 6 * nr;
-    O.max_lo = base + 9 * nr;
-    O.max_hi = base + 12 * nr;
-    O.skipped = (int64_t *)(base + 15 * nr);
-    O.linkw = (int64_t *)(base + 16 * nr);
-    const size_t nl = (size_t)std::max<int64_t>(n_links, 1);
-    O.link_row = (int64_t *)c->links.p;
-    O.link_col = O.link_row + nl;
-    O.link_d = (double *)(O.link_col + nl);
-    O.link_lo = O.link_d + nl;
-    O.link_hi = O.link_lo + nl;
-    O.link_off = loff;
-  }
-  if (nrows > 0) hipLaunchKernelGGL(k_rows, dim3((unsigned)((nrows + 63) / 64)), dim3(64), 0, c->stream, A, O);
-  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-  HIPCHK(c, hipGetLastError());
-  // results
   tn93g_result *R = (tn93g_result *)calloc(1, sizeof(tn93g_result));
   if (!R) return TN93G_E_NOMEM;
-  const size_t nl = (size_t)n_links;
-  R->n_links = n_links;
-  R->link_row = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
-  R->link_col = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
-  R->link_dist = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-  R->link_dist_lo = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-  R->link_dist_hi = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-  std::vector<double> rows((size_t)std::max<int64_t>(nrows, 1) * 17);
-  if (nl) {
-    HIPCHK(c, hipMemcpyAsync(R->link_row, O.link_row, nl * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(R->link_col, O.link_col, nl * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(R->link_dist, O.link_d, nl * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(R->link_dist_lo, O.link_lo, nl * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(R->link_dist_hi, O.link_hi, nl * 8, hipMemcpyDeviceToHost, c->stream));
+  std::vector<unsigned long long> hrep((size_t)NSTRIPE * 600);
+  if (fast) {
+    // ---- fast path: row accumulators, unordered links
+    const size_t nl = n_linkrec;
+    std::vector<LinkRec> lk(nl);
+    std::vector<RowAcc> ra(nr1);
+    size_t at = 0;
+    for (int st = 0; st < NSTRIPE; st++) {
+      const unsigned cnt = stripes[(size_t)st * 32 + ST_LINKS];
+      if (cnt)
+        HIPCHK(c, hipMemcpyAsync(lk.data() + at, (LinkRec *)c->links.p + (size_t)st * link_cap, cnt * sizeof(LinkRec),
+                                 hipMemcpyDeviceToHost, c->stream));
+      at += cnt;
+    }
+    HIPCHK(c, hipMemcpyAsync(ra.data(), c->rowout.p, nr1 * sizeof(RowAcc), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hrep.data(), c->hist.p, hrep.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::sort(lk.begin(), lk.end(), [](const LinkRec &x, const LinkRec &y) {
+      return x.i != y.i ? x.i < y.i : x.j < y.j;
+    });
+    R->n_links = (int64_t)nl;
+    R->link_row = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
+    R->link_col = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
+    R->link_dist = (double *)malloc(std::max<size_t>(nl, 1) * 8);
+    R->link_dist_lo = (double *)malloc(std::max<size_t>(nl, 1) * 8);
+    R->link_dist_hi = (double *)malloc(std::max<size_t>(nl, 1) * 8);
+    for (size_t k = 0; k < nl; k++) {
+      R->link_row[k] = lk[k].i;
+      R->link_col[k] = lk[k].j;
+      R->link_dist[k] = lk[k].d;
+      R->link_dist_lo[k] = lk[k].lo;
+      R->link_dist_hi[k] = lk[k].hi;
+    }
+    // Enclosure of tn93's mean (swifttn93.cpp:2594-2602, 2839-2848).  Row r: tn93 adds its n_r terms t in column
+    // order in FP64; here they were added in another order.  Both results are within gamma_{n_r} * sum|t| of the
+    // exact sum (Higham, Accuracy and Stability of Numerical Algorithms, 2nd ed., eq. 4.4), so tn93's row sum lies in
+    // [slo - 2 g sabs', shi + 2 g sabs'] with sabs' an upper bound of sum|t|.  The rows are then added in row order:
+    // again within gamma_R * sum |row sums|.  Host arithmetic is long double with an extra relative margin.
+    const long double u = 0x1p-53L;
+    auto gam = [&](long double n) { return n * u / (1.0L - n * u); };
+    long double mlo = 0, mhi = 0, mabs = 0;
+    double maxlo = 0., maxhi = 0.;
+    long long wsum = 0, skip = 0, linkw = 0;
+    for (int64_t r = 0; r < nrows; r++) {
+      const RowAcc &q = ra[(size_t)r];
+      const int64_t i = A.r0 + r;
+      const long double n_r = (long double)(A.triangle ? (A.c1 - i - 1) : A.ncols);
+      const long double g = gam(std::max<long double>(n_r, 1));
+      const long double sabs = (long double)q.sabs * (1.0L + g);
+      const long double e = 2.0L * g * sabs;
+      const long double lo = (long double)q.slo - e, hi = (long double)q.shi + e;
+      mlo += lo;
+      mhi += hi;
+      mabs += std::max(fabsl(lo), fabsl(hi));
+      const double a = tn93g_host_from_bits(q.mlo), b = tn93g_host_from_bits(q.mhi);
+      if (a > maxlo) maxlo = a;
+      if (b > maxhi) maxhi = b;
+      wsum += q.wsum;
+      skip += q.skip;
+      linkw += q.linkw;
+    }
+    const long double eR = gam((long double)std::max<int64_t>(nrows, 1)) * mabs + 0x1p-60L * mabs;
+    R->mean_lo[0] = std::nextafter((double)(mlo - eR), -INFINITY);
+    R->mean_hi[0] = std::nextafter((double)(mhi + eR), INFINITY);
+    R->max_lo[0] = maxlo;
+    R->max_hi[0] = maxhi;
+    R->skipped = skip;
+    R->found_links = linkw;
+    long long w_all = wsum;
+    if (A.triangle) {
+      for (int64_t i = 0; i < d.n; i++) {
+        const int64_t wi = S->hw[(size_t)i];
+        if (wi > 1) w_all += (wi * (wi - 1)) >> 1;
+      }
+    }
+    R->weighted[0] = (double)w_all;
+  } else {
+    if (A.exact && n_exact) {
+      if (n_exact > exact_cap) return c->fail(TN93G_E_NOMEM, "host-exact list overflow");
+      std::vector<ExactRec> ex(n_exact);
+      HIPCHK(c, hipMemcpyAsync(ex.data(), c->exact_list.p, n_exact * sizeof(ExactRec), hipMemcpyDeviceToHost,
+                               c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      std::vector<PatchRec> pr(n_exact);
+      auto work = [&](size_t b0, size_t e0) {
+        for (size_t k = b0; k < e0; k++) {
+          double dd = host_tail(ex[k].tin);
+          double x = dd <= 0. ? 0. : dd;
+          pr[k].i = ex[k].i;
+          pr[k].j = ex[k].j;
+          pr[k].thisD = x;
+          pr[k].idx = host_hist_index(dd);
+          pr[k].link = x >= P->min_distance && x <= P->threshold;
+        }
+      };
+      unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+      if (n_exact < 4096) nth = 1;
+      std::vector<std::thread> th;
+      for (unsigned t = 0; t < nth; t++)
+        th.emplace_back(work, (size_t)n_exact * t / nth, (size_t)n_exact * (t + 1) / nth);
+      for (auto &t : th) t.join();
+      if (ensure(c, c->patch, pr.size() * sizeof(PatchRec))) return TN93G_E_NOMEM;
+      HIPCHK(c,
+             hipMemcpyAsync(c->patch.p, pr.data(), pr.size() * sizeof(PatchRec), hipMemcpyHostToDevice, c->stream));
+      hipLaunchKernelGGL(k_patch, dim3((unsigned)((n_exact + 255) / 256)), dim3(256), 0, c->stream, A,
+                         (const PatchRec *)c->patch.p, (int64_t)n_exact);
+    }
+    // link offsets, then the rows in column order
+    int64_t *loff = (int64_t *)c->link_off.p;
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)c->row_links.p, loff, nrows);
+    int64_t n_links = 0;
+    HIPCHK(c, hipMemcpyAsync(&n_links, loff + nrows, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (ensure(c, c->links, (size_t)std::max<int64_t>(n_links, 1) * 40)) return TN93G_E_NOMEM;
+    RowOut O;
+    {
+      double *base = (double *)c->rowout.p;
+      O.sum_lo = base;
+      O.sum_hi = base + 3 * nr1;
+      O.wsum = base + 6 * nr1;
+      O.max_lo = base + 9 * nr1;
+      O.max_hi = base + 12 * nr1;
+      O.skipped = (int64_t *)(base + 15 * nr1);
+      O.linkw = (int64_t *)(base + 16 * nr1);
+      const size_t nl = (size_t)std::max<int64_t>(n_links, 1);
+      O.link_row = (int64_t *)c->links.p;
+      O.link_col = O.link_row + nl;
+      O.link_d = (double *)(O.link_col + nl);
+      O.link_lo = O.link_d + nl;
+      O.link_hi = O.link_lo + nl;
+      O.link_off = loff;
+    }
+    if (nrows > 0) hipLaunchKernelGGL(k_rows, dim3((unsigned)((nrows + RT - 1) / RT)), dim3(RT), 0, c->stream, A, O);
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(c, hipGetLastError());
+    const size_t nl = (size_t)n_links;
+    R->n_links = n_links;
+    R->link_row = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
+    R->link_col = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
+    R->link_dist = (double *)malloc(std::max<size_t>(nl, 1) * 8);
+    R->link_dist_lo = (double *)malloc(std::max<size_t>(nl, 1) * 8);
+    R->link_dist_hi = (double *)malloc(std::max<size_t>(nl, 1) * 8);
+    std::vector<double> rows(nr1 * 17);
+    if (nl) {
+      HIPCHK(c, hipMemcpyAsync(R->link_row, O.link_row, nl * 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(R->link_col, O.link_col, nl * 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(R->link_dist, O.link_d, nl * 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(R->link_dist_lo, O.link_lo, nl * 8, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipMemcpyAsync(R->link_dist_hi, O.link_hi, nl * 8, hipMemcpyDeviceToHost, c->stream));
+    }
+    HIPCHK(c, hipMemcpyAsync(rows.data(), c->rowout.p, rows.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(hrep.data(), c->hist.p, hrep.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    // row-order merge (swifttn93.cpp:2832-2848)
+    for (int b = 0; b < 3; b++) R->mean_lo[b] = R->mean_hi[b] = R->weighted[b] = R->max_lo[b] = R->max_hi[b] = 0.;
+    for (int64_t r = 0; r < nrows; r++) {
+      for (int b = 0; b < 3; b++) {
+        const double mlo = rows[9 * nr1 + b * nr1 + (size_t)r], mhi = rows[12 * nr1 + b * nr1 + (size_t)r];
+        if (mlo > R->max_lo[b]) R->max_lo[b] = mlo;
+        if (mhi > R->max_hi[b]) R->max_hi[b] = mhi;
+        R->mean_lo[b] += rows[b * nr1 + (size_t)r];
+        R->mean_hi[b] += rows[3 * nr1 + b * nr1 + (size_t)r];
+        R->weighted[b] += rows[6 * nr1 + b * nr1 + (size_t)r];
+      }
+      int64_t sk, lw;
+      memcpy(&sk, &rows[15 * nr1 + (size_t)r], 8);
+      memcpy(&lw, &rows[16 * nr1 + (size_t)r], 8);
+      R->skipped += sk;
+      R->found_links += lw;
+    }
   }
-  HIPCHK(c, hipMemcpyAsync(rows.data(), c->rowout.p, rows.size() * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(R->hist, c->hist.p, 600 * 8, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int st = 0; st < NSTRIPE; st++)  // the histogram replicas
+    for (int k = 0; k < 600; k++) (&R->hist[0][0])[k] += hrep[(size_t)st * 600 + k];
   float ms_all = 0, ms_pair = 0;
   (void)hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]);
   (void)hipEventElapsedTime(&ms_pair, c->ev[0], c->ev[1]);
   R->gpu_ms = ms_all;
   R->pair_kernel_ms = ms_pair;
-  // row-order merge (swifttn93.cpp:2832-2848)
-  const size_t nr = (size_t)std::max<int64_t>(nrows, 1);
-  for (int b = 0; b < 3; b++) {
-    R->mean_lo[b] = R->mean_hi[b] = R->weighted[b] = R->max_lo[b] = R->max_hi[b] = 0.;
-  }
-  for (int64_t r = 0; r < nrows; r++) {
-    for (int b = 0; b < 3; b++) {
-      const double mlo = rows[9 * nr + b * nr + (size_t)r], mhi = rows[12 * nr + b * nr + (size_t)r];
-      if (mlo > R->max_lo[b]) R->max_lo[b] = mlo;
-      if (mhi > R->max_hi[b]) R->max_hi[b] = mhi;
-      R->mean_lo[b] += rows[b * nr + (size_t)r];
-      R->mean_hi[b] += rows[3 * nr + b * nr + (size_t)r];
-      R->weighted[b] += rows[6 * nr + b * nr + (size_t)r];
-    }
-    int64_t sk, lw;
-    memcpy(&sk, &rows[15 * nr + (size_t)r], 8);
-    memcpy(&lw, &rows[16 * nr + (size_t)r], 8);
-    R->skipped += sk;
-    R->found_links += lw;
-  }
-  R->exact = (n_unc == 0) ? 1 : 0;
+  float ms_tail = 0;
+  (void)hipEventElapsedTime(&ms_tail, c->ev[1], c->ev[3]);
+  R->tail_kernel_ms = ms_tail;
+  R->exact = (!fast && n_unc == 0) ? 1 : 0;
   R->nrows = (int32_t)nrows;
   R->n_replayed = n_slow;
   R->n_uncertain = n_unc;
   R->n_host_exact = n_exact;
-  // self_v_self histogram entries (swifttn93.cpp:2406-2415): the row kernel adds their weights
-  if (A.triangle) {
-    std::vector<int64_t> w((size_t)std::max<int64_t>(d.n, 1));
-    HIPCHK(c, hipMemcpy(w.data(), d.weights, (size_t)d.n * 8, hipMemcpyDeviceToHost));
-    for (int64_t i = 0; i < d.n; i++)
-      if (w[(size_t)i] > 1) R->hist[(A.fst && i >= A.n1) ? 1 : 0][0] += (unsigned long long)((w[(size_t)i] * (w[(size_t)i] - 1)) >> 1);
-  }
+  // self_v_self histogram entries (swifttn93.cpp:2406-2415)
+  if (A.triangle)
+    for (int64_t i = 0; i < d.n; i++) {
+      const int64_t wi = S->hw[(size_t)i];
+      if (wi > 1) R->hist[(A.fst && i >= A.n1) ? 1 : 0][0] += (unsigned long long)((wi * (wi - 1)) >> 1);
+    }
   *out = R;
   return 0;
 }

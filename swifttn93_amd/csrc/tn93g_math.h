@@ -91,13 +91,13 @@ TN93G_FN tn93g_logres tn93g_log(double x, double window)
   double s0 = r0 * r0;
   double s1 = TN93G_FMA(r0, r0, -s0) + 2.0 * r0 * r1;  // r^2 = s0 + s1
   double p = -1.0 / 10.0;
-  p = p * r0 + 1.0 / 9.0;
-  p = p * r0 - 1.0 / 8.0;
-  p = p * r0 + 1.0 / 7.0;
-  p = p * r0 - 1.0 / 6.0;
-  p = p * r0 + 1.0 / 5.0;
-  p = p * r0 - 1.0 / 4.0;
-  p = p * r0 + 1.0 / 3.0;
+  p = TN93G_FMA(p, r0, 1.0 / 9.0);
+  p = TN93G_FMA(p, r0, -1.0 / 8.0);
+  p = TN93G_FMA(p, r0, 1.0 / 7.0);
+  p = TN93G_FMA(p, r0, -1.0 / 6.0);
+  p = TN93G_FMA(p, r0, 1.0 / 5.0);
+  p = TN93G_FMA(p, r0, -1.0 / 4.0);
+  p = TN93G_FMA(p, r0, 1.0 / 3.0);
   double t3 = s0 * r0 * p;
   // A = e*ln2 + log F (double-double, error-free product for e*ln2)
   double ed = (double)e;
@@ -127,8 +127,8 @@ TN93G_FN tn93g_logres tn93g_log(double x, double window)
     double gap = n - v;  // exact (adjacent doubles)
     if (gap < 0) gap = -gap;
     double al = lo < 0 ? -lo : lo;
-    double dist = (0.5 * gap - al) / gap;  // 0 at the midpoint, 0.5 at v
-    if (dist < window) {
+    // within `window` ulps of the midpoint (gap/2 from v): |lo| > (0.5 - window) gap
+    if (al > (0.5 - window) * gap) {
       R.alt = n;
       R.unc = 1;
     }
@@ -158,8 +158,9 @@ TN93G_FN tn93g_tailout tn93g_tail(const tn93g_tailin &in, double window)
   const double *nucFreq = in.nf;
   double auxd, fY, fR, K1 = 0, K2 = 0, K3 = 0, AG, CT, ti, tv, tng2, nucF[4];
   bool useK2P = false;
-  tng2 = 2. / (nucFreq[0] + nucFreq[1] + nucFreq[2] + nucFreq[3]);
+  // tn93 computes 2./S and 1./S; scaling by 2 commutes with rounding, so 2./S == 2. * (1./S) exactly
   auxd = 1. / (nucFreq[0] + nucFreq[1] + nucFreq[2] + nucFreq[3]);
+  tng2 = 2. * auxd;
   for (int k = 0; k < 4; k++) nucF[k] = nucFreq[k] * auxd;
   fR = nucF[0] + nucF[2];
   fY = nucF[1] + nucF[3];

@@ -7,9 +7,11 @@ every machine and numpy version.  Workloads:
        0.5 % IUPAC codes (R, Y, K, M, S, W, N uniform); 2 % gaps in runs of 1-30.
   c1   6,700 x 1,300 HIV-pol-shaped: 200 transmission clusters (sizes geometric, mean 15) at 0.1-1.5 %
        intra-cluster divergence, 5-12 % between clusters (background sequences at the same spread); 1 % ambiguity
-       codes (R, Y, K, M, S, W, N, with B/D/H/V at 2 % of them so the site-order replay is exercised); 5 % of
-       sequences carry a leading or trailing gap run of 50-400 columns.
-  c1s  a smaller sample of the c1 generator (n sequences, default 600) for parity tests.
+       codes in the proportions of the reference's real HIV-1 pol set (application/results/ambiguity_counts.json:
+       R 45 %, Y 23 %, N 18 %, M 6 %, W 4 %, K 3 %, S 1 %, and B/D/H/V 0.5 %, the codes whose 1/3 weights make a
+       pair order-dependent); 5 % of sequences carry a leading or trailing gap run of 50-400 columns.
+  c1s  a smaller sample of the c1 generator (n sequences) for parity tests; `bdhv` raises the share of B/D/H/V
+       among the ambiguity codes so the site-order replay is exercised heavily.
 """
 import numpy as np
 
@@ -52,10 +54,16 @@ def _mutate(rs, seq, rate):
     return out
 
 
-def _ambiguities(rs, codes, rate, pool):
+def _ambiguities(rs, codes, rate, pool, weights=None):
     n, L = codes.shape
     hit = rs.uniform(n * L).reshape(n, L) < rate
-    pick = np.array(pool, dtype=np.uint8)[rs.integers(len(pool), n * L).reshape(n, L)]
+    if weights is None:
+        pick = np.array(pool, dtype=np.uint8)[rs.integers(len(pool), n * L).reshape(n, L)]
+    else:
+        cdf = np.cumsum(np.asarray(weights, dtype=np.float64))
+        cdf /= cdf[-1]
+        idx = np.minimum(np.searchsorted(cdf, rs.uniform(n * L), side="right"), len(pool) - 1)
+        pick = np.array(pool, dtype=np.uint8)[idx].reshape(n, L)
     codes[hit] = pick[hit]
 
 
@@ -94,7 +102,12 @@ def c0(seed=1):
     return codes
 
 
-def c1(seed=2, n=6700, L=1300, clusters=200):
+# R, Y, N, M, W, K, S, then B, D, H, V
+_C1_POOL = [5, 6, 15, 10, 8, 9, 7, 11, 12, 13, 14]
+_C1_WEIGHTS = [45.0, 23.0, 18.0, 6.0, 4.0, 3.0, 1.0]
+
+
+def c1(seed=2, n=6700, L=1300, clusters=200, bdhv=0.005):
     rs = Stream(seed)
     root = rs.integers(4, L).astype(np.uint8)
     codes = np.empty((n, L), dtype=np.uint8)
@@ -114,14 +127,15 @@ def c1(seed=2, n=6700, L=1300, clusters=200):
     # shuffle rows so clusters are spread over the matrix
     perm = np.argsort(rs.u64(n), kind="stable")
     codes = codes[perm]
-    pool = [5] * 24 + [6] * 12 + [9] * 2 + [10] * 3 + [7] * 1 + [8] * 2 + [15] * 10 + [11, 12, 13, 14]
-    _ambiguities(rs, codes, 0.01, pool)
+    w = list(_C1_WEIGHTS)
+    w += [sum(w) * bdhv / (1.0 - bdhv) / 4.0] * 4
+    _ambiguities(rs, codes, 0.01, _C1_POOL, w)
     _gap_runs(rs, codes, 0.05, 50, 400)
     return codes
 
 
-def c1s(n=600, seed=3):
-    return c1(seed=seed, n=n, L=1300, clusters=max(1, n * 200 // 6700))
+def c1s(n=600, seed=3, bdhv=0.07):
+    return c1(seed=seed, n=n, L=1300, clusters=max(1, n * 200 // 6700), bdhv=bdhv)
 
 
 def names_for(n, prefix="s"):

@@ -41,7 +41,7 @@ def run_batch(cases, tmp_path, timeout=600):
     return res
 
 
-def test_fuzz_fixtures(fuzz_cases, tmp_path):
+def test_a_fuzz_fixtures(fuzz_cases, tmp_path):
     cases = []
     for k, c in enumerate(fuzz_cases):
         d = tmp_path / f"c{k}"
@@ -99,7 +99,7 @@ def test_synthetic_vs_oracle(workload, tmp_path):
                 want.append(((d / "want.txt").read_bytes(), p.stdout))
                 cases.append((d, argv))
     res = run_batch(cases, tmp_path)
-    for (d, argv), (rc, out, so), (wo, ws) in zip(cases, res):
+    for (d, argv), (rc, out, so), (wo, ws) in zip(cases, res, want):
         assert rc == 0, argv
         assert out == wo, (argv, len(out or b""), len(wo))
         assert so == ws, (argv, so[:400], ws[:400])
@@ -173,6 +173,10 @@ def test_fast_path_equals_replay(engine, mode):
     assert slow["n_replayed"] == 400 * 399 // 2
     for r in (fast, slow):
         assert (r["link_lo"] <= exact["link_dist"]).all() and (exact["link_dist"] <= r["link_hi"]).all()
+        # the enclosures of the row-order mean and of the maximum contain the exact values
+        assert r["mean_lo"][0] <= exact["mean_lo"][0] == exact["mean_hi"][0] <= r["mean_hi"][0]
+        assert r["max_lo"][0] <= exact["max_lo"][0] <= r["max_hi"][0]
+        assert (r["mean_hi"][0] - r["mean_lo"][0]) <= 1e-9 * abs(exact["mean_lo"][0])
     s.free()
 
 
@@ -189,3 +193,18 @@ def test_full_size_c1_vs_oracle(tmp_path):
     assert rc == 0
     assert out == (tmp_path / "want.txt").read_bytes()
     assert so == p.stdout
+
+
+@pytest.mark.parametrize("kind", ["triangle", "cross"])
+def test_backends_agree(engine, kind):
+    """The int8 MFMA one-hot contraction and the VALU popcount give the same counts, hence the same outputs."""
+    codes = gen.c1s(333, seed=9)
+    s = engine.upload(codes)
+    kw = dict(mode=T.RESOLVE, threshold=1.0, min_overlap=500, resolve_fraction=0.05)
+    if kind == "cross":
+        kw.update(kind=T.CROSS, n1=101)
+    a = engine.run(s, backend=T.BACKEND_POPC, **kw)
+    b = engine.run(s, backend=T.BACKEND_MFMA, **kw)
+    _same(a, b)
+    assert (a["link_dist"] == b["link_dist"]).all()
+    s.free()
