@@ -61,6 +61,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--backend", default="auto", choices=["auto", "popc", "mfma"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (torch), the reference for the HBM byte counters")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported (the multi-GPU sharding is "
@@ -70,6 +72,15 @@ def main():
     import swifttn93_amd as T
     from swifttn93_amd import gen
 
+    if a.copy_calibration:
+        import torch
+        x = torch.empty(a.copy_calibration << 18, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(f"copy calibration: {a.copy_calibration} MiB read + {a.copy_calibration} MiB written per copy",
+              file=sys.stderr)
     codes = gen.c1()
     n, L = codes.shape
     n_pairs = n * (n - 1) // 2

@@ -134,7 +134,7 @@ def _to_py(r):
     n = r.n_links
     return {
         "exact": bool(r.exact),
-        "hist": np.array([[r.hist[b][k] for k in range(200)] for b in range(3)], dtype=np.uint64),
+        "hist": np.ctypeslib.as_array(r.hist).astype(np.uint64).reshape(3, 200).copy(),
         "link_row": _arr(r.link_row, n, np.int64), "link_col": _arr(r.link_col, n, np.int64),
         "link_dist": _arr(r.link_dist, n, np.float64), "link_lo": _arr(r.link_dist_lo, n, np.float64),
         "link_hi": _arr(r.link_dist_hi, n, np.float64),

@@ -236,6 +236,8 @@ struct RunArgs {
   int32_t *row_links;          // [nrows]
   unsigned long long *slow_list;  // [tile][TI*TJ] pairs for the site-order replay, per pair tile
   unsigned int *slow_cnt;         // [tile]
+  const int64_t *slow_off;        // [tile + 1]: exclusive scan of slow_cnt (k_replay finds a pair's tile in it)
+  int64_t ntiles;
   ExactRec *exact_list;
   unsigned int *exact_cnt;
   unsigned int exact_cap;
@@ -1072,7 +1074,7 @@ __device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const Repl
   tin.ct = fc[7][t] + fc[13][t];
 }
 
-// one block per pair tile: the tile's pairs that need the site-order replay
+// one thread per pair that needs the site-order replay (the tiles' lists, concatenated through slow_off)
 __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
 {
   __shared__ double fc[16][RB];
@@ -1084,10 +1086,17 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
   for (int k = t; k < 600; k += RB) lhist[k] = 0;
   if (t == 0) bs.unc = bs.undecided = 0;
   __syncthreads();
-  const int64_t tile = (int64_t)blockIdx.y * A.gx + blockIdx.x;
-  const unsigned cnt = A.slow_cnt[tile];
-  for (unsigned k = t; k < cnt; k += RB) {
-    const unsigned long long pr = A.slow_list[tile * (TI * TJ) + k];
+  const int64_t total = A.slow_off[A.ntiles];
+  for (int64_t k = (int64_t)blockIdx.x * RB + t; k < total; k += (int64_t)gridDim.x * RB) {
+    int64_t lo = 0, hi = A.ntiles;  // largest tile with slow_off[tile] <= k
+    while (hi - lo > 1) {
+      const int64_t mid = (lo + hi) >> 1;
+      if (A.slow_off[mid] <= k)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const unsigned long long pr = A.slow_list[lo * (TI * TJ) + (k - A.slow_off[lo])];
     const int64_t i = (int64_t)(pr >> 32), j = (int64_t)(pr & 0xffffffffull);
     tn93g_tailin tin;
     double tng;
@@ -1114,6 +1123,26 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
     if (A.fast) rowacc_add(A, i, rp);
   }
   flush_block(A, &bs, lhist, t, RB);
+}
+
+// the links of the NSTRIPE regions, concatenated into `out` (one block per region)
+__global__ void k_link_gather(RunArgs A, LinkRec *__restrict__ out)
+{
+  const int st = blockIdx.x;
+  int64_t off = 0;
+  for (int t = 0; t < st; t++) off += min(A.stripes[t * 32 + ST_LINKS], A.link_cap);
+  const unsigned cnt = min(A.stripes[st * 32 + ST_LINKS], A.link_cap);
+  for (unsigned k = threadIdx.x; k < cnt; k += blockDim.x) out[off + k] = A.links[(size_t)st * A.link_cap + k];
+}
+
+// histogram replicas summed into replica 0
+__global__ void k_hist_sum(RunArgs A)
+{
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 600) return;
+  unsigned long long v = 0;
+  for (int st = 0; st < NSTRIPE; st++) v += A.hist[st * 600 + k];
+  A.hist[k] = v;
 }
 
 // host-exact distances back into D (after TN93G_FLAG_EXACT)
@@ -1305,8 +1334,8 @@ struct tn93g_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4];
   std::string err;
-  DevBuf D, U, hist, row_links, link_off, slow_list, slow_cnt, counters, exact_list, rowout, links, evtab, rtab, patch,
-      chdr, cdata;
+  DevBuf D, U, hist, row_links, link_off, slow_list, slow_cnt, slow_off, counters, exact_list, rowout, links,
+      links_flat, evtab, rtab, patch, chdr, cdata;
   unsigned long long *pin_counters = nullptr;  // host pinned copy of the counters
   int64_t link_cap_hint = 0;                   // link buffer size that sufficed last time
 
@@ -1368,7 +1397,7 @@ void tn93g_close(tn93g_ctx *c)
   (void)hipStreamSynchronize(c->stream);
   for (DevBuf *b : {&c->D, &c->U, &c->hist, &c->row_links, &c->link_off, &c->slow_list, &c->counters,
                     &c->exact_list, &c->rowout, &c->links, &c->evtab, &c->rtab, &c->patch, &c->chdr, &c->cdata,
-                    &c->slow_cnt})
+                    &c->slow_cnt, &c->slow_off, &c->links_flat})
     if (b->p) (void)hipFree(b->p);
   for (auto &e : c->ev) (void)hipEventDestroy(e);
   if (c->pin_counters) (void)hipHostFree(c->pin_counters);
@@ -1612,6 +1641,7 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
       ensure(c, c->hist, (size_t)NSTRIPE * 600 * 8) || ensure(c, c->row_links, nr1 * 4) ||
       ensure(c, c->link_off, (size_t)(nrows + 1) * 8) ||
       ensure(c, c->slow_list, (size_t)ntiles * TI * TJ * 8) || ensure(c, c->slow_cnt, (size_t)ntiles * 4) ||
+      ensure(c, c->slow_off, (size_t)(ntiles + 1) * 8) ||
       ensure(c, c->counters, (size_t)NSTRIPE * 32 * 4) ||
       ensure(c, c->rowout, nr1 * std::max<size_t>(15 * 8 + 16, sizeof(RowAcc))) ||
       ensure(c, c->chdr, (size_t)nslots * sizeof(ChunkHdr)) ||
@@ -1624,7 +1654,9 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
     // links per stripe region; a run that overflows a region is repeated with larger regions
     const int64_t want = std::max<int64_t>(c->link_cap_hint, 1 << 15);
     link_cap = (unsigned)std::min<int64_t>(std::max<int64_t>(npairs, 1), want);
-    if (ensure(c, c->links, (size_t)link_cap * NSTRIPE * sizeof(LinkRec))) return TN93G_E_NOMEM;
+    if (ensure(c, c->links, (size_t)link_cap * NSTRIPE * sizeof(LinkRec)) ||
+        ensure(c, c->links_flat, (size_t)link_cap * NSTRIPE * sizeof(LinkRec)))
+      return TN93G_E_NOMEM;
   }
   A.D = (double *)c->D.p;
   A.U = (uint16_t *)c->U.p;
@@ -1664,7 +1696,12 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
   if (fast && nrows > 0 && A.ncols > 0) hipLaunchKernelGGL(k_tail, dim3(4096), dim3(256), 0, c->stream, A);
   HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-  hipLaunchKernelGGL(k_replay, dim3((unsigned)gx, (unsigned)gy), dim3(RB), 0, c->stream, A);
+  int64_t *soff = (int64_t *)c->slow_off.p;
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)c->slow_cnt.p, soff, ntiles);
+  A.slow_off = soff;
+  A.ntiles = ntiles;
+  hipLaunchKernelGGL(k_replay, dim3(4096), dim3(RB), 0, c->stream, A);
+  if (fast) hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, c->stream, A, (LinkRec *)c->links_flat.p);
   HIPCHK(c, hipGetLastError());
   std::vector<unsigned> stripes((size_t)NSTRIPE * 32), scnt((size_t)ntiles);
   HIPCHK(c, hipMemcpyAsync(stripes.data(), c->counters.p, stripes.size() * 4, hipMemcpyDeviceToHost, c->stream));
@@ -1692,20 +1729,14 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   }
   tn93g_result *R = (tn93g_result *)calloc(1, sizeof(tn93g_result));
   if (!R) return TN93G_E_NOMEM;
-  std::vector<unsigned long long> hrep((size_t)NSTRIPE * 600);
+  std::vector<unsigned long long> hrep(600);
+  hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, c->stream, A);
   if (fast) {
     // ---- fast path: row accumulators, unordered links
     const size_t nl = n_linkrec;
     std::vector<LinkRec> lk(nl);
     std::vector<RowAcc> ra(nr1);
-    size_t at = 0;
-    for (int st = 0; st < NSTRIPE; st++) {
-      const unsigned cnt = stripes[(size_t)st * 32 + ST_LINKS];
-      if (cnt)
-        HIPCHK(c, hipMemcpyAsync(lk.data() + at, (LinkRec *)c->links.p + (size_t)st * link_cap, cnt * sizeof(LinkRec),
-                                 hipMemcpyDeviceToHost, c->stream));
-      at += cnt;
-    }
+    if (nl) HIPCHK(c, hipMemcpyAsync(lk.data(), c->links_flat.p, nl * sizeof(LinkRec), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(ra.data(), c->rowout.p, nr1 * sizeof(RowAcc), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipMemcpyAsync(hrep.data(), c->hist.p, hrep.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
@@ -1864,8 +1895,7 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
       R->found_links += lw;
     }
   }
-  for (int st = 0; st < NSTRIPE; st++)  // the histogram replicas
-    for (int k = 0; k < 600; k++) (&R->hist[0][0])[k] += hrep[(size_t)st * 600 + k];
+  for (int k = 0; k < 600; k++) (&R->hist[0][0])[k] += hrep[(size_t)k];  // replicas summed by k_hist_sum
   float ms_all = 0, ms_pair = 0;
   (void)hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]);
   (void)hipEventElapsedTime(&ms_pair, c->ev[0], c->ev[1]);
