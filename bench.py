@@ -27,6 +27,20 @@ sys.path.insert(0, str(REPO))
 MODE_FLAGS = ["-t", "0.015", "-a", "resolve", "-g", "0.05", "-l", "500", "-f", "csv"]
 VALU_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12  # 39.3: 256 CUs x 64 lanes x 2.4 GHz
 MFMA_INT8_TOPS = 5000.0
+HBM_BYTES_S = 8.0e12
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary of this workload
+    (profiles/c1_popc_hbm_bytes.md: read MB and written MB columns, already scaled by the copy calibration)."""
+    f = REPO / "profiles" / "c1_popc_hbm_bytes.md"
+    if not f.exists():
+        return None
+    for row in f.read_text().splitlines():
+        cells = [c.strip() for c in row.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0] == kernel:
+            return (float(cells[3]) + float(cells[4])) * 1e6
+    return None
 
 
 def cpu_baseline(fa, n_pairs):
@@ -103,10 +117,14 @@ def main():
     words = (L + 31) // 32
     ops_per_pair = 22 * words  # popcount backend: 12 logic + 10 popcount-accumulate per 32-bit word
     achieved = ops_per_pair * n_pairs / (pair_ms * 1e-3) / 1e12
+    kname = "k_pairs<1, true>" if backend in (T.BACKEND_AUTO, T.BACKEND_POPC) else "k_pairs<2, true>"
+    traffic = profiled_traffic(kname)
     roofline = {"bound": "valu", "achieved": achieved, "peak": VALU_INT32_TOPS, "unit": "TOPS",
-                "frac": achieved / VALU_INT32_TOPS, "traffic": None,
-                "kernel": "k_pairs (popcount counts + events + tail)", "kernel_ms": pair_ms,
-                "ops_per_pair": ops_per_pair}
+                "frac": achieved / VALU_INT32_TOPS, "traffic": traffic,
+                "kernel": f"{kname} (popcount counts + events + routing)", "kernel_ms": pair_ms,
+                "ops_per_pair": ops_per_pair,
+                "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/c1_popc_hbm_bytes.md",
+                "hbm_frac": None if traffic is None else traffic / (pair_ms * 1e-3) / HBM_BYTES_S}
     line = {
         "metric": "byte-exact pairs/sec at 1 GPU vs host cores; roofline %; serving p50/p99",
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
