@@ -563,11 +563,13 @@ __device__ inline uint32_t quad_sum(uint32_t x)
 // LDS of the pair kernel.  Per chunk of KW words (popcount) or one 64-site block (MFMA): the bit planes of the tile's
 // rows and columns (both backends: the ambiguity events of the chunk look up their partner's code here), the MFMA
 // operands, and at the end (MFMA) the per-pair counts transposed into the lane = column layout of the epilogue.
+// The plane arrays carry one element of padding per word so that lanes reading different words of one sequence
+// (the staging stores, the column events) fall into different LDS banks.
 template <int KWORDS, bool MFMA>
 struct PairLds {
-  uint4 sI[KWORDS][TI];
-  uint4 sJ[KWORDS][TJ];
-  uint32_t sIa[KWORDS][TI];
+  uint4 sI[KWORDS][TI + 1];
+  uint4 sJ[KWORDS][TJ + 1];
+  uint32_t sIa[KWORDS][TI + 1];
   union {
     uint4 ops[(TI + TJ) / 4][64];  // one 64-site block: TI/4 row groups + 16 column groups, 1 KB each
     uint32_t agg[7][TI][TJ];       // T, DG, TS, CT, B0, B1, BT per pair
@@ -575,9 +577,9 @@ struct PairLds {
 };
 template <int KWORDS>
 struct PairLds<KWORDS, false> {
-  uint4 sI[KWORDS][TI];
-  uint4 sJ[KWORDS][TJ];
-  uint32_t sIa[KWORDS][TI];
+  uint4 sI[KWORDS][TI + 1];
+  uint4 sJ[KWORDS][TJ + 1];
+  uint32_t sIa[KWORDS][TI + 1];
 };
 
 // Stages the planes of words [w0, w0 + kw) of the tile's sequences.
@@ -887,39 +889,44 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   if (tid == 0) A.slow_cnt[tile] = lslow;
 }
 
-// The FP64 tail of the fast path: one wave per chunk (one row, 64 columns), high occupancy so the long dependent
-// chains of divisions and logarithms of many pairs overlap; the row's reductions are summed over the wave.
+// The FP64 tail of the fast path.  A block takes whole rows; its 4 waves take the row's chunks (64 columns each) in
+// turn, at high occupancy so the long dependent chains of divisions and logarithms of many pairs overlap.  Each lane
+// keeps its share of the row's reductions over the chunks; the block combines them into one set of atomics per row.
 __global__ __launch_bounds__(256) void k_tail(RunArgs A)
 {
   __shared__ unsigned long long lhist[3 * 200];
   __shared__ BlockStats bs;
+  __shared__ RowPart wpart[4];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int k = tid; k < 600; k += 256) lhist[k] = 0;
   if (tid == 0) bs.unc = bs.undecided = 0;
   __syncthreads();
-  const int64_t nrows = A.r1 - A.r0, nslots = nrows * A.gx;
+  const int64_t nrows = A.r1 - A.r0;
   const int64_t c0a = A.c0 & ~(int64_t)63;
-  for (int64_t k = (int64_t)blockIdx.x * 4 + wave; k < nslots; k += (int64_t)gridDim.x * 4) {
-    const int64_t r = k / A.gx, bx = k % A.gx;
-    const int64_t i = A.r0 + r, j0 = c0a + bx * TJ;
-    if (A.triangle && j0 + TJ - 1 <= A.r0 + (r / TI) * TI) continue;  // the pair kernel skipped this tile
-    const ChunkHdr h = A.chdr[k];
-    if (!(h.need | h.skip)) continue;
-    const int64_t j = j0 + lane;
+  for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const int64_t i = A.r0 + r;
     RowPart rp = rowpart_zero();
-    if (h.need >> lane & 1ull) {
-      const uint32_t *d = A.cdata + (size_t)k * (7 * 64) + lane;
-      tn93g_tailin tin;
-      tin.nf[0] = (double)d[0] * 0.0625;
-      tin.nf[1] = (double)d[64] * 0.0625;
-      tin.nf[2] = (double)d[128] * 0.0625;
-      tin.nf[3] = (double)d[192] * 0.0625;
-      tin.diag = (double)d[256] * 0.0625;
-      tin.ag = (double)d[320] * 0.0625;
-      tin.ct = (double)d[384] * 0.0625;
-      finish_pair(A, i, j, tin, lhist, &bs, rp);
-    } else if (h.skip >> lane & 1ull) {
-      skip_pair(A, i, j, rp);
+    for (int64_t bx = wave; bx < A.gx; bx += 4) {
+      const int64_t j0 = c0a + bx * TJ;
+      if (A.triangle && j0 + TJ - 1 <= A.r0 + (r / TI) * TI) continue;  // the pair kernel skipped this tile
+      const int64_t k = r * A.gx + bx;
+      const ChunkHdr h = A.chdr[k];
+      if (!(h.need | h.skip)) continue;
+      const int64_t j = j0 + lane;
+      if (h.need >> lane & 1ull) {
+        const uint32_t *d = A.cdata + (size_t)k * (7 * 64) + lane;
+        tn93g_tailin tin;
+        tin.nf[0] = (double)d[0] * 0.0625;
+        tin.nf[1] = (double)d[64] * 0.0625;
+        tin.nf[2] = (double)d[128] * 0.0625;
+        tin.nf[3] = (double)d[192] * 0.0625;
+        tin.diag = (double)d[256] * 0.0625;
+        tin.ag = (double)d[320] * 0.0625;
+        tin.ct = (double)d[384] * 0.0625;
+        finish_pair(A, i, j, tin, lhist, &bs, rp);
+      } else if (h.skip >> lane & 1ull) {
+        skip_pair(A, i, j, rp);
+      }
     }
     RowPart q;
     q.slo = wave_sum_d(rp.slo);
@@ -930,7 +937,23 @@ __global__ __launch_bounds__(256) void k_tail(RunArgs A)
     q.wsum = wave_sum_l(rp.wsum);
     q.skip = wave_sum_l(rp.skip);
     q.linkw = wave_sum_l(rp.linkw);
-    if (lane == 0) rowacc_add(A, i, q);
+    if (lane == 0) wpart[wave] = q;
+    __syncthreads();
+    if (tid == 0) {
+      RowPart s = wpart[0];
+      for (int w = 1; w < 4; w++) {
+        s.slo += wpart[w].slo;
+        s.shi += wpart[w].shi;
+        s.sabs += wpart[w].sabs;
+        s.mlo = fmax(s.mlo, wpart[w].mlo);
+        s.mhi = fmax(s.mhi, wpart[w].mhi);
+        s.wsum += wpart[w].wsum;
+        s.skip += wpart[w].skip;
+        s.linkw += wpart[w].linkw;
+      }
+      rowacc_add(A, i, s);
+    }
+    __syncthreads();
   }
   flush_block(A, &bs, lhist, tid, 256);
 }
@@ -1074,7 +1097,35 @@ __device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const Repl
   tin.ct = fc[7][t] + fc[13][t];
 }
 
-// one thread per pair that needs the site-order replay (the tiles' lists, concatenated through slow_off)
+// Adds each lane's row part to its row's accumulator, one set of atomics per distinct row of the wave: the lanes of
+// a wave mostly share a few rows, and per-lane atomics on one address serialise.  All 64 lanes must call it; lanes
+// without a part pass row < 0.
+__device__ void rowacc_add_wave(const RunArgs &A, int64_t row, const RowPart &rp)
+{
+  const int lane = threadIdx.x & 63;
+  bool pend = row >= 0;
+  for (;;) {
+    const unsigned long long m = __ballot(pend);
+    if (!m) break;
+    const int leader = __ffsll((long long)m) - 1;
+    const int64_t r = __shfl(row, leader, 64);
+    const bool mine = pend && row == r;
+    RowPart q;
+    q.slo = wave_sum_d(mine ? rp.slo : 0.);
+    q.shi = wave_sum_d(mine ? rp.shi : 0.);
+    q.sabs = wave_sum_d(mine ? rp.sabs : 0.);
+    q.mlo = wave_max_d(mine ? rp.mlo : -1.);
+    q.mhi = wave_max_d(mine ? rp.mhi : -1.);
+    q.wsum = wave_sum_l(mine ? rp.wsum : 0);
+    q.skip = wave_sum_l(mine ? rp.skip : 0);
+    q.linkw = wave_sum_l(mine ? rp.linkw : 0);
+    if (lane == leader) rowacc_add(A, r, q);
+    if (mine) pend = false;
+  }
+}
+
+// one thread per pair that needs the site-order replay (the tiles' lists, concatenated through slow_off); every
+// lane of the one-wave block runs the same iterations so the row parts can be combined across the wave
 __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
 {
   __shared__ double fc[16][RB];
@@ -1087,40 +1138,45 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
   if (t == 0) bs.unc = bs.undecided = 0;
   __syncthreads();
   const int64_t total = A.slow_off[A.ntiles];
-  for (int64_t k = (int64_t)blockIdx.x * RB + t; k < total; k += (int64_t)gridDim.x * RB) {
-    int64_t lo = 0, hi = A.ntiles;  // largest tile with slow_off[tile] <= k
-    while (hi - lo > 1) {
-      const int64_t mid = (lo + hi) >> 1;
-      if (A.slow_off[mid] <= k)
-        lo = mid;
-      else
-        hi = mid;
-    }
-    const unsigned long long pr = A.slow_list[lo * (TI * TJ) + (k - A.slow_off[lo])];
-    const int64_t i = (int64_t)(pr >> 32), j = (int64_t)(pr & 0xffffffffull);
-    tn93g_tailin tin;
-    double tng;
-    uint32_t amb;
-    if (A.gapmm) {  // gaps facing bases are events too: scan every site
-      replay_scan(A, i, j, A.rtab, fc, ic, t, tin, tng, amb);
-    } else {
-      cc_table(A, i, j, icc, t);
-      replay_events(A, i, j, A.rtab, fc, ic, icc, t, tin, tng, amb);
-    }
+  for (int64_t base = (int64_t)blockIdx.x * RB; base < total; base += (int64_t)gridDim.x * RB) {
+    const int64_t k = base + t;
+    int64_t row = -1;
     RowPart rp = rowpart_zero();
-    bool skip = tng <= (double)A.min_overlap;
-    if (!skip && A.avg_fallback && tng * A.rf <= (double)amb) {  // recompute in AVERAGE (refkernel.cpp:742-750)
-      if (A.gapmm)
-        replay_scan(A, i, j, A.rtab + NCODE * NCODE, fc, ic, t, tin, tng, amb);
+    if (k < total) {
+      int64_t lo = 0, hi = A.ntiles;  // largest tile with slow_off[tile] <= k
+      while (hi - lo > 1) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (A.slow_off[mid] <= k)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      const unsigned long long pr = A.slow_list[lo * (TI * TJ) + (k - A.slow_off[lo])];
+      const int64_t i = (int64_t)(pr >> 32), j = (int64_t)(pr & 0xffffffffull);
+      tn93g_tailin tin;
+      double tng;
+      uint32_t amb;
+      if (A.gapmm) {  // gaps facing bases are events too: scan every site
+        replay_scan(A, i, j, A.rtab, fc, ic, t, tin, tng, amb);
+      } else {
+        cc_table(A, i, j, icc, t);
+        replay_events(A, i, j, A.rtab, fc, ic, icc, t, tin, tng, amb);
+      }
+      bool skip = tng <= (double)A.min_overlap;
+      if (!skip && A.avg_fallback && tng * A.rf <= (double)amb) {  // recompute in AVERAGE (refkernel.cpp:742-750)
+        if (A.gapmm)
+          replay_scan(A, i, j, A.rtab + NCODE * NCODE, fc, ic, t, tin, tng, amb);
+        else
+          replay_events(A, i, j, A.rtab + NCODE * NCODE, fc, ic, icc, t, tin, tng, amb);
+        skip = tng <= (double)A.min_overlap;
+      }
+      if (skip)
+        skip_pair(A, i, j, rp);
       else
-        replay_events(A, i, j, A.rtab + NCODE * NCODE, fc, ic, icc, t, tin, tng, amb);
-      skip = tng <= (double)A.min_overlap;
+        finish_pair(A, i, j, tin, lhist, &bs, rp);
+      row = i;
     }
-    if (skip)
-      skip_pair(A, i, j, rp);
-    else
-      finish_pair(A, i, j, tin, lhist, &bs, rp);
-    if (A.fast) rowacc_add(A, i, rp);
+    if (A.fast) rowacc_add_wave(A, row, rp);
   }
   flush_block(A, &bs, lhist, t, RB);
 }
@@ -1694,7 +1750,8 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
     HIPCHK(c, hipMemsetAsync(c->slow_cnt.p, 0, (size_t)ntiles * 4, c->stream));
   }
   HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  if (fast && nrows > 0 && A.ncols > 0) hipLaunchKernelGGL(k_tail, dim3(4096), dim3(256), 0, c->stream, A);
+  if (fast && nrows > 0 && A.ncols > 0)
+    hipLaunchKernelGGL(k_tail, dim3((unsigned)std::min<int64_t>(nrows, 65535)), dim3(256), 0, c->stream, A);
   HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
   int64_t *soff = (int64_t *)c->slow_off.p;
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)c->slow_cnt.p, soff, ntiles);
