@@ -1384,6 +1384,10 @@ struct DevBuf {
   void *p = nullptr;
   size_t n = 0;
 };
+struct HostBuf {  // pinned host memory (hipHostMalloc), grown on demand
+  void *p = nullptr;
+  size_t n = 0;
+};
 
 struct tn93g_ctx {
   int device = 0;
@@ -1392,8 +1396,8 @@ struct tn93g_ctx {
   std::string err;
   DevBuf D, U, hist, row_links, link_off, slow_list, slow_cnt, slow_off, counters, exact_list, rowout, links,
       links_flat, evtab, rtab, patch, chdr, cdata;
-  unsigned long long *pin_counters = nullptr;  // host pinned copy of the counters
-  int64_t link_cap_hint = 0;                   // link buffer size that sufficed last time
+  HostBuf pin;                 // staging of the run's read-backs (counters, links, row accumulators, histogram)
+  int64_t link_cap_hint = 0;  // link buffer size that sufficed last time
 
   int fail(int code, const std::string &m)
   {
@@ -1407,6 +1411,19 @@ struct tn93g_ctx {
     hipError_t e_ = (x);                                                                                 \
     if (e_ != hipSuccess) return (ctx)->fail(TN93G_E_HIP, fmt("%s: %s", #x, hipGetErrorString(e_))); \
   } while (0)
+
+static int ensure_pinned(tn93g_ctx *c, HostBuf &b, size_t bytes)
+{
+  if (b.n >= bytes && b.p) return 0;
+  if (b.p) (void)hipHostFree(b.p);
+  b.p = nullptr;
+  b.n = 0;
+  size_t want = std::max(bytes, (size_t)65536);
+  hipError_t e = hipHostMalloc(&b.p, want);
+  if (e != hipSuccess) return c->fail(TN93G_E_NOMEM, fmt("hipHostMalloc(%zu): %s", want, hipGetErrorString(e)));
+  b.n = want;
+  return 0;
+}
 
 static int ensure(tn93g_ctx *c, DevBuf &b, size_t bytes)
 {
@@ -1438,10 +1455,6 @@ int tn93g_open(int device, tn93g_ctx **out)
     return TN93G_E_NODEV;
   }
   for (auto &e : c->ev) (void)hipEventCreate(&e);
-  if (hipHostMalloc((void **)&c->pin_counters, 64 * sizeof(unsigned long long)) != hipSuccess) {
-    delete c;
-    return TN93G_E_NOMEM;
-  }
   *out = c;
   return 0;
 }
@@ -1456,7 +1469,7 @@ void tn93g_close(tn93g_ctx *c)
                     &c->slow_cnt, &c->slow_off, &c->links_flat})
     if (b->p) (void)hipFree(b->p);
   for (auto &e : c->ev) (void)hipEventDestroy(e);
-  if (c->pin_counters) (void)hipHostFree(c->pin_counters);
+  if (c->pin.p) (void)hipHostFree(c->pin.p);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1760,13 +1773,17 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   hipLaunchKernelGGL(k_replay, dim3(4096), dim3(RB), 0, c->stream, A);
   if (fast) hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, c->stream, A, (LinkRec *)c->links_flat.p);
   HIPCHK(c, hipGetLastError());
-  std::vector<unsigned> stripes((size_t)NSTRIPE * 32), scnt((size_t)ntiles);
-  HIPCHK(c, hipMemcpyAsync(stripes.data(), c->counters.p, stripes.size() * 4, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(scnt.data(), c->slow_cnt.p, scnt.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  const size_t stripe_bytes = (size_t)NSTRIPE * 32 * 4;
+  if (ensure_pinned(c, c->pin, stripe_bytes + 8)) return TN93G_E_NOMEM;
+  std::vector<unsigned> stripes((size_t)NSTRIPE * 32);
+  HIPCHK(c, hipMemcpyAsync(c->pin.p, c->counters.p, stripe_bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync((char *)c->pin.p + stripe_bytes, soff + ntiles, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  uint64_t n_slow = 0, n_unc = 0, n_undecided = 0, n_linkrec = 0;
+  memcpy(stripes.data(), c->pin.p, stripe_bytes);
+  int64_t slow_total = 0;
+  memcpy(&slow_total, (const char *)c->pin.p + stripe_bytes, 8);
+  uint64_t n_slow = (uint64_t)slow_total, n_unc = 0, n_undecided = 0, n_linkrec = 0;
   unsigned max_region = 0;
-  for (unsigned v : scnt) n_slow += v;
   for (int st = 0; st < NSTRIPE; st++) {
     n_unc += stripes[(size_t)st * 32 + ST_UNCERTAIN];
     n_undecided += stripes[(size_t)st * 32 + ST_UNDECIDED];
@@ -1791,28 +1808,52 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   if (fast) {
     // ---- fast path: row accumulators, unordered links
     const size_t nl = n_linkrec;
-    std::vector<LinkRec> lk(nl);
-    std::vector<RowAcc> ra(nr1);
-    if (nl) HIPCHK(c, hipMemcpyAsync(lk.data(), c->links_flat.p, nl * sizeof(LinkRec), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(ra.data(), c->rowout.p, nr1 * sizeof(RowAcc), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(hrep.data(), c->hist.p, hrep.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    const size_t off_ra = (nl * sizeof(LinkRec) + 255) & ~(size_t)255;
+    const size_t off_h = off_ra + ((nr1 * sizeof(RowAcc) + 255) & ~(size_t)255);
+    if (ensure_pinned(c, c->pin, off_h + hrep.size() * 8)) {
+      free(R);
+      return TN93G_E_NOMEM;
+    }
+    char *pin = (char *)c->pin.p;
+    if (nl) HIPCHK(c, hipMemcpyAsync(pin, c->links_flat.p, nl * sizeof(LinkRec), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pin + off_ra, c->rowout.p, nr1 * sizeof(RowAcc), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(pin + off_h, c->hist.p, hrep.size() * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    std::sort(lk.begin(), lk.end(), [](const LinkRec &x, const LinkRec &y) {
-      return x.i != y.i ? x.i < y.i : x.j < y.j;
-    });
+    const LinkRec *lk = (const LinkRec *)pin;
+    const RowAcc *ra = (const RowAcc *)(pin + off_ra);
+    memcpy(hrep.data(), pin + off_h, hrep.size() * 8);
+    // tn93's emission order (rows ascending, columns ascending): counting sort by row, then each row's few links by
+    // column
+    std::vector<int64_t> rstart((size_t)nrows + 1, 0);
+    for (size_t k = 0; k < nl; k++) rstart[(size_t)(lk[k].i - A.r0) + 1]++;
+    for (int64_t r = 0; r < nrows; r++) rstart[(size_t)r + 1] += rstart[(size_t)r];
+    std::vector<uint32_t> order(nl);
+    {
+      std::vector<int64_t> cur(rstart.begin(), rstart.end() - 1);
+      for (size_t k = 0; k < nl; k++) order[(size_t)cur[(size_t)(lk[k].i - A.r0)]++] = (uint32_t)k;
+    }
+    for (int64_t r = 0; r < nrows; r++) {
+      uint32_t *b = order.data() + rstart[(size_t)r], *e = order.data() + rstart[(size_t)r + 1];
+      if (e - b > 1) std::sort(b, e, [lk](uint32_t x, uint32_t y) { return lk[x].j < lk[y].j; });
+    }
     R->n_links = (int64_t)nl;
     R->link_row = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
     R->link_col = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
     R->link_dist = (double *)malloc(std::max<size_t>(nl, 1) * 8);
     R->link_dist_lo = (double *)malloc(std::max<size_t>(nl, 1) * 8);
     R->link_dist_hi = (double *)malloc(std::max<size_t>(nl, 1) * 8);
+    if (!R->link_row || !R->link_col || !R->link_dist || !R->link_dist_lo || !R->link_dist_hi) {
+      tn93g_result_free(R);
+      return TN93G_E_NOMEM;
+    }
     for (size_t k = 0; k < nl; k++) {
-      R->link_row[k] = lk[k].i;
-      R->link_col[k] = lk[k].j;
-      R->link_dist[k] = lk[k].d;
-      R->link_dist_lo[k] = lk[k].lo;
-      R->link_dist_hi[k] = lk[k].hi;
+      const LinkRec &q = lk[order[k]];
+      R->link_row[k] = q.i;
+      R->link_col[k] = q.j;
+      R->link_dist[k] = q.d;
+      R->link_dist_lo[k] = q.lo;
+      R->link_dist_hi[k] = q.hi;
     }
     // Enclosure of tn93's mean (swifttn93.cpp:2594-2602, 2839-2848).  Row r: tn93 adds its n_r terms t in column
     // order in FP64; here they were added in another order.  Both results are within gamma_{n_r} * sum|t| of the
