@@ -560,16 +560,12 @@ __device__ inline uint32_t quad_sum(uint32_t x)
   return x;
 }
 
-// LDS of the pair kernel.  Per chunk of KW words (popcount) or one 64-site block (MFMA): the bit planes of the tile's
-// rows and columns (both backends: the ambiguity events of the chunk look up their partner's code here), the MFMA
-// operands, and at the end (MFMA) the per-pair counts transposed into the lane = column layout of the epilogue.
-// The plane arrays carry one element of padding per word so that lanes reading different words of one sequence
-// (the staging stores, the column events) fall into different LDS banks.
+// LDS of the pair kernel.  Popcount backend: per chunk of KW words, the bit planes of the tile's rows and columns,
+// padded by one element per word so that lanes storing different words of one sequence use different LDS banks.
+// MFMA backend: the operands of one 64-site block, and at the end the per-pair counts transposed into the
+// lane = column layout of the epilogue.
 template <int KWORDS, bool MFMA>
 struct PairLds {
-  uint4 sI[KWORDS][TI + 1];
-  uint4 sJ[KWORDS][TJ + 1];
-  uint32_t sIa[KWORDS][TI + 1];
   union {
     uint4 ops[(TI + TJ) / 4][64];  // one 64-site block: TI/4 row groups + 16 column groups, 1 KB each
     uint32_t agg[7][TI][TJ];       // T, DG, TS, CT, B0, B1, BT per pair
@@ -579,7 +575,6 @@ template <int KWORDS>
 struct PairLds<KWORDS, false> {
   uint4 sI[KWORDS][TI + 1];
   uint4 sJ[KWORDS][TJ + 1];
-  uint32_t sIa[KWORDS][TI + 1];
 };
 
 // Stages the planes of words [w0, w0 + kw) of the tile's sequences.
@@ -589,24 +584,14 @@ __device__ inline void stage_planes(const RunArgs &A, LDS &lds, int64_t i0, int6
   for (int k = tid; k < kw * (TI + TJ); k += 256) {
     const int s = k / kw, w = k % kw;  // consecutive threads: consecutive words of one sequence
     const int64_t seq = s < TI ? i0 + s : j0 + (s - TI);
-    uint4 v = make_uint4(0, 0, 0, 0);
-    uint32_t am = 0;
-    if (seq < A.n) {
-      v = A.planes[seq * A.W + w0 + w];
-      if (s < TI) am = A.ambp[seq * A.W + w0 + w];
-    }
-    if (s < TI) {
+    const uint4 v = seq < A.n ? A.planes[seq * A.W + w0 + w] : make_uint4(0, 0, 0, 0);
+    if (s < TI)
       lds.sI[w][s] = v;
-      lds.sIa[w][s] = am;
-    } else
+    else
       lds.sJ[w][s - TI] = v;
   }
 }
 
-// The ambiguity events of sites [w0*32, w1*32) for the wave's 8 rows x the lane's column, from the staged planes.
-// Row events (site order, uniform per row): the column's clean base; a column that is not clean there is either a
-// gap (no contribution outside GAPMM) or ambiguous (handled with the column's events).  Column events (per lane):
-// the row's clean base, or its ambiguity code (both ambiguous, looked up in HBM: rare), or a gap.
 // Event lists of a tile: the rows' and the columns' lists are each one contiguous range of the CSR array; when
 // both fit they are read from the LDS copy in sEv (rows at [0, nr), columns at [nr, ...)), else from HBM.
 struct TileEv {
@@ -617,47 +602,64 @@ struct TileEv {
   __device__ inline uint32_t col(const RunArgs &A, int e) const { return lds ? s[nr + e - ec0] : A.ev[e]; }
 };
 
-template <class LDS>
-__device__ inline void events_chunk(const RunArgs &A, const LDS &lds, const uint4 *sTab, const TileEv &te,
-                                    int64_t w0, int64_t w1, int wave, int lane, int64_t ib, int (&ptr)[RPW],
-                                    const int (&end)[RPW], int &pj, int pj_end, EvAcc (&acc)[RPW])
+// Event tables in LDS, TABW entries per event code; code EVNONE is an all-zero row that pads partial batches.
+//   sTabR[c][x]  a row event of code c facing the column's code x (any code: a site where both sequences are
+//                ambiguous is counted here);
+//   sTabC[c][x]  a column event of code c facing the row's code x, zero where x is ambiguous (that site is counted
+//                as the row's event).
+constexpr int TABW = 32, EVNONE = NCODE, EVB = 4;
+constexpr uint32_t EV_PAD = (uint32_t)EVNONE;  // site 0, code EVNONE
+
+// The ambiguity events of the wave's RPW rows against the lane's column, over all sites, after the clean/clean
+// counts.  Row events are uniform over the wave: each lane reads its column's code at the event's site from the
+// site-major codes (one coalesced 64-byte segment per event).  Column events are per lane: the codes of the wave's
+// RPW rows at the site are one 32-bit word of the site-major codes.  Events go in batches of EVB so that their loads
+// are in flight together.  The packed 16-bit fields are flushed every 1024 events, before any field (at most 32 per
+// event) can overflow.
+__device__ inline void tile_events(const RunArgs &A, const uint4 *sTabR, const uint4 *sTabC, const TileEv &te,
+                                   int64_t ib, int64_t j, int pj, int pj_end, EvAcc (&acc)[RPW], Tot (&tot)[RPW])
 {
-  const uint32_t lim = (uint32_t)(w1 * 32);
+  static_assert(RPW == 4, "column events read the wave's rows as one 32-bit word");
+  const size_t np = (size_t)A.npad;
+  const uint8_t *ctj = A.codesT + j;
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
-    int e = ptr[a];
-    for (; e < end[a]; e++) {
-      const uint32_t ev = te.row(A, e);
-      const uint32_t p = ev >> 5;
-      if (p >= lim) break;
-      const uint4 q = lds.sJ[(p >> 5) - w0][lane];
-      const uint32_t bit = p & 31u;
-      if (q.x >> bit & 1u) {
-        const unsigned cb = (q.y >> bit & 1u) | (q.z >> bit & 1u) << 1;
-        ev_add(acc[a], sTab[(ev & 31u) * NCODE + cb]);
+    const int64_t i = ib + a;
+    if (i >= A.r1) continue;  // uniform over the wave
+    const int e0 = __builtin_amdgcn_readfirstlane(A.ev_off[i]);
+    const int e1 = __builtin_amdgcn_readfirstlane(A.ev_off[i + 1]);
+    for (int f0 = e0; f0 < e1; f0 += 1024) {
+      const int f1 = min(e1, f0 + 1024);
+      for (int e = f0; e < f1; e += EVB) {
+        uint32_t ev[EVB], cj[EVB];
+#pragma unroll
+        for (int k = 0; k < EVB; k++) ev[k] = e + k < f1 ? te.row(A, e + k) : EV_PAD;
+#pragma unroll
+        for (int k = 0; k < EVB; k++) cj[k] = ctj[(size_t)(ev[k] >> 5) * np];
+#pragma unroll
+        for (int k = 0; k < EVB; k++) ev_add(acc[a], sTabR[(ev[k] & 31u) * TABW + cj[k]]);
+      }
+      ev_flush(acc[a], tot[a]);
+    }
+  }
+  const uint8_t *cti = A.codesT + ib;
+  for (int f0 = pj; f0 < pj_end; f0 += 1024) {
+    const int f1 = min(pj_end, f0 + 1024);
+    for (int e = f0; e < f1; e += EVB) {
+      uint32_t ev[EVB], r4[EVB];
+#pragma unroll
+      for (int k = 0; k < EVB; k++) ev[k] = e + k < f1 ? te.col(A, e + k) : EV_PAD;
+#pragma unroll
+      for (int k = 0; k < EVB; k++) r4[k] = *reinterpret_cast<const uint32_t *>(cti + (size_t)(ev[k] >> 5) * np);
+#pragma unroll
+      for (int k = 0; k < EVB; k++) {
+        const uint4 *t = sTabC + (ev[k] & 31u) * TABW;
+#pragma unroll
+        for (int a = 0; a < RPW; a++) ev_add(acc[a], t[(r4[k] >> (8 * a)) & 255u]);
       }
     }
-    ptr[a] = e;
-  }
-  for (; pj < pj_end; pj++) {
-    const uint32_t ev = te.col(A, pj);
-    const uint32_t p = ev >> 5;
-    if (p >= lim) break;
-    const unsigned cb = ev & 31u;
-    const int w = (int)((p >> 5) - w0);
-    const uint32_t bit = p & 31u;
 #pragma unroll
-    for (int a = 0; a < RPW; a++) {
-      const uint4 q = lds.sI[w][wave * RPW + a];
-      unsigned ca;
-      if (q.x >> bit & 1u)
-        ca = (q.y >> bit & 1u) | (q.z >> bit & 1u) << 1;
-      else if (lds.sIa[w][wave * RPW + a] >> bit & 1u)
-        ca = A.codes[(ib + a) * A.Ls + p];
-      else
-        continue;  // gap
-      ev_add(acc[a], sTab[ca * NCODE + cb]);
-    }
+    for (int a = 0; a < RPW; a++) ev_flush(acc[a], tot[a]);
   }
 }
 
@@ -678,14 +680,20 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     lslow = 0;
     bs.unc = bs.undecided = 0;
   }
-  __shared__ PairLds<MF ? 2 : KW, MF> lds;
-  __shared__ uint4 sTab[NCODE * NCODE];
-  __shared__ unsigned long long lhist[3 * 200];
+  __shared__ PairLds<KW, MF> lds;
+  __shared__ uint4 sTabR[(NCODE + 1) * TABW], sTabC[(NCODE + 1) * TABW];
+  __shared__ unsigned long long lhist[FAST ? 1 : 3 * 200];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int k = tid; k < NCODE * NCODE; k += 256) sTab[k] = A.evtab[k];
-  for (int k = tid; k < 600; k += 256) lhist[k] = 0;
+  for (int k = tid; k < (NCODE + 1) * TABW; k += 256) {
+    const int c = k / TABW, x = k % TABW;
+    const bool in = c < NCODE && x < NCODE;
+    const bool xamb = x >= 4 && x < GAPC;
+    sTabR[k] = in ? A.evtab[c * NCODE + x] : make_uint4(0, 0, 0, 0);
+    sTabC[k] = in && !xamb ? A.evtab[x * NCODE + c] : make_uint4(0, 0, 0, 0);
+  }
+  if constexpr (!FAST)
+    for (int k = tid; k < 600; k += 256) lhist[k] = 0;
   const int64_t j = j0 + lane;
-  const int64_t jl = j < A.n ? j : A.n - 1;
   const int64_t ib = i0 + wave * RPW;
   Cnt cnt[RPW];
   Tot tot[RPW];
@@ -696,18 +704,11 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     tot[a] = Tot{{0, 0, 0, 0}, 0, 0, 0, 0};
     acc[a] = EvAcc{0, 0, 0, 0, 0};
   }
-  // event cursors: rows past the set get empty lists
-  int ptr[RPW], pend[RPW];
-#pragma unroll
-  for (int a = 0; a < RPW; a++) {
-    const int64_t i = ib + a;
-    ptr[a] = i < A.r1 ? A.ev_off[i] : 0;
-    pend[a] = i < A.r1 ? A.ev_off[i + 1] : 0;
-  }
-  int pj = A.ev_off[jl], pj_end = A.ev_off[jl + 1];
   const bool do_fast = !A.gapmm && !A.all_slow;
   const bool do_events = do_fast && !A.skip_events;
-  // the tile's event lists into LDS (synchronised by the first staging barrier below)
+  // column event cursors (columns past the set have none)
+  const int pj = j < A.n ? A.ev_off[j] : 0, pj_end = j < A.n ? A.ev_off[j + 1] : 0;
+  // the tile's event lists into LDS when they fit (synchronised by the first barrier of the counts below)
   __shared__ uint32_t sEv[EVCAP];
   TileEv te;
   {
@@ -716,8 +717,8 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     te.s = sEv;
     te.er0 = i0 < A.r1 ? A.ev_off[i0] : 0;
     te.nr = i0 < A.r1 ? A.ev_off[rl] - te.er0 : 0;
-    te.ec0 = A.ev_off[j0];
-    const int nc = A.ev_off[cl] - te.ec0;
+    te.ec0 = j0 < A.n ? A.ev_off[j0] : 0;
+    const int nc = j0 < A.n ? A.ev_off[cl] - te.ec0 : 0;
     te.lds = do_events && te.nr + nc <= EVCAP;
     if (te.lds) {
       for (int k = tid; k < te.nr; k += 256) sEv[k] = A.ev[te.er0 + k];
@@ -726,98 +727,87 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   }
   if constexpr (!MF) {
     if (do_fast)
-    for (int64_t w0 = 0; w0 < A.W; w0 += KW) {
-      const int kw = (int)min((int64_t)KW, A.W - w0);
-      __syncthreads();
-      stage_planes(A, lds, i0, j0, w0, kw, tid);
-      __syncthreads();
-      for (int k = 0; k < kw; k++) {
-        const uint4 pjv = lds.sJ[k][lane];
+      for (int64_t w0 = 0; w0 < A.W; w0 += KW) {
+        const int kw = (int)min((int64_t)KW, A.W - w0);
+        __syncthreads();
+        stage_planes(A, lds, i0, j0, w0, kw, tid);
+        __syncthreads();
+        for (int k = 0; k < kw; k++) {
+          const uint4 pjv = lds.sJ[k][lane];
 #pragma unroll
-        for (int a = 0; a < RPW; a++) count_word(lds.sI[k][wave * RPW + a], pjv, cnt[a]);
+          for (int a = 0; a < RPW; a++) count_word(lds.sI[k][wave * RPW + a], pjv, cnt[a]);
+        }
       }
-      if (do_events) {
-        events_chunk(A, lds, sTab, te, w0, w0 + kw, wave, lane, ib, ptr, pend, pj, pj_end, acc);
-#pragma unroll
-        for (int a = 0; a < RPW; a++) ev_flush(acc[a], tot[a]);  // <= 2 * 512 events per chunk: 16 bits suffice
-      }
-    }
   }
   if constexpr (MF) {
     if (do_fast) {
-    // wave w: all TI/4 row groups x column groups 4w..4w+3; one v_mfma_i32_16x16x64_i8 per (row group, column group)
-    // and 64-site block gives the 4x4 base table of 16 pairs in int32
-    v4i macc[TI / 4][4];
+      // wave w: all TI/4 row groups x column groups 4w..4w+3; one v_mfma_i32_16x16x64_i8 per (row group, column
+      // group) and 64-site block gives the 4x4 base table of 16 pairs in int32
+      v4i macc[TI / 4][4];
 #pragma unroll
-    for (int ig = 0; ig < TI / 4; ig++)
+      for (int ig = 0; ig < TI / 4; ig++)
 #pragma unroll
-      for (int jg = 0; jg < 4; jg++) macc[ig][jg] = v4i{0, 0, 0, 0};
-    const uint4 *gI = reinterpret_cast<const uint4 *>(A.onehot) + (i0 >> 2) * A.W64 * 64;
-    const uint4 *gJ = reinterpret_cast<const uint4 *>(A.onehot) + (j0 >> 2) * A.W64 * 64;
-    for (int64_t blk = 0; blk < A.W64; blk++) {
-      const int kw = (int)min((int64_t)2, A.W - 2 * blk);
+        for (int jg = 0; jg < 4; jg++) macc[ig][jg] = v4i{0, 0, 0, 0};
+      const uint4 *gI = reinterpret_cast<const uint4 *>(A.onehot) + (i0 >> 2) * A.W64 * 64;
+      const uint4 *gJ = reinterpret_cast<const uint4 *>(A.onehot) + (j0 >> 2) * A.W64 * 64;
+      for (int64_t blk = 0; blk < A.W64; blk++) {
+        __syncthreads();
+        for (int k = tid; k < (TI + TJ) / 4 * 64; k += 256) {
+          const int g = k >> 6, q = k & 63;
+          lds.u.ops[g][q] = g < TI / 4 ? gI[((int64_t)g * A.W64 + blk) * 64 + q]
+                                       : gJ[((int64_t)(g - TI / 4) * A.W64 + blk) * 64 + q];
+        }
+        __syncthreads();
+        v4i bop[4];
+#pragma unroll
+        for (int jg = 0; jg < 4; jg++) {
+          const uint4 v = lds.u.ops[TI / 4 + 4 * wave + jg][lane];
+          bop[jg] = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+        }
+#pragma unroll
+        for (int ig = 0; ig < TI / 4; ig++) {
+          const uint4 v = lds.u.ops[ig][lane];
+          const v4i aop = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+#pragma unroll
+          for (int jg = 0; jg < 4; jg++)
+            macc[ig][jg] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aop, bop[jg], macc[ig][jg], 0, 0, 0);
+        }
+      }
       __syncthreads();
-      for (int k = tid; k < (TI + TJ) / 4 * 64; k += 256) {
-        const int g = k >> 6, q = k & 63;
-        lds.u.ops[g][q] = g < TI / 4 ? gI[((int64_t)g * A.W64 + blk) * 64 + q]
-                                     : gJ[((int64_t)(g - TI / 4) * A.W64 + blk) * 64 + q];
-      }
-      if (do_events) stage_planes(A, lds, i0, j0, 2 * blk, kw, tid);
+      // lane l of tile (ig, jg) holds C[a][b] for a = 0..3 of the pair (row 4ig + l/16, column 4(4w+jg) + (l%16)/4)
+      // with b = l%4; reduce over the quad into the seven counts of the popcount backend
+      const int b = lane & 3;
+#pragma unroll
+      for (int ig = 0; ig < TI / 4; ig++)
+#pragma unroll
+        for (int jg = 0; jg < 4; jg++) {
+          const v4i v = macc[ig][jg];
+          const uint32_t v0 = (uint32_t)v.x, v1 = (uint32_t)v.y, v2 = (uint32_t)v.z, v3 = (uint32_t)v.w;
+          const uint32_t col = v0 + v1 + v2 + v3;
+          const uint32_t dg = b == 0 ? v0 : b == 1 ? v1 : b == 2 ? v2 : v3;
+          const uint32_t ct = b == 1 ? v3 : b == 3 ? v1 : 0u;
+          const uint32_t ts = ct + (b == 0 ? v2 : b == 2 ? v0 : 0u);
+          const uint32_t b0 = v1 + v3 + ((b & 1) ? col : 0u);
+          const uint32_t b1 = v2 + v3 + ((b >> 1) ? col : 0u);
+          const uint32_t bt = v3 + (b == 3 ? col : 0u);
+          const uint32_t q0 = quad_sum(col), q1 = quad_sum(dg), q2 = quad_sum(ts), q3 = quad_sum(ct);
+          const uint32_t q4 = quad_sum(b0), q5 = quad_sum(b1), q6 = quad_sum(bt);
+          const int il = 4 * ig + (lane >> 4), jloc = 4 * (4 * wave + jg) + ((lane & 15) >> 2);
+          // lane b of the quad stores fields b and b + 4
+          lds.u.agg[b][il][jloc] = b == 0 ? q0 : b == 1 ? q1 : b == 2 ? q2 : q3;
+          if (b < 3) lds.u.agg[b + 4][il][jloc] = b == 0 ? q4 : b == 1 ? q5 : q6;
+        }
       __syncthreads();
-      v4i bop[4];
 #pragma unroll
-      for (int jg = 0; jg < 4; jg++) {
-        const uint4 v = lds.u.ops[TI / 4 + 4 * wave + jg][lane];
-        bop[jg] = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+      for (int a = 0; a < RPW; a++) {
+        const int il = wave * RPW + a;
+        cnt[a] = Cnt{lds.u.agg[0][il][lane], lds.u.agg[1][il][lane], lds.u.agg[2][il][lane], lds.u.agg[3][il][lane],
+                     lds.u.agg[4][il][lane], lds.u.agg[5][il][lane], lds.u.agg[6][il][lane]};
       }
-#pragma unroll
-      for (int ig = 0; ig < TI / 4; ig++) {
-        const uint4 v = lds.u.ops[ig][lane];
-        const v4i aop = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
-#pragma unroll
-        for (int jg = 0; jg < 4; jg++)
-          macc[ig][jg] = __builtin_amdgcn_mfma_i32_16x16x64_i8(aop, bop[jg], macc[ig][jg], 0, 0, 0);
-      }
-      if (do_events) {
-        events_chunk(A, lds, sTab, te, 2 * blk, 2 * blk + kw, wave, lane, ib, ptr, pend, pj, pj_end, acc);
-#pragma unroll
-        for (int a = 0; a < RPW; a++) ev_flush(acc[a], tot[a]);
-      }
-    }
-    __syncthreads();
-    // lane l of tile (ig, jg) holds C[a][b] for a = 0..3 of the pair (row 4ig + l/16, column 4(4w+jg) + (l%16)/4)
-    // with b = l%4; reduce over the quad into the seven counts of the popcount backend
-    const int b = lane & 3;
-#pragma unroll
-    for (int ig = 0; ig < TI / 4; ig++)
-#pragma unroll
-      for (int jg = 0; jg < 4; jg++) {
-        const v4i v = macc[ig][jg];
-        const uint32_t v0 = (uint32_t)v.x, v1 = (uint32_t)v.y, v2 = (uint32_t)v.z, v3 = (uint32_t)v.w;
-        const uint32_t col = v0 + v1 + v2 + v3;
-        const uint32_t dg = b == 0 ? v0 : b == 1 ? v1 : b == 2 ? v2 : v3;
-        const uint32_t ct = b == 1 ? v3 : b == 3 ? v1 : 0u;
-        const uint32_t ts = ct + (b == 0 ? v2 : b == 2 ? v0 : 0u);
-        const uint32_t b0 = v1 + v3 + ((b & 1) ? col : 0u);
-        const uint32_t b1 = v2 + v3 + ((b >> 1) ? col : 0u);
-        const uint32_t bt = v3 + (b == 3 ? col : 0u);
-        const uint32_t q0 = quad_sum(col), q1 = quad_sum(dg), q2 = quad_sum(ts), q3 = quad_sum(ct);
-        const uint32_t q4 = quad_sum(b0), q5 = quad_sum(b1), q6 = quad_sum(bt);
-        const int il = 4 * ig + (lane >> 4), jloc = 4 * (4 * wave + jg) + ((lane & 15) >> 2);
-        // lane b of the quad stores fields b and b + 4
-        lds.u.agg[b][il][jloc] = b == 0 ? q0 : b == 1 ? q1 : b == 2 ? q2 : q3;
-        if (b < 3) lds.u.agg[b + 4][il][jloc] = b == 0 ? q4 : b == 1 ? q5 : q6;
-      }
-    __syncthreads();
-#pragma unroll
-    for (int a = 0; a < RPW; a++) {
-      const int il = wave * RPW + a;
-      cnt[a] = Cnt{lds.u.agg[0][il][lane], lds.u.agg[1][il][lane], lds.u.agg[2][il][lane], lds.u.agg[3][il][lane],
-                   lds.u.agg[4][il][lane], lds.u.agg[5][il][lane], lds.u.agg[6][il][lane]};
-    }
     }
   }
-  __syncthreads();
+  __syncthreads();  // the event tables and lists are in LDS (also when no counts ran)
+  if (do_events) tile_events(A, sTabR, sTabC, te, ib, j, pj, pj_end, acc, tot);
   // totals x16 from the clean/clean counts, plus the events
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
