@@ -133,9 +133,8 @@ static SiteEffect site_effect(int c1, int c2, int mode, uint32_t rta)
 struct ReplayEnt {
   double v;
   uint32_t mask;
-  int32_t icd;  // -1: none
-  int32_t amb;
-  int32_t pad;
+  int16_t icd;  // -1: none
+  uint16_t amb;
 };
 
 // Event-table entry of the pair kernel: the effect scaled by 16 and aggregated into what the formula needs, packed
@@ -237,6 +236,7 @@ struct RunArgs {
   unsigned long long *slow_list;  // [tile][TI*TJ] pairs for the site-order replay, per pair tile
   unsigned int *slow_cnt;         // [tile]
   const int64_t *slow_off;        // [tile + 1]: exclusive scan of slow_cnt (k_replay finds a pair's tile in it)
+  const int32_t *slow_chunk;      // [k / 64]: the tile holding flat replay index k, for k a multiple of 64
   int64_t ntiles;
   ExactRec *exact_list;
   unsigned int *exact_cnt;
@@ -951,71 +951,65 @@ __global__ __launch_bounds__(256) void k_tail(RunArgs A)
 // ------------------------------------------------------------------------------------------ replay kernel
 // computeTN93's scan (refkernel.cpp:384-729) for one pair per thread, in site order: clean/clean sites counted in
 // integers, every other site applied through the handler table in FP64 in increasing site order, then tn93's merge.
-constexpr int RB = 64;
+// The pair's 16 float cells, 16 clean/clean counts and 4 resolved-diagonal counts live in registers; a site's effect
+// is added to every cell, 0.0 where its mask is clear (the cells hold non-negative sums, so x + 0.0 == x).
+constexpr int RB = 256;
 
-__device__ bool replay_scan(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *__restrict__ tab,
-                            double (*fc)[RB], int (*ic)[RB], int t, tn93g_tailin &tin, double &tng, uint32_t &amb)
+struct ReplayState {
+  double fc[16];
+  uint32_t dgi[4];  // integer_counts[d][d] increments of resolved ambiguities
+  uint32_t amb;
+};
+
+__device__ inline void replay_reset(ReplayState &s)
 {
-  for (int k = 0; k < 16; k++) {
-    fc[k][t] = 0.0;
-    ic[k][t] = 0;
-  }
-  amb = 0;
-  const uint4 *s1 = reinterpret_cast<const uint4 *>(A.codes + i * A.Ls);
-  const uint4 *s2 = reinterpret_cast<const uint4 *>(A.codes + j * A.Ls);
-  for (int64_t q = 0; q < A.Ls / 16; q++) {
-    const uint4 a4 = s1[q], b4 = s2[q];
-    const uint32_t aw[4] = {a4.x, a4.y, a4.z, a4.w}, bw[4] = {b4.x, b4.y, b4.z, b4.w};
-    for (int wd = 0; wd < 4; wd++) {
-      if (((aw[wd] | bw[wd]) & 0xFCFCFCFCu) == 0) {  // four clean/clean sites
-        for (int k = 0; k < 4; k++) {
-          unsigned c1 = aw[wd] >> (8 * k) & 255u, c2 = bw[wd] >> (8 * k) & 255u;
-          ic[c1 * 4 + c2][t]++;
-        }
-        continue;
-      }
-      for (int k = 0; k < 4; k++) {
-        unsigned c1 = aw[wd] >> (8 * k) & 255u, c2 = bw[wd] >> (8 * k) & 255u;
-        if ((c1 | c2) < 4) {
-          ic[c1 * 4 + c2][t]++;
-          continue;
-        }
-        const ReplayEnt E = tab[c1 * NCODE + c2];
-        uint32_t m = E.mask;
-        while (m) {
-          int c = __builtin_ctz(m);
-          m &= m - 1;
-          fc[c][t] += E.v;
-        }
-        if (E.icd >= 0) ic[5 * E.icd][t]++;
-        amb += (uint32_t)E.amb;
-      }
-    }
-  }
-  // merge (refkernel.cpp:714-729)
+#pragma unroll
+  for (int k = 0; k < 16; k++) s.fc[k] = 0.0;
+#pragma unroll
+  for (int d = 0; d < 4; d++) s.dgi[d] = 0;
+  s.amb = 0;
+}
+
+__device__ inline void replay_apply(ReplayState &s, const ReplayEnt &E)
+{
+  const double v = E.v;
+#pragma unroll
+  for (int k = 0; k < 16; k++) s.fc[k] += (E.mask >> k & 1u) ? v : 0.0;
+#pragma unroll
+  for (int d = 0; d < 4; d++) s.dgi[d] += E.icd == d ? 1u : 0u;
+  s.amb += E.amb;
+}
+
+// tn93's merge (refkernel.cpp:714-729): pc = float + integer counts, summed in row-major cell order
+__device__ inline void replay_merge(const ReplayState &s, const uint32_t (&icc)[16], tn93g_tailin &tin, double &tng)
+{
+  double pc[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) pc[k] = s.fc[k] + (double)(icc[k] + (k % 5 == 0 ? s.dgi[k / 5] : 0u));
   double nf[4] = {0., 0., 0., 0.};
   tng = 0.;
+#pragma unroll
   for (int a = 0; a < 4; a++)
+#pragma unroll
     for (int b = 0; b < 4; b++) {
-      double pc = (fc[4 * a + b][t] += (double)ic[4 * a + b][t]);
-      tng += pc;
-      nf[a] += pc;
-      nf[b] += pc;
+      tng += pc[4 * a + b];
+      nf[a] += pc[4 * a + b];
+      nf[b] += pc[4 * a + b];
     }
+#pragma unroll
   for (int k = 0; k < 4; k++) tin.nf[k] = nf[k];
-  tin.diag = fc[0][t] + fc[5][t] + fc[10][t] + fc[15][t];
-  tin.ag = fc[2][t] + fc[8][t];
-  tin.ct = fc[7][t] + fc[13][t];
-  return true;
+  tin.diag = pc[0] + pc[5] + pc[10] + pc[15];
+  tin.ag = pc[2] + pc[8];
+  tin.ct = pc[7] + pc[13];
 }
 
 // The clean/clean 4x4 integer table of a pair from the bit planes (integer_counts before any resolved ambiguity).
-__device__ void cc_table(const RunArgs &A, int64_t i, int64_t j, int (*icc)[RB], int t)
+__device__ inline void cc_table(const RunArgs &A, int64_t i, int64_t j, uint32_t (&c)[16])
 {
-  uint32_t c[16];
 #pragma unroll
   for (int k = 0; k < 16; k++) c[k] = 0;
   const uint4 *Pi = A.planes + i * A.W, *Pj = A.planes + j * A.W;
+#pragma unroll 2
   for (int64_t w = 0; w < A.W; w++) {
     const uint4 pi = Pi[w], pj = Pj[w];
     const uint32_t M = pi.x & pj.x;
@@ -1026,26 +1020,16 @@ __device__ void cc_table(const RunArgs &A, int64_t i, int64_t j, int (*icc)[RB],
 #pragma unroll
       for (int y = 0; y < 4; y++) c[4 * x + y] += __popc(ai[x] & bj[y]);
   }
-#pragma unroll
-  for (int k = 0; k < 16; k++) icc[k][t] = (int)c[k];
 }
 
-// computeTN93's ambiguity handling in site order (refkernel.cpp:384-729) for a pair whose contributions depend on
-// the order: only the sites where either sequence is ambiguous contribute to float_counts, so the two sorted event
-// lists are merged and each event applied in FP64 in increasing site order; then tn93's merge.
-__device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *__restrict__ tab,
-                              double (*fc)[RB], int (*ic)[RB], int (*icc)[RB], int t, tn93g_tailin &tin, double &tng,
-                              uint32_t &amb)
+// The sites where either sequence is ambiguous, in increasing site order: the two sorted event lists merged; a
+// site in one list only takes its partner's code from the codes array.
+__device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *tab, ReplayState &s)
 {
-  for (int k = 0; k < 16; k++) {
-    fc[k][t] = 0.0;
-    ic[k][t] = icc[k][t];
-  }
-  amb = 0;
   int a = A.ev_off[i], ae = A.ev_off[i + 1], b = A.ev_off[j], be = A.ev_off[j + 1];
   const uint8_t *si = A.codes + i * A.Ls, *sj = A.codes + j * A.Ls;
+  uint32_t ua = a < ae ? A.ev[a] : 0xFFFFFFFFu, ub = b < be ? A.ev[b] : 0xFFFFFFFFu;
   while (a < ae || b < be) {
-    const uint32_t ua = a < ae ? A.ev[a] : 0xFFFFFFFFu, ub = b < be ? A.ev[b] : 0xFFFFFFFFu;
     const uint32_t pa = ua >> 5, pb = ub >> 5;
     unsigned c1, c2;
     if (pa == pb) {
@@ -1053,38 +1037,39 @@ __device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const Repl
       c2 = ub & 31u;
       a++;
       b++;
+      ua = a < ae ? A.ev[a] : 0xFFFFFFFFu;
+      ub = b < be ? A.ev[b] : 0xFFFFFFFFu;
     } else if (pa < pb) {
       c1 = ua & 31u;
       c2 = sj[pa];
       a++;
+      ua = a < ae ? A.ev[a] : 0xFFFFFFFFu;
     } else {
       c1 = si[pb];
       c2 = ub & 31u;
       b++;
+      ub = b < be ? A.ev[b] : 0xFFFFFFFFu;
     }
-    const ReplayEnt E = tab[c1 * NCODE + c2];
-    uint32_t m = E.mask;
-    while (m) {
-      const int c = __builtin_ctz(m);
-      m &= m - 1;
-      fc[c][t] += E.v;
-    }
-    if (E.icd >= 0) ic[5 * E.icd][t]++;
-    amb += (uint32_t)E.amb;
+    replay_apply(s, tab[c1 * NCODE + c2]);
   }
-  double nf[4] = {0., 0., 0., 0.};
-  tng = 0.;
-  for (int x = 0; x < 4; x++)
-    for (int y = 0; y < 4; y++) {
-      double pc = (fc[4 * x + y][t] += (double)ic[4 * x + y][t]);
-      tng += pc;
-      nf[x] += pc;
-      nf[y] += pc;
+}
+
+// GAPMM: gaps facing bases are events too, so every site that is not clean in both sequences is applied in order.
+__device__ void replay_scan(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *tab, ReplayState &s)
+{
+  const uint4 *s1 = reinterpret_cast<const uint4 *>(A.codes + i * A.Ls);
+  const uint4 *s2 = reinterpret_cast<const uint4 *>(A.codes + j * A.Ls);
+  for (int64_t q = 0; q < A.Ls / 16; q++) {
+    const uint4 a4 = s1[q], b4 = s2[q];
+    const uint32_t aw[4] = {a4.x, a4.y, a4.z, a4.w}, bw[4] = {b4.x, b4.y, b4.z, b4.w};
+    for (int wd = 0; wd < 4; wd++) {
+      if (((aw[wd] | bw[wd]) & 0xFCFCFCFCu) == 0) continue;  // four clean/clean sites (counted from the planes)
+      for (int k = 0; k < 4; k++) {
+        const unsigned c1 = aw[wd] >> (8 * k) & 255u, c2 = bw[wd] >> (8 * k) & 255u;
+        if ((c1 | c2) >= 4) replay_apply(s, tab[c1 * NCODE + c2]);
+      }
     }
-  for (int k = 0; k < 4; k++) tin.nf[k] = nf[k];
-  tin.diag = fc[0][t] + fc[5][t] + fc[10][t] + fc[15][t];
-  tin.ag = fc[2][t] + fc[8][t];
-  tin.ct = fc[7][t] + fc[13][t];
+  }
 }
 
 // Adds each lane's row part to its row's accumulator, one set of atomics per distinct row of the wave: the lanes of
@@ -1114,50 +1099,80 @@ __device__ void rowacc_add_wave(const RunArgs &A, int64_t row, const RowPart &rp
   }
 }
 
+// The tile holding flat replay index k, for the 64 consecutive indices of a wave starting at kb (a multiple of 64):
+// slow_chunk[kb / 64] is the tile of kb; the next 63 tile offsets are searched across the wave, and an index past
+// them falls back to a binary search over all offsets.
+__device__ inline int64_t replay_tile(const RunArgs &A, int64_t kb, int64_t k)
+{
+  const int lane = threadIdx.x & 63;
+  const int64_t t0 = A.slow_chunk[kb >> 6];
+  const int64_t tl = t0 + lane;
+  const int64_t off = tl <= A.ntiles ? A.slow_off[tl] : INT64_MAX;
+  int lo = 0, hi = 64;  // largest l with slow_off[t0 + l] <= k
+  for (int step = 0; step < 6; step++) {
+    const int mid = (lo + hi) >> 1;
+    const int64_t v = __shfl(off, mid, 64);
+    if (v <= k)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  int64_t t = t0 + lo;
+  if (lo == 63 && t < A.ntiles && A.slow_off[t + 1] <= k) {
+    int64_t l2 = t, h2 = A.ntiles;
+    while (h2 - l2 > 1) {
+      const int64_t mid = (l2 + h2) >> 1;
+      if (A.slow_off[mid] <= k)
+        l2 = mid;
+      else
+        h2 = mid;
+    }
+    t = l2;
+  }
+  return t;
+}
+
 // one thread per pair that needs the site-order replay (the tiles' lists, concatenated through slow_off); every
-// lane of the one-wave block runs the same iterations so the row parts can be combined across the wave
+// lane of a wave runs the same iterations so the row parts can be combined across the wave
 __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
 {
-  __shared__ double fc[16][RB];
-  __shared__ int ic[16][RB];
-  __shared__ int icc[16][RB];
+  __shared__ ReplayEnt tabs[2 * NCODE * NCODE];
   __shared__ unsigned long long lhist[3 * 200];
   __shared__ BlockStats bs;
-  const int t = threadIdx.x;
-  for (int k = t; k < 600; k += RB) lhist[k] = 0;
-  if (t == 0) bs.unc = bs.undecided = 0;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int k = tid; k < 2 * NCODE * NCODE; k += RB) tabs[k] = A.rtab[k];
+  for (int k = tid; k < 600; k += RB) lhist[k] = 0;
+  if (tid == 0) bs.unc = bs.undecided = 0;
   __syncthreads();
   const int64_t total = A.slow_off[A.ntiles];
-  for (int64_t base = (int64_t)blockIdx.x * RB; base < total; base += (int64_t)gridDim.x * RB) {
-    const int64_t k = base + t;
+  const int64_t nw = (int64_t)gridDim.x * (RB / 64);
+  for (int64_t kb = ((int64_t)blockIdx.x * (RB / 64) + wave) * 64; kb < total; kb += nw * 64) {
+    const int64_t k = kb + lane;
+    const int64_t tile = replay_tile(A, kb, k < total ? k : kb);
     int64_t row = -1;
     RowPart rp = rowpart_zero();
     if (k < total) {
-      int64_t lo = 0, hi = A.ntiles;  // largest tile with slow_off[tile] <= k
-      while (hi - lo > 1) {
-        const int64_t mid = (lo + hi) >> 1;
-        if (A.slow_off[mid] <= k)
-          lo = mid;
-        else
-          hi = mid;
-      }
-      const unsigned long long pr = A.slow_list[lo * (TI * TJ) + (k - A.slow_off[lo])];
+      const unsigned long long pr = A.slow_list[tile * (TI * TJ) + (k - A.slow_off[tile])];
       const int64_t i = (int64_t)(pr >> 32), j = (int64_t)(pr & 0xffffffffull);
+      uint32_t icc[16];
+      cc_table(A, i, j, icc);
+      ReplayState s;
+      replay_reset(s);
+      if (A.gapmm)
+        replay_scan(A, i, j, tabs, s);
+      else
+        replay_events(A, i, j, tabs, s);
       tn93g_tailin tin;
       double tng;
-      uint32_t amb;
-      if (A.gapmm) {  // gaps facing bases are events too: scan every site
-        replay_scan(A, i, j, A.rtab, fc, ic, t, tin, tng, amb);
-      } else {
-        cc_table(A, i, j, icc, t);
-        replay_events(A, i, j, A.rtab, fc, ic, icc, t, tin, tng, amb);
-      }
+      replay_merge(s, icc, tin, tng);
       bool skip = tng <= (double)A.min_overlap;
-      if (!skip && A.avg_fallback && tng * A.rf <= (double)amb) {  // recompute in AVERAGE (refkernel.cpp:742-750)
+      if (!skip && A.avg_fallback && tng * A.rf <= (double)s.amb) {  // recompute in AVERAGE (refkernel.cpp:742-750)
+        replay_reset(s);
         if (A.gapmm)
-          replay_scan(A, i, j, A.rtab + NCODE * NCODE, fc, ic, t, tin, tng, amb);
+          replay_scan(A, i, j, tabs + NCODE * NCODE, s);
         else
-          replay_events(A, i, j, A.rtab + NCODE * NCODE, fc, ic, icc, t, tin, tng, amb);
+          replay_events(A, i, j, tabs + NCODE * NCODE, s);
+        replay_merge(s, icc, tin, tng);
         skip = tng <= (double)A.min_overlap;
       }
       if (skip)
@@ -1168,7 +1183,7 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
     }
     if (A.fast) rowacc_add_wave(A, row, rp);
   }
-  flush_block(A, &bs, lhist, t, RB);
+  flush_block(A, &bs, lhist, tid, RB);
 }
 
 // the links of the NSTRIPE regions, concatenated into `out` (one block per region)
@@ -1206,31 +1221,54 @@ __global__ void k_patch(RunArgs A, const PatchRec *__restrict__ P, int64_t np)
 }
 
 // exclusive scan of the per-row link counts (one block)
-__global__ void k_scan(const int32_t *__restrict__ in, int64_t *__restrict__ out, int64_t n)
+// Exclusive prefix sum of n int32 values into out[0..n), total in out[n]; one block of 1024 threads walks the array
+// in chunks of 4096 (4 consecutive values per thread, wave scans, then a scan of the 16 wave totals).  With `chunk`,
+// also records for every multiple m of 64 below the total the index whose range [out[k], out[k] + in[k]) holds it,
+// at chunk[m / 64].
+__global__ __launch_bounds__(1024) void k_scan(const int32_t *__restrict__ in, int64_t *__restrict__ out, int64_t n,
+                                               int32_t *__restrict__ chunk)
 {
-  __shared__ int64_t part[1024];
-  const int t = threadIdx.x;
-  const int64_t per = (n + 1023) / 1024;
-  const int64_t b = t * per, e = min(n, b + per);
-  int64_t s = 0;
-  for (int64_t k = b; k < e; k++) s += in[k];
-  part[t] = s;
-  __syncthreads();
-  if (t == 0) {
-    int64_t run = 0;
-    for (int k = 0; k < 1024; k++) {
-      int64_t v = part[k];
-      part[k] = run;
-      run += v;
+  __shared__ int64_t wsum[16];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int64_t carry = 0;
+  for (int64_t base = 0; base < n; base += 4096) {
+    const int64_t k0 = base + 4 * t;
+    int64_t v[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) v[q] = k0 + q < n ? in[k0 + q] : 0;
+    const int64_t s = v[0] + v[1] + v[2] + v[3];
+    int64_t x = s;  // inclusive scan over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int64_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
     }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    if (w == 0) {
+      int64_t z = lane < 16 ? wsum[lane] : 0;
+#pragma unroll
+      for (int o = 1; o < 16; o <<= 1) {
+        const int64_t y = __shfl_up(z, o, 64);
+        if (lane >= o) z += y;
+      }
+      if (lane < 16) wsum[lane] = z;
+    }
+    __syncthreads();
+    int64_t run = carry + (w ? wsum[w - 1] : 0) + x - s;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (k0 + q < n) {
+        out[k0 + q] = run;
+        if (chunk)
+          for (int64_t m = (run + 63) >> 6; (m << 6) < run + v[q]; m++) chunk[m] = (int32_t)(k0 + q);
+      }
+      run += v[q];
+    }
+    carry += wsum[15];
+    __syncthreads();
   }
-  __syncthreads();
-  int64_t run = part[t];
-  for (int64_t k = b; k < e; k++) {
-    out[k] = run;
-    run += in[k];
-  }
-  if (t == 1023) out[n] = run;
+  if (t == 0) out[n] = carry;
 }
 
 // ------------------------------------------------------------------------------------------ row kernel
@@ -1384,8 +1422,8 @@ struct tn93g_ctx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4];
   std::string err;
-  DevBuf D, U, hist, row_links, link_off, slow_list, slow_cnt, slow_off, counters, exact_list, rowout, links,
-      links_flat, evtab, rtab, patch, chdr, cdata;
+  DevBuf D, U, hist, row_links, link_off, slow_list, slow_cnt, slow_off, slow_chunk, counters, exact_list, rowout,
+      links, links_flat, evtab, rtab, patch, chdr, cdata;
   HostBuf pin;                 // staging of the run's read-backs (counters, links, row accumulators, histogram)
   int64_t link_cap_hint = 0;  // link buffer size that sufficed last time
 
@@ -1456,7 +1494,7 @@ void tn93g_close(tn93g_ctx *c)
   (void)hipStreamSynchronize(c->stream);
   for (DevBuf *b : {&c->D, &c->U, &c->hist, &c->row_links, &c->link_off, &c->slow_list, &c->counters,
                     &c->exact_list, &c->rowout, &c->links, &c->evtab, &c->rtab, &c->patch, &c->chdr, &c->cdata,
-                    &c->slow_cnt, &c->slow_off, &c->links_flat})
+                    &c->slow_cnt, &c->slow_off, &c->slow_chunk, &c->links_flat})
     if (b->p) (void)hipFree(b->p);
   for (auto &e : c->ev) (void)hipEventDestroy(e);
   if (c->pin.p) (void)hipHostFree(c->pin.p);
@@ -1675,9 +1713,8 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
           ReplayEnt &r = rt[(size_t)(t * NCODE * NCODE + a * NCODE + b)];
           r.v = F.v;
           r.mask = F.mask;
-          r.icd = F.icd;
-          r.amb = F.amb;
-          r.pad = 0;
+          r.icd = (int16_t)F.icd;
+          r.amb = (uint16_t)F.amb;
         }
       }
     if (ensure(c, c->evtab, et.size() * sizeof(uint4)) || ensure(c, c->rtab, rt.size() * sizeof(ReplayEnt)))
@@ -1700,7 +1737,7 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
       ensure(c, c->hist, (size_t)NSTRIPE * 600 * 8) || ensure(c, c->row_links, nr1 * 4) ||
       ensure(c, c->link_off, (size_t)(nrows + 1) * 8) ||
       ensure(c, c->slow_list, (size_t)ntiles * TI * TJ * 8) || ensure(c, c->slow_cnt, (size_t)ntiles * 4) ||
-      ensure(c, c->slow_off, (size_t)(ntiles + 1) * 8) ||
+      ensure(c, c->slow_off, (size_t)(ntiles + 1) * 8) || ensure(c, c->slow_chunk, (size_t)(16 * ntiles + 1) * 4) ||
       ensure(c, c->counters, (size_t)NSTRIPE * 32 * 4) ||
       ensure(c, c->rowout, nr1 * std::max<size_t>(15 * 8 + 16, sizeof(RowAcc))) ||
       ensure(c, c->chdr, (size_t)nslots * sizeof(ChunkHdr)) ||
@@ -1757,10 +1794,12 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
     hipLaunchKernelGGL(k_tail, dim3((unsigned)std::min<int64_t>(nrows, 65535)), dim3(256), 0, c->stream, A);
   HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
   int64_t *soff = (int64_t *)c->slow_off.p;
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)c->slow_cnt.p, soff, ntiles);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)c->slow_cnt.p, soff, ntiles,
+                     (int32_t *)c->slow_chunk.p);
   A.slow_off = soff;
+  A.slow_chunk = (const int32_t *)c->slow_chunk.p;
   A.ntiles = ntiles;
-  hipLaunchKernelGGL(k_replay, dim3(4096), dim3(RB), 0, c->stream, A);
+  hipLaunchKernelGGL(k_replay, dim3(2048), dim3(RB), 0, c->stream, A);
   if (fast) hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, c->stream, A, (LinkRec *)c->links_flat.p);
   HIPCHK(c, hipGetLastError());
   const size_t stripe_bytes = (size_t)NSTRIPE * 32 * 4;
@@ -1921,7 +1960,8 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
     }
     // link offsets, then the rows in column order
     int64_t *loff = (int64_t *)c->link_off.p;
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)c->row_links.p, loff, nrows);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)c->row_links.p, loff, nrows,
+                       (int32_t *)nullptr);
     int64_t n_links = 0;
     HIPCHK(c, hipMemcpyAsync(&n_links, loff + nrows, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
