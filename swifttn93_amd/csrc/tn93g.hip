@@ -226,7 +226,7 @@ struct RunArgs {
   int triangle, fst, gapmm, skip_events, avg_fallback, exact, all_slow;
   double rf, thr, mind, window;
   int64_t min_overlap;
-  const uint4 *evtab;          // [18*18] packed event entries (main mode)
+  const uint4 *evtab;          // [2][18*18] packed event entries: main mode, AVERAGE (the -g fallback)
   const ReplayEnt *rtab;       // [2][18*18]: main mode, AVERAGE
   // outputs
   double *D;
@@ -610,22 +610,40 @@ struct TileEv {
 constexpr int TABW = 32, EVNONE = NCODE, EVB = 4;
 constexpr uint32_t EV_PAD = (uint32_t)EVNONE;  // site 0, code EVNONE
 
+// Fills the LDS event tables from one [NCODE][NCODE] table of packed entries (entry [row code][column code]).
+__device__ inline void load_event_tables(const uint4 *src, uint4 *sTabR, uint4 *sTabC, int tid)
+{
+  for (int k = tid; k < (NCODE + 1) * TABW; k += 256) {
+    const int c = k / TABW, x = k % TABW;
+    const bool in = c < NCODE && x < NCODE;
+    const bool xamb = x >= 4 && x < GAPC;
+    sTabR[k] = in ? src[c * NCODE + x] : make_uint4(0, 0, 0, 0);
+    sTabC[k] = in && !xamb ? src[x * NCODE + c] : make_uint4(0, 0, 0, 0);
+  }
+}
+
+__device__ inline uint4 keep_if(bool k, uint4 t) { return k ? t : make_uint4(0, 0, 0, 0); }
+
 // The ambiguity events of the wave's RPW rows against the lane's column, over all sites, after the clean/clean
 // counts.  Row events are uniform over the wave: each lane reads its column's code at the event's site from the
 // site-major codes (one coalesced 64-byte segment per event).  Column events are per lane: the codes of the wave's
 // RPW rows at the site are one 32-bit word of the site-major codes.  Events go in batches of EVB so that their loads
 // are in flight together.  The packed 16-bit fields are flushed every 1024 events, before any field (at most 32 per
-// event) can overflow.
+// event) can overflow.  MASKED: only the (row, lane) pairs with use[a] set take the entries (the others add zero),
+// and rows or lanes with no such pair skip the loops.
+template <bool MASKED>
 __device__ inline void tile_events(const RunArgs &A, const uint4 *sTabR, const uint4 *sTabC, const TileEv &te,
-                                   int64_t ib, int64_t j, int pj, int pj_end, EvAcc (&acc)[RPW], Tot (&tot)[RPW])
+                                   int64_t ib, int64_t j, int pj, int pj_end, const bool (&use)[RPW],
+                                   EvAcc (&acc)[RPW], Tot (&tot)[RPW])
 {
   static_assert(RPW == 4, "column events read the wave's rows as one 32-bit word");
   const size_t np = (size_t)A.npad;
-  const uint8_t *ctj = A.codesT + j;
+  const uint32_t jl = (uint32_t)j;  // < npad
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
     const int64_t i = ib + a;
     if (i >= A.r1) continue;  // uniform over the wave
+    if (MASKED && !__ballot(use[a])) continue;
     const int e0 = __builtin_amdgcn_readfirstlane(A.ev_off[i]);
     const int e1 = __builtin_amdgcn_readfirstlane(A.ev_off[i + 1]);
     for (int f0 = e0; f0 < e1; f0 += 1024) {
@@ -633,15 +651,20 @@ __device__ inline void tile_events(const RunArgs &A, const uint4 *sTabR, const u
       for (int e = f0; e < f1; e += EVB) {
         uint32_t ev[EVB], cj[EVB];
 #pragma unroll
-        for (int k = 0; k < EVB; k++) ev[k] = e + k < f1 ? te.row(A, e + k) : EV_PAD;
+        for (int k = 0; k < EVB; k++)
+          ev[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)(e + k < f1 ? te.row(A, e + k) : EV_PAD));
 #pragma unroll
-        for (int k = 0; k < EVB; k++) cj[k] = ctj[(size_t)(ev[k] >> 5) * np];
+        for (int k = 0; k < EVB; k++) cj[k] = (A.codesT + (size_t)(ev[k] >> 5) * np)[jl];
 #pragma unroll
-        for (int k = 0; k < EVB; k++) ev_add(acc[a], sTabR[(ev[k] & 31u) * TABW + cj[k]]);
+        for (int k = 0; k < EVB; k++) {
+          const uint4 t = sTabR[(ev[k] & 31u) * TABW + cj[k]];
+          ev_add(acc[a], MASKED ? keep_if(use[a], t) : t);
+        }
       }
       ev_flush(acc[a], tot[a]);
     }
   }
+  if (MASKED && !(use[0] || use[1] || use[2] || use[3])) return;
   const uint8_t *cti = A.codesT + ib;
   for (int f0 = pj; f0 < pj_end; f0 += 1024) {
     const int f1 = min(pj_end, f0 + 1024);
@@ -655,7 +678,10 @@ __device__ inline void tile_events(const RunArgs &A, const uint4 *sTabR, const u
       for (int k = 0; k < EVB; k++) {
         const uint4 *t = sTabC + (ev[k] & 31u) * TABW;
 #pragma unroll
-        for (int a = 0; a < RPW; a++) ev_add(acc[a], t[(r4[k] >> (8 * a)) & 255u]);
+        for (int a = 0; a < RPW; a++) {
+          const uint4 v = t[(r4[k] >> (8 * a)) & 255u];
+          ev_add(acc[a], MASKED ? keep_if(use[a], v) : v);
+        }
       }
     }
 #pragma unroll
@@ -684,13 +710,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   __shared__ uint4 sTabR[(NCODE + 1) * TABW], sTabC[(NCODE + 1) * TABW];
   __shared__ unsigned long long lhist[FAST ? 1 : 3 * 200];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int k = tid; k < (NCODE + 1) * TABW; k += 256) {
-    const int c = k / TABW, x = k % TABW;
-    const bool in = c < NCODE && x < NCODE;
-    const bool xamb = x >= 4 && x < GAPC;
-    sTabR[k] = in ? A.evtab[c * NCODE + x] : make_uint4(0, 0, 0, 0);
-    sTabC[k] = in && !xamb ? A.evtab[x * NCODE + c] : make_uint4(0, 0, 0, 0);
-  }
+  load_event_tables(A.evtab, sTabR, sTabC, tid);
   if constexpr (!FAST)
     for (int k = tid; k < 600; k += 256) lhist[k] = 0;
   const int64_t j = j0 + lane;
@@ -807,7 +827,34 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     }
   }
   __syncthreads();  // the event tables and lists are in LDS (also when no counts ran)
-  if (do_events) tile_events(A, sTabR, sTabC, te, ib, j, pj, pj_end, acc, tot);
+  const bool all_rows[RPW] = {true, true, true, true};
+  if (do_events) tile_events<false>(A, sTabR, sTabC, te, ib, j, pj, pj_end, all_rows, acc, tot);
+  // -g: pairs whose resolved ambiguities reach the fraction are computed again in AVERAGE (refkernel.cpp:742-750):
+  // same clean/clean counts, the events through the AVERAGE table.  Order-dependent ones go to the replay instead.
+  bool avg[RPW] = {false, false, false, false};
+  bool any_avg = false;
+  if (A.avg_fallback && do_events) {
+#pragma unroll
+    for (int a = 0; a < RPW; a++) {
+      const int64_t i = ib + a;
+      const bool valid = i < A.r1 && j >= A.c0 && j < A.c1 && (!A.triangle || j > i);
+      const uint32_t nfs = tot[a].nf[0] + tot[a].nf[1] + tot[a].nf[2] + tot[a].nf[3] + 32u * cnt[a].T;
+      const double tng = (double)nfs * 0.03125;
+      avg[a] = valid && !(acc[a].fl >> 31) && tng > (double)A.min_overlap && tng * A.rf <= (double)tot[a].amb;
+      any_avg |= avg[a];
+    }
+  }
+  if (__syncthreads_or(any_avg)) {
+    load_event_tables(A.evtab + NCODE * NCODE, sTabR, sTabC, tid);
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < RPW; a++)
+      if (avg[a]) {
+        tot[a] = Tot{{0, 0, 0, 0}, 0, 0, 0, 0};
+        acc[a] = EvAcc{0, 0, 0, 0, 0};
+      }
+    tile_events<true>(A, sTabR, sTabC, te, ib, j, pj, pj_end, avg, acc, tot);
+  }
   // totals x16 from the clean/clean counts, plus the events
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
@@ -835,7 +882,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
         const double tng = (double)(o.nf[0] + o.nf[1] + o.nf[2] + o.nf[3]) * 0.03125;
         if (tng <= (double)A.min_overlap)  // refkernel.cpp:732-735
           state = 2;
-        else if (A.avg_fallback && tng * A.rf <= (double)o.amb)  // refkernel.cpp:742-750: AVERAGE, replayed
+        else if (A.avg_fallback && !do_events && tng * A.rf <= (double)o.amb)  // refkernel.cpp:742-750
           slow = true;
         else
           state = 1;
@@ -1700,16 +1747,16 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   if (npairs >= (1ll << 32)) return c->fail(TN93G_E_ARG, "more than 2^32 pairs in one run");
   // tables
   {
-    std::vector<uint4> et(NCODE * NCODE);
+    std::vector<uint4> et(2 * NCODE * NCODE);
     std::vector<ReplayEnt> rt(2 * NCODE * NCODE);
     for (int a = 0; a < NCODE; a++)
       for (int b = 0; b < NCODE; b++) {
         SiteEffect E = site_effect(a, b, P->mode, P->subset_mask);
-        uint32_t q[4];
-        event_entry(E, q);
-        et[(size_t)(a * NCODE + b)] = make_uint4(q[0], q[1], q[2], q[3]);
         for (int t = 0; t < 2; t++) {
           SiteEffect F = t ? site_effect(a, b, TN93G_AVERAGE, 0) : E;
+          uint32_t q[4];
+          event_entry(F, q);
+          et[(size_t)(t * NCODE * NCODE + a * NCODE + b)] = make_uint4(q[0], q[1], q[2], q[3]);
           ReplayEnt &r = rt[(size_t)(t * NCODE * NCODE + a * NCODE + b)];
           r.v = F.v;
           r.mask = F.mask;

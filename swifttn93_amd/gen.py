@@ -12,6 +12,10 @@ every machine and numpy version.  Workloads:
        pair order-dependent); 5 % of sequences carry a leading or trailing gap run of 50-400 columns.
   c1s  a smaller sample of the c1 generator (n sequences) for parity tests; `bdhv` raises the share of B/D/H/V
        among the ambiguity codes so the site-order replay is exercised heavily.
+  c2   20,000 x 29,903 SARS-CoV-2-shaped: a random recursive genealogy (each sequence copies an earlier one and
+       mutates each site with probability 0.001-0.02 %, so depths of about ten edges put sequences 0.01-0.2 % from
+       the root); 0-5 N-masked blocks of 100-2,000 columns per sequence; 0.05 % ambiguity codes (c1 proportions).
+  c2s  a smaller sample of the c2 generator (n sequences) for parity tests.
 """
 import numpy as np
 
@@ -138,6 +142,66 @@ def c1s(n=600, seed=3, bdhv=0.07):
     return c1(seed=seed, n=n, L=1300, clusters=max(1, n * 200 // 6700), bdhv=bdhv)
 
 
+def _poisson(rs, lam):
+    """Knuth's product-of-uniforms sampler (lam up to a few hundred)."""
+    if lam <= 0:
+        return 0
+    if lam > 200:  # normal approximation keeps the loop short for large means
+        u = rs.uniform(2)
+        z = np.sqrt(-2.0 * np.log(max(u[0], 1e-300))) * np.cos(2.0 * np.pi * u[1])
+        return max(0, int(round(lam + z * np.sqrt(lam))))
+    lim, k, p = np.exp(-lam), 0, 1.0
+    while True:
+        p *= rs.uniform(1)[0]
+        if p <= lim:
+            return k
+        k += 1
+
+
+def _mutate_sparse(rs, seq, rate):
+    """Like _mutate for small rates: a Poisson number of substituted sites instead of one draw per site."""
+    L = seq.shape[-1]
+    k = _poisson(rs, L * rate)
+    out = seq.copy()
+    if k:
+        pos = rs.integers(L, k)
+        out[pos] = (out[pos] + rs.integers(3, k).astype(np.uint8) + 1) % 4
+    return out
+
+
+def c2(seed=4, n=20000, L=29903):
+    rs = Stream(seed)
+    codes = np.empty((n, L), dtype=np.uint8)
+    codes[0] = rs.integers(4, L).astype(np.uint8)
+    for i in range(1, n):
+        parent = int(rs.integers(i, 1)[0])
+        codes[i] = _mutate_sparse(rs, codes[parent], 0.00001 + 0.00019 * rs.uniform(1)[0])
+    perm = np.argsort(rs.u64(n), kind="stable")
+    codes = codes[perm]
+    # 0-5 N-masked blocks of 100-2,000 columns
+    for i in range(n):
+        nb = int(rs.integers(6, 1)[0])
+        for _ in range(nb):
+            u = rs.uniform(2)
+            run = 100 + int(u[0] * 1901)
+            p = int(u[1] * (L - run))
+            codes[i, p:p + run] = 15
+    # 0.05 % ambiguity codes in the c1 proportions
+    k = _poisson(rs, n * L * 0.0005)
+    pos = rs.integers(n * L, k)
+    w = list(_C1_WEIGHTS)
+    w += [sum(w) * 0.005 / (1.0 - 0.005) / 4.0] * 4
+    cdf = np.cumsum(np.asarray(w, dtype=np.float64))
+    cdf /= cdf[-1]
+    idx = np.minimum(np.searchsorted(cdf, rs.uniform(k), side="right"), len(_C1_POOL) - 1)
+    codes.reshape(-1)[pos] = np.array(_C1_POOL, dtype=np.uint8)[idx]
+    return codes
+
+
+def c2s(n=300, seed=5, L=29903):
+    return c2(seed=seed, n=n, L=L)
+
+
 def names_for(n, prefix="s"):
     return [f"{prefix}{i}" for i in range(n)]
 
@@ -151,4 +215,4 @@ def write_fasta(path, codes, names=None):
             f.write(b">" + nm.encode() + b"\n" + lut[row].tobytes() + b"\n")
 
 
-WORKLOADS = {"c0": c0, "c1": c1, "c1s": c1s}
+WORKLOADS = {"c0": c0, "c1": c1, "c1s": c1s, "c2": c2, "c2s": c2s}

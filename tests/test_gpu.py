@@ -208,3 +208,31 @@ def test_backends_agree(engine, kind):
     _same(a, b)
     assert (a["link_dist"] == b["link_dist"]).all()
     s.free()
+
+
+C2_CONFIGS = [
+    ("-t", "0.0005", "-a", "resolve", "-g", "0.05", "-l", "25000"),
+    ("-t", "1", "-a", "resolve", "-g", "0.05", "-l", "25000"),
+    ("-t", "1", "-a", "average", "-l", "20000"),
+    ("-t", "0.001", "-a", "skip", "-l", "25000", "-f", "csvn"),
+]
+
+
+def test_long_alignment_c2s_vs_oracle(tmp_path):
+    """SARS-CoV-2-shaped long alignment (300 x 29,903, N-masked blocks of 100-2,000 columns) against the oracle."""
+    codes = gen.c2s(300)
+    fa = tmp_path / "in.fa"
+    gen.write_fasta(fa, codes)
+    cases, want = [], []
+    for ci, cfg in enumerate(C2_CONFIGS):
+        d = tmp_path / f"k{ci}"
+        d.mkdir()
+        argv = ["-q", *cfg, "-o", "out.txt", str(fa)]
+        p = subprocess.run([str(_oracle_cli())] + argv[:-3] + ["-o", str(d / "want.txt"), str(fa)],
+                           capture_output=True, check=True, timeout=600)
+        want.append(((d / "want.txt").read_bytes(), p.stdout))
+        cases.append((d, argv))
+    for (d, argv), (rc, out, so), (wo, ws) in zip(cases, run_batch(cases, tmp_path), want):
+        assert rc == 0, argv
+        assert out == wo, (argv, len(out or b""), len(wo))
+        assert so == ws, (argv, so[:400], ws[:400])
