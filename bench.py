@@ -6,7 +6,11 @@ columns (swifttn93_amd/gen.py c1), options -t 0.015 -a resolve -g 0.05 -l 500, 2
 resident in HBM before the timed region; a step is one tn93g_run: clean/clean counts, ambiguity events,
 site-order replays, FP64 tail, histogram, per-row sums in tn93's order and the link list copied back to the host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--backend popc|mfma] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--backend popc|mfma] [--no-cpu] [--config c1|c2|c3]
+
+Other workloads (--config): c2, the SARS-CoV-2-shaped long alignment (20,000 x 29,903, N-masked blocks,
+-t 0.0005 -a resolve -g 0.05 -l 25000); c3, 64 uneven alignments with mixed -a modes run as one batch (one graph
+launch) beside the same jobs run one by one, and the 2,000 x 40,000 -s bipartite block.
 
 Prints one JSON line.  `roofline` is for the pair kernel (HIP events on the engine's stream, averaged over the
 timed steps); `cpu_baseline` times the reference solver (oracle/_ref/swifttn93_avx512, built from the reference
@@ -68,6 +72,160 @@ def cpu_baseline(fa, n_pairs):
                       f"best of 3, OMP_NUM_THREADS={threads}", "wall_s": best}
 
 
+def run_reference(exe, args_list, threads):
+    """Wall time of the reference solver over a list of (argv, fasta) runs, output deleted outside the timer."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    tot = 0.0
+    with tempfile.TemporaryDirectory() as td:
+        out = Path(td) / "out.csv"
+        for argv in args_list:
+            t0 = time.perf_counter()
+            subprocess.run([str(exe), "-q", "-o", str(out)] + argv, check=True, env=env, stdout=subprocess.DEVNULL,
+                           stderr=subprocess.DEVNULL, timeout=1200)
+            tot += time.perf_counter() - t0
+            out.unlink()
+    return tot
+
+
+def ref_exe():
+    ref = REPO / "oracle" / "_ref" / "swifttn93_avx512"
+    if ref.exists():
+        return ref, "reference"
+    port = REPO / "oracle" / "_build" / "tn93_oracle"
+    return (port, "port") if port.exists() else (None, None)
+
+
+def roofline_for(n_pairs, L, pair_ms):
+    words = (L + 31) // 32
+    ops = 22 * words
+    achieved = ops * n_pairs / (pair_ms * 1e-3) / 1e12
+    return {"bound": "valu", "achieved": achieved, "peak": VALU_INT32_TOPS, "unit": "TOPS",
+            "frac": achieved / VALU_INT32_TOPS, "traffic": None, "kernel": "k_pairs<1, true>", "kernel_ms": pair_ms,
+            "ops_per_pair": ops}
+
+
+def bench_c2(a, T, gen):
+    import numpy as np
+    codes = gen.c2()
+    n, L = codes.shape
+    n_pairs = n * (n - 1) // 2
+    eng = T.Engine(0)
+    s = eng.upload(codes)
+    kw = dict(mode=T.RESOLVE, threshold=0.0005, min_overlap=25000, resolve_fraction=0.05, backend=T.BACKEND_POPC)
+    for _ in range(a.warmup):
+        r = eng.run(s, **kw)
+    t0 = time.perf_counter()
+    pk = []
+    for _ in range(a.steps):
+        r = eng.run(s, **kw)
+        pk.append(r["pair_kernel_ms"])
+    dt = time.perf_counter() - t0
+    value = n_pairs * a.steps / dt
+    line = {"metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": dt * 1e3 / a.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int32+f64", "data": "synthetic (seeded SARS-CoV-2-shaped generator, swifttn93_amd/gen.py c2)",
+            "config": {"workload": "SARS-CoV-2-shaped long alignment, 20,000 x 29,903, N-masked blocks, "
+                                   "-t 0.0005 -a resolve -g 0.05 -l 25000", "pairs": n_pairs, "parallelism": "1 GPU"},
+            "roofline": roofline_for(n_pairs, L, float(np.mean(pk))),
+            "links": int(r["found_links"]), "replayed_pairs": int(r["n_replayed"])}
+    s.free()
+    eng.close()
+    if not a.no_cpu:
+        exe, kind = ref_exe()
+        if exe:
+            threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+            with tempfile.TemporaryDirectory() as td:
+                fa = Path(td) / "c2.fa"
+                gen.write_fasta(fa, codes)
+                w = run_reference(exe, [["-t", "0.0005", "-a", "resolve", "-g", "0.05", "-l", "25000", str(fa)]],
+                                  threads)
+            line["cpu_baseline"] = {"value": n_pairs / w, "unit": "pairs/s", "cores": threads, "kind": kind,
+                                    "sample": "whole workload, one full command run (read + compute + write)",
+                                    "wall_s": w}
+            line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line))
+
+
+def bench_c3(a, T, gen):
+    import numpy as np
+    jobs_codes = gen.c3()
+    eng = T.Engine(0)
+    modes = {"resolve": T.RESOLVE, "average": T.AVERAGE, "skip": T.SKIP, "gapmm": T.GAPMM}
+    sets, jobs = [], []
+    for codes, mode in jobs_codes:
+        st = eng.upload(codes)
+        sets.append(st)
+        jobs.append((st, dict(mode=modes[mode], threshold=0.015, min_overlap=min(500, codes.shape[1] // 2),
+                              resolve_fraction=0.05 if mode == "resolve" else 1.0, backend=T.BACKEND_AUTO,
+                              flags=T.FLAG_AUTOTUNE)))
+    n_pairs = sum(c.shape[0] * (c.shape[0] - 1) // 2 for c, _ in jobs_codes)
+    for _ in range(max(a.warmup, 1)):
+        eng.run_batch(jobs)
+    # plain path: the same jobs one tn93g_run each
+    t0 = time.perf_counter()
+    pk = 0.0
+    for _ in range(a.steps):
+        for st, kw in jobs:
+            pk += eng.run(st, **kw)["pair_kernel_ms"]
+    t_plain = (time.perf_counter() - t0) / a.steps
+    # batched: one tn93g_run_batch (one graph launch) per step
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        res = eng.run_batch(jobs)
+    t_batch = (time.perf_counter() - t0) / a.steps
+    value = n_pairs / t_batch
+    # the -s bipartite block
+    bc = gen.c3_bipartite()
+    bs = eng.upload(bc)
+    bkw = dict(mode=T.RESOLVE, kind=T.CROSS, n1=2000, threshold=0.015, min_overlap=500, resolve_fraction=0.05,
+               backend=T.BACKEND_AUTO, flags=T.FLAG_AUTOTUNE)
+    eng.run(bs, **bkw)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        rb = eng.run(bs, **bkw)
+    t_bip = (time.perf_counter() - t0) / a.steps
+    bip_pairs = 2000 * 40000
+    Lw = sum(c.shape[0] * (c.shape[0] - 1) // 2 * ((c.shape[1] + 31) // 32) for c, _ in jobs_codes)
+    achieved = 22 * Lw / (pk / a.steps * 1e-3) / 1e12
+    line = {"metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+            "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int32+f64", "data": "synthetic (seeded generators, swifttn93_amd/gen.py c3, c3_bipartite)",
+            "config": {"workload": "64 alignments, 50-3,000 sequences x 300-10,000 columns, -a resolve/average/skip/"
+                                   "gapmm cycling, -t 0.015, one batch (one graph launch)", "pairs": n_pairs,
+                       "parallelism": "1 GPU"},
+            "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_INT32_TOPS, "unit": "TOPS",
+                         "frac": achieved / VALU_INT32_TOPS, "traffic": None,
+                         "kernel": "k_pairs (summed over the 64 jobs, run one by one)", "kernel_ms": pk / a.steps},
+            "plain_pairs_per_s": n_pairs / t_plain, "plain_ms_per_step": t_plain * 1e3,
+            "batch_over_plain": t_plain / t_batch,
+            "bipartite": {"pairs": bip_pairs, "pairs_per_s": bip_pairs / t_bip, "ms": t_bip * 1e3,
+                          "links": int(rb["found_links"])}}
+    for st in sets:
+        st.free()
+    bs.free()
+    eng.close()
+    if not a.no_cpu:
+        exe, kind = ref_exe()
+        if exe:
+            threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+            with tempfile.TemporaryDirectory() as td:
+                argvs = []
+                for k, (codes, mode) in enumerate(jobs_codes):
+                    fa = Path(td) / f"j{k}.fa"
+                    gen.write_fasta(fa, codes)
+                    argvs.append(["-t", "0.015", "-a", mode, "-l", str(min(500, codes.shape[1] // 2))]
+                                 + (["-g", "0.05"] if mode == "resolve" else []) + [str(fa)])
+                w = run_reference(exe, argvs, threads)
+            line["cpu_baseline"] = {"value": n_pairs / w, "unit": "pairs/s", "cores": threads, "kind": kind,
+                                    "sample": "the 64 alignments, one full command run each (read + compute + write)",
+                                    "wall_s": w}
+            line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    print(json.dumps(line))
+
+
+METRIC = "byte-exact pairs/sec at 1 GPU vs host cores; roofline %; serving p50/p99"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +235,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (torch), the reference for the HBM byte counters")
+    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c3"])
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported (the multi-GPU sharding is "
@@ -85,6 +244,10 @@ def main():
     import numpy as np
     import swifttn93_amd as T
     from swifttn93_amd import gen
+    if a.config == "c2":
+        return bench_c2(a, T, gen)
+    if a.config == "c3":
+        return bench_c3(a, T, gen)
 
     if a.copy_calibration:
         import torch
@@ -126,7 +289,7 @@ def main():
                 "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/c1_popc_hbm_bytes.md",
                 "hbm_frac": None if traffic is None else traffic / (pair_ms * 1e-3) / HBM_BYTES_S}
     line = {
-        "metric": "byte-exact pairs/sec at 1 GPU vs host cores; roofline %; serving p50/p99",
+        "metric": METRIC,
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32+f64",
         "data": "synthetic (seeded HIV-pol-shaped generator, swifttn93_amd/gen.py)",

@@ -52,6 +52,7 @@ extern "C" {
 /* run flags */
 #define TN93G_FLAG_EXACT 1u      /* resolve every uncertain log on the host (libm) instead of bracketing it   */
 #define TN93G_FLAG_ALL_SLOW 2u   /* route every pair through the site-order replay kernel (tests)            */
+#define TN93G_FLAG_AUTOTUNE 4u   /* with TN93G_BACKEND_AUTO: time the backends on the first job of each shape */
 
 typedef struct tn93g_ctx tn93g_ctx;
 typedef struct tn93g_set tn93g_set;
@@ -120,6 +121,16 @@ typedef struct {
  * and released with tn93g_result_free. */
 int tn93g_run(tn93g_ctx *ctx, const tn93g_set *set, const tn93g_params *params, tn93g_result **out);
 void tn93g_result_free(tn93g_result *res);
+
+/* Runs n independent jobs (sets[k] with params[k]; sets may repeat) as one batch: every job's kernels are enqueued
+ * back to back on the context's stream, captured into one HIP graph and launched once (a later batch that enqueues
+ * exactly the same work launches the same graph again), then all results are read back together.  out[k] is the
+ * result tn93g_run(ctx, sets[k], &params[k], &out[k]) would give; jobs with TN93G_FLAG_EXACT or TN93G_FST, and jobs
+ * whose outputs cannot be certified from the batch, are run alone after it.  gpu_ms of a batched result is the
+ * device time of the whole batch.  Replaces the per-alignment loop a caller of the reference would run
+ * (one swifttn93 process per alignment, swifttn93.cpp:2222). */
+int tn93g_run_batch(tn93g_ctx *ctx, int n, const tn93g_set *const *sets, const tn93g_params *params,
+                    tn93g_result **out);
 
 /* Library version string. */
 const char *tn93g_version(void);

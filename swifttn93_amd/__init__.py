@@ -16,7 +16,7 @@ CLI_PATH = PKG / "bin" / "tn93g"
 RESOLVE, AVERAGE, SKIP, GAPMM, SUBSET = 0, 1, 2, 3, 4
 TRIANGLE, CROSS, FST = 0, 1, 2
 BACKEND_AUTO, BACKEND_POPC, BACKEND_MFMA = 0, 1, 2
-FLAG_EXACT, FLAG_ALL_SLOW = 1, 2
+FLAG_EXACT, FLAG_ALL_SLOW, FLAG_AUTOTUNE = 1, 2, 4
 
 MODES = {"resolve": RESOLVE, "average": AVERAGE, "skip": SKIP, "gapmm": GAPMM}
 
@@ -41,7 +41,7 @@ class Result(ctypes.Structure):
 
 
 EXPORTS = ["tn93g_open", "tn93g_close", "tn93g_error", "tn93g_set_upload", "tn93g_set_free", "tn93g_run",
-           "tn93g_result_free", "tn93g_version"]
+           "tn93g_result_free", "tn93g_version", "tn93g_run_batch"]
 
 _lib = None
 
@@ -64,9 +64,16 @@ def load():
     lib.tn93g_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Params),
                               ctypes.POINTER(ctypes.POINTER(Result))]
     lib.tn93g_result_free.argtypes = [ctypes.POINTER(Result)]
+    lib.tn93g_run_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                    ctypes.POINTER(Params), ctypes.POINTER(ctypes.POINTER(Result))]
     lib.tn93g_version.restype = ctypes.c_char_p
     _lib = lib
     return lib
+
+
+def make_params(mode=RESOLVE, kind=TRIANGLE, n1=0, subset_mask=0, backend=BACKEND_AUTO, resolve_fraction=1.0,
+                min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0):
+    return Params(mode, kind, n1, subset_mask, backend, resolve_fraction, min_overlap, threshold, min_distance, flags)
 
 
 class Engine:
@@ -94,13 +101,28 @@ class Engine:
 
     def run(self, seqset, mode=RESOLVE, kind=TRIANGLE, n1=0, subset_mask=0, backend=BACKEND_AUTO,
             resolve_fraction=1.0, min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0):
-        p = Params(mode, kind, n1, subset_mask, backend, resolve_fraction, min_overlap, threshold, min_distance, flags)
+        p = make_params(mode, kind, n1, subset_mask, backend, resolve_fraction, min_overlap, threshold, min_distance,
+                        flags)
         r = ctypes.POINTER(Result)()
         self._check(self.lib.tn93g_run(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r)))
         try:
             return _to_py(r.contents)
         finally:
             self.lib.tn93g_result_free(r)
+
+    def run_batch(self, jobs):
+        """jobs: list of (SeqSet, dict of run() keyword arguments).  One tn93g_run_batch call; a list of results."""
+        n = len(jobs)
+        sets = (ctypes.c_void_p * max(n, 1))(*[s.handle for s, _ in jobs])
+        params = (Params * max(n, 1))(*[make_params(**kw) for _, kw in jobs])
+        outs = (ctypes.POINTER(Result) * max(n, 1))()
+        self._check(self.lib.tn93g_run_batch(self.ctx, n, sets, params, outs))
+        try:
+            return [_to_py(outs[k].contents) for k in range(n)]
+        finally:
+            for k in range(n):
+                if outs[k]:
+                    self.lib.tn93g_result_free(outs[k])
 
     def close(self):
         if self.ctx:

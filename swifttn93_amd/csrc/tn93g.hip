@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -1244,6 +1245,11 @@ __global__ void k_link_gather(RunArgs A, LinkRec *__restrict__ out)
 }
 
 // histogram replicas summed into replica 0
+__global__ void k_fill(uint32_t *__restrict__ p, uint32_t v, int64_t n)
+{
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) p[k] = v;
+}
+
 __global__ void k_hist_sum(RunArgs A)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1464,16 +1470,43 @@ struct HostBuf {  // pinned host memory (hipHostMalloc), grown on demand
   size_t n = 0;
 };
 
+// The device buffers of one run.  A single run uses slot 0; a batch gives each of its jobs a slot of its own, so all
+// of a batch's kernels can be enqueued before any result is read back.
+struct RunBufs {
+  DevBuf D, U, hist, row_links, link_off, slow_list, slow_cnt, slow_off, slow_chunk, counters, exact_list, rowout,
+      links, links_flat, evtab, rtab, patch, chdr, cdata;
+  int64_t link_cap_hint = 0;  // link region size that sufficed last time
+  int tab_mode = -1;           // mode and subset of the tables in evtab/rtab (uploaded again only when they change)
+  uint32_t tab_mask = 0;
+  void release()
+  {
+    for (DevBuf *b : {&D, &U, &hist, &row_links, &link_off, &slow_list, &slow_cnt, &slow_off, &slow_chunk, &counters,
+                      &exact_list, &rowout, &links, &links_flat, &evtab, &rtab, &patch, &chdr, &cdata}) {
+      if (b->p) (void)hipFree(b->p);
+      b->p = nullptr;
+      b->n = 0;
+    }
+  }
+};
+
 struct tn93g_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[4];
   std::string err;
-  DevBuf D, U, hist, row_links, link_off, slow_list, slow_cnt, slow_off, slow_chunk, counters, exact_list, rowout,
-      links, links_flat, evtab, rtab, patch, chdr, cdata;
-  HostBuf pin;                 // staging of the run's read-backs (counters, links, row accumulators, histogram)
-  int64_t link_cap_hint = 0;  // link buffer size that sufficed last time
+  std::vector<RunBufs *> slots;
+  HostBuf pin;  // staging of the read-backs (counters, links, row accumulators, histogram)
+  // the instantiated graph of the last batch, launched again when a batch enqueues exactly the same work
+  hipGraphExec_t gexec = nullptr;
+  std::vector<unsigned char> gkey;
+  // autotuner: the fastest pair-kernel variant measured for each job shape (see tune_shape)
+  std::map<uint64_t, int> tuned;
 
+  RunBufs *slot(size_t k)
+  {
+    while (slots.size() <= k) slots.push_back(new RunBufs());
+    return slots[k];
+  }
   int fail(int code, const std::string &m)
   {
     err = m;
@@ -1513,6 +1546,519 @@ static int ensure(tn93g_ctx *c, DevBuf &b, size_t bytes)
   return 0;
 }
 
+// tn93's tail with the host libm (refkernel.cpp:752-815), for the host-exact path
+static double host_tail(const tn93g_tailin &in)
+{
+  const double *nucFreq = in.nf;
+  double auxd, fY, fR, K1 = 0, K2 = 0, K3 = 0, AG, CT, ti, tv, tng2, nucF[4];
+  bool useK2P = false;
+  tng2 = 2. / (nucFreq[0] + nucFreq[1] + nucFreq[2] + nucFreq[3]);
+  auxd = 1. / (nucFreq[0] + nucFreq[1] + nucFreq[2] + nucFreq[3]);
+  for (int k = 0; k < 4; k++) nucF[k] = nucFreq[k] * auxd;
+  fR = nucF[0] + nucF[2];
+  fY = nucF[1] + nucF[3];
+  if (nucFreq[0] == 0 || nucFreq[1] == 0 || nucFreq[2] == 0 || nucFreq[3] == 0)
+    useK2P = true;
+  else {
+    K1 = 2. * nucF[0] * nucF[2] / fR;
+    K2 = 2. * nucF[1] * nucF[3] / fY;
+    K3 = 2. * (fR * fY - nucF[0] * nucF[2] * fY / fR - nucF[1] * nucF[3] * fR / fY);
+  }
+  AG = in.ag * tng2;
+  CT = in.ct * tng2;
+  tv = 1. - (in.diag * tng2 + AG + CT);
+  if (useK2P) {
+    ti = AG + CT;
+    AG = 1. - 2. * ti - tv;
+    CT = 1. - 2. * tv;
+    return (AG > 0. && CT > 0.) ? -0.5 * log(AG) - 0.25 * log(CT) : 1000.0;
+  }
+  AG = 1. - AG / K1 - 0.5 * tv / fR;
+  CT = 1. - CT / K2 - 0.5 * tv / fY;
+  tv = 1. - 0.5 * tv / fY / fR;
+  return (AG > 0. && CT > 0. && tv > 0) ? -K1 * log(AG) - K2 * log(CT) - K3 * log(tv) : 1000.0;
+}
+
+static int host_hist_index(double d)
+{
+  double f = floor(d * 200.0);
+  if (f >= 0.0 && f < 200.0) return (int)f;
+  return 199;
+}
+
+// Zero fill of a device range (bytes a multiple of 4) as a kernel, so that runs can be captured in a graph.
+static void fill_zero(hipStream_t s, void *p, size_t bytes)
+{
+  const int64_t n = (int64_t)(bytes / 4);
+  if (n <= 0) return;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, s, (uint32_t *)p, 0u, n);
+}
+
+constexpr size_t STRIPE_BYTES = (size_t)NSTRIPE * 32 * 4;
+constexpr size_t COUNTER_STRIDE = STRIPE_BYTES + 256;  // pinned bytes per job: the stripes, then the replay total
+
+// One run: a set, its parameters, its buffers and what the host learns about it along the way.
+struct Job {
+  const tn93g_set *S = nullptr;
+  tn93g_params P{};
+  RunBufs *B = nullptr;
+  RunArgs A{};
+  bool fast = false;
+  int64_t nrows = 0, ntiles = 0, gx = 0, gy = 0, npairs = 0;
+  size_t nr1 = 0;
+  unsigned link_cap = 1, exact_cap = 1;
+  uint64_t n_slow = 0, n_unc = 0, n_undecided = 0, n_linkrec = 0;
+  unsigned max_region = 0, n_exact = 0;
+};
+
+// Checks the parameters, uploads the job's event tables and sizes its buffers: everything before its kernels.
+static int job_prepare(tn93g_ctx *c, Job &J)
+{
+  const tn93g_set *S = J.S;
+  const tn93g_params *P = &J.P;
+  const SetDev &d = S->d;
+  RunBufs *B = J.B;
+  if (P->mode < 0 || P->mode > 4 || P->kind < 0 || P->kind > 2) return c->fail(TN93G_E_ARG, "bad mode or kind");
+  if (P->kind != TN93G_TRIANGLE && (P->n1 < 0 || P->n1 > d.n)) return c->fail(TN93G_E_ARG, "bad n1");
+  RunArgs &A = J.A;
+  memset(&A, 0, sizeof A);
+  A.planes = d.planes;
+  A.ambp = d.ambp;
+  A.onehot = d.onehot;
+  A.codes = d.codes;
+  A.codesT = d.codesT;
+  A.ev_off = d.ev_off;
+  A.ev = d.ev;
+  A.weights = d.weights;
+  A.n = d.n;
+  A.Ls = d.Ls;
+  A.npad = d.npad;
+  A.W = d.W;
+  A.W64 = d.W64;
+  if (P->kind == TN93G_CROSS) {
+    A.r0 = 0;
+    A.r1 = P->n1;
+    A.c0 = P->n1;
+    A.c1 = d.n;
+    A.triangle = 0;
+  } else {
+    A.r0 = 0;
+    A.r1 = d.n;
+    A.c0 = 0;
+    A.c1 = d.n;
+    A.triangle = 1;
+  }
+  A.fst = P->kind == TN93G_FST;
+  A.n1 = P->kind == TN93G_TRIANGLE ? d.n : P->n1;
+  A.ncols = A.c1 - A.c0;
+  const int64_t nrows = A.r1 - A.r0;
+  J.nrows = nrows;
+  A.gapmm = P->mode == TN93G_GAPMM;
+  A.skip_events = P->mode == TN93G_SKIP;
+  A.avg_fallback = (P->mode == TN93G_RESOLVE || P->mode == TN93G_SUBSET) && P->resolve_fraction < 1.;
+  A.exact = (P->flags & TN93G_FLAG_EXACT) ? 1 : 0;
+  A.all_slow = (P->flags & TN93G_FLAG_ALL_SLOW) ? 1 : 0;
+  A.rf = P->resolve_fraction;
+  A.thr = P->threshold;
+  A.mind = P->min_distance;
+  A.window = TN93G_LOG_WINDOW;
+  A.min_overlap = P->min_overlap;
+  const int64_t npairs_box = nrows * A.ncols;
+  J.npairs = A.triangle ? d.n * (d.n - 1) / 2 : npairs_box;
+  if (J.npairs >= (1ll << 32)) return c->fail(TN93G_E_ARG, "more than 2^32 pairs in one run");
+  // tables: [2][18*18] packed event entries (main mode, AVERAGE), [2][18*18] replay entries
+  if (B->tab_mode != P->mode || B->tab_mask != P->subset_mask || !B->evtab.p || !B->rtab.p) {
+    std::vector<uint4> et(2 * NCODE * NCODE);
+    std::vector<ReplayEnt> rt(2 * NCODE * NCODE);
+    for (int a = 0; a < NCODE; a++)
+      for (int b = 0; b < NCODE; b++) {
+        SiteEffect E = site_effect(a, b, P->mode, P->subset_mask);
+        for (int t = 0; t < 2; t++) {
+          SiteEffect F = t ? site_effect(a, b, TN93G_AVERAGE, 0) : E;
+          uint32_t q[4];
+          event_entry(F, q);
+          et[(size_t)(t * NCODE * NCODE + a * NCODE + b)] = make_uint4(q[0], q[1], q[2], q[3]);
+          ReplayEnt &r = rt[(size_t)(t * NCODE * NCODE + a * NCODE + b)];
+          r.v = F.v;
+          r.mask = F.mask;
+          r.icd = (int16_t)F.icd;
+          r.amb = (uint16_t)F.amb;
+        }
+      }
+    if (ensure(c, B->evtab, et.size() * sizeof(uint4)) || ensure(c, B->rtab, rt.size() * sizeof(ReplayEnt)))
+      return TN93G_E_NOMEM;
+    HIPCHK(c, hipMemcpyAsync(B->evtab.p, et.data(), et.size() * sizeof(uint4), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(B->rtab.p, rt.data(), rt.size() * sizeof(ReplayEnt), hipMemcpyHostToDevice, c->stream));
+    B->tab_mode = P->mode;
+    B->tab_mask = P->subset_mask;
+  }
+  A.evtab = (const uint4 *)B->evtab.p;
+  A.rtab = (const ReplayEnt *)B->rtab.p;
+  // buffers
+  J.fast = !A.exact && P->kind != TN93G_FST;  // F_ST combines the means non-monotonically: exact path
+  A.fast = J.fast ? 1 : 0;
+  const size_t nbox = (size_t)std::max<int64_t>(npairs_box, 1);
+  J.nr1 = (size_t)std::max<int64_t>(nrows, 1);
+  const size_t nr1 = J.nr1;
+  const int64_t c0a = A.c0 & ~(int64_t)63;
+  J.gx = std::max<int64_t>((A.c1 - c0a + TJ - 1) / TJ, 1);
+  J.gy = std::max<int64_t>((nrows + TI - 1) / TI, 1);
+  J.ntiles = J.gx * J.gy;
+  const int64_t ntiles = J.ntiles, nslots = J.fast ? (int64_t)nr1 * J.gx : 1;
+  if (J.gx >= (1ll << 31) || J.gy >= 65536) return c->fail(TN93G_E_ARG, "too many pair tiles in one run");
+  if ((!J.fast && (ensure(c, B->D, nbox * 8) || ensure(c, B->U, nbox * 2))) ||
+      ensure(c, B->hist, (size_t)NSTRIPE * 600 * 8) || ensure(c, B->row_links, nr1 * 4) ||
+      ensure(c, B->link_off, (size_t)(nrows + 1) * 8) ||
+      ensure(c, B->slow_list, (size_t)ntiles * TI * TJ * 8) || ensure(c, B->slow_cnt, (size_t)ntiles * 4) ||
+      ensure(c, B->slow_off, (size_t)(ntiles + 1) * 8) || ensure(c, B->slow_chunk, (size_t)(16 * ntiles + 1) * 4) ||
+      ensure(c, B->counters, STRIPE_BYTES) ||
+      ensure(c, B->rowout, nr1 * std::max<size_t>(15 * 8 + 16, sizeof(RowAcc))) ||
+      ensure(c, B->chdr, (size_t)nslots * sizeof(ChunkHdr)) ||
+      ensure(c, B->cdata, (size_t)nslots * 7 * 64 * sizeof(uint32_t)))
+    return TN93G_E_NOMEM;
+  J.exact_cap = A.exact ? (unsigned)std::min<int64_t>(J.npairs, 1ll << 31) : 1u;
+  if (ensure(c, B->exact_list, (size_t)J.exact_cap * sizeof(ExactRec))) return TN93G_E_NOMEM;
+  J.link_cap = 1;
+  if (J.fast) {
+    // links per stripe region; a run that overflows a region is repeated with larger regions
+    const int64_t want = std::max<int64_t>(B->link_cap_hint, 1 << 15);
+    J.link_cap = (unsigned)std::min<int64_t>(std::max<int64_t>(J.npairs, 1), want);
+    if (ensure(c, B->links, (size_t)J.link_cap * NSTRIPE * sizeof(LinkRec)) ||
+        ensure(c, B->links_flat, (size_t)J.link_cap * NSTRIPE * sizeof(LinkRec)))
+      return TN93G_E_NOMEM;
+  }
+  A.D = (double *)B->D.p;
+  A.U = (uint16_t *)B->U.p;
+  A.hist = (unsigned long long *)B->hist.p;
+  A.row_links = (int32_t *)B->row_links.p;
+  A.slow_list = (unsigned long long *)B->slow_list.p;
+  A.slow_cnt = (unsigned *)B->slow_cnt.p;
+  A.slow_off = (const int64_t *)B->slow_off.p;
+  A.slow_chunk = (const int32_t *)B->slow_chunk.p;
+  A.ntiles = ntiles;
+  A.stripes = (unsigned *)B->counters.p;
+  A.exact_cnt = A.stripes + ST_EXACT;
+  A.exact_list = (ExactRec *)B->exact_list.p;
+  A.exact_cap = J.exact_cap;
+  A.racc = (RowAcc *)B->rowout.p;
+  A.links = (LinkRec *)B->links.p;
+  A.link_cap = J.link_cap;
+  A.chdr = (ChunkHdr *)B->chdr.p;
+  A.cdata = (uint32_t *)B->cdata.p;
+  A.gx = J.gx;
+  return 0;
+}
+
+// The job's kernels, in stream order, with no host synchronisation (a batch captures them into one graph).  With
+// `timing`, events around the pair and tail kernels give the device times of a single run.
+static void job_enqueue(tn93g_ctx *c, Job &J, bool timing)
+{
+  RunArgs &A = J.A;
+  RunBufs *B = J.B;
+  const int64_t nrows = J.nrows;
+  fill_zero(c->stream, B->hist.p, (size_t)NSTRIPE * 600 * 8);
+  fill_zero(c->stream, B->counters.p, STRIPE_BYTES);
+  fill_zero(c->stream, B->row_links.p, J.nr1 * 4);
+  if (J.fast) fill_zero(c->stream, B->rowout.p, J.nr1 * sizeof(RowAcc));
+  if (timing) (void)hipEventRecord(c->ev[0], c->stream);
+  if (nrows > 0 && A.ncols > 0) {
+    dim3 g((unsigned)J.gx, (unsigned)J.gy);
+    const bool mf = J.P.backend == TN93G_BACKEND_MFMA;
+    if (mf && J.fast)
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, true>), g, dim3(256), 0, c->stream, A);
+    else if (mf)
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, false>), g, dim3(256), 0, c->stream, A);
+    else if (J.fast)
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, true>), g, dim3(256), 0, c->stream, A);
+    else
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, false>), g, dim3(256), 0, c->stream, A);
+  } else {
+    fill_zero(c->stream, B->slow_cnt.p, (size_t)J.ntiles * 4);
+  }
+  if (timing) (void)hipEventRecord(c->ev[1], c->stream);
+  if (J.fast && nrows > 0 && A.ncols > 0)
+    hipLaunchKernelGGL(k_tail, dim3((unsigned)std::min<int64_t>(nrows, 65535)), dim3(256), 0, c->stream, A);
+  if (timing) (void)hipEventRecord(c->ev[3], c->stream);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)B->slow_cnt.p,
+                     (int64_t *)B->slow_off.p, J.ntiles, (int32_t *)B->slow_chunk.p);
+  hipLaunchKernelGGL(k_replay, dim3(2048), dim3(RB), 0, c->stream, A);
+  if (J.fast) hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, c->stream, A, (LinkRec *)B->links_flat.p);
+  hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, c->stream, A);
+}
+
+// Device-to-host copies of the job's counters and replay total into `pin` (COUNTER_STRIDE bytes).
+static int job_enqueue_counters(tn93g_ctx *c, Job &J, char *pin)
+{
+  HIPCHK(c, hipMemcpyAsync(pin, J.B->counters.p, STRIPE_BYTES, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(pin + STRIPE_BYTES, (const int64_t *)J.B->slow_off.p + J.ntiles, 8, hipMemcpyDeviceToHost,
+                           c->stream));
+  return 0;
+}
+
+static void job_parse_counters(Job &J, const char *pin)
+{
+  std::vector<unsigned> st((size_t)NSTRIPE * 32);
+  memcpy(st.data(), pin, STRIPE_BYTES);
+  int64_t slow_total = 0;
+  memcpy(&slow_total, pin + STRIPE_BYTES, 8);
+  J.n_slow = (uint64_t)slow_total;
+  J.n_unc = J.n_undecided = J.n_linkrec = 0;
+  J.max_region = 0;
+  for (int s = 0; s < NSTRIPE; s++) {
+    J.n_unc += st[(size_t)s * 32 + ST_UNCERTAIN];
+    J.n_undecided += st[(size_t)s * 32 + ST_UNDECIDED];
+    J.n_linkrec += st[(size_t)s * 32 + ST_LINKS];
+    J.max_region = std::max(J.max_region, st[(size_t)s * 32 + ST_LINKS]);
+  }
+  J.n_exact = st[ST_EXACT];
+}
+
+// Read-back layout of a fast-path job in pinned memory: link records, row accumulators, histogram.
+struct FastLayout {
+  size_t off_ra, off_h, bytes;
+};
+static FastLayout fast_layout(const Job &J)
+{
+  FastLayout F;
+  F.off_ra = (J.n_linkrec * sizeof(LinkRec) + 255) & ~(size_t)255;
+  F.off_h = F.off_ra + ((J.nr1 * sizeof(RowAcc) + 255) & ~(size_t)255);
+  F.bytes = F.off_h + ((600 * 8 + 255) & ~(size_t)255);
+  return F;
+}
+
+static int job_enqueue_fast_readback(tn93g_ctx *c, Job &J, char *pin)
+{
+  const FastLayout F = fast_layout(J);
+  if (J.n_linkrec)
+    HIPCHK(c, hipMemcpyAsync(pin, J.B->links_flat.p, J.n_linkrec * sizeof(LinkRec), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(pin + F.off_ra, J.B->rowout.p, J.nr1 * sizeof(RowAcc), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(pin + F.off_h, J.B->hist.p, 600 * 8, hipMemcpyDeviceToHost, c->stream));
+  return 0;
+}
+
+// What every result gets at the end: histogram replicas, self_v_self entries, diagnostics.
+static void job_finish_common(const Job &J, const unsigned long long *hrep, tn93g_result *R)
+{
+  for (int k = 0; k < 600; k++) (&R->hist[0][0])[k] += hrep[k];  // replicas summed by k_hist_sum
+  R->exact = (!J.fast && J.n_unc == 0) ? 1 : 0;
+  R->nrows = (int32_t)J.nrows;
+  R->n_replayed = (int64_t)J.n_slow;
+  R->n_uncertain = (int64_t)J.n_unc;
+  R->n_host_exact = J.n_exact;
+  // self_v_self histogram entries (swifttn93.cpp:2406-2415)
+  if (J.A.triangle)
+    for (int64_t i = 0; i < J.S->d.n; i++) {
+      const int64_t wi = J.S->hw[(size_t)i];
+      if (wi > 1) R->hist[(J.A.fst && i >= J.A.n1) ? 1 : 0][0] += (unsigned long long)((wi * (wi - 1)) >> 1);
+    }
+}
+
+static int alloc_links(tn93g_result *R, size_t nl)
+{
+  R->n_links = (int64_t)nl;
+  R->link_row = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
+  R->link_col = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
+  R->link_dist = (double *)malloc(std::max<size_t>(nl, 1) * 8);
+  R->link_dist_lo = (double *)malloc(std::max<size_t>(nl, 1) * 8);
+  R->link_dist_hi = (double *)malloc(std::max<size_t>(nl, 1) * 8);
+  return (R->link_row && R->link_col && R->link_dist && R->link_dist_lo && R->link_dist_hi) ? 0 : TN93G_E_NOMEM;
+}
+
+// The fast path's result from its read-back: links in tn93's order, the enclosures of the mean and the maximum.
+static int job_fast_result(const Job &J, const char *pin, tn93g_result *R)
+{
+  const FastLayout F = fast_layout(J);
+  const LinkRec *lk = (const LinkRec *)pin;
+  const RowAcc *ra = (const RowAcc *)(pin + F.off_ra);
+  const size_t nl = J.n_linkrec;
+  const int64_t nrows = J.nrows, r0 = J.A.r0;
+  // tn93's emission order (rows ascending, columns ascending): counting sort by row, then each row's few links by
+  // column
+  std::vector<int64_t> rstart((size_t)nrows + 1, 0);
+  for (size_t k = 0; k < nl; k++) rstart[(size_t)(lk[k].i - r0) + 1]++;
+  for (int64_t r = 0; r < nrows; r++) rstart[(size_t)r + 1] += rstart[(size_t)r];
+  std::vector<uint32_t> order(nl);
+  {
+    std::vector<int64_t> cur(rstart.begin(), rstart.end() - 1);
+    for (size_t k = 0; k < nl; k++) order[(size_t)cur[(size_t)(lk[k].i - r0)]++] = (uint32_t)k;
+  }
+  for (int64_t r = 0; r < nrows; r++) {
+    uint32_t *b = order.data() + rstart[(size_t)r], *e = order.data() + rstart[(size_t)r + 1];
+    if (e - b > 1) std::sort(b, e, [lk](uint32_t x, uint32_t y) { return lk[x].j < lk[y].j; });
+  }
+  if (alloc_links(R, nl)) return TN93G_E_NOMEM;
+  for (size_t k = 0; k < nl; k++) {
+    const LinkRec &q = lk[order[k]];
+    R->link_row[k] = q.i;
+    R->link_col[k] = q.j;
+    R->link_dist[k] = q.d;
+    R->link_dist_lo[k] = q.lo;
+    R->link_dist_hi[k] = q.hi;
+  }
+  // Enclosure of tn93's mean (swifttn93.cpp:2594-2602, 2839-2848).  Row r: tn93 adds its n_r terms t in column
+  // order in FP64; here they were added in another order.  Both results are within gamma_{n_r} * sum|t| of the
+  // exact sum (Higham, Accuracy and Stability of Numerical Algorithms, 2nd ed., eq. 4.4), so tn93's row sum lies in
+  // [slo - 2 g sabs', shi + 2 g sabs'] with sabs' an upper bound of sum|t|.  The rows are then added in row order:
+  // again within gamma_R * sum |row sums|.  Host arithmetic is long double with an extra relative margin.
+  const long double u = 0x1p-53L;
+  auto gam = [&](long double n) { return n * u / (1.0L - n * u); };
+  long double mlo = 0, mhi = 0, mabs = 0;
+  double maxlo = 0., maxhi = 0.;
+  long long wsum = 0, skip = 0, linkw = 0;
+  for (int64_t r = 0; r < nrows; r++) {
+    const RowAcc &q = ra[(size_t)r];
+    const int64_t i = r0 + r;
+    const long double n_r = (long double)(J.A.triangle ? (J.A.c1 - i - 1) : J.A.ncols);
+    const long double g = gam(std::max<long double>(n_r, 1));
+    const long double sabs = (long double)q.sabs * (1.0L + g);
+    const long double e = 2.0L * g * sabs;
+    const long double lo = (long double)q.slo - e, hi = (long double)q.shi + e;
+    mlo += lo;
+    mhi += hi;
+    mabs += std::max(fabsl(lo), fabsl(hi));
+    const double a = tn93g_host_from_bits(q.mlo), b = tn93g_host_from_bits(q.mhi);
+    if (a > maxlo) maxlo = a;
+    if (b > maxhi) maxhi = b;
+    wsum += q.wsum;
+    skip += q.skip;
+    linkw += q.linkw;
+  }
+  const long double eR = gam((long double)std::max<int64_t>(nrows, 1)) * mabs + 0x1p-60L * mabs;
+  R->mean_lo[0] = std::nextafter((double)(mlo - eR), -INFINITY);
+  R->mean_hi[0] = std::nextafter((double)(mhi + eR), INFINITY);
+  R->max_lo[0] = maxlo;
+  R->max_hi[0] = maxhi;
+  R->skipped = skip;
+  R->found_links = linkw;
+  long long w_all = wsum;
+  if (J.A.triangle) {
+    for (int64_t i = 0; i < J.S->d.n; i++) {
+      const int64_t wi = J.S->hw[(size_t)i];
+      if (wi > 1) w_all += (wi * (wi - 1)) >> 1;
+    }
+  }
+  R->weighted[0] = (double)w_all;
+  unsigned long long hrep[600];
+  memcpy(hrep, pin + F.off_h, sizeof hrep);
+  job_finish_common(J, hrep, R);
+  return 0;
+}
+
+// The exact path after the run's kernels: host-exact tails patched in, rows summed in column order by k_rows.
+static int job_exact_result(tn93g_ctx *c, Job &J, tn93g_result *R)
+{
+  RunArgs &A = J.A;
+  RunBufs *B = J.B;
+  const int64_t nrows = J.nrows;
+  const size_t nr1 = J.nr1;
+  const unsigned n_exact = J.n_exact;
+  if (A.exact && n_exact) {
+    if (n_exact > J.exact_cap) return c->fail(TN93G_E_NOMEM, "host-exact list overflow");
+    std::vector<ExactRec> ex(n_exact);
+    HIPCHK(c, hipMemcpyAsync(ex.data(), B->exact_list.p, n_exact * sizeof(ExactRec), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::vector<PatchRec> pr(n_exact);
+    auto work = [&](size_t b0, size_t e0) {
+      for (size_t k = b0; k < e0; k++) {
+        double dd = host_tail(ex[k].tin);
+        double x = dd <= 0. ? 0. : dd;
+        pr[k].i = ex[k].i;
+        pr[k].j = ex[k].j;
+        pr[k].thisD = x;
+        pr[k].idx = host_hist_index(dd);
+        pr[k].link = x >= J.P.min_distance && x <= J.P.threshold;
+      }
+    };
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (n_exact < 4096) nth = 1;
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nth; t++)
+      th.emplace_back(work, (size_t)n_exact * t / nth, (size_t)n_exact * (t + 1) / nth);
+    for (auto &t : th) t.join();
+    if (ensure(c, B->patch, pr.size() * sizeof(PatchRec))) return TN93G_E_NOMEM;
+    HIPCHK(c, hipMemcpyAsync(B->patch.p, pr.data(), pr.size() * sizeof(PatchRec), hipMemcpyHostToDevice, c->stream));
+    hipLaunchKernelGGL(k_patch, dim3((unsigned)((n_exact + 255) / 256)), dim3(256), 0, c->stream, A,
+                       (const PatchRec *)B->patch.p, (int64_t)n_exact);
+  }
+  // link offsets, then the rows in column order
+  int64_t *loff = (int64_t *)B->link_off.p;
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)B->row_links.p, loff, nrows,
+                     (int32_t *)nullptr);
+  int64_t n_links = 0;
+  HIPCHK(c, hipMemcpyAsync(&n_links, loff + nrows, 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (ensure(c, B->links, (size_t)std::max<int64_t>(n_links, 1) * 40)) return TN93G_E_NOMEM;
+  RowOut O;
+  {
+    double *base = (double *)B->rowout.p;
+    O.sum_lo = base;
+    O.sum_hi = base + 3 * nr1;
+    O.wsum = base + 6 * nr1;
+    O.max_lo = base + 9 * nr1;
+    O.max_hi = base + 12 * nr1;
+    O.skipped = (int64_t *)(base + 15 * nr1);
+    O.linkw = (int64_t *)(base + 16 * nr1);
+    const size_t nl = (size_t)std::max<int64_t>(n_links, 1);
+    O.link_row = (int64_t *)B->links.p;
+    O.link_col = O.link_row + nl;
+    O.link_d = (double *)(O.link_col + nl);
+    O.link_lo = O.link_d + nl;
+    O.link_hi = O.link_lo + nl;
+    O.link_off = loff;
+  }
+  if (nrows > 0) hipLaunchKernelGGL(k_rows, dim3((unsigned)((nrows + RT - 1) / RT)), dim3(RT), 0, c->stream, A, O);
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  HIPCHK(c, hipGetLastError());
+  const size_t nl = (size_t)n_links;
+  if (alloc_links(R, nl)) return TN93G_E_NOMEM;
+  std::vector<double> rows(nr1 * 17);
+  std::vector<unsigned long long> hrep(600);
+  if (nl) {
+    HIPCHK(c, hipMemcpyAsync(R->link_row, O.link_row, nl * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(R->link_col, O.link_col, nl * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(R->link_dist, O.link_d, nl * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(R->link_dist_lo, O.link_lo, nl * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(R->link_dist_hi, O.link_hi, nl * 8, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(c, hipMemcpyAsync(rows.data(), B->rowout.p, rows.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(hrep.data(), B->hist.p, hrep.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // row-order merge (swifttn93.cpp:2832-2848)
+  for (int b = 0; b < 3; b++) R->mean_lo[b] = R->mean_hi[b] = R->weighted[b] = R->max_lo[b] = R->max_hi[b] = 0.;
+  for (int64_t r = 0; r < nrows; r++) {
+    for (int b = 0; b < 3; b++) {
+      const double mlo = rows[9 * nr1 + b * nr1 + (size_t)r], mhi = rows[12 * nr1 + b * nr1 + (size_t)r];
+      if (mlo > R->max_lo[b]) R->max_lo[b] = mlo;
+      if (mhi > R->max_hi[b]) R->max_hi[b] = mhi;
+      R->mean_lo[b] += rows[b * nr1 + (size_t)r];
+      R->mean_hi[b] += rows[3 * nr1 + b * nr1 + (size_t)r];
+      R->weighted[b] += rows[6 * nr1 + b * nr1 + (size_t)r];
+    }
+    int64_t sk, lw;
+    memcpy(&sk, &rows[15 * nr1 + (size_t)r], 8);
+    memcpy(&lw, &rows[16 * nr1 + (size_t)r], 8);
+    R->skipped += sk;
+    R->found_links += lw;
+  }
+  job_finish_common(J, hrep.data(), R);
+  return 0;
+}
+
+static void set_times(tn93g_ctx *c, tn93g_result *R, bool single)
+{
+  float ms_all = 0, ms_pair = 0, ms_tail = 0;
+  (void)hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]);
+  if (single) {
+    (void)hipEventElapsedTime(&ms_pair, c->ev[0], c->ev[1]);
+    (void)hipEventElapsedTime(&ms_tail, c->ev[1], c->ev[3]);
+  }
+  R->gpu_ms = ms_all;
+  R->pair_kernel_ms = ms_pair;
+  R->tail_kernel_ms = ms_tail;
+}
+
 extern "C" {
 
 const char *tn93g_version(void) { return TN93G_VERSION; }
@@ -1539,10 +2085,11 @@ void tn93g_close(tn93g_ctx *c)
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  for (DevBuf *b : {&c->D, &c->U, &c->hist, &c->row_links, &c->link_off, &c->slow_list, &c->counters,
-                    &c->exact_list, &c->rowout, &c->links, &c->evtab, &c->rtab, &c->patch, &c->chdr, &c->cdata,
-                    &c->slow_cnt, &c->slow_off, &c->slow_chunk, &c->links_flat})
-    if (b->p) (void)hipFree(b->p);
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  for (RunBufs *b : c->slots) {
+    b->release();
+    delete b;
+  }
   for (auto &e : c->ev) (void)hipEventDestroy(e);
   if (c->pin.p) (void)hipHostFree(c->pin.p);
   (void)hipStreamDestroy(c->stream);
@@ -1652,446 +2199,213 @@ void tn93g_result_free(tn93g_result *r)
   free(r);
 }
 
-// tn93's tail with the host libm (refkernel.cpp:752-815), for the host-exact path
-static double host_tail(const tn93g_tailin &in)
+
+// Autotuner.  A job's shape is its words per sequence, its size class (log2 of rows and of columns), its mode
+// class and its pair kind; the first job of a shape with TN93G_BACKEND_AUTO and TN93G_FLAG_AUTOTUNE runs its pair
+// kernel once with every variant (the counts are exact either way, so the results are the same) and the fastest
+// is kept for the context's lifetime.  Without the flag, AUTO takes the tuned choice when there is one, else the
+// popcount backend.
+static uint64_t tune_shape(const Job &J)
 {
-  const double *nucFreq = in.nf;
-  double auxd, fY, fR, K1 = 0, K2 = 0, K3 = 0, AG, CT, ti, tv, tng2, nucF[4];
-  bool useK2P = false;
-  tng2 = 2. / (nucFreq[0] + nucFreq[1] + nucFreq[2] + nucFreq[3]);
-  auxd = 1. / (nucFreq[0] + nucFreq[1] + nucFreq[2] + nucFreq[3]);
-  for (int k = 0; k < 4; k++) nucF[k] = nucFreq[k] * auxd;
-  fR = nucF[0] + nucF[2];
-  fY = nucF[1] + nucF[3];
-  if (nucFreq[0] == 0 || nucFreq[1] == 0 || nucFreq[2] == 0 || nucFreq[3] == 0)
-    useK2P = true;
-  else {
-    K1 = 2. * nucF[0] * nucF[2] / fR;
-    K2 = 2. * nucF[1] * nucF[3] / fY;
-    K3 = 2. * (fR * fY - nucF[0] * nucF[2] * fY / fR - nucF[1] * nucF[3] * fR / fY);
-  }
-  AG = in.ag * tng2;
-  CT = in.ct * tng2;
-  tv = 1. - (in.diag * tng2 + AG + CT);
-  if (useK2P) {
-    ti = AG + CT;
-    AG = 1. - 2. * ti - tv;
-    CT = 1. - 2. * tv;
-    return (AG > 0. && CT > 0.) ? -0.5 * log(AG) - 0.25 * log(CT) : 1000.0;
-  }
-  AG = 1. - AG / K1 - 0.5 * tv / fR;
-  CT = 1. - CT / K2 - 0.5 * tv / fY;
-  tv = 1. - 0.5 * tv / fY / fR;
-  return (AG > 0. && CT > 0. && tv > 0) ? -K1 * log(AG) - K2 * log(CT) - K3 * log(tv) : 1000.0;
+  auto lg = [](int64_t v) { return (uint64_t)(v > 0 ? 64 - __builtin_clzll((unsigned long long)v) : 0); };
+  const uint64_t mode = J.A.gapmm ? 3 : J.A.all_slow ? 2 : J.A.skip_events ? 1 : 0;
+  return (uint64_t)J.A.W << 24 | lg(J.nrows) << 16 | lg(J.A.ncols) << 8 | mode << 2 | (uint64_t)J.P.kind;
 }
 
-static int host_hist_index(double d)
+static int autotune(tn93g_ctx *c, Job &J)
 {
-  double f = floor(d * 200.0);
-  if (f >= 0.0 && f < 200.0) return (int)f;
-  return 199;
+  if (J.P.backend != TN93G_BACKEND_AUTO) return 0;
+  const uint64_t key = tune_shape(J);
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) {
+    J.P.backend = it->second;
+    return 0;
+  }
+  if (!(J.P.flags & TN93G_FLAG_AUTOTUNE)) {
+    J.P.backend = TN93G_BACKEND_POPC;
+    return 0;
+  }
+  const int cand[2] = {TN93G_BACKEND_POPC, TN93G_BACKEND_MFMA};
+  float best = 0;
+  int pick = TN93G_BACKEND_POPC;
+  for (int v : cand) {
+    J.P.backend = v;
+    job_enqueue(c, J, true);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    if (v == cand[0] || ms < best) {
+      best = ms;
+      pick = v;
+    }
+  }
+  c->tuned[key] = pick;
+  J.P.backend = pick;
+  return 0;
 }
 
 int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_result **out)
 {
   if (!c || !S || !P || !out) return TN93G_E_ARG;
   *out = nullptr;
-  const SetDev &d = S->d;
-  if (P->mode < 0 || P->mode > 4 || P->kind < 0 || P->kind > 2) return c->fail(TN93G_E_ARG, "bad mode or kind");
-  if (P->kind != TN93G_TRIANGLE && (P->n1 < 0 || P->n1 > d.n)) return c->fail(TN93G_E_ARG, "bad n1");
   HIPCHK(c, hipSetDevice(c->device));
-  RunArgs A;
-  memset(&A, 0, sizeof A);
-  A.planes = d.planes;
-  A.ambp = d.ambp;
-  A.onehot = d.onehot;
-  A.codes = d.codes;
-  A.codesT = d.codesT;
-  A.ev_off = d.ev_off;
-  A.ev = d.ev;
-  A.weights = d.weights;
-  A.n = d.n;
-  A.Ls = d.Ls;
-  A.npad = d.npad;
-  A.W = d.W;
-  A.W64 = d.W64;
-  if (P->kind == TN93G_CROSS) {
-    A.r0 = 0;
-    A.r1 = P->n1;
-    A.c0 = P->n1;
-    A.c1 = d.n;
-    A.triangle = 0;
-  } else {
-    A.r0 = 0;
-    A.r1 = d.n;
-    A.c0 = 0;
-    A.c1 = d.n;
-    A.triangle = 1;
-  }
-  A.fst = P->kind == TN93G_FST;
-  A.n1 = P->kind == TN93G_TRIANGLE ? d.n : P->n1;
-  A.ncols = A.c1 - A.c0;
-  const int64_t nrows = A.r1 - A.r0;
-  A.gapmm = P->mode == TN93G_GAPMM;
-  A.skip_events = P->mode == TN93G_SKIP;
-  A.avg_fallback = (P->mode == TN93G_RESOLVE || P->mode == TN93G_SUBSET) && P->resolve_fraction < 1.;
-  A.exact = (P->flags & TN93G_FLAG_EXACT) ? 1 : 0;
-  A.all_slow = (P->flags & TN93G_FLAG_ALL_SLOW) ? 1 : 0;
-  A.rf = P->resolve_fraction;
-  A.thr = P->threshold;
-  A.mind = P->min_distance;
-  A.window = TN93G_LOG_WINDOW;
-  A.min_overlap = P->min_overlap;
-  const int64_t npairs_box = nrows * A.ncols;
-  int64_t npairs = A.triangle ? d.n * (d.n - 1) / 2 : npairs_box;
-  if (npairs >= (1ll << 32)) return c->fail(TN93G_E_ARG, "more than 2^32 pairs in one run");
-  // tables
-  {
-    std::vector<uint4> et(2 * NCODE * NCODE);
-    std::vector<ReplayEnt> rt(2 * NCODE * NCODE);
-    for (int a = 0; a < NCODE; a++)
-      for (int b = 0; b < NCODE; b++) {
-        SiteEffect E = site_effect(a, b, P->mode, P->subset_mask);
-        for (int t = 0; t < 2; t++) {
-          SiteEffect F = t ? site_effect(a, b, TN93G_AVERAGE, 0) : E;
-          uint32_t q[4];
-          event_entry(F, q);
-          et[(size_t)(t * NCODE * NCODE + a * NCODE + b)] = make_uint4(q[0], q[1], q[2], q[3]);
-          ReplayEnt &r = rt[(size_t)(t * NCODE * NCODE + a * NCODE + b)];
-          r.v = F.v;
-          r.mask = F.mask;
-          r.icd = (int16_t)F.icd;
-          r.amb = (uint16_t)F.amb;
-        }
-      }
-    if (ensure(c, c->evtab, et.size() * sizeof(uint4)) || ensure(c, c->rtab, rt.size() * sizeof(ReplayEnt)))
-      return TN93G_E_NOMEM;
-    HIPCHK(c, hipMemcpyAsync(c->evtab.p, et.data(), et.size() * sizeof(uint4), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->rtab.p, rt.data(), rt.size() * sizeof(ReplayEnt), hipMemcpyHostToDevice, c->stream));
-  }
-  A.evtab = (const uint4 *)c->evtab.p;
-  A.rtab = (const ReplayEnt *)c->rtab.p;
-  // buffers
-  const bool fast = !A.exact && P->kind != TN93G_FST;  // F_ST combines the means non-monotonically: exact path
-  A.fast = fast ? 1 : 0;
-  const size_t nbox = (size_t)std::max<int64_t>(npairs_box, 1);
-  const size_t nr1 = (size_t)std::max<int64_t>(nrows, 1);
-  const int64_t c0a = A.c0 & ~(int64_t)63;
-  const int64_t gx = std::max<int64_t>((A.c1 - c0a + TJ - 1) / TJ, 1), gy = std::max<int64_t>((nrows + TI - 1) / TI, 1);
-  const int64_t ntiles = gx * gy, nslots = fast ? (int64_t)nr1 * gx : 1;
-  if (gx >= (1ll << 31) || gy >= 65536) return c->fail(TN93G_E_ARG, "too many pair tiles in one run");
-  if ((!fast && (ensure(c, c->D, nbox * 8) || ensure(c, c->U, nbox * 2))) ||
-      ensure(c, c->hist, (size_t)NSTRIPE * 600 * 8) || ensure(c, c->row_links, nr1 * 4) ||
-      ensure(c, c->link_off, (size_t)(nrows + 1) * 8) ||
-      ensure(c, c->slow_list, (size_t)ntiles * TI * TJ * 8) || ensure(c, c->slow_cnt, (size_t)ntiles * 4) ||
-      ensure(c, c->slow_off, (size_t)(ntiles + 1) * 8) || ensure(c, c->slow_chunk, (size_t)(16 * ntiles + 1) * 4) ||
-      ensure(c, c->counters, (size_t)NSTRIPE * 32 * 4) ||
-      ensure(c, c->rowout, nr1 * std::max<size_t>(15 * 8 + 16, sizeof(RowAcc))) ||
-      ensure(c, c->chdr, (size_t)nslots * sizeof(ChunkHdr)) ||
-      ensure(c, c->cdata, (size_t)nslots * 7 * 64 * sizeof(uint32_t)))
-    return TN93G_E_NOMEM;
-  const unsigned exact_cap = A.exact ? (unsigned)std::min<int64_t>(npairs, 1ll << 31) : 1u;
-  if (ensure(c, c->exact_list, (size_t)exact_cap * sizeof(ExactRec))) return TN93G_E_NOMEM;
-  unsigned link_cap = 1;
-  if (fast) {
-    // links per stripe region; a run that overflows a region is repeated with larger regions
-    const int64_t want = std::max<int64_t>(c->link_cap_hint, 1 << 15);
-    link_cap = (unsigned)std::min<int64_t>(std::max<int64_t>(npairs, 1), want);
-    if (ensure(c, c->links, (size_t)link_cap * NSTRIPE * sizeof(LinkRec)) ||
-        ensure(c, c->links_flat, (size_t)link_cap * NSTRIPE * sizeof(LinkRec)))
-      return TN93G_E_NOMEM;
-  }
-  A.D = (double *)c->D.p;
-  A.U = (uint16_t *)c->U.p;
-  A.hist = (unsigned long long *)c->hist.p;
-  A.row_links = (int32_t *)c->row_links.p;
-  A.slow_list = (unsigned long long *)c->slow_list.p;
-  A.slow_cnt = (unsigned *)c->slow_cnt.p;
-  A.stripes = (unsigned *)c->counters.p;
-  A.exact_cnt = A.stripes + ST_EXACT;
-  A.exact_list = (ExactRec *)c->exact_list.p;
-  A.exact_cap = exact_cap;
-  A.racc = (RowAcc *)c->rowout.p;
-  A.links = (LinkRec *)c->links.p;
-  A.link_cap = link_cap;
-  A.chdr = (ChunkHdr *)c->chdr.p;
-  A.cdata = (uint32_t *)c->cdata.p;
-  A.gx = gx;
-  HIPCHK(c, hipMemsetAsync(c->hist.p, 0, (size_t)NSTRIPE * 600 * 8, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->counters.p, 0, (size_t)NSTRIPE * 32 * 4, c->stream));
-  HIPCHK(c, hipMemsetAsync(c->row_links.p, 0, nr1 * 4, c->stream));
-  if (fast) HIPCHK(c, hipMemsetAsync(c->rowout.p, 0, nr1 * sizeof(RowAcc), c->stream));
-  HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-  if (nrows > 0 && A.ncols > 0) {
-    dim3 g((unsigned)gx, (unsigned)gy);
-    const bool mf = P->backend == TN93G_BACKEND_MFMA;
-    if (mf && fast)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, true>), g, dim3(256), 0, c->stream, A);
-    else if (mf)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, false>), g, dim3(256), 0, c->stream, A);
-    else if (fast)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, true>), g, dim3(256), 0, c->stream, A);
-    else
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, false>), g, dim3(256), 0, c->stream, A);
-  } else {
-    HIPCHK(c, hipMemsetAsync(c->slow_cnt.p, 0, (size_t)ntiles * 4, c->stream));
-  }
-  HIPCHK(c, hipEventRecord(c->ev[1], c->stream));
-  if (fast && nrows > 0 && A.ncols > 0)
-    hipLaunchKernelGGL(k_tail, dim3((unsigned)std::min<int64_t>(nrows, 65535)), dim3(256), 0, c->stream, A);
-  HIPCHK(c, hipEventRecord(c->ev[3], c->stream));
-  int64_t *soff = (int64_t *)c->slow_off.p;
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)c->slow_cnt.p, soff, ntiles,
-                     (int32_t *)c->slow_chunk.p);
-  A.slow_off = soff;
-  A.slow_chunk = (const int32_t *)c->slow_chunk.p;
-  A.ntiles = ntiles;
-  hipLaunchKernelGGL(k_replay, dim3(2048), dim3(RB), 0, c->stream, A);
-  if (fast) hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, c->stream, A, (LinkRec *)c->links_flat.p);
+  Job J;
+  J.S = S;
+  J.P = *P;
+  J.B = c->slot(0);
+  int rc = job_prepare(c, J);
+  if (rc) return rc;
+  if ((rc = autotune(c, J))) return rc;
+  job_enqueue(c, J, true);
   HIPCHK(c, hipGetLastError());
-  const size_t stripe_bytes = (size_t)NSTRIPE * 32 * 4;
-  if (ensure_pinned(c, c->pin, stripe_bytes + 8)) return TN93G_E_NOMEM;
-  std::vector<unsigned> stripes((size_t)NSTRIPE * 32);
-  HIPCHK(c, hipMemcpyAsync(c->pin.p, c->counters.p, stripe_bytes, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync((char *)c->pin.p + stripe_bytes, soff + ntiles, 8, hipMemcpyDeviceToHost, c->stream));
+  if (ensure_pinned(c, c->pin, COUNTER_STRIDE)) return TN93G_E_NOMEM;
+  if ((rc = job_enqueue_counters(c, J, (char *)c->pin.p))) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  memcpy(stripes.data(), c->pin.p, stripe_bytes);
-  int64_t slow_total = 0;
-  memcpy(&slow_total, (const char *)c->pin.p + stripe_bytes, 8);
-  uint64_t n_slow = (uint64_t)slow_total, n_unc = 0, n_undecided = 0, n_linkrec = 0;
-  unsigned max_region = 0;
-  for (int st = 0; st < NSTRIPE; st++) {
-    n_unc += stripes[(size_t)st * 32 + ST_UNCERTAIN];
-    n_undecided += stripes[(size_t)st * 32 + ST_UNDECIDED];
-    n_linkrec += stripes[(size_t)st * 32 + ST_LINKS];
-    max_region = std::max(max_region, stripes[(size_t)st * 32 + ST_LINKS]);
-  }
-  const unsigned n_exact = stripes[ST_EXACT];
-  if (n_undecided && !A.exact) {
+  job_parse_counters(J, (const char *)c->pin.p);
+  if (J.n_undecided && !J.A.exact) {
     // an output decision could not be certified from the enclosures: run again resolving logs on the host
     tn93g_params Q = *P;
     Q.flags |= TN93G_FLAG_EXACT;
     return tn93g_run(c, S, &Q, out);
   }
-  if (fast && max_region > link_cap) {  // more links than a region holds: grow the regions and run again
-    c->link_cap_hint = (int64_t)max_region;
+  if (J.fast && J.max_region > J.link_cap) {  // more links than a region holds: grow the regions and run again
+    J.B->link_cap_hint = (int64_t)J.max_region;
     return tn93g_run(c, S, P, out);
   }
   tn93g_result *R = (tn93g_result *)calloc(1, sizeof(tn93g_result));
   if (!R) return TN93G_E_NOMEM;
-  std::vector<unsigned long long> hrep(600);
-  hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, c->stream, A);
-  if (fast) {
-    // ---- fast path: row accumulators, unordered links
-    const size_t nl = n_linkrec;
-    const size_t off_ra = (nl * sizeof(LinkRec) + 255) & ~(size_t)255;
-    const size_t off_h = off_ra + ((nr1 * sizeof(RowAcc) + 255) & ~(size_t)255);
-    if (ensure_pinned(c, c->pin, off_h + hrep.size() * 8)) {
-      free(R);
-      return TN93G_E_NOMEM;
-    }
-    char *pin = (char *)c->pin.p;
-    if (nl) HIPCHK(c, hipMemcpyAsync(pin, c->links_flat.p, nl * sizeof(LinkRec), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(pin + off_ra, c->rowout.p, nr1 * sizeof(RowAcc), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(pin + off_h, c->hist.p, hrep.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    const LinkRec *lk = (const LinkRec *)pin;
-    const RowAcc *ra = (const RowAcc *)(pin + off_ra);
-    memcpy(hrep.data(), pin + off_h, hrep.size() * 8);
-    // tn93's emission order (rows ascending, columns ascending): counting sort by row, then each row's few links by
-    // column
-    std::vector<int64_t> rstart((size_t)nrows + 1, 0);
-    for (size_t k = 0; k < nl; k++) rstart[(size_t)(lk[k].i - A.r0) + 1]++;
-    for (int64_t r = 0; r < nrows; r++) rstart[(size_t)r + 1] += rstart[(size_t)r];
-    std::vector<uint32_t> order(nl);
-    {
-      std::vector<int64_t> cur(rstart.begin(), rstart.end() - 1);
-      for (size_t k = 0; k < nl; k++) order[(size_t)cur[(size_t)(lk[k].i - A.r0)]++] = (uint32_t)k;
-    }
-    for (int64_t r = 0; r < nrows; r++) {
-      uint32_t *b = order.data() + rstart[(size_t)r], *e = order.data() + rstart[(size_t)r + 1];
-      if (e - b > 1) std::sort(b, e, [lk](uint32_t x, uint32_t y) { return lk[x].j < lk[y].j; });
-    }
-    R->n_links = (int64_t)nl;
-    R->link_row = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
-    R->link_col = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
-    R->link_dist = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-    R->link_dist_lo = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-    R->link_dist_hi = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-    if (!R->link_row || !R->link_col || !R->link_dist || !R->link_dist_lo || !R->link_dist_hi) {
-      tn93g_result_free(R);
-      return TN93G_E_NOMEM;
-    }
-    for (size_t k = 0; k < nl; k++) {
-      const LinkRec &q = lk[order[k]];
-      R->link_row[k] = q.i;
-      R->link_col[k] = q.j;
-      R->link_dist[k] = q.d;
-      R->link_dist_lo[k] = q.lo;
-      R->link_dist_hi[k] = q.hi;
-    }
-    // Enclosure of tn93's mean (swifttn93.cpp:2594-2602, 2839-2848).  Row r: tn93 adds its n_r terms t in column
-    // order in FP64; here they were added in another order.  Both results are within gamma_{n_r} * sum|t| of the
-    // exact sum (Higham, Accuracy and Stability of Numerical Algorithms, 2nd ed., eq. 4.4), so tn93's row sum lies in
-    // [slo - 2 g sabs', shi + 2 g sabs'] with sabs' an upper bound of sum|t|.  The rows are then added in row order:
-    // again within gamma_R * sum |row sums|.  Host arithmetic is long double with an extra relative margin.
-    const long double u = 0x1p-53L;
-    auto gam = [&](long double n) { return n * u / (1.0L - n * u); };
-    long double mlo = 0, mhi = 0, mabs = 0;
-    double maxlo = 0., maxhi = 0.;
-    long long wsum = 0, skip = 0, linkw = 0;
-    for (int64_t r = 0; r < nrows; r++) {
-      const RowAcc &q = ra[(size_t)r];
-      const int64_t i = A.r0 + r;
-      const long double n_r = (long double)(A.triangle ? (A.c1 - i - 1) : A.ncols);
-      const long double g = gam(std::max<long double>(n_r, 1));
-      const long double sabs = (long double)q.sabs * (1.0L + g);
-      const long double e = 2.0L * g * sabs;
-      const long double lo = (long double)q.slo - e, hi = (long double)q.shi + e;
-      mlo += lo;
-      mhi += hi;
-      mabs += std::max(fabsl(lo), fabsl(hi));
-      const double a = tn93g_host_from_bits(q.mlo), b = tn93g_host_from_bits(q.mhi);
-      if (a > maxlo) maxlo = a;
-      if (b > maxhi) maxhi = b;
-      wsum += q.wsum;
-      skip += q.skip;
-      linkw += q.linkw;
-    }
-    const long double eR = gam((long double)std::max<int64_t>(nrows, 1)) * mabs + 0x1p-60L * mabs;
-    R->mean_lo[0] = std::nextafter((double)(mlo - eR), -INFINITY);
-    R->mean_hi[0] = std::nextafter((double)(mhi + eR), INFINITY);
-    R->max_lo[0] = maxlo;
-    R->max_hi[0] = maxhi;
-    R->skipped = skip;
-    R->found_links = linkw;
-    long long w_all = wsum;
-    if (A.triangle) {
-      for (int64_t i = 0; i < d.n; i++) {
-        const int64_t wi = S->hw[(size_t)i];
-        if (wi > 1) w_all += (wi * (wi - 1)) >> 1;
-      }
-    }
-    R->weighted[0] = (double)w_all;
-  } else {
-    if (A.exact && n_exact) {
-      if (n_exact > exact_cap) return c->fail(TN93G_E_NOMEM, "host-exact list overflow");
-      std::vector<ExactRec> ex(n_exact);
-      HIPCHK(c, hipMemcpyAsync(ex.data(), c->exact_list.p, n_exact * sizeof(ExactRec), hipMemcpyDeviceToHost,
-                               c->stream));
+  auto finish = [&]() -> int {
+    if (J.fast) {
+      const FastLayout F = fast_layout(J);
+      if (ensure_pinned(c, c->pin, F.bytes)) return TN93G_E_NOMEM;
+      int r = job_enqueue_fast_readback(c, J, (char *)c->pin.p);
+      if (r) return r;
+      HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
-      std::vector<PatchRec> pr(n_exact);
-      auto work = [&](size_t b0, size_t e0) {
-        for (size_t k = b0; k < e0; k++) {
-          double dd = host_tail(ex[k].tin);
-          double x = dd <= 0. ? 0. : dd;
-          pr[k].i = ex[k].i;
-          pr[k].j = ex[k].j;
-          pr[k].thisD = x;
-          pr[k].idx = host_hist_index(dd);
-          pr[k].link = x >= P->min_distance && x <= P->threshold;
-        }
-      };
-      unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-      if (n_exact < 4096) nth = 1;
-      std::vector<std::thread> th;
-      for (unsigned t = 0; t < nth; t++)
-        th.emplace_back(work, (size_t)n_exact * t / nth, (size_t)n_exact * (t + 1) / nth);
-      for (auto &t : th) t.join();
-      if (ensure(c, c->patch, pr.size() * sizeof(PatchRec))) return TN93G_E_NOMEM;
-      HIPCHK(c,
-             hipMemcpyAsync(c->patch.p, pr.data(), pr.size() * sizeof(PatchRec), hipMemcpyHostToDevice, c->stream));
-      hipLaunchKernelGGL(k_patch, dim3((unsigned)((n_exact + 255) / 256)), dim3(256), 0, c->stream, A,
-                         (const PatchRec *)c->patch.p, (int64_t)n_exact);
+      return job_fast_result(J, (const char *)c->pin.p, R);
     }
-    // link offsets, then the rows in column order
-    int64_t *loff = (int64_t *)c->link_off.p;
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)c->row_links.p, loff, nrows,
-                       (int32_t *)nullptr);
-    int64_t n_links = 0;
-    HIPCHK(c, hipMemcpyAsync(&n_links, loff + nrows, 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (ensure(c, c->links, (size_t)std::max<int64_t>(n_links, 1) * 40)) return TN93G_E_NOMEM;
-    RowOut O;
-    {
-      double *base = (double *)c->rowout.p;
-      O.sum_lo = base;
-      O.sum_hi = base + 3 * nr1;
-      O.wsum = base + 6 * nr1;
-      O.max_lo = base + 9 * nr1;
-      O.max_hi = base + 12 * nr1;
-      O.skipped = (int64_t *)(base + 15 * nr1);
-      O.linkw = (int64_t *)(base + 16 * nr1);
-      const size_t nl = (size_t)std::max<int64_t>(n_links, 1);
-      O.link_row = (int64_t *)c->links.p;
-      O.link_col = O.link_row + nl;
-      O.link_d = (double *)(O.link_col + nl);
-      O.link_lo = O.link_d + nl;
-      O.link_hi = O.link_lo + nl;
-      O.link_off = loff;
-    }
-    if (nrows > 0) hipLaunchKernelGGL(k_rows, dim3((unsigned)((nrows + RT - 1) / RT)), dim3(RT), 0, c->stream, A, O);
-    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    HIPCHK(c, hipGetLastError());
-    const size_t nl = (size_t)n_links;
-    R->n_links = n_links;
-    R->link_row = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
-    R->link_col = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
-    R->link_dist = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-    R->link_dist_lo = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-    R->link_dist_hi = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-    std::vector<double> rows(nr1 * 17);
-    if (nl) {
-      HIPCHK(c, hipMemcpyAsync(R->link_row, O.link_row, nl * 8, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(c, hipMemcpyAsync(R->link_col, O.link_col, nl * 8, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(c, hipMemcpyAsync(R->link_dist, O.link_d, nl * 8, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(c, hipMemcpyAsync(R->link_dist_lo, O.link_lo, nl * 8, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(c, hipMemcpyAsync(R->link_dist_hi, O.link_hi, nl * 8, hipMemcpyDeviceToHost, c->stream));
-    }
-    HIPCHK(c, hipMemcpyAsync(rows.data(), c->rowout.p, rows.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipMemcpyAsync(hrep.data(), c->hist.p, hrep.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    // row-order merge (swifttn93.cpp:2832-2848)
-    for (int b = 0; b < 3; b++) R->mean_lo[b] = R->mean_hi[b] = R->weighted[b] = R->max_lo[b] = R->max_hi[b] = 0.;
-    for (int64_t r = 0; r < nrows; r++) {
-      for (int b = 0; b < 3; b++) {
-        const double mlo = rows[9 * nr1 + b * nr1 + (size_t)r], mhi = rows[12 * nr1 + b * nr1 + (size_t)r];
-        if (mlo > R->max_lo[b]) R->max_lo[b] = mlo;
-        if (mhi > R->max_hi[b]) R->max_hi[b] = mhi;
-        R->mean_lo[b] += rows[b * nr1 + (size_t)r];
-        R->mean_hi[b] += rows[3 * nr1 + b * nr1 + (size_t)r];
-        R->weighted[b] += rows[6 * nr1 + b * nr1 + (size_t)r];
-      }
-      int64_t sk, lw;
-      memcpy(&sk, &rows[15 * nr1 + (size_t)r], 8);
-      memcpy(&lw, &rows[16 * nr1 + (size_t)r], 8);
-      R->skipped += sk;
-      R->found_links += lw;
-    }
+    return job_exact_result(c, J, R);
+  };
+  if ((rc = finish())) {
+    tn93g_result_free(R);
+    return rc;
   }
-  for (int k = 0; k < 600; k++) (&R->hist[0][0])[k] += hrep[(size_t)k];  // replicas summed by k_hist_sum
-  float ms_all = 0, ms_pair = 0;
-  (void)hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]);
-  (void)hipEventElapsedTime(&ms_pair, c->ev[0], c->ev[1]);
-  R->gpu_ms = ms_all;
-  R->pair_kernel_ms = ms_pair;
-  float ms_tail = 0;
-  (void)hipEventElapsedTime(&ms_tail, c->ev[1], c->ev[3]);
-  R->tail_kernel_ms = ms_tail;
-  R->exact = (!fast && n_unc == 0) ? 1 : 0;
-  R->nrows = (int32_t)nrows;
-  R->n_replayed = n_slow;
-  R->n_uncertain = n_unc;
-  R->n_host_exact = n_exact;
-  // self_v_self histogram entries (swifttn93.cpp:2406-2415)
-  if (A.triangle)
-    for (int64_t i = 0; i < d.n; i++) {
-      const int64_t wi = S->hw[(size_t)i];
-      if (wi > 1) R->hist[(A.fst && i >= A.n1) ? 1 : 0][0] += (unsigned long long)((wi * (wi - 1)) >> 1);
-    }
+  set_times(c, R, true);
   *out = R;
   return 0;
 }
 
+int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn93g_params *params,
+                    tn93g_result **out)
+{
+  if (!c || n < 0 || (n > 0 && (!sets || !params || !out))) return TN93G_E_ARG;
+  for (int k = 0; k < n; k++) out[k] = nullptr;
+  for (int k = 0; k < n; k++)
+    if (!sets[k]) return TN93G_E_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  auto fail_all = [&](int rc) {
+    for (int k = 0; k < n; k++) {
+      tn93g_result_free(out[k]);
+      out[k] = nullptr;
+    }
+    return rc;
+  };
+  // exact-path jobs (host work between kernels) run alone after the batch
+  std::vector<int> idx, alone;
+  for (int k = 0; k < n; k++)
+    ((params[k].flags & TN93G_FLAG_EXACT) || params[k].kind == TN93G_FST ? alone : idx).push_back(k);
+  const size_t m = idx.size();
+  std::vector<Job> jobs(m);
+  for (size_t t = 0; t < m; t++) {
+    jobs[t].S = sets[idx[t]];
+    jobs[t].P = params[idx[t]];
+    jobs[t].B = c->slot(t);
+    int rc = job_prepare(c, jobs[t]);
+    if (!rc) rc = autotune(c, jobs[t]);
+    if (rc) return fail_all(rc);
+  }
+  if (m) {
+    // the batch's work as one graph; a batch that enqueues exactly the same launches relaunches the last graph
+    std::vector<unsigned char> key;
+    for (const Job &J : jobs) {
+      const unsigned char *a = (const unsigned char *)&J.A;
+      key.insert(key.end(), a, a + sizeof(RunArgs));
+      const int64_t dims[6] = {J.gx, J.gy, J.ntiles, J.nrows, (int64_t)J.nr1, (int64_t)J.fast * 4 + J.P.backend};
+      const unsigned char *b = (const unsigned char *)dims;
+      key.insert(key.end(), b, b + sizeof dims);
+    }
+    if (!c->gexec || key != c->gkey) {
+      if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+      c->gexec = nullptr;
+      c->gkey.clear();
+      hipGraph_t g = nullptr;
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      for (Job &J : jobs) job_enqueue(c, J, false);
+      const hipError_t e1 = hipStreamEndCapture(c->stream, &g);
+      if (e1 != hipSuccess)
+        return fail_all(c->fail(TN93G_E_HIP, fmt("hipStreamEndCapture: %s", hipGetErrorString(e1))));
+      const hipError_t e2 = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (e2 != hipSuccess) {
+        c->gexec = nullptr;
+        return fail_all(c->fail(TN93G_E_HIP, fmt("hipGraphInstantiate: %s", hipGetErrorString(e2))));
+      }
+      c->gkey = key;
+    }
+    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+    if (ensure_pinned(c, c->pin, COUNTER_STRIDE * m)) return fail_all(TN93G_E_NOMEM);
+    for (size_t t = 0; t < m; t++) {
+      int rc = job_enqueue_counters(c, jobs[t], (char *)c->pin.p + t * COUNTER_STRIDE);
+      if (rc) return fail_all(rc);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[2]);
+    std::vector<char> redo(m, 0);
+    std::vector<size_t> off(m, 0);
+    size_t total = 0;
+    for (size_t t = 0; t < m; t++) {
+      Job &J = jobs[t];
+      job_parse_counters(J, (const char *)c->pin.p + t * COUNTER_STRIDE);
+      if ((J.n_undecided && !J.A.exact) || (J.fast && J.max_region > J.link_cap)) {
+        if (J.fast && J.max_region > J.link_cap) J.B->link_cap_hint = (int64_t)J.max_region;
+        redo[t] = 1;
+        continue;
+      }
+      off[t] = total;
+      total += fast_layout(J).bytes;
+    }
+    if (ensure_pinned(c, c->pin, std::max<size_t>(total, 1))) return fail_all(TN93G_E_NOMEM);
+    for (size_t t = 0; t < m; t++)
+      if (!redo[t]) {
+        int rc = job_enqueue_fast_readback(c, jobs[t], (char *)c->pin.p + off[t]);
+        if (rc) return fail_all(rc);
+      }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t t = 0; t < m; t++) {
+      if (redo[t]) continue;
+      tn93g_result *R = (tn93g_result *)calloc(1, sizeof(tn93g_result));
+      if (!R) return fail_all(TN93G_E_NOMEM);
+      out[idx[t]] = R;
+      int rc = job_fast_result(jobs[t], (const char *)c->pin.p + off[t], R);
+      if (rc) return fail_all(rc);
+      R->gpu_ms = ms;  // device time of the whole batch
+    }
+    // jobs whose outputs could not be certified, or whose links overflowed a region, run again alone
+    for (size_t t = 0; t < m; t++)
+      if (redo[t]) {
+        int rc = tn93g_run(c, sets[idx[t]], &params[idx[t]], &out[idx[t]]);
+        if (rc) return fail_all(rc);
+      }
+  }
+  for (int k : alone) {
+    int rc = tn93g_run(c, sets[k], &params[k], &out[k]);
+    if (rc) return fail_all(rc);
+  }
+  return 0;
+}
+
 }  // extern "C"
+

@@ -16,6 +16,8 @@ every machine and numpy version.  Workloads:
        mutates each site with probability 0.001-0.02 %, so depths of about ten edges put sequences 0.01-0.2 % from
        the root); 0-5 N-masked blocks of 100-2,000 columns per sequence; 0.05 % ambiguity codes (c1 proportions).
   c2s  a smaller sample of the c2 generator (n sequences) for parity tests.
+  c3   64 alignments of uneven size and length (c0 generator) with mixed -a modes, and c3_bipartite, the 2,000 x
+       40,000 query-by-reference block at 1,300 columns (c1 generator).
 """
 import numpy as np
 
@@ -84,9 +86,8 @@ def _gap_runs(rs, codes, frac, lo, hi):
                 codes[i, L - run:] = GAP
 
 
-def c0(seed=1):
+def c0(seed=1, N=500, L=1000):
     rs = Stream(seed)
-    N, L = 500, 1000
     root = rs.integers(4, L).astype(np.uint8)
     codes = np.empty((N, L), dtype=np.uint8)
     centers = [_mutate(rs, root, 0.03) for _ in range(8)]
@@ -202,6 +203,29 @@ def c2s(n=300, seed=5, L=29903):
     return c2(seed=seed, n=n, L=L)
 
 
+C3_MODES = ["resolve", "average", "skip", "gapmm"]
+
+
+def c3(seed=6, count=64):
+    """The batched workload: `count` independent alignments from the c0 generator, sizes log-uniform in 50-3,000
+    sequences, lengths uniform in 300-10,000 columns, -a modes cycling resolve/average/skip/gapmm.  A list of
+    (codes, mode) pairs."""
+    rs = Stream(seed)
+    out = []
+    for k in range(count):
+        u = rs.uniform(3)
+        n = int(np.exp(np.log(50.0) + u[0] * (np.log(3000.0) - np.log(50.0))))
+        L = int(300 + u[1] * 9700)
+        out.append((c0(seed=int(rs.u64(1)[0] >> np.uint64(1)), N=n, L=L), C3_MODES[k % 4]))
+    return out
+
+
+def c3_bipartite(seed=7, nq=2000, nr=40000, L=1300):
+    """The -s bipartite block: nq query and nr reference sequences from the c1 generator (one set, queries first)."""
+    n = nq + nr
+    return c1(seed=seed, n=n, L=L, clusters=max(1, n * 200 // 6700))
+
+
 def names_for(n, prefix="s"):
     return [f"{prefix}{i}" for i in range(n)]
 
@@ -215,4 +239,4 @@ def write_fasta(path, codes, names=None):
             f.write(b">" + nm.encode() + b"\n" + lut[row].tobytes() + b"\n")
 
 
-WORKLOADS = {"c0": c0, "c1": c1, "c1s": c1s, "c2": c2, "c2s": c2s}
+WORKLOADS = {"c0": c0, "c1": c1, "c1s": c1s, "c2": c2, "c2s": c2s, "c3": c3, "c3_bipartite": c3_bipartite}

@@ -236,3 +236,42 @@ def test_long_alignment_c2s_vs_oracle(tmp_path):
         assert rc == 0, argv
         assert out == wo, (argv, len(out or b""), len(wo))
         assert so == ws, (argv, so[:400], ws[:400])
+
+
+def _same_summary(r1, r2):
+    _same(r1, r2)
+    assert (r1["link_dist"] == r2["link_dist"]).all()
+    assert r1["max_lo"][0] == r2["max_lo"][0] and r1["max_hi"][0] == r2["max_hi"][0]
+    # the row sums are atomics in either order: the mean enclosures must overlap and print alike
+    assert r1["mean_lo"][0] <= r2["mean_hi"][0] and r2["mean_lo"][0] <= r1["mean_hi"][0]
+    assert "%.6g" % r1["mean_lo"][0] == "%.6g" % r2["mean_lo"][0]
+
+
+def test_batch_equals_single(engine):
+    """tn93g_run_batch (one graph launch for all jobs) gives every job its single-run result.  The cached graph is
+    launched again with eager runs in between, and every launch is checked."""
+    specs = [(gen.c0(seed=21, N=120, L=700), T.RESOLVE), (gen.c0(seed=22, N=64, L=2100), T.AVERAGE),
+             (gen.c1s(200, seed=23), T.SKIP), (gen.c0(seed=24, N=90, L=333), T.GAPMM),
+             (gen.c1s(150, seed=25), T.SUBSET)]
+    sets, jobs = [], []
+    for codes, mode in specs:
+        st = engine.upload(codes)
+        sets.append(st)
+        jobs.append((st, dict(mode=mode, threshold=0.05, min_overlap=min(300, codes.shape[1] // 2),
+                              resolve_fraction=0.05 if mode in (T.RESOLVE, T.SUBSET) else 1.0,
+                              subset_mask=(1 << 5) | (1 << 6) | (1 << 15))))
+    jobs.append((sets[2], dict(mode=T.RESOLVE, kind=T.CROSS, n1=77, threshold=0.05, min_overlap=300,
+                               resolve_fraction=0.05)))
+    jobs.append((sets[0], dict(mode=T.RESOLVE, threshold=0.05, min_overlap=300, flags=T.FLAG_EXACT)))
+    jobs.append((sets[4], dict(mode=T.AVERAGE, threshold=1.0, min_overlap=300, backend=T.BACKEND_AUTO,
+                               flags=T.FLAG_AUTOTUNE)))
+    want = [engine.run(st, **kw) for st, kw in jobs]
+    for rep in range(3):
+        got = engine.run_batch(jobs)
+        assert len(got) == len(want)
+        for k, (w, g) in enumerate(zip(want, got)):
+            print(f"batch launch {rep}, job {k}", flush=True)
+            _same_summary(w, g)
+        engine.run(sets[1], **jobs[1][1])  # eager work between the launches of the cached graph
+    for st in sets:
+        st.free()
