@@ -178,6 +178,8 @@ struct SetDev {
   uint8_t *onehot;
   int32_t *ev_off;
   uint32_t *ev;
+  int32_t *evg_off;  // the lists with gap sites (GAPMM)
+  uint32_t *evg;
   int64_t *weights;
   int64_t total_events;
 };
@@ -467,29 +469,31 @@ __global__ void k_onehot(const uint8_t *__restrict__ codes, uint8_t *__restrict_
   for (int b = 0; b < 4; b++) o[b * 16] = code == (unsigned)b ? 1 : 0;
 }
 
-__global__ void k_evcount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls)
+// Event lists: the sites of codes 4..16 (ambiguous), and with `gaps` also the gap sites (the lists of GAPMM, where a
+// gap facing a non-gap is a site of its own).
+__device__ inline bool is_event(unsigned code, int gaps) { return code >= 4 && (code < (unsigned)GAPC || gaps); }
+
+__global__ void k_evcount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls,
+                          int gaps)
 {
   const int64_t s = blockIdx.x;
   int c = 0;
-  for (int64_t p = threadIdx.x; p < L; p += blockDim.x) {
-    unsigned code = codes[s * Ls + p];
-    c += (code >= 4 && code < (unsigned)GAPC) ? 1 : 0;
-  }
+  for (int64_t p = threadIdx.x; p < L; p += blockDim.x) c += is_event(codes[s * Ls + p], gaps) ? 1 : 0;
   for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
   if (threadIdx.x == 0) cnt[s] = c;
 }
 
 // events in site order: one wave per sequence
 __global__ void k_evfill(const uint8_t *__restrict__ codes, const int32_t *__restrict__ ev_off,
-                         uint32_t *__restrict__ ev, int64_t L, int64_t Ls)
+                         uint32_t *__restrict__ ev, int64_t L, int64_t Ls, int gaps)
 {
   const int64_t s = blockIdx.x;
   const int lane = threadIdx.x;
   int base = ev_off[s];
   for (int64_t p0 = 0; p0 < L; p0 += 64) {
     int64_t p = p0 + lane;
-    unsigned code = p < L ? codes[s * Ls + p] : (unsigned)GAPC;
-    bool f = code >= 4 && code < (unsigned)GAPC;
+    const unsigned code = p < L ? codes[s * Ls + p] : 0u;
+    const bool f = is_event(code, gaps);
     unsigned long long m = __ballot(f);
     if (f) {
       int k = __popcll(m & ((1ull << lane) - 1ull));
@@ -612,12 +616,13 @@ constexpr int TABW = 32, EVNONE = NCODE, EVB = 4;
 constexpr uint32_t EV_PAD = (uint32_t)EVNONE;  // site 0, code EVNONE
 
 // Fills the LDS event tables from one [NCODE][NCODE] table of packed entries (entry [row code][column code]).
-__device__ inline void load_event_tables(const uint4 *src, uint4 *sTabR, uint4 *sTabC, int tid)
+// With `gaps` (GAPMM), gap sites are events too, so a column event facing a row gap is the row's event.
+__device__ inline void load_event_tables(const uint4 *src, uint4 *sTabR, uint4 *sTabC, int tid, bool gaps)
 {
   for (int k = tid; k < (NCODE + 1) * TABW; k += 256) {
     const int c = k / TABW, x = k % TABW;
     const bool in = c < NCODE && x < NCODE;
-    const bool xamb = x >= 4 && x < GAPC;
+    const bool xamb = x >= 4 && (x < GAPC || gaps);
     sTabR[k] = in ? src[c * NCODE + x] : make_uint4(0, 0, 0, 0);
     sTabC[k] = in && !xamb ? src[x * NCODE + c] : make_uint4(0, 0, 0, 0);
   }
@@ -711,7 +716,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   __shared__ uint4 sTabR[(NCODE + 1) * TABW], sTabC[(NCODE + 1) * TABW];
   __shared__ unsigned long long lhist[FAST ? 1 : 3 * 200];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  load_event_tables(A.evtab, sTabR, sTabC, tid);
+  load_event_tables(A.evtab, sTabR, sTabC, tid, A.gapmm);
   if constexpr (!FAST)
     for (int k = tid; k < 600; k += 256) lhist[k] = 0;
   const int64_t j = j0 + lane;
@@ -725,7 +730,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     tot[a] = Tot{{0, 0, 0, 0}, 0, 0, 0, 0};
     acc[a] = EvAcc{0, 0, 0, 0, 0};
   }
-  const bool do_fast = !A.gapmm && !A.all_slow;
+  const bool do_fast = !A.all_slow;
   const bool do_events = do_fast && !A.skip_events;
   // column event cursors (columns past the set have none)
   const int pj = j < A.n ? A.ev_off[j] : 0, pj_end = j < A.n ? A.ev_off[j + 1] : 0;
@@ -846,7 +851,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     }
   }
   if (__syncthreads_or(any_avg)) {
-    load_event_tables(A.evtab + NCODE * NCODE, sTabR, sTabC, tid);
+    load_event_tables(A.evtab + NCODE * NCODE, sTabR, sTabC, tid, A.gapmm);
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < RPW; a++)
@@ -1628,8 +1633,8 @@ static int job_prepare(tn93g_ctx *c, Job &J)
   A.onehot = d.onehot;
   A.codes = d.codes;
   A.codesT = d.codesT;
-  A.ev_off = d.ev_off;
-  A.ev = d.ev;
+  A.ev_off = P->mode == TN93G_GAPMM ? d.evg_off : d.ev_off;
+  A.ev = P->mode == TN93G_GAPMM ? d.evg : d.ev;
   A.weights = d.weights;
   A.n = d.n;
   A.Ls = d.Ls;
@@ -2137,7 +2142,6 @@ int tn93g_set_upload(tn93g_ctx *c, const uint8_t *codes, int64_t n, int64_t L, c
   SCHK(hipMalloc(&d.planes, (size_t)(nn * d.W) * sizeof(uint4)));
   SCHK(hipMalloc(&d.ambp, (size_t)(nn * d.W) * sizeof(uint32_t)));
   SCHK(hipMalloc(&d.onehot, (size_t)(d.npad * d.W64 * 256)));
-  SCHK(hipMalloc(&d.ev_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.weights, (size_t)nn * sizeof(int64_t)));
   SCHK(hipMemcpyAsync(d.codes, padded.data(), padded.size(), hipMemcpyHostToDevice, c->stream));
   SCHK(hipMemcpyAsync(d.weights, wts.data(), wts.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
@@ -2154,22 +2158,33 @@ int tn93g_set_upload(tn93g_ctx *c, const uint8_t *codes, int64_t n, int64_t L, c
   }
   int32_t *cnt_d = nullptr;
   SCHK(hipMalloc(&cnt_d, (size_t)nn * sizeof(int32_t)));
-  if (n > 0) hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls);
-  std::vector<int32_t> cnt((size_t)nn, 0), off((size_t)nn + 1, 0);
-  SCHK(hipMemcpyAsync(cnt.data(), cnt_d, (size_t)nn * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-  SCHK(hipStreamSynchronize(c->stream));
-  (void)hipFree(cnt_d);
-  int64_t total = 0;
-  for (int64_t s = 0; s < n; s++) {
-    off[(size_t)s] = (int32_t)total;
-    total += cnt[(size_t)s];
-    if (total >= (1ll << 31)) return fail(TN93G_E_ARG, "too many ambiguity events");
+  for (int gaps = 0; gaps < 2; gaps++) {
+    int32_t *&eoff = gaps ? d.evg_off : d.ev_off;
+    uint32_t *&ev = gaps ? d.evg : d.ev;
+    SCHK(hipMalloc(&eoff, (size_t)(nn + 1) * sizeof(int32_t)));
+    if (n > 0)
+      hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, gaps);
+    std::vector<int32_t> cnt((size_t)nn, 0), off((size_t)nn + 1, 0);
+    SCHK(hipMemcpyAsync(cnt.data(), cnt_d, (size_t)nn * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+    SCHK(hipStreamSynchronize(c->stream));
+    int64_t total = 0;
+    for (int64_t s = 0; s < n; s++) {
+      off[(size_t)s] = (int32_t)total;
+      total += cnt[(size_t)s];
+      if (total >= (1ll << 31)) {
+        (void)hipFree(cnt_d);
+        return fail(TN93G_E_ARG, "too many ambiguity events");
+      }
+    }
+    off[(size_t)n] = (int32_t)total;
+    if (!gaps) d.total_events = total;
+    SCHK(hipMalloc(&ev, (size_t)std::max<int64_t>(total, 1) * sizeof(uint32_t)));
+    SCHK(hipMemcpyAsync(eoff, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    if (n > 0)
+      hipLaunchKernelGGL(k_evfill, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, eoff, ev, L, d.Ls, gaps);
+    SCHK(hipStreamSynchronize(c->stream));  // `off` is pageable host memory
   }
-  off[(size_t)n] = (int32_t)total;
-  d.total_events = total;
-  SCHK(hipMalloc(&d.ev, (size_t)std::max<int64_t>(total, 1) * sizeof(uint32_t)));
-  SCHK(hipMemcpyAsync(d.ev_off, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-  if (n > 0) hipLaunchKernelGGL(k_evfill, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, d.ev_off, d.ev, L, d.Ls);
+  (void)hipFree(cnt_d);
   SCHK(hipGetLastError());
   SCHK(hipStreamSynchronize(c->stream));
 #undef SCHK
@@ -2182,7 +2197,8 @@ void tn93g_set_free(tn93g_set *S)
   if (!S) return;
   (void)hipSetDevice(S->ctx->device);
   (void)hipStreamSynchronize(S->ctx->stream);
-  void *ps[] = {S->d.codes, S->d.codesT, S->d.planes, S->d.ambp, S->d.onehot, S->d.ev_off, S->d.ev, S->d.weights};
+  void *ps[] = {S->d.codes,  S->d.codesT, S->d.planes,  S->d.ambp,    S->d.onehot,
+                S->d.ev_off, S->d.ev,     S->d.evg_off, S->d.evg, S->d.weights};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   delete S;

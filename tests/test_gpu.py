@@ -157,7 +157,7 @@ def _same(r1, r2):
         assert "%g" % a == "%g" % b
 
 
-@pytest.mark.parametrize("mode", [T.RESOLVE, T.AVERAGE, T.SKIP, T.SUBSET])
+@pytest.mark.parametrize("mode", [T.RESOLVE, T.AVERAGE, T.SKIP, T.SUBSET, T.GAPMM])
 def test_fast_path_equals_replay(engine, mode):
     """The popcount + event-table path and the site-order replay kernel agree on every pair (all written)."""
     codes = gen.c1s(400, seed=5)
