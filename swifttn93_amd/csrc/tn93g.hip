@@ -1494,9 +1494,13 @@ struct RunBufs {
   }
 };
 
+constexpr int NBRANCH = 8;  // parallel branches of a batch graph
+
 struct tn93g_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  hipStream_t side[NBRANCH] = {};  // the branches of a batch, forked from and joined back into `stream`
+  hipEvent_t fork = nullptr, join[NBRANCH] = {};
   hipEvent_t ev[4];
   std::string err;
   std::vector<RunBufs *> slots;
@@ -1755,41 +1759,41 @@ static int job_prepare(tn93g_ctx *c, Job &J)
   return 0;
 }
 
-// The job's kernels, in stream order, with no host synchronisation (a batch captures them into one graph).  With
-// `timing`, events around the pair and tail kernels give the device times of a single run.
-static void job_enqueue(tn93g_ctx *c, Job &J, bool timing)
+// The job's kernels, in order on stream `st`, with no host synchronisation (a batch captures them into one graph).
+// With `timing`, events around the pair and tail kernels give the device times of a single run.
+static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
 {
   RunArgs &A = J.A;
   RunBufs *B = J.B;
   const int64_t nrows = J.nrows;
-  fill_zero(c->stream, B->hist.p, (size_t)NSTRIPE * 600 * 8);
-  fill_zero(c->stream, B->counters.p, STRIPE_BYTES);
-  fill_zero(c->stream, B->row_links.p, J.nr1 * 4);
-  if (J.fast) fill_zero(c->stream, B->rowout.p, J.nr1 * sizeof(RowAcc));
-  if (timing) (void)hipEventRecord(c->ev[0], c->stream);
+  fill_zero(st, B->hist.p, (size_t)NSTRIPE * 600 * 8);
+  fill_zero(st, B->counters.p, STRIPE_BYTES);
+  fill_zero(st, B->row_links.p, J.nr1 * 4);
+  if (J.fast) fill_zero(st, B->rowout.p, J.nr1 * sizeof(RowAcc));
+  if (timing) (void)hipEventRecord(c->ev[0], st);
   if (nrows > 0 && A.ncols > 0) {
     dim3 g((unsigned)J.gx, (unsigned)J.gy);
     const bool mf = J.P.backend == TN93G_BACKEND_MFMA;
     if (mf && J.fast)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, true>), g, dim3(256), 0, c->stream, A);
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, true>), g, dim3(256), 0, st, A);
     else if (mf)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, false>), g, dim3(256), 0, c->stream, A);
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, false>), g, dim3(256), 0, st, A);
     else if (J.fast)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, true>), g, dim3(256), 0, c->stream, A);
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, true>), g, dim3(256), 0, st, A);
     else
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, false>), g, dim3(256), 0, c->stream, A);
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, false>), g, dim3(256), 0, st, A);
   } else {
-    fill_zero(c->stream, B->slow_cnt.p, (size_t)J.ntiles * 4);
+    fill_zero(st, B->slow_cnt.p, (size_t)J.ntiles * 4);
   }
-  if (timing) (void)hipEventRecord(c->ev[1], c->stream);
+  if (timing) (void)hipEventRecord(c->ev[1], st);
   if (J.fast && nrows > 0 && A.ncols > 0)
-    hipLaunchKernelGGL(k_tail, dim3((unsigned)std::min<int64_t>(nrows, 65535)), dim3(256), 0, c->stream, A);
-  if (timing) (void)hipEventRecord(c->ev[3], c->stream);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)B->slow_cnt.p,
+    hipLaunchKernelGGL(k_tail, dim3((unsigned)std::min<int64_t>(nrows, 65535)), dim3(256), 0, st, A);
+  if (timing) (void)hipEventRecord(c->ev[3], st);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const int32_t *)B->slow_cnt.p,
                      (int64_t *)B->slow_off.p, J.ntiles, (int32_t *)B->slow_chunk.p);
-  hipLaunchKernelGGL(k_replay, dim3(2048), dim3(RB), 0, c->stream, A);
-  if (J.fast) hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, c->stream, A, (LinkRec *)B->links_flat.p);
-  hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, c->stream, A);
+  hipLaunchKernelGGL(k_replay, dim3(2048), dim3(RB), 0, st, A);
+  if (J.fast) hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, st, A, (LinkRec *)B->links_flat.p);
+  hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, st, A);
 }
 
 // Device-to-host copies of the job's counters and replay total into `pin` (COUNTER_STRIDE bytes).
@@ -2081,6 +2085,17 @@ int tn93g_open(int device, tn93g_ctx **out)
     return TN93G_E_NODEV;
   }
   for (auto &e : c->ev) (void)hipEventCreate(&e);
+  for (int b = 0; b < NBRANCH; b++) {
+    if (hipStreamCreateWithFlags(&c->side[b], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join[b], hipEventDisableTiming) != hipSuccess) {
+      tn93g_close(c);
+      return TN93G_E_NODEV;
+    }
+  }
+  if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess) {
+    tn93g_close(c);
+    return TN93G_E_NODEV;
+  }
   *out = c;
   return 0;
 }
@@ -2096,6 +2111,11 @@ void tn93g_close(tn93g_ctx *c)
     delete b;
   }
   for (auto &e : c->ev) (void)hipEventDestroy(e);
+  for (int b = 0; b < NBRANCH; b++) {
+    if (c->side[b]) (void)hipStreamDestroy(c->side[b]);
+    if (c->join[b]) (void)hipEventDestroy(c->join[b]);
+  }
+  if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->pin.p) (void)hipHostFree(c->pin.p);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -2246,7 +2266,7 @@ static int autotune(tn93g_ctx *c, Job &J)
   int pick = TN93G_BACKEND_POPC;
   for (int v : cand) {
     J.P.backend = v;
-    job_enqueue(c, J, true);
+    job_enqueue(c, J, true, c->stream);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
@@ -2272,7 +2292,7 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   int rc = job_prepare(c, J);
   if (rc) return rc;
   if ((rc = autotune(c, J))) return rc;
-  job_enqueue(c, J, true);
+  job_enqueue(c, J, true, c->stream);
   HIPCHK(c, hipGetLastError());
   if (ensure_pinned(c, c->pin, COUNTER_STRIDE)) return TN93G_E_NOMEM;
   if ((rc = job_enqueue_counters(c, J, (char *)c->pin.p))) return rc;
@@ -2355,8 +2375,25 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       c->gexec = nullptr;
       c->gkey.clear();
       hipGraph_t g = nullptr;
+      // the jobs are independent: spread over NBRANCH forked streams (largest first, each to the least loaded
+      // branch) so that the graph has parallel branches and small jobs overlap on the device
+      std::vector<size_t> order(m);
+      for (size_t t = 0; t < m; t++) order[t] = t;
+      auto cost = [&](const Job &J) { return (double)std::max<int64_t>(J.npairs, 1) * (double)(J.A.W + 8); };
+      std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost(jobs[x]) > cost(jobs[y]); });
+      double load[NBRANCH] = {};
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      for (Job &J : jobs) job_enqueue(c, J, false);
+      HIPCHK(c, hipEventRecord(c->fork, c->stream));
+      for (int b = 0; b < NBRANCH; b++) HIPCHK(c, hipStreamWaitEvent(c->side[b], c->fork, 0));
+      for (size_t t : order) {
+        const int b = (int)(std::min_element(load, load + NBRANCH) - load);
+        load[b] += cost(jobs[t]);
+        job_enqueue(c, jobs[t], false, c->side[b]);
+      }
+      for (int b = 0; b < NBRANCH; b++) {
+        HIPCHK(c, hipEventRecord(c->join[b], c->side[b]));
+        HIPCHK(c, hipStreamWaitEvent(c->stream, c->join[b], 0));
+      }
       const hipError_t e1 = hipStreamEndCapture(c->stream, &g);
       if (e1 != hipSuccess)
         return fail_all(c->fail(TN93G_E_HIP, fmt("hipStreamEndCapture: %s", hipGetErrorString(e1))));
