@@ -10,7 +10,8 @@ site-order replays, FP64 tail, histogram, per-row sums in tn93's order and the l
 
 Other workloads (--config): c2, the SARS-CoV-2-shaped long alignment (20,000 x 29,903, N-masked blocks,
 -t 0.0005 -a resolve -g 0.05 -l 25000); c3, 64 uneven alignments with mixed -a modes run as one batch (one graph
-launch) beside the same jobs run one by one, and the 2,000 x 40,000 -s bipartite block.
+launch) beside the same jobs run one by one, and the 2,000 x 40,000 -s bipartite block; c4, serving: a 50,000 x
+1,300 device-resident panel answering a Poisson stream of query batches (p50/p99 latency, pairs/s).
 
 Prints one JSON line.  `roofline` is for the pair kernel (HIP events on the engine's stream, averaged over the
 timed steps); `cpu_baseline` times the reference solver (oracle/_ref/swifttn93_avx512, built from the reference
@@ -223,6 +224,41 @@ def bench_c3(a, T, gen):
     print(json.dumps(line))
 
 
+def bench_c4(a, T, gen):
+    import numpy as np
+    from swifttn93_amd import serve
+    n_panel = 50000
+    panel = gen.c1(n=n_panel, clusters=n_panel * 200 // 6700)
+    stream = serve.query_stream(panel, seed=8, rate=20.0, seconds=a.serve_seconds)
+    nq = sum(c.shape[0] for _, c in stream)
+    srv = serve.Server(panel, capacity=n_panel + nq, workers=4, hbm_budget=32 << 30)
+    # warm the worker contexts (their run buffers) with the last panel rows as if they had just arrived
+    for e in srv.free:
+        e.run(srv.set, kind=T.APPEND, n1=n_panel - 512, n2=n_panel, **srv.run_kw)
+    t0 = time.perf_counter()
+    reqs = serve.serve(srv, stream)
+    wall = time.perf_counter() - t0
+    errors = [r.error for r in reqs if r.error]
+    lat = np.array([r.latency for r in reqs]) * 1e3
+    pairs = sum((r.n2 * (r.n2 - 1) - r.n1 * (r.n1 - 1)) // 2 for r in reqs)
+    busy = sum(r.done - r.started for r in reqs)
+    srv.close()
+    line = {"metric": METRIC, "value": pairs / wall, "unit": "pairs/s", "n_gpus": 1, "steps": len(reqs), "warmup": 4,
+            "ms_per_step": float(np.mean(lat)), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int32+f64", "data": "synthetic (seeded c1 generator panel, Poisson query stream, "
+                                          "swifttn93_amd/serve.py)",
+            "config": {"workload": f"serving: 50,000 x 1,300 device-resident panel, query batches of 1-512 at "
+                                   f"20/s for {a.serve_seconds:g} s, each against the panel and all earlier queries, "
+                                   "-t 0.015 -a resolve -g 0.05 -l 500, 4 workers, 32 GiB budget",
+                       "pairs": pairs, "parallelism": "1 GPU"},
+            "roofline": None,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "max": float(lat.max())},
+            "batches": len(reqs), "queries": nq, "errors": len(errors),
+            "pairs_per_busy_s": pairs / busy, "offered_pairs_per_s": pairs / a.serve_seconds}
+    print(json.dumps(line))
+
+
 METRIC = "byte-exact pairs/sec at 1 GPU vs host cores; roofline %; serving p50/p99"
 
 
@@ -235,7 +271,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (torch), the reference for the HBM byte counters")
-    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="c4: length of the query stream")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported (the multi-GPU sharding is "
@@ -248,6 +285,8 @@ def main():
         return bench_c2(a, T, gen)
     if a.config == "c3":
         return bench_c3(a, T, gen)
+    if a.config == "c4":
+        return bench_c4(a, T, gen)
 
     if a.copy_calibration:
         import torch

@@ -43,6 +43,8 @@ extern "C" {
 #define TN93G_TRIANGLE 0  /* single file: pairs i < j over the set (swifttn93.cpp:2392)            */
 #define TN93G_CROSS 1     /* -s without -m: rows [0, n1) x columns [n1, n) (swifttn93.cpp:2321-2322) */
 #define TN93G_FST 2       /* -s -m: triangle over both files, three histograms (swifttn93.cpp:2680-2686) */
+#define TN93G_APPEND 3    /* pairs i < j with j in [n1, n2): sequences appended to a set against all earlier ones
+                             and each other, exactly the pairs a triangle over [0, n2) adds to one over [0, n1)  */
 
 /* backends of the clean/clean site counts */
 #define TN93G_BACKEND_AUTO 0
@@ -66,15 +68,25 @@ const char *tn93g_error(const tn93g_ctx *ctx);
 
 /* Copies n sequences of L codes (row-major, n*L bytes, values 0..17) to the device and packs them (bit planes,
  * site-major codes, per-sequence ambiguity event lists).  Replaces Engine::prepare (swifttn93.cpp:570-1044).
- * The host keeps ownership of `codes`.  weights: n copy numbers (swifttn93.cpp:324-333), or NULL for all 1. */
+ * The host keeps ownership of `codes`.  weights: n copy numbers (swifttn93.cpp:324-333), or NULL for all 1.
+ * Equivalent to tn93g_set_create(ctx, L, n, out) followed by tn93g_set_append(*out, codes, n, weights). */
 int tn93g_set_upload(tn93g_ctx *ctx, const uint8_t *codes, int64_t n, int64_t L, const int64_t *weights,
                      tn93g_set **out);
+/* An empty set of sequences of L sites with room for `capacity` sequences, kept on the device: a reference panel
+ * that query batches are appended to.  No counterpart in the reference, which reads every input once per run. */
+int tn93g_set_create(tn93g_ctx *ctx, int64_t L, int64_t capacity, tn93g_set **out);
+/* Appends n sequences (as in tn93g_set_upload) after the set's current ones.  Runs on the set from other threads
+ * or contexts may be in flight: each run sees the set as it was when it started (TN93G_APPEND runs name their
+ * range explicitly).  Fails with TN93G_E_ARG beyond the capacity. */
+int tn93g_set_append(tn93g_set *set, const uint8_t *codes, int64_t n, const int64_t *weights);
+/* Number of sequences in the set. */
+int64_t tn93g_set_size(const tn93g_set *set);
 void tn93g_set_free(tn93g_set *set);
 
 typedef struct {
   int32_t mode;            /* TN93G_RESOLVE .. TN93G_SUBSET                                     */
   int32_t kind;            /* TN93G_TRIANGLE, TN93G_CROSS or TN93G_FST                          */
-  int64_t n1;              /* sequences of the first file (CROSS, FST); ignored for TRIANGLE    */
+  int64_t n1;              /* sequences of the first file (CROSS, FST); first appended (APPEND) */
   uint32_t subset_mask;    /* SUBSET: bit c set when code c (4..16) is resolved (-a RY ...)     */
   int32_t backend;         /* TN93G_BACKEND_*                                                   */
   double resolve_fraction; /* -g (1.0 = off)                                                    */
@@ -82,6 +94,7 @@ typedef struct {
   double threshold;        /* -t: links have min_distance <= d <= threshold                    */
   double min_distance;     /* -w (default -1e-10)                                               */
   uint32_t flags;          /* TN93G_FLAG_*                                                      */
+  int64_t n2;              /* APPEND: end of the appended range (0: the set's size)             */
 } tn93g_params;
 
 /* Result of one run.  All arrays are owned by the result.  Everything tn93 prints is here:
@@ -115,6 +128,10 @@ typedef struct {
   double pair_kernel_ms;/* device time of the pair kernel (clean/clean counts + events)   */
   double tail_kernel_ms;/* device time of the FP64 tail kernel                            */
 } tn93g_result;
+
+/* Device memory (bytes) that tn93g_run of this set and these parameters works in, beyond the set itself: what a
+ * caller admitting runs under an HBM budget reserves.  No counterpart in the reference. */
+int tn93g_run_bytes(tn93g_ctx *ctx, const tn93g_set *set, const tn93g_params *params, int64_t *bytes);
 
 /* Computes all pairs of `set` described by `params` (set must hold n1 + n2 sequences for CROSS/FST).
  * Replaces the OpenMP tile loop of swifttn93's main (swifttn93.cpp:2339-2864).  *out is allocated by the library

@@ -14,7 +14,7 @@ LIB_PATH = PKG / "lib" / "libtn93g.so"
 CLI_PATH = PKG / "bin" / "tn93g"
 
 RESOLVE, AVERAGE, SKIP, GAPMM, SUBSET = 0, 1, 2, 3, 4
-TRIANGLE, CROSS, FST = 0, 1, 2
+TRIANGLE, CROSS, FST, APPEND = 0, 1, 2, 3
 BACKEND_AUTO, BACKEND_POPC, BACKEND_MFMA = 0, 1, 2
 FLAG_EXACT, FLAG_ALL_SLOW, FLAG_AUTOTUNE = 1, 2, 4
 
@@ -25,7 +25,8 @@ class Params(ctypes.Structure):
     _fields_ = [("mode", ctypes.c_int32), ("kind", ctypes.c_int32), ("n1", ctypes.c_int64),
                 ("subset_mask", ctypes.c_uint32), ("backend", ctypes.c_int32),
                 ("resolve_fraction", ctypes.c_double), ("min_overlap", ctypes.c_int64),
-                ("threshold", ctypes.c_double), ("min_distance", ctypes.c_double), ("flags", ctypes.c_uint32)]
+                ("threshold", ctypes.c_double), ("min_distance", ctypes.c_double), ("flags", ctypes.c_uint32),
+                ("n2", ctypes.c_int64)]
 
 
 class Result(ctypes.Structure):
@@ -41,7 +42,8 @@ class Result(ctypes.Structure):
 
 
 EXPORTS = ["tn93g_open", "tn93g_close", "tn93g_error", "tn93g_set_upload", "tn93g_set_free", "tn93g_run",
-           "tn93g_result_free", "tn93g_version", "tn93g_run_batch"]
+           "tn93g_result_free", "tn93g_version", "tn93g_run_batch", "tn93g_set_create", "tn93g_set_append",
+           "tn93g_set_size", "tn93g_run_bytes"]
 
 _lib = None
 
@@ -61,6 +63,12 @@ def load():
     lib.tn93g_set_upload.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64,
                                      ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]
     lib.tn93g_set_free.argtypes = [ctypes.c_void_p]
+    lib.tn93g_set_create.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.POINTER(ctypes.c_void_p)]
+    lib.tn93g_set_append.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+    lib.tn93g_set_size.argtypes = [ctypes.c_void_p]
+    lib.tn93g_set_size.restype = ctypes.c_int64
+    lib.tn93g_run_bytes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Params),
+                                    ctypes.POINTER(ctypes.c_int64)]
     lib.tn93g_run.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Params),
                               ctypes.POINTER(ctypes.POINTER(Result))]
     lib.tn93g_result_free.argtypes = [ctypes.POINTER(Result)]
@@ -72,8 +80,9 @@ def load():
 
 
 def make_params(mode=RESOLVE, kind=TRIANGLE, n1=0, subset_mask=0, backend=BACKEND_AUTO, resolve_fraction=1.0,
-                min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0):
-    return Params(mode, kind, n1, subset_mask, backend, resolve_fraction, min_overlap, threshold, min_distance, flags)
+                min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0, n2=0):
+    return Params(mode, kind, n1, subset_mask, backend, resolve_fraction, min_overlap, threshold, min_distance, flags,
+                  n2)
 
 
 class Engine:
@@ -99,10 +108,23 @@ class Engine:
                                               None if w is None else w.ctypes.data, ctypes.byref(s)))
         return SeqSet(self, s, n, L)
 
+    def create(self, L, capacity):
+        """An empty set of sequences of L sites with room for `capacity` sequences (tn93g_set_create)."""
+        s = ctypes.c_void_p()
+        self._check(self.lib.tn93g_set_create(self.ctx, L, capacity, ctypes.byref(s)))
+        return SeqSet(self, s, 0, L)
+
+    def run_bytes(self, seqset, **kw):
+        """Device bytes a run of these parameters works in (tn93g_run_bytes)."""
+        p = make_params(**kw)
+        b = ctypes.c_int64()
+        self._check(self.lib.tn93g_run_bytes(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(b)))
+        return b.value
+
     def run(self, seqset, mode=RESOLVE, kind=TRIANGLE, n1=0, subset_mask=0, backend=BACKEND_AUTO,
-            resolve_fraction=1.0, min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0):
+            resolve_fraction=1.0, min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0, n2=0):
         p = make_params(mode, kind, n1, subset_mask, backend, resolve_fraction, min_overlap, threshold, min_distance,
-                        flags)
+                        flags, n2)
         r = ctypes.POINTER(Result)()
         self._check(self.lib.tn93g_run(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r)))
         try:
@@ -139,6 +161,23 @@ class Engine:
 class SeqSet:
     def __init__(self, eng, handle, n, L):
         self.eng, self.handle, self.n, self.L = eng, handle, n, L
+
+    def append(self, codes, weights=None):
+        """Appends sequences (tn93g_set_append); returns the index of the first one."""
+        codes = np.ascontiguousarray(codes, dtype=np.uint8)
+        n, L = codes.shape
+        assert L == self.L
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.int64)
+        lib = self.eng.lib
+        first = lib.tn93g_set_size(self.handle)
+        rc = lib.tn93g_set_append(self.handle, codes.ctypes.data, n, None if w is None else w.ctypes.data)
+        if rc:
+            raise RuntimeError(f"tn93g_set_append failed ({rc}): {lib.tn93g_error(self.eng.ctx).decode()}")
+        self.n = lib.tn93g_set_size(self.handle)
+        return first
+
+    def size(self):
+        return self.eng.lib.tn93g_set_size(self.handle)
 
     def free(self):
         if self.handle:

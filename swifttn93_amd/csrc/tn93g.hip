@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <atomic>
 #include <cmath>
 #include <cstdarg>
@@ -174,7 +175,6 @@ struct SetDev {
   int64_t n, L, Ls, npad, W, W64;
   uint8_t *codes, *codesT;
   uint4 *planes;
-  uint32_t *ambp;
   uint8_t *onehot;
   int32_t *ev_off;
   uint32_t *ev;
@@ -217,7 +217,6 @@ struct RunArgs;
 struct RunArgs {
   // set
   const uint4 *planes;
-  const uint32_t *ambp;
   const uint8_t *onehot;
   const uint8_t *codes, *codesT;
   const int32_t *ev_off;
@@ -413,10 +412,10 @@ __device__ inline long long wave_sum_l(long long x)
 
 // ------------------------------------------------------------------------------------------ pack kernels
 __global__ void k_transpose(const uint8_t *__restrict__ codes, uint8_t *__restrict__ codesT, int64_t n, int64_t Ls,
-                            int64_t npad)
+                            int64_t npad, int64_t sbase)
 {
   __shared__ uint8_t t[64][65];
-  const int64_t s0 = (int64_t)blockIdx.y * 64, p0 = (int64_t)blockIdx.x * 64;
+  const int64_t s0 = sbase + (int64_t)blockIdx.y * 64, p0 = (int64_t)blockIdx.x * 64;
   for (int k = threadIdx.x; k < 64 * 64; k += blockDim.x) {
     int r = k / 64, c = k % 64;
     int64_t s = s0 + r, p = p0 + c;
@@ -432,13 +431,13 @@ __global__ void k_transpose(const uint8_t *__restrict__ codes, uint8_t *__restri
 
 // planes: per 32 sites, bit k of .x = site clean, .y = clean and bit0 of the code (C, T), .z = clean and bit1
 // (G, T), .w = code T
-__global__ void k_planes(const uint8_t *__restrict__ codes, uint4 *__restrict__ planes, uint32_t *__restrict__ ambp,
-                         int64_t n, int64_t L, int64_t Ls, int64_t W)
+__global__ void k_planes(const uint8_t *__restrict__ codes, uint4 *__restrict__ planes, int64_t s0, int64_t s1,
+                         int64_t L, int64_t Ls, int64_t W)
 {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n * W) return;
-  int64_t s = t / W, w = t % W;
-  uint32_t c = 0, h = 0, g = 0, tt = 0, am = 0;
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t0 >= (s1 - s0) * W) return;
+  const int64_t s = s0 + t0 / W, w = t0 % W, t = s * W + w;
+  uint32_t c = 0, h = 0, g = 0, tt = 0;
   for (int k = 0; k < 32; k++) {
     int64_t p = w * 32 + k;
     if (p >= L) break;
@@ -448,21 +447,18 @@ __global__ void k_planes(const uint8_t *__restrict__ codes, uint4 *__restrict__ 
       h |= (code & 1u) << k;
       g |= (code >> 1 & 1u) << k;
       tt |= (code == 3 ? 1u : 0u) << k;
-    } else if (code < (unsigned)GAPC) {
-      am |= 1u << k;
     }
   }
   planes[t] = make_uint4(c, h, g, tt);
-  ambp[t] = am;
 }
 
 // one-hot bytes for the MFMA backend, in operand order (see the layout above); sequences >= n are all zero
 __global__ void k_onehot(const uint8_t *__restrict__ codes, uint8_t *__restrict__ onehot, int64_t n, int64_t L,
-                         int64_t Ls, int64_t W64, int64_t npad)
+                         int64_t Ls, int64_t W64, int64_t sbase, int64_t send)
 {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one thread per (sequence, site)
-  if (t >= npad * W64 * 64) return;
-  int64_t s = t / (W64 * 64), p = t % (W64 * 64);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one thread per (sequence, site)
+  if (t >= (send - sbase) * W64 * 64) return;
+  const int64_t s = sbase + t / (W64 * 64), p = t % (W64 * 64);
   unsigned code = (s < n && p < L) ? codes[s * Ls + p] : (unsigned)GAPC;
   int64_t g = s >> 2, sl = s & 3, blk = p >> 6, kg = (p >> 4) & 3, m = p & 15;
   uint8_t *o = onehot + (((g * W64 + blk) * 4 + kg) * 16 + 4 * sl) * 16 + m;
@@ -474,20 +470,20 @@ __global__ void k_onehot(const uint8_t *__restrict__ codes, uint8_t *__restrict_
 __device__ inline bool is_event(unsigned code, int gaps) { return code >= 4 && (code < (unsigned)GAPC || gaps); }
 
 __global__ void k_evcount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls,
-                          int gaps)
+                          int gaps, int64_t s0)
 {
-  const int64_t s = blockIdx.x;
+  const int64_t s = s0 + blockIdx.x;
   int c = 0;
   for (int64_t p = threadIdx.x; p < L; p += blockDim.x) c += is_event(codes[s * Ls + p], gaps) ? 1 : 0;
   for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
-  if (threadIdx.x == 0) cnt[s] = c;
+  if (threadIdx.x == 0) cnt[blockIdx.x] = c;
 }
 
 // events in site order: one wave per sequence
 __global__ void k_evfill(const uint8_t *__restrict__ codes, const int32_t *__restrict__ ev_off,
-                         uint32_t *__restrict__ ev, int64_t L, int64_t Ls, int gaps)
+                         uint32_t *__restrict__ ev, int64_t L, int64_t Ls, int gaps, int64_t s0)
 {
-  const int64_t s = blockIdx.x;
+  const int64_t s = s0 + blockIdx.x;
   const int lane = threadIdx.x;
   int base = ev_off[s];
   for (int64_t p0 = 0; p0 < L; p0 += 64) {
@@ -1453,10 +1449,18 @@ std::string fmt(const char *f, ...)
 
 }  // namespace
 
+// A set of sequences on the device, sized for `cap` sequences; appends fill it in order.  Runs take a snapshot of
+// (d, n) under the mutex and only look at the first n sequences, so a set may grow while runs on it are in flight.
+// Event arrays that must grow are replaced, and the old ones kept until the set is freed (in-flight runs may still
+// read them; their prefix is unchanged).
 struct tn93g_set {
-  tn93g_ctx *ctx;
-  SetDev d;
-  std::vector<int64_t> hw;  // copy numbers, host copy
+  tn93g_ctx *ctx = nullptr;
+  SetDev d{};
+  int64_t cap = 0;
+  int64_t ev_total[2] = {0, 0}, ev_cap[2] = {0, 0};  // events used / allocated: ev (ambiguity), evg (with gaps)
+  std::vector<int64_t> hw;                            // copy numbers (host), cap entries
+  std::vector<void *> retired;                        // replaced event arrays
+  std::mutex mu;
 };
 
 static double tn93g_host_from_bits(unsigned long long u)
@@ -1545,10 +1549,11 @@ static int ensure_pinned(tn93g_ctx *c, HostBuf &b, size_t bytes)
 static int ensure(tn93g_ctx *c, DevBuf &b, size_t bytes)
 {
   if (b.n >= bytes && b.p) return 0;
+  // a buffer that has to grow grows by at least a quarter, so that runs on a growing set reallocate rarely
+  size_t want = std::max({bytes, (size_t)256, b.n + b.n / 4});
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.n = 0;
-  size_t want = std::max(bytes, (size_t)256);
   hipError_t e = hipMalloc(&b.p, want);
   if (e != hipSuccess) return c->fail(TN93G_E_NOMEM, fmt("hipMalloc(%zu): %s", want, hipGetErrorString(e)));
   b.n = want;
@@ -1622,18 +1627,34 @@ struct Job {
 };
 
 // Checks the parameters, uploads the job's event tables and sizes its buffers: everything before its kernels.
-static int job_prepare(tn93g_ctx *c, Job &J)
+static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
 {
-  const tn93g_set *S = J.S;
+  // with dry_bytes: only the sizes of the run's buffers are added up, nothing is allocated or uploaded
+  auto need = [&](DevBuf &b, size_t bytes) -> int {
+    if (dry_bytes) {
+      *dry_bytes += std::max(bytes, (size_t)256);
+      return 0;
+    }
+    return ensure(c, b, bytes);
+  };
+  tn93g_set *S = const_cast<tn93g_set *>(J.S);
   const tn93g_params *P = &J.P;
-  const SetDev &d = S->d;
+  SetDev d;
+  {
+    std::lock_guard<std::mutex> lk(S->mu);
+    d = S->d;
+  }
   RunBufs *B = J.B;
-  if (P->mode < 0 || P->mode > 4 || P->kind < 0 || P->kind > 2) return c->fail(TN93G_E_ARG, "bad mode or kind");
-  if (P->kind != TN93G_TRIANGLE && (P->n1 < 0 || P->n1 > d.n)) return c->fail(TN93G_E_ARG, "bad n1");
+  if (P->mode < 0 || P->mode > 4 || P->kind < 0 || P->kind > 3) return c->fail(TN93G_E_ARG, "bad mode or kind");
+  if (P->kind == TN93G_APPEND) {
+    const int64_t end = P->n2 ? P->n2 : d.n;
+    if (P->n1 < 0 || P->n1 > end || end > d.n) return c->fail(TN93G_E_ARG, "bad appended range n1..n2");
+    d.n = end;  // the run sees the set as it was when [n1, n2) had just been appended
+  } else if (P->kind != TN93G_TRIANGLE && (P->n1 < 0 || P->n1 > d.n))
+    return c->fail(TN93G_E_ARG, "bad n1");
   RunArgs &A = J.A;
   memset(&A, 0, sizeof A);
   A.planes = d.planes;
-  A.ambp = d.ambp;
   A.onehot = d.onehot;
   A.codes = d.codes;
   A.codesT = d.codesT;
@@ -1651,6 +1672,12 @@ static int job_prepare(tn93g_ctx *c, Job &J)
     A.c0 = P->n1;
     A.c1 = d.n;
     A.triangle = 0;
+  } else if (P->kind == TN93G_APPEND) {  // pairs i < j with j in [n1, n): rows [0, n), columns [n1, n)
+    A.r0 = 0;
+    A.r1 = d.n;
+    A.c0 = P->n1;
+    A.c1 = d.n;
+    A.triangle = 1;
   } else {
     A.r0 = 0;
     A.r1 = d.n;
@@ -1659,7 +1686,7 @@ static int job_prepare(tn93g_ctx *c, Job &J)
     A.triangle = 1;
   }
   A.fst = P->kind == TN93G_FST;
-  A.n1 = P->kind == TN93G_TRIANGLE ? d.n : P->n1;
+  A.n1 = (P->kind == TN93G_TRIANGLE || P->kind == TN93G_APPEND) ? d.n : P->n1;
   A.ncols = A.c1 - A.c0;
   const int64_t nrows = A.r1 - A.r0;
   J.nrows = nrows;
@@ -1674,10 +1701,10 @@ static int job_prepare(tn93g_ctx *c, Job &J)
   A.window = TN93G_LOG_WINDOW;
   A.min_overlap = P->min_overlap;
   const int64_t npairs_box = nrows * A.ncols;
-  J.npairs = A.triangle ? d.n * (d.n - 1) / 2 : npairs_box;
+  J.npairs = A.triangle ? (d.n * (d.n - 1) - A.c0 * (A.c0 - 1)) / 2 : npairs_box;
   if (J.npairs >= (1ll << 32)) return c->fail(TN93G_E_ARG, "more than 2^32 pairs in one run");
   // tables: [2][18*18] packed event entries (main mode, AVERAGE), [2][18*18] replay entries
-  if (B->tab_mode != P->mode || B->tab_mask != P->subset_mask || !B->evtab.p || !B->rtab.p) {
+  if (!dry_bytes && (B->tab_mode != P->mode || B->tab_mask != P->subset_mask || !B->evtab.p || !B->rtab.p)) {
     std::vector<uint4> et(2 * NCODE * NCODE);
     std::vector<ReplayEnt> rt(2 * NCODE * NCODE);
     for (int a = 0; a < NCODE; a++)
@@ -1695,7 +1722,7 @@ static int job_prepare(tn93g_ctx *c, Job &J)
           r.amb = (uint16_t)F.amb;
         }
       }
-    if (ensure(c, B->evtab, et.size() * sizeof(uint4)) || ensure(c, B->rtab, rt.size() * sizeof(ReplayEnt)))
+    if (need(B->evtab, et.size() * sizeof(uint4)) || need(B->rtab, rt.size() * sizeof(ReplayEnt)))
       return TN93G_E_NOMEM;
     HIPCHK(c, hipMemcpyAsync(B->evtab.p, et.data(), et.size() * sizeof(uint4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(B->rtab.p, rt.data(), rt.size() * sizeof(ReplayEnt), hipMemcpyHostToDevice, c->stream));
@@ -1716,25 +1743,25 @@ static int job_prepare(tn93g_ctx *c, Job &J)
   J.ntiles = J.gx * J.gy;
   const int64_t ntiles = J.ntiles, nslots = J.fast ? (int64_t)nr1 * J.gx : 1;
   if (J.gx >= (1ll << 31) || J.gy >= 65536) return c->fail(TN93G_E_ARG, "too many pair tiles in one run");
-  if ((!J.fast && (ensure(c, B->D, nbox * 8) || ensure(c, B->U, nbox * 2))) ||
-      ensure(c, B->hist, (size_t)NSTRIPE * 600 * 8) || ensure(c, B->row_links, nr1 * 4) ||
-      ensure(c, B->link_off, (size_t)(nrows + 1) * 8) ||
-      ensure(c, B->slow_list, (size_t)ntiles * TI * TJ * 8) || ensure(c, B->slow_cnt, (size_t)ntiles * 4) ||
-      ensure(c, B->slow_off, (size_t)(ntiles + 1) * 8) || ensure(c, B->slow_chunk, (size_t)(16 * ntiles + 1) * 4) ||
-      ensure(c, B->counters, STRIPE_BYTES) ||
-      ensure(c, B->rowout, nr1 * std::max<size_t>(15 * 8 + 16, sizeof(RowAcc))) ||
-      ensure(c, B->chdr, (size_t)nslots * sizeof(ChunkHdr)) ||
-      ensure(c, B->cdata, (size_t)nslots * 7 * 64 * sizeof(uint32_t)))
+  if ((!J.fast && (need(B->D, nbox * 8) || need(B->U, nbox * 2))) ||
+      need(B->hist, (size_t)NSTRIPE * 600 * 8) || need(B->row_links, nr1 * 4) ||
+      need(B->link_off, (size_t)(nrows + 1) * 8) ||
+      need(B->slow_list, (size_t)ntiles * TI * TJ * 8) || need(B->slow_cnt, (size_t)ntiles * 4) ||
+      need(B->slow_off, (size_t)(ntiles + 1) * 8) || need(B->slow_chunk, (size_t)(16 * ntiles + 1) * 4) ||
+      need(B->counters, STRIPE_BYTES) ||
+      need(B->rowout, nr1 * std::max<size_t>(15 * 8 + 16, sizeof(RowAcc))) ||
+      need(B->chdr, (size_t)nslots * sizeof(ChunkHdr)) ||
+      need(B->cdata, (size_t)nslots * 7 * 64 * sizeof(uint32_t)))
     return TN93G_E_NOMEM;
   J.exact_cap = A.exact ? (unsigned)std::min<int64_t>(J.npairs, 1ll << 31) : 1u;
-  if (ensure(c, B->exact_list, (size_t)J.exact_cap * sizeof(ExactRec))) return TN93G_E_NOMEM;
+  if (need(B->exact_list, (size_t)J.exact_cap * sizeof(ExactRec))) return TN93G_E_NOMEM;
   J.link_cap = 1;
   if (J.fast) {
     // links per stripe region; a run that overflows a region is repeated with larger regions
     const int64_t want = std::max<int64_t>(B->link_cap_hint, 1 << 15);
     J.link_cap = (unsigned)std::min<int64_t>(std::max<int64_t>(J.npairs, 1), want);
-    if (ensure(c, B->links, (size_t)J.link_cap * NSTRIPE * sizeof(LinkRec)) ||
-        ensure(c, B->links_flat, (size_t)J.link_cap * NSTRIPE * sizeof(LinkRec)))
+    if (need(B->links, (size_t)J.link_cap * NSTRIPE * sizeof(LinkRec)) ||
+        need(B->links_flat, (size_t)J.link_cap * NSTRIPE * sizeof(LinkRec)))
       return TN93G_E_NOMEM;
   }
   A.D = (double *)B->D.p;
@@ -1857,7 +1884,7 @@ static void job_finish_common(const Job &J, const unsigned long long *hrep, tn93
   R->n_host_exact = J.n_exact;
   // self_v_self histogram entries (swifttn93.cpp:2406-2415)
   if (J.A.triangle)
-    for (int64_t i = 0; i < J.S->d.n; i++) {
+    for (int64_t i = J.A.c0; i < J.A.n; i++) {
       const int64_t wi = J.S->hw[(size_t)i];
       if (wi > 1) R->hist[(J.A.fst && i >= J.A.n1) ? 1 : 0][0] += (unsigned long long)((wi * (wi - 1)) >> 1);
     }
@@ -1918,7 +1945,7 @@ static int job_fast_result(const Job &J, const char *pin, tn93g_result *R)
   for (int64_t r = 0; r < nrows; r++) {
     const RowAcc &q = ra[(size_t)r];
     const int64_t i = r0 + r;
-    const long double n_r = (long double)(J.A.triangle ? (J.A.c1 - i - 1) : J.A.ncols);
+    const long double n_r = (long double)(J.A.triangle ? (J.A.c1 - std::max(J.A.c0, i + 1)) : J.A.ncols);
     const long double g = gam(std::max<long double>(n_r, 1));
     const long double sabs = (long double)q.sabs * (1.0L + g);
     const long double e = 2.0L * g * sabs;
@@ -1942,7 +1969,7 @@ static int job_fast_result(const Job &J, const char *pin, tn93g_result *R)
   R->found_links = linkw;
   long long w_all = wsum;
   if (J.A.triangle) {
-    for (int64_t i = 0; i < J.S->d.n; i++) {
+    for (int64_t i = J.A.c0; i < J.A.n; i++) {
       const int64_t wi = J.S->hw[(size_t)i];
       if (wi > 1) w_all += (wi * (wi - 1)) >> 1;
     }
@@ -2123,31 +2150,24 @@ void tn93g_close(tn93g_ctx *c)
 
 const char *tn93g_error(const tn93g_ctx *c) { return c ? c->err.c_str() : "no context"; }
 
-int tn93g_set_upload(tn93g_ctx *c, const uint8_t *codes, int64_t n, int64_t L, const int64_t *weights,
-                     tn93g_set **out)
+int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
 {
-  if (!c || !out || (n > 0 && !codes) || n < 0 || L <= 0 || n >= (1ll << 31) || L >= (1ll << 26))
-    return TN93G_E_ARG;
-  for (int64_t k = 0; k < n * L; k++)
-    if (codes[k] > GAPC) return c->fail(TN93G_E_ARG, "codes must be ValidChars indices 0..17");
+  if (!c || !out || capacity < 0 || L <= 0 || capacity >= (1ll << 31) || L >= (1ll << 26)) return TN93G_E_ARG;
+  *out = nullptr;
   HIPCHK(c, hipSetDevice(c->device));
   tn93g_set *S = new tn93g_set();
   S->ctx = c;
+  S->cap = capacity;
   SetDev &d = S->d;
   memset(&d, 0, sizeof d);
-  d.n = n;
+  d.n = 0;
   d.L = L;
   d.Ls = (L + 15) / 16 * 16;
-  d.npad = (n + 63) / 64 * 64 + 64;
+  d.npad = (capacity + 63) / 64 * 64 + 64;
   d.W = (L + 31) / 32;
   d.W64 = (L + 63) / 64;
-  const int64_t nn = std::max<int64_t>(n, 1);
-  std::vector<uint8_t> padded((size_t)(nn * d.Ls), (uint8_t)GAPC);
-  for (int64_t s = 0; s < n; s++) memcpy(&padded[(size_t)(s * d.Ls)], codes + s * L, (size_t)L);
-  std::vector<int64_t> wts((size_t)nn, 1);
-  if (weights)
-    for (int64_t s = 0; s < n; s++) wts[(size_t)s] = weights[s];
-  S->hw.assign(wts.begin(), wts.begin() + n);
+  const int64_t nn = std::max<int64_t>(capacity, 1);
+  S->hw.assign((size_t)nn, 1);
   auto fail = [&](int code, const std::string &m) {
     tn93g_set_free(S);
     return c->fail(code, m);
@@ -2157,57 +2177,138 @@ int tn93g_set_upload(tn93g_ctx *c, const uint8_t *codes, int64_t n, int64_t L, c
     hipError_t e_ = (x);                                                                     \
     if (e_ != hipSuccess) return fail(TN93G_E_HIP, fmt("%s: %s", #x, hipGetErrorString(e_))); \
   } while (0)
-  SCHK(hipMalloc(&d.codes, padded.size()));
+  SCHK(hipMalloc(&d.codes, (size_t)(nn * d.Ls)));
   SCHK(hipMalloc(&d.codesT, (size_t)(d.Ls * d.npad)));
   SCHK(hipMalloc(&d.planes, (size_t)(nn * d.W) * sizeof(uint4)));
-  SCHK(hipMalloc(&d.ambp, (size_t)(nn * d.W) * sizeof(uint32_t)));
   SCHK(hipMalloc(&d.onehot, (size_t)(d.npad * d.W64 * 256)));
+  SCHK(hipMalloc(&d.ev_off, (size_t)(nn + 1) * sizeof(int32_t)));
+  SCHK(hipMalloc(&d.evg_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.weights, (size_t)nn * sizeof(int64_t)));
-  SCHK(hipMemcpyAsync(d.codes, padded.data(), padded.size(), hipMemcpyHostToDevice, c->stream));
-  SCHK(hipMemcpyAsync(d.weights, wts.data(), wts.size() * sizeof(int64_t), hipMemcpyHostToDevice, c->stream));
-  {
-    dim3 g((unsigned)((d.Ls + 63) / 64), (unsigned)((d.npad + 63) / 64));
-    hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, c->stream, d.codes, d.codesT, n, d.Ls, d.npad);
-    int64_t tot = n * d.W;
-    if (tot > 0)
-      hipLaunchKernelGGL(k_planes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.planes,
-                         d.ambp, n, L, d.Ls, d.W);
-    int64_t toh = d.npad * d.W64 * 64;
-    hipLaunchKernelGGL(k_onehot, dim3((unsigned)((toh + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.onehot, n,
-                       L, d.Ls, d.W64, d.npad);
-  }
-  int32_t *cnt_d = nullptr;
-  SCHK(hipMalloc(&cnt_d, (size_t)nn * sizeof(int32_t)));
-  for (int gaps = 0; gaps < 2; gaps++) {
-    int32_t *&eoff = gaps ? d.evg_off : d.ev_off;
-    uint32_t *&ev = gaps ? d.evg : d.ev;
-    SCHK(hipMalloc(&eoff, (size_t)(nn + 1) * sizeof(int32_t)));
-    if (n > 0)
-      hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, gaps);
-    std::vector<int32_t> cnt((size_t)nn, 0), off((size_t)nn + 1, 0);
-    SCHK(hipMemcpyAsync(cnt.data(), cnt_d, (size_t)nn * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
-    SCHK(hipStreamSynchronize(c->stream));
-    int64_t total = 0;
-    for (int64_t s = 0; s < n; s++) {
-      off[(size_t)s] = (int32_t)total;
-      total += cnt[(size_t)s];
-      if (total >= (1ll << 31)) {
-        (void)hipFree(cnt_d);
-        return fail(TN93G_E_ARG, "too many ambiguity events");
-      }
-    }
-    off[(size_t)n] = (int32_t)total;
-    if (!gaps) d.total_events = total;
-    SCHK(hipMalloc(&ev, (size_t)std::max<int64_t>(total, 1) * sizeof(uint32_t)));
-    SCHK(hipMemcpyAsync(eoff, off.data(), off.size() * sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
-    if (n > 0)
-      hipLaunchKernelGGL(k_evfill, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, eoff, ev, L, d.Ls, gaps);
-    SCHK(hipStreamSynchronize(c->stream));  // `off` is pageable host memory
-  }
-  (void)hipFree(cnt_d);
-  SCHK(hipGetLastError());
+  // padding reads as gaps (codes, site-major codes) and as nothing (planes, one-hot operands, event offsets)
+  SCHK(hipMemsetAsync(d.codes, GAPC, (size_t)(nn * d.Ls), c->stream));
+  SCHK(hipMemsetAsync(d.codesT, GAPC, (size_t)(d.Ls * d.npad), c->stream));
+  SCHK(hipMemsetAsync(d.planes, 0, (size_t)(nn * d.W) * sizeof(uint4), c->stream));
+  SCHK(hipMemsetAsync(d.onehot, 0, (size_t)(d.npad * d.W64 * 256), c->stream));
+  SCHK(hipMemsetAsync(d.ev_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
+  SCHK(hipMemsetAsync(d.evg_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
   SCHK(hipStreamSynchronize(c->stream));
 #undef SCHK
+  *out = S;
+  return 0;
+}
+
+int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_t *weights)
+{
+  if (!S || n < 0 || (n > 0 && !codes)) return TN93G_E_ARG;
+  tn93g_ctx *c = S->ctx;
+  std::lock_guard<std::mutex> lk(S->mu);
+  SetDev &d = S->d;
+  const int64_t L = d.L, n0 = d.n;
+  if (n0 + n > S->cap) return c->fail(TN93G_E_ARG, "set capacity exceeded");
+  if (n == 0) return 0;
+  for (int64_t k = 0; k < n * L; k++)
+    if (codes[k] > GAPC) return c->fail(TN93G_E_ARG, "codes must be ValidChars indices 0..17");
+  HIPCHK(c, hipSetDevice(c->device));
+  std::vector<uint8_t> padded((size_t)(n * d.Ls), (uint8_t)GAPC);
+  for (int64_t s = 0; s < n; s++) memcpy(&padded[(size_t)(s * d.Ls)], codes + s * L, (size_t)L);
+  std::vector<int64_t> wts((size_t)n, 1);
+  if (weights)
+    for (int64_t s = 0; s < n; s++) wts[(size_t)s] = weights[s];
+  HIPCHK(c, hipMemcpyAsync(d.codes + n0 * d.Ls, padded.data(), padded.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.weights + n0, wts.data(), wts.size() * sizeof(int64_t), hipMemcpyHostToDevice,
+                           c->stream));
+  const int64_t n1 = n0 + n;
+  {
+    const int64_t sb = n0 & ~(int64_t)63;
+    dim3 g((unsigned)((d.Ls + 63) / 64), (unsigned)((n1 - sb + 63) / 64));
+    hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, c->stream, d.codes, d.codesT, n1, d.Ls, d.npad, sb);
+    const int64_t tot = n * d.W;
+    hipLaunchKernelGGL(k_planes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.planes, n0,
+                       n1, L, d.Ls, d.W);
+    const int64_t ob = n0 & ~(int64_t)3, oe = (n1 + 3) & ~(int64_t)3;
+    const int64_t toh = (oe - ob) * d.W64 * 64;
+    hipLaunchKernelGGL(k_onehot, dim3((unsigned)((toh + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.onehot, n1,
+                       L, d.Ls, d.W64, ob, oe);
+  }
+  int32_t *cnt_d = nullptr;
+  HIPCHK(c, hipMalloc(&cnt_d, (size_t)n * sizeof(int32_t)));
+  int rc = 0;
+  for (int gaps = 0; gaps < 2 && !rc; gaps++) {
+    int32_t *eoff = gaps ? d.evg_off : d.ev_off;
+    uint32_t *&ev = gaps ? d.evg : d.ev;
+    hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, gaps, n0);
+    std::vector<int32_t> cnt((size_t)n, 0), off((size_t)n, 0);
+    if (hipMemcpyAsync(cnt.data(), cnt_d, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+      rc = c->fail(TN93G_E_HIP, "event count read-back failed");
+      break;
+    }
+    int64_t total = S->ev_total[gaps];
+    for (int64_t s = 0; s < n; s++) {
+      total += cnt[(size_t)s];
+      off[(size_t)s] = (int32_t)total;  // ev_off[n0 + 1 + s]
+    }
+    if (total >= (1ll << 31)) {
+      rc = c->fail(TN93G_E_ARG, "too many ambiguity events");
+      break;
+    }
+    if (total > S->ev_cap[gaps] || !ev) {
+      // grow: a new array with the old events copied; the old one stays valid for runs in flight
+      const int64_t want = std::max<int64_t>({total, 2 * S->ev_cap[gaps], 1024});
+      uint32_t *nev = nullptr;
+      if (hipMalloc(&nev, (size_t)want * sizeof(uint32_t)) != hipSuccess) {
+        rc = c->fail(TN93G_E_NOMEM, "event array allocation failed");
+        break;
+      }
+      if (S->ev_total[gaps])
+        (void)hipMemcpyAsync(nev, ev, (size_t)S->ev_total[gaps] * sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                             c->stream);
+      if (ev) S->retired.push_back(ev);
+      ev = nev;
+      S->ev_cap[gaps] = want;
+    }
+    if (hipMemcpyAsync(eoff + n0 + 1, off.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream) !=
+        hipSuccess) {
+      rc = c->fail(TN93G_E_HIP, "event offsets upload failed");
+      break;
+    }
+    hipLaunchKernelGGL(k_evfill, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, eoff, ev, L, d.Ls, gaps, n0);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {  // `off` is pageable host memory
+      rc = c->fail(TN93G_E_HIP, "event packing failed");
+      break;
+    }
+    S->ev_total[gaps] = total;
+  }
+  (void)hipFree(cnt_d);
+  if (rc) return rc;
+  HIPCHK(c, hipGetLastError());
+  for (int64_t s = 0; s < n; s++) S->hw[(size_t)(n0 + s)] = wts[(size_t)s];
+  d.total_events = S->ev_total[0];
+  d.n = n1;  // published: runs that start from now on see the new sequences
+  return 0;
+}
+
+int64_t tn93g_set_size(const tn93g_set *S)
+{
+  if (!S) return 0;
+  std::lock_guard<std::mutex> lk(const_cast<tn93g_set *>(S)->mu);
+  return S->d.n;
+}
+
+int tn93g_set_upload(tn93g_ctx *c, const uint8_t *codes, int64_t n, int64_t L, const int64_t *weights,
+                     tn93g_set **out)
+{
+  if (!c || !out || (n > 0 && !codes) || n < 0 || L <= 0 || n >= (1ll << 31) || L >= (1ll << 26))
+    return TN93G_E_ARG;
+  *out = nullptr;
+  tn93g_set *S = nullptr;
+  int rc = tn93g_set_create(c, L, n, &S);
+  if (rc) return rc;
+  if ((rc = tn93g_set_append(S, codes, n, weights))) {
+    tn93g_set_free(S);
+    return rc;
+  }
   *out = S;
   return 0;
 }
@@ -2216,11 +2317,12 @@ void tn93g_set_free(tn93g_set *S)
 {
   if (!S) return;
   (void)hipSetDevice(S->ctx->device);
-  (void)hipStreamSynchronize(S->ctx->stream);
-  void *ps[] = {S->d.codes,  S->d.codesT, S->d.planes,  S->d.ambp,    S->d.onehot,
-                S->d.ev_off, S->d.ev,     S->d.evg_off, S->d.evg, S->d.weights};
+  (void)hipDeviceSynchronize();  // runs of any context may still read the set
+  void *ps[] = {S->d.codes, S->d.codesT, S->d.planes, S->d.onehot, S->d.ev_off,
+                S->d.ev,    S->d.evg_off, S->d.evg,   S->d.weights};
   for (void *p : ps)
     if (p) (void)hipFree(p);
+  for (void *p : S->retired) (void)hipFree(p);
   delete S;
 }
 
@@ -2277,6 +2379,20 @@ static int autotune(tn93g_ctx *c, Job &J)
   }
   c->tuned[key] = pick;
   J.P.backend = pick;
+  return 0;
+}
+
+int tn93g_run_bytes(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, int64_t *bytes)
+{
+  if (!c || !S || !P || !bytes) return TN93G_E_ARG;
+  Job J;
+  J.S = S;
+  J.P = *P;
+  J.B = c->slot(0);
+  size_t total = 0;
+  const int rc = job_prepare(c, J, &total);
+  if (rc) return rc;
+  *bytes = (int64_t)total;
   return 0;
 }
 

@@ -1,0 +1,158 @@
+"""Serving: a device-resident reference panel answering query batches, with continuous batching.
+
+A query batch (1-512 sequences) is appended to the panel set and scored against every earlier sequence -- the panel,
+all earlier queries, and the earlier queries of its own batch -- by one TN93G_APPEND run over its index range.
+Those are exactly the pairs a tn93 all-pairs run over everything received so far adds for the batch, each computed
+with the earlier sequence first as tn93 does (rows before columns), so a batch's result does not depend on what
+else is running.
+
+Admission: a batch waits until a worker is free and the device memory of the runs in flight plus its own
+(tn93g_run_bytes) fits in the HBM budget.  Each worker thread owns a context (its own stream and run buffers), so
+admitted runs overlap on the device while later batches are appended and admitted.  ctypes releases the GIL during
+the library calls.
+"""
+import collections
+import threading
+import time
+
+import numpy as np
+
+from . import APPEND, RESOLVE, Engine
+from . import gen
+
+
+class Request:
+    def __init__(self, rid, codes, arrival):
+        self.rid, self.codes, self.arrival = rid, codes, arrival
+        self.n1 = self.n2 = None  # index range in the panel set after the append
+        self.bytes = 0
+        self.started = self.done = None
+        self.result = None
+        self.error = None
+        self.event = threading.Event()
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+class Server:
+    """Panel set on device 0; `workers` contexts run the admitted batches concurrently."""
+
+    def __init__(self, panel, capacity, workers=4, hbm_budget=32 << 30, run_kw=None):
+        panel = np.ascontiguousarray(panel, dtype=np.uint8)
+        self.L = panel.shape[1]
+        self.owner = Engine(0)
+        self.set = self.owner.create(self.L, capacity)
+        self.set.append(panel)
+        self.run_kw = dict(mode=RESOLVE, threshold=0.015, min_overlap=500, resolve_fraction=0.05)
+        if run_kw:
+            self.run_kw.update(run_kw)
+        self.budget = hbm_budget
+        self.in_flight = 0
+        self.lock = threading.Condition()
+        self.pending = collections.deque()
+        self.free = [Engine(0) for _ in range(workers)]
+        self.stopping = False
+        self.threads = []
+        self.admitter = threading.Thread(target=self._admit, daemon=True)
+        self.admitter.start()
+
+    # -- arrival: append in arrival order (this defines "earlier"), then queue for admission
+    def submit(self, rid, codes, arrival=None):
+        req = Request(rid, np.ascontiguousarray(codes, dtype=np.uint8), time.perf_counter() if arrival is None
+                      else arrival)
+        with self.lock:
+            req.n1 = self.set.append(req.codes)
+            req.n2 = req.n1 + req.codes.shape[0]
+            req.bytes = self.owner.run_bytes(self.set, kind=APPEND, n1=req.n1, n2=req.n2, **self.run_kw)
+            self.pending.append(req)
+            self.lock.notify_all()
+        return req
+
+    def _admissible(self):
+        # a run larger than the whole budget is admitted alone
+        return bool(self.pending and self.free and
+                    (self.in_flight == 0 or self.in_flight + self.pending[0].bytes <= self.budget))
+
+    def _admit(self):
+        while True:
+            with self.lock:
+                while not self._admissible() and not (self.stopping and not self.pending):
+                    self.lock.wait()
+                if not self._admissible():  # stopping, nothing left
+                    return
+                req = self.pending.popleft()
+                eng = self.free.pop()
+                self.in_flight += req.bytes
+            t = threading.Thread(target=self._work, args=(eng, req), daemon=True)
+            t.start()
+            self.threads.append(t)
+
+    def _work(self, eng, req):
+        req.started = time.perf_counter()
+        try:
+            req.result = eng.run(self.set, kind=APPEND, n1=req.n1, n2=req.n2, **self.run_kw)
+        except Exception as e:  # reported with the request
+            req.error = e
+        req.done = time.perf_counter()
+        with self.lock:
+            self.in_flight -= req.bytes
+            self.free.append(eng)
+            self.lock.notify_all()
+        req.event.set()
+
+    def run_alone(self, req):
+        """The same batch computed again, alone, on the owner context (for the alone-vs-mixed equality check)."""
+        return self.owner.run(self.set, kind=APPEND, n1=req.n1, n2=req.n2, **self.run_kw)
+
+    def close(self):
+        with self.lock:
+            self.stopping = True
+            self.lock.notify_all()
+        self.admitter.join()
+        for t in self.threads:
+            t.join()
+        for e in self.free:
+            e.close()
+        self.set.free()
+        self.owner.close()
+
+
+def query_stream(panel, seed=8, rate=20.0, seconds=60.0, max_batch=512):
+    """Poisson arrivals at `rate` batches/s for `seconds`; batch sizes uniform in 1..max_batch; each query is a
+    panel sequence mutated at 0.5-2 % with 1 % ambiguity codes (the c1 generator's proportions)."""
+    rs = gen.Stream(seed)
+    t, out = 0.0, []
+    n, L = panel.shape
+    while True:
+        t += -np.log(1.0 - rs.uniform(1)[0]) / rate
+        if t >= seconds:
+            return out
+        q = 1 + int(rs.integers(max_batch, 1)[0])
+        src = rs.integers(n, q)
+        codes = panel[src].copy()
+        for k in range(q):  # substitutions at clean sites only
+            m = gen._poisson(rs, L * (0.005 + 0.015 * rs.uniform(1)[0]))
+            pos = rs.integers(L, m)
+            shift = rs.integers(3, m).astype(np.uint8) + 1
+            ok = codes[k, pos] < 4
+            codes[k, pos[ok]] = (codes[k, pos[ok]] + shift[ok]) % 4
+        w = list(gen._C1_WEIGHTS)
+        w += [sum(w) * 0.005 / (1.0 - 0.005) / 4.0] * 4
+        gen._ambiguities(rs, codes, 0.01, gen._C1_POOL, w)
+        out.append((t, codes))
+
+
+def serve(server, stream, clock_start=None):
+    """Plays an arrival stream against the server in real time; returns the requests when all are answered."""
+    t0 = time.perf_counter() if clock_start is None else clock_start
+    reqs = []
+    for k, (t, codes) in enumerate(stream):
+        now = time.perf_counter() - t0
+        if t > now:
+            time.sleep(t - now)
+        reqs.append(server.submit(k, codes, arrival=t0 + t))
+    for r in reqs:
+        r.event.wait()
+    return reqs

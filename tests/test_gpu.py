@@ -275,3 +275,49 @@ def test_batch_equals_single(engine):
         engine.run(sets[1], **jobs[1][1])  # eager work between the launches of the cached graph
     for st in sets:
         st.free()
+
+
+def test_append_runs_match_triangles(engine):
+    """A set grown by appends: the APPEND run of a batch gives exactly the pairs a triangle over everything up to the
+    batch adds to the triangle before it (histogram difference, links with a column in the batch, bit for bit)."""
+    codes = gen.c1s(420, seed=31)
+    st = engine.create(1300, 500)
+    assert st.append(codes[:300]) == 0
+    assert st.append(codes[300:360]) == 300
+    assert st.append(codes[360:420]) == 360 and st.size() == 420
+    kw = dict(mode=T.RESOLVE, threshold=0.05, min_overlap=500, resolve_fraction=0.05)
+    tri = {m: engine.upload(codes[:m]) for m in (300, 360, 420)}
+    full = {m: engine.run(s, **kw) for m, s in tri.items()}
+    for n1, n2 in ((300, 360), (360, 420), (300, 420)):
+        a = engine.run(st, kind=T.APPEND, n1=n1, n2=n2, **kw)
+        hi, lo = full[n2], full[n1]
+        assert (a["hist"] == hi["hist"] - lo["hist"]).all()
+        m = hi["link_col"] >= n1
+        for k in ("link_row", "link_col", "link_dist", "link_lo", "link_hi"):
+            assert (a[k] == hi[k][m]).all(), (n1, n2, k)
+        assert a["found_links"] == hi["found_links"] - lo["found_links"]
+        assert a["skipped"] == hi["skipped"] - lo["skipped"]
+    for s in tri.values():
+        s.free()
+    st.free()
+
+
+def test_serving_alone_vs_mixed():
+    """Continuous batching: query batches arriving together are admitted as workers free up and run concurrently;
+    every batch's result is the same as when it is computed alone afterwards."""
+    from swifttn93_amd import serve
+    panel = gen.c1s(1500, seed=41)
+    stream = serve.query_stream(panel, seed=42, rate=200.0, seconds=0.06, max_batch=200)
+    nq = sum(c.shape[0] for _, c in stream)
+    srv = serve.Server(panel, capacity=1500 + nq, workers=3, hbm_budget=1 << 30, run_kw=dict(threshold=0.03))
+    try:
+        reqs = serve.serve(srv, stream)
+        assert len(reqs) >= 4
+        assert all(r.error is None for r in reqs), [r.error for r in reqs if r.error]
+        overlap = any(a.started < b.done and b.started < a.done for a in reqs for b in reqs if a is not b)
+        assert overlap, "no two batches were on the device together"
+        for r in reqs:
+            print(f"request {r.rid}: rows [{r.n1}, {r.n2})", flush=True)
+            _same_summary(r.result, srv.run_alone(r))
+    finally:
+        srv.close()
