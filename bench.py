@@ -43,8 +43,8 @@ def profiled_traffic(kernel):
         return None
     for row in f.read_text().splitlines():
         cells = [c.strip() for c in row.strip().strip("|").split("|")]
-        if len(cells) == 5 and cells[0] == kernel:
-            return (float(cells[3]) + float(cells[4])) * 1e6
+        if len(cells) >= 5 and cells[0] == kernel:  # the last two columns: read MB, written MB
+            return (float(cells[-2]) + float(cells[-1])) * 1e6
     return None
 
 
