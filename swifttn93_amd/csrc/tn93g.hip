@@ -43,6 +43,7 @@ namespace {
 
 constexpr int NCODE = 18;
 constexpr int GAPC = 17;
+constexpr int NC = 15;  // N
 constexpr int NCH = 15;
 // resolution sets (refkernel.cpp:19-35): bit b set iff base b (A, C, G, T) resolves the code
 constexpr unsigned RES[NCODE] = {1, 2, 4, 8, 8, 5, 10, 6, 9, 12, 3, 14, 13, 11, 7, 15, 15, 0};
@@ -180,6 +181,10 @@ struct SetDev {
   uint32_t *ev;
   int32_t *evg_off;  // the lists with gap sites (GAPMM)
   uint32_t *evg;
+  uint32_t *nmask;   // [n][W] N sites
+  int32_t *nrun_off; // [n+1] N runs, for sets with dense N (ndense)
+  uint2 *nruns;
+  int ndense;        // N sites are not in ev: handled through nmask (pair kernel) and nruns (replay)
   int64_t *weights;
   int64_t total_events;
 };
@@ -221,6 +226,10 @@ struct RunArgs {
   const uint8_t *codes, *codesT;
   const int32_t *ev_off;
   const uint32_t *ev;
+  const uint32_t *nmask;   // [n][W] N sites (dense-N sets)
+  const int32_t *nrun_off; // [n+1] and the N runs [start, end) of each sequence (dense-N sets)
+  const uint2 *nruns;
+  int nplane;              // N sites are counted from nmask / nruns, not in the event lists
   const int64_t *weights;
   int64_t n, Ls, npad, W, W64;
   // pair space: rows [r0, r1) x cols [c0, c1), triangle => only j > i
@@ -431,13 +440,13 @@ __global__ void k_transpose(const uint8_t *__restrict__ codes, uint8_t *__restri
 
 // planes: per 32 sites, bit k of .x = site clean, .y = clean and bit0 of the code (C, T), .z = clean and bit1
 // (G, T), .w = code T
-__global__ void k_planes(const uint8_t *__restrict__ codes, uint4 *__restrict__ planes, int64_t s0, int64_t s1,
-                         int64_t L, int64_t Ls, int64_t W)
+__global__ void k_planes(const uint8_t *__restrict__ codes, uint4 *__restrict__ planes, uint32_t *__restrict__ nmask,
+                         int64_t s0, int64_t s1, int64_t L, int64_t Ls, int64_t W)
 {
   const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t0 >= (s1 - s0) * W) return;
   const int64_t s = s0 + t0 / W, w = t0 % W, t = s * W + w;
-  uint32_t c = 0, h = 0, g = 0, tt = 0;
+  uint32_t c = 0, h = 0, g = 0, tt = 0, nn = 0;
   for (int k = 0; k < 32; k++) {
     int64_t p = w * 32 + k;
     if (p >= L) break;
@@ -447,9 +456,12 @@ __global__ void k_planes(const uint8_t *__restrict__ codes, uint4 *__restrict__ 
       h |= (code & 1u) << k;
       g |= (code >> 1 & 1u) << k;
       tt |= (code == 3 ? 1u : 0u) << k;
+    } else if (code == (unsigned)NC) {
+      nn |= 1u << k;
     }
   }
   planes[t] = make_uint4(c, h, g, tt);
+  nmask[t] = nn;
 }
 
 // one-hot bytes for the MFMA backend, in operand order (see the layout above); sequences >= n are all zero
@@ -465,9 +477,49 @@ __global__ void k_onehot(const uint8_t *__restrict__ codes, uint8_t *__restrict_
   for (int b = 0; b < 4; b++) o[b * 16] = code == (unsigned)b ? 1 : 0;
 }
 
-// Event lists: the sites of codes 4..16 (ambiguous), and with `gaps` also the gap sites (the lists of GAPMM, where a
-// gap facing a non-gap is a site of its own).
-__device__ inline bool is_event(unsigned code, int gaps) { return code >= 4 && (code < (unsigned)GAPC || gaps); }
+// Event lists.  EV_AMB: the sites of codes 4..16 (ambiguous); EV_GAPS: also the gap sites (the lists of GAPMM,
+// where a gap facing a non-gap is a site of its own); EV_NO_N: codes 4..16 but N, for sets whose N sites are long
+// runs handled as a bit plane (nmask) and as run lists instead.
+enum { EV_AMB = 0, EV_GAPS = 1, EV_NO_N = 2 };
+__device__ inline bool is_event(unsigned code, int mode)
+{
+  if (code < 4) return false;
+  if (code == (unsigned)GAPC) return mode == EV_GAPS;
+  if (code == (unsigned)NC) return mode != EV_NO_N;
+  return true;
+}
+
+// N runs of each sequence: run k = [start, end) of consecutive N sites, in site order.
+__global__ void k_nruncount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls,
+                            int64_t s0)
+{
+  const int64_t s = s0 + blockIdx.x;
+  int c = 0;
+  for (int64_t p = threadIdx.x; p < L; p += blockDim.x)
+    c += (codes[s * Ls + p] == NC && (p == 0 || codes[s * Ls + p - 1] != NC)) ? 1 : 0;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = c;
+}
+__global__ void k_nrunfill(const uint8_t *__restrict__ codes, const int32_t *__restrict__ off, uint2 *__restrict__ runs,
+                           int64_t L, int64_t Ls, int64_t s0)
+{
+  const int64_t s = s0 + blockIdx.x;
+  const int lane = threadIdx.x;
+  int bs = off[s], be = off[s];
+  for (int64_t p0 = 0; p0 < L; p0 += 64) {
+    const int64_t p = p0 + lane;
+    const bool in = p < L && codes[s * Ls + p] == NC;
+    const bool prev = p > 0 && p - 1 < L && codes[s * Ls + p - 1] == NC;
+    const bool st = in && !prev, en = !in && prev;  // a run ends where its first non-N site (or L) is
+    const unsigned long long ms = __ballot(st), me = __ballot(en);
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (st) runs[bs + __popcll(ms & below)].x = (uint32_t)p;
+    if (en) runs[be + __popcll(me & below)].y = (uint32_t)p;
+    bs += __popcll(ms);
+    be += __popcll(me);
+  }
+  if (lane == 0 && be < bs) runs[be].y = (uint32_t)L;  // a run reaching the last site
+}
 
 __global__ void k_evcount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls,
                           int gaps, int64_t s0)
@@ -612,13 +664,15 @@ constexpr int TABW = 32, EVNONE = NCODE, EVB = 4;
 constexpr uint32_t EV_PAD = (uint32_t)EVNONE;  // site 0, code EVNONE
 
 // Fills the LDS event tables from one [NCODE][NCODE] table of packed entries (entry [row code][column code]).
-// With `gaps` (GAPMM), gap sites are events too, so a column event facing a row gap is the row's event.
-__device__ inline void load_event_tables(const uint4 *src, uint4 *sTabR, uint4 *sTabC, int tid, bool gaps)
+// With `gaps` (GAPMM), gap sites are events too, so a column event facing a row gap is the row's event.  With
+// `dense_n`, N sites are not events, so a column event facing a row N counts the site itself.
+__device__ inline void load_event_tables(const uint4 *src, uint4 *sTabR, uint4 *sTabC, int tid, bool gaps,
+                                         bool dense_n)
 {
   for (int k = tid; k < (NCODE + 1) * TABW; k += 256) {
     const int c = k / TABW, x = k % TABW;
     const bool in = c < NCODE && x < NCODE;
-    const bool xamb = x >= 4 && (x < GAPC || gaps);
+    const bool xamb = x >= 4 && (x < GAPC || gaps) && !(dense_n && x == NC);
     sTabR[k] = in ? src[c * NCODE + x] : make_uint4(0, 0, 0, 0);
     sTabC[k] = in && !xamb ? src[x * NCODE + c] : make_uint4(0, 0, 0, 0);
   }
@@ -691,7 +745,41 @@ __device__ inline void tile_events(const RunArgs &A, const uint4 *sTabR, const u
   }
 }
 
-template <int BACKEND, bool FAST>
+// Dense N (NP): the pair's sites where either sequence is N are counted from the N planes, by the partner's base:
+// nr[a] = row N against the column's clean bases (clean, C|T, G|T, T counts as in count_word), nc[a] = column N
+// against the row's clean bases, nn[a] = both N.  nplane_add folds them into the totals through the table entries.
+struct NCnt {
+  uint32_t r[4], c[4], nn;
+};
+
+__device__ inline void tot_add_mul(Tot &t, EvAcc &acc, const uint4 e, uint32_t k)
+{
+  if (!k) return;
+  t.nf[0] += (e.x & 0xffffu) * k;
+  t.nf[1] += (e.x >> 16) * k;
+  t.nf[2] += (e.y & 0xffffu) * k;
+  t.nf[3] += (e.y >> 16) * k;
+  t.dg += (e.z & 0xffffu) * k;
+  t.ag += (e.z >> 16) * k;
+  t.ct += (e.w & 0xffffu) * k;
+  t.amb += ((e.w >> 16) & 0x7fffu) * k;
+  acc.fl |= e.w & 0x80000000u;
+}
+
+__device__ inline void nplane_add(const NCnt &q, const uint4 *sTabR, Tot &t, EvAcc &acc)
+{
+  // base counts A, C, G, T from (clean, C|T, G|T, T)
+  const uint32_t rb[4] = {q.r[0] - q.r[1] - q.r[2] + q.r[3], q.r[1] - q.r[3], q.r[2] - q.r[3], q.r[3]};
+  const uint32_t cb[4] = {q.c[0] - q.c[1] - q.c[2] + q.c[3], q.c[1] - q.c[3], q.c[2] - q.c[3], q.c[3]};
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    tot_add_mul(t, acc, sTabR[NC * TABW + b], rb[b]);  // row N, column base b
+    tot_add_mul(t, acc, sTabR[b * TABW + NC], cb[b]);  // row base b, column N
+  }
+  tot_add_mul(t, acc, sTabR[NC * TABW + NC], q.nn);
+}
+
+template <int BACKEND, bool FAST, bool NP>
 __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
 {
   const int64_t c0a = A.c0 & ~(int64_t)63;  // column tiles start on a 64 boundary (whole 4-sequence groups)
@@ -709,10 +797,11 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     bs.unc = bs.undecided = 0;
   }
   __shared__ PairLds<KW, MF> lds;
+  __shared__ uint32_t sNI[NP ? KW : 1][TI + 1], sNJ[NP ? KW : 1][TJ + 1];
   __shared__ uint4 sTabR[(NCODE + 1) * TABW], sTabC[(NCODE + 1) * TABW];
   __shared__ unsigned long long lhist[FAST ? 1 : 3 * 200];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  load_event_tables(A.evtab, sTabR, sTabC, tid, A.gapmm);
+  load_event_tables(A.evtab, sTabR, sTabC, tid, A.gapmm, A.nplane);
   if constexpr (!FAST)
     for (int k = tid; k < 600; k += 256) lhist[k] = 0;
   const int64_t j = j0 + lane;
@@ -720,12 +809,16 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   Cnt cnt[RPW];
   Tot tot[RPW];
   EvAcc acc[RPW];
+  NCnt ncn[NP ? RPW : 1];
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
     cnt[a] = Cnt{0, 0, 0, 0, 0, 0, 0};
     tot[a] = Tot{{0, 0, 0, 0}, 0, 0, 0, 0};
     acc[a] = EvAcc{0, 0, 0, 0, 0};
   }
+  if constexpr (NP)
+#pragma unroll
+    for (int a = 0; a < RPW; a++) ncn[a] = NCnt{{0, 0, 0, 0}, {0, 0, 0, 0}, 0};
   const bool do_fast = !A.all_slow;
   const bool do_events = do_fast && !A.skip_events;
   // column event cursors (columns past the set have none)
@@ -753,11 +846,43 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
         const int kw = (int)min((int64_t)KW, A.W - w0);
         __syncthreads();
         stage_planes(A, lds, i0, j0, w0, kw, tid);
+        if constexpr (NP)
+          for (int k = tid; k < kw * (TI + TJ); k += 256) {
+            const int sq = k / kw, w = k % kw;
+            const int64_t seq = sq < TI ? i0 + sq : j0 + (sq - TI);
+            const uint32_t v = seq < A.n ? A.nmask[seq * A.W + w0 + w] : 0u;
+            if (sq < TI)
+              sNI[w][sq] = v;
+            else
+              sNJ[w][sq - TI] = v;
+          }
         __syncthreads();
         for (int k = 0; k < kw; k++) {
           const uint4 pjv = lds.sJ[k][lane];
 #pragma unroll
           for (int a = 0; a < RPW; a++) count_word(lds.sI[k][wave * RPW + a], pjv, cnt[a]);
+          if constexpr (NP) {
+            const uint32_t nj = sNJ[k][lane];
+            const bool coln = __ballot(nj != 0) != 0;
+#pragma unroll
+            for (int a = 0; a < RPW; a++) {
+              const uint32_t ni = sNI[k][wave * RPW + a];  // the same in every lane
+              if (ni) {
+                ncn[a].r[0] += __popc(ni & pjv.x);
+                ncn[a].r[1] += __popc(ni & pjv.y);
+                ncn[a].r[2] += __popc(ni & pjv.z);
+                ncn[a].r[3] += __popc(ni & pjv.w);
+                ncn[a].nn += __popc(ni & nj);
+              }
+              if (coln) {
+                const uint4 pi = lds.sI[k][wave * RPW + a];
+                ncn[a].c[0] += __popc(nj & pi.x);
+                ncn[a].c[1] += __popc(nj & pi.y);
+                ncn[a].c[2] += __popc(nj & pi.z);
+                ncn[a].c[3] += __popc(nj & pi.w);
+              }
+            }
+          }
         }
       }
   }
@@ -831,6 +956,10 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   __syncthreads();  // the event tables and lists are in LDS (also when no counts ran)
   const bool all_rows[RPW] = {true, true, true, true};
   if (do_events) tile_events<false>(A, sTabR, sTabC, te, ib, j, pj, pj_end, all_rows, acc, tot);
+  if constexpr (NP)
+    if (do_events)
+#pragma unroll
+      for (int a = 0; a < RPW; a++) nplane_add(ncn[a], sTabR, tot[a], acc[a]);
   // -g: pairs whose resolved ambiguities reach the fraction are computed again in AVERAGE (refkernel.cpp:742-750):
   // same clean/clean counts, the events through the AVERAGE table.  Order-dependent ones go to the replay instead.
   bool avg[RPW] = {false, false, false, false};
@@ -847,7 +976,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     }
   }
   if (__syncthreads_or(any_avg)) {
-    load_event_tables(A.evtab + NCODE * NCODE, sTabR, sTabC, tid, A.gapmm);
+    load_event_tables(A.evtab + NCODE * NCODE, sTabR, sTabC, tid, A.gapmm, A.nplane);
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < RPW; a++)
@@ -856,6 +985,10 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
         acc[a] = EvAcc{0, 0, 0, 0, 0};
       }
     tile_events<true>(A, sTabR, sTabC, te, ib, j, pj, pj_end, avg, acc, tot);
+    if constexpr (NP)
+#pragma unroll
+      for (int a = 0; a < RPW; a++)
+        if (avg[a]) nplane_add(ncn[a], sTabR, tot[a], acc[a]);
   }
   // totals x16 from the clean/clean counts, plus the events
 #pragma unroll
@@ -1103,6 +1236,108 @@ __device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const Repl
   }
 }
 
+// m additions of v (a power of two, the value of every N-site effect) to x >= 0, one after the other in FP64 as
+// tn93's site loop makes them: exact while the sum stays below the next power of two above x (v is a multiple of
+// the spacing of doubles there), and rounded once, to nearest even, at the addition that crosses it; so the run is
+// applied as one exact addition up to each crossing plus the crossing addition itself.
+__device__ __noinline__ double add_repeat(double x, double v, uint32_t m)
+{
+  while (m) {
+    if (x == 0.0) {
+      x = v;
+      m--;
+      continue;
+    }
+    const uint64_t e = (tn93g_bits(x) >> 52) & 0x7ffu;
+    const double B = tn93g_from_bits((e + 1) << 52);  // the power of two above x
+    const double steps = ceil((B - x) / v);           // additions that reach B; B - x and the quotient are exact
+    if ((double)m < steps) return x + (double)m * v;  // stays below B: exact
+    const uint32_t t = (uint32_t)steps - 1;
+    x = x + (double)t * v;  // exact, < B
+    x = x + v;              // the crossing addition, rounded like tn93's
+    m -= t + 1;
+  }
+  return x;
+}
+
+__device__ inline void replay_apply_rep(ReplayState &s, const ReplayEnt &E, uint32_t m)
+{
+  if (!m) return;
+#pragma unroll
+  for (int k = 0; k < 16; k++)
+    if (E.mask >> k & 1u) s.fc[k] = add_repeat(s.fc[k], E.v, m);
+#pragma unroll
+  for (int d = 0; d < 4; d++) s.dgi[d] += E.icd == d ? m : 0u;
+  s.amb += E.amb * m;
+}
+
+// Counts of A, C, G, T of sequence q over sites [p, e) from its bit planes.
+__device__ inline void base_counts(const RunArgs &A, int64_t q, uint32_t p, uint32_t e, uint32_t (&n)[4])
+{
+  uint32_t c = 0, b0 = 0, b1 = 0, t = 0;
+  const uint4 *P = A.planes + q * A.W;
+  for (uint32_t w = p >> 5; w <= (e - 1) >> 5; w++) {
+    uint32_t m = 0xffffffffu;
+    if (w == p >> 5) m &= 0xffffffffu << (p & 31);
+    if (w == (e - 1) >> 5) m &= 0xffffffffu >> (31 - ((e - 1) & 31));
+    const uint4 v = P[w];
+    c += __popc(v.x & m);
+    b0 += __popc(v.y & m);
+    b1 += __popc(v.z & m);
+    t += __popc(v.w & m);
+  }
+  n[0] = c - b0 - b1 + t;
+  n[1] = b0 - t;
+  n[2] = b1 - t;
+  n[3] = t;
+}
+
+// Site-order replay for a dense-N set: the ambiguity events of both sequences plus their N runs, swept as segments
+// between breakpoints (events, run starts and ends).  Inside a segment neither sequence has an event, so its sites
+// are N against a base, a gap or N: the same effect repeated, column (row) b's cells only for partner base b, so the
+// segment is applied per base with replay_apply_rep.  An event inside the other sequence's N run is the site
+// (N, code).
+__device__ void replay_events_dense(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *tab, ReplayState &s)
+{
+  constexpr uint32_t INF = 0xffffffffu;
+  int a = A.ev_off[i], ae = A.ev_off[i + 1], b = A.ev_off[j], be = A.ev_off[j + 1];
+  int ra = A.nrun_off[i], rae = A.nrun_off[i + 1], rb = A.nrun_off[j], rbe = A.nrun_off[j + 1];
+  const uint8_t *si = A.codes + i * A.Ls, *sj = A.codes + j * A.Ls;
+  const ReplayEnt &nn = tab[NC * NCODE + NC];
+  uint32_t p = 0;
+  for (;;) {
+    const uint32_t ea = a < ae ? A.ev[a] >> 5 : INF, eb = b < be ? A.ev[b] >> 5 : INF;
+    const uint2 ri = ra < rae ? A.nruns[ra] : make_uint2(INF, INF), rj = rb < rbe ? A.nruns[rb] : make_uint2(INF, INF);
+    const bool ni = ri.x <= p, nj = rj.x <= p;  // inside an N run at p
+    const uint32_t q = min(min(ea, eb), min(ni ? ri.y : ri.x, nj ? rj.y : rj.x));
+    if (q == INF) break;
+    if (q > p && (ni || nj)) {  // the segment [p, q)
+      if (ni && nj) {
+        replay_apply_rep(s, nn, q - p);
+      } else {
+        uint32_t cnt[4];
+        base_counts(A, ni ? j : i, p, q, cnt);
+#pragma unroll
+        for (int x = 0; x < 4; x++) replay_apply_rep(s, ni ? tab[NC * NCODE + x] : tab[x * NCODE + NC], cnt[x]);
+      }
+    }
+    p = q;
+    if (ra < rae && ri.y <= p) ra++;
+    if (rb < rbe && rj.y <= p) rb++;
+    if (ea == p || eb == p) {  // the site p
+      const bool ni2 = ra < rae && A.nruns[ra].x <= p, nj2 = rb < rbe && A.nruns[rb].x <= p;
+      const unsigned c1 = ea == p ? (A.ev[a] & 31u) : ni2 ? (unsigned)NC : si[p];
+      const unsigned c2 = eb == p ? (A.ev[b] & 31u) : nj2 ? (unsigned)NC : sj[p];
+      replay_apply(s, tab[c1 * NCODE + c2]);
+      if (ea == p) a++;
+      if (eb == p) b++;
+      p++;
+      if (ra < rae && A.nruns[ra].y <= p) ra++;
+      if (rb < rbe && A.nruns[rb].y <= p) rb++;
+    }
+  }
+}
+
 // GAPMM: gaps facing bases are events too, so every site that is not clean in both sequences is applied in order.
 __device__ void replay_scan(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *tab, ReplayState &s)
 {
@@ -1183,6 +1418,7 @@ __device__ inline int64_t replay_tile(const RunArgs &A, int64_t kb, int64_t k)
 
 // one thread per pair that needs the site-order replay (the tiles' lists, concatenated through slow_off); every
 // lane of a wave runs the same iterations so the row parts can be combined across the wave
+template <bool DENSE>
 __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
 {
   __shared__ ReplayEnt tabs[2 * NCODE * NCODE];
@@ -1207,7 +1443,9 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
       cc_table(A, i, j, icc);
       ReplayState s;
       replay_reset(s);
-      if (A.gapmm)
+      if constexpr (DENSE)
+        replay_events_dense(A, i, j, tabs, s);
+      else if (A.gapmm)
         replay_scan(A, i, j, tabs, s);
       else
         replay_events(A, i, j, tabs, s);
@@ -1217,7 +1455,9 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
       bool skip = tng <= (double)A.min_overlap;
       if (!skip && A.avg_fallback && tng * A.rf <= (double)s.amb) {  // recompute in AVERAGE (refkernel.cpp:742-750)
         replay_reset(s);
-        if (A.gapmm)
+        if constexpr (DENSE)
+          replay_events_dense(A, i, j, tabs + NCODE * NCODE, s);
+        else if (A.gapmm)
           replay_scan(A, i, j, tabs + NCODE * NCODE, s);
         else
           replay_events(A, i, j, tabs + NCODE * NCODE, s);
@@ -1457,7 +1697,8 @@ struct tn93g_set {
   tn93g_ctx *ctx = nullptr;
   SetDev d{};
   int64_t cap = 0;
-  int64_t ev_total[2] = {0, 0}, ev_cap[2] = {0, 0};  // events used / allocated: ev (ambiguity), evg (with gaps)
+  int64_t ev_total[3] = {0, 0, 0}, ev_cap[3] = {0, 0, 0};  // entries used / allocated: ev, evg, nruns
+  bool decided = false;                                  // ndense fixed by the first append
   std::vector<int64_t> hw;                            // copy numbers (host), cap entries
   std::vector<void *> retired;                        // replaced event arrays
   std::mutex mu;
@@ -1660,6 +1901,11 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   A.codesT = d.codesT;
   A.ev_off = P->mode == TN93G_GAPMM ? d.evg_off : d.ev_off;
   A.ev = P->mode == TN93G_GAPMM ? d.evg : d.ev;
+  // dense N: the ambiguity lists leave N out; GAPMM's lists (with gaps) keep it
+  A.nplane = d.ndense && P->mode != TN93G_GAPMM ? 1 : 0;
+  A.nmask = d.nmask;
+  A.nrun_off = d.nrun_off;
+  A.nruns = d.nruns;
   A.weights = d.weights;
   A.n = d.n;
   A.Ls = d.Ls;
@@ -1800,15 +2046,19 @@ static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
   if (timing) (void)hipEventRecord(c->ev[0], st);
   if (nrows > 0 && A.ncols > 0) {
     dim3 g((unsigned)J.gx, (unsigned)J.gy);
-    const bool mf = J.P.backend == TN93G_BACKEND_MFMA;
-    if (mf && J.fast)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, true>), g, dim3(256), 0, st, A);
+    const bool mf = J.P.backend == TN93G_BACKEND_MFMA && !A.nplane;  // dense-N counting is popcount only
+    if (A.nplane && J.fast)
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, true, true>), g, dim3(256), 0, st, A);
+    else if (A.nplane)
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, false, true>), g, dim3(256), 0, st, A);
+    else if (mf && J.fast)
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, true, false>), g, dim3(256), 0, st, A);
     else if (mf)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, false>), g, dim3(256), 0, st, A);
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, false, false>), g, dim3(256), 0, st, A);
     else if (J.fast)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, true>), g, dim3(256), 0, st, A);
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, true, false>), g, dim3(256), 0, st, A);
     else
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, false>), g, dim3(256), 0, st, A);
+      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, false, false>), g, dim3(256), 0, st, A);
   } else {
     fill_zero(st, B->slow_cnt.p, (size_t)J.ntiles * 4);
   }
@@ -1818,7 +2068,10 @@ static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
   if (timing) (void)hipEventRecord(c->ev[3], st);
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const int32_t *)B->slow_cnt.p,
                      (int64_t *)B->slow_off.p, J.ntiles, (int32_t *)B->slow_chunk.p);
-  hipLaunchKernelGGL(k_replay, dim3(2048), dim3(RB), 0, st, A);
+  if (A.nplane)
+    hipLaunchKernelGGL(k_replay<true>, dim3(2048), dim3(RB), 0, st, A);
+  else
+    hipLaunchKernelGGL(k_replay<false>, dim3(2048), dim3(RB), 0, st, A);
   if (J.fast) hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, st, A, (LinkRec *)B->links_flat.p);
   hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, st, A);
 }
@@ -2183,6 +2436,8 @@ int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
   SCHK(hipMalloc(&d.onehot, (size_t)(d.npad * d.W64 * 256)));
   SCHK(hipMalloc(&d.ev_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.evg_off, (size_t)(nn + 1) * sizeof(int32_t)));
+  SCHK(hipMalloc(&d.nrun_off, (size_t)(nn + 1) * sizeof(int32_t)));
+  SCHK(hipMalloc(&d.nmask, (size_t)(nn * d.W) * sizeof(uint32_t)));
   SCHK(hipMalloc(&d.weights, (size_t)nn * sizeof(int64_t)));
   // padding reads as gaps (codes, site-major codes) and as nothing (planes, one-hot operands, event offsets)
   SCHK(hipMemsetAsync(d.codes, GAPC, (size_t)(nn * d.Ls), c->stream));
@@ -2191,6 +2446,8 @@ int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
   SCHK(hipMemsetAsync(d.onehot, 0, (size_t)(d.npad * d.W64 * 256), c->stream));
   SCHK(hipMemsetAsync(d.ev_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
   SCHK(hipMemsetAsync(d.evg_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
+  SCHK(hipMemsetAsync(d.nrun_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
+  SCHK(hipMemsetAsync(d.nmask, 0, (size_t)(nn * d.W) * sizeof(uint32_t), c->stream));
   SCHK(hipStreamSynchronize(c->stream));
 #undef SCHK
   *out = S;
@@ -2223,20 +2480,45 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
     dim3 g((unsigned)((d.Ls + 63) / 64), (unsigned)((n1 - sb + 63) / 64));
     hipLaunchKernelGGL(k_transpose, g, dim3(256), 0, c->stream, d.codes, d.codesT, n1, d.Ls, d.npad, sb);
     const int64_t tot = n * d.W;
-    hipLaunchKernelGGL(k_planes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.planes, n0,
-                       n1, L, d.Ls, d.W);
+    hipLaunchKernelGGL(k_planes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.planes,
+                       d.nmask, n0, n1, L, d.Ls, d.W);
     const int64_t ob = n0 & ~(int64_t)3, oe = (n1 + 3) & ~(int64_t)3;
     const int64_t toh = (oe - ob) * d.W64 * 64;
     hipLaunchKernelGGL(k_onehot, dim3((unsigned)((toh + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.onehot, n1,
                        L, d.Ls, d.W64, ob, oe);
   }
-  int32_t *cnt_d = nullptr;
+  int32_t *cnt_d = nullptr, *cnt2_d = nullptr;
   HIPCHK(c, hipMalloc(&cnt_d, (size_t)n * sizeof(int32_t)));
+  HIPCHK(c, hipMalloc(&cnt2_d, (size_t)n * sizeof(int32_t)));
   int rc = 0;
-  for (int gaps = 0; gaps < 2 && !rc; gaps++) {
-    int32_t *eoff = gaps ? d.evg_off : d.ev_off;
-    uint32_t *&ev = gaps ? d.evg : d.ev;
-    hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, gaps, n0);
+  if (!S->decided) {
+    // dense N: on average 32 or more N sites per sequence of the first batch; then N stays out of the ambiguity
+    // event lists for the life of the set
+    hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, (int)EV_AMB,
+                       n0);
+    hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt2_d, L, d.Ls, (int)EV_NO_N,
+                       n0);
+    std::vector<int32_t> a((size_t)n), b((size_t)n);
+    if (hipMemcpyAsync(a.data(), cnt_d, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipMemcpyAsync(b.data(), cnt2_d, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      rc = c->fail(TN93G_E_HIP, "event count read-back failed");
+    int64_t nsites = 0;
+    for (int64_t k = 0; k < n; k++) nsites += a[(size_t)k] - b[(size_t)k];
+    d.ndense = nsites >= 32 * n ? 1 : 0;
+    S->decided = true;
+  }
+  // lists: 0 = ambiguity events (without N when dense), 1 = with gap sites (GAPMM), 2 = N runs (dense N only)
+  for (int li = 0; li < 3 && !rc; li++) {
+    if (li == 2 && !d.ndense) break;
+    int32_t *eoff = li == 0 ? d.ev_off : li == 1 ? d.evg_off : d.nrun_off;
+    const size_t esz = li == 2 ? sizeof(uint2) : sizeof(uint32_t);
+    void *arr = li == 0 ? (void *)d.ev : li == 1 ? (void *)d.evg : (void *)d.nruns;
+    const int mode = li == 1 ? EV_GAPS : d.ndense ? EV_NO_N : EV_AMB;
+    if (li < 2)
+      hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, mode, n0);
+    else
+      hipLaunchKernelGGL(k_nruncount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, n0);
     std::vector<int32_t> cnt((size_t)n, 0), off((size_t)n, 0);
     if (hipMemcpyAsync(cnt.data(), cnt_d, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream) !=
             hipSuccess ||
@@ -2244,42 +2526,50 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
       rc = c->fail(TN93G_E_HIP, "event count read-back failed");
       break;
     }
-    int64_t total = S->ev_total[gaps];
-    for (int64_t s = 0; s < n; s++) {
-      total += cnt[(size_t)s];
-      off[(size_t)s] = (int32_t)total;  // ev_off[n0 + 1 + s]
+    int64_t total = S->ev_total[li];
+    for (int64_t k = 0; k < n; k++) {
+      total += cnt[(size_t)k];
+      off[(size_t)k] = (int32_t)total;  // offsets of sequences n0 + 1 + k
     }
     if (total >= (1ll << 31)) {
       rc = c->fail(TN93G_E_ARG, "too many ambiguity events");
       break;
     }
-    if (total > S->ev_cap[gaps] || !ev) {
-      // grow: a new array with the old events copied; the old one stays valid for runs in flight
-      const int64_t want = std::max<int64_t>({total, 2 * S->ev_cap[gaps], 1024});
-      uint32_t *nev = nullptr;
-      if (hipMalloc(&nev, (size_t)want * sizeof(uint32_t)) != hipSuccess) {
+    if (total > S->ev_cap[li] || !arr) {
+      // grow: a new array with the old entries copied; the old one stays valid for runs in flight
+      const int64_t want = std::max<int64_t>({total, 2 * S->ev_cap[li], 1024});
+      void *na = nullptr;
+      if (hipMalloc(&na, (size_t)want * esz) != hipSuccess) {
         rc = c->fail(TN93G_E_NOMEM, "event array allocation failed");
         break;
       }
-      if (S->ev_total[gaps])
-        (void)hipMemcpyAsync(nev, ev, (size_t)S->ev_total[gaps] * sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                             c->stream);
-      if (ev) S->retired.push_back(ev);
-      ev = nev;
-      S->ev_cap[gaps] = want;
+      if (S->ev_total[li])
+        (void)hipMemcpyAsync(na, arr, (size_t)S->ev_total[li] * esz, hipMemcpyDeviceToDevice, c->stream);
+      if (arr) S->retired.push_back(arr);
+      arr = na;
+      if (li == 0) d.ev = (uint32_t *)na;
+      else if (li == 1) d.evg = (uint32_t *)na;
+      else d.nruns = (uint2 *)na;
+      S->ev_cap[li] = want;
     }
     if (hipMemcpyAsync(eoff + n0 + 1, off.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream) !=
         hipSuccess) {
       rc = c->fail(TN93G_E_HIP, "event offsets upload failed");
       break;
     }
-    hipLaunchKernelGGL(k_evfill, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, eoff, ev, L, d.Ls, gaps, n0);
+    if (li < 2)
+      hipLaunchKernelGGL(k_evfill, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, eoff, (uint32_t *)arr, L, d.Ls,
+                         mode, n0);
+    else
+      hipLaunchKernelGGL(k_nrunfill, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, eoff, (uint2 *)arr, L, d.Ls,
+                         n0);
     if (hipStreamSynchronize(c->stream) != hipSuccess) {  // `off` is pageable host memory
       rc = c->fail(TN93G_E_HIP, "event packing failed");
       break;
     }
-    S->ev_total[gaps] = total;
+    S->ev_total[li] = total;
   }
+  (void)hipFree(cnt2_d);
   (void)hipFree(cnt_d);
   if (rc) return rc;
   HIPCHK(c, hipGetLastError());
@@ -2318,8 +2608,8 @@ void tn93g_set_free(tn93g_set *S)
   if (!S) return;
   (void)hipSetDevice(S->ctx->device);
   (void)hipDeviceSynchronize();  // runs of any context may still read the set
-  void *ps[] = {S->d.codes, S->d.codesT, S->d.planes, S->d.onehot, S->d.ev_off,
-                S->d.ev,    S->d.evg_off, S->d.evg,   S->d.weights};
+  void *ps[] = {S->d.codes,   S->d.codesT, S->d.planes,   S->d.onehot, S->d.ev_off, S->d.ev,
+                S->d.evg_off, S->d.evg,    S->d.nrun_off, S->d.nruns,  S->d.nmask,  S->d.weights};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (void *p : S->retired) (void)hipFree(p);

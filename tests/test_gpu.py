@@ -321,3 +321,20 @@ def test_serving_alone_vs_mixed():
             _same_summary(r.result, srv.run_alone(r))
     finally:
         srv.close()
+
+
+@pytest.mark.parametrize("mode", [T.RESOLVE, T.AVERAGE, T.SKIP])
+def test_dense_n_fast_equals_replay(engine, mode):
+    """A set with long N blocks counts N from bit planes in the pair kernel and replays N runs as segments; both
+    agree with the site-order replay of every pair and with the host-exact path."""
+    codes = gen.c2s(160, seed=51)
+    s = engine.upload(codes)
+    kw = dict(mode=mode, threshold=1.0, min_overlap=20000, resolve_fraction=0.05 if mode == T.RESOLVE else 1.0)
+    fast = engine.run(s, **kw)
+    slow = engine.run(s, flags=T.FLAG_ALL_SLOW, **kw)
+    exact = engine.run(s, flags=T.FLAG_EXACT, **kw)
+    _same(fast, slow)
+    _same(fast, exact)
+    assert slow["n_replayed"] == 160 * 159 // 2
+    assert (fast["link_dist"] == slow["link_dist"]).all()
+    s.free()
