@@ -1024,7 +1024,9 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
       }
       if (slow) {
         const unsigned k = atomicAdd(&lslow, 1u);  // < TI * TJ
-        A.slow_list[tile * (TI * TJ) + k] = (unsigned long long)i << 32 | (unsigned long long)j;
+        // bit 63: the -g test already chose AVERAGE for this pair (the replay starts there)
+        A.slow_list[tile * (TI * TJ) + k] =
+            (unsigned long long)i << 32 | (unsigned long long)j | (avg[a] ? 1ull << 63 : 0ull);
       }
     }
     if constexpr (FAST) {
@@ -1204,6 +1206,40 @@ __device__ inline void cc_table(const RunArgs &A, int64_t i, int64_t j, uint32_t
   }
 }
 
+// The same table for the 64 pairs of a wave at once (long alignments): the wave takes the pairs one after the other,
+// its lanes split the words (coalesced loads of both rows) and the 16 counts are summed over the wave into
+// out[pair][16].  Lanes with valid == false bring no pair.  Every lane of the wave must call it.
+__device__ void cc_table_wave(const RunArgs &A, int64_t i, int64_t j, bool valid, uint32_t (*out)[16])
+{
+  const int lane = threadIdx.x & 63;
+  unsigned long long m = __ballot(valid);
+  while (m) {
+    const int src = __ffsll((long long)m) - 1;
+    m &= m - 1;
+    const int64_t pi = __shfl(i, src, 64), pj = __shfl(j, src, 64);
+    uint32_t c[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) c[k] = 0;
+    const uint4 *Pi = A.planes + pi * A.W, *Pj = A.planes + pj * A.W;
+    for (int64_t w = lane; w < A.W; w += 64) {
+      const uint4 a = Pi[w], b = Pj[w];
+      const uint32_t M = a.x & b.x;
+      const uint32_t ai[4] = {M & ~(a.y | a.z), M & a.y & ~a.z, M & a.z & ~a.y, M & a.w};
+      const uint32_t bj[4] = {~(b.y | b.z), b.y & ~b.z, b.z & ~b.y, b.w};
+#pragma unroll
+      for (int x = 0; x < 4; x++)
+#pragma unroll
+        for (int y = 0; y < 4; y++) c[4 * x + y] += __popc(ai[x] & bj[y]);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+      uint32_t v = c[k];
+      for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
+      if (lane == src) out[src][k] = v;
+    }
+  }
+}
+
 // The sites where either sequence is ambiguous, in increasing site order: the two sorted event lists merged; a
 // site in one list only takes its partner's code from the codes array.
 __device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *tab, ReplayState &s)
@@ -1240,7 +1276,7 @@ __device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const Repl
 // tn93's site loop makes them: exact while the sum stays below the next power of two above x (v is a multiple of
 // the spacing of doubles there), and rounded once, to nearest even, at the addition that crosses it; so the run is
 // applied as one exact addition up to each crossing plus the crossing addition itself.
-__device__ __noinline__ double add_repeat(double x, double v, uint32_t m)
+__device__ __noinline__ double add_repeat_crossing(double x, double v, uint32_t m)
 {
   while (m) {
     if (x == 0.0) {
@@ -1258,6 +1294,13 @@ __device__ __noinline__ double add_repeat(double x, double v, uint32_t m)
     m -= t + 1;
   }
   return x;
+}
+
+__device__ inline double add_repeat(double x, double v, uint32_t m)
+{
+  // a multiple of v stays one: every partial sum is exact (the cells hold at most 2^26 sites, far below 2^53 v)
+  if (floor(x / v) * v == x) return x + (double)m * v;
+  return add_repeat_crossing(x, v, m);
 }
 
 __device__ inline void replay_apply_rep(ReplayState &s, const ReplayEnt &E, uint32_t m)
@@ -1424,6 +1467,7 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
   __shared__ ReplayEnt tabs[2 * NCODE * NCODE];
   __shared__ unsigned long long lhist[3 * 200];
   __shared__ BlockStats bs;
+  __shared__ uint32_t scc[DENSE ? RB : 1][16];  // clean/clean tables built by the wave (long alignments)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int k = tid; k < 2 * NCODE * NCODE; k += RB) tabs[k] = A.rtab[k];
   for (int k = tid; k < 600; k += RB) lhist[k] = 0;
@@ -1436,24 +1480,38 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
     const int64_t tile = replay_tile(A, kb, k < total ? k : kb);
     int64_t row = -1;
     RowPart rp = rowpart_zero();
+    const bool coop = DENSE && A.W >= 128;  // uniform
+    unsigned long long pr = 0;
+    if (k < total) pr = A.slow_list[tile * (TI * TJ) + (k - A.slow_off[tile])];
+    if (coop)
+      cc_table_wave(A, (int64_t)(pr >> 32 & 0x7fffffffu), (int64_t)(pr & 0xffffffffull), k < total, scc + wave * 64);
+    const bool avg_first = pr >> 63;
+    pr &= ~(1ull << 63);
     if (k < total) {
-      const unsigned long long pr = A.slow_list[tile * (TI * TJ) + (k - A.slow_off[tile])];
       const int64_t i = (int64_t)(pr >> 32), j = (int64_t)(pr & 0xffffffffull);
       uint32_t icc[16];
-      cc_table(A, i, j, icc);
+      if (coop) {
+#pragma unroll
+        for (int q = 0; q < 16; q++) icc[q] = scc[wave * 64 + lane][q];
+      } else {
+        cc_table(A, i, j, icc);
+      }
       ReplayState s;
       replay_reset(s);
+      // the main mode first, unless the pair kernel's -g test (on exact, dyadic sums) already chose AVERAGE
+      const ReplayEnt *t0 = avg_first ? tabs + NCODE * NCODE : tabs;
       if constexpr (DENSE)
-        replay_events_dense(A, i, j, tabs, s);
+        replay_events_dense(A, i, j, t0, s);
       else if (A.gapmm)
-        replay_scan(A, i, j, tabs, s);
+        replay_scan(A, i, j, t0, s);
       else
-        replay_events(A, i, j, tabs, s);
+        replay_events(A, i, j, t0, s);
       tn93g_tailin tin;
       double tng;
       replay_merge(s, icc, tin, tng);
       bool skip = tng <= (double)A.min_overlap;
-      if (!skip && A.avg_fallback && tng * A.rf <= (double)s.amb) {  // recompute in AVERAGE (refkernel.cpp:742-750)
+      if (!avg_first && !skip && A.avg_fallback &&
+          tng * A.rf <= (double)s.amb) {  // recompute in AVERAGE (refkernel.cpp:742-750)
         replay_reset(s);
         if constexpr (DENSE)
           replay_events_dense(A, i, j, tabs + NCODE * NCODE, s);
