@@ -1,23 +1,26 @@
 // tn93g.hip -- MI355X (gfx950) all-pairs TN93 engine behind the C ABI of include/tn93g.h.
 //
-// Data layout in HBM (one uploaded set of n sequences, L sites; Ls = L rounded up to 16, npad = n rounded up to
-// 64, plus 64):
-//   codes   [n][Ls]    u8   ValidChars codes, row-major, padded with gaps        (replay kernel, j-side lookups)
-//   codesT  [Ls][npad] u8   site-major copy                                     (i-side event lookups, coalesced)
-//   planes  [n][W]     uint4 per 32 sites: clean, C|T bit, G|T bit, T bit       (clean/clean counts, VALU popcount)
-//   ambp    [n][W]     u32 per 32 sites: ambiguity code (4..16)                 (event partners, from LDS)
-//   onehot  [npad/4][W64][4][16][16] u8 one-hot A,C,G,T per site in MFMA operand order: per group of 4 sequences
-//           and 64-site block, 1 KB = the A/B operand of one v_mfma_i32_16x16x64_i8 (lane l: row l%16 = 4*seq+base,
-//           16 sites of quarter l/16)                                      (clean/clean counts, int8 MFMA)
-//   ev_off  [n+1] i32, ev [] u32 (site << 5 | code) ambiguity events (codes 4..16) in site order
-// Per run: D [rows][cols] f64 (tn93's thisD of each pair), U [rows][cols] u16 (ulps below/above D that bound the
-// value when a logarithm was uncertain), the histogram, per-row link counts, replay and host-exact lists.
+// Data layout in HBM (one set of n sequences, L sites, grown by appends up to its capacity; Ls = L rounded up to
+// 16, npad = capacity rounded up to 64, plus 64; W = 32-site words):
+//   codes   [n][Ls]    u8    ValidChars codes, row-major, padded with gaps        (replay kernel, j-side lookups)
+//   codesT  [Ls][npad] u8    site-major copy                                     (row-side event codes, coalesced)
+//   planes  [n][W]     uint4 per 32 sites: clean, C|T bit, G|T bit, T bit        (clean/clean counts, VALU popcount)
+//   nmask   [n][W]     u32   N sites, for sets whose N occur in long runs         (pair kernel, dense-N sets only)
+//   onehot  [npad/4][W64][4][16][16] u8 one-hot A,C,G,T per site in the operand order of v_mfma_i32_16x16x64_i8
+//                                                                                (clean/clean counts, MFMA backend)
+//   ev_off/ev   [n+1] i32, u32 (site << 5 | code): ambiguity events in site order, gaps excluded
+//   evg_off/evg the same with gap sites (-a gapmm);   nrun_off/nruns  N runs (first site, length) of dense-N sets
+// Per run: either the fast-path row accumulators and unordered links (distances certified in the kernel), or D
+// [rows][cols] f64 + U [rows][cols] u16 (ulps that bound an uncertain logarithm) for the exact path; the histogram
+// in per-stripe replicas, the replay and host-exact lists.
 //
-// Kernels of a run (all on the context's stream):
-//   k_pairs   clean/clean counts (popcount or MFMA) + ambiguity events + tests + FP64 tail + histogram + D
-//   k_replay  site-order FP64 replay for pairs with order-dependent (non-dyadic) contributions or a -g fallback
-//   k_patch   host-exact distances back into D (only after an uncertain logarithm could not be certified)
-//   k_scan    per-row link offsets;  k_rows  per-row sums in column order, links in tn93's emission order
+// Kernels of a run (on the context's stream, or one of its side streams in a captured batch):
+//   k_pairs   clean/clean counts (popcount, popcount+nmask, or MFMA) + event tables from LDS + -g fallback pass;
+//             dyadic pairs complete here, order-dependent ones go to the replay list
+//   k_tail    FP64 TN93/K2P tail, certification, histogram, row accumulators and links (fast path) or D/U
+//   k_replay  site-order FP64 replay for pairs with non-dyadic contributions (runs of N as repeated additions)
+//   k_scan    per-row link offsets;  k_link_gather  links into one array;  k_hist_sum  histogram replicas summed
+//   k_patch   host-exact distances into D;  k_rows  per-row sums in column order (exact path only)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
