@@ -35,6 +35,7 @@
 #include <cstring>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/tn93g.h"
@@ -187,6 +188,9 @@ struct SetDev {
   uint32_t *nmask;   // [n][W] N sites
   int32_t *nrun_off; // [n+1] N runs, for sets with dense N (ndense)
   uint2 *nruns;
+  uint4 *pre;        // [n][W+1] plane counts before each word boundary; pre[s][W] = totals of sequence s
+  int32_t *grun_off; // [n+1] gap runs
+  uint2 *gruns;
   int ndense;        // N sites are not in ev: handled through nmask (pair kernel) and nruns (replay)
   int64_t *weights;
   int64_t total_events;
@@ -233,6 +237,9 @@ struct RunArgs {
   const int32_t *nrun_off; // [n+1] and the N runs [start, end) of each sequence (dense-N sets)
   const uint2 *nruns;
   int nplane;              // N sites are counted from nmask / nruns, not in the event lists
+  const uint4 *pre;        // [n][W+1] plane counts (clean, C|T, G|T, T) of the words before each word boundary
+  const int32_t *grun_off; // [n+1] and the gap runs [start, end) of each sequence
+  const uint2 *gruns;
   const int64_t *weights;
   int64_t n, Ls, npad, W, W64;
   // pair space: rows [r0, r1) x cols [c0, c1), triangle => only j > i
@@ -240,7 +247,8 @@ struct RunArgs {
   int triangle, fst, gapmm, skip_events, avg_fallback, exact, all_slow;
   double rf, thr, mind, window;
   int64_t min_overlap;
-  const uint4 *evtab;          // [2][18*18] packed event entries: main mode, AVERAGE (the -g fallback)
+  const uint4 *evtab;          // [4][18*18] packed event entries: main mode, AVERAGE (the -g fallback), and the same
+                               // two with the clean base facing an event subtracted (the marginal form of k_pairs)
   const ReplayEnt *rtab;       // [2][18*18]: main mode, AVERAGE
   // outputs
   double *D;
@@ -423,6 +431,21 @@ __device__ inline long long wave_sum_l(long long x)
 }
 
 // ------------------------------------------------------------------------------------------ pack kernels
+__device__ inline void add4(uint4 &a, const uint4 b)
+{
+  a.x += b.x;
+  a.y += b.y;
+  a.z += b.z;
+  a.w += b.w;
+}
+__device__ inline void sub4(uint4 &a, const uint4 b)
+{
+  a.x -= b.x;
+  a.y -= b.y;
+  a.z -= b.z;
+  a.w -= b.w;
+}
+
 __global__ void k_transpose(const uint8_t *__restrict__ codes, uint8_t *__restrict__ codesT, int64_t n, int64_t Ls,
                             int64_t npad, int64_t sbase)
 {
@@ -492,28 +515,31 @@ __device__ inline bool is_event(unsigned code, int mode)
   return true;
 }
 
-// N runs of each sequence: run k = [start, end) of consecutive N sites, in site order.
-__global__ void k_nruncount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls,
-                            int64_t s0)
+// Runs of one code (N, or the gap) in each sequence: run k = [start, end) of consecutive sites holding it, in site
+// order.
+__global__ void k_runcount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls,
+                           int64_t s0, int code)
 {
   const int64_t s = s0 + blockIdx.x;
+  const uint8_t x = (uint8_t)code;
   int c = 0;
   for (int64_t p = threadIdx.x; p < L; p += blockDim.x)
-    c += (codes[s * Ls + p] == NC && (p == 0 || codes[s * Ls + p - 1] != NC)) ? 1 : 0;
+    c += (codes[s * Ls + p] == x && (p == 0 || codes[s * Ls + p - 1] != x)) ? 1 : 0;
   for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
   if (threadIdx.x == 0) cnt[blockIdx.x] = c;
 }
-__global__ void k_nrunfill(const uint8_t *__restrict__ codes, const int32_t *__restrict__ off, uint2 *__restrict__ runs,
-                           int64_t L, int64_t Ls, int64_t s0)
+__global__ void k_runfill(const uint8_t *__restrict__ codes, const int32_t *__restrict__ off, uint2 *__restrict__ runs,
+                          int64_t L, int64_t Ls, int64_t s0, int code)
 {
   const int64_t s = s0 + blockIdx.x;
   const int lane = threadIdx.x;
+  const uint8_t x = (uint8_t)code;
   int bs = off[s], be = off[s];
   for (int64_t p0 = 0; p0 < L; p0 += 64) {
     const int64_t p = p0 + lane;
-    const bool in = p < L && codes[s * Ls + p] == NC;
-    const bool prev = p > 0 && p - 1 < L && codes[s * Ls + p - 1] == NC;
-    const bool st = in && !prev, en = !in && prev;  // a run ends where its first non-N site (or L) is
+    const bool in = p < L && codes[s * Ls + p] == x;
+    const bool prev = p > 0 && p - 1 < L && codes[s * Ls + p - 1] == x;
+    const bool st = in && !prev, en = !in && prev;  // a run ends where its first site of another code (or L) is
     const unsigned long long ms = __ballot(st), me = __ballot(en);
     const unsigned long long below = (1ull << lane) - 1ull;
     if (st) runs[bs + __popcll(ms & below)].x = (uint32_t)p;
@@ -522,6 +548,31 @@ __global__ void k_nrunfill(const uint8_t *__restrict__ codes, const int32_t *__r
     be += __popcll(me);
   }
   if (lane == 0 && be < bs) runs[be].y = (uint32_t)L;  // a run reaching the last site
+}
+
+// pre[s][w] = plane counts (clean, C|T, G|T, T) of sequence s over its words [0, w), w = 0..W: one wave per sequence,
+// 64 word boundaries per step
+__global__ void k_prefix(const uint4 *__restrict__ planes, uint4 *__restrict__ pre, int64_t W, int64_t s0)
+{
+  const int64_t s = s0 + blockIdx.x;
+  const int lane = threadIdx.x;
+  uint4 carry = make_uint4(0, 0, 0, 0);
+  for (int64_t w0 = 0; w0 <= W; w0 += 64) {
+    const int64_t w = w0 + lane;
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (w > 0 && w <= W) {
+      const uint4 q = planes[s * W + w - 1];
+      v = make_uint4(__popc(q.x), __popc(q.y), __popc(q.z), __popc(q.w));
+    }
+    for (int o = 1; o < 64; o <<= 1) {  // inclusive scan over the wave
+      const uint4 t = make_uint4(__shfl_up(v.x, o, 64), __shfl_up(v.y, o, 64), __shfl_up(v.z, o, 64),
+                                 __shfl_up(v.w, o, 64));
+      if (lane >= o) add4(v, t);
+    }
+    add4(v, carry);
+    if (w <= W) pre[s * (W + 1) + w] = v;
+    carry = make_uint4(__shfl(v.x, 63, 64), __shfl(v.y, 63, 64), __shfl(v.z, 63, 64), __shfl(v.w, 63, 64));
+  }
 }
 
 __global__ void k_evcount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls,
@@ -575,6 +626,44 @@ __device__ inline void count_word(const uint4 pi, const uint4 pj, Cnt &c)
   c.B0 += __popc(pj.x & pi.y) + __popc(pi.x & pj.y);
   c.B1 += __popc(pj.x & pi.z) + __popc(pi.x & pj.z);
   c.BT += __popc(pj.x & pi.w) + __popc(pi.x & pj.w);
+}
+
+// The same without the base sums.  With per-sequence totals the base sums over the sites where both sequences are
+// clean are the totals of both sequences less their clean sites facing a non-clean site of the other sequence, and
+// those sites are all visited anyway: ambiguity events (their table entries subtract the clean base), N runs of
+// dense-N sets (the N-plane counts, through the same entries) and gap runs (prefix counts).  11 operations per word
+// pair instead of 24.
+struct CntM {
+  uint32_t DG, TS, CT;
+};
+
+__device__ inline void count_word(const uint4 pi, const uint4 pj, CntM &c)
+{
+  const uint32_t M = pi.x & pj.x;
+  const uint32_t x = pi.y ^ pj.y, y = pi.z ^ pj.z;
+  c.DG += __popc(M & ~(x | y));
+  const uint32_t tr = M & y & ~x;
+  c.TS += __popc(tr);
+  c.CT += __popc(tr & pi.y);
+}
+
+// Plane counts (clean, C|T, G|T, T) of sequence s over the sites [0, p): the prefix at the word boundary plus the
+// bits of the partial word.
+__device__ inline uint4 plane_prefix(const uint4 *__restrict__ pre, const uint4 *__restrict__ planes, int64_t W,
+                                     int64_t s, uint32_t p)
+{
+  const int64_t w = p >> 5;
+  uint4 r = pre[s * (W + 1) + w];
+  const uint32_t b = p & 31u;
+  if (b) {
+    const uint4 v = planes[s * W + w];
+    const uint32_t m = (1u << b) - 1u;
+    r.x += __popc(v.x & m);
+    r.y += __popc(v.y & m);
+    r.z += __popc(v.z & m);
+    r.w += __popc(v.w & m);
+  }
+  return r;
 }
 
 // event accumulators: packed 16-bit fields plus 32-bit totals (flushed before a 16-bit field can overflow)
@@ -782,7 +871,51 @@ __device__ inline void nplane_add(const NCnt &q, const uint4 *sTabR, Tot &t, EvA
   tot_add_mul(t, acc, sTabR[NC * TABW + NC], q.nn);
 }
 
-template <int BACKEND, bool FAST, bool NP>
+// Base sums of the marginal form: S[a] = plane totals of row ib + a and of the lane's column, less the clean sites
+// of each facing a gap run of the other (gaps are not events outside GAPMM, so no table entry sees them).
+__device__ inline void base_sums(const RunArgs &A, int64_t ib, int64_t j, uint4 (&S)[RPW])
+{
+  const int64_t W = A.W;
+  const bool jin = j < A.n;
+  const uint4 tj = jin ? A.pre[j * (W + 1) + W] : make_uint4(0, 0, 0, 0);
+#pragma unroll
+  for (int a = 0; a < RPW; a++) {
+    const int64_t i = ib + a;
+    S[a] = tj;
+    if (i < A.r1) add4(S[a], A.pre[i * (W + 1) + W]);
+  }
+  if (A.gapmm) return;
+#pragma unroll
+  for (int a = 0; a < RPW; a++) {
+    const int64_t i = ib + a;
+    if (i >= A.r1) continue;  // uniform over the wave
+    const int g0 = __builtin_amdgcn_readfirstlane(A.grun_off[i]);
+    const int g1 = __builtin_amdgcn_readfirstlane(A.grun_off[i + 1]);
+    for (int g = g0; g < g1; g++) {
+      const uint2 r = A.gruns[g];
+      if (jin) {
+        sub4(S[a], plane_prefix(A.pre, A.planes, W, j, r.y));
+        add4(S[a], plane_prefix(A.pre, A.planes, W, j, r.x));
+      }
+    }
+  }
+  if (!jin) return;
+  const int h0 = A.grun_off[j], h1 = A.grun_off[j + 1];
+  for (int g = h0; g < h1; g++) {
+    const uint2 r = A.gruns[g];
+#pragma unroll
+    for (int a = 0; a < RPW; a++) {
+      const int64_t i = ib + a;
+      if (i >= A.r1) continue;
+      sub4(S[a], plane_prefix(A.pre, A.planes, W, i, r.y));
+      add4(S[a], plane_prefix(A.pre, A.planes, W, i, r.x));
+    }
+  }
+}
+
+// MARG: the base sums come from base_sums and tables whose entries facing a clean base subtract it (popcount
+// backend, every mode but SKIP, whose ambiguity sites are not visited).
+template <int BACKEND, bool FAST, bool NP, bool MARG>
 __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
 {
   const int64_t c0a = A.c0 & ~(int64_t)63;  // column tiles start on a 64 boundary (whole 4-sequence groups)
@@ -804,18 +937,21 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   __shared__ uint4 sTabR[(NCODE + 1) * TABW], sTabC[(NCODE + 1) * TABW];
   __shared__ unsigned long long lhist[FAST ? 1 : 3 * 200];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  load_event_tables(A.evtab, sTabR, sTabC, tid, A.gapmm, A.nplane);
+  const uint4 *tabs = A.evtab + (MARG ? 2 * NCODE * NCODE : 0);  // [main, AVERAGE], plain or folded
+  load_event_tables(tabs, sTabR, sTabC, tid, A.gapmm, A.nplane);
   if constexpr (!FAST)
     for (int k = tid; k < 600; k += 256) lhist[k] = 0;
   const int64_t j = j0 + lane;
   const int64_t ib = i0 + wave * RPW;
-  Cnt cnt[RPW];
+  using CntT = typename std::conditional<MARG, CntM, Cnt>::type;
+  CntT cnt[RPW];
+  uint4 S[MARG ? RPW : 1];
   Tot tot[RPW];
   EvAcc acc[RPW];
   NCnt ncn[NP ? RPW : 1];
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
-    cnt[a] = Cnt{0, 0, 0, 0, 0, 0, 0};
+    cnt[a] = CntT{};
     tot[a] = Tot{{0, 0, 0, 0}, 0, 0, 0, 0};
     acc[a] = EvAcc{0, 0, 0, 0, 0};
   }
@@ -956,6 +1092,8 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
       }
     }
   }
+  if constexpr (MARG)
+    if (do_fast) base_sums(A, ib, j, S);
   __syncthreads();  // the event tables and lists are in LDS (also when no counts ran)
   const bool all_rows[RPW] = {true, true, true, true};
   if (do_events) tile_events<false>(A, sTabR, sTabC, te, ib, j, pj, pj_end, all_rows, acc, tot);
@@ -972,14 +1110,18 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     for (int a = 0; a < RPW; a++) {
       const int64_t i = ib + a;
       const bool valid = i < A.r1 && j >= A.c0 && j < A.c1 && (!A.triangle || j > i);
-      const uint32_t nfs = tot[a].nf[0] + tot[a].nf[1] + tot[a].nf[2] + tot[a].nf[3] + 32u * cnt[a].T;
+      uint32_t nfs = tot[a].nf[0] + tot[a].nf[1] + tot[a].nf[2] + tot[a].nf[3];
+      if constexpr (MARG)
+        nfs += 16u * S[a].x;
+      else
+        nfs += 32u * cnt[a].T;
       const double tng = (double)nfs * 0.03125;
       avg[a] = valid && !(acc[a].fl >> 31) && tng > (double)A.min_overlap && tng * A.rf <= (double)tot[a].amb;
       any_avg |= avg[a];
     }
   }
   if (__syncthreads_or(any_avg)) {
-    load_event_tables(A.evtab + NCODE * NCODE, sTabR, sTabC, tid, A.gapmm, A.nplane);
+    load_event_tables(tabs + NCODE * NCODE, sTabR, sTabC, tid, A.gapmm, A.nplane);
     __syncthreads();
 #pragma unroll
     for (int a = 0; a < RPW; a++)
@@ -996,8 +1138,20 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   // totals x16 from the clean/clean counts, plus the events
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
-    const Cnt &c = cnt[a];
-    const uint32_t nf3 = c.BT, nf1 = c.B0 - c.BT, nf2 = c.B1 - c.BT, nf0 = 2 * c.T - c.B0 - c.B1 + c.BT;
+    const CntT &c = cnt[a];
+    uint32_t nf0, nf1, nf2, nf3;
+    if constexpr (MARG) {
+      const uint4 &q = S[a];  // clean, C|T, G|T, T summed over both sequences
+      nf3 = q.w;
+      nf1 = q.y - q.w;
+      nf2 = q.z - q.w;
+      nf0 = q.x - q.y - q.z + q.w;
+    } else {
+      nf3 = c.BT;
+      nf1 = c.B0 - c.BT;
+      nf2 = c.B1 - c.BT;
+      nf0 = 2 * c.T - c.B0 - c.B1 + c.BT;
+    }
     tot[a].nf[0] += 16 * nf0;
     tot[a].nf[1] += 16 * nf1;
     tot[a].nf[2] += 16 * nf2;
@@ -1758,7 +1912,7 @@ struct tn93g_set {
   tn93g_ctx *ctx = nullptr;
   SetDev d{};
   int64_t cap = 0;
-  int64_t ev_total[3] = {0, 0, 0}, ev_cap[3] = {0, 0, 0};  // entries used / allocated: ev, evg, nruns
+  int64_t ev_total[4] = {0, 0, 0, 0}, ev_cap[4] = {0, 0, 0, 0};  // entries used / allocated: ev, evg, nruns, gruns
   bool decided = false;                                  // ndense fixed by the first append
   std::vector<int64_t> hw;                            // copy numbers (host), cap entries
   std::vector<void *> retired;                        // replaced event arrays
@@ -1789,6 +1943,7 @@ struct RunBufs {
   int64_t link_cap_hint = 0;  // link region size that sufficed last time
   int tab_mode = -1;           // mode and subset of the tables in evtab/rtab (uploaded again only when they change)
   uint32_t tab_mask = 0;
+  bool tab_fold = false;       // the folded tables are valid (the marginal form of the pair kernel may run)
   void release()
   {
     for (DevBuf *b : {&D, &U, &hist, &row_links, &link_off, &slow_list, &slow_cnt, &slow_off, &slow_chunk, &counters,
@@ -1929,6 +2084,26 @@ struct Job {
 };
 
 // Checks the parameters, uploads the job's event tables and sizes its buffers: everything before its kernels.
+// The entry of the marginal form of the pair kernel: a site where exactly one code is clean and the other is an
+// event code (4..16, or the gap under GAPMM) has the clean base subtracted from its nf field, because that base is
+// counted in the sequence's totals.  Such a site always adds at least 1 (x16) to that field (the clean row or column
+// of the site's cells sums to 1, or the site resolves to the base's diagonal cell, 2), so the result stays a
+// non-negative 16-bit field; false when it would not.  Order-dependent entries are left alone (their pairs are
+// replayed).
+static bool fold_entry(int a, int b, bool gapmm, uint32_t q[4])
+{
+  const bool ca = a < 4, cb = b < 4;
+  if (ca == cb || (q[3] >> 31)) return true;
+  const int other = ca ? b : a;
+  if (other == GAPC && !gapmm) return true;  // never an event: gap runs cover these sites
+  const int x = ca ? a : b;
+  uint32_t &w = q[x >> 1];
+  const int sh = (x & 1) * 16;
+  if (((w >> sh) & 0xffffu) < 16u) return false;
+  w -= 16u << sh;
+  return true;
+}
+
 static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
 {
   // with dry_bytes: only the sizes of the run's buffers are added up, nothing is allocated or uploaded
@@ -1967,6 +2142,9 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   A.nmask = d.nmask;
   A.nrun_off = d.nrun_off;
   A.nruns = d.nruns;
+  A.pre = d.pre;
+  A.grun_off = d.grun_off;
+  A.gruns = d.gruns;
   A.weights = d.weights;
   A.n = d.n;
   A.Ls = d.Ls;
@@ -2010,10 +2188,11 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   const int64_t npairs_box = nrows * A.ncols;
   J.npairs = A.triangle ? (d.n * (d.n - 1) - A.c0 * (A.c0 - 1)) / 2 : npairs_box;
   if (J.npairs >= (1ll << 32)) return c->fail(TN93G_E_ARG, "more than 2^32 pairs in one run");
-  // tables: [2][18*18] packed event entries (main mode, AVERAGE), [2][18*18] replay entries
+  // tables: [4][18*18] packed event entries (main mode, AVERAGE; then both folded), [2][18*18] replay entries
   if (!dry_bytes && (B->tab_mode != P->mode || B->tab_mask != P->subset_mask || !B->evtab.p || !B->rtab.p)) {
-    std::vector<uint4> et(2 * NCODE * NCODE);
+    std::vector<uint4> et(4 * NCODE * NCODE);
     std::vector<ReplayEnt> rt(2 * NCODE * NCODE);
+    bool fold_ok = true;
     for (int a = 0; a < NCODE; a++)
       for (int b = 0; b < NCODE; b++) {
         SiteEffect E = site_effect(a, b, P->mode, P->subset_mask);
@@ -2022,6 +2201,8 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
           uint32_t q[4];
           event_entry(F, q);
           et[(size_t)(t * NCODE * NCODE + a * NCODE + b)] = make_uint4(q[0], q[1], q[2], q[3]);
+          fold_ok = fold_entry(a, b, P->mode == TN93G_GAPMM, q) && fold_ok;
+          et[(size_t)((2 + t) * NCODE * NCODE + a * NCODE + b)] = make_uint4(q[0], q[1], q[2], q[3]);
           ReplayEnt &r = rt[(size_t)(t * NCODE * NCODE + a * NCODE + b)];
           r.v = F.v;
           r.mask = F.mask;
@@ -2035,6 +2216,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
     HIPCHK(c, hipMemcpyAsync(B->rtab.p, rt.data(), rt.size() * sizeof(ReplayEnt), hipMemcpyHostToDevice, c->stream));
     B->tab_mode = P->mode;
     B->tab_mask = P->subset_mask;
+    B->tab_fold = fold_ok;
   }
   A.evtab = (const uint4 *)B->evtab.p;
   A.rtab = (const ReplayEnt *)B->rtab.p;
@@ -2108,18 +2290,29 @@ static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
   if (nrows > 0 && A.ncols > 0) {
     dim3 g((unsigned)J.gx, (unsigned)J.gy);
     const bool mf = J.P.backend == TN93G_BACKEND_MFMA && !A.nplane;  // dense-N counting is popcount only
-    if (A.nplane && J.fast)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, true, true>), g, dim3(256), 0, st, A);
-    else if (A.nplane)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, false, true>), g, dim3(256), 0, st, A);
-    else if (mf && J.fast)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, true, false>), g, dim3(256), 0, st, A);
+    const bool marg = !mf && B->tab_fold && !A.skip_events;
+    const int v = (J.fast ? 1 : 0) | (A.nplane ? 2 : 0) | (marg ? 4 : 0);
+    constexpr int PC = TN93G_BACKEND_POPC, MF = TN93G_BACKEND_MFMA;
+    if (mf && J.fast)
+      hipLaunchKernelGGL((k_pairs<MF, true, false, false>), g, dim3(256), 0, st, A);
     else if (mf)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_MFMA, false, false>), g, dim3(256), 0, st, A);
-    else if (J.fast)
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, true, false>), g, dim3(256), 0, st, A);
+      hipLaunchKernelGGL((k_pairs<MF, false, false, false>), g, dim3(256), 0, st, A);
+    else if (v == 0)
+      hipLaunchKernelGGL((k_pairs<PC, false, false, false>), g, dim3(256), 0, st, A);
+    else if (v == 1)
+      hipLaunchKernelGGL((k_pairs<PC, true, false, false>), g, dim3(256), 0, st, A);
+    else if (v == 2)
+      hipLaunchKernelGGL((k_pairs<PC, false, true, false>), g, dim3(256), 0, st, A);
+    else if (v == 3)
+      hipLaunchKernelGGL((k_pairs<PC, true, true, false>), g, dim3(256), 0, st, A);
+    else if (v == 4)
+      hipLaunchKernelGGL((k_pairs<PC, false, false, true>), g, dim3(256), 0, st, A);
+    else if (v == 5)
+      hipLaunchKernelGGL((k_pairs<PC, true, false, true>), g, dim3(256), 0, st, A);
+    else if (v == 6)
+      hipLaunchKernelGGL((k_pairs<PC, false, true, true>), g, dim3(256), 0, st, A);
     else
-      hipLaunchKernelGGL((k_pairs<TN93G_BACKEND_POPC, false, false>), g, dim3(256), 0, st, A);
+      hipLaunchKernelGGL((k_pairs<PC, true, true, true>), g, dim3(256), 0, st, A);
   } else {
     fill_zero(st, B->slow_cnt.p, (size_t)J.ntiles * 4);
   }
@@ -2498,6 +2691,8 @@ int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
   SCHK(hipMalloc(&d.ev_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.evg_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.nrun_off, (size_t)(nn + 1) * sizeof(int32_t)));
+  SCHK(hipMalloc(&d.grun_off, (size_t)(nn + 1) * sizeof(int32_t)));
+  SCHK(hipMalloc(&d.pre, (size_t)(nn * (d.W + 1)) * sizeof(uint4)));
   SCHK(hipMalloc(&d.nmask, (size_t)(nn * d.W) * sizeof(uint32_t)));
   SCHK(hipMalloc(&d.weights, (size_t)nn * sizeof(int64_t)));
   // padding reads as gaps (codes, site-major codes) and as nothing (planes, one-hot operands, event offsets)
@@ -2508,6 +2703,7 @@ int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
   SCHK(hipMemsetAsync(d.ev_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
   SCHK(hipMemsetAsync(d.evg_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
   SCHK(hipMemsetAsync(d.nrun_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
+  SCHK(hipMemsetAsync(d.grun_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
   SCHK(hipMemsetAsync(d.nmask, 0, (size_t)(nn * d.W) * sizeof(uint32_t), c->stream));
   SCHK(hipStreamSynchronize(c->stream));
 #undef SCHK
@@ -2543,6 +2739,7 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
     const int64_t tot = n * d.W;
     hipLaunchKernelGGL(k_planes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.planes,
                        d.nmask, n0, n1, L, d.Ls, d.W);
+    hipLaunchKernelGGL(k_prefix, dim3((unsigned)n), dim3(64), 0, c->stream, (const uint4 *)d.planes, d.pre, d.W, n0);
     const int64_t ob = n0 & ~(int64_t)3, oe = (n1 + 3) & ~(int64_t)3;
     const int64_t toh = (oe - ob) * d.W64 * 64;
     hipLaunchKernelGGL(k_onehot, dim3((unsigned)((toh + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.onehot, n1,
@@ -2569,17 +2766,19 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
     d.ndense = nsites >= 32 * n ? 1 : 0;
     S->decided = true;
   }
-  // lists: 0 = ambiguity events (without N when dense), 1 = with gap sites (GAPMM), 2 = N runs (dense N only)
-  for (int li = 0; li < 3 && !rc; li++) {
-    if (li == 2 && !d.ndense) break;
-    int32_t *eoff = li == 0 ? d.ev_off : li == 1 ? d.evg_off : d.nrun_off;
-    const size_t esz = li == 2 ? sizeof(uint2) : sizeof(uint32_t);
-    void *arr = li == 0 ? (void *)d.ev : li == 1 ? (void *)d.evg : (void *)d.nruns;
+  // lists: 0 = ambiguity events (without N when dense), 1 = with gap sites (GAPMM), 2 = N runs (dense N only),
+  // 3 = gap runs
+  for (int li = 0; li < 4 && !rc; li++) {
+    if (li == 2 && !d.ndense) continue;
+    int32_t *eoff = li == 0 ? d.ev_off : li == 1 ? d.evg_off : li == 2 ? d.nrun_off : d.grun_off;
+    const size_t esz = li >= 2 ? sizeof(uint2) : sizeof(uint32_t);
+    void *arr = li == 0 ? (void *)d.ev : li == 1 ? (void *)d.evg : li == 2 ? (void *)d.nruns : (void *)d.gruns;
     const int mode = li == 1 ? EV_GAPS : d.ndense ? EV_NO_N : EV_AMB;
+    const int rcode = li == 2 ? NC : GAPC;
     if (li < 2)
       hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, mode, n0);
     else
-      hipLaunchKernelGGL(k_nruncount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, n0);
+      hipLaunchKernelGGL(k_runcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, n0, rcode);
     std::vector<int32_t> cnt((size_t)n, 0), off((size_t)n, 0);
     if (hipMemcpyAsync(cnt.data(), cnt_d, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream) !=
             hipSuccess ||
@@ -2610,7 +2809,8 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
       arr = na;
       if (li == 0) d.ev = (uint32_t *)na;
       else if (li == 1) d.evg = (uint32_t *)na;
-      else d.nruns = (uint2 *)na;
+      else if (li == 2) d.nruns = (uint2 *)na;
+      else d.gruns = (uint2 *)na;
       S->ev_cap[li] = want;
     }
     if (hipMemcpyAsync(eoff + n0 + 1, off.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream) !=
@@ -2622,8 +2822,8 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
       hipLaunchKernelGGL(k_evfill, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, eoff, (uint32_t *)arr, L, d.Ls,
                          mode, n0);
     else
-      hipLaunchKernelGGL(k_nrunfill, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, eoff, (uint2 *)arr, L, d.Ls,
-                         n0);
+      hipLaunchKernelGGL(k_runfill, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, eoff, (uint2 *)arr, L, d.Ls,
+                         n0, rcode);
     if (hipStreamSynchronize(c->stream) != hipSuccess) {  // `off` is pageable host memory
       rc = c->fail(TN93G_E_HIP, "event packing failed");
       break;
@@ -2669,8 +2869,9 @@ void tn93g_set_free(tn93g_set *S)
   if (!S) return;
   (void)hipSetDevice(S->ctx->device);
   (void)hipDeviceSynchronize();  // runs of any context may still read the set
-  void *ps[] = {S->d.codes,   S->d.codesT, S->d.planes,   S->d.onehot, S->d.ev_off, S->d.ev,
-                S->d.evg_off, S->d.evg,    S->d.nrun_off, S->d.nruns,  S->d.nmask,  S->d.weights};
+  void *ps[] = {S->d.codes,   S->d.codesT, S->d.planes,   S->d.onehot, S->d.ev_off,   S->d.ev,    S->d.evg_off,
+                S->d.evg,     S->d.nrun_off, S->d.nruns, S->d.nmask,  S->d.grun_off, S->d.gruns, S->d.pre,
+                S->d.weights};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (void *p : S->retired) (void)hipFree(p);

@@ -338,3 +338,48 @@ def test_dense_n_fast_equals_replay(engine, mode):
     assert slow["n_replayed"] == 160 * 159 // 2
     assert (fast["link_dist"] == slow["link_dist"]).all()
     s.free()
+
+
+def _edge_codes(seed, n, L, dense_n):
+    """Random bases with 3 % ambiguity codes (B/D/H/V and U included), leading, trailing and inner gap runs (some
+    reaching the first or the last site), and with dense_n an N block per sequence."""
+    rs = gen.Stream(seed)
+    codes = rs.integers(4, n * L).reshape(n, L).astype(np.uint8)
+    amb = rs.uniform(n * L).reshape(n, L) < 0.03
+    pool = np.array([4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16], dtype=np.uint8)
+    pick = pool[rs.integers(len(pool), n * L).reshape(n, L)]
+    codes[amb] = pick[amb]
+    for i in range(n):
+        u = rs.uniform(10)
+        if u[0] < 0.4:
+            codes[i, :1 + int(u[1] * min(L, 70))] = gen.GAP
+        if u[2] < 0.4:
+            codes[i, L - 1 - int(u[3] * min(L - 1, 70)):] = gen.GAP
+        if u[4] < 0.6:
+            a = int(u[5] * L)
+            codes[i, a:min(L, a + 1 + int(u[6] * 40))] = gen.GAP
+        if dense_n and u[7] < 0.8:
+            a = int(u[8] * L)
+            codes[i, a:min(L, a + 40 + int(u[9] * 60))] = 15
+    return codes
+
+
+@pytest.mark.parametrize("L,dense", [(31, False), (32, False), (33, False), (64, False), (96, False), (300, False),
+                                     (160, True), (256, True)])
+def test_marginal_counts_edges(engine, L, dense):
+    """The popcount kernel's base sums from per-sequence prefix totals (corrected at events, N runs and gap runs)
+    against the site-order replay of every pair and against the MFMA backend's direct 4x4 counts, at lengths on and
+    around word boundaries with gap runs touching both ends."""
+    codes = _edge_codes(61 + L, 150, L, dense)
+    s = engine.upload(codes)
+    for mode in (T.RESOLVE, T.AVERAGE, T.GAPMM, T.SUBSET):
+        print(f"L={L} dense={dense} mode={mode}", flush=True)
+        kw = dict(mode=mode, threshold=1.0, min_overlap=5, resolve_fraction=0.05 if mode == T.RESOLVE else 1.0,
+                  subset_mask=(1 << 5) | (1 << 6) | (1 << 15))
+        fast = engine.run(s, backend=T.BACKEND_POPC, **kw)
+        slow = engine.run(s, flags=T.FLAG_ALL_SLOW, **kw)
+        mf = engine.run(s, backend=T.BACKEND_MFMA, **kw)
+        _same(fast, slow)
+        _same(fast, mf)
+        assert (fast["link_dist"] == mf["link_dist"]).all()
+    s.free()
