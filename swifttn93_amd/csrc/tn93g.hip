@@ -188,7 +188,8 @@ struct SetDev {
   uint32_t *nmask;   // [n][W] N sites
   int32_t *nrun_off; // [n+1] N runs, for sets with dense N (ndense)
   uint2 *nruns;
-  uint4 *pre;        // [n][W+1] plane counts before each word boundary; pre[s][W] = totals of sequence s
+  uint4 *pre;        // [n][W+1] plane counts before each word boundary
+  uint4 *tb;         // [n] pre[s][W], the totals of sequence s
   int32_t *grun_off; // [n+1] gap runs
   uint2 *gruns;
   int ndense;        // N sites are not in ev: handled through nmask (pair kernel) and nruns (replay)
@@ -238,6 +239,7 @@ struct RunArgs {
   const uint2 *nruns;
   int nplane;              // N sites are counted from nmask / nruns, not in the event lists
   const uint4 *pre;        // [n][W+1] plane counts (clean, C|T, G|T, T) of the words before each word boundary
+  const uint4 *tb;         // [n] the same over all words (the totals)
   const int32_t *grun_off; // [n+1] and the gap runs [start, end) of each sequence
   const uint2 *gruns;
   const int64_t *weights;
@@ -552,7 +554,8 @@ __global__ void k_runfill(const uint8_t *__restrict__ codes, const int32_t *__re
 
 // pre[s][w] = plane counts (clean, C|T, G|T, T) of sequence s over its words [0, w), w = 0..W: one wave per sequence,
 // 64 word boundaries per step
-__global__ void k_prefix(const uint4 *__restrict__ planes, uint4 *__restrict__ pre, int64_t W, int64_t s0)
+__global__ void k_prefix(const uint4 *__restrict__ planes, uint4 *__restrict__ pre, uint4 *__restrict__ tb, int64_t W,
+                         int64_t s0)
 {
   const int64_t s = s0 + blockIdx.x;
   const int lane = threadIdx.x;
@@ -571,6 +574,7 @@ __global__ void k_prefix(const uint4 *__restrict__ planes, uint4 *__restrict__ p
     }
     add4(v, carry);
     if (w <= W) pre[s * (W + 1) + w] = v;
+    if (w == W) tb[s] = v;
     carry = make_uint4(__shfl(v.x, 63, 64), __shfl(v.y, 63, 64), __shfl(v.z, 63, 64), __shfl(v.w, 63, 64));
   }
 }
@@ -737,22 +741,70 @@ __device__ inline void stage_planes(const RunArgs &A, LDS &lds, int64_t i0, int6
   }
 }
 
-// Event lists of a tile: the rows' and the columns' lists are each one contiguous range of the CSR array; when
-// both fit they are read from the LDS copy in sEv (rows at [0, nr), columns at [nr, ...)), else from HBM.
-struct TileEv {
-  const uint32_t *s;
-  int er0, ec0, nr;
-  bool lds;
-  __device__ inline uint32_t row(const RunArgs &A, int e) const { return lds ? s[e - er0] : A.ev[e]; }
-  __device__ inline uint32_t col(const RunArgs &A, int e) const { return lds ? s[nr + e - ec0] : A.ev[e]; }
+// Events of a tile.  The rows' and the columns' event lists are each one contiguous range of the set's CSR array:
+// rows [er0, er0 + nr), columns [ec0, ec0 + nc).  They are taken in chunks that fit the LDS: chunk rows [r0, r1) and
+// columns [c0, c1) of those ranges (relative), row events first.  A chunk holds its event words in sEv (rows, then
+// columns) and, in the LDS of the counting buffers, the codes its events face: row event e the codes of the tile's
+// 64 columns at its site (g[64 e ...]), column event c the codes of the tile's 16 rows at its site
+// (g[64 R + 16 c ...], R = r1 - r0).  Almost every tile is one chunk.
+constexpr int GCAP = 30 * 1024;
+template <bool MF>
+union TileLds {
+  PairLds<KW, MF> p;
+  uint8_t g[GCAP];
 };
+
+struct TileEv {
+  int er0, ec0, nr, nc;
+  int r0, r1, c0, c1;
+  __device__ inline void next()
+  {
+    r0 = r1;
+    c0 = c1;
+    const int R = min(nr - r0, min(GCAP / 64, EVCAP));
+    r1 = r0 + R;
+    c1 = c0 + min(nc - c0, min((GCAP - 64 * R) / 16, EVCAP - R));
+  }
+  __device__ inline void first()
+  {
+    r1 = c1 = 0;
+    next();
+  }
+  __device__ inline bool last() const { return r1 == nr && c1 == nc; }
+  __device__ inline bool single() const { return r0 == 0 && c0 == 0 && last(); }
+};
+
+// The chunk's event words into sEv.
+__device__ inline void chunk_events(const RunArgs &A, const TileEv &te, uint32_t *sEv, int tid)
+{
+  const int R = te.r1 - te.r0, C = te.c1 - te.c0;
+  for (int k = tid; k < R; k += 256) sEv[k] = A.ev[te.er0 + te.r0 + k];
+  for (int k = tid; k < C; k += 256) sEv[R + k] = A.ev[te.ec0 + te.c0 + k];
+}
+
+// The codes the chunk's events face, from the site-major codes into g (16 bytes per load, all loads of a thread in
+// flight together).
+__device__ __attribute__((always_inline)) inline void chunk_codes(const RunArgs &A, const TileEv &te,
+                                                                   const uint32_t *sEv, uint8_t *g, int64_t i0,
+                                                                   int64_t j0, int tid)
+{
+  const size_t np = (size_t)A.npad;
+  const int R = te.r1 - te.r0, C = te.c1 - te.c0;
+  const int nrow = 4 * R, tot = nrow + C;
+#pragma unroll 4
+  for (int k = tid; k < tot; k += 256) {
+    const uint8_t *src = k < nrow ? A.codesT + (size_t)(sEv[k >> 2] >> 5) * np + j0 + (k & 3) * 16
+                                  : A.codesT + (size_t)(sEv[R + k - nrow] >> 5) * np + i0;
+    *reinterpret_cast<uint4 *>(g + k * 16) = *reinterpret_cast<const uint4 *>(src);
+  }
+}
 
 // Event tables in LDS, TABW entries per event code; code EVNONE is an all-zero row that pads partial batches.
 //   sTabR[c][x]  a row event of code c facing the column's code x (any code: a site where both sequences are
 //                ambiguous is counted here);
 //   sTabC[c][x]  a column event of code c facing the row's code x, zero where x is ambiguous (that site is counted
 //                as the row's event).
-constexpr int TABW = 32, EVNONE = NCODE, EVB = 4;
+constexpr int TABW = NCODE + 1, EVNONE = NCODE, EVB = 4;
 constexpr uint32_t EV_PAD = (uint32_t)EVNONE;  // site 0, code EVNONE
 
 // Fills the LDS event tables from one [NCODE][NCODE] table of packed entries (entry [row code][column code]).
@@ -772,64 +824,61 @@ __device__ inline void load_event_tables(const uint4 *src, uint4 *sTabR, uint4 *
 
 __device__ inline uint4 keep_if(bool k, uint4 t) { return k ? t : make_uint4(0, 0, 0, 0); }
 
-// The ambiguity events of the wave's RPW rows against the lane's column, over all sites, after the clean/clean
-// counts.  Row events are uniform over the wave: each lane reads its column's code at the event's site from the
-// site-major codes (one coalesced 64-byte segment per event).  Column events are per lane: the codes of the wave's
-// RPW rows at the site are one 32-bit word of the site-major codes.  Events go in batches of EVB so that their loads
-// are in flight together.  The packed 16-bit fields are flushed every 1024 events, before any field (at most 32 per
-// event) can overflow.  MASKED: only the (row, lane) pairs with use[a] set take the entries (the others add zero),
-// and rows or lanes with no such pair skip the loops.
+// The events of one chunk for the wave's RPW rows against the lane's column, every operand in LDS.  Row events
+// are uniform over the wave; column events are per lane, the codes of the wave's rows at the site being one 32-bit
+// word of the event's 16-byte record.  sRo[TI + 1]: the rows' event offsets relative to the tile's row list.  The
+// packed 16-bit fields are flushed at the end of the chunk and every 1024 column events, before any field (at most
+// 32 per event) can overflow (a chunk holds at most GCAP / 64 row events).  MASKED: only the (row, lane) pairs with
+// use[a] set take the entries (the others add zero), and rows or lanes with no such pair skip the loops.
 template <bool MASKED>
-__device__ inline void tile_events(const RunArgs &A, const uint4 *sTabR, const uint4 *sTabC, const TileEv &te,
-                                   int64_t ib, int64_t j, int pj, int pj_end, const bool (&use)[RPW],
-                                   EvAcc (&acc)[RPW], Tot (&tot)[RPW])
+__device__ __attribute__((always_inline)) inline void chunk_apply(const RunArgs &A, const uint4 *sTabR,
+                                                                  const uint4 *sTabC, const TileEv &te,
+                                                                  const uint32_t *sEv, const uint8_t *g,
+                                                                  const int *sRo, int64_t ib, int wave, int lane,
+                                                                  int pj, int pj_end, const bool (&use)[RPW],
+                                                                  EvAcc (&acc)[RPW], Tot (&tot)[RPW])
 {
-  static_assert(RPW == 4, "column events read the wave's rows as one 32-bit word");
-  const size_t np = (size_t)A.npad;
-  const uint32_t jl = (uint32_t)j;  // < npad
+  const int R = te.r1 - te.r0;
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
     const int64_t i = ib + a;
     if (i >= A.r1) continue;  // uniform over the wave
     if (MASKED && !__ballot(use[a])) continue;
-    const int e0 = __builtin_amdgcn_readfirstlane(A.ev_off[i]);
-    const int e1 = __builtin_amdgcn_readfirstlane(A.ev_off[i + 1]);
-    for (int f0 = e0; f0 < e1; f0 += 1024) {
-      const int f1 = min(e1, f0 + 1024);
-      for (int e = f0; e < f1; e += EVB) {
-        uint32_t ev[EVB], cj[EVB];
-#pragma unroll
-        for (int k = 0; k < EVB; k++)
-          ev[k] = (uint32_t)__builtin_amdgcn_readfirstlane((int)(e + k < f1 ? te.row(A, e + k) : EV_PAD));
-#pragma unroll
-        for (int k = 0; k < EVB; k++) cj[k] = (A.codesT + (size_t)(ev[k] >> 5) * np)[jl];
-#pragma unroll
-        for (int k = 0; k < EVB; k++) {
-          const uint4 t = sTabR[(ev[k] & 31u) * TABW + cj[k]];
-          ev_add(acc[a], MASKED ? keep_if(use[a], t) : t);
-        }
-      }
-      ev_flush(acc[a], tot[a]);
-    }
-  }
-  if (MASKED && !(use[0] || use[1] || use[2] || use[3])) return;
-  const uint8_t *cti = A.codesT + ib;
-  for (int f0 = pj; f0 < pj_end; f0 += 1024) {
-    const int f1 = min(pj_end, f0 + 1024);
-    for (int e = f0; e < f1; e += EVB) {
-      uint32_t ev[EVB], r4[EVB];
-#pragma unroll
-      for (int k = 0; k < EVB; k++) ev[k] = e + k < f1 ? te.col(A, e + k) : EV_PAD;
-#pragma unroll
-      for (int k = 0; k < EVB; k++) r4[k] = *reinterpret_cast<const uint32_t *>(cti + (size_t)(ev[k] >> 5) * np);
+    const int e0 = max(sRo[wave * RPW + a], te.r0) - te.r0, e1 = min(sRo[wave * RPW + a + 1], te.r1) - te.r0;
+    if (e0 >= e1) continue;
+    int e = e0;
+    for (; e + EVB <= e1; e += EVB) {
+      uint32_t c[EVB], x[EVB];
 #pragma unroll
       for (int k = 0; k < EVB; k++) {
-        const uint4 *t = sTabC + (ev[k] & 31u) * TABW;
+        c[k] = sEv[e + k] & 31u;
+        x[k] = g[(e + k) * 64 + lane];
+      }
 #pragma unroll
-        for (int a = 0; a < RPW; a++) {
-          const uint4 v = t[(r4[k] >> (8 * a)) & 255u];
-          ev_add(acc[a], MASKED ? keep_if(use[a], v) : v);
-        }
+      for (int k = 0; k < EVB; k++) {
+        const uint4 t = sTabR[c[k] * TABW + x[k]];
+        ev_add(acc[a], MASKED ? keep_if(use[a], t) : t);
+      }
+    }
+    for (; e < e1; e++) {
+      const uint4 t = sTabR[(sEv[e] & 31u) * TABW + g[e * 64 + lane]];
+      ev_add(acc[a], MASKED ? keep_if(use[a], t) : t);
+    }
+    ev_flush(acc[a], tot[a]);
+  }
+  if (MASKED && !(use[0] || use[1] || use[2] || use[3])) return;
+  const int k0 = max(pj - te.ec0, te.c0) - te.c0, k1 = min(pj_end - te.ec0, te.c1) - te.c0;
+  const uint32_t *evc = sEv + R;
+  const uint8_t *gc = g + R * 64 + wave * RPW;
+  for (int f0 = k0; f0 < k1; f0 += 1024) {
+    const int f1 = min(k1, f0 + 1024);
+    for (int k = f0; k < f1; k++) {
+      const uint4 *t = sTabC + (evc[k] & 31u) * TABW;
+      const uint32_t r4 = *reinterpret_cast<const uint32_t *>(gc + k * 16);
+#pragma unroll
+      for (int a = 0; a < RPW; a++) {
+        const uint4 v = t[(r4 >> (8 * a)) & 255u];
+        ev_add(acc[a], MASKED ? keep_if(use[a], v) : v);
       }
     }
 #pragma unroll
@@ -837,9 +886,29 @@ __device__ inline void tile_events(const RunArgs &A, const uint4 *sTabR, const u
   }
 }
 
-// Dense N (NP): the pair's sites where either sequence is N are counted from the N planes, by the partner's base:
-// nr[a] = row N against the column's clean bases (clean, C|T, G|T, T counts as in count_word), nc[a] = column N
-// against the row's clean bases, nn[a] = both N.  nplane_add folds them into the totals through the table entries.
+// All events of the tile, chunk by chunk.  The first chunk's event words and codes are in LDS already (ready); every
+// further chunk is loaded between barriers, so all threads of the block call this together.
+template <bool MASKED>
+__device__ __attribute__((always_inline)) inline void tile_events(const RunArgs &A, const uint4 *sTabR,
+                                                                  const uint4 *sTabC, TileEv &te, uint32_t *sEv,
+                                                                  uint8_t *g, const int *sRo, int64_t i0, int64_t j0,
+                                                                  int64_t ib, int tid, int pj, int pj_end,
+                                                                  const bool (&use)[RPW], EvAcc (&acc)[RPW],
+                                                                  Tot (&tot)[RPW])
+{
+  const int wave = tid >> 6, lane = tid & 63;
+  for (;;) {
+    chunk_apply<MASKED>(A, sTabR, sTabC, te, sEv, g, sRo, ib, wave, lane, pj, pj_end, use, acc, tot);
+    if (te.last()) return;
+    te.next();
+    __syncthreads();
+    chunk_events(A, te, sEv, tid);
+    __syncthreads();
+    chunk_codes(A, te, sEv, g, i0, j0, tid);
+    __syncthreads();
+  }
+}
+
 struct NCnt {
   uint32_t r[4], c[4], nn;
 };
@@ -858,7 +927,7 @@ __device__ inline void tot_add_mul(Tot &t, EvAcc &acc, const uint4 e, uint32_t k
   acc.fl |= e.w & 0x80000000u;
 }
 
-__device__ inline void nplane_add(const NCnt &q, const uint4 *sTabR, Tot &t, EvAcc &acc)
+__device__ __attribute__((always_inline)) inline void nplane_add(const NCnt &q, const uint4 *sTabR, Tot &t, EvAcc &acc)
 {
   // base counts A, C, G, T from (clean, C|T, G|T, T)
   const uint32_t rb[4] = {q.r[0] - q.r[1] - q.r[2] + q.r[3], q.r[1] - q.r[3], q.r[2] - q.r[3], q.r[3]};
@@ -871,38 +940,47 @@ __device__ inline void nplane_add(const NCnt &q, const uint4 *sTabR, Tot &t, EvA
   tot_add_mul(t, acc, sTabR[NC * TABW + NC], q.nn);
 }
 
-// Base sums of the marginal form: S[a] = plane totals of row ib + a and of the lane's column, less the clean sites
-// of each facing a gap run of the other (gaps are not events outside GAPMM, so no table entry sees them).
-__device__ inline void base_sums(const RunArgs &A, int64_t ib, int64_t j, uint4 (&S)[RPW])
+// Gap runs of a tile: the rows' runs then the columns' runs (each a contiguous range of the set's run list), in LDS
+// when they fit.
+constexpr int GRCAP = 128;
+struct TileRuns {
+  const uint2 *s;
+  int gr0, nrr, gc0;
+  bool lds;
+  __device__ inline uint2 row(const RunArgs &A, int g) const { return lds ? s[g] : A.gruns[gr0 + g]; }
+  __device__ inline uint2 col(const RunArgs &A, int g) const { return lds ? s[nrr + g - gc0] : A.gruns[g]; }
+};
+
+// Base sums of the marginal form: S[a] = plane totals of row ib + a and of the lane's column (sTB: the tile's rows,
+// then its columns), less the clean sites of each facing a gap run of the other (gaps are not events outside GAPMM,
+// so no table entry sees them).  sRg[TI + 1]: the rows' run offsets relative to the tile's row runs; [h0, h1): the
+// column's runs in the set's list.
+__device__ __attribute__((always_inline)) inline void base_sums(const RunArgs &A, const uint4 *sTB, const TileRuns &tr, const int *sRg, int64_t ib,
+                                 int wave, int lane, int64_t j, int h0, int h1, uint4 (&S)[RPW])
 {
   const int64_t W = A.W;
   const bool jin = j < A.n;
-  const uint4 tj = jin ? A.pre[j * (W + 1) + W] : make_uint4(0, 0, 0, 0);
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
-    const int64_t i = ib + a;
-    S[a] = tj;
-    if (i < A.r1) add4(S[a], A.pre[i * (W + 1) + W]);
+    S[a] = sTB[TI + lane];
+    add4(S[a], sTB[wave * RPW + a]);
   }
   if (A.gapmm) return;
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
     const int64_t i = ib + a;
     if (i >= A.r1) continue;  // uniform over the wave
-    const int g0 = __builtin_amdgcn_readfirstlane(A.grun_off[i]);
-    const int g1 = __builtin_amdgcn_readfirstlane(A.grun_off[i + 1]);
+    const int g0 = sRg[wave * RPW + a], g1 = sRg[wave * RPW + a + 1];
     for (int g = g0; g < g1; g++) {
-      const uint2 r = A.gruns[g];
+      const uint2 r = tr.row(A, g);
       if (jin) {
         sub4(S[a], plane_prefix(A.pre, A.planes, W, j, r.y));
         add4(S[a], plane_prefix(A.pre, A.planes, W, j, r.x));
       }
     }
   }
-  if (!jin) return;
-  const int h0 = A.grun_off[j], h1 = A.grun_off[j + 1];
   for (int g = h0; g < h1; g++) {
-    const uint2 r = A.gruns[g];
+    const uint2 r = tr.col(A, g);
 #pragma unroll
     for (int a = 0; a < RPW; a++) {
       const int64_t i = ib + a;
@@ -932,7 +1010,8 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     lslow = 0;
     bs.unc = bs.undecided = 0;
   }
-  __shared__ PairLds<KW, MF> lds;
+  __shared__ TileLds<MF> L;
+  PairLds<KW, MF> &lds = L.p;
   __shared__ uint32_t sNI[NP ? KW : 1][TI + 1], sNJ[NP ? KW : 1][TJ + 1];
   __shared__ uint4 sTabR[(NCODE + 1) * TABW], sTabC[(NCODE + 1) * TABW];
   __shared__ unsigned long long lhist[FAST ? 1 : 3 * 200];
@@ -962,21 +1041,48 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   const bool do_events = do_fast && !A.skip_events;
   // column event cursors (columns past the set have none)
   const int pj = j < A.n ? A.ev_off[j] : 0, pj_end = j < A.n ? A.ev_off[j + 1] : 0;
-  // the tile's event lists into LDS when they fit (synchronised by the first barrier of the counts below)
+  // the tile's event lists, the rows' event offsets, and (marginal form) the tile's plane totals and gap runs into
+  // LDS when they fit (synchronised by the first barrier of the counts below)
   __shared__ uint32_t sEv[EVCAP];
+  __shared__ int sRo[TI + 1], sRg[MARG ? TI + 1 : 1];
+  __shared__ uint4 sTB[MARG ? TI + TJ : 1];
+  __shared__ uint2 sGR[MARG ? GRCAP : 1];
   TileEv te;
+  TileRuns tr{sGR, 0, 0, 0, false};
+  int h0 = 0, h1 = 0;
   {
     const int64_t rl = i0 < A.r1 ? min(i0 + (int64_t)TI, A.r1) : i0;
     const int64_t cl = min(j0 + (int64_t)TJ, A.n);
-    te.s = sEv;
     te.er0 = i0 < A.r1 ? A.ev_off[i0] : 0;
     te.nr = i0 < A.r1 ? A.ev_off[rl] - te.er0 : 0;
     te.ec0 = j0 < A.n ? A.ev_off[j0] : 0;
-    const int nc = j0 < A.n ? A.ev_off[cl] - te.ec0 : 0;
-    te.lds = do_events && te.nr + nc <= EVCAP;
-    if (te.lds) {
-      for (int k = tid; k < te.nr; k += 256) sEv[k] = A.ev[te.er0 + k];
-      for (int k = tid; k < nc; k += 256) sEv[te.nr + k] = A.ev[te.ec0 + k];
+    te.nc = j0 < A.n ? A.ev_off[cl] - te.ec0 : 0;
+    te.first();
+    if (do_events) {
+      chunk_events(A, te, sEv, tid);
+      if (tid <= TI) sRo[tid] = (i0 < A.r1 ? A.ev_off[min(i0 + tid, rl)] : 0) - te.er0;
+    }
+    if constexpr (MARG) {
+      if (do_fast) {
+        if (tid < TI + TJ) {
+          const int64_t q = tid < TI ? i0 + tid : j0 + (tid - TI);
+          sTB[tid] = q < A.n ? A.tb[q] : make_uint4(0, 0, 0, 0);
+        }
+        if (j < A.n) {
+          h0 = A.grun_off[j];
+          h1 = A.grun_off[j + 1];
+        }
+        tr.gr0 = i0 < A.r1 ? A.grun_off[i0] : 0;
+        tr.nrr = i0 < A.r1 ? A.grun_off[rl] - tr.gr0 : 0;
+        tr.gc0 = j0 < A.n ? A.grun_off[j0] : 0;
+        const int ncr = j0 < A.n ? A.grun_off[cl] - tr.gc0 : 0;
+        tr.lds = tr.nrr + ncr <= GRCAP;
+        if (tr.lds) {
+          for (int k = tid; k < tr.nrr; k += 256) sGR[k] = A.gruns[tr.gr0 + k];
+          for (int k = tid; k < ncr; k += 256) sGR[tr.nrr + k] = A.gruns[tr.gc0 + k];
+        }
+        if (tid <= TI) sRg[tid] = (i0 < A.r1 ? A.grun_off[min(i0 + tid, rl)] : 0) - tr.gr0;
+      }
     }
   }
   if constexpr (!MF) {
@@ -1092,11 +1198,13 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
       }
     }
   }
+  __syncthreads();  // the event tables and lists are in LDS (also when no counts ran); the counting buffers are free
+  if (do_events) chunk_codes(A, te, sEv, L.g, i0, j0, tid);
   if constexpr (MARG)
-    if (do_fast) base_sums(A, ib, j, S);
-  __syncthreads();  // the event tables and lists are in LDS (also when no counts ran)
+    if (do_fast) base_sums(A, sTB, tr, sRg, ib, wave, lane, j, h0, h1, S);
+  __syncthreads();
   const bool all_rows[RPW] = {true, true, true, true};
-  if (do_events) tile_events<false>(A, sTabR, sTabC, te, ib, j, pj, pj_end, all_rows, acc, tot);
+  if (do_events) tile_events<false>(A, sTabR, sTabC, te, sEv, L.g, sRo, i0, j0, ib, tid, pj, pj_end, all_rows, acc, tot);
   if constexpr (NP)
     if (do_events)
 #pragma unroll
@@ -1129,7 +1237,14 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
         tot[a] = Tot{{0, 0, 0, 0}, 0, 0, 0, 0};
         acc[a] = EvAcc{0, 0, 0, 0, 0};
       }
-    tile_events<true>(A, sTabR, sTabC, te, ib, j, pj, pj_end, avg, acc, tot);
+    if (!te.single()) {  // the tile's first chunk again
+      te.first();
+      chunk_events(A, te, sEv, tid);
+      __syncthreads();
+      chunk_codes(A, te, sEv, L.g, i0, j0, tid);
+      __syncthreads();
+    }
+    tile_events<true>(A, sTabR, sTabC, te, sEv, L.g, sRo, i0, j0, ib, tid, pj, pj_end, avg, acc, tot);
     if constexpr (NP)
 #pragma unroll
       for (int a = 0; a < RPW; a++)
@@ -2143,6 +2258,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   A.nrun_off = d.nrun_off;
   A.nruns = d.nruns;
   A.pre = d.pre;
+  A.tb = d.tb;
   A.grun_off = d.grun_off;
   A.gruns = d.gruns;
   A.weights = d.weights;
@@ -2693,6 +2809,7 @@ int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
   SCHK(hipMalloc(&d.nrun_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.grun_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.pre, (size_t)(nn * (d.W + 1)) * sizeof(uint4)));
+  SCHK(hipMalloc(&d.tb, (size_t)nn * sizeof(uint4)));
   SCHK(hipMalloc(&d.nmask, (size_t)(nn * d.W) * sizeof(uint32_t)));
   SCHK(hipMalloc(&d.weights, (size_t)nn * sizeof(int64_t)));
   // padding reads as gaps (codes, site-major codes) and as nothing (planes, one-hot operands, event offsets)
@@ -2739,7 +2856,8 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
     const int64_t tot = n * d.W;
     hipLaunchKernelGGL(k_planes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.planes,
                        d.nmask, n0, n1, L, d.Ls, d.W);
-    hipLaunchKernelGGL(k_prefix, dim3((unsigned)n), dim3(64), 0, c->stream, (const uint4 *)d.planes, d.pre, d.W, n0);
+    hipLaunchKernelGGL(k_prefix, dim3((unsigned)n), dim3(64), 0, c->stream, (const uint4 *)d.planes, d.pre, d.tb, d.W,
+                       n0);
     const int64_t ob = n0 & ~(int64_t)3, oe = (n1 + 3) & ~(int64_t)3;
     const int64_t toh = (oe - ob) * d.W64 * 64;
     hipLaunchKernelGGL(k_onehot, dim3((unsigned)((toh + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.onehot, n1,
@@ -2870,7 +2988,7 @@ void tn93g_set_free(tn93g_set *S)
   (void)hipSetDevice(S->ctx->device);
   (void)hipDeviceSynchronize();  // runs of any context may still read the set
   void *ps[] = {S->d.codes,   S->d.codesT, S->d.planes,   S->d.onehot, S->d.ev_off,   S->d.ev,    S->d.evg_off,
-                S->d.evg,     S->d.nrun_off, S->d.nruns, S->d.nmask,  S->d.grun_off, S->d.gruns, S->d.pre,
+                S->d.evg,     S->d.nrun_off, S->d.nruns, S->d.nmask,  S->d.grun_off, S->d.gruns, S->d.pre, S->d.tb,
                 S->d.weights};
   for (void *p : ps)
     if (p) (void)hipFree(p);

@@ -383,3 +383,26 @@ def test_marginal_counts_edges(engine, L, dense):
         _same(fast, mf)
         assert (fast["link_dist"] == mf["link_dist"]).all()
     s.free()
+
+
+def test_event_chunks(engine):
+    """Tiles whose events overflow the LDS (15 % ambiguity codes over 2,000 columns: about 300 events per sequence,
+    several chunks per tile) and, under -g 0.05, most pairs computed again in AVERAGE over all chunks: the pair
+    kernel against the site-order replay of every pair and the MFMA backend."""
+    rs = gen.Stream(71)
+    n, L = 120, 2000
+    codes = rs.integers(4, n * L).reshape(n, L).astype(np.uint8)
+    amb = rs.uniform(n * L).reshape(n, L) < 0.15
+    pool = np.array([5, 6, 7, 8, 9, 10, 15], dtype=np.uint8)
+    codes[amb] = pool[rs.integers(len(pool), n * L).reshape(n, L)][amb]
+    codes[::7, :150] = gen.GAP
+    s = engine.upload(codes)
+    for mode, rf in ((T.RESOLVE, 0.05), (T.RESOLVE, 1.0), (T.AVERAGE, 1.0), (T.GAPMM, 1.0)):
+        print(f"mode={mode} rf={rf}", flush=True)
+        kw = dict(mode=mode, threshold=1.0, min_overlap=100, resolve_fraction=rf)
+        fast = engine.run(s, backend=T.BACKEND_POPC, **kw)
+        slow = engine.run(s, flags=T.FLAG_ALL_SLOW, **kw)
+        mf = engine.run(s, backend=T.BACKEND_MFMA, **kw)
+        _same(fast, slow)
+        _same(fast, mf)
+    s.free()
