@@ -127,10 +127,7 @@ class Engine:
                         flags, n2)
         r = ctypes.POINTER(Result)()
         self._check(self.lib.tn93g_run(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r)))
-        try:
-            return _to_py(r.contents)
-        finally:
-            self.lib.tn93g_result_free(r)
+        return _to_py(r, _ResultOwner(self.lib, r))
 
     def run_batch(self, jobs):
         """jobs: list of (SeqSet, dict of run() keyword arguments).  One tn93g_run_batch call; a list of results."""
@@ -139,12 +136,8 @@ class Engine:
         params = (Params * max(n, 1))(*[make_params(**kw) for _, kw in jobs])
         outs = (ctypes.POINTER(Result) * max(n, 1))()
         self._check(self.lib.tn93g_run_batch(self.ctx, n, sets, params, outs))
-        try:
-            return [_to_py(outs[k].contents) for k in range(n)]
-        finally:
-            for k in range(n):
-                if outs[k]:
-                    self.lib.tn93g_result_free(outs[k])
+        owners = [_ResultOwner(self.lib, outs[k]) for k in range(n)]
+        return [_to_py(outs[k], owners[k]) for k in range(n)]
 
     def close(self):
         if self.ctx:
@@ -185,20 +178,43 @@ class SeqSet:
             self.handle = None
 
 
-def _arr(ptr, n, dt):
+class _ResultOwner:
+    """Frees a tn93g_result when the last array viewing its link block is gone."""
+
+    def __init__(self, lib, ptr):
+        self.lib, self.ptr = lib, ptr
+
+    def __del__(self):
+        if self.ptr:
+            try:
+                self.lib.tn93g_result_free(self.ptr)
+            except Exception:  # interpreter shutdown
+                pass
+            self.ptr = None
+
+
+class _View:
+    def __init__(self, owner, addr, n, typestr):
+        self.owner = owner  # keeps the result alive as long as the array
+        self.__array_interface__ = {"shape": (n,), "typestr": typestr, "data": (addr, False), "version": 3}
+
+
+def _arr(owner, ptr, n, dt):
+    """The n values at ptr as a numpy array that shares the library's memory (no copy)."""
     if n == 0:
         return np.zeros(0, dtype=dt)
-    return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+    return np.asarray(_View(owner, ctypes.cast(ptr, ctypes.c_void_p).value, n, np.dtype(dt).str))
 
 
-def _to_py(r):
+def _to_py(rp, owner):
+    r = rp.contents
     n = r.n_links
     return {
         "exact": bool(r.exact),
         "hist": np.ctypeslib.as_array(r.hist).astype(np.uint64).reshape(3, 200).copy(),
-        "link_row": _arr(r.link_row, n, np.int64), "link_col": _arr(r.link_col, n, np.int64),
-        "link_dist": _arr(r.link_dist, n, np.float64), "link_lo": _arr(r.link_dist_lo, n, np.float64),
-        "link_hi": _arr(r.link_dist_hi, n, np.float64),
+        "link_row": _arr(owner, r.link_row, n, np.int64), "link_col": _arr(owner, r.link_col, n, np.int64),
+        "link_dist": _arr(owner, r.link_dist, n, np.float64), "link_lo": _arr(owner, r.link_dist_lo, n, np.float64),
+        "link_hi": _arr(owner, r.link_dist_hi, n, np.float64),
         "mean_lo": list(r.mean_lo), "mean_hi": list(r.mean_hi), "weighted": list(r.weighted),
         "max_lo": list(r.max_lo), "max_hi": list(r.max_hi), "found_links": r.found_links, "skipped": r.skipped,
         "n_replayed": r.n_replayed, "n_uncertain": r.n_uncertain, "n_host_exact": r.n_host_exact,

@@ -301,6 +301,7 @@ __device__ inline void append_link(const RunArgs &A, int64_t i, int64_t j, doubl
   const unsigned st = stripe_of_block();
   const unsigned k = atomicAdd(&A.stripes[st * 32 + ST_LINKS], 1u);
   if (k < A.link_cap) A.links[(size_t)st * A.link_cap + k] = LinkRec{i, j, x, lo, hi};
+  atomicAdd(&A.row_links[i - A.r0], 1);  // per-row counts, for the ordering of the links
 }
 
 // block-local counters flushed once per block into its stripe
@@ -1806,13 +1807,99 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
 }
 
 // the links of the NSTRIPE regions, concatenated into `out` (one block per region)
-__global__ void k_link_gather(RunArgs A, LinkRec *__restrict__ out)
+__global__ void k_link_gather(RunArgs A, LinkRec *__restrict__ out, const int64_t *__restrict__ row_off,
+                              uint32_t *__restrict__ slot)
 {
   const int st = blockIdx.x;
   int64_t off = 0;
   for (int t = 0; t < st; t++) off += min(A.stripes[t * 32 + ST_LINKS], A.link_cap);
   const unsigned cnt = min(A.stripes[st * 32 + ST_LINKS], A.link_cap);
-  for (unsigned k = threadIdx.x; k < cnt; k += blockDim.x) out[off + k] = A.links[(size_t)st * A.link_cap + k];
+  const int64_t cap = (int64_t)A.link_cap * NSTRIPE;
+  for (unsigned k = threadIdx.x; k < cnt; k += blockDim.x) {
+    const LinkRec L = A.links[(size_t)st * A.link_cap + k];
+    out[off + k] = L;
+    const int64_t r = L.i - A.r0;
+    const int64_t pos = row_off[r] + atomicSub(&A.row_links[r], 1) - 1;  // the row's count serves as its cursor
+    if (pos < cap) slot[pos] = (uint32_t)(off + k);
+  }
+}
+
+// The links in tn93's emission order (rows ascending, then columns ascending), into out.  Row r's links are
+// slot[row_off[r] .. row_off[r + 1]) (indices into flat); their rank within the row is their column's rank.  Up to 64
+// links: ranks by comparison across one wave.  More: a bitmap of the row's link columns, BMW words per window of
+// columns, and a prefix count of its words.  Columns of a row are distinct (one record per pair).  A run whose link
+// regions overflowed is repeated, so indices are checked against the capacity only to stay in bounds.
+constexpr int BMW = 4096;
+__global__ __launch_bounds__(256) void k_link_order(RunArgs A, const LinkRec *__restrict__ flat,
+                                                    const int64_t *__restrict__ row_off,
+                                                    const uint32_t *__restrict__ slot, LinkRec *__restrict__ out)
+{
+  __shared__ uint32_t bits[BMW];
+  __shared__ uint32_t pre[BMW];
+  __shared__ uint32_t wtot[4];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int64_t cap = (int64_t)A.link_cap * NSTRIPE;
+  const int64_t nrows = A.r1 - A.r0;
+  for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+    const int64_t b = row_off[r], e = min(row_off[r + 1], cap);
+    const int64_t cnt = e - b;
+    if (cnt <= 0) continue;
+    if (cnt <= 64) {
+      if (wave == 0) {
+        const uint32_t idx = lane < cnt ? slot[b + lane] : 0u;
+        const bool ok = lane < cnt && (int64_t)idx < cap;
+        const int64_t jj = ok ? flat[idx].j : INT64_MAX;
+        int rank = 0;
+        for (int m = 0; m < cnt; m++) rank += __shfl(jj, m, 64) < jj ? 1 : 0;
+        if (ok) out[b + rank] = flat[idx];
+      }
+      continue;
+    }
+    int64_t base = 0;  // links of the row in earlier windows
+    for (int64_t w0 = A.c0; w0 < A.c1; w0 += (int64_t)BMW * 32) {
+      for (int k = tid; k < BMW; k += 256) bits[k] = 0;
+      __syncthreads();
+      for (int64_t k = b + tid; k < e; k += 256) {
+        const uint32_t idx = slot[k];
+        if ((int64_t)idx >= cap) continue;
+        const int64_t c = flat[idx].j - w0;
+        if (c >= 0 && c < (int64_t)BMW * 32) atomicOr(&bits[c >> 5], 1u << (c & 31));
+      }
+      __syncthreads();
+      // exclusive prefix of the words' popcounts: 16 words per thread, then across the block
+      uint32_t loc[BMW / 256], sum = 0;
+#pragma unroll
+      for (int q = 0; q < BMW / 256; q++) {
+        loc[q] = sum;
+        sum += __popc(bits[tid * (BMW / 256) + q]);
+      }
+      uint32_t x = sum;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+      }
+      if (lane == 63) wtot[wave] = x;
+      __syncthreads();
+      uint32_t wpre = 0;
+      for (int v = 0; v < wave; v++) wpre += wtot[v];
+      const uint32_t tpre = wpre + x - sum;
+#pragma unroll
+      for (int q = 0; q < BMW / 256; q++) pre[tid * (BMW / 256) + q] = tpre + loc[q];
+      const uint32_t wincnt = wtot[0] + wtot[1] + wtot[2] + wtot[3];
+      __syncthreads();
+      for (int64_t k = b + tid; k < e; k += 256) {
+        const uint32_t idx = slot[k];
+        if ((int64_t)idx >= cap) continue;
+        const LinkRec L = flat[idx];
+        const int64_t c = L.j - w0;
+        if (c < 0 || c >= (int64_t)BMW * 32) continue;
+        const int64_t rank = base + pre[c >> 5] + __popc(bits[c >> 5] & ((1u << (c & 31)) - 1u));
+        if (b + rank < e) out[b + rank] = L;
+      }
+      base += wincnt;
+      __syncthreads();
+    }
+  }
 }
 
 // histogram replicas summed into replica 0
@@ -2054,15 +2141,16 @@ struct HostBuf {  // pinned host memory (hipHostMalloc), grown on demand
 // of a batch's kernels can be enqueued before any result is read back.
 struct RunBufs {
   DevBuf D, U, hist, row_links, link_off, slow_list, slow_cnt, slow_off, slow_chunk, counters, exact_list, rowout,
-      links, links_flat, evtab, rtab, patch, chdr, cdata;
+      links, links_flat, lslot, evtab, rtab, patch, chdr, cdata;
   int64_t link_cap_hint = 0;  // link region size that sufficed last time
+  size_t link_spec = 4096;     // link records a single run reads back with its counters
   int tab_mode = -1;           // mode and subset of the tables in evtab/rtab (uploaded again only when they change)
   uint32_t tab_mask = 0;
   bool tab_fold = false;       // the folded tables are valid (the marginal form of the pair kernel may run)
   void release()
   {
     for (DevBuf *b : {&D, &U, &hist, &row_links, &link_off, &slow_list, &slow_cnt, &slow_off, &slow_chunk, &counters,
-                      &exact_list, &rowout, &links, &links_flat, &evtab, &rtab, &patch, &chdr, &cdata}) {
+                      &exact_list, &rowout, &links, &links_flat, &lslot, &evtab, &rtab, &patch, &chdr, &cdata}) {
       if (b->p) (void)hipFree(b->p);
       b->p = nullptr;
       b->n = 0;
@@ -2077,10 +2165,14 @@ struct tn93g_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side[NBRANCH] = {};  // the branches of a batch, forked from and joined back into `stream`
   hipEvent_t fork = nullptr, join[NBRANCH] = {};
+  // per branch (a single run uses 0): the stream on which a job's replay runs beside its tail kernel
+  hipStream_t rside[NBRANCH] = {};
+  hipEvent_t rfork[NBRANCH] = {}, rjoin[NBRANCH] = {};
   hipEvent_t ev[4];
   std::string err;
   std::vector<RunBufs *> slots;
   HostBuf pin;  // staging of the read-backs (counters, links, row accumulators, histogram)
+  HostBuf pin_links;  // the links of a single run that produced more than its read-back expected
   // the instantiated graph of the last batch, launched again when a batch enqueues exactly the same work
   hipGraphExec_t gexec = nullptr;
   std::vector<unsigned char> gkey;
@@ -2366,7 +2458,8 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
     const int64_t want = std::max<int64_t>(B->link_cap_hint, 1 << 15);
     J.link_cap = (unsigned)std::min<int64_t>(std::max<int64_t>(J.npairs, 1), want);
     if (need(B->links, (size_t)J.link_cap * NSTRIPE * sizeof(LinkRec)) ||
-        need(B->links_flat, (size_t)J.link_cap * NSTRIPE * sizeof(LinkRec)))
+        need(B->links_flat, (size_t)J.link_cap * NSTRIPE * sizeof(LinkRec)) ||
+        need(B->lslot, (size_t)J.link_cap * NSTRIPE * sizeof(uint32_t)))
       return TN93G_E_NOMEM;
   }
   A.D = (double *)B->D.p;
@@ -2391,9 +2484,11 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   return 0;
 }
 
-// The job's kernels, in order on stream `st`, with no host synchronisation (a batch captures them into one graph).
-// With `timing`, events around the pair and tail kernels give the device times of a single run.
-static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
+// The job's kernels on stream `st`, with no host synchronisation (a batch captures them into one graph).  The scan
+// and the replay depend only on the pair kernel, like the tail kernel: they run on the branch's replay stream beside
+// it (both fold their results in with atomics), joined back before the link gather.  With `timing`, events around
+// the pair and tail kernels give the device times of a single run.
+static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int branch)
 {
   RunArgs &A = J.A;
   RunBufs *B = J.B;
@@ -2433,16 +2528,30 @@ static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
     fill_zero(st, B->slow_cnt.p, (size_t)J.ntiles * 4);
   }
   if (timing) (void)hipEventRecord(c->ev[1], st);
+  hipStream_t rs = c->rside[branch];
+  (void)hipEventRecord(c->rfork[branch], st);
+  (void)hipStreamWaitEvent(rs, c->rfork[branch], 0);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, rs, (const int32_t *)B->slow_cnt.p,
+                     (int64_t *)B->slow_off.p, J.ntiles, (int32_t *)B->slow_chunk.p);
+  if (A.nplane)
+    hipLaunchKernelGGL(k_replay<true>, dim3(2048), dim3(RB), 0, rs, A);
+  else
+    hipLaunchKernelGGL(k_replay<false>, dim3(2048), dim3(RB), 0, rs, A);
+  (void)hipEventRecord(c->rjoin[branch], rs);
   if (J.fast && nrows > 0 && A.ncols > 0)
     hipLaunchKernelGGL(k_tail, dim3((unsigned)std::min<int64_t>(nrows, 65535)), dim3(256), 0, st, A);
   if (timing) (void)hipEventRecord(c->ev[3], st);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const int32_t *)B->slow_cnt.p,
-                     (int64_t *)B->slow_off.p, J.ntiles, (int32_t *)B->slow_chunk.p);
-  if (A.nplane)
-    hipLaunchKernelGGL(k_replay<true>, dim3(2048), dim3(RB), 0, st, A);
-  else
-    hipLaunchKernelGGL(k_replay<false>, dim3(2048), dim3(RB), 0, st, A);
-  if (J.fast) hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, st, A, (LinkRec *)B->links_flat.p);
+  (void)hipStreamWaitEvent(st, c->rjoin[branch], 0);
+  if (J.fast) {
+    int64_t *roff = (int64_t *)B->link_off.p;
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const int32_t *)B->row_links.p, roff, nrows,
+                       (int32_t *)nullptr);
+    hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, st, A, (LinkRec *)B->links_flat.p,
+                       (const int64_t *)roff, (uint32_t *)B->lslot.p);
+    hipLaunchKernelGGL(k_link_order, dim3((unsigned)std::min<int64_t>(std::max<int64_t>(nrows, 1), 65535)), dim3(256),
+                       0, st, A, (const LinkRec *)B->links_flat.p, (const int64_t *)roff, (const uint32_t *)B->lslot.p,
+                       A.links);
+  }
   hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, st, A);
 }
 
@@ -2473,26 +2582,29 @@ static void job_parse_counters(Job &J, const char *pin)
   J.n_exact = st[ST_EXACT];
 }
 
-// Read-back layout of a fast-path job in pinned memory: link records, row accumulators, histogram.
+// Read-back layout of a fast-path job in pinned memory: row accumulators, histogram, then nl link records.
 struct FastLayout {
-  size_t off_ra, off_h, bytes;
+  size_t off_ra, off_h, off_lk, bytes;
 };
-static FastLayout fast_layout(const Job &J)
+static FastLayout fast_layout(const Job &J, size_t nl)
 {
   FastLayout F;
-  F.off_ra = (J.n_linkrec * sizeof(LinkRec) + 255) & ~(size_t)255;
-  F.off_h = F.off_ra + ((J.nr1 * sizeof(RowAcc) + 255) & ~(size_t)255);
-  F.bytes = F.off_h + ((600 * 8 + 255) & ~(size_t)255);
+  F.off_ra = 0;
+  F.off_h = (J.nr1 * sizeof(RowAcc) + 255) & ~(size_t)255;
+  F.off_lk = F.off_h + ((600 * 8 + 255) & ~(size_t)255);
+  F.bytes = F.off_lk + ((nl * sizeof(LinkRec) + 255) & ~(size_t)255);
   return F;
 }
 
-static int job_enqueue_fast_readback(tn93g_ctx *c, Job &J, char *pin)
+// Copies of the row accumulators, the histogram and the first nl link records (nl may exceed the job's count: the
+// rest is not read).
+static int job_enqueue_fast_readback(tn93g_ctx *c, Job &J, char *pin, size_t nl)
 {
-  const FastLayout F = fast_layout(J);
-  if (J.n_linkrec)
-    HIPCHK(c, hipMemcpyAsync(pin, J.B->links_flat.p, J.n_linkrec * sizeof(LinkRec), hipMemcpyDeviceToHost, c->stream));
+  const FastLayout F = fast_layout(J, nl);
   HIPCHK(c, hipMemcpyAsync(pin + F.off_ra, J.B->rowout.p, J.nr1 * sizeof(RowAcc), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(pin + F.off_h, J.B->hist.p, 600 * 8, hipMemcpyDeviceToHost, c->stream));
+  if (nl) HIPCHK(c, hipMemcpyAsync(pin + F.off_lk, J.B->links.p, nl * sizeof(LinkRec), hipMemcpyDeviceToHost,
+                                   c->stream));
   return 0;
 }
 
@@ -2513,42 +2625,71 @@ static void job_finish_common(const Job &J, const unsigned long long *hrep, tn93
     }
 }
 
+// The five link arrays of a result are one block, with its capacity in a 64-byte header.  Freed blocks are kept
+// (at most LINK_CACHE of them) and handed out again, so a caller running the same workload repeatedly does not map
+// and fault in fresh pages for every result.
+constexpr int LINK_CACHE = 4;
+static std::mutex g_link_mu;
+static std::vector<void *> g_link_cache;
+
 static int alloc_links(tn93g_result *R, size_t nl)
 {
   R->n_links = (int64_t)nl;
-  R->link_row = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
-  R->link_col = (int64_t *)malloc(std::max<size_t>(nl, 1) * 8);
-  R->link_dist = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-  R->link_dist_lo = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-  R->link_dist_hi = (double *)malloc(std::max<size_t>(nl, 1) * 8);
-  return (R->link_row && R->link_col && R->link_dist && R->link_dist_lo && R->link_dist_hi) ? 0 : TN93G_E_NOMEM;
+  const size_t cap = std::max<size_t>(nl, 1), bytes = 64 + 5 * cap * 8;
+  char *blk = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_link_mu);
+    for (size_t k = 0; k < g_link_cache.size(); k++) {
+      size_t have;
+      memcpy(&have, g_link_cache[k], sizeof have);
+      if (have >= cap && have <= 2 * cap + 4096) {
+        blk = (char *)g_link_cache[k];
+        g_link_cache.erase(g_link_cache.begin() + (long)k);
+        break;
+      }
+    }
+  }
+  if (!blk) {
+    blk = (char *)malloc(bytes);
+    if (!blk) return TN93G_E_NOMEM;
+    memcpy(blk, &cap, sizeof cap);
+  }
+  size_t have;
+  memcpy(&have, blk, sizeof have);
+  int64_t *base = (int64_t *)(blk + 64);
+  R->link_row = base;
+  R->link_col = base + have;
+  R->link_dist = (double *)(base + 2 * have);
+  R->link_dist_lo = (double *)(base + 3 * have);
+  R->link_dist_hi = (double *)(base + 4 * have);
+  return 0;
+}
+
+static void free_links(int64_t *link_row)
+{
+  if (!link_row) return;
+  char *blk = (char *)link_row - 64;
+  {
+    std::lock_guard<std::mutex> lk(g_link_mu);
+    if (g_link_cache.size() < LINK_CACHE) {
+      g_link_cache.push_back(blk);
+      return;
+    }
+  }
+  free(blk);
 }
 
 // The fast path's result from its read-back: links in tn93's order, the enclosures of the mean and the maximum.
-static int job_fast_result(const Job &J, const char *pin, tn93g_result *R)
+static int job_fast_result(const Job &J, const char *pin, const LinkRec *lk, tn93g_result *R)
 {
-  const FastLayout F = fast_layout(J);
-  const LinkRec *lk = (const LinkRec *)pin;
+  const FastLayout F = fast_layout(J, 0);
   const RowAcc *ra = (const RowAcc *)(pin + F.off_ra);
   const size_t nl = J.n_linkrec;
   const int64_t nrows = J.nrows, r0 = J.A.r0;
-  // tn93's emission order (rows ascending, columns ascending): counting sort by row, then each row's few links by
-  // column
-  std::vector<int64_t> rstart((size_t)nrows + 1, 0);
-  for (size_t k = 0; k < nl; k++) rstart[(size_t)(lk[k].i - r0) + 1]++;
-  for (int64_t r = 0; r < nrows; r++) rstart[(size_t)r + 1] += rstart[(size_t)r];
-  std::vector<uint32_t> order(nl);
-  {
-    std::vector<int64_t> cur(rstart.begin(), rstart.end() - 1);
-    for (size_t k = 0; k < nl; k++) order[(size_t)cur[(size_t)(lk[k].i - r0)]++] = (uint32_t)k;
-  }
-  for (int64_t r = 0; r < nrows; r++) {
-    uint32_t *b = order.data() + rstart[(size_t)r], *e = order.data() + rstart[(size_t)r + 1];
-    if (e - b > 1) std::sort(b, e, [lk](uint32_t x, uint32_t y) { return lk[x].j < lk[y].j; });
-  }
+  // the records arrive in tn93's emission order (rows ascending, columns ascending: k_link_order)
   if (alloc_links(R, nl)) return TN93G_E_NOMEM;
   for (size_t k = 0; k < nl; k++) {
-    const LinkRec &q = lk[order[k]];
+    const LinkRec &q = lk[k];
     R->link_row[k] = q.i;
     R->link_col[k] = q.j;
     R->link_dist[k] = q.d;
@@ -2559,23 +2700,25 @@ static int job_fast_result(const Job &J, const char *pin, tn93g_result *R)
   // order in FP64; here they were added in another order.  Both results are within gamma_{n_r} * sum|t| of the
   // exact sum (Higham, Accuracy and Stability of Numerical Algorithms, 2nd ed., eq. 4.4), so tn93's row sum lies in
   // [slo - 2 g sabs', shi + 2 g sabs'] with sabs' an upper bound of sum|t|.  The rows are then added in row order:
-  // again within gamma_R * sum |row sums|.  Host arithmetic is long double with an extra relative margin.
-  const long double u = 0x1p-53L;
-  auto gam = [&](long double n) { return n * u / (1.0L - n * u); };
-  long double mlo = 0, mhi = 0, mabs = 0;
+  // again within gamma_R * sum |row sums|.
+  // Host arithmetic is double.  For n u <= 2^-22 (n < 2^31), gamma_n <= n u (1 + 2^-21) <= fl(n cu) with
+  // cu = u (1 + 2^-20); K = 1 + 2^-20 covers the factor (1 + g) and the roundings of each product.  Adding the
+  // bounds lo_r, hi_r up in double is again within gamma_{R+1} sum max(|lo_r|, |hi_r|) (their own rounding
+  // included), so eR below covers both tn93's row-order sum and this one; mabs itself is inflated by K.
+  const double cu = 0x1.00001p-53, K = 1.0 + 0x1p-20;
+  double mlo = 0, mhi = 0, mabs = 0;
   double maxlo = 0., maxhi = 0.;
   long long wsum = 0, skip = 0, linkw = 0;
   for (int64_t r = 0; r < nrows; r++) {
     const RowAcc &q = ra[(size_t)r];
     const int64_t i = r0 + r;
-    const long double n_r = (long double)(J.A.triangle ? (J.A.c1 - std::max(J.A.c0, i + 1)) : J.A.ncols);
-    const long double g = gam(std::max<long double>(n_r, 1));
-    const long double sabs = (long double)q.sabs * (1.0L + g);
-    const long double e = 2.0L * g * sabs;
-    const long double lo = (long double)q.slo - e, hi = (long double)q.shi + e;
+    const double n_r = (double)(J.A.triangle ? (J.A.c1 - std::max(J.A.c0, i + 1)) : J.A.ncols);
+    const double g = std::max(n_r, 1.0) * cu;
+    const double e = (2.0 * g) * q.sabs * K;
+    const double lo = q.slo - e, hi = q.shi + e;
     mlo += lo;
     mhi += hi;
-    mabs += std::max(fabsl(lo), fabsl(hi));
+    mabs += std::max(std::fabs(lo), std::fabs(hi));
     const double a = tn93g_host_from_bits(q.mlo), b = tn93g_host_from_bits(q.mhi);
     if (a > maxlo) maxlo = a;
     if (b > maxhi) maxhi = b;
@@ -2583,9 +2726,10 @@ static int job_fast_result(const Job &J, const char *pin, tn93g_result *R)
     skip += q.skip;
     linkw += q.linkw;
   }
-  const long double eR = gam((long double)std::max<int64_t>(nrows, 1)) * mabs + 0x1p-60L * mabs;
-  R->mean_lo[0] = std::nextafter((double)(mlo - eR), -INFINITY);
-  R->mean_hi[0] = std::nextafter((double)(mhi + eR), INFINITY);
+  const double gR = ((double)std::max<int64_t>(nrows, 1) + 1.0) * cu;
+  const double eR = (2.0 * gR) * (mabs * K) * K + 0x1p-50 * mabs;
+  R->mean_lo[0] = std::nextafter(mlo - eR, -INFINITY);
+  R->mean_hi[0] = std::nextafter(mhi + eR, INFINITY);
   R->max_lo[0] = maxlo;
   R->max_hi[0] = maxhi;
   R->skipped = skip;
@@ -2737,7 +2881,10 @@ int tn93g_open(int device, tn93g_ctx **out)
   for (auto &e : c->ev) (void)hipEventCreate(&e);
   for (int b = 0; b < NBRANCH; b++) {
     if (hipStreamCreateWithFlags(&c->side[b], hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join[b], hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&c->join[b], hipEventDisableTiming) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->rside[b], hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->rfork[b], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->rjoin[b], hipEventDisableTiming) != hipSuccess) {
       tn93g_close(c);
       return TN93G_E_NODEV;
     }
@@ -2764,9 +2911,13 @@ void tn93g_close(tn93g_ctx *c)
   for (int b = 0; b < NBRANCH; b++) {
     if (c->side[b]) (void)hipStreamDestroy(c->side[b]);
     if (c->join[b]) (void)hipEventDestroy(c->join[b]);
+    if (c->rside[b]) (void)hipStreamDestroy(c->rside[b]);
+    if (c->rfork[b]) (void)hipEventDestroy(c->rfork[b]);
+    if (c->rjoin[b]) (void)hipEventDestroy(c->rjoin[b]);
   }
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->pin.p) (void)hipHostFree(c->pin.p);
+  if (c->pin_links.p) (void)hipHostFree(c->pin_links.p);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -2999,11 +3150,7 @@ void tn93g_set_free(tn93g_set *S)
 void tn93g_result_free(tn93g_result *r)
 {
   if (!r) return;
-  free(r->link_row);
-  free(r->link_col);
-  free(r->link_dist);
-  free(r->link_dist_lo);
-  free(r->link_dist_hi);
+  free_links(r->link_row);
   free(r);
 }
 
@@ -3038,7 +3185,7 @@ static int autotune(tn93g_ctx *c, Job &J)
   int pick = TN93G_BACKEND_POPC;
   for (int v : cand) {
     J.P.backend = v;
-    job_enqueue(c, J, true, c->stream);
+    job_enqueue(c, J, true, c->stream, 0);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
@@ -3078,12 +3225,19 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   int rc = job_prepare(c, J);
   if (rc) return rc;
   if ((rc = autotune(c, J))) return rc;
-  job_enqueue(c, J, true, c->stream);
+  job_enqueue(c, J, true, c->stream, 0);
   HIPCHK(c, hipGetLastError());
-  if (ensure_pinned(c, c->pin, COUNTER_STRIDE)) return TN93G_E_NOMEM;
-  if ((rc = job_enqueue_counters(c, J, (char *)c->pin.p))) return rc;
+  // one synchronisation: the counters, and for the fast path the row accumulators, the histogram and as many link
+  // records as the last run of this slot produced (plus a margin); more links take a second copy
+  const size_t spec = J.fast ? std::min<size_t>(J.B->link_spec, (size_t)J.link_cap * NSTRIPE) : 0;
+  const size_t fbytes = J.fast ? fast_layout(J, spec).bytes : 0;
+  if (ensure_pinned(c, c->pin, COUNTER_STRIDE + fbytes)) return TN93G_E_NOMEM;
+  char *pin = (char *)c->pin.p;
+  if ((rc = job_enqueue_counters(c, J, pin))) return rc;
+  if (J.fast && (rc = job_enqueue_fast_readback(c, J, pin + COUNTER_STRIDE, spec))) return rc;
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  job_parse_counters(J, (const char *)c->pin.p);
+  job_parse_counters(J, pin);
   if (J.n_undecided && !J.A.exact) {
     // an output decision could not be certified from the enclosures: run again resolving logs on the host
     tn93g_params Q = *P;
@@ -3098,13 +3252,17 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   if (!R) return TN93G_E_NOMEM;
   auto finish = [&]() -> int {
     if (J.fast) {
-      const FastLayout F = fast_layout(J);
-      if (ensure_pinned(c, c->pin, F.bytes)) return TN93G_E_NOMEM;
-      int r = job_enqueue_fast_readback(c, J, (char *)c->pin.p);
-      if (r) return r;
-      HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      return job_fast_result(J, (const char *)c->pin.p, R);
+      const char *pf = pin + COUNTER_STRIDE;
+      const LinkRec *lk = (const LinkRec *)(pf + fast_layout(J, 0).off_lk);
+      J.B->link_spec = std::max<size_t>(4096, J.n_linkrec + J.n_linkrec / 4);
+      if (J.n_linkrec > spec) {  // more links than copied: all of them again, into a buffer of their own
+        if (ensure_pinned(c, c->pin_links, J.n_linkrec * sizeof(LinkRec))) return TN93G_E_NOMEM;
+        HIPCHK(c, hipMemcpyAsync(c->pin_links.p, J.B->links.p, J.n_linkrec * sizeof(LinkRec),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        lk = (const LinkRec *)c->pin_links.p;
+      }
+      return job_fast_result(J, pf, lk, R);
     }
     return job_exact_result(c, J, R);
   };
@@ -3174,7 +3332,7 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       for (size_t t : order) {
         const int b = (int)(std::min_element(load, load + NBRANCH) - load);
         load[b] += cost(jobs[t]);
-        job_enqueue(c, jobs[t], false, c->side[b]);
+        job_enqueue(c, jobs[t], false, c->side[b], b);
       }
       for (int b = 0; b < NBRANCH; b++) {
         HIPCHK(c, hipEventRecord(c->join[b], c->side[b]));
@@ -3214,12 +3372,12 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
         continue;
       }
       off[t] = total;
-      total += fast_layout(J).bytes;
+      total += fast_layout(J, J.n_linkrec).bytes;
     }
     if (ensure_pinned(c, c->pin, std::max<size_t>(total, 1))) return fail_all(TN93G_E_NOMEM);
     for (size_t t = 0; t < m; t++)
       if (!redo[t]) {
-        int rc = job_enqueue_fast_readback(c, jobs[t], (char *)c->pin.p + off[t]);
+        int rc = job_enqueue_fast_readback(c, jobs[t], (char *)c->pin.p + off[t], jobs[t].n_linkrec);
         if (rc) return fail_all(rc);
       }
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -3228,7 +3386,8 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       tn93g_result *R = (tn93g_result *)calloc(1, sizeof(tn93g_result));
       if (!R) return fail_all(TN93G_E_NOMEM);
       out[idx[t]] = R;
-      int rc = job_fast_result(jobs[t], (const char *)c->pin.p + off[t], R);
+      const char *pj = (const char *)c->pin.p + off[t];
+      int rc = job_fast_result(jobs[t], pj, (const LinkRec *)(pj + fast_layout(jobs[t], 0).off_lk), R);
       if (rc) return fail_all(rc);
       R->gpu_ms = ms;  // device time of the whole batch
     }
