@@ -145,7 +145,9 @@ struct ReplayEnt {
 };
 
 // Event-table entry of the pair kernel: the effect scaled by 16 and aggregated into what the formula needs, packed
-// as 16-bit fields: x = nf0 | nf1 << 16, y = nf2 | nf3 << 16, z = diag | AG << 16, w = CT | amb << 16 | nondyadic << 31.
+// as fields: x = nf0 | nf1 << 16, y = nf2 | nf3 << 16, z = diag | AG << 16 (16 bits each), w = CT | amb << 14 |
+// nondyadic << 23 (14, 9 and 9 bits).  Sums of up to EVFLUSH = 256 entries fit every field (at most 32 per entry in
+// the 16- and 14-bit fields, at most 1 in the 9-bit ones), so entries are added as plain 32-bit words.
 // Exact whenever every value is a multiple of 1/16 (then all of tn93's float sums are exact and order-free).
 static void event_entry(const SiteEffect &E, uint32_t out[4])
 {
@@ -172,7 +174,7 @@ static void event_entry(const SiteEffect &E, uint32_t out[4])
   out[0] = nf[0] | nf[1] << 16;
   out[1] = nf[2] | nf[3] << 16;
   out[2] = dg | ag << 16;
-  out[3] = ct | (uint32_t)E.amb << 16 | (dyadic ? 0u : 1u << 31);
+  out[3] = ct | (uint32_t)E.amb << 14 | (dyadic ? 0u : 1u << 23);
 }
 
 // ------------------------------------------------------------------------------------------ device structures
@@ -273,7 +275,9 @@ struct RunArgs {
   // fast path: the pair kernel hands each wave-row (row i, 64 columns of a tile) to k_tail in a fixed slot
   // (i - r0) * gx + column tile: `need` lanes carry the tail inputs, `skip` lanes had too small an overlap
   struct ChunkHdr *chdr;
-  uint32_t *cdata;             // [slot][7][64]: nf0..nf3, diag, AG, CT (x16) per lane
+  uint32_t *cdata;             // [slot][7][64]: nf0..nf3, diag, AG, CT (x16) per lane; with c16, [slot][4][64]:
+                               // nf0 | nf1 << 16, nf2 | nf3 << 16, diag | AG << 16, CT
+  int c16;                     // 32 * Ls < 2^16: every x16 count fits 16 bits
   int64_t gx;                  // column tiles
 };
 
@@ -672,8 +676,9 @@ __device__ inline uint4 plane_prefix(const uint4 *__restrict__ pre, const uint4 
 }
 
 // event accumulators: packed 16-bit fields plus 32-bit totals (flushed before a 16-bit field can overflow)
+constexpr int EVFLUSH = 256;  // entries summed in the packed words between flushes
 struct EvAcc {
-  uint32_t px, py, pz, pw, fl;
+  uint32_t px, py, pz, pw, fl;  // fl: non-zero once an order-dependent entry was added
 };
 struct Tot {
   uint32_t nf[4], dg, ag, ct, amb;
@@ -684,8 +689,7 @@ __device__ inline void ev_add(EvAcc &e, const uint4 t)
   e.px += t.x;
   e.py += t.y;
   e.pz += t.z;
-  e.pw += t.w & 0x7fffffffu;
-  e.fl |= t.w;
+  e.pw += t.w;
 }
 __device__ inline void ev_flush(EvAcc &e, Tot &o)
 {
@@ -695,8 +699,9 @@ __device__ inline void ev_flush(EvAcc &e, Tot &o)
   o.nf[3] += e.py >> 16;
   o.dg += e.pz & 0xffffu;
   o.ag += e.pz >> 16;
-  o.ct += e.pw & 0xffffu;
-  o.amb += e.pw >> 16;
+  o.ct += e.pw & 0x3fffu;
+  o.amb += (e.pw >> 14) & 0x1ffu;
+  e.fl |= e.pw >> 23;
   e.px = e.py = e.pz = e.pw = 0;
 }
 
@@ -727,18 +732,46 @@ struct PairLds<KWORDS, false> {
   uint4 sJ[KWORDS][TJ + 1];
 };
 
-// Stages the planes of words [w0, w0 + kw) of the tile's sequences.
-template <class LDS>
-__device__ inline void stage_planes(const RunArgs &A, LDS &lds, int64_t i0, int64_t j0, int64_t w0, int kw, int tid)
+// Staging of the planes of words [w0, w0 + kw) of the tile's sequences, in two steps so that the loads of the next
+// chunk are in flight while the current one is counted: SPT values per thread (consecutive threads: consecutive
+// words of one sequence), N-plane words too for dense-N sets.
+constexpr int SPT = (KW * (TI + TJ) + 255) / 256;
+template <bool NP>
+struct StageRegs {
+  uint4 v[SPT];
+  uint32_t nm[NP ? SPT : 1];
+};
+template <bool NP>
+__device__ __attribute__((always_inline)) inline void stage_load(const RunArgs &A, int64_t i0, int64_t j0, int64_t w0,
+                                                                 int kw, int tid, StageRegs<NP> &r)
 {
-  for (int k = tid; k < kw * (TI + TJ); k += 256) {
-    const int s = k / kw, w = k % kw;  // consecutive threads: consecutive words of one sequence
+#pragma unroll
+  for (int q = 0; q < SPT; q++) {
+    const int k = tid + 256 * q;
+    const int s = k / kw, w = k % kw;
     const int64_t seq = s < TI ? i0 + s : j0 + (s - TI);
-    const uint4 v = seq < A.n ? A.planes[seq * A.W + w0 + w] : make_uint4(0, 0, 0, 0);
-    if (s < TI)
-      lds.sI[w][s] = v;
-    else
-      lds.sJ[w][s - TI] = v;
+    const bool ok = k < kw * (TI + TJ) && seq < A.n;
+    r.v[q] = ok ? A.planes[seq * A.W + w0 + w] : make_uint4(0, 0, 0, 0);
+    if constexpr (NP) r.nm[q] = ok ? A.nmask[seq * A.W + w0 + w] : 0u;
+  }
+}
+template <bool NP, class LDS>
+__device__ __attribute__((always_inline)) inline void stage_store(LDS &lds, uint32_t (*sNI)[TI + 1],
+                                                                  uint32_t (*sNJ)[TJ + 1], int kw, int tid,
+                                                                  const StageRegs<NP> &r)
+{
+#pragma unroll
+  for (int q = 0; q < SPT; q++) {
+    const int k = tid + 256 * q;
+    if (k >= kw * (TI + TJ)) break;
+    const int s = k / kw, w = k % kw;
+    if (s < TI) {
+      lds.sI[w][s] = r.v[q];
+      if constexpr (NP) sNI[w][s] = r.nm[q];
+    } else {
+      lds.sJ[w][s - TI] = r.v[q];
+      if constexpr (NP) sNJ[w][s - TI] = r.nm[q];
+    }
   }
 }
 
@@ -828,9 +861,8 @@ __device__ inline uint4 keep_if(bool k, uint4 t) { return k ? t : make_uint4(0, 
 // The events of one chunk for the wave's RPW rows against the lane's column, every operand in LDS.  Row events
 // are uniform over the wave; column events are per lane, the codes of the wave's rows at the site being one 32-bit
 // word of the event's 16-byte record.  sRo[TI + 1]: the rows' event offsets relative to the tile's row list.  The
-// packed 16-bit fields are flushed at the end of the chunk and every 1024 column events, before any field (at most
-// 32 per event) can overflow (a chunk holds at most GCAP / 64 row events).  MASKED: only the (row, lane) pairs with
-// use[a] set take the entries (the others add zero), and rows or lanes with no such pair skip the loops.
+// packed words are flushed every EVFLUSH events.  MASKED: only the (row, lane) pairs with use[a] set take the entries
+// (the others add zero), and rows or lanes with no such pair skip the loops.
 template <bool MASKED>
 __device__ __attribute__((always_inline)) inline void chunk_apply(const RunArgs &A, const uint4 *sTabR,
                                                                   const uint4 *sTabC, const TileEv &te,
@@ -846,9 +878,10 @@ __device__ __attribute__((always_inline)) inline void chunk_apply(const RunArgs 
     if (i >= A.r1) continue;  // uniform over the wave
     if (MASKED && !__ballot(use[a])) continue;
     const int e0 = max(sRo[wave * RPW + a], te.r0) - te.r0, e1 = min(sRo[wave * RPW + a + 1], te.r1) - te.r0;
-    if (e0 >= e1) continue;
-    int e = e0;
-    for (; e + EVB <= e1; e += EVB) {
+    for (int h0 = e0; h0 < e1; h0 += EVFLUSH) {
+    const int h1 = min(e1, h0 + EVFLUSH);
+    int e = h0;
+    for (; e + EVB <= h1; e += EVB) {
       uint32_t c[EVB], x[EVB];
 #pragma unroll
       for (int k = 0; k < EVB; k++) {
@@ -861,18 +894,19 @@ __device__ __attribute__((always_inline)) inline void chunk_apply(const RunArgs 
         ev_add(acc[a], MASKED ? keep_if(use[a], t) : t);
       }
     }
-    for (; e < e1; e++) {
+    for (; e < h1; e++) {
       const uint4 t = sTabR[(sEv[e] & 31u) * TABW + g[e * 64 + lane]];
       ev_add(acc[a], MASKED ? keep_if(use[a], t) : t);
     }
     ev_flush(acc[a], tot[a]);
+    }
   }
   if (MASKED && !(use[0] || use[1] || use[2] || use[3])) return;
   const int k0 = max(pj - te.ec0, te.c0) - te.c0, k1 = min(pj_end - te.ec0, te.c1) - te.c0;
   const uint32_t *evc = sEv + R;
   const uint8_t *gc = g + R * 64 + wave * RPW;
-  for (int f0 = k0; f0 < k1; f0 += 1024) {
-    const int f1 = min(k1, f0 + 1024);
+  for (int f0 = k0; f0 < k1; f0 += EVFLUSH) {
+    const int f1 = min(k1, f0 + EVFLUSH);
     for (int k = f0; k < f1; k++) {
       const uint4 *t = sTabC + (evc[k] & 31u) * TABW;
       const uint32_t r4 = *reinterpret_cast<const uint32_t *>(gc + k * 16);
@@ -923,9 +957,9 @@ __device__ inline void tot_add_mul(Tot &t, EvAcc &acc, const uint4 e, uint32_t k
   t.nf[3] += (e.y >> 16) * k;
   t.dg += (e.z & 0xffffu) * k;
   t.ag += (e.z >> 16) * k;
-  t.ct += (e.w & 0xffffu) * k;
-  t.amb += ((e.w >> 16) & 0x7fffu) * k;
-  acc.fl |= e.w & 0x80000000u;
+  t.ct += (e.w & 0x3fffu) * k;
+  t.amb += ((e.w >> 14) & 0x1ffu) * k;
+  acc.fl |= e.w >> 23;
 }
 
 __device__ __attribute__((always_inline)) inline void nplane_add(const NCnt &q, const uint4 *sTabR, Tot &t, EvAcc &acc)
@@ -1017,6 +1051,9 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   __shared__ uint4 sTabR[(NCODE + 1) * TABW], sTabC[(NCODE + 1) * TABW];
   __shared__ unsigned long long lhist[FAST ? 1 : 3 * 200];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  StageRegs<NP> stg;  // the first chunk's planes, loaded before everything else
+  if constexpr (!MF)
+    if (!A.all_slow) stage_load<NP>(A, i0, j0, 0, (int)min((int64_t)KW, A.W), tid, stg);
   const uint4 *tabs = A.evtab + (MARG ? 2 * NCODE * NCODE : 0);  // [main, AVERAGE], plain or folded
   load_event_tables(tabs, sTabR, sTabC, tid, A.gapmm, A.nplane);
   if constexpr (!FAST)
@@ -1091,18 +1128,9 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
       for (int64_t w0 = 0; w0 < A.W; w0 += KW) {
         const int kw = (int)min((int64_t)KW, A.W - w0);
         __syncthreads();
-        stage_planes(A, lds, i0, j0, w0, kw, tid);
-        if constexpr (NP)
-          for (int k = tid; k < kw * (TI + TJ); k += 256) {
-            const int sq = k / kw, w = k % kw;
-            const int64_t seq = sq < TI ? i0 + sq : j0 + (sq - TI);
-            const uint32_t v = seq < A.n ? A.nmask[seq * A.W + w0 + w] : 0u;
-            if (sq < TI)
-              sNI[w][sq] = v;
-            else
-              sNJ[w][sq - TI] = v;
-          }
+        stage_store<NP>(lds, sNI, sNJ, kw, tid, stg);
         __syncthreads();
+        if (w0 + KW < A.W) stage_load<NP>(A, i0, j0, w0 + KW, (int)min((int64_t)KW, A.W - w0 - KW), tid, stg);
         for (int k = 0; k < kw; k++) {
           const uint4 pjv = lds.sJ[k][lane];
 #pragma unroll
@@ -1225,7 +1253,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
       else
         nfs += 32u * cnt[a].T;
       const double tng = (double)nfs * 0.03125;
-      avg[a] = valid && !(acc[a].fl >> 31) && tng > (double)A.min_overlap && tng * A.rf <= (double)tot[a].amb;
+      avg[a] = valid && !acc[a].fl && tng > (double)A.min_overlap && tng * A.rf <= (double)tot[a].amb;
       any_avg |= avg[a];
     }
   }
@@ -1285,7 +1313,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     int state = 0;  // 1: tail, 2: overlap too small
     const Tot &o = tot[a];
     if (valid) {
-      bool slow = !do_fast || (acc[a].fl >> 31);  // order-dependent: exact site-order replay
+      bool slow = !do_fast || acc[a].fl;  // order-dependent: exact site-order replay
       if (!slow) {
         const double tng = (double)(o.nf[0] + o.nf[1] + o.nf[2] + o.nf[3]) * 0.03125;
         if (tng <= (double)A.min_overlap)  // refkernel.cpp:732-735
@@ -1309,13 +1337,20 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
         if (lane == 0) A.chdr[slot] = ChunkHdr{need, skp};
         if (state == 1) {
           uint32_t *d = A.cdata + slot * (7 * 64) + lane;
-          d[0] = o.nf[0];
-          d[64] = o.nf[1];
-          d[128] = o.nf[2];
-          d[192] = o.nf[3];
-          d[256] = o.dg;
-          d[320] = o.ag;
-          d[384] = o.ct;
+          if (A.c16) {  // every value below 2^16
+            d[0] = o.nf[0] | o.nf[1] << 16;
+            d[64] = o.nf[2] | o.nf[3] << 16;
+            d[128] = o.dg | o.ag << 16;
+            d[192] = o.ct;
+          } else {
+            d[0] = o.nf[0];
+            d[64] = o.nf[1];
+            d[128] = o.nf[2];
+            d[192] = o.nf[3];
+            d[256] = o.dg;
+            d[320] = o.ag;
+            d[384] = o.ct;
+          }
         }
       }
     } else if (state) {
@@ -1362,14 +1397,22 @@ __global__ __launch_bounds__(256) void k_tail(RunArgs A)
       const int64_t j = j0 + lane;
       if (h.need >> lane & 1ull) {
         const uint32_t *d = A.cdata + (size_t)k * (7 * 64) + lane;
+        uint32_t q[7];
+        if (A.c16) {
+          const uint32_t a = d[0], b = d[64], c = d[128];
+          q[0] = a & 0xffffu, q[1] = a >> 16, q[2] = b & 0xffffu, q[3] = b >> 16, q[4] = c & 0xffffu, q[5] = c >> 16;
+          q[6] = d[192];
+        } else {
+          for (int f = 0; f < 7; f++) q[f] = d[64 * f];
+        }
         tn93g_tailin tin;
-        tin.nf[0] = (double)d[0] * 0.0625;
-        tin.nf[1] = (double)d[64] * 0.0625;
-        tin.nf[2] = (double)d[128] * 0.0625;
-        tin.nf[3] = (double)d[192] * 0.0625;
-        tin.diag = (double)d[256] * 0.0625;
-        tin.ag = (double)d[320] * 0.0625;
-        tin.ct = (double)d[384] * 0.0625;
+        tin.nf[0] = (double)q[0] * 0.0625;
+        tin.nf[1] = (double)q[1] * 0.0625;
+        tin.nf[2] = (double)q[2] * 0.0625;
+        tin.nf[3] = (double)q[3] * 0.0625;
+        tin.diag = (double)q[4] * 0.0625;
+        tin.ag = (double)q[5] * 0.0625;
+        tin.ct = (double)q[6] * 0.0625;
         finish_pair(A, i, j, tin, lhist, &bs, rp);
       } else if (h.skip >> lane & 1ull) {
         skip_pair(A, i, j, rp);
@@ -2300,7 +2343,7 @@ struct Job {
 static bool fold_entry(int a, int b, bool gapmm, uint32_t q[4])
 {
   const bool ca = a < 4, cb = b < 4;
-  if (ca == cb || (q[3] >> 31)) return true;
+  if (ca == cb || (q[3] >> 23)) return true;
   const int other = ca ? b : a;
   if (other == GAPC && !gapmm) return true;  // never an event: gap runs cover these sites
   const int x = ca ? a : b;
@@ -2393,6 +2436,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   A.mind = P->min_distance;
   A.window = TN93G_LOG_WINDOW;
   A.min_overlap = P->min_overlap;
+  A.c16 = 32 * A.Ls < 65536 ? 1 : 0;
   const int64_t npairs_box = nrows * A.ncols;
   J.npairs = A.triangle ? (d.n * (d.n - 1) - A.c0 * (A.c0 - 1)) / 2 : npairs_box;
   if (J.npairs >= (1ll << 32)) return c->fail(TN93G_E_ARG, "more than 2^32 pairs in one run");
