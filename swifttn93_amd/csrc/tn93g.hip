@@ -327,10 +327,13 @@ __device__ inline void flush_block(const RunArgs &A, BlockStats *bs, unsigned lo
 
 // Everything after the counts of one pair: tests, tail, histogram, link (refkernel.cpp:732-815 and the phase C of
 // swifttn93.cpp:2547-2603).  Fast path: the pair's share of its row's reductions in rp; otherwise D/U for k_rows.
+// FASTLOG (fast path only): the tail with the enclosed fast logarithm, every output certified from [dlo, dhi];
+// otherwise the double-double logarithm, exact unless a log lies near a rounding midpoint.
+template <bool FASTLOG>
 __device__ void finish_pair(const RunArgs &A, int64_t i, int64_t j, const tn93g_tailin &tin,
                             unsigned long long *lhist, BlockStats *bs, RowPart &rp)
 {
-  tn93g_tailout t = tn93g_tail(tin, A.window);
+  tn93g_tailout t = FASTLOG ? tn93g_tail_fast(tin) : tn93g_tail(tin, A.window);
   if (A.exact && t.unc) {
     unsigned k = atomicAdd(A.exact_cnt, 1u);
     if (k < A.exact_cap) {
@@ -1363,7 +1366,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
         tin.diag = (double)o.dg * 0.0625;
         tin.ag = (double)o.ag * 0.0625;
         tin.ct = (double)o.ct * 0.0625;
-        finish_pair(A, i, j, tin, lhist, &bs, rp);
+        finish_pair<false>(A, i, j, tin, lhist, &bs, rp);
       }
     }
   }
@@ -1413,7 +1416,7 @@ __global__ __launch_bounds__(256) void k_tail(RunArgs A)
         tin.diag = (double)q[4] * 0.0625;
         tin.ag = (double)q[5] * 0.0625;
         tin.ct = (double)q[6] * 0.0625;
-        finish_pair(A, i, j, tin, lhist, &bs, rp);
+        finish_pair<true>(A, i, j, tin, lhist, &bs, rp);
       } else if (h.skip >> lane & 1ull) {
         skip_pair(A, i, j, rp);
       }
@@ -1841,7 +1844,7 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
       if (skip)
         skip_pair(A, i, j, rp);
       else
-        finish_pair(A, i, j, tin, lhist, &bs, rp);
+        A.fast ? finish_pair<true>(A, i, j, tin, lhist, &bs, rp) : finish_pair<false>(A, i, j, tin, lhist, &bs, rp);
       row = i;
     }
     if (A.fast) rowacc_add_wave(A, row, rp);

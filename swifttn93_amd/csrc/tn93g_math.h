@@ -136,6 +136,65 @@ TN93G_FN tn93g_logres tn93g_log(double x, double window)
   return R;
 }
 
+// A cheaper logarithm with an enclosure, for callers that certify from enclosures anyway.  v approximates log(x)
+// with |v - log x| <= 8 u |v| (u = 2^-53):
+//   |x - 1| < 2^-9: log1p(f) with f = x - 1 exact (Sterbenz), f + f^2 Q(f), Q the Taylor polynomial of degree 5
+//     (remainder below 2^-66 |f|); Q to about 1.1 u, so v within 1.1 u |v|.
+//   otherwise: x = 2^e m, F = 1 + j/256 nearest m, r = (m - F) * RN(1/F) within 2 u |r|, log1p(r) as above
+//     (|r| <= 2^-9); v = S + T with S = RN(e ln2_hi + RN(log F)) (exact or within u |S|) and T = RN(log1p(r) + C),
+//     C = the low parts of e ln2 and log F.  Here |v| >= 2^-9 (1 - 2^-10) and |log1p(r)| <= 2^-9, so |S| and |T|
+//     are at most about 2 |v| and 1.02 |v|, and the errors add up to less than 8 u |v|.
+// libm's log is within 0.52 ulp (<= 1.04 u |log x|) of log x, so it lies in [lo, hi] = v -+ 16 u |v| (2^-49 |v|).
+struct tn93g_logenc {
+  double v, lo, hi;
+};
+
+TN93G_FN double tn93g_log1p_small(double f)
+{
+  double q = 1.0 / 7.0;
+  q = TN93G_FMA(q, f, -1.0 / 6.0);
+  q = TN93G_FMA(q, f, 1.0 / 5.0);
+  q = TN93G_FMA(q, f, -1.0 / 4.0);
+  q = TN93G_FMA(q, f, 1.0 / 3.0);
+  q = TN93G_FMA(q, f, -0.5);
+  return f + (f * f) * q;
+}
+
+// x > 0, finite.
+TN93G_FN tn93g_logenc tn93g_log_fast(double x)
+{
+  double v;
+  const double f = x - 1.0;
+  if (f > -0x1p-9 && f < 0x1p-9) {
+    v = tn93g_log1p_small(f);
+  } else {
+    uint64_t u = tn93g_bits(x);
+    int e = 0;
+    if (u < 0x0010000000000000ULL) {  // subnormal: scale into the normal range
+      x = x * 0x1p54;
+      u = tn93g_bits(x);
+      e = -54;
+    }
+    e += (int)(u >> 52) - 1023;
+    const double m = tn93g_from_bits((u & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL);  // [1, 2)
+    const int j = (int)((m - 1.0) * 256.0 + 0.5);                                         // 0..256
+    const double F = 1.0 + (double)j * (1.0 / 256.0);
+    const double r = (m - F) * tn93g_logtab[j][2];  // m - F exact
+    const double l1 = tn93g_log1p_small(r);
+    const double ed = (double)e;
+    const double a0 = ed * TN93G_LN2_HI;
+    const double c = (TN93G_FMA(ed, TN93G_LN2_HI, -a0) + ed * TN93G_LN2_LO) + tn93g_logtab[j][1];
+    const double S = a0 + tn93g_logtab[j][0];
+    v = S + (l1 + c);
+  }
+  tn93g_logenc R;
+  const double d = (v < 0 ? -v : v) * 0x1p-49;
+  R.v = v;
+  R.lo = v - d;
+  R.hi = v + d;
+  return R;
+}
+
 // Inputs of the tail: tn93's nucFreq[4] and float_counts sums after the merge (refkernel.cpp:714-729).
 struct tn93g_tailin {
   double nf[4];
@@ -212,6 +271,57 @@ TN93G_FN tn93g_tailout tn93g_tail(const tn93g_tailin &in, double window)
     } else
       o.d = o.dlo = o.dhi = 1000.0;
   }
+  return o;
+}
+
+// tn93g_tail with tn93g_log_fast: the same operations; d from the logs' estimates, [dlo, dhi] from their
+// enclosures (d is non-increasing in each log whose coefficient is positive, and every IEEE operation rounds
+// monotonically), so [dlo, dhi] holds tn93's distance.  unc is set whenever the enclosure is not a single value.
+TN93G_FN tn93g_tailout tn93g_tail_fast(const tn93g_tailin &in)
+{
+  tn93g_tailout o;
+  const double *nucFreq = in.nf;
+  double auxd, fY, fR, K1 = 0, K2 = 0, K3 = 0, AG, CT, ti, tv, tng2, nucF[4];
+  bool useK2P = false;
+  auxd = 1. / (nucFreq[0] + nucFreq[1] + nucFreq[2] + nucFreq[3]);
+  tng2 = 2. * auxd;
+  for (int k = 0; k < 4; k++) nucF[k] = nucFreq[k] * auxd;
+  fR = nucF[0] + nucF[2];
+  fY = nucF[1] + nucF[3];
+  if (nucFreq[0] == 0 || nucFreq[1] == 0 || nucFreq[2] == 0 || nucFreq[3] == 0) {
+    useK2P = true;
+  } else {
+    K1 = 2. * nucF[0] * nucF[2] / fR;
+    K2 = 2. * nucF[1] * nucF[3] / fY;
+    K3 = 2. * (fR * fY - nucF[0] * nucF[2] * fY / fR - nucF[1] * nucF[3] * fR / fY);
+  }
+  AG = in.ag * tng2;
+  CT = in.ct * tng2;
+  tv = 1. - (in.diag * tng2 + AG + CT);
+  if (useK2P) {
+    ti = AG + CT;
+    AG = 1. - 2. * ti - tv;
+    CT = 1. - 2. * tv;
+    if (AG > 0. && CT > 0.) {
+      const tn93g_logenc la = tn93g_log_fast(AG), lc = tn93g_log_fast(CT);
+      o.d = -0.5 * la.v - 0.25 * lc.v;
+      o.dlo = -0.5 * la.hi - 0.25 * lc.hi;
+      o.dhi = -0.5 * la.lo - 0.25 * lc.lo;
+    } else
+      o.d = o.dlo = o.dhi = 1000.0;
+  } else {
+    AG = 1. - AG / K1 - 0.5 * tv / fR;
+    CT = 1. - CT / K2 - 0.5 * tv / fY;
+    tv = 1. - 0.5 * tv / fY / fR;
+    if (AG > 0. && CT > 0. && tv > 0) {
+      const tn93g_logenc la = tn93g_log_fast(AG), lc = tn93g_log_fast(CT), lt = tn93g_log_fast(tv);
+      o.d = -K1 * la.v - K2 * lc.v - K3 * lt.v;
+      o.dlo = -K1 * la.hi - K2 * lc.hi - K3 * (K3 > 0 ? lt.hi : lt.lo);
+      o.dhi = -K1 * la.lo - K2 * lc.lo - K3 * (K3 > 0 ? lt.lo : lt.hi);
+    } else
+      o.d = o.dlo = o.dhi = 1000.0;
+  }
+  o.unc = o.dlo != o.dhi;
   return o;
 }
 

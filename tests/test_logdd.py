@@ -34,3 +34,24 @@ def test_window_is_needed(logcheck):
     out = subprocess.run([str(logcheck), "3000000", "3", "0.0"], capture_output=True, text=True)
     assert out.returncode == 1
     assert int(out.stdout.split()[2]) > 0
+
+
+@pytest.fixture(scope="module")
+def logfast(tmp_path_factory):
+    exe = tmp_path_factory.mktemp("logf") / "logfast"
+    subprocess.run(["g++", "-O2", "-ffp-contract=off", "-std=c++17", "-o", str(exe),
+                    str(REPO / "tests" / "native" / "logfast.cpp")], check=True)
+    return exe
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+def test_fast_log_encloses_libm(logfast, seed):
+    """The fast path's logarithm: libm's value always lies in its enclosure; its error stays under the proven
+    8 u |v| (measured against the double-double logarithm) and the enclosure is 16 u |v| either side (the rounding of
+    its ends adds at most one ulp, 2 u |v|)."""
+    out = subprocess.run([str(logfast), "2000000", str(seed)], capture_output=True, text=True)
+    n, outside, max_err, max_hw = out.stdout.split()
+    assert out.returncode == 0, out.stderr
+    assert int(outside) == 0
+    assert float(max_err) < 8.0
+    assert 15.0 <= float(max_hw) <= 18.0
