@@ -24,6 +24,7 @@
 #include <cstring>
 #include <iomanip>
 #include <iostream>
+#include <sstream>
 #include <algorithm>
 #include <string>
 #include <utility>
@@ -365,7 +366,9 @@ struct Seqs {
   long N = 0;
 };
 
-// add a finished record: the length rules of readFASTA (swifttn93.cpp:259-273)
+// add a finished record: the length rules of readFASTA (swifttn93.cpp:259-273); tn93 exits with status 1 after
+// the message, here the reader returns 1 (LengthError unwinds to it)
+struct LengthError {};
 static void add_sequence(Seqs &S, size_t begin, long &firstLen)
 {
   long len = (long)(S.codes.size() - begin);
@@ -373,12 +376,12 @@ static void add_sequence(Seqs &S, size_t begin, long &firstLen)
     firstLen = len;
     if (firstLen <= 0) {
       std::cerr << "First sequence length must be positive." << std::endl;
-      exit(1);
+      throw LengthError{};
     }
   } else if (len != firstLen) {
     std::cerr << "All sequences must have the same length (" << firstLen << "), but sequence '" << S.names.back()
               << "' had length " << len;
-    exit(1);
+    throw LengthError{};
   }
   S.N++;
 }
@@ -386,7 +389,17 @@ static void add_sequence(Seqs &S, size_t begin, long &firstLen)
 // readFASTA state machine (swifttn93.cpp:292-406): records start at '>' or '#'; a name runs to '\r' or '\n';
 // sequence bytes are mapped through validFlags[toupper(c)], unknown bytes dropped; copy number = atoi of what
 // follows the last separator (at least one character after it, not the first character), 1 if < 1 as unsigned.
+static int read_fasta_body(FILE *F, Seqs &S, long &firstLen, char sep);
 static int read_fasta(FILE *F, Seqs &S, long &firstLen, char sep)
+{
+  try {
+    return read_fasta_body(F, S, firstLen, sep);
+  } catch (const LengthError &) {
+    return 1;
+  }
+}
+
+static int read_fasta_body(FILE *F, Seqs &S, long &firstLen, char sep)
 {
   std::vector<unsigned char> buf;
   {
@@ -749,6 +762,42 @@ void oracle_triangle(const uint8_t *codes, long n, long L, int mode, long min_ov
 }
 
 long oracle_hist_index(double d) { return oracle::hist_index(d); }
+
+// Reads path1 (and path2 when not NULL, as tn93's second file) with the reader above; returns a malloc'd dump:
+// "rc <rc>\nerr <bytes written to std::cerr>\nN <n> L <L>\n" then one line per record: name, tab, copy number, tab,
+// codes as letters of ACGTURYSWKMBDHVN?-.  The caller frees it.
+char *oracle_fasta_dump(const char *path1, const char *path2, char sep)
+{
+  oracle::init_alphabets(nullptr);
+  oracle::Seqs S;
+  long firstLen = 0;
+  std::ostringstream err;
+  std::streambuf *old = std::cerr.rdbuf(err.rdbuf());
+  int rc = 0;
+  for (const char *path : {path1, path2}) {
+    if (!path || rc) continue;
+    FILE *F = fopen(path, "rb");
+    if (!F) {
+      rc = 2;
+      break;
+    }
+    rc = oracle::read_fasta(F, S, firstLen, sep);
+    fclose(F);
+  }
+  std::cerr.rdbuf(old);
+  std::string o = "rc " + std::to_string(rc) + "\nerr " + err.str() + "\n";
+  if (rc == 0) {
+    o += "N " + std::to_string(S.N) + " L " + std::to_string(firstLen) + "\n";
+    for (long k = 0; k < S.N; k++) {
+      o += S.names[(size_t)k] + "\t" + std::to_string(S.counts[(size_t)k]) + "\t";
+      for (long c = 0; c < firstLen; c++) o += "ACGTURYSWKMBDHVN?-"[S.codes[(size_t)(k * firstLen + c)]];
+      o += "\n";
+    }
+  }
+  char *r = (char *)malloc(o.size() + 1);
+  memcpy(r, o.c_str(), o.size() + 1);
+  return r;
+}
 }
 
 #ifdef ORACLE_MAIN

@@ -33,7 +33,7 @@ def build(verbose=True):
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, CSRC / "tn93g.hip"])
     os.replace(tmp, LIB)
     ctmp = CLI.with_name("tn93g.tmp")
-    _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-o", ctmp, CSRC / "tn93g_cli.cpp",
+    _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-pthread", "-o", ctmp, CSRC / "tn93g_cli.cpp",
           f"-L{LIB.parent}", "-ltn93g", "-Wl,-rpath,$ORIGIN/../lib", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(ctmp, CLI)
     return LIB, CLI

@@ -16,7 +16,7 @@
 //
 // Kernels of a run (on the context's stream, or one of its side streams in a captured batch):
 //   k_pairs   clean/clean counts (popcount, popcount+nmask, or MFMA) + event tables from LDS + -g fallback pass;
-//             dyadic pairs complete here, order-dependent ones go to the replay list
+//             dyadic pairs complete here, order-dependent ones are appended to the flat replay list
 //   k_tail    FP64 TN93/K2P tail, certification, histogram, row accumulators and links (fast path) or D/U
 //   k_replay  site-order FP64 replay for pairs with non-dyadic contributions (runs of N as repeated additions)
 //   k_scan    per-row link offsets;  k_link_gather  links into one array;  k_hist_sum  histogram replicas summed
@@ -259,11 +259,9 @@ struct RunArgs {
   uint16_t *U;
   unsigned long long *hist;    // [NSTRIPE][3*200]: histogram replicas, block b adds into replica b % NSTRIPE
   int32_t *row_links;          // [nrows]
-  unsigned long long *slow_list;  // [tile][TI*TJ] pairs for the site-order replay, per pair tile
-  unsigned int *slow_cnt;         // [tile]
-  const int64_t *slow_off;        // [tile + 1]: exclusive scan of slow_cnt (k_replay finds a pair's tile in it)
-  const int32_t *slow_chunk;      // [k / 64]: the tile holding flat replay index k, for k a multiple of 64
-  int64_t ntiles;
+  unsigned long long *slow_list;  // [slow_cap] pairs for the site-order replay: i << 32 | j (bit 63: AVERAGE first);
+                                  // each pair tile appends its pairs as one contiguous range
+  unsigned long long *slow_total; // entries in slow_list (64-bit counter in stripe 0)
   ExactRec *exact_list;
   unsigned int *exact_cnt;
   unsigned int exact_cap;
@@ -286,7 +284,7 @@ struct ChunkHdr {
 };
 
 constexpr int NSTRIPE = 64;
-enum { ST_UNCERTAIN = 0, ST_UNDECIDED = 1, ST_LINKS = 2, ST_EXACT = 3 };
+enum { ST_UNCERTAIN = 0, ST_UNDECIDED = 1, ST_LINKS = 2, ST_EXACT = 3, ST_SLOW = 4 /* u64, stripe 0 only */ };
 
 __device__ inline int fst_bin(const RunArgs &A, int64_t i, int64_t j)
 {
@@ -1036,13 +1034,10 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
 {
   const int64_t c0a = A.c0 & ~(int64_t)63;  // column tiles start on a 64 boundary (whole 4-sequence groups)
   const int64_t i0 = A.r0 + (int64_t)blockIdx.y * TI, j0 = c0a + (int64_t)blockIdx.x * TJ;
-  const int64_t tile = (int64_t)blockIdx.y * A.gx + blockIdx.x;
-  if (A.triangle && j0 + TJ - 1 <= i0) {  // no pair j > i in this tile
-    if (threadIdx.x == 0) A.slow_cnt[tile] = 0;
-    return;
-  }
+  if (A.triangle && j0 + TJ - 1 <= i0) return;  // no pair j > i in this tile
   constexpr bool MF = BACKEND == TN93G_BACKEND_MFMA;
   __shared__ unsigned lslow;
+  __shared__ unsigned long long sbase;
   __shared__ BlockStats bs;
   if (threadIdx.x == 0) {
     lslow = 0;
@@ -1308,7 +1303,9 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     tot[a].ct += 16 * c.CT;
   }
   // per pair: tests and routing.  Fast path: the tail inputs go to k_tail in one chunk per wave-row; exact path:
-  // the tail right here, into D/U
+  // the tail right here, into D/U.  Pairs for the replay are marked in smask (bit a: row ib + a; bit 4 + a: the -g
+  // test already chose AVERAGE) and appended after the loop.
+  unsigned smask = 0;
 #pragma unroll
   for (int a = 0; a < RPW; a++) {
     const int64_t i = ib + a;
@@ -1326,12 +1323,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
         else
           state = 1;
       }
-      if (slow) {
-        const unsigned k = atomicAdd(&lslow, 1u);  // < TI * TJ
-        // bit 63: the -g test already chose AVERAGE for this pair (the replay starts there)
-        A.slow_list[tile * (TI * TJ) + k] =
-            (unsigned long long)i << 32 | (unsigned long long)j | (avg[a] ? 1ull << 63 : 0ull);
-      }
+      if (slow) smask |= (1u << a) | (avg[a] ? 1u << (RPW + a) : 0u);
     }
     if constexpr (FAST) {
       const unsigned long long need = __ballot(state == 1), skp = __ballot(state == 2);
@@ -1370,8 +1362,21 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
       }
     }
   }
-  flush_block(A, &bs, FAST ? nullptr : lhist, tid, 256);
-  if (tid == 0) A.slow_cnt[tile] = lslow;
+  // the tile's replay pairs: a slot range per thread in LDS, one range of the flat list per tile
+  const unsigned nmine = __popc(smask & ((1u << RPW) - 1));
+  const unsigned k0 = nmine ? atomicAdd(&lslow, nmine) : 0u;  // lslow <= TI * TJ
+  flush_block(A, &bs, FAST ? nullptr : lhist, tid, 256);  // begins with a barrier: lslow is final
+  if (tid == 0 && lslow) sbase = atomicAdd(A.slow_total, (unsigned long long)lslow);
+  __syncthreads();
+  if (nmine) {
+    unsigned long long *dst = A.slow_list + sbase + k0;
+    unsigned q = 0;
+#pragma unroll
+    for (int a = 0; a < RPW; a++)
+      if (smask >> a & 1u)  // bit 63: the replay starts in AVERAGE
+        dst[q++] = (unsigned long long)(ib + a) << 32 | (unsigned long long)j |
+                   (smask >> (RPW + a) & 1u ? 1ull << 63 : 0ull);
+  }
 }
 
 // The FP64 tail of the fast path.  A block takes whole rows; its 4 waves take the row's chunks (64 columns each) in
@@ -1512,7 +1517,7 @@ __device__ inline void cc_table(const RunArgs &A, int64_t i, int64_t j, uint32_t
 #pragma unroll
   for (int k = 0; k < 16; k++) c[k] = 0;
   const uint4 *Pi = A.planes + i * A.W, *Pj = A.planes + j * A.W;
-#pragma unroll 2
+#pragma unroll 4
   for (int64_t w = 0; w < A.W; w++) {
     const uint4 pi = Pi[w], pj = Pj[w];
     const uint32_t M = pi.x & pj.x;
@@ -1745,40 +1750,7 @@ __device__ void rowacc_add_wave(const RunArgs &A, int64_t row, const RowPart &rp
   }
 }
 
-// The tile holding flat replay index k, for the 64 consecutive indices of a wave starting at kb (a multiple of 64):
-// slow_chunk[kb / 64] is the tile of kb; the next 63 tile offsets are searched across the wave, and an index past
-// them falls back to a binary search over all offsets.
-__device__ inline int64_t replay_tile(const RunArgs &A, int64_t kb, int64_t k)
-{
-  const int lane = threadIdx.x & 63;
-  const int64_t t0 = A.slow_chunk[kb >> 6];
-  const int64_t tl = t0 + lane;
-  const int64_t off = tl <= A.ntiles ? A.slow_off[tl] : INT64_MAX;
-  int lo = 0, hi = 64;  // largest l with slow_off[t0 + l] <= k
-  for (int step = 0; step < 6; step++) {
-    const int mid = (lo + hi) >> 1;
-    const int64_t v = __shfl(off, mid, 64);
-    if (v <= k)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  int64_t t = t0 + lo;
-  if (lo == 63 && t < A.ntiles && A.slow_off[t + 1] <= k) {
-    int64_t l2 = t, h2 = A.ntiles;
-    while (h2 - l2 > 1) {
-      const int64_t mid = (l2 + h2) >> 1;
-      if (A.slow_off[mid] <= k)
-        l2 = mid;
-      else
-        h2 = mid;
-    }
-    t = l2;
-  }
-  return t;
-}
-
-// one thread per pair that needs the site-order replay (the tiles' lists, concatenated through slow_off); every
+// one thread per pair that needs the site-order replay (the flat list the pair kernel's tiles appended to); every
 // lane of a wave runs the same iterations so the row parts can be combined across the wave
 template <bool DENSE>
 __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
@@ -1792,16 +1764,15 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
   for (int k = tid; k < 600; k += RB) lhist[k] = 0;
   if (tid == 0) bs.unc = bs.undecided = 0;
   __syncthreads();
-  const int64_t total = A.slow_off[A.ntiles];
+  const int64_t total = (int64_t)*A.slow_total;
   const int64_t nw = (int64_t)gridDim.x * (RB / 64);
   for (int64_t kb = ((int64_t)blockIdx.x * (RB / 64) + wave) * 64; kb < total; kb += nw * 64) {
     const int64_t k = kb + lane;
-    const int64_t tile = replay_tile(A, kb, k < total ? k : kb);
     int64_t row = -1;
     RowPart rp = rowpart_zero();
     const bool coop = DENSE && A.W >= 128;  // uniform
     unsigned long long pr = 0;
-    if (k < total) pr = A.slow_list[tile * (TI * TJ) + (k - A.slow_off[tile])];
+    if (k < total) pr = A.slow_list[k];
     if (coop)
       cc_table_wave(A, (int64_t)(pr >> 32 & 0x7fffffffu), (int64_t)(pr & 0xffffffffull), k < total, scc + wave * 64);
     const bool avg_first = pr >> 63;
@@ -1977,13 +1948,10 @@ __global__ void k_patch(RunArgs A, const PatchRec *__restrict__ P, int64_t np)
   if (r.link) atomicAdd(&A.row_links[r.i - A.r0], 1);
 }
 
-// exclusive scan of the per-row link counts (one block)
-// Exclusive prefix sum of n int32 values into out[0..n), total in out[n]; one block of 1024 threads walks the array
-// in chunks of 4096 (4 consecutive values per thread, wave scans, then a scan of the 16 wave totals).  With `chunk`,
-// also records for every multiple m of 64 below the total the index whose range [out[k], out[k] + in[k]) holds it,
-// at chunk[m / 64].
-__global__ __launch_bounds__(1024) void k_scan(const int32_t *__restrict__ in, int64_t *__restrict__ out, int64_t n,
-                                               int32_t *__restrict__ chunk)
+// Exclusive prefix sum of n int32 values (the per-row link counts) into out[0..n), total in out[n]; one block of
+// 1024 threads walks the array in chunks of 4096 (4 consecutive values per thread, wave scans, then a scan of the 16
+// wave totals).
+__global__ __launch_bounds__(1024) void k_scan(const int32_t *__restrict__ in, int64_t *__restrict__ out, int64_t n)
 {
   __shared__ int64_t wsum[16];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -2015,11 +1983,7 @@ __global__ __launch_bounds__(1024) void k_scan(const int32_t *__restrict__ in, i
     int64_t run = carry + (w ? wsum[w - 1] : 0) + x - s;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      if (k0 + q < n) {
-        out[k0 + q] = run;
-        if (chunk)
-          for (int64_t m = (run + 63) >> 6; (m << 6) < run + v[q]; m++) chunk[m] = (int32_t)(k0 + q);
-      }
+      if (k0 + q < n) out[k0 + q] = run;
       run += v[q];
     }
     carry += wsum[15];
@@ -2186,7 +2150,7 @@ struct HostBuf {  // pinned host memory (hipHostMalloc), grown on demand
 // The device buffers of one run.  A single run uses slot 0; a batch gives each of its jobs a slot of its own, so all
 // of a batch's kernels can be enqueued before any result is read back.
 struct RunBufs {
-  DevBuf D, U, hist, row_links, link_off, slow_list, slow_cnt, slow_off, slow_chunk, counters, exact_list, rowout,
+  DevBuf D, U, hist, row_links, link_off, slow_list, counters, exact_list, rowout,
       links, links_flat, lslot, evtab, rtab, patch, chdr, cdata;
   int64_t link_cap_hint = 0;  // link region size that sufficed last time
   size_t link_spec = 4096;     // link records a single run reads back with its counters
@@ -2195,7 +2159,7 @@ struct RunBufs {
   bool tab_fold = false;       // the folded tables are valid (the marginal form of the pair kernel may run)
   void release()
   {
-    for (DevBuf *b : {&D, &U, &hist, &row_links, &link_off, &slow_list, &slow_cnt, &slow_off, &slow_chunk, &counters,
+    for (DevBuf *b : {&D, &U, &hist, &row_links, &link_off, &slow_list, &counters,
                       &exact_list, &rowout, &links, &links_flat, &lslot, &evtab, &rtab, &patch, &chdr, &cdata}) {
       if (b->p) (void)hipFree(b->p);
       b->p = nullptr;
@@ -2211,7 +2175,7 @@ struct tn93g_ctx {
   hipStream_t stream = nullptr;
   hipStream_t side[NBRANCH] = {};  // the branches of a batch, forked from and joined back into `stream`
   hipEvent_t fork = nullptr, join[NBRANCH] = {};
-  // per branch (a single run uses 0): the stream on which a job's replay runs beside its tail kernel
+  // per branch (a single run uses 0): the stream on which a job's tail kernel runs beside its replay
   hipStream_t rside[NBRANCH] = {};
   hipEvent_t rfork[NBRANCH] = {}, rjoin[NBRANCH] = {};
   hipEvent_t ev[4];
@@ -2320,7 +2284,7 @@ static void fill_zero(hipStream_t s, void *p, size_t bytes)
 }
 
 constexpr size_t STRIPE_BYTES = (size_t)NSTRIPE * 32 * 4;
-constexpr size_t COUNTER_STRIDE = STRIPE_BYTES + 256;  // pinned bytes per job: the stripes, then the replay total
+constexpr size_t COUNTER_STRIDE = STRIPE_BYTES + 256;  // pinned bytes per job: the stripes (replay total included)
 
 // One run: a set, its parameters, its buffers and what the host learns about it along the way.
 struct Job {
@@ -2490,8 +2454,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   if ((!J.fast && (need(B->D, nbox * 8) || need(B->U, nbox * 2))) ||
       need(B->hist, (size_t)NSTRIPE * 600 * 8) || need(B->row_links, nr1 * 4) ||
       need(B->link_off, (size_t)(nrows + 1) * 8) ||
-      need(B->slow_list, (size_t)ntiles * TI * TJ * 8) || need(B->slow_cnt, (size_t)ntiles * 4) ||
-      need(B->slow_off, (size_t)(ntiles + 1) * 8) || need(B->slow_chunk, (size_t)(16 * ntiles + 1) * 4) ||
+      need(B->slow_list, (size_t)std::max<int64_t>(J.npairs, 1) * 8) ||
       need(B->counters, STRIPE_BYTES) ||
       need(B->rowout, nr1 * std::max<size_t>(15 * 8 + 16, sizeof(RowAcc))) ||
       need(B->chdr, (size_t)nslots * sizeof(ChunkHdr)) ||
@@ -2514,10 +2477,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   A.hist = (unsigned long long *)B->hist.p;
   A.row_links = (int32_t *)B->row_links.p;
   A.slow_list = (unsigned long long *)B->slow_list.p;
-  A.slow_cnt = (unsigned *)B->slow_cnt.p;
-  A.slow_off = (const int64_t *)B->slow_off.p;
-  A.slow_chunk = (const int32_t *)B->slow_chunk.p;
-  A.ntiles = ntiles;
+  A.slow_total = (unsigned long long *)((unsigned *)B->counters.p + ST_SLOW);
   A.stripes = (unsigned *)B->counters.p;
   A.exact_cnt = A.stripes + ST_EXACT;
   A.exact_list = (ExactRec *)B->exact_list.p;
@@ -2531,9 +2491,9 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   return 0;
 }
 
-// The job's kernels on stream `st`, with no host synchronisation (a batch captures them into one graph).  The scan
-// and the replay depend only on the pair kernel, like the tail kernel: they run on the branch's replay stream beside
-// it (both fold their results in with atomics), joined back before the link gather.  With `timing`, events around
+// The job's kernels on stream `st`, with no host synchronisation (a batch captures them into one graph).  The replay
+// and the tail kernel depend only on the pair kernel: the tail runs on the branch's side stream beside the replay
+// (both fold their results in with atomics), joined back before the link gather.  With `timing`, events around
 // the pair and tail kernels give the device times of a single run.
 static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int branch)
 {
@@ -2571,28 +2531,25 @@ static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int b
       hipLaunchKernelGGL((k_pairs<PC, false, true, true>), g, dim3(256), 0, st, A);
     else
       hipLaunchKernelGGL((k_pairs<PC, true, true, true>), g, dim3(256), 0, st, A);
-  } else {
-    fill_zero(st, B->slow_cnt.p, (size_t)J.ntiles * 4);
   }
   if (timing) (void)hipEventRecord(c->ev[1], st);
+  // the replay is enqueued first so that its long, latency-bound waves are resident before the tail's blocks fill
+  // the machine; the tail runs on the side stream and is joined back before the link gather
   hipStream_t rs = c->rside[branch];
   (void)hipEventRecord(c->rfork[branch], st);
   (void)hipStreamWaitEvent(rs, c->rfork[branch], 0);
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, rs, (const int32_t *)B->slow_cnt.p,
-                     (int64_t *)B->slow_off.p, J.ntiles, (int32_t *)B->slow_chunk.p);
   if (A.nplane)
-    hipLaunchKernelGGL(k_replay<true>, dim3(2048), dim3(RB), 0, rs, A);
+    hipLaunchKernelGGL(k_replay<true>, dim3(2048), dim3(RB), 0, st, A);
   else
-    hipLaunchKernelGGL(k_replay<false>, dim3(2048), dim3(RB), 0, rs, A);
-  (void)hipEventRecord(c->rjoin[branch], rs);
+    hipLaunchKernelGGL(k_replay<false>, dim3(2048), dim3(RB), 0, st, A);
   if (J.fast && nrows > 0 && A.ncols > 0)
-    hipLaunchKernelGGL(k_tail, dim3((unsigned)std::min<int64_t>(nrows, 65535)), dim3(256), 0, st, A);
-  if (timing) (void)hipEventRecord(c->ev[3], st);
+    hipLaunchKernelGGL(k_tail, dim3((unsigned)std::min<int64_t>(nrows, 65535)), dim3(256), 0, rs, A);
+  if (timing) (void)hipEventRecord(c->ev[3], rs);
+  (void)hipEventRecord(c->rjoin[branch], rs);
   (void)hipStreamWaitEvent(st, c->rjoin[branch], 0);
   if (J.fast) {
     int64_t *roff = (int64_t *)B->link_off.p;
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const int32_t *)B->row_links.p, roff, nrows,
-                       (int32_t *)nullptr);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const int32_t *)B->row_links.p, roff, nrows);
     hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, st, A, (LinkRec *)B->links_flat.p,
                        (const int64_t *)roff, (uint32_t *)B->lslot.p);
     hipLaunchKernelGGL(k_link_order, dim3((unsigned)std::min<int64_t>(std::max<int64_t>(nrows, 1), 65535)), dim3(256),
@@ -2602,12 +2559,10 @@ static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int b
   hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, st, A);
 }
 
-// Device-to-host copies of the job's counters and replay total into `pin` (COUNTER_STRIDE bytes).
+// Device-to-host copy of the job's counters (the replay total among them) into `pin` (COUNTER_STRIDE bytes).
 static int job_enqueue_counters(tn93g_ctx *c, Job &J, char *pin)
 {
   HIPCHK(c, hipMemcpyAsync(pin, J.B->counters.p, STRIPE_BYTES, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipMemcpyAsync(pin + STRIPE_BYTES, (const int64_t *)J.B->slow_off.p + J.ntiles, 8, hipMemcpyDeviceToHost,
-                           c->stream));
   return 0;
 }
 
@@ -2616,7 +2571,7 @@ static void job_parse_counters(Job &J, const char *pin)
   std::vector<unsigned> st((size_t)NSTRIPE * 32);
   memcpy(st.data(), pin, STRIPE_BYTES);
   int64_t slow_total = 0;
-  memcpy(&slow_total, pin + STRIPE_BYTES, 8);
+  memcpy(&slow_total, pin + ST_SLOW * 4, 8);
   J.n_slow = (uint64_t)slow_total;
   J.n_unc = J.n_undecided = J.n_linkrec = 0;
   J.max_region = 0;
@@ -2834,8 +2789,7 @@ static int job_exact_result(tn93g_ctx *c, Job &J, tn93g_result *R)
   }
   // link offsets, then the rows in column order
   int64_t *loff = (int64_t *)B->link_off.p;
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)B->row_links.p, loff, nrows,
-                     (int32_t *)nullptr);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, c->stream, (const int32_t *)B->row_links.p, loff, nrows);
   int64_t n_links = 0;
   HIPCHK(c, hipMemcpyAsync(&n_links, loff + nrows, 8, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -1,10 +1,12 @@
 // tn93g_cli.cpp -- tn93 v1.1.0-compatible command line over libtn93g (include/tn93g.h).
 //
 // Same command line (argparse.cpp:138-445), same reader (swifttn93.cpp:152-406), same distance file and JSON summary
-// (swifttn93.cpp:2222-2966) as single-thread tn93; the pair computation runs on the GPU through tn93g_run.
+// (swifttn93.cpp:2222-2966) as single-thread tn93; the pair computation runs on the GPU through tn93g_run, the
+// reader and the distance-file formatting on all host threads (tn93g_fasta.h).
 // Printed values that depend on a GPU logarithm are certified from the engine's enclosures: every printed string
 // must be the same at both ends of the enclosure, otherwise the run is repeated with TN93G_FLAG_EXACT (logarithms
 // resolved with the host libm).
+#include <atomic>
 #include <cctype>
 #include <cmath>
 #include <cstdarg>
@@ -21,6 +23,7 @@
 #include <vector>
 
 #include "../../include/tn93g.h"
+#include "tn93g_fasta.h"
 
 namespace {
 
@@ -178,103 +181,15 @@ void parse_args(Args &A, int argc, const char **argv)
   }
 }
 
-struct Seqs {
-  std::vector<uint8_t> codes;
-  std::vector<std::string> names;
-  std::vector<int64_t> counts;
-  long N = 0;
-};
+using tn93g_io::Seqs;
 
 signed char g_tab[256];
 
-void add_sequence(Seqs &S, size_t begin, long &firstLen)  // swifttn93.cpp:259-273
-{
-  long len = (long)(S.codes.size() - begin);
-  if (S.N == 0) {
-    firstLen = len;
-    if (firstLen <= 0) {
-      std::cerr << "First sequence length must be positive." << std::endl;
-      throw Exit{1};
-    }
-  } else if (len != firstLen) {
-    std::cerr << "All sequences must have the same length (" << firstLen << "), but sequence '" << S.names.back()
-              << "' had length " << len;
-    throw Exit{1};
-  }
-  S.N++;
-}
-
-// readFASTA (swifttn93.cpp:292-406)
+// readFASTA (swifttn93.cpp:292-406) on all host threads
 int read_fasta(FILE *F, Seqs &S, long &firstLen, char sep)
 {
-  std::vector<unsigned char> buf;
-  {
-    std::vector<unsigned char> tmp(1 << 20);
-    size_t r;
-    while ((r = fread(tmp.data(), 1, tmp.size(), F)) > 0) buf.insert(buf.end(), tmp.data(), tmp.data() + r);
-  }
-  S.codes.reserve(S.codes.size() + buf.size());
-  const unsigned char *p = buf.data(), *end = p + buf.size();
-  int state = 0;
-  size_t seq_begin = S.codes.size();
-  std::string name;
-  while (p < end) {
-    if (state == 0) {
-      while (p < end && *p != '>' && *p != '#') p++;
-      if (p == end) break;
-      p++;
-      state = 1;
-      name.clear();
-      continue;
-    }
-    if (state == 1) {
-      const unsigned char *q = p;
-      while (q < end && *q != '\n' && *q != '\r') q++;
-      name.append((const char *)p, (size_t)(q - p));
-      if (q == end) {
-        p = q;
-        break;
-      }
-      p = q + 1;
-      const long len = (long)name.size();
-      if (len <= 0) {
-        std::cerr << "Sequence names must be non-empty." << std::endl;
-        return 1;
-      }
-      unsigned long count = 1UL;
-      if (len >= 3) {
-        long k;
-        for (k = 2; k < len; k++)
-          if (name[(size_t)(len - k)] == sep) break;
-        if (k < len) count = (unsigned long)(long)atoi(name.c_str() + (len - k + 1));
-        if (count < 1) count = 1;
-      }
-      S.counts.push_back((int64_t)(long)count);
-      S.names.push_back(name);
-      state = 2;
-      seq_begin = S.codes.size();
-      continue;
-    }
-    while (p < end) {
-      const signed char v = g_tab[*p];
-      if (v < 0) break;
-      S.codes.push_back((uint8_t)v);
-      p++;
-    }
-    if (p == end) break;
-    const unsigned char c = *p++;
-    if (c == '>' || c == '#') {
-      add_sequence(S, seq_begin, firstLen);
-      state = 1;
-      name.clear();
-    }
-  }
-  if (state == 2) {
-    add_sequence(S, seq_begin, firstLen);
-    return 0;
-  }
-  std::cerr << "Unexpected end of file: state " << state << std::endl;
-  return 1;
+  const tn93g_io::Input buf(F);
+  return tn93g_io::read_fasta(buf.data(), buf.size(), S, firstLen, sep, g_tab, tn93g_io::io_threads());
 }
 
 int fmt_g(char *out, double d) { return snprintf(out, 64, "%g", d); }
@@ -373,6 +288,7 @@ int run(int argc, const char **argv)
     }
   Seqs S;
   long firstLen = 0;
+  const int threads = tn93g_io::io_threads();
   if (read_fasta(A.input1, S, firstLen, A.counts_in_name) == 1) return 1;
   const unsigned long n1 = (unsigned long)S.N;
   unsigned long n2 = 0;
@@ -434,14 +350,18 @@ int run(int argc, const char **argv)
     }
     if (R->exact || attempt > 0) break;
     // certify: every printed link distance and the summary must be the same at both ends of the enclosures
-    bool ok = true;
-    char a[64], b[64];
-    for (int64_t k = 0; k < R->n_links && ok; k++)
-      if (R->link_dist_lo[k] != R->link_dist_hi[k]) {
-        fmt_g(a, R->link_dist_lo[k]);
-        fmt_g(b, R->link_dist_hi[k]);
-        ok = !strcmp(a, b);
-      }
+    std::atomic<bool> all_same{true};
+    const int64_t nl = R->n_links, per = 1 << 16;
+    tn93g_io::parallel_for(threads, (nl + per - 1) / per, [&](int64_t t) {
+      char a[64], b[64];
+      for (int64_t k = t * per; k < std::min(nl, (t + 1) * per) && all_same.load(std::memory_order_relaxed); k++)
+        if (R->link_dist_lo[k] != R->link_dist_hi[k]) {
+          fmt_g(a, R->link_dist_lo[k]);
+          fmt_g(b, R->link_dist_hi[k]);
+          if (strcmp(a, b)) all_same = false;
+        }
+    });
+    bool ok = all_same;
     if (ok) {
       std::ostringstream s1, s2;
       s1 << std::setprecision(prec);
@@ -476,81 +396,79 @@ int run(int argc, const char **argv)
   const bool emit = !A.do_count;
   const bool cross = A.input2 && !do_fst;
   const long upper = cross ? (long)n1 : (long)N;
-  std::string out;
-  out.reserve(1 << 20);
-  char t[64];
-  int64_t k = 0;
-  std::vector<std::vector<std::pair<long, double>>> hylinks(emit && A.format == F_HYPHY ? N : 0);
-  for (long i = 0; i < upper; i++) {
-    if (!cross && report_self) {
-      if (A.format == F_CSV) {
-        out += S.names[(size_t)i];
-        out += A.delimiter;
-        out += S.names[(size_t)i];
-        out += A.delimiter;
-        out += "0\n";
-      } else if (A.format == F_CSVN) {
-        out += std::to_string(i);
-        out += A.delimiter;
-        out += std::to_string(i);
-        out += A.delimiter;
-        out += "0\n";
+  const int64_t nl = R->n_links;
+  if (A.format != F_HYPHY && (emit || report_self)) {
+    // rows in pieces of about 64K links, formatted on all threads and written in row order
+    std::vector<long> rb;
+    std::vector<int64_t> kb;
+    tn93g_io::split_rows(R->link_row, nl, upper, 1 << 16, rb, kb);
+    tn93g_io::emit_pieces(A.output, threads, rb.size() - 1, (size_t)threads * 4, [&](size_t p, std::string &out) {
+      char t[64];
+      int64_t k = kb[p];
+      for (long i = rb[p]; i < rb[p + 1]; i++) {
+        if (!cross && report_self) {
+          if (A.format == F_CSV) {
+            out += S.names[(size_t)i];
+            out += A.delimiter;
+            out += S.names[(size_t)i];
+          } else {
+            out += std::to_string(i);
+            out += A.delimiter;
+            out += std::to_string(i);
+          }
+          out += A.delimiter;
+          out += "0\n";
+        }
+        for (; k < kb[p + 1] && R->link_row[k] == i; k++) {
+          if (!emit) continue;
+          const long j = (long)R->link_col[k];
+          if (A.format == F_CSV) {
+            out += S.names[(size_t)i];
+            out += A.delimiter;
+            out += S.names[(size_t)j];
+          } else {
+            out += std::to_string(i);
+            out += A.delimiter;
+            out += std::to_string(j);
+          }
+          out += A.delimiter;
+          out.append(t, (size_t)fmt_g(t, R->link_dist[k]));
+          out += '\n';
+        }
       }
-    }
-    for (; k < R->n_links && R->link_row[k] == i; k++) {
-      const long j = (long)R->link_col[k];
-      const double d = R->link_dist[k];
-      if (!emit) continue;
-      if (A.format == F_CSV) {
-        out += S.names[(size_t)i];
-        out += A.delimiter;
-        out += S.names[(size_t)j];
-        out += A.delimiter;
-        out.append(t, (size_t)fmt_g(t, d));
-        out += '\n';
-      } else if (A.format == F_CSVN) {
-        out += std::to_string(i);
-        out += A.delimiter;
-        out += std::to_string(j);
-        out += A.delimiter;
-        out.append(t, (size_t)fmt_g(t, d));
-        out += '\n';
-      } else {
-        hylinks[(size_t)i].push_back(std::make_pair(j, d));
-        hylinks[(size_t)j].push_back(std::make_pair(i, d));
-      }
-    }
-    if (out.size() > (1u << 20)) {
-      fwrite(out.data(), 1, out.size(), A.output);
-      out.clear();
-    }
+    });
   }
-  fwrite(out.data(), 1, out.size(), A.output);
-  out.clear();
   if (emit && A.format == F_HYPHY) {
-    std::string o = "{";
-    for (long r = 0; r < (long)N; r++) {
-      o += "\n{";
-      auto &lk = hylinks[(size_t)r];
-      size_t li = 0;
-      for (long c = 0; c < (long)N; c++) {
-        if (c) o += ',';
-        if (c == r)
-          o += '0';
-        else if (li < lk.size() && lk[li].first == c) {
-          o.append(t, (size_t)fmt_g(t, lk[li].second));
-          li++;
-        } else
-          o += "100";
-      }
-      o += '}';
-      if (o.size() > (1u << 20)) {
-        fwrite(o.data(), 1, o.size(), A.output);
-        o.clear();
-      }
+    // the full N x N matrix, 100 where no link (swifttn93.cpp:2869-2897); rows formatted on all threads
+    std::vector<std::vector<std::pair<long, double>>> hylinks(N);
+    for (int64_t k = 0; k < nl; k++) {
+      const long i = (long)R->link_row[k], j = (long)R->link_col[k];
+      hylinks[(size_t)i].push_back(std::make_pair(j, R->link_dist[k]));
+      hylinks[(size_t)j].push_back(std::make_pair(i, R->link_dist[k]));
     }
-    o += "\n}";
-    fwrite(o.data(), 1, o.size(), A.output);
+    fputc('{', A.output);
+    const size_t rows_per = std::max<size_t>(1, ((size_t)1 << 20) / std::max<size_t>(4 * N, 1));
+    const size_t npieces = (N + rows_per - 1) / rows_per;
+    tn93g_io::emit_pieces(A.output, threads, npieces, (size_t)threads * 4, [&](size_t p, std::string &o) {
+      char t[64];
+      for (long r = (long)(p * rows_per); r < (long)std::min<size_t>(N, (p + 1) * rows_per); r++) {
+        o += "\n{";
+        auto &lk = hylinks[(size_t)r];
+        size_t li = 0;
+        for (long c = 0; c < (long)N; c++) {
+          if (c) o += ',';
+          if (c == r)
+            o += '0';
+          else if (li < lk.size() && lk[li].first == c) {
+            o.append(t, (size_t)fmt_g(t, lk[li].second));
+            li++;
+          } else
+            o += "100";
+        }
+        o += '}';
+      }
+    });
+    fputs("\n}", A.output);
   }
   if (!A.quiet)
     std::cerr << std::endl
