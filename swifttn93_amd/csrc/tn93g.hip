@@ -1555,12 +1555,25 @@ __device__ void cc_table_wave(const RunArgs &A, int64_t i, int64_t j, bool valid
 #pragma unroll
         for (int y = 0; y < 4; y++) c[4 * x + y] += __popc(ai[x] & bj[y]);
     }
+    // butterfly reduce-scatter of the 16 counters over the wave: at the step over lane bit b each lane keeps half of
+    // its counters, adds its partner's copy of that half and sends the other half (8 + 4 + 2 + 1 exchanges), so
+    // that lane l holds counter l >> 2 summed over the lanes that differ in bits 2..5; two more exchanges add the
+    // lanes that differ in bits 0..1.  17 exchanges instead of 16 x 6.
+    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8, h2 = lane & 4;
+    uint32_t c8[8], c4[4], c2[2];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-      uint32_t v = c[k];
-      for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o, 64);
-      if (lane == src) out[src][k] = v;
-    }
+    for (int q = 0; q < 8; q++)
+      c8[q] = (h5 ? c[q + 8] : c[q]) + (uint32_t)__shfl_xor((int)(h5 ? c[q] : c[q + 8]), 32, 64);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      c4[q] = (h4 ? c8[q + 4] : c8[q]) + (uint32_t)__shfl_xor((int)(h4 ? c8[q] : c8[q + 4]), 16, 64);
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+      c2[q] = (h3 ? c4[q + 2] : c4[q]) + (uint32_t)__shfl_xor((int)(h3 ? c4[q] : c4[q + 2]), 8, 64);
+    uint32_t v = (h2 ? c2[1] : c2[0]) + (uint32_t)__shfl_xor((int)(h2 ? c2[0] : c2[1]), 4, 64);
+    v += (uint32_t)__shfl_xor((int)v, 2, 64);
+    v += (uint32_t)__shfl_xor((int)v, 1, 64);
+    if ((lane & 3) == 0) out[src][lane >> 2] = v;
   }
 }
 
@@ -1620,39 +1633,35 @@ __device__ __noinline__ double add_repeat_crossing(double x, double v, uint32_t 
   return x;
 }
 
-__device__ inline double add_repeat(double x, double v, uint32_t m)
+// add_repeat(x, v, m) given inv = 1 / v (exact: v is a power of two, so x * inv is x / v exactly).  A multiple of v
+// stays one: every partial sum is exact (the cells hold at most 2^26 sites, far below 2^53 v).
+__device__ inline double add_repeat(double x, double v, double inv, uint32_t m)
 {
-  // a multiple of v stays one: every partial sum is exact (the cells hold at most 2^26 sites, far below 2^53 v)
-  if (floor(x / v) * v == x) return x + (double)m * v;
+  const double y = x * inv;
+  if (floor(y) == y) return x + (double)m * v;
   return add_repeat_crossing(x, v, m);
 }
 
 __device__ inline void replay_apply_rep(ReplayState &s, const ReplayEnt &E, uint32_t m)
 {
   if (!m) return;
+  if (E.mask) {
+    // 1 / v for a normal power of two v = 2^e: the exponent field 2046 - (e + 1023)
+    const double inv = tn93g_from_bits((2046ull - ((tn93g_bits(E.v) >> 52) & 0x7ffu)) << 52);
 #pragma unroll
-  for (int k = 0; k < 16; k++)
-    if (E.mask >> k & 1u) s.fc[k] = add_repeat(s.fc[k], E.v, m);
+    for (int k = 0; k < 16; k++)
+      if (E.mask >> k & 1u) s.fc[k] = add_repeat(s.fc[k], E.v, inv, m);
+  }
 #pragma unroll
   for (int d = 0; d < 4; d++) s.dgi[d] += E.icd == d ? m : 0u;
   s.amb += E.amb * m;
 }
 
-// Counts of A, C, G, T of sequence q over sites [p, e) from its bit planes.
+// Counts of A, C, G, T of sequence q over sites [p, e): differences of its plane prefix counts (clean, C|T, G|T, T)
 __device__ inline void base_counts(const RunArgs &A, int64_t q, uint32_t p, uint32_t e, uint32_t (&n)[4])
 {
-  uint32_t c = 0, b0 = 0, b1 = 0, t = 0;
-  const uint4 *P = A.planes + q * A.W;
-  for (uint32_t w = p >> 5; w <= (e - 1) >> 5; w++) {
-    uint32_t m = 0xffffffffu;
-    if (w == p >> 5) m &= 0xffffffffu << (p & 31);
-    if (w == (e - 1) >> 5) m &= 0xffffffffu >> (31 - ((e - 1) & 31));
-    const uint4 v = P[w];
-    c += __popc(v.x & m);
-    b0 += __popc(v.y & m);
-    b1 += __popc(v.z & m);
-    t += __popc(v.w & m);
-  }
+  const uint4 hi = plane_prefix(A.pre, A.planes, A.W, q, e), lo = plane_prefix(A.pre, A.planes, A.W, q, p);
+  const uint32_t c = hi.x - lo.x, b0 = hi.y - lo.y, b1 = hi.z - lo.z, t = hi.w - lo.w;
   n[0] = c - b0 - b1 + t;
   n[1] = b0 - t;
   n[2] = b1 - t;
