@@ -197,6 +197,7 @@ struct SetDev {
   int ndense;        // N sites are not in ev: handled through nmask (pair kernel) and nruns (replay)
   int64_t *weights;
   int64_t total_events;
+  int64_t total_gruns;  // gap runs of all sequences
 };
 
 struct ExactRec {
@@ -2302,6 +2303,7 @@ struct Job {
   RunBufs *B = nullptr;
   RunArgs A{};
   bool fast = false;
+  bool sparse_gaps = true;  // the marginal form's gap-run corrections cost less than counting the base sums
   int64_t nrows = 0, ntiles = 0, gx = 0, gy = 0, npairs = 0;
   size_t nr1 = 0;
   unsigned link_cap = 1, exact_cap = 1;
@@ -2372,6 +2374,11 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   A.tb = d.tb;
   A.grun_off = d.grun_off;
   A.gruns = d.gruns;
+  // The marginal form of the pair kernel trades 13 of 24 operations per word pair for, per pair, two prefix
+  // look-ups (scattered loads) per gap run of either sequence; measured on the MI355X one gap run costs about as
+  // much as 105 words of counting, so it is used when 2 x 105 x (gap runs per sequence) < W.  Under GAPMM gaps are
+  // events and cost nothing extra.
+  J.sparse_gaps = P->mode == TN93G_GAPMM || 210 * d.total_gruns < d.W * std::max<int64_t>(d.n, 1);
   A.weights = d.weights;
   A.n = d.n;
   A.Ls = d.Ls;
@@ -2517,7 +2524,7 @@ static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int b
   if (nrows > 0 && A.ncols > 0) {
     dim3 g((unsigned)J.gx, (unsigned)J.gy);
     const bool mf = J.P.backend == TN93G_BACKEND_MFMA && !A.nplane;  // dense-N counting is popcount only
-    const bool marg = !mf && B->tab_fold && !A.skip_events;
+    const bool marg = !mf && B->tab_fold && !A.skip_events && J.sparse_gaps;
     const int v = (J.fast ? 1 : 0) | (A.nplane ? 2 : 0) | (marg ? 4 : 0);
     constexpr int PC = TN93G_BACKEND_POPC, MF = TN93G_BACKEND_MFMA;
     if (mf && J.fast)
@@ -3115,6 +3122,7 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
   HIPCHK(c, hipGetLastError());
   for (int64_t s = 0; s < n; s++) S->hw[(size_t)(n0 + s)] = wts[(size_t)s];
   d.total_events = S->ev_total[0];
+  d.total_gruns = S->ev_total[3];
   d.n = n1;  // published: runs that start from now on see the new sequences
   return 0;
 }
