@@ -101,7 +101,7 @@ def roofline_for(n_pairs, L, pair_ms):
     ops = 22 * words
     achieved = ops * n_pairs / (pair_ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": VALU_INT32_TOPS, "unit": "TOPS",
-            "frac": achieved / VALU_INT32_TOPS, "traffic": None, "kernel": "k_pairs<1, true>", "kernel_ms": pair_ms,
+            "frac": achieved / VALU_INT32_TOPS, "traffic": None, "kernel": "k_pairs<1, true, true, true>", "kernel_ms": pair_ms,
             "ops_per_pair": ops}
 
 
@@ -319,7 +319,9 @@ def main():
     words = (L + 31) // 32
     ops_per_pair = 22 * words  # popcount backend: 12 logic + 10 popcount-accumulate per 32-bit word
     achieved = ops_per_pair * n_pairs / (pair_ms * 1e-3) / 1e12
-    kname = "k_pairs<1, true>" if backend in (T.BACKEND_AUTO, T.BACKEND_POPC) else "k_pairs<2, true>"
+    # template arguments: backend (1 popcount, 2 MFMA), fast path, dense-N planes, marginal form
+    kname = "k_pairs<1, true, false, true>" if backend in (T.BACKEND_AUTO, T.BACKEND_POPC) else \
+        "k_pairs<2, true, false, false>"
     traffic = profiled_traffic(kname)
     roofline = {"bound": "valu", "achieved": achieved, "peak": VALU_INT32_TOPS, "unit": "TOPS",
                 "frac": achieved / VALU_INT32_TOPS, "traffic": traffic,
