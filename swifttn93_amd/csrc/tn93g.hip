@@ -148,8 +148,12 @@ struct ReplayEnt {
 // as fields: x = nf0 | nf1 << 16, y = nf2 | nf3 << 16, z = diag | AG << 16 (16 bits each), w = CT | amb << 14 |
 // nondyadic << 23 (14, 9 and 9 bits).  Sums of up to EVFLUSH = 256 entries fit every field (at most 32 per entry in
 // the 16- and 14-bit fields, at most 1 in the 9-bit ones), so entries are added as plain 32-bit words.
-// Exact whenever every value is a multiple of 1/16 (then all of tn93's float sums are exact and order-free).
-static void event_entry(const SiteEffect &E, uint32_t out[4])
+// Exact whenever every value is a multiple of 1/16 (then all of tn93's float sums are exact and order-free).  An
+// entry with other values (its pairs are replayed) carries only its site's total weight in the nf fields, 32 (2 x 16)
+// for a counted site: on the field of the clean base when one code (a or b) is clean, where the marginal form
+// subtracts that base, else on nf0; so the nf fields of any pair still sum to 32 x its counted sites.  Returns false
+// when a weight is neither 0 nor 1 (then no pair's count is certified from the fields).
+static bool event_entry(const SiteEffect &E, uint32_t out[4], int a = -1, int b = -1)
 {
   double cell[16];
   bool dyadic = true;
@@ -171,10 +175,23 @@ static void event_entry(const SiteEffect &E, uint32_t out[4])
     ag = (uint32_t)(cell[2] + cell[8]);
     ct = (uint32_t)(cell[7] + cell[13]);
   }
+  bool unit = true;
+  if (!dyadic) {
+    // the site's weight: each cell of the mask gets v = 1/k of its resolutions, so k cells add up to 1 (to within
+    // the rounding of v)
+    const double w = (double)__builtin_popcount(E.mask) * E.v + (E.icd >= 0 ? 1.0 : 0.0);
+    if (std::fabs(w - 1.0) < 1e-9) {
+      const bool ca = a >= 0 && a < 4, cb = b >= 0 && b < 4;
+      nf[ca != cb ? (ca ? a : b) : 0] = 32;
+    } else {
+      unit = w == 0.0;
+    }
+  }
   out[0] = nf[0] | nf[1] << 16;
   out[1] = nf[2] | nf[3] << 16;
   out[2] = dg | ag << 16;
   out[3] = ct | (uint32_t)E.amb << 14 | (dyadic ? 0u : 1u << 23);
+  return unit;
 }
 
 // ------------------------------------------------------------------------------------------ device structures
@@ -250,6 +267,7 @@ struct RunArgs {
   // pair space: rows [r0, r1) x cols [c0, c1), triangle => only j > i
   int64_t r0, r1, c0, c1, ncols, n1;
   int triangle, fst, gapmm, skip_events, avg_fallback, exact, all_slow;
+  int gcert;                   // the event tables' nf fields count the sites of order-dependent pairs exactly
   double rf, thr, mind, window;
   int64_t min_overlap;
   const uint4 *evtab;          // [4][18*18] packed event entries: main mode, AVERAGE (the -g fallback), and the same
@@ -263,6 +281,14 @@ struct RunArgs {
   unsigned long long *slow_list;  // [slow_cap] pairs for the site-order replay: i << 32 | j (bit 63: AVERAGE first);
                                   // each pair tile appends its pairs as one contiguous range
   unsigned long long *slow_total; // entries in slow_list (64-bit counter in stripe 0)
+  // Clean/clean 4x4 count tables of the "special" sequences (those with a B, D, H or V code; every order-dependent
+  // pair of the fast path has one) against the other side of the run, for the replay.  Null when not built.
+  const int32_t *spec_row;  // [nrows]: rank of row r0 + r among the special rows, or -1
+  const int32_t *spec_col;  // [ncols]: rank of column c0 + c among the special columns, or -1
+  const uint32_t *tab_row;  // [nsr][16][ncols]: cell 4a + b counts row base a against column base b
+  const uint32_t *tab_col;  // [nsc][16][nrows]: cell 4a + b counts column base a against row base b
+  const int64_t *spec_rows; // [nsr] the special rows (set indices), then [nsc] the special columns
+  int nsr, nsc;
   ExactRec *exact_list;
   unsigned int *exact_cnt;
   unsigned int exact_cap;
@@ -1240,6 +1266,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
   // -g: pairs whose resolved ambiguities reach the fraction are computed again in AVERAGE (refkernel.cpp:742-750):
   // same clean/clean counts, the events through the AVERAGE table.  Order-dependent ones go to the replay instead.
   bool avg[RPW] = {false, false, false, false};
+  bool avgr[RPW] = {false, false, false, false};  // order-dependent pairs whose -g test is decided for AVERAGE
   bool any_avg = false;
   if (A.avg_fallback && do_events) {
 #pragma unroll
@@ -1254,6 +1281,15 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
       const double tng = (double)nfs * 0.03125;
       avg[a] = valid && !acc[a].fl && tng > (double)A.min_overlap && tng * A.rf <= (double)tot[a].amb;
       any_avg |= avg[a];
+      // an order-dependent pair: tng is the exact count of the sites tn93 sums in FP64, whose rounded sum stays within
+      // (16 L^2 + 64 L) 2^-53 of it (16 cells of at most L additions of values below L, v rounded, 32 merge
+      // additions); when tng rf is farther than that from amb (and tng > the overlap, by at least 1), tn93's -g test
+      // is decided: true => the replay starts in AVERAGE (margin doubled, the product's rounding included)
+      if (valid && acc[a].fl && A.gcert && tng > (double)A.min_overlap) {
+        const double Lf = 32.0 * (double)A.W;
+        const double M = ((16.0 * Lf * Lf + 64.0 * Lf) * A.rf + 2.0 * tng * A.rf) * 0x1p-52;
+        avgr[a] = tng * A.rf + M < (double)tot[a].amb;
+      }
     }
   }
   if (__syncthreads_or(any_avg)) {
@@ -1324,7 +1360,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
         else
           state = 1;
       }
-      if (slow) smask |= (1u << a) | (avg[a] ? 1u << (RPW + a) : 0u);
+      if (slow) smask |= (1u << a) | (avg[a] || avgr[a] ? 1u << (RPW + a) : 0u);
     }
     if constexpr (FAST) {
       const unsigned long long need = __ballot(state == 1), skp = __ballot(state == 2);
@@ -1578,6 +1614,88 @@ __device__ void cc_table_wave(const RunArgs &A, int64_t i, int64_t j, bool valid
   }
 }
 
+// Clean/clean 4x4 count tables of `ns` special sequences ids[0..ns) against the sequences [x0, x0 + nx):
+// T[(s * 16 + 4a + b) * nx + x - x0] = sites where sequence ids[s] has clean base a and sequence x clean base b.  A
+// block takes 16 special sequences x 64 others (the pair kernel's tile), their planes staged through LDS KW words at
+// a time; a wave's lanes are the 64 others, each with the 16 counters of the wave's 4 special sequences.
+__global__ __launch_bounds__(256) void k_spec16(RunArgs A, const int64_t *__restrict__ ids, int ns, int64_t x0,
+                                                int64_t nx, uint32_t *__restrict__ T)
+{
+  __shared__ uint4 sI[KW][TI + 1], sJ[KW][TJ + 1];
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int sb = blockIdx.y * TI;
+  const int64_t xb = (int64_t)blockIdx.x * TJ;
+  uint32_t c[RPW][16];
+#pragma unroll
+  for (int a = 0; a < RPW; a++)
+#pragma unroll
+    for (int k = 0; k < 16; k++) c[a][k] = 0;
+  for (int64_t w0 = 0; w0 < A.W; w0 += KW) {
+    const int kw = (int)min((int64_t)KW, A.W - w0);
+    __syncthreads();
+    for (int k = tid; k < kw * (TI + TJ); k += 256) {
+      const int q = k / kw, w = k % kw;
+      int64_t seq = -1;
+      if (q < TI) {
+        if (sb + q < ns) seq = ids[sb + q];
+      } else if (xb + (q - TI) < nx) {
+        seq = x0 + xb + (q - TI);
+      }
+      const uint4 v = seq >= 0 ? A.planes[seq * A.W + w0 + w] : make_uint4(0, 0, 0, 0);
+      if (q < TI)
+        sI[w][q] = v;
+      else
+        sJ[w][q - TI] = v;
+    }
+    __syncthreads();
+    for (int k = 0; k < kw; k++) {
+      const uint4 b = sJ[k][lane];
+      const uint32_t bj[4] = {~(b.y | b.z), b.y & ~b.z, b.z & ~b.y, b.w};
+#pragma unroll
+      for (int a = 0; a < RPW; a++) {
+        const uint4 p = sI[k][wave * RPW + a];
+        const uint32_t M = p.x & b.x;
+        const uint32_t ai[4] = {M & ~(p.y | p.z), M & p.y & ~p.z, M & p.z & ~p.y, M & p.w};
+#pragma unroll
+        for (int x = 0; x < 4; x++)
+#pragma unroll
+          for (int y = 0; y < 4; y++) c[a][4 * x + y] += __popc(ai[x] & bj[y]);
+      }
+    }
+  }
+  const int64_t xl = xb + lane;
+  if (xl < nx)
+#pragma unroll
+    for (int a = 0; a < RPW; a++) {
+      const int sp = sb + wave * RPW + a;
+      if (sp < ns)
+#pragma unroll
+        for (int k = 0; k < 16; k++) T[((size_t)sp * 16 + k) * (size_t)nx + (size_t)xl] = c[a][k];
+    }
+}
+
+// The clean/clean table of pair (i, j) from the special-sequence tables when i or j is special.
+__device__ inline bool cc_table_lookup(const RunArgs &A, int64_t i, int64_t j, uint32_t (&icc)[16])
+{
+  if (!A.spec_row) return false;
+  const int32_t si = A.spec_row[i - A.r0];
+  if (si >= 0) {
+    const uint32_t *t = A.tab_row + (size_t)si * 16 * (size_t)A.ncols + (size_t)(j - A.c0);
+#pragma unroll
+    for (int k = 0; k < 16; k++) icc[k] = t[(size_t)k * (size_t)A.ncols];
+    return true;
+  }
+  const int32_t sj = A.spec_col[j - A.c0];
+  if (sj >= 0) {
+    const int64_t nr = A.r1 - A.r0;
+    const uint32_t *t = A.tab_col + (size_t)sj * 16 * (size_t)nr + (size_t)(i - A.r0);
+#pragma unroll
+    for (int k = 0; k < 16; k++) icc[k] = t[(size_t)(4 * (k & 3) + (k >> 2)) * (size_t)nr];  // transposed
+    return true;
+  }
+  return false;
+}
+
 // The sites where either sequence is ambiguous, in increasing site order: the two sorted event lists merged; a
 // site in one list only takes its partner's code from the codes array.
 __device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *tab, ReplayState &s)
@@ -1783,14 +1901,18 @@ __global__ __launch_bounds__(RB) void k_replay(RunArgs A)
     const bool coop = DENSE && A.W >= 128;  // uniform
     unsigned long long pr = 0;
     if (k < total) pr = A.slow_list[k];
+    uint32_t icc[16];
+    const bool looked = k < total && cc_table_lookup(A, (int64_t)(pr >> 32 & 0x7fffffffu),
+                                                     (int64_t)(pr & 0xffffffffull), icc);
     if (coop)
-      cc_table_wave(A, (int64_t)(pr >> 32 & 0x7fffffffu), (int64_t)(pr & 0xffffffffull), k < total, scc + wave * 64);
+      cc_table_wave(A, (int64_t)(pr >> 32 & 0x7fffffffu), (int64_t)(pr & 0xffffffffull), k < total && !looked,
+                    scc + wave * 64);
     const bool avg_first = pr >> 63;
     pr &= ~(1ull << 63);
     if (k < total) {
       const int64_t i = (int64_t)(pr >> 32), j = (int64_t)(pr & 0xffffffffull);
-      uint32_t icc[16];
-      if (coop) {
+      if (looked) {
+      } else if (coop) {
 #pragma unroll
         for (int q = 0; q < 16; q++) icc[q] = scc[wave * 64 + lane][q];
       } else {
@@ -2137,6 +2259,7 @@ struct tn93g_set {
   int64_t ev_total[4] = {0, 0, 0, 0}, ev_cap[4] = {0, 0, 0, 0};  // entries used / allocated: ev, evg, nruns, gruns
   bool decided = false;                                  // ndense fixed by the first append
   std::vector<int64_t> hw;                            // copy numbers (host), cap entries
+  std::vector<uint8_t> spec;                          // 1: the sequence has a B, D, H or V code (host), cap entries
   std::vector<void *> retired;                        // replaced event arrays
   std::mutex mu;
 };
@@ -2161,16 +2284,19 @@ struct HostBuf {  // pinned host memory (hipHostMalloc), grown on demand
 // of a batch's kernels can be enqueued before any result is read back.
 struct RunBufs {
   DevBuf D, U, hist, row_links, link_off, slow_list, counters, exact_list, rowout,
-      links, links_flat, lslot, evtab, rtab, patch, chdr, cdata;
+      links, links_flat, lslot, evtab, rtab, patch, chdr, cdata, spec_idx, spec_ids, spec_tab;
+  std::vector<int32_t> h_spec_idx;  // host staging of spec_idx / spec_ids (kept until the next run of the slot)
+  std::vector<int64_t> h_spec_ids;
   int64_t link_cap_hint = 0;  // link region size that sufficed last time
   size_t link_spec = 4096;     // link records a single run reads back with its counters
   int tab_mode = -1;           // mode and subset of the tables in evtab/rtab (uploaded again only when they change)
   uint32_t tab_mask = 0;
   bool tab_fold = false;       // the folded tables are valid (the marginal form of the pair kernel may run)
+  bool tab_gcert = false;      // every order-dependent site of the main table weighs 0 or 1 (see event_entry)
   void release()
   {
-    for (DevBuf *b : {&D, &U, &hist, &row_links, &link_off, &slow_list, &counters,
-                      &exact_list, &rowout, &links, &links_flat, &lslot, &evtab, &rtab, &patch, &chdr, &cdata}) {
+    for (DevBuf *b : {&D, &U, &hist, &row_links, &link_off, &slow_list, &counters, &exact_list, &rowout, &links,
+                      &links_flat, &lslot, &evtab, &rtab, &patch, &chdr, &cdata, &spec_idx, &spec_ids, &spec_tab}) {
       if (b->p) (void)hipFree(b->p);
       b->p = nullptr;
       b->n = 0;
@@ -2188,7 +2314,7 @@ struct tn93g_ctx {
   // per branch (a single run uses 0): the stream on which a job's tail kernel runs beside its replay
   hipStream_t rside[NBRANCH] = {};
   hipEvent_t rfork[NBRANCH] = {}, rjoin[NBRANCH] = {};
-  hipEvent_t ev[4];
+  hipEvent_t ev[5];  // pair kernel start, its end, run end, tail end, run start
   std::string err;
   std::vector<RunBufs *> slots;
   HostBuf pin;  // staging of the read-backs (counters, links, row accumulators, histogram)
@@ -2316,12 +2442,12 @@ struct Job {
 // event code (4..16, or the gap under GAPMM) has the clean base subtracted from its nf field, because that base is
 // counted in the sequence's totals.  Such a site always adds at least 1 (x16) to that field (the clean row or column
 // of the site's cells sums to 1, or the site resolves to the base's diagonal cell, 2), so the result stays a
-// non-negative 16-bit field; false when it would not.  Order-dependent entries are left alone (their pairs are
-// replayed).
+// non-negative 16-bit field; false when it would not.  Order-dependent entries are folded the same way: their nf
+// fields hold only the site's weight (event_entry), which the fold keeps consistent with the base sums.
 static bool fold_entry(int a, int b, bool gapmm, uint32_t q[4])
 {
   const bool ca = a < 4, cb = b < 4;
-  if (ca == cb || (q[3] >> 23)) return true;
+  if (ca == cb) return true;
   const int other = ca ? b : a;
   if (other == GAPC && !gapmm) return true;  // never an event: gap runs cover these sites
   const int x = ca ? a : b;
@@ -2427,14 +2553,15 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   if (!dry_bytes && (B->tab_mode != P->mode || B->tab_mask != P->subset_mask || !B->evtab.p || !B->rtab.p)) {
     std::vector<uint4> et(4 * NCODE * NCODE);
     std::vector<ReplayEnt> rt(2 * NCODE * NCODE);
-    bool fold_ok = true;
+    bool fold_ok = true, gcert = true;
     for (int a = 0; a < NCODE; a++)
       for (int b = 0; b < NCODE; b++) {
         SiteEffect E = site_effect(a, b, P->mode, P->subset_mask);
         for (int t = 0; t < 2; t++) {
           SiteEffect F = t ? site_effect(a, b, TN93G_AVERAGE, 0) : E;
           uint32_t q[4];
-          event_entry(F, q);
+          const bool unit = event_entry(F, q, a, b);
+          if (t == 0) gcert = gcert && unit;
           et[(size_t)(t * NCODE * NCODE + a * NCODE + b)] = make_uint4(q[0], q[1], q[2], q[3]);
           fold_ok = fold_entry(a, b, P->mode == TN93G_GAPMM, q) && fold_ok;
           et[(size_t)((2 + t) * NCODE * NCODE + a * NCODE + b)] = make_uint4(q[0], q[1], q[2], q[3]);
@@ -2452,7 +2579,9 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
     B->tab_mode = P->mode;
     B->tab_mask = P->subset_mask;
     B->tab_fold = fold_ok;
+    B->tab_gcert = gcert;
   }
+  A.gcert = B->tab_gcert ? 1 : 0;
   A.evtab = (const uint4 *)B->evtab.p;
   A.rtab = (const ReplayEnt *)B->rtab.p;
   // buffers
@@ -2504,6 +2633,52 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   A.chdr = (ChunkHdr *)B->chdr.p;
   A.cdata = (uint32_t *)B->cdata.p;
   A.gx = J.gx;
+  // count tables of the special sequences for the replay: rows against the run's columns and, unless the run is a
+  // triangle over one range (then a column's table is its row table), columns against the run's rows.  Long
+  // alignments only (128 words or more): below that the replay's own per-pair count is cheaper than 16 gathers.
+  A.spec_row = A.spec_col = nullptr;
+  A.tab_row = A.tab_col = nullptr;
+  A.spec_rows = nullptr;
+  A.nsr = A.nsc = 0;
+  if (J.fast && !A.exact && !A.all_slow && P->mode != TN93G_SKIP && nrows > 0 && A.ncols > 0 && A.W >= 128) {
+    const bool same = A.r0 == A.c0 && A.r1 == A.c1;
+    std::vector<int32_t> &idx = B->h_spec_idx;
+    std::vector<int64_t> &ids = B->h_spec_ids;
+    idx.assign((size_t)(nrows + (same ? 0 : A.ncols)), -1);
+    ids.clear();
+    for (int64_t r = 0; r < nrows; r++)
+      if (S->spec[(size_t)(A.r0 + r)]) {
+        idx[(size_t)r] = (int32_t)ids.size();
+        ids.push_back(A.r0 + r);
+      }
+    const int64_t nsr = (int64_t)ids.size();
+    if (!same)
+      for (int64_t q = 0; q < A.ncols; q++)
+        if (S->spec[(size_t)(A.c0 + q)]) {
+          idx[(size_t)(nrows + q)] = (int32_t)(ids.size() - (size_t)nsr);
+          ids.push_back(A.c0 + q);
+        }
+    const int64_t nsc = (int64_t)ids.size() - nsr;
+    const size_t cells = ((size_t)nsr * (size_t)A.ncols + (size_t)nsc * (size_t)nrows) * 16;
+    // beyond 8 GB (or 2^20 special sequences on a side) the replay counts per pair instead
+    if (nsr + nsc > 0 && cells * 4 <= ((size_t)8 << 30) && nsr < 65535 * TI && nsc < 65535 * TI) {
+      if (need(B->spec_idx, idx.size() * 4) || need(B->spec_ids, ids.size() * 8) || need(B->spec_tab, cells * 4))
+        return TN93G_E_NOMEM;
+      if (!dry_bytes) {
+        HIPCHK(c, hipMemcpyAsync(B->spec_idx.p, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(B->spec_ids.p, ids.data(), ids.size() * 8, hipMemcpyHostToDevice, c->stream));
+        const int32_t *ix = (const int32_t *)B->spec_idx.p;
+        const uint32_t *tb = (const uint32_t *)B->spec_tab.p;
+        A.spec_row = ix;
+        A.spec_col = same ? ix : ix + nrows;
+        A.tab_row = tb;
+        A.tab_col = same ? tb : tb + (size_t)nsr * (size_t)A.ncols * 16;
+        A.spec_rows = (const int64_t *)B->spec_ids.p;
+        A.nsr = (int)nsr;
+        A.nsc = (int)nsc;
+      }
+    }
+  }
   return 0;
 }
 
@@ -2516,10 +2691,17 @@ static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int b
   RunArgs &A = J.A;
   RunBufs *B = J.B;
   const int64_t nrows = J.nrows;
+  if (timing) (void)hipEventRecord(c->ev[4], st);
   fill_zero(st, B->hist.p, (size_t)NSTRIPE * 600 * 8);
   fill_zero(st, B->counters.p, STRIPE_BYTES);
   fill_zero(st, B->row_links.p, J.nr1 * 4);
   if (J.fast) fill_zero(st, B->rowout.p, J.nr1 * sizeof(RowAcc));
+  if (A.nsr)
+    hipLaunchKernelGGL(k_spec16, dim3((unsigned)((A.ncols + TJ - 1) / TJ), (unsigned)((A.nsr + TI - 1) / TI)),
+                       dim3(256), 0, st, A, A.spec_rows, A.nsr, A.c0, A.ncols, const_cast<uint32_t *>(A.tab_row));
+  if (A.nsc)
+    hipLaunchKernelGGL(k_spec16, dim3((unsigned)((nrows + TJ - 1) / TJ), (unsigned)((A.nsc + TI - 1) / TI)), dim3(256),
+                       0, st, A, A.spec_rows + A.nsr, A.nsc, A.r0, nrows, const_cast<uint32_t *>(A.tab_col));
   if (timing) (void)hipEventRecord(c->ev[0], st);
   if (nrows > 0 && A.ncols > 0) {
     dim3 g((unsigned)J.gx, (unsigned)J.gy);
@@ -2869,7 +3051,7 @@ static int job_exact_result(tn93g_ctx *c, Job &J, tn93g_result *R)
 static void set_times(tn93g_ctx *c, tn93g_result *R, bool single)
 {
   float ms_all = 0, ms_pair = 0, ms_tail = 0;
-  (void)hipEventElapsedTime(&ms_all, c->ev[0], c->ev[2]);
+  (void)hipEventElapsedTime(&ms_all, c->ev[4], c->ev[2]);
   if (single) {
     (void)hipEventElapsedTime(&ms_pair, c->ev[0], c->ev[1]);
     (void)hipEventElapsedTime(&ms_tail, c->ev[1], c->ev[3]);
@@ -2959,6 +3141,7 @@ int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
   d.W64 = (L + 63) / 64;
   const int64_t nn = std::max<int64_t>(capacity, 1);
   S->hw.assign((size_t)nn, 1);
+  S->spec.assign((size_t)nn, 0);
   auto fail = [&](int code, const std::string &m) {
     tn93g_set_free(S);
     return c->fail(code, m);
@@ -3005,8 +3188,14 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
   const int64_t L = d.L, n0 = d.n;
   if (n0 + n > S->cap) return c->fail(TN93G_E_ARG, "set capacity exceeded");
   if (n == 0) return 0;
-  for (int64_t k = 0; k < n * L; k++)
-    if (codes[k] > GAPC) return c->fail(TN93G_E_ARG, "codes must be ValidChars indices 0..17");
+  for (int64_t q = 0; q < n; q++) {
+    bool sp = false;
+    for (int64_t k = q * L; k < (q + 1) * L; k++) {
+      if (codes[k] > GAPC) return c->fail(TN93G_E_ARG, "codes must be ValidChars indices 0..17");
+      sp |= (unsigned)(codes[k] - 11) < 4u;  // B, D, H, V: the only codes whose effects are not dyadic
+    }
+    S->spec[(size_t)(n0 + q)] = sp ? 1 : 0;
+  }
   HIPCHK(c, hipSetDevice(c->device));
   std::vector<uint8_t> padded((size_t)(n * d.Ls), (uint8_t)GAPC);
   for (int64_t s = 0; s < n; s++) memcpy(&padded[(size_t)(s * d.Ls)], codes + s * L, (size_t)L);
