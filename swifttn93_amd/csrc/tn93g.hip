@@ -1879,9 +1879,11 @@ __device__ void rowacc_add_wave(const RunArgs &A, int64_t row, const RowPart &rp
 }
 
 // one thread per pair that needs the site-order replay (the flat list the pair kernel's tiles appended to); every
-// lane of a wave runs the same iterations so the row parts can be combined across the wave
+// lane of a wave runs the same iterations so the row parts can be combined across the wave.  The replay is latency
+// bound: register budgets for 4 waves per SIMD (3 with dense N) measured fastest on the MI355X, although the compiler
+// then spills a few registers to scratch.
 template <bool DENSE>
-__global__ __launch_bounds__(RB) void k_replay(RunArgs A)
+__global__ __launch_bounds__(RB, DENSE ? 3 : 4) void k_replay(RunArgs A)
 {
   __shared__ ReplayEnt tabs[2 * NCODE * NCODE];
   __shared__ unsigned long long lhist[3 * 200];
