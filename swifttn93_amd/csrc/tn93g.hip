@@ -1446,21 +1446,44 @@ __global__ __launch_bounds__(256) void k_tail(RunArgs A)
   for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
     const int64_t i = A.r0 + r;
     RowPart rp = rowpart_zero();
-    for (int64_t bx = wave; bx < A.gx; bx += 4) {
-      const int64_t j0 = c0a + bx * TJ;
-      if (A.triangle && j0 + TJ - 1 <= A.r0 + (r / TI) * TI) continue;  // the pair kernel skipped this tile
+    // the wave's chunks of the row, bx = wave, wave + 4, ...: the next chunk's header and (16-bit packed) inputs are
+    // loaded before the current one is computed, so the loads of one chunk overlap the FP64 work of the previous
+    const int64_t tri = A.r0 + (r / TI) * TI;  // first row of the pair kernel's tile row
+    auto live = [&](int64_t bx) { return bx < A.gx && !(A.triangle && c0a + bx * TJ + TJ - 1 <= tri); };
+    int64_t bx = wave;
+    while (bx < A.gx && !live(bx)) bx += 4;
+    ChunkHdr h{0, 0};
+    uint32_t p4[4] = {0, 0, 0, 0};
+    if (bx < A.gx) {
       const int64_t k = r * A.gx + bx;
-      const ChunkHdr h = A.chdr[k];
-      if (!(h.need | h.skip)) continue;
-      const int64_t j = j0 + lane;
-      if (h.need >> lane & 1ull) {
+      h = A.chdr[k];
+      if (A.c16) {
         const uint32_t *d = A.cdata + (size_t)k * (7 * 64) + lane;
+        p4[0] = d[0], p4[1] = d[64], p4[2] = d[128], p4[3] = d[192];
+      }
+    }
+    while (bx < A.gx) {
+      int64_t nb = bx + 4;
+      while (nb < A.gx && !live(nb)) nb += 4;
+      ChunkHdr hn{0, 0};
+      uint32_t n4[4] = {0, 0, 0, 0};
+      if (nb < A.gx) {
+        const int64_t k = r * A.gx + nb;
+        hn = A.chdr[k];
+        if (A.c16) {
+          const uint32_t *d = A.cdata + (size_t)k * (7 * 64) + lane;
+          n4[0] = d[0], n4[1] = d[64], n4[2] = d[128], n4[3] = d[192];
+        }
+      }
+      const int64_t k = r * A.gx + bx;
+      const int64_t j = c0a + bx * TJ + lane;
+      if (h.need >> lane & 1ull) {
         uint32_t q[7];
         if (A.c16) {
-          const uint32_t a = d[0], b = d[64], c = d[128];
-          q[0] = a & 0xffffu, q[1] = a >> 16, q[2] = b & 0xffffu, q[3] = b >> 16, q[4] = c & 0xffffu, q[5] = c >> 16;
-          q[6] = d[192];
+          q[0] = p4[0] & 0xffffu, q[1] = p4[0] >> 16, q[2] = p4[1] & 0xffffu, q[3] = p4[1] >> 16;
+          q[4] = p4[2] & 0xffffu, q[5] = p4[2] >> 16, q[6] = p4[3];
         } else {
+          const uint32_t *d = A.cdata + (size_t)k * (7 * 64) + lane;
           for (int f = 0; f < 7; f++) q[f] = d[64 * f];
         }
         tn93g_tailin tin;
@@ -1475,6 +1498,10 @@ __global__ __launch_bounds__(256) void k_tail(RunArgs A)
       } else if (h.skip >> lane & 1ull) {
         skip_pair(A, i, j, rp);
       }
+      bx = nb;
+      h = hn;
+#pragma unroll
+      for (int f = 0; f < 4; f++) p4[f] = n4[f];
     }
     RowPart q;
     q.slo = wave_sum_d(rp.slo);
