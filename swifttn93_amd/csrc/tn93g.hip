@@ -310,7 +310,7 @@ struct ChunkHdr {
   unsigned long long need, skip;
 };
 
-constexpr int NSTRIPE = 64;
+constexpr int NSTRIPE = 64;  // one per lane of a wave (k_link_gather sums them across one)
 enum { ST_UNCERTAIN = 0, ST_UNDECIDED = 1, ST_LINKS = 2, ST_EXACT = 3, ST_SLOW = 4 /* u64, stripe 0 only */ };
 
 __device__ inline int fst_bin(const RunArgs &A, int64_t i, int64_t j)
@@ -1930,11 +1930,12 @@ __global__ __launch_bounds__(RB, DENSE ? 3 : 4) void k_replay(RunArgs A)
   __shared__ BlockStats bs;
   __shared__ uint32_t scc[DENSE ? RB : 1][16];  // clean/clean tables built by the wave (long alignments)
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int64_t total = (int64_t)*A.slow_total;
+  if ((int64_t)blockIdx.x * RB >= total) return;  // no pair for this block (uniform): a run without replays is cheap
   for (int k = tid; k < 2 * NCODE * NCODE; k += RB) tabs[k] = A.rtab[k];
   for (int k = tid; k < 600; k += RB) lhist[k] = 0;
   if (tid == 0) bs.unc = bs.undecided = 0;
   __syncthreads();
-  const int64_t total = (int64_t)*A.slow_total;
   const int64_t nw = (int64_t)gridDim.x * (RB / 64);
   for (int64_t kb = ((int64_t)blockIdx.x * (RB / 64) + wave) * 64; kb < total; kb += nw * 64) {
     const int64_t k = kb + lane;
@@ -2002,8 +2003,14 @@ __global__ void k_link_gather(RunArgs A, LinkRec *__restrict__ out, const int64_
                               uint32_t *__restrict__ slot)
 {
   const int st = blockIdx.x;
-  int64_t off = 0;
-  for (int t = 0; t < st; t++) off += min(A.stripes[t * 32 + ST_LINKS], A.link_cap);
+  __shared__ int64_t soff;
+  if (threadIdx.x < 64) {  // the links of the stripes before st, summed across one wave (NSTRIPE = 64)
+    int64_t v = (int)threadIdx.x < st ? (int64_t)min(A.stripes[threadIdx.x * 32 + ST_LINKS], A.link_cap) : 0;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (threadIdx.x == 0) soff = v;
+  }
+  __syncthreads();
+  const int64_t off = soff;
   const unsigned cnt = min(A.stripes[st * 32 + ST_LINKS], A.link_cap);
   const int64_t cap = (int64_t)A.link_cap * NSTRIPE;
   for (unsigned k = threadIdx.x; k < cnt; k += blockDim.x) {
