@@ -2196,18 +2196,18 @@ __global__ __launch_bounds__(RT) void k_rows(RunArgs A, RowOut O)
   const int64_t rb = (int64_t)blockIdx.x * RT, r = rb + lane;
   const bool live = r < nrows;
   const int64_t i = A.r0 + (live ? r : nrows - 1);
-  const int64_t lower = A.triangle ? i + 1 : A.c0;
+  const int64_t lower = A.triangle ? max(i + 1, A.c0) : A.c0;  // an append run's columns start at c0
   const int64_t wi = A.weights[i];
   double slo[3] = {0., 0., 0.}, shi[3] = {0., 0., 0.}, ws[3] = {0., 0., 0.}, mlo[3] = {0., 0., 0.},
          mhi[3] = {0., 0., 0.};
-  if (A.triangle && wi > 1) {  // self_v_self (swifttn93.cpp:2406-2415)
+  if (A.triangle && i >= A.c0 && wi > 1) {  // self_v_self (swifttn93.cpp:2406-2415), the run's own sequences
     unsigned long long svs = (unsigned long long)((wi * (wi - 1)) >> 1);
     ws[(A.fst && i >= A.n1) ? 1 : 0] += (double)svs;
   }
   int64_t skipped = 0, linkw = 0, k = O.link_off[live ? r : 0];
   const int64_t kend = live ? O.link_off[r + 1] : k;
   // column tiles: from the tile holding the block's first column to the end
-  const int64_t cfirst = A.triangle ? A.r0 + rb + 1 : A.c0;
+  const int64_t cfirst = A.triangle ? max(A.r0 + rb + 1, A.c0) : A.c0;
   const int64_t jt0 = cfirst & ~(int64_t)(RT - 1);
   // tile loader: lane = column; rows rb .. rb + 63 of the tile
   double nd[RT];

@@ -180,6 +180,26 @@ def test_fast_path_equals_replay(engine, mode):
     s.free()
 
 
+@pytest.mark.parametrize("mode", [T.RESOLVE, T.AVERAGE])
+def test_long_alignment_count_tables(engine, mode):
+    """Long alignments (5,000 columns): the replay takes the clean/clean counts of order-dependent pairs from the
+    special-sequence tables, rows against columns and (cross, append) columns against rows; every kind must match the
+    per-pair counts of the all-replay run and the exact mode pair for pair."""
+    codes = gen.c1(seed=9, n=260, L=5000, clusters=8, bdhv=0.002)  # 24 sequences with B/D/H/V codes
+    s = engine.upload(codes)
+    base = dict(mode=mode, threshold=1.0, min_overlap=500, resolve_fraction=0.05 if mode == T.RESOLVE else 1.0)
+    for kw in [dict(), dict(kind=T.CROSS, n1=100), dict(kind=T.APPEND, n1=200)]:
+        kw = dict(base, **kw)
+        fast = engine.run(s, **kw)
+        slow = engine.run(s, flags=T.FLAG_ALL_SLOW, **kw)
+        exact = engine.run(s, flags=T.FLAG_EXACT, **kw)
+        assert fast["n_replayed"] > 0, kw
+        _same(fast, slow)
+        _same(fast, exact)
+        assert (fast["link_lo"] <= exact["link_dist"]).all() and (exact["link_dist"] <= fast["link_hi"]).all()
+    s.free()
+
+
 def test_full_size_c1_vs_oracle(tmp_path):
     """The benchmark workload itself (6,700 x 1,300, 22.4 M pairs) against the oracle, byte for byte."""
     fa = tmp_path / "c1.fa"
