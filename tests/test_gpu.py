@@ -180,14 +180,15 @@ def test_fast_path_equals_replay(engine, mode):
     s.free()
 
 
-@pytest.mark.parametrize("mode", [T.RESOLVE, T.AVERAGE])
+@pytest.mark.parametrize("mode", [T.RESOLVE, T.AVERAGE, T.GAPMM, T.SUBSET])
 def test_long_alignment_count_tables(engine, mode):
     """Long alignments (5,000 columns): the replay takes the clean/clean counts of order-dependent pairs from the
     special-sequence tables, rows against columns and (cross, append) columns against rows; every kind must match the
     per-pair counts of the all-replay run and the exact mode pair for pair."""
     codes = gen.c1(seed=9, n=260, L=5000, clusters=8, bdhv=0.002)  # 24 sequences with B/D/H/V codes
     s = engine.upload(codes)
-    base = dict(mode=mode, threshold=1.0, min_overlap=500, resolve_fraction=0.05 if mode == T.RESOLVE else 1.0)
+    base = dict(mode=mode, threshold=1.0, min_overlap=500, resolve_fraction=0.05 if mode == T.RESOLVE else 1.0,
+                subset_mask=(1 << 5) | (1 << 6) | (1 << 11) | (1 << 15))
     for kw in [dict(), dict(kind=T.CROSS, n1=100), dict(kind=T.APPEND, n1=200)]:
         kw = dict(base, **kw)
         fast = engine.run(s, **kw)
