@@ -1834,15 +1834,18 @@ __device__ inline void base_counts(const RunArgs &A, int64_t q, uint32_t p, uint
 // (N, code).
 __device__ void replay_events_dense(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *tab, ReplayState &s)
 {
-  constexpr uint32_t INF = 0xffffffffu;
+  constexpr uint32_t INF = 0xffffffffu, NOEV = 0xffffffffu;  // no event word is all ones (sites < 2^26)
+  const uint2 NORUN = make_uint2(INF, INF);
   int a = A.ev_off[i], ae = A.ev_off[i + 1], b = A.ev_off[j], be = A.ev_off[j + 1];
   int ra = A.nrun_off[i], rae = A.nrun_off[i + 1], rb = A.nrun_off[j], rbe = A.nrun_off[j + 1];
   const uint8_t *si = A.codes + i * A.Ls, *sj = A.codes + j * A.Ls;
   const ReplayEnt &nn = tab[NC * NCODE + NC];
+  // the heads of the four lists, each read again only when it advances
+  uint32_t wa = a < ae ? A.ev[a] : NOEV, wb = b < be ? A.ev[b] : NOEV;
+  uint2 ri = ra < rae ? A.nruns[ra] : NORUN, rj = rb < rbe ? A.nruns[rb] : NORUN;
   uint32_t p = 0;
   for (;;) {
-    const uint32_t ea = a < ae ? A.ev[a] >> 5 : INF, eb = b < be ? A.ev[b] >> 5 : INF;
-    const uint2 ri = ra < rae ? A.nruns[ra] : make_uint2(INF, INF), rj = rb < rbe ? A.nruns[rb] : make_uint2(INF, INF);
+    const uint32_t ea = wa == NOEV ? INF : wa >> 5, eb = wb == NOEV ? INF : wb >> 5;
     const bool ni = ri.x <= p, nj = rj.x <= p;  // inside an N run at p
     const uint32_t q = min(min(ea, eb), min(ni ? ri.y : ri.x, nj ? rj.y : rj.x));
     if (q == INF) break;
@@ -1857,18 +1860,17 @@ __device__ void replay_events_dense(const RunArgs &A, int64_t i, int64_t j, cons
       }
     }
     p = q;
-    if (ra < rae && ri.y <= p) ra++;
-    if (rb < rbe && rj.y <= p) rb++;
+    if (ri.y <= p) ri = ++ra < rae ? A.nruns[ra] : NORUN;  // runs are maximal: the next starts after p
+    if (rj.y <= p) rj = ++rb < rbe ? A.nruns[rb] : NORUN;
     if (ea == p || eb == p) {  // the site p
-      const bool ni2 = ra < rae && A.nruns[ra].x <= p, nj2 = rb < rbe && A.nruns[rb].x <= p;
-      const unsigned c1 = ea == p ? (A.ev[a] & 31u) : ni2 ? (unsigned)NC : si[p];
-      const unsigned c2 = eb == p ? (A.ev[b] & 31u) : nj2 ? (unsigned)NC : sj[p];
+      const unsigned c1 = ea == p ? (wa & 31u) : ri.x <= p ? (unsigned)NC : si[p];
+      const unsigned c2 = eb == p ? (wb & 31u) : rj.x <= p ? (unsigned)NC : sj[p];
       replay_apply(s, tab[c1 * NCODE + c2]);
-      if (ea == p) a++;
-      if (eb == p) b++;
+      if (ea == p) wa = ++a < ae ? A.ev[a] : NOEV;
+      if (eb == p) wb = ++b < be ? A.ev[b] : NOEV;
       p++;
-      if (ra < rae && A.nruns[ra].y <= p) ra++;
-      if (rb < rbe && A.nruns[rb].y <= p) rb++;
+      if (ri.y <= p) ri = ++ra < rae ? A.nruns[ra] : NORUN;
+      if (rj.y <= p) rj = ++rb < rbe ? A.nruns[rb] : NORUN;
     }
   }
 }
