@@ -9,17 +9,21 @@
 //   onehot  [npad/4][W64][4][16][16] u8 one-hot A,C,G,T per site in the operand order of v_mfma_i32_16x16x64_i8
 //                                                                                (clean/clean counts, MFMA backend)
 //   ev_off/ev   [n+1] i32, u32 (site << 5 | code): ambiguity events in site order, gaps excluded
-//   evg_off/evg the same with gap sites (-a gapmm);   nrun_off/nruns  N runs (first site, length) of dense-N sets
+//   evg_off/evg the same with gap sites (-a gapmm);   nrun_off/nruns  N runs [first, end) of dense-N sets;
+//   grun_off/gruns gap runs;   pre [n][W+1] uint4 plane counts before each word boundary, tb [n] their totals
 // Per run: either the fast-path row accumulators and unordered links (distances certified in the kernel), or D
 // [rows][cols] f64 + U [rows][cols] u16 (ulps that bound an uncertain logarithm) for the exact path; the histogram
-// in per-stripe replicas, the replay and host-exact lists.
+// in per-stripe replicas, the flat replay list and the host-exact list; for long alignments the 16-cell count
+// tables of the sequences with B/D/H/V codes against the other side of the run.
 //
 // Kernels of a run (on the context's stream, or one of its side streams in a captured batch):
+//   k_spec16  16-cell count tables of the special sequences (long alignments with B/D/H/V codes)
 //   k_pairs   clean/clean counts (popcount, popcount+nmask, or MFMA) + event tables from LDS + -g fallback pass;
 //             dyadic pairs complete here, order-dependent ones are appended to the flat replay list
 //   k_tail    FP64 TN93/K2P tail, certification, histogram, row accumulators and links (fast path) or D/U
 //   k_replay  site-order FP64 replay for pairs with non-dyadic contributions (runs of N as repeated additions)
-//   k_scan    per-row link offsets;  k_link_gather  links into one array;  k_hist_sum  histogram replicas summed
+//   k_scan    per-row link offsets;  k_link_gather  links into one array;  k_link_order  links in tn93's row and
+//             column order;  k_hist_sum  histogram replicas summed
 //   k_patch   host-exact distances into D;  k_rows  per-row sums in column order (exact path only)
 #include <hip/hip_runtime.h>
 
@@ -867,7 +871,6 @@ __device__ __attribute__((always_inline)) inline void chunk_codes(const RunArgs 
 //   sTabC[c][x]  a column event of code c facing the row's code x, zero where x is ambiguous (that site is counted
 //                as the row's event).
 constexpr int TABW = NCODE + 1, EVNONE = NCODE, EVB = 4;
-constexpr uint32_t EV_PAD = (uint32_t)EVNONE;  // site 0, code EVNONE
 
 // Fills the LDS event tables from one [NCODE][NCODE] table of packed entries (entry [row code][column code]).
 // With `gaps` (GAPMM), gap sites are events too, so a column event facing a row gap is the row's event.  With
@@ -2645,7 +2648,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   J.gx = std::max<int64_t>((A.c1 - c0a + TJ - 1) / TJ, 1);
   J.gy = std::max<int64_t>((nrows + TI - 1) / TI, 1);
   J.ntiles = J.gx * J.gy;
-  const int64_t ntiles = J.ntiles, nslots = J.fast ? (int64_t)nr1 * J.gx : 1;
+  const int64_t nslots = J.fast ? (int64_t)nr1 * J.gx : 1;
   if (J.gx >= (1ll << 31) || J.gy >= 65536) return c->fail(TN93G_E_ARG, "too many pair tiles in one run");
   if ((!J.fast && (need(B->D, nbox * 8) || need(B->U, nbox * 2))) ||
       need(B->hist, (size_t)NSTRIPE * 600 * 8) || need(B->row_links, nr1 * 4) ||
