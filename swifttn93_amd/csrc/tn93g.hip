@@ -1299,12 +1299,13 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
       any_avg |= avg[a];
       // an order-dependent pair: tng is the exact count of the sites tn93 sums in FP64, whose rounded sum stays within
       // (16 L^2 + 64 L) 2^-53 of it (16 cells of at most L additions of values below L, v rounded, 32 merge
-      // additions); when tng rf is farther than that from amb (and tng > the overlap, by at least 1), tn93's -g test
-      // is decided: true => the replay starts in AVERAGE (margin doubled, the product's rounding included)
-      if (valid && acc[a].fl && A.gcert && tng > (double)A.min_overlap) {
+      // additions); when tng is farther than that above the overlap and tng rf farther than that from amb, tn93's
+      // -g test is decided: true => the replay starts in AVERAGE (margins doubled, the product's rounding included)
+      if (valid && acc[a].fl && A.gcert) {
         const double Lf = 32.0 * (double)A.W;
-        const double M = ((16.0 * Lf * Lf + 64.0 * Lf) * A.rf + 2.0 * tng * A.rf) * 0x1p-52;
-        avgr[a] = tng * A.rf + M < (double)tot[a].amb;
+        const double E = (16.0 * Lf * Lf + 64.0 * Lf) * 0x1p-52;  // twice the bound on |tng_fp - tng|
+        const double M = E * A.rf + 2.0 * tng * A.rf * 0x1p-52;
+        avgr[a] = tng - E > (double)A.min_overlap && tng * A.rf + M < (double)tot[a].amb;
       }
     }
   }
