@@ -317,16 +317,19 @@ def main():
     value = n_pairs * a.steps / dt
     pair_ms = float(np.mean(pk))
     words = (L + 31) // 32
-    ops_per_pair = 22 * words  # popcount backend: 12 logic + 10 popcount-accumulate per 32-bit word
+    popc = backend in (T.BACKEND_AUTO, T.BACKEND_POPC)
+    # popcount backend: 12 logic + 10 popcount-accumulate int32 operations per 32-bit word, against the VALU peak;
+    # MFMA backend: the 4 x 4 one-hot product, 16 int8 multiply-adds = 32 operations per site, against the int8 peak
+    ops_per_pair = 22 * words if popc else 32 * L
+    peak = VALU_INT32_TOPS if popc else MFMA_INT8_TOPS
     achieved = ops_per_pair * n_pairs / (pair_ms * 1e-3) / 1e12
     # template arguments: backend (1 popcount, 2 MFMA), fast path, dense-N planes, marginal form
-    kname = "k_pairs<1, true, false, true>" if backend in (T.BACKEND_AUTO, T.BACKEND_POPC) else \
-        "k_pairs<2, true, false, false>"
+    kname = "k_pairs<1, true, false, true>" if popc else "k_pairs<2, true, false, false>"
     traffic = profiled_traffic(kname)
-    roofline = {"bound": "valu", "achieved": achieved, "peak": VALU_INT32_TOPS, "unit": "TOPS",
-                "frac": achieved / VALU_INT32_TOPS, "traffic": traffic,
-                "kernel": f"{kname} (popcount counts + events + routing)", "kernel_ms": pair_ms,
-                "ops_per_pair": ops_per_pair,
+    roofline = {"bound": "valu" if popc else "mfma", "achieved": achieved, "peak": peak, "unit": "TOPS",
+                "frac": achieved / peak, "traffic": traffic,
+                "kernel": f"{kname} ({'popcount' if popc else 'int8 MFMA'} counts + events + routing)",
+                "kernel_ms": pair_ms, "ops_per_pair": ops_per_pair,
                 "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/c1_popc_hbm_bytes.md",
                 "hbm_frac": None if traffic is None else traffic / (pair_ms * 1e-3) / HBM_BYTES_S}
     line = {
