@@ -1059,11 +1059,12 @@ __device__ __attribute__((always_inline)) inline void base_sums(const RunArgs &A
 
 // MARG: the base sums come from base_sums and tables whose entries facing a clean base subtract it (popcount
 // backend, every mode but SKIP, whose ambiguity sites are not visited).
+// One pair tile: rows A.r0 + by * TI .. + TI, columns (c0 rounded down to 64) + bx * TJ .. + TJ.
 template <int BACKEND, bool FAST, bool NP, bool MARG>
-__global__ __launch_bounds__(256) void k_pairs(RunArgs A)
+__device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &A, const int64_t bx, const int64_t by)
 {
   const int64_t c0a = A.c0 & ~(int64_t)63;  // column tiles start on a 64 boundary (whole 4-sequence groups)
-  const int64_t i0 = A.r0 + (int64_t)blockIdx.y * TI, j0 = c0a + (int64_t)blockIdx.x * TJ;
+  const int64_t i0 = A.r0 + by * TI, j0 = c0a + bx * TJ;
   if (A.triangle && j0 + TJ - 1 <= i0) return;  // no pair j > i in this tile
   constexpr bool MF = BACKEND == TN93G_BACKEND_MFMA;
   __shared__ unsigned lslow;
@@ -1382,7 +1383,7 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
     if constexpr (FAST) {
       const unsigned long long need = __ballot(state == 1), skp = __ballot(state == 2);
       if (i < A.r1) {
-        const size_t slot = (size_t)(i - A.r0) * (size_t)A.gx + blockIdx.x;
+        const size_t slot = (size_t)(i - A.r0) * (size_t)A.gx + (size_t)bx;
         if (lane == 0) A.chdr[slot] = ChunkHdr{need, skp};
         if (state == 1) {
           uint32_t *d = A.cdata + slot * (7 * 64) + lane;
@@ -1431,6 +1432,32 @@ __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
         dst[q++] = (unsigned long long)(ib + a) << 32 | (unsigned long long)j |
                    (smask >> (RPW + a) & 1u ? 1ull << 63 : 0ull);
   }
+}
+
+template <int BACKEND, bool FAST, bool NP, bool MARG>
+__global__ __launch_bounds__(256) void k_pairs(RunArgs A)
+{
+  pairs_tile<BACKEND, FAST, NP, MARG>(A, blockIdx.x, blockIdx.y);
+}
+
+// The pair tiles of several jobs of one kernel variant in one launch (a batch): block b is tile b - toff[k] of job k,
+// toff[k] <= b < toff[k + 1], so small alignments fill the device together instead of one after another.
+template <int BACKEND, bool FAST, bool NP, bool MARG>
+__global__ __launch_bounds__(256) void k_pairs_multi(const RunArgs *__restrict__ jobs, const int64_t *__restrict__ toff,
+                                                     int njobs)
+{
+  const int64_t b = blockIdx.x;
+  int lo = 0, hi = njobs;  // uniform over the block
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (toff[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const RunArgs &A = jobs[lo];
+  const int64_t t = b - toff[lo];
+  pairs_tile<BACKEND, FAST, NP, MARG>(A, t % A.gx, t / A.gx);
 }
 
 // The FP64 tail of the fast path.  A block takes whole rows; its 4 waves take the row's chunks (64 columns each) in
@@ -2379,6 +2406,8 @@ struct tn93g_ctx {
   std::vector<unsigned char> gkey;
   // autotuner: the fastest pair-kernel variant measured for each job shape (see tune_shape)
   std::map<uint64_t, int> tuned;
+  // a batch's merged pair-kernel launches: per launch the jobs' RunArgs, then their tile offsets
+  DevBuf multi;
 
   RunBufs *slot(size_t k)
   {
@@ -2737,11 +2766,58 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   return 0;
 }
 
-// The job's kernels on stream `st`, with no host synchronisation (a batch captures them into one graph).  The replay
-// and the tail kernel depend only on the pair kernel: the tail runs on the branch's side stream beside the replay
-// (both fold their results in with atomics), joined back before the link gather.  With `timing`, events around
-// the pair and tail kernels give the device times of a single run.
-static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int branch)
+// The pair-kernel variant of a job: 0-7 the popcount backend (bit 0 fast path, bit 1 dense-N planes, bit 2 marginal
+// form), 8 / 9 the MFMA backend (fast / exact path); -1 when the run has no pair.
+static int pairs_variant(const Job &J)
+{
+  const RunArgs &A = J.A;
+  if (J.nrows <= 0 || A.ncols <= 0) return -1;
+  const bool mf = J.P.backend == TN93G_BACKEND_MFMA && !A.nplane;  // dense-N counting is popcount only
+  if (mf) return J.fast ? 8 : 9;
+  const bool marg = J.B->tab_fold && !A.skip_events && J.sparse_gaps;
+  return (J.fast ? 1 : 0) | (A.nplane ? 2 : 0) | (marg ? 4 : 0);
+}
+
+static void launch_pairs(int v, dim3 g, hipStream_t st, const RunArgs &A)
+{
+  constexpr int PC = TN93G_BACKEND_POPC, MF = TN93G_BACKEND_MFMA;
+  switch (v) {
+    case 0: hipLaunchKernelGGL((k_pairs<PC, false, false, false>), g, dim3(256), 0, st, A); break;
+    case 1: hipLaunchKernelGGL((k_pairs<PC, true, false, false>), g, dim3(256), 0, st, A); break;
+    case 2: hipLaunchKernelGGL((k_pairs<PC, false, true, false>), g, dim3(256), 0, st, A); break;
+    case 3: hipLaunchKernelGGL((k_pairs<PC, true, true, false>), g, dim3(256), 0, st, A); break;
+    case 4: hipLaunchKernelGGL((k_pairs<PC, false, false, true>), g, dim3(256), 0, st, A); break;
+    case 5: hipLaunchKernelGGL((k_pairs<PC, true, false, true>), g, dim3(256), 0, st, A); break;
+    case 6: hipLaunchKernelGGL((k_pairs<PC, false, true, true>), g, dim3(256), 0, st, A); break;
+    case 7: hipLaunchKernelGGL((k_pairs<PC, true, true, true>), g, dim3(256), 0, st, A); break;
+    case 8: hipLaunchKernelGGL((k_pairs<MF, true, false, false>), g, dim3(256), 0, st, A); break;
+    case 9: hipLaunchKernelGGL((k_pairs<MF, false, false, false>), g, dim3(256), 0, st, A); break;
+    default: break;
+  }
+}
+
+// One launch over the tiles of several jobs of fast-path variant v (a batch); false for other variants.
+static bool launch_pairs_multi(int v, unsigned nblocks, hipStream_t st, const RunArgs *jobs, const int64_t *toff,
+                               int nj)
+{
+  constexpr int PC = TN93G_BACKEND_POPC, MF = TN93G_BACKEND_MFMA;
+  const dim3 g(nblocks), b(256);
+  switch (v) {
+    case 1: hipLaunchKernelGGL((k_pairs_multi<PC, true, false, false>), g, b, 0, st, jobs, toff, nj); return true;
+    case 3: hipLaunchKernelGGL((k_pairs_multi<PC, true, true, false>), g, b, 0, st, jobs, toff, nj); return true;
+    case 5: hipLaunchKernelGGL((k_pairs_multi<PC, true, false, true>), g, b, 0, st, jobs, toff, nj); return true;
+    case 7: hipLaunchKernelGGL((k_pairs_multi<PC, true, true, true>), g, b, 0, st, jobs, toff, nj); return true;
+    case 8: hipLaunchKernelGGL((k_pairs_multi<MF, true, false, false>), g, b, 0, st, jobs, toff, nj); return true;
+    default: return false;
+  }
+}
+
+// A job's kernels in three parts on stream `st`, with no host synchronisation (a batch captures them into one graph):
+// before the pair kernel (fills, special-sequence tables), the pair kernel, and after it.  The replay and the tail
+// kernel depend only on the pair kernel: the tail runs on the branch's side stream beside the replay (both fold
+// their results in with atomics), joined back before the link gather.  With `timing`, events around the pair and
+// tail kernels give the device times of a single run.
+static void job_enqueue_pre(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
 {
   RunArgs &A = J.A;
   RunBufs *B = J.B;
@@ -2757,35 +2833,21 @@ static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int b
   if (A.nsc)
     hipLaunchKernelGGL(k_spec16, dim3((unsigned)((nrows + TJ - 1) / TJ), (unsigned)((A.nsc + TI - 1) / TI)), dim3(256),
                        0, st, A, A.spec_rows + A.nsr, A.nsc, A.r0, nrows, const_cast<uint32_t *>(A.tab_col));
+}
+
+static void job_enqueue_pairs(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
+{
   if (timing) (void)hipEventRecord(c->ev[0], st);
-  if (nrows > 0 && A.ncols > 0) {
-    dim3 g((unsigned)J.gx, (unsigned)J.gy);
-    const bool mf = J.P.backend == TN93G_BACKEND_MFMA && !A.nplane;  // dense-N counting is popcount only
-    const bool marg = !mf && B->tab_fold && !A.skip_events && J.sparse_gaps;
-    const int v = (J.fast ? 1 : 0) | (A.nplane ? 2 : 0) | (marg ? 4 : 0);
-    constexpr int PC = TN93G_BACKEND_POPC, MF = TN93G_BACKEND_MFMA;
-    if (mf && J.fast)
-      hipLaunchKernelGGL((k_pairs<MF, true, false, false>), g, dim3(256), 0, st, A);
-    else if (mf)
-      hipLaunchKernelGGL((k_pairs<MF, false, false, false>), g, dim3(256), 0, st, A);
-    else if (v == 0)
-      hipLaunchKernelGGL((k_pairs<PC, false, false, false>), g, dim3(256), 0, st, A);
-    else if (v == 1)
-      hipLaunchKernelGGL((k_pairs<PC, true, false, false>), g, dim3(256), 0, st, A);
-    else if (v == 2)
-      hipLaunchKernelGGL((k_pairs<PC, false, true, false>), g, dim3(256), 0, st, A);
-    else if (v == 3)
-      hipLaunchKernelGGL((k_pairs<PC, true, true, false>), g, dim3(256), 0, st, A);
-    else if (v == 4)
-      hipLaunchKernelGGL((k_pairs<PC, false, false, true>), g, dim3(256), 0, st, A);
-    else if (v == 5)
-      hipLaunchKernelGGL((k_pairs<PC, true, false, true>), g, dim3(256), 0, st, A);
-    else if (v == 6)
-      hipLaunchKernelGGL((k_pairs<PC, false, true, true>), g, dim3(256), 0, st, A);
-    else
-      hipLaunchKernelGGL((k_pairs<PC, true, true, true>), g, dim3(256), 0, st, A);
-  }
+  const int v = pairs_variant(J);
+  if (v >= 0) launch_pairs(v, dim3((unsigned)J.gx, (unsigned)J.gy), st, J.A);
   if (timing) (void)hipEventRecord(c->ev[1], st);
+}
+
+static void job_enqueue_post(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int branch)
+{
+  RunArgs &A = J.A;
+  RunBufs *B = J.B;
+  const int64_t nrows = J.nrows;
   // the replay is enqueued first so that its long, latency-bound waves are resident before the tail's blocks fill
   // the machine; the tail runs on the side stream and is joined back before the link gather
   hipStream_t rs = c->rside[branch];
@@ -2810,6 +2872,13 @@ static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int b
                        A.links);
   }
   hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, st, A);
+}
+
+static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int branch)
+{
+  job_enqueue_pre(c, J, timing, st);
+  job_enqueue_pairs(c, J, timing, st);
+  job_enqueue_post(c, J, timing, st, branch);
 }
 
 // Device-to-host copy of the job's counters (the replay total among them) into `pin` (COUNTER_STRIDE bytes).
@@ -3172,6 +3241,7 @@ void tn93g_close(tn93g_ctx *c)
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->pin.p) (void)hipHostFree(c->pin.p);
   if (c->pin_links.p) (void)hipHostFree(c->pin_links.p);
+  if (c->multi.p) (void)hipFree(c->multi.p);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -3587,8 +3657,60 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
     if (rc) return fail_all(rc);
   }
   if (m) {
+    // Pair kernels of jobs with the same fast-path variant run as one launch over all their tiles (small alignments
+    // fill the device together): per group its RunArgs and tile offsets, in c->multi, uploaded before the graph.
+    struct Group {
+      int v;
+      std::vector<size_t> jobs;
+      size_t off_args, off_toff;
+      int64_t tiles;
+    };
+    std::vector<Group> groups;
+    std::vector<char> merged(m, 0);
+    {
+      std::map<int, std::vector<size_t>> byv;
+      for (size_t t = 0; t < m; t++) {
+        const int v = pairs_variant(jobs[t]);
+        if (v == 1 || v == 3 || v == 5 || v == 7 || v == 8) byv[v].push_back(t);
+      }
+      size_t bytes = 0;
+      for (auto &kv : byv) {
+        if (kv.second.size() < 2) continue;
+        Group G;
+        G.v = kv.first;
+        G.jobs = kv.second;
+        G.off_args = bytes;
+        bytes = (bytes + G.jobs.size() * sizeof(RunArgs) + 255) & ~(size_t)255;
+        G.off_toff = bytes;
+        bytes = (bytes + (G.jobs.size() + 1) * 8 + 255) & ~(size_t)255;
+        G.tiles = 0;
+        for (size_t t : G.jobs) G.tiles += jobs[t].gx * jobs[t].gy;
+        if (G.tiles >= (1ll << 31)) continue;  // beyond one grid: these jobs launch one by one
+        for (size_t t : G.jobs) merged[t] = 1;
+        groups.push_back(G);
+      }
+      if (bytes && ensure(c, c->multi, bytes)) return fail_all(TN93G_E_NOMEM);
+      std::vector<unsigned char> host(bytes, 0);
+      for (const Group &G : groups) {
+        RunArgs *ra = (RunArgs *)(host.data() + G.off_args);
+        int64_t *to = (int64_t *)(host.data() + G.off_toff);
+        int64_t acc = 0;
+        for (size_t q = 0; q < G.jobs.size(); q++) {
+          ra[q] = jobs[G.jobs[q]].A;
+          to[q] = acc;
+          acc += jobs[G.jobs[q]].gx * jobs[G.jobs[q]].gy;
+        }
+        to[G.jobs.size()] = acc;
+      }
+      if (bytes) HIPCHK(c, hipMemcpyAsync(c->multi.p, host.data(), bytes, hipMemcpyHostToDevice, c->stream));
+    }
     // the batch's work as one graph; a batch that enqueues exactly the same launches relaunches the last graph
     std::vector<unsigned char> key;
+    {
+      const void *mp = c->multi.p;
+      const unsigned char *a = (const unsigned char *)&mp;
+      key.insert(key.end(), a, a + sizeof mp);
+    }
     for (const Job &J : jobs) {
       const unsigned char *a = (const unsigned char *)&J.A;
       key.insert(key.end(), a, a + sizeof(RunArgs));
@@ -3608,17 +3730,52 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       auto cost = [&](const Job &J) { return (double)std::max<int64_t>(J.npairs, 1) * (double)(J.A.W + 8); };
       std::sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost(jobs[x]) > cost(jobs[y]); });
       double load[NBRANCH] = {};
-      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      HIPCHK(c, hipEventRecord(c->fork, c->stream));
-      for (int b = 0; b < NBRANCH; b++) HIPCHK(c, hipStreamWaitEvent(c->side[b], c->fork, 0));
+      std::vector<int> branch(m, 0);
       for (size_t t : order) {
         const int b = (int)(std::min_element(load, load + NBRANCH) - load);
         load[b] += cost(jobs[t]);
-        job_enqueue(c, jobs[t], false, c->side[b], b);
+        branch[t] = b;
       }
-      for (int b = 0; b < NBRANCH; b++) {
-        HIPCHK(c, hipEventRecord(c->join[b], c->side[b]));
-        HIPCHK(c, hipStreamWaitEvent(c->stream, c->join[b], 0));
+      auto fork = [&]() -> int {
+        HIPCHK(c, hipEventRecord(c->fork, c->stream));
+        for (int b = 0; b < NBRANCH; b++) HIPCHK(c, hipStreamWaitEvent(c->side[b], c->fork, 0));
+        return 0;
+      };
+      auto join = [&]() -> int {
+        for (int b = 0; b < NBRANCH; b++) {
+          HIPCHK(c, hipEventRecord(c->join[b], c->side[b]));
+          HIPCHK(c, hipStreamWaitEvent(c->stream, c->join[b], 0));
+        }
+        return 0;
+      };
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      // 1: each job's fills and tables on its branch; 2: the merged pair kernels on the main stream, the other
+      // jobs' whole sequence on their branches; 3: the merged jobs' replay, tail and link kernels on their branches
+      int rc = fork();
+      for (size_t t : order) {
+        if (rc) break;
+        if (merged[t])
+          job_enqueue_pre(c, jobs[t], false, c->side[branch[t]]);
+        else
+          job_enqueue(c, jobs[t], false, c->side[branch[t]], branch[t]);
+      }
+      if (!rc && !groups.empty()) {
+        rc = join();
+        for (const Group &G : groups)
+          if (!rc)
+            (void)launch_pairs_multi(G.v, (unsigned)G.tiles, c->stream,
+                                     (const RunArgs *)((char *)c->multi.p + G.off_args),
+                                     (const int64_t *)((char *)c->multi.p + G.off_toff), (int)G.jobs.size());
+        if (!rc) rc = fork();
+        for (size_t t : order)
+          if (!rc && merged[t]) job_enqueue_post(c, jobs[t], false, c->side[branch[t]], branch[t]);
+      }
+      if (!rc) rc = join();
+      if (rc) {
+        hipGraph_t dropped = nullptr;
+        (void)hipStreamEndCapture(c->stream, &dropped);
+        if (dropped) (void)hipGraphDestroy(dropped);
+        return fail_all(rc);
       }
       const hipError_t e1 = hipStreamEndCapture(c->stream, &g);
       if (e1 != hipSuccess)
