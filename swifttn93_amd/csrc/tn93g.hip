@@ -2139,6 +2139,21 @@ __global__ void k_fill(uint32_t *__restrict__ p, uint32_t v, int64_t n)
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) p[k] = v;
 }
 
+// Zeroes a list of ranges in one launch (a batch's per-job buffers): grid row y takes ranges y, y + gridDim.y, ...
+struct FillRange {
+  uint32_t *p;
+  int64_t n;  // 32-bit words
+};
+__global__ void k_fill_ranges(const FillRange *__restrict__ r, int nr)
+{
+  for (int k = blockIdx.y; k < nr; k += gridDim.y) {
+    uint32_t *p = r[k].p;
+    const int64_t n = r[k].n;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+      p[q] = 0u;
+  }
+}
+
 __global__ void k_hist_sum(RunArgs A)
 {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2817,16 +2832,27 @@ static bool launch_pairs_multi(int v, unsigned nblocks, hipStream_t st, const Ru
 // kernel depend only on the pair kernel: the tail runs on the branch's side stream beside the replay (both fold
 // their results in with atomics), joined back before the link gather.  With `timing`, events around the pair and
 // tail kernels give the device times of a single run.
-static void job_enqueue_pre(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
+// The buffers a run starts from zero (histogram replicas, counters, per-row link counts, row accumulators).
+static void job_fill_ranges(const Job &J, std::vector<FillRange> &out)
+{
+  RunBufs *B = J.B;
+  out.push_back(FillRange{(uint32_t *)B->hist.p, (int64_t)NSTRIPE * 600 * 2});
+  out.push_back(FillRange{(uint32_t *)B->counters.p, (int64_t)(STRIPE_BYTES / 4)});
+  out.push_back(FillRange{(uint32_t *)B->row_links.p, (int64_t)J.nr1});
+  if (J.fast) out.push_back(FillRange{(uint32_t *)B->rowout.p, (int64_t)(J.nr1 * sizeof(RowAcc) / 4)});
+}
+
+// fills: false when a batch clears every job's buffers with one k_fill_ranges launch
+static void job_enqueue_pre(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, bool fills = true)
 {
   RunArgs &A = J.A;
-  RunBufs *B = J.B;
   const int64_t nrows = J.nrows;
   if (timing) (void)hipEventRecord(c->ev[4], st);
-  fill_zero(st, B->hist.p, (size_t)NSTRIPE * 600 * 8);
-  fill_zero(st, B->counters.p, STRIPE_BYTES);
-  fill_zero(st, B->row_links.p, J.nr1 * 4);
-  if (J.fast) fill_zero(st, B->rowout.p, J.nr1 * sizeof(RowAcc));
+  if (fills) {
+    std::vector<FillRange> fr;
+    job_fill_ranges(J, fr);
+    for (const FillRange &f : fr) fill_zero(st, f.p, (size_t)f.n * 4);
+  }
   if (A.nsr)
     hipLaunchKernelGGL(k_spec16, dim3((unsigned)((A.ncols + TJ - 1) / TJ), (unsigned)((A.nsr + TI - 1) / TI)),
                        dim3(256), 0, st, A, A.spec_rows, A.nsr, A.c0, A.ncols, const_cast<uint32_t *>(A.tab_row));
@@ -2874,9 +2900,9 @@ static void job_enqueue_post(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, 
   hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, st, A);
 }
 
-static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int branch)
+static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int branch, bool fills = true)
 {
-  job_enqueue_pre(c, J, timing, st);
+  job_enqueue_pre(c, J, timing, st, fills);
   job_enqueue_pairs(c, J, timing, st);
   job_enqueue_post(c, J, timing, st, branch);
 }
@@ -3667,6 +3693,8 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
     };
     std::vector<Group> groups;
     std::vector<char> merged(m, 0);
+    std::vector<FillRange> fills;
+    size_t off_fills = 0;
     {
       std::map<int, std::vector<size_t>> byv;
       for (size_t t = 0; t < m; t++) {
@@ -3689,8 +3717,13 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
         for (size_t t : G.jobs) merged[t] = 1;
         groups.push_back(G);
       }
+      // every job's zero-started buffers, cleared by one launch before the branches fork
+      for (size_t t = 0; t < m; t++) job_fill_ranges(jobs[t], fills);
+      off_fills = bytes;
+      bytes = (bytes + fills.size() * sizeof(FillRange) + 255) & ~(size_t)255;
       if (bytes && ensure(c, c->multi, bytes)) return fail_all(TN93G_E_NOMEM);
       std::vector<unsigned char> host(bytes, 0);
+      memcpy(host.data() + off_fills, fills.data(), fills.size() * sizeof(FillRange));
       for (const Group &G : groups) {
         RunArgs *ra = (RunArgs *)(host.data() + G.off_args);
         int64_t *to = (int64_t *)(host.data() + G.off_toff);
@@ -3751,13 +3784,15 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       // 1: each job's fills and tables on its branch; 2: the merged pair kernels on the main stream, the other
       // jobs' whole sequence on their branches; 3: the merged jobs' replay, tail and link kernels on their branches
+      hipLaunchKernelGGL(k_fill_ranges, dim3(64, (unsigned)std::min<size_t>(fills.size(), 65535)), dim3(256), 0,
+                         c->stream, (const FillRange *)((char *)c->multi.p + off_fills), (int)fills.size());
       int rc = fork();
       for (size_t t : order) {
         if (rc) break;
         if (merged[t])
-          job_enqueue_pre(c, jobs[t], false, c->side[branch[t]]);
+          job_enqueue_pre(c, jobs[t], false, c->side[branch[t]], false);
         else
-          job_enqueue(c, jobs[t], false, c->side[branch[t]], branch[t]);
+          job_enqueue(c, jobs[t], false, c->side[branch[t]], branch[t], false);
       }
       if (!rc && !groups.empty()) {
         rc = join();
