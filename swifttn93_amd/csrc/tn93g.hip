@@ -2144,6 +2144,23 @@ struct FillRange {
   uint32_t *p;
   int64_t n;  // 32-bit words
 };
+// Copies a list of ranges (8-byte units) in one launch: a batch's read-back staged contiguously for one transfer.
+struct CopyRange {
+  const uint2 *src;
+  uint2 *dst;
+  int64_t n;
+};
+__global__ void k_copy_ranges(const CopyRange *__restrict__ r, int nr)
+{
+  for (int k = blockIdx.y; k < nr; k += gridDim.y) {
+    const uint2 *src = r[k].src;
+    uint2 *dst = r[k].dst;
+    const int64_t n = r[k].n;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+      dst[q] = src[q];
+  }
+}
+
 __global__ void k_fill_ranges(const FillRange *__restrict__ r, int nr)
 {
   for (int k = blockIdx.y; k < nr; k += gridDim.y) {
@@ -2421,8 +2438,9 @@ struct tn93g_ctx {
   std::vector<unsigned char> gkey;
   // autotuner: the fastest pair-kernel variant measured for each job shape (see tune_shape)
   std::map<uint64_t, int> tuned;
-  // a batch's merged pair-kernel launches: per launch the jobs' RunArgs, then their tile offsets
-  DevBuf multi;
+  // a batch's merged pair-kernel launches: per launch the jobs' RunArgs, then their tile offsets; the fill and copy
+  // range lists; and the staging of the batch's read-back
+  DevBuf multi, stage;
 
   RunBufs *slot(size_t k)
   {
@@ -3268,6 +3286,7 @@ void tn93g_close(tn93g_ctx *c)
   if (c->pin.p) (void)hipHostFree(c->pin.p);
   if (c->pin_links.p) (void)hipHostFree(c->pin_links.p);
   if (c->multi.p) (void)hipFree(c->multi.p);
+  if (c->stage.p) (void)hipFree(c->stage.p);
   (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -3694,7 +3713,9 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
     std::vector<Group> groups;
     std::vector<char> merged(m, 0);
     std::vector<FillRange> fills;
-    size_t off_fills = 0;
+    std::vector<CopyRange> copies;
+    std::vector<size_t> spec(m, 0), soff(m, 0);
+    size_t off_fills = 0, off_copies = 0, stage_bytes = 0;
     {
       std::map<int, std::vector<size_t>> byv;
       for (size_t t = 0; t < m; t++) {
@@ -3721,9 +3742,33 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       for (size_t t = 0; t < m; t++) job_fill_ranges(jobs[t], fills);
       off_fills = bytes;
       bytes = (bytes + fills.size() * sizeof(FillRange) + 255) & ~(size_t)255;
+      // the read-back of every job (counters, row accumulators, histogram replicas summed, the first spec[t] links)
+      // staged in one device buffer by one launch at the end of the graph, then one transfer
+      for (size_t t = 0; t < m; t++) {
+        const Job &J = jobs[t];
+        spec[t] = std::min<size_t>(J.B->link_spec, (size_t)J.link_cap * NSTRIPE);
+        soff[t] = stage_bytes;
+        stage_bytes += COUNTER_STRIDE + fast_layout(J, spec[t]).bytes;
+      }
+      if (ensure(c, c->stage, stage_bytes)) return fail_all(TN93G_E_NOMEM);
+      for (size_t t = 0; t < m; t++) {
+        const Job &J = jobs[t];
+        char *d = (char *)c->stage.p + soff[t];
+        const FastLayout F = fast_layout(J, spec[t]);
+        copies.push_back(CopyRange{(const uint2 *)J.B->counters.p, (uint2 *)d, (int64_t)(STRIPE_BYTES / 8)});
+        copies.push_back(CopyRange{(const uint2 *)J.B->rowout.p, (uint2 *)(d + COUNTER_STRIDE + F.off_ra),
+                                   (int64_t)(J.nr1 * sizeof(RowAcc) / 8)});
+        copies.push_back(CopyRange{(const uint2 *)J.B->hist.p, (uint2 *)(d + COUNTER_STRIDE + F.off_h), 600});
+        if (spec[t])
+          copies.push_back(CopyRange{(const uint2 *)J.B->links.p, (uint2 *)(d + COUNTER_STRIDE + F.off_lk),
+                                     (int64_t)(spec[t] * sizeof(LinkRec) / 8)});
+      }
+      off_copies = bytes;
+      bytes = (bytes + copies.size() * sizeof(CopyRange) + 255) & ~(size_t)255;
       if (bytes && ensure(c, c->multi, bytes)) return fail_all(TN93G_E_NOMEM);
       std::vector<unsigned char> host(bytes, 0);
       memcpy(host.data() + off_fills, fills.data(), fills.size() * sizeof(FillRange));
+      memcpy(host.data() + off_copies, copies.data(), copies.size() * sizeof(CopyRange));
       for (const Group &G : groups) {
         RunArgs *ra = (RunArgs *)(host.data() + G.off_args);
         int64_t *to = (int64_t *)(host.data() + G.off_toff);
@@ -3806,6 +3851,9 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
           if (!rc && merged[t]) job_enqueue_post(c, jobs[t], false, c->side[branch[t]], branch[t]);
       }
       if (!rc) rc = join();
+      if (!rc)
+        hipLaunchKernelGGL(k_copy_ranges, dim3(64, (unsigned)std::min<size_t>(copies.size(), 65535)), dim3(256), 0,
+                           c->stream, (const CopyRange *)((char *)c->multi.p + off_copies), (int)copies.size());
       if (rc) {
         hipGraph_t dropped = nullptr;
         (void)hipStreamEndCapture(c->stream, &dropped);
@@ -3826,42 +3874,41 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
     HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-    if (ensure_pinned(c, c->pin, COUNTER_STRIDE * m)) return fail_all(TN93G_E_NOMEM);
-    for (size_t t = 0; t < m; t++) {
-      int rc = job_enqueue_counters(c, jobs[t], (char *)c->pin.p + t * COUNTER_STRIDE);
-      if (rc) return fail_all(rc);
-    }
+    // one transfer of the staged read-back of every job
+    if (ensure_pinned(c, c->pin, std::max<size_t>(stage_bytes, 1))) return fail_all(TN93G_E_NOMEM);
+    HIPCHK(c, hipMemcpyAsync(c->pin.p, c->stage.p, stage_bytes, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[2]);
     std::vector<char> redo(m, 0);
-    std::vector<size_t> off(m, 0);
-    size_t total = 0;
+    // jobs with more links than staged: all their links again, each into a block of its own, in one more round
+    std::vector<std::vector<LinkRec>> more(m);
+    bool any_more = false;
     for (size_t t = 0; t < m; t++) {
       Job &J = jobs[t];
-      job_parse_counters(J, (const char *)c->pin.p + t * COUNTER_STRIDE);
+      job_parse_counters(J, (const char *)c->pin.p + soff[t]);
       if ((J.n_undecided && !J.A.exact) || (J.fast && J.max_region > J.link_cap)) {
         if (J.fast && J.max_region > J.link_cap) J.B->link_cap_hint = (int64_t)J.max_region;
         redo[t] = 1;
         continue;
       }
-      off[t] = total;
-      total += fast_layout(J, J.n_linkrec).bytes;
-    }
-    if (ensure_pinned(c, c->pin, std::max<size_t>(total, 1))) return fail_all(TN93G_E_NOMEM);
-    for (size_t t = 0; t < m; t++)
-      if (!redo[t]) {
-        int rc = job_enqueue_fast_readback(c, jobs[t], (char *)c->pin.p + off[t], jobs[t].n_linkrec);
-        if (rc) return fail_all(rc);
+      J.B->link_spec = std::max<size_t>(4096, J.n_linkrec + J.n_linkrec / 4);
+      if (J.n_linkrec > spec[t]) {
+        more[t].resize(J.n_linkrec);
+        HIPCHK(c, hipMemcpyAsync(more[t].data(), J.B->links.p, J.n_linkrec * sizeof(LinkRec), hipMemcpyDeviceToHost,
+                                 c->stream));
+        any_more = true;
       }
-    HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    if (any_more) HIPCHK(c, hipStreamSynchronize(c->stream));
     for (size_t t = 0; t < m; t++) {
       if (redo[t]) continue;
       tn93g_result *R = (tn93g_result *)calloc(1, sizeof(tn93g_result));
       if (!R) return fail_all(TN93G_E_NOMEM);
       out[idx[t]] = R;
-      const char *pj = (const char *)c->pin.p + off[t];
-      int rc = job_fast_result(jobs[t], pj, (const LinkRec *)(pj + fast_layout(jobs[t], 0).off_lk), R);
+      const char *pj = (const char *)c->pin.p + soff[t] + COUNTER_STRIDE;
+      const LinkRec *lk = more[t].empty() ? (const LinkRec *)(pj + fast_layout(jobs[t], 0).off_lk) : more[t].data();
+      int rc = job_fast_result(jobs[t], pj, lk, R);
       if (rc) return fail_all(rc);
       R->gpu_ms = ms;  // device time of the whole batch
     }
