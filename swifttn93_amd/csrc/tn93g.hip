@@ -25,6 +25,9 @@
 //   k_scan    per-row link offsets;  k_link_gather  links into one array;  k_link_order  links in tn93's row and
 //             column order;  k_hist_sum  histogram replicas summed
 //   k_patch   host-exact distances into D;  k_rows  per-row sums in column order (exact path only)
+// A captured batch runs k_pairs, k_tail, k_scan, k_link_gather, k_link_order and k_hist_sum of the jobs that share
+// a pair-kernel variant as one launch each (the *_multi kernels; each block finds its job from a table of offsets),
+// and clears and reads back every job's buffers with one k_fill_ranges and one k_copy_ranges launch.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -299,7 +302,11 @@ struct RunArgs {
   unsigned int *stripes;       // [NSTRIPE][32]: counters spread over cache lines (ST_* fields), summed on the host
   int fast;                    // 1: row reductions in RowAcc, links unordered in LinkRec; 0: D/U + k_rows
   RowAcc *racc;                // [nrows]
-  LinkRec *links;              // [NSTRIPE][link_cap]: region s holds the links counted in stripe s
+  LinkRec *links;              // [NSTRIPE][link_cap]: region s holds the links counted in stripe s; after the
+                               // ordering, the run's links in tn93's order
+  LinkRec *links_flat;         // the regions concatenated (k_link_gather)
+  int64_t *link_off;           // [nrows + 1] exclusive scan of the per-row link counts
+  uint32_t *lslot;             // [links] index into links_flat of each link, grouped by row
   unsigned int link_cap;       // per region
   // fast path: the pair kernel hands each wave-row (row i, 64 columns of a tile) to k_tail in a fixed slot
   // (i - r0) * gx + column tile: `need` lanes carry the tail inputs, `skip` lanes had too small an overlap
@@ -1463,7 +1470,7 @@ __global__ __launch_bounds__(256) void k_pairs_multi(const RunArgs *__restrict__
 // The FP64 tail of the fast path.  A block takes whole rows; its 4 waves take the row's chunks (64 columns each) in
 // turn, at high occupancy so the long dependent chains of divisions and logarithms of many pairs overlap.  Each lane
 // keeps its share of the row's reductions over the chunks; the block combines them into one set of atomics per row.
-__global__ __launch_bounds__(256) void k_tail(RunArgs A)
+__device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A, int64_t rbeg, int64_t rstep)
 {
   __shared__ unsigned long long lhist[3 * 200];
   __shared__ BlockStats bs;
@@ -1474,7 +1481,7 @@ __global__ __launch_bounds__(256) void k_tail(RunArgs A)
   __syncthreads();
   const int64_t nrows = A.r1 - A.r0;
   const int64_t c0a = A.c0 & ~(int64_t)63;
-  for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+  for (int64_t r = rbeg; r < nrows; r += rstep) {
     const int64_t i = A.r0 + r;
     RowPart rp = rowpart_zero();
     // the wave's chunks of the row, bx = wave, wave + 4, ...: the next chunk's header and (16-bit packed) inputs are
@@ -1562,6 +1569,31 @@ __global__ __launch_bounds__(256) void k_tail(RunArgs A)
     __syncthreads();
   }
   flush_block(A, &bs, lhist, tid, 256);
+}
+
+__global__ __launch_bounds__(256) void k_tail(RunArgs A) { tail_rows(A, blockIdx.x, gridDim.x); }
+
+// The block of flat index b in a list of jobs: the largest k with off[k] <= b (uniform over the block).
+__device__ inline int job_of(const int64_t *__restrict__ off, int nj, int64_t b)
+{
+  int lo = 0, hi = nj;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// The tails of several jobs (a batch) in one launch: job k owns blocks [boff[k], boff[k + 1]), which stride over
+// its rows as k_tail's grid does.
+__global__ __launch_bounds__(256) void k_tail_multi(const RunArgs *__restrict__ jobs, const int64_t *__restrict__ boff,
+                                                    int nj)
+{
+  const int k = job_of(boff, nj, blockIdx.x);
+  tail_rows(jobs[k], (int64_t)blockIdx.x - boff[k], boff[k + 1] - boff[k]);
 }
 
 // ------------------------------------------------------------------------------------------ replay kernel
@@ -2032,10 +2064,10 @@ __global__ __launch_bounds__(RB, DENSE ? 3 : 4) void k_replay(RunArgs A)
 }
 
 // the links of the NSTRIPE regions, concatenated into `out` (one block per region)
-__global__ void k_link_gather(RunArgs A, LinkRec *__restrict__ out, const int64_t *__restrict__ row_off,
-                              uint32_t *__restrict__ slot)
+__device__ __attribute__((always_inline)) inline void link_gather(const RunArgs &A, LinkRec *__restrict__ out,
+                                                                  const int64_t *__restrict__ row_off,
+                                                                  uint32_t *__restrict__ slot, const int st)
 {
-  const int st = blockIdx.x;
   __shared__ int64_t soff;
   if (threadIdx.x < 64) {  // the links of the stripes before st, summed across one wave (NSTRIPE = 64)
     int64_t v = (int)threadIdx.x < st ? (int64_t)min(A.stripes[threadIdx.x * 32 + ST_LINKS], A.link_cap) : 0;
@@ -2055,15 +2087,25 @@ __global__ void k_link_gather(RunArgs A, LinkRec *__restrict__ out, const int64_
   }
 }
 
+__global__ void k_link_gather(RunArgs A) { link_gather(A, A.links_flat, A.link_off, A.lslot, blockIdx.x); }
+
+// the same for several jobs: block (st, k) is stripe st of job k
+__global__ void k_link_gather_multi(const RunArgs *__restrict__ jobs)
+{
+  const RunArgs &A = jobs[blockIdx.y];
+  link_gather(A, A.links_flat, A.link_off, A.lslot, blockIdx.x);
+}
+
 // The links in tn93's emission order (rows ascending, then columns ascending), into out.  Row r's links are
 // slot[row_off[r] .. row_off[r + 1]) (indices into flat); their rank within the row is their column's rank.  Up to 64
 // links: ranks by comparison across one wave.  More: a bitmap of the row's link columns, BMW words per window of
 // columns, and a prefix count of its words.  Columns of a row are distinct (one record per pair).  A run whose link
 // regions overflowed is repeated, so indices are checked against the capacity only to stay in bounds.
 constexpr int BMW = 4096;
-__global__ __launch_bounds__(256) void k_link_order(RunArgs A, const LinkRec *__restrict__ flat,
-                                                    const int64_t *__restrict__ row_off,
-                                                    const uint32_t *__restrict__ slot, LinkRec *__restrict__ out)
+__device__ __attribute__((always_inline)) inline void link_order(const RunArgs &A, const LinkRec *__restrict__ flat,
+                                                                 const int64_t *__restrict__ row_off,
+                                                                 const uint32_t *__restrict__ slot,
+                                                                 LinkRec *__restrict__ out, int64_t rbeg, int64_t rstep)
 {
   __shared__ uint32_t bits[BMW];
   __shared__ uint32_t pre[BMW];
@@ -2071,7 +2113,7 @@ __global__ __launch_bounds__(256) void k_link_order(RunArgs A, const LinkRec *__
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int64_t cap = (int64_t)A.link_cap * NSTRIPE;
   const int64_t nrows = A.r1 - A.r0;
-  for (int64_t r = blockIdx.x; r < nrows; r += gridDim.x) {
+  for (int64_t r = rbeg; r < nrows; r += rstep) {
     const int64_t b = row_off[r], e = min(row_off[r + 1], cap);
     const int64_t cnt = e - b;
     if (cnt <= 0) continue;
@@ -2133,6 +2175,20 @@ __global__ __launch_bounds__(256) void k_link_order(RunArgs A, const LinkRec *__
   }
 }
 
+__global__ __launch_bounds__(256) void k_link_order(RunArgs A)
+{
+  link_order(A, A.links_flat, A.link_off, A.lslot, A.links, blockIdx.x, gridDim.x);
+}
+
+// the same for several jobs: job k owns blocks [boff[k], boff[k + 1])
+__global__ __launch_bounds__(256) void k_link_order_multi(const RunArgs *__restrict__ jobs,
+                                                          const int64_t *__restrict__ boff, int nj)
+{
+  const int k = job_of(boff, nj, blockIdx.x);
+  const RunArgs &A = jobs[k];
+  link_order(A, A.links_flat, A.link_off, A.lslot, A.links, (int64_t)blockIdx.x - boff[k], boff[k + 1] - boff[k]);
+}
+
 // histogram replicas summed into replica 0
 __global__ void k_fill(uint32_t *__restrict__ p, uint32_t v, int64_t n)
 {
@@ -2171,13 +2227,18 @@ __global__ void k_fill_ranges(const FillRange *__restrict__ r, int nr)
   }
 }
 
-__global__ void k_hist_sum(RunArgs A)
+__device__ inline void hist_sum(const RunArgs &A, int k)
 {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= 600) return;
   unsigned long long v = 0;
   for (int st = 0; st < NSTRIPE; st++) v += A.hist[st * 600 + k];
   A.hist[k] = v;
+}
+__global__ void k_hist_sum(RunArgs A) { hist_sum(A, blockIdx.x * blockDim.x + threadIdx.x); }
+// the same for several jobs: grid row y is job y
+__global__ void k_hist_sum_multi(const RunArgs *__restrict__ jobs)
+{
+  hist_sum(jobs[blockIdx.y], blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // host-exact distances back into D (after TN93G_FLAG_EXACT)
@@ -2197,7 +2258,8 @@ __global__ void k_patch(RunArgs A, const PatchRec *__restrict__ P, int64_t np)
 // Exclusive prefix sum of n int32 values (the per-row link counts) into out[0..n), total in out[n]; one block of
 // 1024 threads walks the array in chunks of 4096 (4 consecutive values per thread, wave scans, then a scan of the 16
 // wave totals).
-__global__ __launch_bounds__(1024) void k_scan(const int32_t *__restrict__ in, int64_t *__restrict__ out, int64_t n)
+__device__ __attribute__((always_inline)) inline void scan_rows(const int32_t *__restrict__ in,
+                                                                int64_t *__restrict__ out, int64_t n)
 {
   __shared__ int64_t wsum[16];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -2236,6 +2298,16 @@ __global__ __launch_bounds__(1024) void k_scan(const int32_t *__restrict__ in, i
     __syncthreads();
   }
   if (t == 0) out[n] = carry;
+}
+__global__ __launch_bounds__(1024) void k_scan(const int32_t *__restrict__ in, int64_t *__restrict__ out, int64_t n)
+{
+  scan_rows(in, out, n);
+}
+// the link offsets of several jobs (fast path): block k scans job k's row counts
+__global__ __launch_bounds__(1024) void k_scan_multi(const RunArgs *__restrict__ jobs)
+{
+  const RunArgs &A = jobs[blockIdx.x];
+  scan_rows(A.row_links, A.link_off, A.r1 - A.r0);
 }
 
 // ------------------------------------------------------------------------------------------ row kernel
@@ -2746,6 +2818,9 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   A.exact_cap = J.exact_cap;
   A.racc = (RowAcc *)B->rowout.p;
   A.links = (LinkRec *)B->links.p;
+  A.links_flat = (LinkRec *)B->links_flat.p;
+  A.link_off = (int64_t *)B->link_off.p;
+  A.lslot = (uint32_t *)B->lslot.p;
   A.link_cap = J.link_cap;
   A.chdr = (ChunkHdr *)B->chdr.p;
   A.cdata = (uint32_t *)B->cdata.p;
@@ -2887,6 +2962,21 @@ static void job_enqueue_pairs(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
   if (timing) (void)hipEventRecord(c->ev[1], st);
 }
 
+// grid sizes of a job's k_tail and k_link_order (the kernels stride over the rows)
+static unsigned tail_blocks(const Job &J)
+{
+  return J.fast && J.nrows > 0 && J.A.ncols > 0 ? (unsigned)std::min<int64_t>(J.nrows, 65535) : 0u;
+}
+static unsigned order_blocks(const Job &J) { return (unsigned)std::min<int64_t>(std::max<int64_t>(J.nrows, 1), 65535); }
+
+static void job_enqueue_replay(const Job &J, hipStream_t st)
+{
+  if (J.A.nplane)
+    hipLaunchKernelGGL(k_replay<true>, dim3(2048), dim3(RB), 0, st, J.A);
+  else
+    hipLaunchKernelGGL(k_replay<false>, dim3(2048), dim3(RB), 0, st, J.A);
+}
+
 static void job_enqueue_post(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int branch)
 {
   RunArgs &A = J.A;
@@ -2897,23 +2987,15 @@ static void job_enqueue_post(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, 
   hipStream_t rs = c->rside[branch];
   (void)hipEventRecord(c->rfork[branch], st);
   (void)hipStreamWaitEvent(rs, c->rfork[branch], 0);
-  if (A.nplane)
-    hipLaunchKernelGGL(k_replay<true>, dim3(2048), dim3(RB), 0, st, A);
-  else
-    hipLaunchKernelGGL(k_replay<false>, dim3(2048), dim3(RB), 0, st, A);
-  if (J.fast && nrows > 0 && A.ncols > 0)
-    hipLaunchKernelGGL(k_tail, dim3((unsigned)std::min<int64_t>(nrows, 65535)), dim3(256), 0, rs, A);
+  job_enqueue_replay(J, st);
+  if (tail_blocks(J)) hipLaunchKernelGGL(k_tail, dim3(tail_blocks(J)), dim3(256), 0, rs, A);
   if (timing) (void)hipEventRecord(c->ev[3], rs);
   (void)hipEventRecord(c->rjoin[branch], rs);
   (void)hipStreamWaitEvent(st, c->rjoin[branch], 0);
   if (J.fast) {
-    int64_t *roff = (int64_t *)B->link_off.p;
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const int32_t *)B->row_links.p, roff, nrows);
-    hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, st, A, (LinkRec *)B->links_flat.p,
-                       (const int64_t *)roff, (uint32_t *)B->lslot.p);
-    hipLaunchKernelGGL(k_link_order, dim3((unsigned)std::min<int64_t>(std::max<int64_t>(nrows, 1), 65535)), dim3(256),
-                       0, st, A, (const LinkRec *)B->links_flat.p, (const int64_t *)roff, (const uint32_t *)B->lslot.p,
-                       A.links);
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const int32_t *)B->row_links.p, A.link_off, nrows);
+    hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_link_order, dim3(order_blocks(J)), dim3(256), 0, st, A);
   }
   hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, st, A);
 }
@@ -3715,7 +3797,9 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
     std::vector<FillRange> fills;
     std::vector<CopyRange> copies;
     std::vector<size_t> spec(m, 0), soff(m, 0);
-    size_t off_fills = 0, off_copies = 0, stage_bytes = 0;
+    std::vector<size_t> post;  // the merged jobs, whose tails and link kernels also run as one launch each
+    size_t off_fills = 0, off_copies = 0, stage_bytes = 0, off_pargs = 0, off_tb = 0, off_ob = 0;
+    int64_t tail_total = 0, order_total = 0;
     {
       std::map<int, std::vector<size_t>> byv;
       for (size_t t = 0; t < m; t++) {
@@ -3738,6 +3822,14 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
         for (size_t t : G.jobs) merged[t] = 1;
         groups.push_back(G);
       }
+      for (size_t t = 0; t < m; t++)
+        if (merged[t]) post.push_back(t);
+      off_pargs = bytes;
+      bytes = (bytes + post.size() * sizeof(RunArgs) + 255) & ~(size_t)255;
+      off_tb = bytes;
+      bytes = (bytes + (post.size() + 1) * 8 + 255) & ~(size_t)255;
+      off_ob = bytes;
+      bytes = (bytes + (post.size() + 1) * 8 + 255) & ~(size_t)255;
       // every job's zero-started buffers, cleared by one launch before the branches fork
       for (size_t t = 0; t < m; t++) job_fill_ranges(jobs[t], fills);
       off_fills = bytes;
@@ -3779,6 +3871,20 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
           acc += jobs[G.jobs[q]].gx * jobs[G.jobs[q]].gy;
         }
         to[G.jobs.size()] = acc;
+      }
+      {
+        RunArgs *ra = (RunArgs *)(host.data() + off_pargs);
+        int64_t *tb = (int64_t *)(host.data() + off_tb), *ob = (int64_t *)(host.data() + off_ob);
+        for (size_t q = 0; q < post.size(); q++) {
+          const Job &J = jobs[post[q]];
+          ra[q] = J.A;
+          tb[q] = tail_total;
+          ob[q] = order_total;
+          tail_total += tail_blocks(J);
+          order_total += order_blocks(J);
+        }
+        tb[post.size()] = tail_total;
+        ob[post.size()] = order_total;
       }
       if (bytes) HIPCHK(c, hipMemcpyAsync(c->multi.p, host.data(), bytes, hipMemcpyHostToDevice, c->stream));
     }
@@ -3827,8 +3933,8 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
         return 0;
       };
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      // 1: each job's fills and tables on its branch; 2: the merged pair kernels on the main stream, the other
-      // jobs' whole sequence on their branches; 3: the merged jobs' replay, tail and link kernels on their branches
+      // 0: every job's fills; 1: each job's tables on its branch; 2: the merged pair kernels on the main stream, the
+      // other jobs' whole sequence on their branches; 3: the merged jobs' replays, tails and link kernels
       hipLaunchKernelGGL(k_fill_ranges, dim3(64, (unsigned)std::min<size_t>(fills.size(), 65535)), dim3(256), 0,
                          c->stream, (const FillRange *)((char *)c->multi.p + off_fills), (int)fills.size());
       int rc = fork();
@@ -3846,9 +3952,24 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
             (void)launch_pairs_multi(G.v, (unsigned)G.tiles, c->stream,
                                      (const RunArgs *)((char *)c->multi.p + G.off_args),
                                      (const int64_t *)((char *)c->multi.p + G.off_toff), (int)G.jobs.size());
+        // 3: the merged jobs' replays on their branches beside one tail launch over all of them on the main stream,
+        // then their link offsets, gather, ordering and histogram sums as one launch each
         if (!rc) rc = fork();
         for (size_t t : order)
-          if (!rc && merged[t]) job_enqueue_post(c, jobs[t], false, c->side[branch[t]], branch[t]);
+          if (!rc && merged[t]) job_enqueue_replay(jobs[t], c->side[branch[t]]);
+        const RunArgs *pa = (const RunArgs *)((char *)c->multi.p + off_pargs);
+        const int np = (int)post.size();
+        if (!rc && tail_total)
+          hipLaunchKernelGGL(k_tail_multi, dim3((unsigned)tail_total), dim3(256), 0, c->stream, pa,
+                             (const int64_t *)((char *)c->multi.p + off_tb), np);
+        if (!rc) rc = join();
+        if (!rc) {
+          hipLaunchKernelGGL(k_scan_multi, dim3(np), dim3(1024), 0, c->stream, pa);
+          hipLaunchKernelGGL(k_link_gather_multi, dim3(NSTRIPE, np), dim3(256), 0, c->stream, pa);
+          hipLaunchKernelGGL(k_link_order_multi, dim3((unsigned)order_total), dim3(256), 0, c->stream, pa,
+                             (const int64_t *)((char *)c->multi.p + off_ob), np);
+          hipLaunchKernelGGL(k_hist_sum_multi, dim3(3, np), dim3(256), 0, c->stream, pa);
+        }
       }
       if (!rc) rc = join();
       if (!rc)
