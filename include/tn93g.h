@@ -125,7 +125,9 @@ typedef struct {
   int64_t n_uncertain;  /* pairs whose distance is an enclosure (lo < hi)               */
   int64_t n_host_exact; /* pairs whose logarithms were resolved on the host (EXACT)     */
   double gpu_ms;        /* device time of the run's kernels (HIP events)                 */
-  double pair_kernel_ms;/* device time of the pair kernel (clean/clean counts + events)   */
+  double pair_kernel_ms;/* device time of the pair kernel (clean/clean counts + events); in a batch, the time of
+                           the one pair-kernel launch shared by the jobs of the same kernel variant (all such launches
+                           together, from the device clock), 0 for a job that ran its own                          */
   double tail_kernel_ms;/* device time of the FP64 tail kernel                            */
 } tn93g_result;
 

@@ -2217,6 +2217,12 @@ __global__ void k_copy_ranges(const CopyRange *__restrict__ r, int nr)
   }
 }
 
+// The device's constant-rate wall clock into *t (one lane): stamps inside a captured graph, where events time nothing.
+__global__ void k_stamp(unsigned long long *t)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *t = (unsigned long long)wall_clock64();
+}
+
 __global__ void k_fill_ranges(const FillRange *__restrict__ r, int nr)
 {
   for (int k = blockIdx.y; k < nr; k += gridDim.y) {
@@ -2513,6 +2519,7 @@ struct tn93g_ctx {
   // a batch's merged pair-kernel launches: per launch the jobs' RunArgs, then their tile offsets; the fill and copy
   // range lists; and the staging of the batch's read-back
   DevBuf multi, stage;
+  int wall_khz = 0;  // rate of the device's constant wall clock (the stamps around a batch's merged pair kernels)
 
   RunBufs *slot(size_t k)
   {
@@ -3328,6 +3335,7 @@ int tn93g_open(int device, tn93g_ctx **out)
     return TN93G_E_NODEV;
   }
   for (auto &e : c->ev) (void)hipEventCreate(&e);
+  if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->wall_khz = 0;
   for (int b = 0; b < NBRANCH; b++) {
     if (hipStreamCreateWithFlags(&c->side[b], hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->join[b], hipEventDisableTiming) != hipSuccess ||
@@ -3835,7 +3843,9 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       off_fills = bytes;
       bytes = (bytes + fills.size() * sizeof(FillRange) + 255) & ~(size_t)255;
       // the read-back of every job (counters, row accumulators, histogram replicas summed, the first spec[t] links)
-      // staged in one device buffer by one launch at the end of the graph, then one transfer
+      // staged in one device buffer by one launch at the end of the graph, then one transfer; the buffer begins with
+      // the two wall-clock stamps around the merged pair kernels
+      stage_bytes = 256;
       for (size_t t = 0; t < m; t++) {
         const Job &J = jobs[t];
         spec[t] = std::min<size_t>(J.B->link_spec, (size_t)J.link_cap * NSTRIPE);
@@ -3891,8 +3901,8 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
     // the batch's work as one graph; a batch that enqueues exactly the same launches relaunches the last graph
     std::vector<unsigned char> key;
     {
-      const void *mp = c->multi.p;
-      const unsigned char *a = (const unsigned char *)&mp;
+      const void *mp[2] = {c->multi.p, c->stage.p};  // both are baked into the graph's launches
+      const unsigned char *a = (const unsigned char *)mp;
       key.insert(key.end(), a, a + sizeof mp);
     }
     for (const Job &J : jobs) {
@@ -3947,11 +3957,14 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       }
       if (!rc && !groups.empty()) {
         rc = join();
+        unsigned long long *stamps = (unsigned long long *)c->stage.p;
+        if (!rc) hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, stamps);
         for (const Group &G : groups)
           if (!rc)
             (void)launch_pairs_multi(G.v, (unsigned)G.tiles, c->stream,
                                      (const RunArgs *)((char *)c->multi.p + G.off_args),
                                      (const int64_t *)((char *)c->multi.p + G.off_toff), (int)G.jobs.size());
+        if (!rc) hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, stamps + 1);
         // 3: the merged jobs' replays on their branches beside one tail launch over all of them on the main stream,
         // then their link offsets, gather, ordering and histogram sums as one launch each
         if (!rc) rc = fork();
@@ -4001,6 +4014,11 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[2]);
+    double pair_ms = 0;  // the merged pair kernels, from the stamps around them
+    if (!groups.empty() && c->wall_khz > 0) {
+      const unsigned long long *stamps = (const unsigned long long *)c->pin.p;
+      pair_ms = stamps[1] >= stamps[0] ? (double)(stamps[1] - stamps[0]) / (double)c->wall_khz : 0.0;
+    }
     std::vector<char> redo(m, 0);
     // jobs with more links than staged: all their links again, each into a block of its own, in one more round
     std::vector<std::vector<LinkRec>> more(m);
@@ -4032,6 +4050,7 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       int rc = job_fast_result(jobs[t], pj, lk, R);
       if (rc) return fail_all(rc);
       R->gpu_ms = ms;  // device time of the whole batch
+      R->pair_kernel_ms = merged[t] ? pair_ms : 0.0;  // the merged pair launches of all merged jobs
     }
     // jobs whose outputs could not be certified, or whose links overflowed a region, run again alone
     for (size_t t = 0; t < m; t++)
