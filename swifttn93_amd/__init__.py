@@ -206,15 +206,31 @@ def _arr(owner, ptr, n, dt):
     return np.asarray(_View(owner, ctypes.cast(ptr, ctypes.c_void_p).value, n, np.dtype(dt).str))
 
 
+_SMALL_LINKS = 4096  # up to this many links are copied out (cheaper than views), after which the result is freed
+
+
+def _copy(ptr, n, ctype, dt):
+    """A copy of the n values at ptr."""
+    if n == 0:
+        return np.zeros(0, dtype=dt)
+    return np.frombuffer((ctype * n).from_address(ctypes.cast(ptr, ctypes.c_void_p).value), dtype=dt).copy()
+
+
 def _to_py(rp, owner):
     r = rp.contents
     n = r.n_links
+    if n <= _SMALL_LINKS:
+        lk = [_copy(r.link_row, n, ctypes.c_int64, np.int64), _copy(r.link_col, n, ctypes.c_int64, np.int64),
+              _copy(r.link_dist, n, ctypes.c_double, np.float64), _copy(r.link_dist_lo, n, ctypes.c_double, np.float64),
+              _copy(r.link_dist_hi, n, ctypes.c_double, np.float64)]
+    else:
+        lk = [_arr(owner, r.link_row, n, np.int64), _arr(owner, r.link_col, n, np.int64),
+              _arr(owner, r.link_dist, n, np.float64), _arr(owner, r.link_dist_lo, n, np.float64),
+              _arr(owner, r.link_dist_hi, n, np.float64)]
     return {
         "exact": bool(r.exact),
-        "hist": np.ctypeslib.as_array(r.hist).astype(np.uint64).reshape(3, 200).copy(),
-        "link_row": _arr(owner, r.link_row, n, np.int64), "link_col": _arr(owner, r.link_col, n, np.int64),
-        "link_dist": _arr(owner, r.link_dist, n, np.float64), "link_lo": _arr(owner, r.link_dist_lo, n, np.float64),
-        "link_hi": _arr(owner, r.link_dist_hi, n, np.float64),
+        "hist": np.frombuffer(r.hist, dtype=np.uint64).reshape(3, 200).copy(),
+        "link_row": lk[0], "link_col": lk[1], "link_dist": lk[2], "link_lo": lk[3], "link_hi": lk[4],
         "mean_lo": list(r.mean_lo), "mean_hi": list(r.mean_hi), "weighted": list(r.weighted),
         "max_lo": list(r.max_lo), "max_hi": list(r.max_hi), "found_links": r.found_links, "skipped": r.skipped,
         "n_replayed": r.n_replayed, "n_uncertain": r.n_uncertain, "n_host_exact": r.n_host_exact,
