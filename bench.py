@@ -6,7 +6,11 @@ columns (swifttn93_amd/gen.py c1), options -t 0.015 -a resolve -g 0.05 -l 500, 2
 resident in HBM before the timed region; a step is one tn93g_run: clean/clean counts, ambiguity events,
 site-order replays, FP64 tail, histogram, per-row sums in tn93's order and the link list copied back to the host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--backend popc|mfma] [--no-cpu] [--config c1|c2|c3]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--backend popc|mfma] [--no-cpu] [--config c1|c2|c3|c4]
+                    [--mode resolve|average|skip|gapmm]
+
+--mode runs the same 6,700 x 1,300 set with another -a mode (-g 0.05 only with resolve; the default line is
+resolve).
 
 Other workloads (--config): c2, the SARS-CoV-2-shaped long alignment (20,000 x 29,903, N-masked blocks,
 -t 0.0005 -a resolve -g 0.05 -l 25000); c3, 64 uneven alignments with mixed -a modes run as one batch (one graph
@@ -29,7 +33,9 @@ from pathlib import Path
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
 
-MODE_FLAGS = ["-t", "0.015", "-a", "resolve", "-g", "0.05", "-l", "500", "-f", "csv"]
+def mode_flags(mode):
+    """tn93's flags for the C1 workload in -a mode (-g 0.05 with resolve, as the headline line is quoted)."""
+    return ["-t", "0.015", "-a", mode] + (["-g", "0.05"] if mode == "resolve" else []) + ["-l", "500", "-f", "csv"]
 VALU_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12  # 39.3: 256 CUs x 64 lanes x 2.4 GHz
 MFMA_INT8_TOPS = 5000.0
 HBM_BYTES_S = 8.0e12
@@ -48,7 +54,7 @@ def profiled_traffic(kernel):
     return None
 
 
-def cpu_baseline(fa, n_pairs):
+def cpu_baseline(fa, n_pairs, mode):
     ref = REPO / "oracle" / "_ref" / "swifttn93_avx512"
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
@@ -63,7 +69,7 @@ def cpu_baseline(fa, n_pairs):
         out = Path(td) / "out.csv"
         for _ in range(3):
             t0 = time.perf_counter()
-            subprocess.run(cmd + ["-o", str(out)] + MODE_FLAGS + [str(fa)], check=True, env=env,
+            subprocess.run(cmd + ["-o", str(out)] + mode_flags(mode) + [str(fa)], check=True, env=env,
                            stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
             dt = time.perf_counter() - t0
             best = dt if best is None else min(best, dt)
@@ -282,6 +288,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (torch), the reference for the HBM byte counters")
     ap.add_argument("--config", default="c1", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--mode", default="resolve", choices=["resolve", "average", "skip", "gapmm"],
+                    help="c1: tn93's -a mode (resolve runs with -g 0.05)")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="c4: length of the query stream")
     a = ap.parse_args()
     if a.gpus != 1:
@@ -313,7 +321,9 @@ def main():
     backend = {"auto": T.BACKEND_AUTO, "popc": T.BACKEND_POPC, "mfma": T.BACKEND_MFMA}[a.backend]
     eng = T.Engine(0)
     s = eng.upload(codes)
-    kw = dict(mode=T.RESOLVE, threshold=0.015, min_overlap=500, resolve_fraction=0.05, backend=backend)
+    tmode = {"resolve": T.RESOLVE, "average": T.AVERAGE, "skip": T.SKIP, "gapmm": T.GAPMM}[a.mode]
+    kw = dict(mode=tmode, threshold=0.015, min_overlap=500, resolve_fraction=0.05 if a.mode == "resolve" else 1.0,
+              backend=backend)
     for _ in range(a.warmup):
         r = eng.run(s, **kw)
     t0 = time.perf_counter()
@@ -333,9 +343,11 @@ def main():
     ops_per_pair = 22 * words if popc else 32 * L
     peak = VALU_INT32_TOPS if popc else MFMA_INT8_TOPS
     achieved = ops_per_pair * n_pairs / (pair_ms * 1e-3) / 1e12
-    # template arguments: backend (1 popcount, 2 MFMA), fast path, dense-N planes, marginal form
-    kname = "k_pairs<1, true, false, true>" if popc else "k_pairs<2, true, false, false>"
-    traffic = profiled_traffic(kname)
+    # template arguments: backend (1 popcount, 2 MFMA), fast path, dense-N planes, marginal form (not in SKIP mode,
+    # whose ambiguity sites are not visited)
+    kname = ("k_pairs<1, true, false, %s>" % ("false" if a.mode == "skip" else "true")) if popc \
+        else "k_pairs<2, true, false, false>"
+    traffic = profiled_traffic(kname) if a.mode == "resolve" else None  # the profile is of the resolve workload
     roofline = {"bound": "valu" if popc else "mfma", "achieved": achieved, "peak": peak, "unit": "TOPS",
                 "frac": achieved / peak, "traffic": traffic,
                 "kernel": f"{kname} ({'popcount' if popc else 'int8 MFMA'} counts + events + routing)",
@@ -347,12 +359,12 @@ def main():
         "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int32+f64",
         "data": "synthetic (seeded HIV-pol-shaped generator, swifttn93_amd/gen.py)",
-        "config": {"workload": "HIV-pol-shaped single set, 6,700 x 1,300, -t 0.015 -a resolve -g 0.05 -l 500",
+        "config": {"workload": "HIV-pol-shaped single set, 6,700 x 1,300, " + " ".join(mode_flags(a.mode)[:-2]),
                    "pairs": n_pairs, "parallelism": "1 GPU"},
         "roofline": roofline,
         "gpu_ms_per_step": float(np.mean(gm)),
         "links": int(r["found_links"]), "replayed_pairs": int(r["n_replayed"]),
-        "uncertain_pairs": int(r["n_uncertain"]),
+        "enclosed_pairs": int(r["n_uncertain"]),  # distances carried as an enclosure and certified on output
     }
     s.free()
     eng.close()
@@ -360,7 +372,7 @@ def main():
         with tempfile.TemporaryDirectory() as td:
             fa = Path(td) / "c1.fa"
             gen.write_fasta(fa, codes)
-            cb = cpu_baseline(fa, n_pairs)
+            cb = cpu_baseline(fa, n_pairs, a.mode)
         line["cpu_baseline"] = cb
         if cb:
             line["gpu_over_cpu"] = value / cb["value"]
