@@ -1799,8 +1799,9 @@ __device__ inline bool cc_table_lookup(const RunArgs &A, int64_t i, int64_t j, u
   return false;
 }
 
-// The sites where either sequence is ambiguous, in increasing site order: the two sorted event lists merged; a
-// site in one list only takes its partner's code from the codes array.
+// The sites where either sequence is ambiguous (under GAPMM also gapped: its lists hold the gap sites), in
+// increasing site order: the two sorted event lists merged; a site in one list only takes its partner's code from
+// the codes array.
 __device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *tab, ReplayState &s)
 {
   int a = A.ev_off[i], ae = A.ev_off[i + 1], b = A.ev_off[j], be = A.ev_off[j + 1];
@@ -1938,24 +1939,6 @@ __device__ void replay_events_dense(const RunArgs &A, int64_t i, int64_t j, cons
   }
 }
 
-// GAPMM: gaps facing bases are events too, so every site that is not clean in both sequences is applied in order.
-__device__ void replay_scan(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *tab, ReplayState &s)
-{
-  const uint4 *s1 = reinterpret_cast<const uint4 *>(A.codes + i * A.Ls);
-  const uint4 *s2 = reinterpret_cast<const uint4 *>(A.codes + j * A.Ls);
-  for (int64_t q = 0; q < A.Ls / 16; q++) {
-    const uint4 a4 = s1[q], b4 = s2[q];
-    const uint32_t aw[4] = {a4.x, a4.y, a4.z, a4.w}, bw[4] = {b4.x, b4.y, b4.z, b4.w};
-    for (int wd = 0; wd < 4; wd++) {
-      if (((aw[wd] | bw[wd]) & 0xFCFCFCFCu) == 0) continue;  // four clean/clean sites (counted from the planes)
-      for (int k = 0; k < 4; k++) {
-        const unsigned c1 = aw[wd] >> (8 * k) & 255u, c2 = bw[wd] >> (8 * k) & 255u;
-        if ((c1 | c2) >= 4) replay_apply(s, tab[c1 * NCODE + c2]);
-      }
-    }
-  }
-}
-
 // Adds each lane's row part to its row's accumulator, one set of atomics per distinct row of the wave: the lanes of
 // a wave mostly share a few rows, and per-lane atomics on one address serialise.  All 64 lanes must call it; lanes
 // without a part pass row < 0.
@@ -2032,10 +2015,8 @@ __global__ __launch_bounds__(RB, DENSE ? 3 : 4) void k_replay(RunArgs A)
       const ReplayEnt *t0 = avg_first ? tabs + NCODE * NCODE : tabs;
       if constexpr (DENSE)
         replay_events_dense(A, i, j, t0, s);
-      else if (A.gapmm)
-        replay_scan(A, i, j, t0, s);
       else
-        replay_events(A, i, j, t0, s);
+        replay_events(A, i, j, t0, s);  // GAPMM: the lists hold the gap sites too
       tn93g_tailin tin;
       double tng;
       replay_merge(s, icc, tin, tng);
@@ -2045,8 +2026,6 @@ __global__ __launch_bounds__(RB, DENSE ? 3 : 4) void k_replay(RunArgs A)
         replay_reset(s);
         if constexpr (DENSE)
           replay_events_dense(A, i, j, tabs + NCODE * NCODE, s);
-        else if (A.gapmm)
-          replay_scan(A, i, j, tabs + NCODE * NCODE, s);
         else
           replay_events(A, i, j, tabs + NCODE * NCODE, s);
         replay_merge(s, icc, tin, tng);
