@@ -293,8 +293,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="c4: length of the query stream")
     a = ap.parse_args()
     if a.gpus != 1:
-        print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported (the multi-GPU sharding is "
-              f"described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported (sharding across GPUs is "
+              f"a design only in this build)", file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import swifttn93_amd as T
