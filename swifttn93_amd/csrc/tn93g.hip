@@ -2168,12 +2168,6 @@ __global__ __launch_bounds__(256) void k_link_order_multi(const RunArgs *__restr
   link_order(A, A.links_flat, A.link_off, A.lslot, A.links, (int64_t)blockIdx.x - boff[k], boff[k + 1] - boff[k]);
 }
 
-// histogram replicas summed into replica 0
-__global__ void k_fill(uint32_t *__restrict__ p, uint32_t v, int64_t n)
-{
-  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) p[k] = v;
-}
-
 // Zeroes a list of ranges in one launch (a batch's per-job buffers): grid row y takes ranges y, y + gridDim.y, ...
 struct FillRange {
   uint32_t *p;
@@ -2202,6 +2196,18 @@ __global__ void k_stamp(unsigned long long *t)
   if (threadIdx.x == 0 && blockIdx.x == 0) *t = (unsigned long long)wall_clock64();
 }
 
+// A single run's zero-started buffers (at most 4 ranges, passed by value): grid row y clears range y.
+struct FillList {
+  FillRange r[4];
+};
+__global__ void k_fill_list(FillList L)
+{
+  const unsigned y = blockIdx.y;  // uniform: the range is picked without indexing the parameter by a variable
+  const FillRange f = y == 0 ? L.r[0] : y == 1 ? L.r[1] : y == 2 ? L.r[2] : L.r[3];
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < f.n; q += (int64_t)gridDim.x * blockDim.x)
+    f.p[q] = 0u;
+}
+
 __global__ void k_fill_ranges(const FillRange *__restrict__ r, int nr)
 {
   for (int k = blockIdx.y; k < nr; k += gridDim.y) {
@@ -2212,6 +2218,7 @@ __global__ void k_fill_ranges(const FillRange *__restrict__ r, int nr)
   }
 }
 
+// histogram replicas summed into replica 0
 __device__ inline void hist_sum(const RunArgs &A, int k)
 {
   if (k >= 600) return;
@@ -2585,15 +2592,6 @@ static int host_hist_index(double d)
   return 199;
 }
 
-// Zero fill of a device range (bytes a multiple of 4) as a kernel, so that runs can be captured in a graph.
-static void fill_zero(hipStream_t s, void *p, size_t bytes)
-{
-  const int64_t n = (int64_t)(bytes / 4);
-  if (n <= 0) return;
-  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_fill, dim3(blocks), dim3(256), 0, s, (uint32_t *)p, 0u, n);
-}
-
 constexpr size_t STRIPE_BYTES = (size_t)NSTRIPE * 32 * 4;
 constexpr size_t COUNTER_STRIDE = STRIPE_BYTES + 256;  // pinned bytes per job: the stripes (replay total included)
 
@@ -2927,10 +2925,12 @@ static void job_enqueue_pre(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, b
   RunArgs &A = J.A;
   const int64_t nrows = J.nrows;
   if (timing) (void)hipEventRecord(c->ev[4], st);
-  if (fills) {
+  if (fills) {  // one launch for the (at most 4) ranges
     std::vector<FillRange> fr;
     job_fill_ranges(J, fr);
-    for (const FillRange &f : fr) fill_zero(st, f.p, (size_t)f.n * 4);
+    FillList L{};
+    for (size_t k = 0; k < fr.size() && k < 4; k++) L.r[k] = fr[k];
+    hipLaunchKernelGGL(k_fill_list, dim3(64, (unsigned)std::min<size_t>(fr.size(), 4)), dim3(256), 0, st, L);
   }
   if (A.nsr)
     hipLaunchKernelGGL(k_spec16, dim3((unsigned)((A.ncols + TJ - 1) / TJ), (unsigned)((A.nsr + TI - 1) / TI)),
