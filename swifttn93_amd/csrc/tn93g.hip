@@ -2957,8 +2957,8 @@ static unsigned order_blocks(const Job &J) { return (unsigned)std::min<int64_t>(
 
 static void job_enqueue_replay(const Job &J, hipStream_t st)
 {
-  if (J.A.nplane)
-    hipLaunchKernelGGL(k_replay<true>, dim3(2048), dim3(RB), 0, st, J.A);
+  if (J.A.nplane)  // 3072 blocks measured best for the dense replay (C2: 482 -> 470 ms per run), 2048 otherwise
+    hipLaunchKernelGGL(k_replay<true>, dim3(3072), dim3(RB), 0, st, J.A);
   else
     hipLaunchKernelGGL(k_replay<false>, dim3(2048), dim3(RB), 0, st, J.A);
 }
