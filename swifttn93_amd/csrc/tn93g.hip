@@ -1968,10 +1968,10 @@ __device__ void rowacc_add_wave(const RunArgs &A, int64_t row, const RowPart &rp
 
 // one thread per pair that needs the site-order replay (the flat list the pair kernel's tiles appended to); every
 // lane of a wave runs the same iterations so the row parts can be combined across the wave.  The replay is latency
-// bound: register budgets for 4 waves per SIMD (3 with dense N) measured fastest on the MI355X, although the compiler
-// then spills a few registers to scratch.
+// bound: a register budget for 4 waves per SIMD measured fastest on the MI355X (also with dense N: C2 470 -> 460 ms
+// against 3 waves), although the compiler then spills some registers to scratch.
 template <bool DENSE>
-__global__ __launch_bounds__(RB, DENSE ? 3 : 4) void k_replay(RunArgs A)
+__global__ __launch_bounds__(RB, 4) void k_replay(RunArgs A)
 {
   __shared__ ReplayEnt tabs[2 * NCODE * NCODE];
   __shared__ unsigned long long lhist[3 * 200];
