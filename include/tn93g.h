@@ -104,10 +104,10 @@ typedef struct {
  *   mean, weighted, max  tn93's row-order reductions (swifttn93.cpp:2839-2848) of the per-row sums;
  *   found_links          sum of link weights; skipped = pairs with overlap <= min_overlap.
  * Floating values that depend on a GPU logarithm are exact when `exact` is 1.  Otherwise each comes as an
- * enclosure [lo, hi] that contains tn93's value (lo == hi when it is known exactly); link distances are then
- * certified only to tn93's printed precision (%g, 6 digits: link_dist_lo/hi give the same string).  A caller
- * that prints with another precision and finds lo and hi print differently calls tn93g_run again with
- * TN93G_FLAG_EXACT. */
+ * enclosure [lo, hi] that contains tn93's value (lo == hi when it is known exactly); link_dist lies in it.  The
+ * histogram bins and the link decisions are certified on the device (an undecidable pair makes the run repeat
+ * itself in exact mode).  Whether a link distance prints as tn93's does is checked by tn93g_result_certify; a
+ * caller that finds lo and hi print differently calls tn93g_run again with TN93G_FLAG_EXACT. */
 typedef struct {
   int32_t exact;
   int32_t nrows;
@@ -140,6 +140,12 @@ int tn93g_run_bytes(tn93g_ctx *ctx, const tn93g_set *set, const tn93g_params *pa
  * and released with tn93g_result_free. */
 int tn93g_run(tn93g_ctx *ctx, const tn93g_set *set, const tn93g_params *params, tn93g_result **out);
 void tn93g_result_free(tn93g_result *res);
+
+/* 1 when every link distance of the result prints the same with "%.{precision}g" at both ends of its enclosure
+ * (always for an exact result), 0 otherwise, or TN93G_E_ARG.  tn93 prints distances with %g (precision 6,
+ * swifttn93.cpp:2575-2585 through fastfmt.h:75): a caller that prints them, or hands them on as tn93's values, and gets 0
+ * runs again with TN93G_FLAG_EXACT.  The check the reference needs none of: it computes every logarithm with libm. */
+int tn93g_result_certify(const tn93g_result *res, int precision);
 
 /* Runs n independent jobs (sets[k] with params[k]; sets may repeat) as one batch: every job's kernels are enqueued
  * back to back on the context's stream, captured into one HIP graph and launched once (a later batch that enqueues

@@ -43,7 +43,7 @@ class Result(ctypes.Structure):
 
 EXPORTS = ["tn93g_open", "tn93g_close", "tn93g_error", "tn93g_set_upload", "tn93g_set_free", "tn93g_run",
            "tn93g_result_free", "tn93g_version", "tn93g_run_batch", "tn93g_set_create", "tn93g_set_append",
-           "tn93g_set_size", "tn93g_run_bytes"]
+           "tn93g_set_size", "tn93g_run_bytes", "tn93g_result_certify"]
 
 _lib = None
 
@@ -74,6 +74,7 @@ def load():
     lib.tn93g_result_free.argtypes = [ctypes.POINTER(Result)]
     lib.tn93g_run_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
                                     ctypes.POINTER(Params), ctypes.POINTER(ctypes.POINTER(Result))]
+    lib.tn93g_result_certify.argtypes = [ctypes.POINTER(Result), ctypes.c_int]
     lib.tn93g_version.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -122,11 +123,20 @@ class Engine:
         return b.value
 
     def run(self, seqset, mode=RESOLVE, kind=TRIANGLE, n1=0, subset_mask=0, backend=BACKEND_AUTO,
-            resolve_fraction=1.0, min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0, n2=0):
+            resolve_fraction=1.0, min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0, n2=0,
+            certify=False):
+        """One tn93g_run.  With certify, a result whose link distances do not print as tn93's would (%g at both
+        ends of their enclosures, tn93g_result_certify) is replaced by the run again with FLAG_EXACT, so every
+        returned link_dist prints as tn93 prints it."""
         p = make_params(mode, kind, n1, subset_mask, backend, resolve_fraction, min_overlap, threshold, min_distance,
                         flags, n2)
         r = ctypes.POINTER(Result)()
         self._check(self.lib.tn93g_run(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r)))
+        if certify and self.lib.tn93g_result_certify(r, 6) != 1:
+            self.lib.tn93g_result_free(r)
+            p.flags |= FLAG_EXACT
+            r = ctypes.POINTER(Result)()
+            self._check(self.lib.tn93g_run(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r)))
         return _to_py(r, _ResultOwner(self.lib, r))
 
     def run_batch(self, jobs):
@@ -228,7 +238,7 @@ def _to_py(rp, owner):
               _arr(owner, r.link_dist, n, np.float64), _arr(owner, r.link_dist_lo, n, np.float64),
               _arr(owner, r.link_dist_hi, n, np.float64)]
     return {
-        "exact": bool(r.exact),
+        "exact": bool(r.exact), "certified": owner.lib.tn93g_result_certify(rp, 6) == 1,
         "hist": np.frombuffer(r.hist, dtype=np.uint64).reshape(3, 200).copy(),
         "link_row": lk[0], "link_col": lk[1], "link_dist": lk[2], "link_lo": lk[3], "link_hi": lk[4],
         "mean_lo": list(r.mean_lo), "mean_hi": list(r.mean_hi), "weighted": list(r.weighted),

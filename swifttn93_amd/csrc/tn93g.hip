@@ -3620,6 +3620,54 @@ void tn93g_result_free(tn93g_result *r)
   free(r);
 }
 
+}  // extern "C"
+
+// True when lo and hi print the same with "%.{prec}g".  Most enclosures are a few ulps wide and far from a
+// rounding midpoint of the printed digits: then the scaled values lo 10^(prec-1-e) and hi 10^(prec-1-e) (e the
+// decimal exponent of lo) lie in [10^(prec-1), 10^prec) and round to the same integer with a margin of 1e-6, far
+// above the error of the scaling (a few ulps of 10^prec); anything else is decided by formatting both.
+static bool same_printed(double lo, double hi, int prec)
+{
+  if (lo == hi) return true;
+  if (lo > 0. && hi > lo && hi < 1e300 && lo > 1e-300) {
+    const int e = (int)std::floor(std::log10(lo));
+    const double p = std::pow(10.0, (double)(prec - 1 - e)), lo_d = std::pow(10.0, (double)(prec - 1)),
+                 hi_d = 10.0 * lo_d;
+    const double a = lo * p, b = hi * p;
+    if (a >= lo_d * (1. + 1e-9) && b <= hi_d * (1. - 1e-9)) {
+      const double fa = std::floor(a + 0.5), fb = std::floor(b + 0.5);
+      if (fa == fb && std::fabs(a - fa) < 0.5 - 1e-6 && std::fabs(b - fb) < 0.5 - 1e-6) return true;
+    }
+  }
+  char x[64], y[64];
+  snprintf(x, sizeof x, "%.*g", prec, lo);
+  snprintf(y, sizeof y, "%.*g", prec, hi);
+  return strcmp(x, y) == 0;
+}
+
+extern "C" {
+
+int tn93g_result_certify(const tn93g_result *r, int precision)
+{
+  if (!r || precision < 1 || precision > 17) return TN93G_E_ARG;
+  if (r->exact) return 1;
+  const int64_t nl = r->n_links;
+  const unsigned nth = nl < 65536 ? 1u : std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  std::atomic<bool> ok{true};
+  auto work = [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e && ok.load(std::memory_order_relaxed); k++)
+      if (!same_printed(r->link_dist_lo[k], r->link_dist_hi[k], precision)) ok = false;
+  };
+  if (nth == 1)
+    work(0, nl);
+  else {
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nth; t++) th.emplace_back(work, nl * t / nth, nl * (t + 1) / nth);
+    for (auto &t : th) t.join();
+  }
+  return ok ? 1 : 0;
+}
+
 
 // Autotuner.  A job's shape is its words per sequence, its size class (log2 of rows and of columns), its mode
 // class and its pair kind; the first job of a shape with TN93G_BACKEND_AUTO and TN93G_FLAG_AUTOTUNE runs its pair

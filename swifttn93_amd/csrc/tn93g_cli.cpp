@@ -16,6 +16,7 @@
 #include <cstring>
 #include <iomanip>
 #include <iostream>
+#include <memory>
 #include <fstream>
 #include <unistd.h>
 #include <sstream>
@@ -263,9 +264,32 @@ struct ResGuard {
   ~ResGuard() { tn93g_result_free(r); }
 };
 
-int run(int argc, const char **argv)
-{
+// One tn93 invocation in three phases: prepare (arguments, reader, upload: everything before the pair computation),
+// the computation (tn93g_run, or one job of a tn93g_run_batch), then certify + output.
+struct Case {
   Args A;
+  Seqs S;
+  long firstLen = 0;
+  unsigned long n1 = 0, n2 = 0, N = 0, pairwise = 0;
+  bool two = false, do_fst = false, report_self = false;  // two: -s (a second file)
+  int prec = 6;
+  tn93g_params P{};
+  SetGuard sg;
+  ResGuard rg;
+};
+
+void close_inputs(Args &A)
+{
+  if (A.input1 && A.input1 != stdin) fclose(A.input1);
+  if (A.input2 && A.input2 != stdin) fclose(A.input2);
+  A.input1 = A.input2 = nullptr;
+}
+
+// Arguments, reader and upload (swifttn93.cpp:2222-2306).  Returns -1 when the case is ready for its computation,
+// otherwise the exit code the process ends with.
+int prepare(Case &C, int argc, const char **argv)
+{
+  Args &A = C.A;
   parse_args(A, argc, argv);
   if (A.do_bootstrap || A.do_bootstrap_two_files || A.include_prob < 1.) {
     std::cerr << "swifttn93: -b, -r and -u (random resampling) are not supported" << std::endl;
@@ -286,29 +310,28 @@ int run(int argc, const char **argv)
       signed char v = valid[(unsigned char)toupper((unsigned char)*q)];
       if (v > 3) subset_mask |= 1u << v;
     }
-  Seqs S;
-  long firstLen = 0;
-  const int threads = tn93g_io::io_threads();
-  if (read_fasta(A.input1, S, firstLen, A.counts_in_name) == 1) return 1;
-  const unsigned long n1 = (unsigned long)S.N;
-  unsigned long n2 = 0;
+  Seqs &S = C.S;
+  if (read_fasta(A.input1, S, C.firstLen, A.counts_in_name) == 1) return 1;
+  C.n1 = (unsigned long)S.N;
   if (A.input2) {
-    if (read_fasta(A.input2, S, firstLen, A.counts_in_name) == 1) return 1;
-    n2 = (unsigned long)S.N - n1;
+    if (read_fasta(A.input2, S, C.firstLen, A.counts_in_name) == 1) return 1;
+    C.n2 = (unsigned long)S.N - C.n1;
   }
-  const bool do_fst = A.input2 && A.do_fst;
-  const bool report_self = A.input2 == nullptr && A.report_self && !A.do_count;
-  const unsigned long N = (unsigned long)S.N;
-  const unsigned long pairwise = (A.input2 && !do_fst) ? n2 * n1 : (N - 1) * N / 2;
+  const bool two = C.two = A.input2 != nullptr;
+  close_inputs(A);
+  C.do_fst = two && A.do_fst;
+  C.report_self = !two && A.report_self && !A.do_count;
+  C.N = (unsigned long)S.N;
+  C.pairwise = (two && !C.do_fst) ? C.n2 * C.n1 : (C.N - 1) * C.N / 2;
   if (!A.quiet) {
-    if (A.input2 == nullptr)
-      std::cerr << "Read " << N << " sequences of length " << firstLen << std::endl
-                << "Will perform " << pairwise << " pairwise distance calculations";
+    if (!two)
+      std::cerr << "Read " << C.N << " sequences of length " << C.firstLen << std::endl
+                << "Will perform " << C.pairwise << " pairwise distance calculations";
     else {
-      std::cerr << "Read " << n1 << " sequences from file 1 and " << n2 << " sequences from file 2 of length "
-                << firstLen << std::endl
-                << "Will perform " << pairwise << " pairwise distance calculations";
-      if (do_fst) std::cerr << ". Running in F_ST mode";
+      std::cerr << "Read " << C.n1 << " sequences from file 1 and " << C.n2 << " sequences from file 2 of length "
+                << C.firstLen << std::endl
+                << "Will perform " << C.pairwise << " pairwise distance calculations";
+      if (C.do_fst) std::cerr << ". Running in F_ST mode";
     }
   }
   // ---- the GPU
@@ -317,76 +340,87 @@ int run(int argc, const char **argv)
     std::cerr << "tn93g: no usable GPU (error " << rc << ")" << std::endl;
     return 1;
   }
-  tn93g_ctx *ctx = g_ctx;
-  SetGuard sg;
-  rc = tn93g_set_upload(ctx, S.codes.data(), (int64_t)N, firstLen, S.counts.data(), &sg.s);
-  tn93g_set *set = sg.s;
+  rc = tn93g_set_upload(g_ctx, S.codes.data(), (int64_t)C.N, C.firstLen, S.counts.data(), &C.sg.s);
   if (rc) {
-    std::cerr << "tn93g: " << tn93g_error(ctx) << std::endl;
+    std::cerr << "tn93g: " << tn93g_error(g_ctx) << std::endl;
     return 1;
   }
-  tn93g_params P;
+  tn93g_params &P = C.P;
   memset(&P, 0, sizeof P);
   P.mode = A.ambig;
-  P.kind = (A.input2 && !do_fst) ? TN93G_CROSS : (do_fst ? TN93G_FST : TN93G_TRIANGLE);
-  P.n1 = (int64_t)n1;
+  P.kind = (two && !C.do_fst) ? TN93G_CROSS : (C.do_fst ? TN93G_FST : TN93G_TRIANGLE);
+  P.n1 = (int64_t)C.n1;
   P.subset_mask = subset_mask;
   P.backend = TN93G_BACKEND_AUTO;
   P.resolve_fraction = A.resolve_fraction;
   P.min_overlap = (long)A.overlap;
   P.threshold = A.distance;
   P.min_distance = A.min_distance;
-  P.flags = do_fst ? TN93G_FLAG_EXACT : 0u;  // F_ST combines the means non-monotonically: resolve logs exactly
+  P.flags = C.do_fst ? TN93G_FLAG_EXACT : 0u;  // F_ST combines the means non-monotonically: resolve logs exactly
   if (getenv("TN93G_FLAGS")) P.flags |= (uint32_t)strtoul(getenv("TN93G_FLAGS"), nullptr, 0);
-  ResGuard rg;
-  tn93g_result *&R = rg.r;
-  const int prec = (!A.quiet && A.output == stdout) ? 3 : 6;  // tn93's progress line leaves cerr at precision 3
+  C.prec = (!A.quiet && A.output == stdout) ? 3 : 6;  // tn93's progress line leaves cerr at precision 3
+  return -1;
+}
+
+// True when every printed link distance and the summary are the same at both ends of the result's enclosures.
+bool certified(const Case &C, const tn93g_result *R)
+{
+  if (R->exact) return true;
+  if (tn93g_result_certify(R, 6) != 1) return false;  // the link distances, printed with %g
+  std::ostringstream s1, s2;
+  s1 << std::setprecision(C.prec);
+  s2 << std::setprecision(C.prec);
   uint64_t comb[200];
-  for (int attempt = 0;; attempt++) {
-    rc = tn93g_run(ctx, set, &P, &R);
-    if (rc) {
-      std::cerr << "tn93g: " << tn93g_error(ctx) << std::endl;
-      return 1;
+  Summary lo{C.pairwise, C.n1, C.n2, C.N, R->found_links, R->skipped, {R->weighted[0], R->weighted[1], R->weighted[2]},
+             R->max_lo[0], {R->mean_lo[0], R->mean_lo[1], R->mean_lo[2]}, &R->hist[0][0]};
+  Summary hi = lo;
+  hi.max0 = R->max_hi[0];
+  for (int k = 0; k < 3; k++) hi.mean[k] = R->mean_hi[k];
+  write_json(s1, lo, C.two, C.do_fst, comb);
+  write_json(s2, hi, C.two, C.do_fst, comb);
+  return s1.str() == s2.str();
+}
+
+// The case's certified result in C.rg: the one it has when that certifies, else the run again with TN93G_FLAG_EXACT
+// (logarithms resolved with the host libm).  Returns 0 or an exit code.
+int settle(Case &C)
+{
+  for (;;) {
+    if (!C.rg.r) {
+      if (tn93g_run(g_ctx, C.sg.s, &C.P, &C.rg.r)) {
+        std::cerr << "tn93g: " << tn93g_error(g_ctx) << std::endl;
+        return 1;
+      }
     }
-    if (R->exact || attempt > 0) break;
-    // certify: every printed link distance and the summary must be the same at both ends of the enclosures
-    std::atomic<bool> all_same{true};
-    const int64_t nl = R->n_links, per = 1 << 16;
-    tn93g_io::parallel_for(threads, (nl + per - 1) / per, [&](int64_t t) {
-      char a[64], b[64];
-      for (int64_t k = t * per; k < std::min(nl, (t + 1) * per) && all_same.load(std::memory_order_relaxed); k++)
-        if (R->link_dist_lo[k] != R->link_dist_hi[k]) {
-          fmt_g(a, R->link_dist_lo[k]);
-          fmt_g(b, R->link_dist_hi[k]);
-          if (strcmp(a, b)) all_same = false;
-        }
-    });
-    bool ok = all_same;
-    if (ok) {
-      std::ostringstream s1, s2;
-      s1 << std::setprecision(prec);
-      s2 << std::setprecision(prec);
-      Summary lo{pairwise, n1, n2, N, R->found_links, R->skipped, {R->weighted[0], R->weighted[1], R->weighted[2]},
-                 R->max_lo[0], {R->mean_lo[0], R->mean_lo[1], R->mean_lo[2]}, &R->hist[0][0]};
-      Summary hi = lo;
-      hi.max0 = R->max_hi[0];
-      for (int k = 0; k < 3; k++) hi.mean[k] = R->mean_hi[k];
-      write_json(s1, lo, A.input2 != nullptr, do_fst, comb);
-      write_json(s2, hi, A.input2 != nullptr, do_fst, comb);
-      ok = s1.str() == s2.str();
-    }
-    if (ok) break;
-    tn93g_result_free(R);
-    R = nullptr;
-    P.flags |= TN93G_FLAG_EXACT;
+    if (certified(C, C.rg.r) || (C.P.flags & TN93G_FLAG_EXACT)) return 0;
+    tn93g_result_free(C.rg.r);
+    C.rg.r = nullptr;
+    C.P.flags |= TN93G_FLAG_EXACT;
   }
+}
+
+// The distance file and the summary (swifttn93.cpp:2307-2316, 2406-2434, 2547-2603, 2869-2963).
+int output(Case &C)
+{
+  Args &A = C.A;
+  const Seqs &S = C.S;
+  const tn93g_result *R = C.rg.r;
+  const unsigned long N = C.N, n1 = C.n1, pairwise = C.pairwise;
+  const bool two = C.two, do_fst = C.do_fst, report_self = C.report_self;
+  const int threads = tn93g_io::io_threads();
   if (getenv("TN93G_STATS"))
     fprintf(stderr, "tn93g: gpu %.3f ms (pair kernel %.3f ms) links %lld replayed %lld uncertain %lld host-exact %lld\n",
             R->gpu_ms, R->pair_kernel_ms, (long long)R->n_links, (long long)R->n_replayed,
             (long long)R->n_uncertain, (long long)R->n_host_exact);
-  // ---- output (swifttn93.cpp:2307-2316, 2406-2434, 2547-2603, 2869-2897)
-  static char obuf[1 << 16];
-  setvbuf(A.output, obuf, _IOFBF, sizeof obuf);
+  // a full buffer for the distance file; stdout gets its own once, before anything is written to it
+  static char obuf[1 << 16], sbuf[1 << 16];
+  static bool stdout_buffered = false;
+  if (A.output != stdout)
+    setvbuf(A.output, obuf, _IOFBF, sizeof obuf);
+  else if (!stdout_buffered) {
+    setvbuf(stdout, sbuf, _IOFBF, sizeof sbuf);
+    stdout_buffered = true;
+  }
   if (A.format != F_HYPHY && !A.do_count && !A.skip_header) {
     if (A.format == F_CSV)
       fprintf(A.output, "ID1%cID2%cDistance\n", A.delimiter, A.delimiter);
@@ -394,7 +428,7 @@ int run(int argc, const char **argv)
       fprintf(A.output, "Seq1%cSeq2%cDistance\n", A.delimiter, A.delimiter);
   }
   const bool emit = !A.do_count;
-  const bool cross = A.input2 && !do_fst;
+  const bool cross = two && !do_fst;
   const long upper = cross ? (long)n1 : (long)N;
   const int64_t nl = R->n_links;
   if (A.format != F_HYPHY && (emit || report_self)) {
@@ -476,50 +510,151 @@ int run(int argc, const char **argv)
               << std::setw(12) << std::setprecision(3) << (double)pairwise << " evals)";
   std::cerr << std::endl;
   std::ostream &os = A.output == stdout ? std::cerr : std::cout;
-  Summary s{pairwise, n1, n2, N, R->found_links, R->skipped, {R->weighted[0], R->weighted[1], R->weighted[2]},
+  uint64_t comb[200];
+  Summary s{pairwise, n1, C.n2, N, R->found_links, R->skipped, {R->weighted[0], R->weighted[1], R->weighted[2]},
             R->max_lo[0], {R->mean_lo[0], R->mean_lo[1], R->mean_lo[2]}, &R->hist[0][0]};
-  write_json(os, s, A.input2 != nullptr, do_fst, comb);
+  write_json(os, s, two, do_fst, comb);
   if (A.do_count)
     fprintf(A.output, "Found %ld links among %ld pairwise comparisons\n", (long)R->found_links,
             (long)(pairwise - (unsigned long)R->skipped));
+  fflush(A.output);
+  if (A.output != stdout) {
+    fclose(A.output);
+    A.output = stdout;
+  }
   return 0;
 }
 
-int run_guarded(int argc, const char **argv)
+int run(int argc, const char **argv)
+{
+  Case C;
+  int rc = prepare(C, argc, argv);
+  if (rc >= 0) return rc;
+  if ((rc = settle(C))) return rc;
+  return output(C);
+}
+
+template <class F>
+int guarded(F f)
 {
   try {
-    return run(argc, argv);
+    return f();
   } catch (const Exit &e) {
     return e.code;
   }
 }
 
-// Batch driver for tests: every line of FILE is one tn93 invocation, fields separated by 0x1f: the working
-// directory, then the arguments.  Each case runs in this process on one GPU context; its standard output goes to
-// DIR/stdout.bin and its exit code to DIR/rc.txt.
-int batch(const char *file)
+int run_guarded(int argc, const char **argv)
+{
+  return guarded([&] { return run(argc, argv); });
+}
+
+// Test drivers: every line of FILE is one tn93 invocation, fields separated by 0x1f: the working directory, then the
+// arguments.  Each case runs in this process on one GPU context; its standard output goes to DIR/stdout.bin and its
+// exit code to DIR/rc.txt.
+//   --batch FILE   the cases one after the other, each with tn93g_run
+//   --jobs FILE    the cases in groups of up to 256: all of a group's cases are prepared, their computations run as
+//                  one tn93g_run_batch (one graph launch), then each is certified (a case whose printed values differ
+//                  at the ends of their enclosures runs again alone with TN93G_FLAG_EXACT) and written out
+std::vector<std::vector<std::string>> read_cases(const char *file)
 {
   std::ifstream in(file, std::ios::binary);
   std::string line;
-  char cwd[4096];
-  if (!getcwd(cwd, sizeof cwd)) return 2;
+  std::vector<std::vector<std::string>> out;
   while (std::getline(in, line)) {
     std::vector<std::string> f;
     size_t a = 0;
     for (size_t b; (b = line.find('\x1f', a)) != std::string::npos; a = b + 1) f.push_back(line.substr(a, b - a));
     f.push_back(line.substr(a));
+    out.push_back(f);
+  }
+  return out;
+}
+
+std::vector<const char *> case_argv(const std::vector<std::string> &f)
+{
+  std::vector<const char *> av{"tn93g"};
+  for (size_t k = 1; k < f.size(); k++) av.push_back(f[k].c_str());
+  return av;
+}
+
+// Runs fn with std::cout captured into DIR/stdout.bin; writes the exit code to DIR/rc.txt.
+template <class F>
+void captured(const std::string &dir, F fn)
+{
+  std::ostringstream cap;
+  std::streambuf *old = std::cout.rdbuf(cap.rdbuf());
+  std::cerr.precision(6);
+  const int rc = guarded(fn);
+  std::cout.flush();
+  std::cout.rdbuf(old);
+  std::ofstream(dir + "/stdout.bin", std::ios::binary | std::ios::app) << cap.str();
+  if (rc >= 0) std::ofstream(dir + "/rc.txt") << rc << "\n";
+}
+
+int batch(const char *file)
+{
+  char cwd[4096];
+  if (!getcwd(cwd, sizeof cwd)) return 2;
+  for (auto &f : read_cases(file)) {
     if (f.empty() || chdir(f[0].c_str())) return 2;
-    std::vector<const char *> av{"tn93g"};
-    for (size_t k = 1; k < f.size(); k++) av.push_back(f[k].c_str());
-    std::ostringstream cap;
-    std::streambuf *old = std::cout.rdbuf(cap.rdbuf());
-    std::cerr.precision(6);
-    int rc = run_guarded((int)av.size(), av.data());
-    std::cout.flush();
-    std::cout.rdbuf(old);
-    std::ofstream(f[0] + "/stdout.bin", std::ios::binary) << cap.str();
-    std::ofstream(f[0] + "/rc.txt") << rc << "\n";
+    std::vector<const char *> av = case_argv(f);
+    std::ofstream(f[0] + "/stdout.bin", std::ios::binary | std::ios::trunc);
+    captured(f[0], [&] { return run((int)av.size(), av.data()); });
     if (chdir(cwd)) return 2;
+  }
+  return 0;
+}
+
+int jobs(const char *file)
+{
+  char cwd[4096];
+  if (!getcwd(cwd, sizeof cwd)) return 2;
+  const auto all = read_cases(file);
+  constexpr size_t GROUP = 256;  // cases prepared at once (each holds its output file open)
+  for (size_t g0 = 0; g0 < all.size(); g0 += GROUP) {
+    const size_t g1 = std::min(all.size(), g0 + GROUP);
+    std::vector<std::unique_ptr<Case>> cs(g1 - g0);
+    std::vector<std::vector<const char *>> avs(g1 - g0);
+    std::vector<size_t> ready;
+    for (size_t k = g0; k < g1; k++) {
+      const auto &f = all[k];
+      if (f.empty() || chdir(f[0].c_str())) return 2;
+      cs[k - g0].reset(new Case());
+      avs[k - g0] = case_argv(f);
+      std::ofstream(f[0] + "/stdout.bin", std::ios::binary | std::ios::trunc);
+      Case &C = *cs[k - g0];
+      // a case that ends here (bad arguments, reader errors) writes its exit code now; -1 means "prepared"
+      captured(f[0], [&] {
+        int rc = prepare(C, (int)avs[k - g0].size(), avs[k - g0].data());
+        if (rc < 0) ready.push_back(k - g0);
+        return rc;
+      });
+      if (chdir(cwd)) return 2;
+    }
+    std::vector<const tn93g_set *> sets;
+    std::vector<tn93g_params> params;
+    for (size_t q : ready) {
+      sets.push_back(cs[q]->sg.s);
+      params.push_back(cs[q]->P);
+    }
+    std::vector<tn93g_result *> res(ready.size(), nullptr);
+    if (!ready.empty() && tn93g_run_batch(g_ctx, (int)ready.size(), sets.data(), params.data(), res.data())) {
+      std::cerr << "tn93g: " << tn93g_error(g_ctx) << std::endl;
+      return 1;
+    }
+    for (size_t r = 0; r < ready.size(); r++) cs[ready[r]]->rg.r = res[r];
+    for (size_t q : ready) {
+      const auto &f = all[g0 + q];
+      if (chdir(f[0].c_str())) return 2;
+      Case &C = *cs[q];
+      captured(f[0], [&] {
+        const int rc = settle(C);
+        return rc ? rc : output(C);
+      });
+      cs[q].reset();
+      if (chdir(cwd)) return 2;
+    }
   }
   return 0;
 }
@@ -528,7 +663,9 @@ int batch(const char *file)
 
 int main(int argc, const char **argv)
 {
-  int rc = (argc == 3 && !strcmp(argv[1], "--batch")) ? batch(argv[2]) : run_guarded(argc, argv);
+  int rc = (argc == 3 && !strcmp(argv[1], "--batch"))  ? batch(argv[2])
+           : (argc == 3 && !strcmp(argv[1], "--jobs")) ? jobs(argv[2])
+                                                        : run_guarded(argc, argv);
   if (g_ctx) tn93g_close(g_ctx);
   return rc;
 }

@@ -206,16 +206,18 @@ def c2s(n=300, seed=5, L=29903):
 C3_MODES = ["resolve", "average", "skip", "gapmm"]
 
 
-def c3(seed=6, count=64):
+def c3(seed=6, count=64, n_range=(50, 3000), L_range=(300, 10000)):
     """The batched workload: `count` independent alignments from the c0 generator, sizes log-uniform in 50-3,000
     sequences, lengths uniform in 300-10,000 columns, -a modes cycling resolve/average/skip/gapmm.  A list of
-    (codes, mode) pairs."""
+    (codes, mode) pairs.  Parity tests draw from the same generator over smaller ranges."""
     rs = Stream(seed)
     out = []
+    n0, n1 = n_range
+    l0, l1 = L_range
     for k in range(count):
         u = rs.uniform(3)
-        n = int(np.exp(np.log(50.0) + u[0] * (np.log(3000.0) - np.log(50.0))))
-        L = int(300 + u[1] * 9700)
+        n = int(np.exp(np.log(float(n0)) + u[0] * (np.log(float(n1)) - np.log(float(n0)))))
+        L = int(l0 + u[1] * (l1 - l0))
         out.append((c0(seed=int(rs.u64(1)[0] >> np.uint64(1)), N=n, L=L), C3_MODES[k % 4]))
     return out
 

@@ -10,6 +10,14 @@ Admission: a batch waits until a worker is free and the device memory of the run
 (tn93g_run_bytes) fits in the HBM budget.  Each worker thread owns a context (its own stream and run buffers), so
 admitted runs overlap on the device while later batches are appended and admitted.  ctypes releases the GIL during
 the library calls.
+
+Merging: batches that are waiting together when a worker frees up are admitted as one group (up to `merge_rows`
+queries) and computed by ONE APPEND run over the group's whole range [n1 of the first, n2 of the last).  That run's
+pairs are exactly the union of the batches' own pairs (pair (i, j) with i < j belongs to the batch holding j), so a
+batch's links are the run's links whose column lies in its range, bit for bit the same as alone.
+
+Results are certified: each run's link distances must print with %g as tn93 prints them at both ends of their
+enclosures (tn93g_result_certify), or the run is repeated with FLAG_EXACT.
 """
 import collections
 import threading
@@ -27,7 +35,8 @@ class Request:
         self.n1 = self.n2 = None  # index range in the panel set after the append
         self.bytes = 0
         self.started = self.done = None
-        self.result = None
+        self.result = None  # {"link_row", "link_col", "link_dist"}: the batch's links in tn93's order
+        self.group = 1      # batches computed by the same APPEND run (this one included)
         self.error = None
         self.event = threading.Event()
 
@@ -39,8 +48,9 @@ class Request:
 class Server:
     """Panel set on device 0; `workers` contexts run the admitted batches concurrently."""
 
-    def __init__(self, panel, capacity, workers=4, hbm_budget=32 << 30, run_kw=None):
+    def __init__(self, panel, capacity, workers=4, hbm_budget=32 << 30, run_kw=None, merge_rows=0):
         panel = np.ascontiguousarray(panel, dtype=np.uint8)
+        self.merge_rows = merge_rows  # 0: every batch is its own run
         self.L = panel.shape[1]
         self.owner = Engine(0)
         self.set = self.owner.create(self.L, capacity)
@@ -82,29 +92,51 @@ class Server:
                     self.lock.wait()
                 if not self._admissible():  # stopping, nothing left
                     return
-                req = self.pending.popleft()
+                group = [self.pending.popleft()]
+                rows = group[0].n2 - group[0].n1
+                nbytes = group[0].bytes
+                # later batches waiting behind it join the group while the merged run stays within the budget
+                while self.merge_rows and self.pending and rows + self.pending[0].n2 - self.pending[0].n1 <= \
+                        self.merge_rows:
+                    nxt = self.pending[0]
+                    b = self.owner.run_bytes(self.set, kind=APPEND, n1=group[0].n1, n2=nxt.n2, **self.run_kw)
+                    if self.in_flight and self.in_flight + b > self.budget:
+                        break
+                    group.append(self.pending.popleft())
+                    rows += nxt.n2 - nxt.n1
+                    nbytes = b
                 eng = self.free.pop()
-                self.in_flight += req.bytes
-            t = threading.Thread(target=self._work, args=(eng, req), daemon=True)
+                self.in_flight += nbytes
+            t = threading.Thread(target=self._work, args=(eng, group, nbytes), daemon=True)
             t.start()
             self.threads.append(t)
 
-    def _work(self, eng, req):
-        req.started = time.perf_counter()
+    def _work(self, eng, group, nbytes):
+        t0 = time.perf_counter()
+        for req in group:
+            req.started = t0
+            req.group = len(group)
         try:
-            req.result = eng.run(self.set, kind=APPEND, n1=req.n1, n2=req.n2, **self.run_kw)
-        except Exception as e:  # reported with the request
-            req.error = e
-        req.done = time.perf_counter()
+            r = eng.run(self.set, kind=APPEND, n1=group[0].n1, n2=group[-1].n2, certify=True, **self.run_kw)
+            for req, part in zip(group, split_links(r, [q.n1 for q in group] + [group[-1].n2])):
+                req.result = part
+        except Exception as e:  # reported with the requests
+            for req in group:
+                req.error = e
+        t1 = time.perf_counter()
+        for req in group:
+            req.done = t1
         with self.lock:
-            self.in_flight -= req.bytes
+            self.in_flight -= nbytes
             self.free.append(eng)
             self.lock.notify_all()
-        req.event.set()
+        for req in group:
+            req.event.set()
 
     def run_alone(self, req):
         """The same batch computed again, alone, on the owner context (for the alone-vs-mixed equality check)."""
-        return self.owner.run(self.set, kind=APPEND, n1=req.n1, n2=req.n2, **self.run_kw)
+        r = self.owner.run(self.set, kind=APPEND, n1=req.n1, n2=req.n2, certify=True, **self.run_kw)
+        return split_links(r, [req.n1, req.n2])[0]
 
     def close(self):
         with self.lock:
@@ -117,6 +149,19 @@ class Server:
             e.close()
         self.set.free()
         self.owner.close()
+
+
+def split_links(r, bounds):
+    """The links of an APPEND run over [bounds[0], bounds[-1]) split by column into the ranges between consecutive
+    bounds (the links stay in tn93's order within each part)."""
+    col = r["link_col"]
+    parts = []
+    for a, b in zip(bounds[:-1], bounds[1:]):
+        m = (col >= a) & (col < b)
+        parts.append({"link_row": np.asarray(r["link_row"])[m].copy(), "link_col": np.asarray(col)[m].copy(),
+                      "link_dist": np.asarray(r["link_dist"])[m].copy(), "certified": r["certified"],
+                      "exact": r["exact"]})
+    return parts
 
 
 def query_stream(panel, seed=8, rate=20.0, seconds=60.0, max_batch=512):

@@ -1,0 +1,298 @@
+"""GPU parity of the paths around a single run, against the oracle and the published fixtures:
+
+* the batch runner: many alignments in one tn93g_run_batch (tn93g --jobs), each case's output byte for byte;
+* the process interface as callers use it: tn93g exec'd once per call, with and without -o (links on stdout, the
+  JSON on stderr, swifttn93.cpp:2904-2912), quiet or not, from a file or stdin, and error exits;
+* alignments of 65,535 columns or more (the reference's slow path, swifttn93.cpp:580-590);
+* serving: every served batch's links against the oracle's triangle over everything received, merged or not, and
+  alone = mixed on the full 50,000-sequence panel.
+
+Needs an MI355X: run with ``pytest -m gpu``."""
+import ctypes
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+
+pytestmark = pytest.mark.gpu
+
+sys.path.insert(0, str(REPO))
+import swifttn93_amd as T  # noqa: E402
+from swifttn93_amd import gen  # noqa: E402
+
+SEP = "\x1f"
+
+
+def _cli():
+    if not T.CLI_PATH.exists() or not T.LIB_PATH.exists():
+        pytest.fail("the engine is not built (python3 -m swifttn93_amd.build)")
+    return T.CLI_PATH
+
+
+def _oracle_cli():
+    cli = REPO / "oracle" / "_build" / "tn93_oracle"
+    if not cli.exists():
+        subprocess.run(["make", "-s", "-C", str(REPO / "oracle"), "oracle"], check=True)
+    return cli
+
+
+def run_jobs(cases, tmp_path, timeout=600):
+    """cases: list of (dir, argv without program name); all run by one `tn93g --jobs` process as batches."""
+    bf = tmp_path / "jobs.txt"
+    bf.write_text("".join(SEP.join([str(d)] + list(a)) + "\n" for d, a in cases))
+    print(f"tn93g --jobs: {len(cases)} cases", flush=True)
+    p = subprocess.run([str(_cli()), "--jobs", str(bf)], capture_output=True, timeout=timeout)
+    assert p.returncode == 0, p.stderr[-2000:]
+    res = []
+    for d, _ in cases:
+        rc = int((d / "rc.txt").read_text())
+        out = (d / "out.txt").read_bytes() if (d / "out.txt").exists() else None
+        res.append((rc, out, (d / "stdout.bin").read_bytes()))
+    return res
+
+
+def test_jobs_fuzz_fixtures(fuzz_cases, tmp_path):
+    """The 600 reference-built fuzz fixtures as batches of up to 256 jobs in one tn93g_run_batch each."""
+    cases = []
+    for k, c in enumerate(fuzz_cases):
+        d = tmp_path / f"c{k}"
+        d.mkdir()
+        for key, v in c["files"].items():
+            (d / f"{key}.fa").write_bytes(v.encode("latin-1"))
+        argv = ["-q"] + [f"{f.strip('{}')}.fa" if f.startswith("{") else f for f in c["flags"]] + ["-o", "out.txt",
+                                                                                                   "in.fa"]
+        cases.append((d, argv))
+    res = run_jobs(cases, tmp_path)
+    bad = []
+    for k, (c, (rc, out, so)) in enumerate(zip(fuzz_cases, res)):
+        if c["rc"] != 0:
+            if rc == 0:
+                bad.append((k, c["flags"], "expected failure"))
+            continue
+        exp = None if c["out"] is None else c["out"].encode("latin-1")
+        if rc != 0 or out != exp or so != c["stdout"].encode("latin-1"):
+            bad.append((k, c["flags"], rc, (out or b"")[:200], (exp or b"")[:200], so[:300]))
+    assert not bad, bad[:3]
+
+
+def _mode_flags(mode):
+    return ["-a", mode] + (["-g", "0.05"] if mode == "resolve" else [])
+
+
+def test_jobs_c3_vs_oracle(tmp_path):
+    """The batched workload's generator (c0 alignments, log-uniform sizes, uniform lengths, -a modes cycling
+    resolve/average/skip/gapmm) at sizes the oracle finishes in seconds: 40 alignments of 20-400 sequences x
+    300-4,000 columns, plus -s bipartite jobs, an F_ST job and all-pairs-written jobs, all in one batch."""
+    jobs = gen.c3(seed=106, count=40, n_range=(20, 400), L_range=(300, 4000))
+    cases, want = [], []
+    orc = _oracle_cli()
+
+    def add(name, argv, files):
+        d = tmp_path / name
+        d.mkdir()
+        for fn, codes in files.items():
+            gen.write_fasta(d / fn, codes, gen.names_for(codes.shape[0], fn[0]))
+        p = subprocess.run([str(orc), "-q", *argv, "-o", "want.txt", "in.fa"], cwd=d, capture_output=True,
+                           check=True, timeout=600)
+        want.append(((d / "want.txt").read_bytes(), p.stdout))
+        cases.append((d, ["-q", *argv, "-o", "out.txt", "in.fa"]))
+
+    for k, (codes, mode) in enumerate(jobs):
+        L = codes.shape[1]
+        t = ["0.015", "0.05", "1"][k % 3]
+        add(f"j{k}", ["-t", t, "-l", str(min(500, L // 2)), *_mode_flags(mode)], {"in.fa": codes})
+    for k in range(4):  # -s query-by-reference blocks (CROSS), and one -m (F_ST, exact path, run alone)
+        codes = gen.c1s(260 + 40 * k, seed=120 + k)
+        nq = 40 + 20 * k
+        extra = ["-m"] if k == 3 else []
+        add(f"s{k}", ["-t", "0.02", "-l", "500", "-s", "ref.fa", *extra, *_mode_flags(gen.C3_MODES[k])],
+            {"in.fa": codes[:nq], "ref.fa": codes[nq:]})
+    res = run_jobs(cases, tmp_path)
+    bad = [(str(d), rc) for (d, _), (rc, out, so), (wo, ws) in zip(cases, res, want)
+           if rc != 0 or out != wo or so != ws]
+    assert not bad, bad[:5]
+
+
+def _exec(argv, cwd, stdin=None, exe=None):
+    p = subprocess.run([str(exe or _cli()), *argv], cwd=cwd, stdin=stdin, capture_output=True, timeout=300)
+    return p.returncode, p.stdout, p.stderr
+
+
+PROCESS_CASES = [
+    ("o_quiet", ["-q", "-t", "0.02", "-l", "500", "-a", "resolve", "-g", "0.05", "-o", "out.csv", "in.fa"]),
+    ("o_verbose", ["-t", "0.02", "-l", "500", "-a", "average", "-o", "out.csv", "in.fa"]),
+    ("stdout_quiet", ["-q", "-t", "0.02", "-l", "500", "-a", "resolve", "-g", "0.05", "in.fa"]),
+    ("stdout_verbose", ["-t", "0.03", "-l", "500", "-a", "skip", "-f", "csvn", "in.fa"]),
+    ("stdout_dash", ["-t", "0.02", "-l", "500", "-o", "-", "-0", "in.fa"]),
+    ("stdin", ["-q", "-t", "0.02", "-l", "500", "-a", "gapmm"]),
+    ("two_files", ["-t", "0.03", "-l", "500", "-s", "ref.fa", "-f", "csvn", "in.fa"]),
+    ("fst_verbose", ["-t", "0.03", "-l", "500", "-s", "ref.fa", "-m", "in.fa"]),
+    ("hyphy_stdout", ["-q", "-t", "0.03", "-l", "500", "-f", "hyphy", "small.fa"]),
+    ("count", ["-t", "0.02", "-l", "500", "-c", "in.fa"]),
+    ("bad_overlap", ["-t", "0.02", "-l", "0", "in.fa"]),
+    ("bad_format", ["-q", "-f", "xml", "in.fa"]),
+    ("missing_file", ["-q", "nonexistent.fa"]),
+    ("unequal_lengths", ["-q", "-t", "0.02", "ragged.fa"]),
+    ("version", ["-v"]),
+]
+
+
+def test_process_interface(tmp_path):
+    """tn93g exec'd as its own process once per call, as HIV-TRACE and the reference's own scripts call tn93
+    (run_network.py:16-17, evaluate.py:75-76): exit code, stdout, stderr and the output file against the oracle
+    CLI and, where it was built, the reference solver itself (oracle/_ref), byte for byte."""
+    codes = gen.c1s(360, seed=131)
+    gen.write_fasta(tmp_path / "in.fa", codes[:240])
+    gen.write_fasta(tmp_path / "ref.fa", codes[240:], gen.names_for(120, "r"))
+    gen.write_fasta(tmp_path / "small.fa", codes[:30])
+    (tmp_path / "ragged.fa").write_bytes(b">a\nACGTACGT\n>b\nACGTAC\n")
+    witnesses = [_oracle_cli()]
+    ref = REPO / "oracle" / "_ref" / "swifttn93"
+    if ref.exists():
+        witnesses.append(ref)
+    for name, argv in PROCESS_CASES:
+        got_dir = tmp_path / f"g_{name}"
+        got_dir.mkdir()
+        for f in ("in.fa", "ref.fa", "small.fa", "ragged.fa"):
+            os.link(tmp_path / f, got_dir / f)
+        print(f"process case {name}: {' '.join(argv)}", flush=True)
+        stdin = open(tmp_path / "in.fa", "rb") if name == "stdin" else None
+        got = _exec(argv, got_dir, stdin=stdin)
+        got_file = (got_dir / "out.csv").read_bytes() if (got_dir / "out.csv").exists() else None
+        for w in witnesses:
+            wd = tmp_path / f"w_{name}_{w.name}"
+            wd.mkdir()
+            for f in ("in.fa", "ref.fa", "small.fa", "ragged.fa"):
+                os.link(tmp_path / f, wd / f)
+            stdin = open(tmp_path / "in.fa", "rb") if name == "stdin" else None
+            exp = _exec(argv, wd, stdin=stdin, exe=w)
+            exp_file = (wd / "out.csv").read_bytes() if (wd / "out.csv").exists() else None
+            assert got[0] == exp[0], (name, w.name, got[0], exp[0], got[2][-500:])
+            if name in ("bad_overlap", "bad_format", "missing_file"):  # after the usage line: the same message
+                assert got[2].endswith(exp[2].split(b"tn93: error:")[-1]), (name, got[2][-300:], exp[2][-300:])
+                continue
+            assert got[1] == exp[1], (name, w.name, "stdout", got[1][:300], exp[1][:300])
+            assert got[2] == exp[2], (name, w.name, "stderr", got[2][:600], exp[2][:600])
+            assert got_file == exp_file, (name, w.name, "output file")
+
+
+def _long_codes(seed, n, L):
+    """SARS-CoV-2-shaped sequences of L columns (N blocks, low divergence) with more ambiguity codes (B/D/H/V
+    among them: order-dependent pairs) and long gap runs at either end."""
+    codes = gen.c2(seed=seed, n=n, L=L)
+    rs = gen.Stream(seed + 1)
+    gen._ambiguities(rs, codes, 0.002, gen._C1_POOL)
+    gen._gap_runs(rs, codes, 0.4, 50, 3000)
+    return codes
+
+
+LONG_CONFIGS = [
+    ("-t", "1", "-a", "resolve", "-g", "0.05", "-l", "25000"),
+    ("-t", "0.001", "-a", "resolve", "-l", "60000"),
+    ("-t", "1", "-a", "average", "-l", "25000"),
+    ("-t", "1", "-a", "skip", "-l", "25000", "-f", "csvn"),
+    ("-t", "1", "-a", "gapmm", "-l", "25000"),
+    ("-t", "1", "-a", "RYN", "-g", "0.5", "-l", "25000"),
+]
+
+
+@pytest.mark.parametrize("L", [65534, 65535, 70000])
+def test_long_alignment_over_65535_vs_oracle(L, tmp_path):
+    """Alignments at and beyond 65,535 columns, where the reference leaves its packed kernels for the per-pair
+    slow path (swifttn93.cpp:580-590): 48 sequences, every -a mode, every pair written, against the oracle."""
+    codes = _long_codes(140 + L % 7, 48, L)
+    fa = tmp_path / "in.fa"
+    gen.write_fasta(fa, codes)
+    cases, want = [], []
+    for ci, cfg in enumerate(LONG_CONFIGS):
+        d = tmp_path / f"k{ci}"
+        d.mkdir()
+        p = subprocess.run([str(_oracle_cli()), "-q", *cfg, "-o", str(d / "want.txt"), str(fa)],
+                           capture_output=True, check=True, timeout=600)
+        want.append(((d / "want.txt").read_bytes(), p.stdout))
+        cases.append((d, ["-q", *cfg, "-o", "out.txt", str(fa)]))
+    for (d, argv), (rc, out, so), (wo, ws) in zip(cases, run_jobs(cases, tmp_path), want):
+        assert rc == 0, argv
+        assert out == wo, (L, argv, len(out or b""), len(wo))
+        assert so == ws, (L, argv, so[:400], ws[:400])
+
+
+def _oracle_matrix(codes, mode, min_overlap, rf):
+    lib = ctypes.CDLL(str(REPO / "oracle" / "_build" / "liboracle.so"))
+    lib.oracle_init.argtypes = [ctypes.c_char_p]
+    lib.oracle_triangle.argtypes = [ctypes.c_void_p, ctypes.c_long, ctypes.c_long, ctypes.c_int, ctypes.c_long,
+                                    ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p]
+    lib.oracle_init(None)
+    c = np.ascontiguousarray(codes, dtype=np.uint8)
+    n, L = c.shape
+    out = np.zeros((n, n))
+    lib.oracle_triangle(c.ctypes.data, n, L, mode, min_overlap, rf, out.ctypes.data, None)
+    return out
+
+
+def _check_served(reqs, everything, thr, mind=-1e-10):
+    want = _oracle_matrix(everything, 0, 500, 0.05)
+    for r in reqs:
+        assert r.error is None, r.error
+        cols = np.arange(r.n1, r.n2)
+        rows_e, cols_e, d_e = [], [], []
+        for i in range(r.n2):  # tn93's order: rows ascending, then columns
+            js = cols[cols > i]
+            d = want[i, js]
+            m = (d >= mind) & (d <= thr)
+            rows_e += [i] * int(m.sum())
+            cols_e += list(js[m])
+            d_e += list(d[m])
+        res = r.result
+        assert list(res["link_row"]) == rows_e and list(res["link_col"]) == cols_e, r.rid
+        assert ["%g" % x for x in res["link_dist"]] == ["%g" % x for x in d_e], r.rid
+
+
+@pytest.mark.parametrize("merge_rows", [0, 1024])
+def test_serving_vs_oracle(merge_rows):
+    """Served batches (appended to a 1,200-sequence device panel, each scored against the panel and all earlier
+    queries) against the oracle's all-pairs triangle over everything received, restricted to each batch's
+    columns; with merging, batches waiting together share one APPEND run and are split back by column."""
+    from swifttn93_amd import serve
+    panel = gen.c1s(1200, seed=91)
+    # arrivals within a few milliseconds: batches wait behind the first runs, so a free worker finds several
+    stream = serve.query_stream(panel, seed=92 + merge_rows, rate=2000.0, seconds=0.01, max_batch=120)
+    nq = sum(c.shape[0] for _, c in stream)
+    srv = serve.Server(panel, capacity=1200 + nq, workers=1 if merge_rows else 2, hbm_budget=1 << 30,
+                       run_kw=dict(threshold=0.03), merge_rows=merge_rows)
+    try:
+        reqs = serve.serve(srv, stream)
+    finally:
+        srv.close()
+    print(f"{len(reqs)} batches, {nq} queries, groups {[r.group for r in reqs]}", flush=True)
+    assert len(reqs) >= 6
+    if merge_rows:
+        assert any(r.group > 1 for r in reqs), "no batches were merged"
+    everything = np.concatenate([panel] + [c for _, c in stream])
+    _check_served(reqs, everything, 0.03)
+
+
+def test_serving_full_panel_alone_vs_mixed():
+    """The serving workload's 50,000 x 1,300 panel: a short Poisson stream with merging and 4 workers; every
+    batch's links in the mix = the batch computed alone afterwards (row, column and printed distance)."""
+    from swifttn93_amd import serve
+    panel = gen.c1(n=50000, clusters=50000 * 200 // 6700)
+    stream = serve.query_stream(panel, seed=8, rate=20.0, seconds=1.5)
+    nq = sum(c.shape[0] for _, c in stream)
+    srv = serve.Server(panel, capacity=50000 + nq, workers=4, hbm_budget=32 << 30, merge_rows=2048)
+    try:
+        reqs = serve.serve(srv, stream)
+        assert all(r.error is None for r in reqs), [r.error for r in reqs if r.error]
+        for r in reqs:
+            print(f"request {r.rid}: rows [{r.n1}, {r.n2}) group {r.group}", flush=True)
+            alone = srv.run_alone(r)
+            for k in ("link_row", "link_col"):
+                assert (alone[k] == r.result[k]).all(), (r.rid, k)
+            assert ["%g" % x for x in alone["link_dist"]] == ["%g" % x for x in r.result["link_dist"]], r.rid
+            assert r.result["certified"] or r.result["exact"]
+    finally:
+        srv.close()
