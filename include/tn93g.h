@@ -141,6 +141,28 @@ int tn93g_run_bytes(tn93g_ctx *ctx, const tn93g_set *set, const tn93g_params *pa
 int tn93g_run(tn93g_ctx *ctx, const tn93g_set *set, const tn93g_params *params, tn93g_result **out);
 void tn93g_result_free(tn93g_result *res);
 
+/* Scratch of tn93g_run_noalloc: one device array of this many bytes per GPU, held from the library's loading until
+ * the process ends. */
+#define TN93G_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+
+/* tn93g_run without any allocation: no counterpart in the reference.  For callers that cannot allocate device or
+ * host memory during a run (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Every work buffer of the run (chunk slots, replay list, row accumulators, link regions, histogram replicas,
+ * counters, event tables) is carved from the device's TN93G_NOALLOC_SCRATCH_BYTES scratch array; read-backs pass
+ * through fixed buffers on the caller's stack.  The caller passes the result: `res` is overwritten, except that its
+ * link_row, link_col, link_dist, link_dist_lo and link_dist_hi must point to link_capacity entries each, which
+ * receive the links.  The values are those tn93g_run gives (the same kernels; the mean's enclosure may differ in
+ * its last bits, both contain tn93's value).
+ * Fails with TN93G_E_ARG, writing no link, and tn93g_error names the reason, when: the run's buffers exceed the
+ * scratch (a triangle of up to about 3,400 sequences of fewer than 2,048 columns fits); it has more links than
+ * link_capacity or than the scratch's link regions hold; it is an exact-path run (TN93G_FLAG_EXACT, TN93G_FST); or an
+ * output decision needs logarithms resolved on the host.  In each case tn93g_run computes it.  With
+ * TN93G_BACKEND_AUTO the backend is the context's tuned choice for the shape, else popcount (no trial runs).
+ * Calls share their device's scratch, which keeps the last call's contents (a run clears what it needs cleared):
+ * calls that overlap on one GPU, from any thread or context, race and must be serialised by the caller. */
+int tn93g_run_noalloc(tn93g_ctx *ctx, const tn93g_set *set, const tn93g_params *params, tn93g_result *res,
+                      int64_t link_capacity);
+
 /* 1 when every link distance of the result prints the same with "%.{precision}g" at both ends of its enclosure
  * (always for an exact result), 0 otherwise, or TN93G_E_ARG.  tn93 prints distances with %g (precision 6,
  * swifttn93.cpp:2575-2585 through fastfmt.h:75): a caller that prints them, or hands them on as tn93's values, and gets 0

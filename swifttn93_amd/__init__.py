@@ -43,7 +43,8 @@ class Result(ctypes.Structure):
 
 EXPORTS = ["tn93g_open", "tn93g_close", "tn93g_error", "tn93g_set_upload", "tn93g_set_free", "tn93g_run",
            "tn93g_result_free", "tn93g_version", "tn93g_run_batch", "tn93g_set_create", "tn93g_set_append",
-           "tn93g_set_size", "tn93g_run_bytes", "tn93g_result_certify"]
+           "tn93g_set_size", "tn93g_run_bytes", "tn93g_result_certify", "tn93g_run_noalloc"]
+NOALLOC_SCRATCH_BYTES = 256 << 20  # TN93G_NOALLOC_SCRATCH_BYTES
 
 _lib = None
 
@@ -75,6 +76,8 @@ def load():
     lib.tn93g_run_batch.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
                                     ctypes.POINTER(Params), ctypes.POINTER(ctypes.POINTER(Result))]
     lib.tn93g_result_certify.argtypes = [ctypes.POINTER(Result), ctypes.c_int]
+    lib.tn93g_run_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(Params), ctypes.POINTER(Result),
+                                      ctypes.c_int64]
     lib.tn93g_version.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -138,6 +141,32 @@ class Engine:
             r = ctypes.POINTER(Result)()
             self._check(self.lib.tn93g_run(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r)))
         return _to_py(r, _ResultOwner(self.lib, r))
+
+    def run_noalloc(self, seqset, link_capacity=1 << 16, **kw):
+        """tn93g_run_noalloc: the run in the device's fixed scratch, its links written into arrays allocated here
+        beforehand (the library allocates nothing).  Raises with the library's message when the run does not fit."""
+        p = make_params(**kw)
+        arrs = [np.zeros(max(link_capacity, 1), dtype=t) for t in (np.int64, np.int64, np.float64, np.float64,
+                                                                   np.float64)]
+        r = Result()
+        r.link_row = arrs[0].ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        r.link_col = arrs[1].ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        r.link_dist = arrs[2].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        r.link_dist_lo = arrs[3].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        r.link_dist_hi = arrs[4].ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        self._check(self.lib.tn93g_run_noalloc(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r),
+                                               link_capacity))
+        n = r.n_links
+        return {
+            "exact": bool(r.exact), "certified": self.lib.tn93g_result_certify(ctypes.byref(r), 6) == 1,
+            "hist": np.frombuffer(r.hist, dtype=np.uint64).reshape(3, 200).copy(),
+            "link_row": arrs[0][:n], "link_col": arrs[1][:n], "link_dist": arrs[2][:n], "link_lo": arrs[3][:n],
+            "link_hi": arrs[4][:n],
+            "mean_lo": list(r.mean_lo), "mean_hi": list(r.mean_hi), "weighted": list(r.weighted),
+            "max_lo": list(r.max_lo), "max_hi": list(r.max_hi), "found_links": r.found_links, "skipped": r.skipped,
+            "n_replayed": r.n_replayed, "n_uncertain": r.n_uncertain, "n_host_exact": r.n_host_exact,
+            "gpu_ms": r.gpu_ms, "pair_kernel_ms": r.pair_kernel_ms, "tail_kernel_ms": r.tail_kernel_ms,
+        }
 
     def run_batch(self, jobs):
         """jobs: list of (SeqSet, dict of run() keyword arguments).  One tn93g_run_batch call; a list of results."""

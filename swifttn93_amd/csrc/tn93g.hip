@@ -31,6 +31,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <mutex>
 #include <atomic>
@@ -2484,6 +2485,35 @@ struct RunBufs {
 
 constexpr int NBRANCH = 8;  // parallel branches of a batch graph
 
+// The work memory of tn93g_run_noalloc: one file-scope device array, of which every GPU the library runs on holds its
+// own copy from the loading of the library until the process ends.  In host code its name is only a stand-in: the
+// array's address on a device comes from hipGetSymbolAddress with that device current (g_scratch below, taken once
+// per device when a context opens), and only that address is ever passed to kernels or copies.
+static __device__ __align__(256) unsigned char scratch[TN93G_NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_SCRATCH_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch[MAX_SCRATCH_DEVICES];
+static hipError_t g_scratch_err[MAX_SCRATCH_DEVICES];
+
+// The device address of `scratch` on `device` (which must be current), looked up once; nullptr and the HIP error
+// when the lookup failed.
+static void *scratch_address(int device, hipError_t *err)
+{
+  *err = hipErrorInvalidDevice;
+  if (device < 0 || device >= MAX_SCRATCH_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[device] && g_scratch_err[device] == hipSuccess) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && ptr)
+      g_scratch[device] = ptr;
+    else
+      g_scratch_err[device] = e == hipSuccess ? hipErrorInvalidSymbol : e;
+  }
+  *err = g_scratch_err[device];
+  return g_scratch[device];
+}
+
 struct tn93g_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -2506,6 +2536,8 @@ struct tn93g_ctx {
   // range lists; and the staging of the batch's read-back
   DevBuf multi, stage;
   int wall_khz = 0;  // rate of the device's constant wall clock (the stamps around a batch's merged pair kernels)
+  void *scratch = nullptr;  // device address of this device's `scratch` (tn93g_run_noalloc), set when opened
+  hipError_t scratch_err = hipSuccess;
 
   RunBufs *slot(size_t k)
   {
@@ -2631,12 +2663,30 @@ static bool fold_entry(int a, int b, bool gapmm, uint32_t q[4])
   return true;
 }
 
-static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
+// A fixed device region that a run's buffers are carved from in order (the no-allocation entry point).
+struct Arena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+};
+
+static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena *arena = nullptr)
 {
-  // with dry_bytes: only the sizes of the run's buffers are added up, nothing is allocated or uploaded
+  // with dry_bytes: only the sizes of the run's buffers are added up, nothing is allocated or uploaded; with arena:
+  // every buffer is carved from the arena (no allocation), and a run that does not fit fails
   auto need = [&](DevBuf &b, size_t bytes) -> int {
     if (dry_bytes) {
       *dry_bytes += std::max(bytes, (size_t)256);
+      return 0;
+    }
+    if (arena) {
+      const size_t want = (std::max(bytes, (size_t)256) + 255) & ~(size_t)255;
+      if (arena->used + want > arena->size)
+        return c->fail(TN93G_E_ARG, fmt("the run needs more than the %zu bytes of scratch of the no-allocation entry "
+                                        "point (%zu already placed, %zu more asked); use tn93g_run",
+                                        arena->size, arena->used, want));
+      b.p = arena->base + arena->used;
+      b.n = want;
+      arena->used += want;
       return 0;
     }
     return ensure(c, b, bytes);
@@ -2724,8 +2774,9 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   if (J.npairs >= (1ll << 32)) return c->fail(TN93G_E_ARG, "more than 2^32 pairs in one run");
   // tables: [4][18*18] packed event entries (main mode, AVERAGE; then both folded), [2][18*18] replay entries
   if (!dry_bytes && (B->tab_mode != P->mode || B->tab_mask != P->subset_mask || !B->evtab.p || !B->rtab.p)) {
-    std::vector<uint4> et(4 * NCODE * NCODE);
-    std::vector<ReplayEnt> rt(2 * NCODE * NCODE);
+    // built on the stack (31 KB): copied in the context's stream order, which is synchronised before they go
+    std::array<uint4, 4 * NCODE * NCODE> et;
+    std::array<ReplayEnt, 2 * NCODE * NCODE> rt;
     bool fold_ok = true, gcert = true;
     for (int a = 0; a < NCODE; a++)
       for (int b = 0; b < NCODE; b++) {
@@ -2745,10 +2796,12 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
           r.amb = (uint16_t)F.amb;
         }
       }
-    if (need(B->evtab, et.size() * sizeof(uint4)) || need(B->rtab, rt.size() * sizeof(ReplayEnt)))
-      return TN93G_E_NOMEM;
+    int rc;
+    if ((rc = need(B->evtab, et.size() * sizeof(uint4))) || (rc = need(B->rtab, rt.size() * sizeof(ReplayEnt))))
+      return rc;
     HIPCHK(c, hipMemcpyAsync(B->evtab.p, et.data(), et.size() * sizeof(uint4), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(B->rtab.p, rt.data(), rt.size() * sizeof(ReplayEnt), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     B->tab_mode = P->mode;
     B->tab_mask = P->subset_mask;
     B->tab_fold = fold_ok;
@@ -2776,19 +2829,26 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
       need(B->counters, STRIPE_BYTES) ||
       need(B->rowout, nr1 * std::max<size_t>(15 * 8 + 16, sizeof(RowAcc))) ||
       need(B->chdr, (size_t)nslots * sizeof(ChunkHdr)) ||
-      need(B->cdata, (size_t)nslots * 7 * 64 * sizeof(uint32_t)))
-    return TN93G_E_NOMEM;
+      need(B->cdata, (size_t)nslots * 7 * 64 * sizeof(uint32_t)))  // slots are 7 x 64 words apart in both layouts
+    return arena ? TN93G_E_ARG : TN93G_E_NOMEM;
   J.exact_cap = A.exact ? (unsigned)std::min<int64_t>(J.npairs, 1ll << 31) : 1u;
-  if (need(B->exact_list, (size_t)J.exact_cap * sizeof(ExactRec))) return TN93G_E_NOMEM;
+  if (need(B->exact_list, (size_t)J.exact_cap * sizeof(ExactRec))) return arena ? TN93G_E_ARG : TN93G_E_NOMEM;
   J.link_cap = 1;
   if (J.fast) {
-    // links per stripe region; a run that overflows a region is repeated with larger regions
-    const int64_t want = std::max<int64_t>(B->link_cap_hint, 1 << 15);
+    // links per stripe region; a run that overflows a region is repeated with larger regions (in an arena: the
+    // regions take what is left of it, up to the usual size, and an overflow is an error)
+    const size_t per_link = NSTRIPE * (2 * sizeof(LinkRec) + sizeof(uint32_t)), slack = 3 * 256;
+    int64_t want = std::max<int64_t>(B->link_cap_hint, 1 << 15);
+    if (arena) {
+      const size_t left = arena->size - arena->used;
+      want = std::min<int64_t>(want, left > slack ? (int64_t)((left - slack) / per_link) : 0);
+      if (want < 1) return need(B->links, per_link + slack + left);  // fails with the arena's message
+    }
     J.link_cap = (unsigned)std::min<int64_t>(std::max<int64_t>(J.npairs, 1), want);
     if (need(B->links, (size_t)J.link_cap * NSTRIPE * sizeof(LinkRec)) ||
         need(B->links_flat, (size_t)J.link_cap * NSTRIPE * sizeof(LinkRec)) ||
         need(B->lslot, (size_t)J.link_cap * NSTRIPE * sizeof(uint32_t)))
-      return TN93G_E_NOMEM;
+      return arena ? TN93G_E_ARG : TN93G_E_NOMEM;
   }
   A.D = (double *)B->D.p;
   A.U = (uint16_t *)B->U.p;
@@ -2816,7 +2876,8 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr)
   A.tab_row = A.tab_col = nullptr;
   A.spec_rows = nullptr;
   A.nsr = A.nsc = 0;
-  if (J.fast && !A.exact && !A.all_slow && P->mode != TN93G_SKIP && nrows > 0 && A.ncols > 0 && A.W >= 128) {
+  if (J.fast && !A.exact && !A.all_slow && P->mode != TN93G_SKIP && nrows > 0 && A.ncols > 0 && A.W >= 128 &&
+      !arena) {  // (in an arena the replay counts per pair: the tables' host index lists would allocate)
     const bool same = A.r0 == A.c0 && A.r1 == A.c1;
     std::vector<int32_t> &idx = B->h_spec_idx;
     std::vector<int64_t> &ids = B->h_spec_ids;
@@ -2910,13 +2971,21 @@ static bool launch_pairs_multi(int v, unsigned nblocks, hipStream_t st, const Ru
 // their results in with atomics), joined back before the link gather.  With `timing`, events around the pair and
 // tail kernels give the device times of a single run.
 // The buffers a run starts from zero (histogram replicas, counters, per-row link counts, row accumulators).
-static void job_fill_ranges(const Job &J, std::vector<FillRange> &out)
+static int job_fill_list(const Job &J, FillList &L)
 {
   RunBufs *B = J.B;
-  out.push_back(FillRange{(uint32_t *)B->hist.p, (int64_t)NSTRIPE * 600 * 2});
-  out.push_back(FillRange{(uint32_t *)B->counters.p, (int64_t)(STRIPE_BYTES / 4)});
-  out.push_back(FillRange{(uint32_t *)B->row_links.p, (int64_t)J.nr1});
-  if (J.fast) out.push_back(FillRange{(uint32_t *)B->rowout.p, (int64_t)(J.nr1 * sizeof(RowAcc) / 4)});
+  L.r[0] = FillRange{(uint32_t *)B->hist.p, (int64_t)NSTRIPE * 600 * 2};
+  L.r[1] = FillRange{(uint32_t *)B->counters.p, (int64_t)(STRIPE_BYTES / 4)};
+  L.r[2] = FillRange{(uint32_t *)B->row_links.p, (int64_t)J.nr1};
+  if (!J.fast) return 3;
+  L.r[3] = FillRange{(uint32_t *)B->rowout.p, (int64_t)(J.nr1 * sizeof(RowAcc) / 4)};
+  return 4;
+}
+static void job_fill_ranges(const Job &J, std::vector<FillRange> &out)
+{
+  FillList L{};
+  const int nf = job_fill_list(J, L);
+  for (int k = 0; k < nf; k++) out.push_back(L.r[k]);
 }
 
 // fills: false when a batch clears every job's buffers with one k_fill_ranges launch
@@ -2926,11 +2995,9 @@ static void job_enqueue_pre(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, b
   const int64_t nrows = J.nrows;
   if (timing) (void)hipEventRecord(c->ev[4], st);
   if (fills) {  // one launch for the (at most 4) ranges
-    std::vector<FillRange> fr;
-    job_fill_ranges(J, fr);
     FillList L{};
-    for (size_t k = 0; k < fr.size() && k < 4; k++) L.r[k] = fr[k];
-    hipLaunchKernelGGL(k_fill_list, dim3(64, (unsigned)std::min<size_t>(fr.size(), 4)), dim3(256), 0, st, L);
+    const int nf = job_fill_list(J, L);
+    hipLaunchKernelGGL(k_fill_list, dim3(64, (unsigned)nf), dim3(256), 0, st, L);
   }
   if (A.nsr)
     hipLaunchKernelGGL(k_spec16, dim3((unsigned)((A.ncols + TJ - 1) / TJ), (unsigned)((A.nsr + TI - 1) / TI)),
@@ -3002,8 +3069,8 @@ static int job_enqueue_counters(tn93g_ctx *c, Job &J, char *pin)
 
 static void job_parse_counters(Job &J, const char *pin)
 {
-  std::vector<unsigned> st((size_t)NSTRIPE * 32);
-  memcpy(st.data(), pin, STRIPE_BYTES);
+  unsigned st[NSTRIPE * 32];
+  memcpy(st, pin, STRIPE_BYTES);
   int64_t slow_total = 0;
   memcpy(&slow_total, pin + ST_SLOW * 4, 8);
   J.n_slow = (uint64_t)slow_total;
@@ -3115,39 +3182,24 @@ static void free_links(int64_t *link_row)
   free(blk);
 }
 
-// The fast path's result from its read-back: links in tn93's order, the enclosures of the mean and the maximum.
-static int job_fast_result(const Job &J, const char *pin, const LinkRec *lk, tn93g_result *R)
-{
-  const FastLayout F = fast_layout(J, 0);
-  const RowAcc *ra = (const RowAcc *)(pin + F.off_ra);
-  const size_t nl = J.n_linkrec;
-  const int64_t nrows = J.nrows, r0 = J.A.r0;
-  // the records arrive in tn93's emission order (rows ascending, columns ascending: k_link_order)
-  if (alloc_links(R, nl)) return TN93G_E_NOMEM;
-  for (size_t k = 0; k < nl; k++) {
-    const LinkRec &q = lk[k];
-    R->link_row[k] = q.i;
-    R->link_col[k] = q.j;
-    R->link_dist[k] = q.d;
-    R->link_dist_lo[k] = q.lo;
-    R->link_dist_hi[k] = q.hi;
-  }
-  // Enclosure of tn93's mean (swifttn93.cpp:2594-2602, 2839-2848).  Row r: tn93 adds its n_r terms t in column
-  // order in FP64; here they were added in another order.  Both results are within gamma_{n_r} * sum|t| of the
-  // exact sum (Higham, Accuracy and Stability of Numerical Algorithms, 2nd ed., eq. 4.4), so tn93's row sum lies in
-  // [slo - 2 g sabs', shi + 2 g sabs'] with sabs' an upper bound of sum|t|.  The rows are then added in row order:
-  // again within gamma_R * sum |row sums|.
-  // Host arithmetic is double.  For n u <= 2^-22 (n < 2^31), gamma_n <= n u (1 + 2^-21) <= fl(n cu) with
-  // cu = u (1 + 2^-20); K = 1 + 2^-20 covers the factor (1 + g) and the roundings of each product.  Adding the
-  // bounds lo_r, hi_r up in double is again within gamma_{R+1} sum max(|lo_r|, |hi_r|) (their own rounding
-  // included), so eR below covers both tn93's row-order sum and this one; mabs itself is inflated by K.
-  const double cu = 0x1.00001p-53, K = 1.0 + 0x1p-20;
-  double mlo = 0, mhi = 0, mabs = 0;
-  double maxlo = 0., maxhi = 0.;
+// Enclosure of tn93's mean (swifttn93.cpp:2594-2602, 2839-2848), the maximum and the integer sums, from the fast
+// path's row accumulators added in row order.  Row r: tn93 adds its n_r terms t in column order in FP64; here they
+// were added in another order.  Both results are within gamma_{n_r} * sum|t| of the exact sum (Higham, Accuracy and
+// Stability of Numerical Algorithms, 2nd ed., eq. 4.4), so tn93's row sum lies in [slo - 2 g sabs', shi + 2 g sabs']
+// with sabs' an upper bound of sum|t|.  The rows are then added in row order: again within gamma_R * sum |row sums|.
+// Host arithmetic is double.  For n u <= 2^-22 (n < 2^31), gamma_n <= n u (1 + 2^-21) <= fl(n cu) with
+// cu = u (1 + 2^-20); K = 1 + 2^-20 covers the factor (1 + g) and the roundings of each product.  Adding the
+// bounds lo_r, hi_r up in double is again within gamma_{R+1} sum max(|lo_r|, |hi_r|) (their own rounding
+// included), so eR below covers both tn93's row-order sum and this one; mabs itself is inflated by K.
+struct RowMerge {
+  const Job &J;
+  double mlo = 0, mhi = 0, mabs = 0, maxlo = 0., maxhi = 0.;
   long long wsum = 0, skip = 0, linkw = 0;
-  for (int64_t r = 0; r < nrows; r++) {
-    const RowAcc &q = ra[(size_t)r];
-    const int64_t i = r0 + r;
+  static constexpr double cu = 0x1.00001p-53, K = 1.0 + 0x1p-20;
+  explicit RowMerge(const Job &j) : J(j) {}
+  void add(const RowAcc &q, int64_t r)  // row r0 + r of the run; rows must come in order
+  {
+    const int64_t i = J.A.r0 + r;
     const double n_r = (double)(J.A.triangle ? (J.A.c1 - std::max(J.A.c0, i + 1)) : J.A.ncols);
     const double g = std::max(n_r, 1.0) * cu;
     const double e = (2.0 * g) * q.sabs * K;
@@ -3162,22 +3214,46 @@ static int job_fast_result(const Job &J, const char *pin, const LinkRec *lk, tn9
     skip += q.skip;
     linkw += q.linkw;
   }
-  const double gR = ((double)std::max<int64_t>(nrows, 1) + 1.0) * cu;
-  const double eR = (2.0 * gR) * (mabs * K) * K + 0x1p-50 * mabs;
-  R->mean_lo[0] = std::nextafter(mlo - eR, -INFINITY);
-  R->mean_hi[0] = std::nextafter(mhi + eR, INFINITY);
-  R->max_lo[0] = maxlo;
-  R->max_hi[0] = maxhi;
-  R->skipped = skip;
-  R->found_links = linkw;
-  long long w_all = wsum;
-  if (J.A.triangle) {
-    for (int64_t i = J.A.c0; i < J.A.n; i++) {
-      const int64_t wi = J.S->hw[(size_t)i];
-      if (wi > 1) w_all += (wi * (wi - 1)) >> 1;
+  void finish(tn93g_result *R) const
+  {
+    const double gR = ((double)std::max<int64_t>(J.nrows, 1) + 1.0) * cu;
+    const double eR = (2.0 * gR) * (mabs * K) * K + 0x1p-50 * mabs;
+    R->mean_lo[0] = std::nextafter(mlo - eR, -INFINITY);
+    R->mean_hi[0] = std::nextafter(mhi + eR, INFINITY);
+    R->max_lo[0] = maxlo;
+    R->max_hi[0] = maxhi;
+    R->skipped = skip;
+    R->found_links = linkw;
+    long long w_all = wsum;
+    if (J.A.triangle) {
+      for (int64_t i = J.A.c0; i < J.A.n; i++) {
+        const int64_t wi = J.S->hw[(size_t)i];
+        if (wi > 1) w_all += (wi * (wi - 1)) >> 1;
+      }
     }
+    R->weighted[0] = (double)w_all;
   }
-  R->weighted[0] = (double)w_all;
+};
+
+// The fast path's result from its read-back: links in tn93's order, the enclosures of the mean and the maximum.
+static int job_fast_result(const Job &J, const char *pin, const LinkRec *lk, tn93g_result *R)
+{
+  const FastLayout F = fast_layout(J, 0);
+  const RowAcc *ra = (const RowAcc *)(pin + F.off_ra);
+  const size_t nl = J.n_linkrec;
+  // the records arrive in tn93's emission order (rows ascending, columns ascending: k_link_order)
+  if (alloc_links(R, nl)) return TN93G_E_NOMEM;
+  for (size_t k = 0; k < nl; k++) {
+    const LinkRec &q = lk[k];
+    R->link_row[k] = q.i;
+    R->link_col[k] = q.j;
+    R->link_dist[k] = q.d;
+    R->link_dist_lo[k] = q.lo;
+    R->link_dist_hi[k] = q.hi;
+  }
+  RowMerge M(J);
+  for (int64_t r = 0; r < J.nrows; r++) M.add(ra[(size_t)r], r);
+  M.finish(R);
   unsigned long long hrep[600];
   memcpy(hrep, pin + F.off_h, sizeof hrep);
   job_finish_common(J, hrep, R);
@@ -3329,6 +3405,8 @@ int tn93g_open(int device, tn93g_ctx **out)
     tn93g_close(c);
     return TN93G_E_NODEV;
   }
+  // the no-allocation entry point's scratch on this device (a failed lookup only disables that entry point)
+  c->scratch = scratch_address(device, &c->scratch_err);
   *out = c;
   return 0;
 }
@@ -3786,6 +3864,99 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   }
   set_times(c, R, true);
   *out = R;
+  return 0;
+}
+
+int tn93g_run_noalloc(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_result *R,
+                      int64_t link_capacity)
+{
+  if (!c || !S || !P || !R || link_capacity < 0) return TN93G_E_ARG;
+  if (link_capacity > 0 &&
+      (!R->link_row || !R->link_col || !R->link_dist || !R->link_dist_lo || !R->link_dist_hi))
+    return c->fail(TN93G_E_ARG, "tn93g_run_noalloc: the result's five link arrays must hold link_capacity entries");
+  if (!c->scratch)
+    return c->fail(TN93G_E_HIP, fmt("tn93g_run_noalloc: no scratch on device %d (hipGetSymbolAddress: %s)", c->device,
+                                    hipGetErrorString(c->scratch_err)));
+  if ((P->flags & TN93G_FLAG_EXACT) || P->kind == TN93G_FST)
+    return c->fail(TN93G_E_ARG, "tn93g_run_noalloc: exact-path runs (TN93G_FLAG_EXACT, TN93G_FST) need host and "
+                                "device work memory: use tn93g_run");
+  HIPCHK(c, hipSetDevice(c->device));
+  Job J;
+  J.S = S;
+  J.P = *P;
+  RunBufs B;  // every buffer points into the scratch; nothing to release
+  J.B = &B;
+  Arena ar;
+  ar.base = (char *)c->scratch;
+  ar.size = TN93G_NOALLOC_SCRATCH_BYTES;
+  int rc = job_prepare(c, J, nullptr, &ar);
+  if (rc) return rc;
+  if (J.P.backend == TN93G_BACKEND_AUTO) {  // the tuned choice of this shape when there is one (no trial runs here)
+    auto it = c->tuned.find(tune_shape(J));
+    J.P.backend = it != c->tuned.end() ? it->second : TN93G_BACKEND_POPC;
+  }
+  job_enqueue(c, J, true, c->stream, 0);
+  HIPCHK(c, hipGetLastError());
+  // read-backs through fixed host buffers on the stack, each copy completed before the buffer is reused
+  char cnt[STRIPE_BYTES];
+  HIPCHK(c, hipMemcpyAsync(cnt, B.counters.p, STRIPE_BYTES, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  job_parse_counters(J, cnt);
+  if (J.n_undecided)
+    return c->fail(TN93G_E_ARG, "tn93g_run_noalloc: an output decision needs logarithms resolved on the host, which "
+                                "needs work memory: use tn93g_run");
+  if (J.max_region > J.link_cap)
+    return c->fail(TN93G_E_ARG, fmt("tn93g_run_noalloc: %u links in one region, more than the scratch holds (%u): "
+                                    "use tn93g_run", J.max_region, J.link_cap));
+  if ((int64_t)J.n_linkrec > link_capacity)
+    return c->fail(TN93G_E_ARG, fmt("tn93g_run_noalloc: %llu links, more than link_capacity (%lld)",
+                                    (unsigned long long)J.n_linkrec, (long long)link_capacity));
+  int64_t *lrow = R->link_row, *lcol = R->link_col;
+  double *ld = R->link_dist, *llo = R->link_dist_lo, *lhi = R->link_dist_hi;
+  memset(R, 0, sizeof *R);
+  R->link_row = lrow;
+  R->link_col = lcol;
+  R->link_dist = ld;
+  R->link_dist_lo = llo;
+  R->link_dist_hi = lhi;
+  const size_t nl = J.n_linkrec;
+  R->n_links = (int64_t)nl;
+  {
+    constexpr size_t CH = 1536;  // 60 KB of link records per copy
+    LinkRec buf[CH];
+    for (size_t k0 = 0; k0 < nl; k0 += CH) {
+      const size_t m = std::min(CH, nl - k0);
+      HIPCHK(c, hipMemcpyAsync(buf, (const LinkRec *)B.links.p + k0, m * sizeof(LinkRec), hipMemcpyDeviceToHost,
+                               c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (size_t k = 0; k < m; k++) {
+        lrow[k0 + k] = buf[k].i;
+        lcol[k0 + k] = buf[k].j;
+        ld[k0 + k] = buf[k].d;
+        llo[k0 + k] = buf[k].lo;
+        lhi[k0 + k] = buf[k].hi;
+      }
+    }
+  }
+  {
+    constexpr int64_t CH = 1024;  // 64 KB of row accumulators per copy, merged in row order
+    RowAcc buf[CH];
+    RowMerge M(J);
+    for (int64_t r0 = 0; r0 < J.nrows; r0 += CH) {
+      const int64_t m = std::min(CH, J.nrows - r0);
+      HIPCHK(c, hipMemcpyAsync(buf, (const RowAcc *)B.rowout.p + r0, (size_t)m * sizeof(RowAcc),
+                               hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      for (int64_t r = 0; r < m; r++) M.add(buf[r], r0 + r);
+    }
+    M.finish(R);
+  }
+  unsigned long long hrep[600];
+  HIPCHK(c, hipMemcpyAsync(hrep, B.hist.p, sizeof hrep, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  job_finish_common(J, hrep, R);
+  set_times(c, R, true);
   return 0;
 }
 

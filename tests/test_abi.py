@@ -19,7 +19,7 @@ def test_header_declares_the_entry_points():
     assert _declared() == sorted(["tn93g_open", "tn93g_close", "tn93g_error", "tn93g_set_upload", "tn93g_set_free",
                                   "tn93g_run", "tn93g_result_free", "tn93g_version", "tn93g_run_batch",
                                   "tn93g_set_create", "tn93g_set_append", "tn93g_set_size", "tn93g_run_bytes",
-                                  "tn93g_result_certify"])
+                                  "tn93g_result_certify", "tn93g_run_noalloc"])
 
 
 def test_library_exports_every_declared_symbol():
