@@ -17,11 +17,22 @@ Other workloads (--config): c2, the SARS-CoV-2-shaped long alignment (20,000 x 2
 launch) beside the same jobs run one by one, and the 2,000 x 40,000 -s bipartite block; c4, serving: a 50,000 x
 1,300 device-resident panel answering a Poisson stream of query batches (p50/p99 latency, pairs/s).
 
-Prints one JSON line.  `roofline` is for the pair kernel (HIP events on the engine's stream, averaged over the
-timed steps); `cpu_baseline` times the reference solver (oracle/_ref/swifttn93_avx512, built from the reference
-sources by oracle/Makefile) on the same input on this host's cores, or the oracle when that build is absent.
+Prints one JSON line.  Beside `value` (the engine with the set resident) it carries:
+  steady_state     the whole tn93 command in a warm process (tn93g's --batch driver, the same invocation repeated):
+                   FASTA read (page cache), packing and upload, the run, certification of the printed values,
+                   formatting and the distance file written; cold_cli: the same as a fresh process (device init
+                   included), which is what the reference's wall time measures;
+  parity           sha256 of the GPU command's distance file and stdout against the reference solver's on the same
+                   input and flags (every C1 mode, C2, each C3 job and the -s block, C4's served links against the
+                   reference's triangle over everything received);
+  roofline         for the pair kernel (HIP events on the engine's stream, averaged over the timed steps);
+  cpu_baseline     the reference solver (oracle/_ref/swifttn93_avx512, built from the reference sources by
+                   oracle/Makefile) on the same input on this host's cores: best of 3 full-command walls with the
+                   threads of the process's affinity set (and OMP_NUM_THREADS when set and different), and one run on
+                   1 thread; with the host's CPU model, core counts and AVX-512 flags.
 """
 import argparse
+import hashlib
 import json
 import os
 import subprocess
@@ -32,13 +43,22 @@ from pathlib import Path
 
 REPO = Path(__file__).resolve().parent
 sys.path.insert(0, str(REPO))
+SEP = "\x1f"
+
 
 def mode_flags(mode):
     """tn93's flags for the C1 workload in -a mode (-g 0.05 with resolve, as the headline line is quoted)."""
     return ["-t", "0.015", "-a", mode] + (["-g", "0.05"] if mode == "resolve" else []) + ["-l", "500", "-f", "csv"]
+
+
 VALU_INT32_TOPS = 256 * 64 * 2.4e9 / 1e12  # 39.3: 256 CUs x 64 lanes x 2.4 GHz
 MFMA_INT8_TOPS = 5000.0
 HBM_BYTES_S = 8.0e12
+REF_VARIANT = ("swifttn93 built from the reference sources (oracle/Makefile): g++ -O3 -march=x86-64-v4 "
+               "-mavx512vpopcntdq -mavx512vbmi -mavx512vbmi2 -mavx512ifma -mno-avx512dq -fopenmp -static; its "
+               "binary-derived vector log (vlog.h, vlog_data.h) is excluded, which also leaves out its AVX512DQ "
+               "code paths, so logarithms are libm's; the reference's own -march=native build with that file "
+               "measured about 2.3x faster on the 8 cores where the builds were made")
 
 
 def profiled_traffic(kernel):
@@ -54,44 +74,36 @@ def profiled_traffic(kernel):
     return None
 
 
-def cpu_baseline(fa, n_pairs, mode):
-    ref = REPO / "oracle" / "_ref" / "swifttn93_avx512"
-    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-    if ref.exists():
-        kind, cmd = "reference", [str(ref), "-q"]
-    else:
-        kind, cmd = "port", [str(REPO / "oracle" / "_build" / "tn93_oracle"), "-q"]
-        if not Path(cmd[0]).exists():
-            return None
-    best = None
-    with tempfile.TemporaryDirectory() as td:
-        out = Path(td) / "out.csv"
-        for _ in range(3):
-            t0 = time.perf_counter()
-            subprocess.run(cmd + ["-o", str(out)] + mode_flags(mode) + [str(fa)], check=True, env=env,
-                           stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=600)
-            dt = time.perf_counter() - t0
-            best = dt if best is None else min(best, dt)
-            out.unlink()
-    return {"value": n_pairs / best, "unit": "pairs/s", "cores": threads, "kind": kind,
-            "sample": f"whole workload (6,700 x 1,300, {n_pairs} pairs), full command wall incl. read and write, "
-                      f"best of 3, OMP_NUM_THREADS={threads}", "wall_s": best}
+def host_info():
+    info = {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                info["model"] = ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        flags = set()
+        for ln in Path("/proc/cpuinfo").read_text().splitlines():
+            if ln.startswith("flags"):
+                flags = set(ln.split(":", 1)[1].split())
+                break
+        info["avx512"] = sorted(f for f in flags if f.startswith("avx512"))
+    except OSError:
+        pass
+    return info
 
 
-def run_reference(exe, args_list, threads):
-    """Wall time of the reference solver over a list of (argv, fasta) runs, output deleted outside the timer."""
-    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
-    tot = 0.0
-    with tempfile.TemporaryDirectory() as td:
-        out = Path(td) / "out.csv"
-        for argv in args_list:
-            t0 = time.perf_counter()
-            subprocess.run([str(exe), "-q", "-o", str(out)] + argv, check=True, env=env, stdout=subprocess.DEVNULL,
-                           stderr=subprocess.DEVNULL, timeout=1200)
-            tot += time.perf_counter() - t0
-            out.unlink()
-    return tot
+def thread_counts():
+    """Thread counts the baseline is timed with: the affinity set, and OMP_NUM_THREADS when set and different."""
+    aff = len(os.sched_getaffinity(0))
+    out = [aff]
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp and omp.isdigit() and 0 < int(omp) != aff:
+        out.append(int(omp))
+    return out
 
 
 def ref_exe():
@@ -102,13 +114,107 @@ def ref_exe():
     return (port, "port") if port.exists() else (None, None)
 
 
+def sha(b):
+    return hashlib.sha256(b).hexdigest()
+
+
+def run_reference(exe, argvs, threads, keep=None):
+    """Wall time of the reference solver over a list of argv lists (each with -o OUT substituted), output deleted
+    outside the timer; with keep (a list), the sha256 of each run's output file and stdout are appended to it."""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    tot = 0.0
+    with tempfile.TemporaryDirectory() as td:
+        out = Path(td) / "out.csv"
+        for argv in argvs:
+            t0 = time.perf_counter()
+            p = subprocess.run([str(exe), "-q", "-o", str(out)] + argv, check=True, env=env, capture_output=True,
+                               timeout=1200)
+            tot += time.perf_counter() - t0
+            if keep is not None:
+                keep.append((sha(out.read_bytes()), sha(p.stdout)))
+            out.unlink()
+    return tot
+
+
+def cpu_baseline(argvs, n_pairs, sample, one_thread=True):
+    """The reference on the given runs: best of 3 at each thread count of thread_counts(), and once on 1 thread.
+    Returns (baseline dict or None, [(file sha, stdout sha)] of the reference's outputs)."""
+    exe, kind = ref_exe()
+    if exe is None:
+        return None, None
+    keep = []
+    best = None
+    for th in thread_counts():
+        for rep in range(3):
+            w = run_reference(exe, argvs, th, keep if not keep_done(keep, argvs) else None)
+            if best is None or w < best[0]:
+                best = (w, th)
+    b = {"value": n_pairs / best[0], "unit": "pairs/s", "cores": best[1], "kind": kind,
+         "variant": REF_VARIANT if kind == "reference" else "the oracle (scalar CPU restatement)",
+         "sample": sample + f"; best of 3 full-command walls (read + compute + write) at {thread_counts()} threads",
+         "wall_s": best[0], "host": host_info()}
+    if one_thread:
+        w1 = run_reference(exe, argvs, 1)
+        b["one_thread"] = {"value": n_pairs / w1, "wall_s": w1, "cores": 1}
+    return b, keep
+
+
+def keep_done(keep, argvs):
+    return len(keep) >= len(argvs)
+
+
+def tn93g_batch(cases, td, timeout=1200):
+    """Runs `tn93g --batch` (the cases one after the other in one process, each a full tn93 invocation); returns
+    per case (rc, output sha, stdout sha, wall seconds)."""
+    bf = Path(td) / "cases.txt"
+    bf.write_text("".join(SEP.join([str(d)] + list(a)) + "\n" for d, a in cases))
+    subprocess.run([str(REPO / "swifttn93_amd" / "bin" / "tn93g"), "--batch", str(bf)], check=True,
+                   capture_output=True, timeout=timeout)
+    res = []
+    for d, _ in cases:
+        d = Path(d)
+        out = d / "out.csv"
+        res.append((int((d / "rc.txt").read_text()), sha(out.read_bytes()) if out.exists() else None,
+                    sha((d / "stdout.bin").read_bytes()), float((d / "wall.txt").read_text())))
+    return res
+
+
+def tn93g_jobs(cases, td, timeout=1200):
+    """Runs `tn93g --jobs` (all cases prepared, computed as one tn93g_run_batch, then certified and written);
+    returns (per case (rc, output sha, stdout sha), wall seconds of the whole command)."""
+    bf = Path(td) / "jobs.txt"
+    bf.write_text("".join(SEP.join([str(d)] + list(a)) + "\n" for d, a in cases))
+    t0 = time.perf_counter()
+    subprocess.run([str(REPO / "swifttn93_amd" / "bin" / "tn93g"), "--jobs", str(bf)], check=True,
+                   capture_output=True, timeout=timeout)
+    wall = time.perf_counter() - t0
+    res = []
+    for d, _ in cases:
+        d = Path(d)
+        out = d / "out.csv"
+        res.append((int((d / "rc.txt").read_text()), sha(out.read_bytes()) if out.exists() else None,
+                    sha((d / "stdout.bin").read_bytes())))
+    return res, wall
+
+
+def cold_cli(argv, td):
+    """Wall time of one fresh tn93g process (device init, read, pack, run, certify, format, write)."""
+    out = Path(td) / "cold.csv"
+    t0 = time.perf_counter()
+    subprocess.run([str(REPO / "swifttn93_amd" / "bin" / "tn93g"), "-q", "-o", str(out)] + argv, check=True,
+                   capture_output=True, timeout=1200)
+    w = time.perf_counter() - t0
+    out.unlink()
+    return w
+
+
 def roofline_for(n_pairs, L, pair_ms):
     words = (L + 31) // 32
     ops = 22 * words
     achieved = ops * n_pairs / (pair_ms * 1e-3) / 1e12
     return {"bound": "valu", "achieved": achieved, "peak": VALU_INT32_TOPS, "unit": "TOPS",
-            "frac": achieved / VALU_INT32_TOPS, "traffic": None, "kernel": "k_pairs<1, true, true, true>", "kernel_ms": pair_ms,
-            "ops_per_pair": ops}
+            "frac": achieved / VALU_INT32_TOPS, "traffic": None, "kernel": "k_pairs<1, true, true, true>",
+            "kernel_ms": pair_ms, "ops_per_pair": ops}
 
 
 def bench_c2(a, T, gen):
@@ -137,19 +243,34 @@ def bench_c2(a, T, gen):
             "links": int(r["found_links"]), "replayed_pairs": int(r["n_replayed"])}
     s.free()
     eng.close()
-    if not a.no_cpu:
-        exe, kind = ref_exe()
-        if exe:
-            threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
-            with tempfile.TemporaryDirectory() as td:
-                fa = Path(td) / "c2.fa"
-                gen.write_fasta(fa, codes)
-                w = run_reference(exe, [["-t", "0.0005", "-a", "resolve", "-g", "0.05", "-l", "25000", str(fa)]],
-                                  threads)
-            line["cpu_baseline"] = {"value": n_pairs / w, "unit": "pairs/s", "cores": threads, "kind": kind,
-                                    "sample": "whole workload, one full command run (read + compute + write)",
-                                    "wall_s": w}
-            line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    flags = ["-t", "0.0005", "-a", "resolve", "-g", "0.05", "-l", "25000", "-f", "csv"]
+    with tempfile.TemporaryDirectory() as td:
+        fa = Path(td) / "c2.fa"
+        gen.write_fasta(fa, codes)
+        cases = []
+        for k in range(3):  # the first case also pays the device's initialisation
+            d = Path(td) / f"e{k}"
+            d.mkdir()
+            cases.append((d, ["-q", *flags, "-o", "out.csv", str(fa)]))
+        res = tn93g_batch(cases, td)
+        warm = float(np.median([w for _, _, _, w in res[1:]]))
+        line["steady_state"] = {"value": n_pairs / warm, "wall_s": warm, "cold_cli_wall_s": cold_cli([*flags, str(fa)],
+                                                                                                     td)}
+        if not a.no_cpu:
+            exe, kind = ref_exe()
+            if exe:
+                keep = []
+                th = thread_counts()[0]
+                w = run_reference(exe, [[*flags, str(fa)]], th, keep)
+                line["cpu_baseline"] = {"value": n_pairs / w, "unit": "pairs/s", "cores": th, "kind": kind,
+                                        "variant": REF_VARIANT, "host": host_info(),
+                                        "sample": "whole workload, one full command run (read + compute + write); "
+                                                  "no 1-thread run (about 16 x longer)", "wall_s": w}
+                line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+                line["steady_state_over_cpu"] = line["steady_state"]["value"] / line["cpu_baseline"]["value"]
+                ok = all(rc == 0 and (o, so) == keep[0] for rc, o, so, _ in res)
+                line["parity"] = {"sha256_ok": ok, "output_sha256": res[0][1], "checked": "distance file + stdout of "
+                                  "every GPU command vs the reference's"}
     print(json.dumps(line))
 
 
@@ -221,22 +342,49 @@ def bench_c3(a, T, gen):
         st.free()
     bs.free()
     eng.close()
-    if not a.no_cpu:
-        exe, kind = ref_exe()
-        if exe:
-            threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
-            with tempfile.TemporaryDirectory() as td:
-                argvs = []
-                for k, (codes, mode) in enumerate(jobs_codes):
-                    fa = Path(td) / f"j{k}.fa"
-                    gen.write_fasta(fa, codes)
-                    argvs.append(["-t", "0.015", "-a", mode, "-l", str(min(500, codes.shape[1] // 2))]
-                                 + (["-g", "0.05"] if mode == "resolve" else []) + [str(fa)])
-                w = run_reference(exe, argvs, threads)
-            line["cpu_baseline"] = {"value": n_pairs / w, "unit": "pairs/s", "cores": threads, "kind": kind,
-                                    "sample": "the 64 alignments, one full command run each (read + compute + write)",
-                                    "wall_s": w}
-            line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    with tempfile.TemporaryDirectory() as td:
+        argvs, cases = [], []
+        for k, (codes, mode) in enumerate(jobs_codes):
+            fa = Path(td) / f"j{k}.fa"
+            gen.write_fasta(fa, codes)
+            argv = ["-t", "0.015", "-a", mode, "-l", str(min(500, codes.shape[1] // 2))] + \
+                (["-g", "0.05"] if mode == "resolve" else []) + [str(fa)]
+            argvs.append(argv)
+            d = Path(td) / f"g{k}"
+            d.mkdir()
+            cases.append((d, ["-q", *argv[:-1], "-o", "out.csv", argv[-1]]))
+        qa, ra = Path(td) / "q.fa", Path(td) / "r.fa"
+        gen.write_fasta(qa, bc[:2000], gen.names_for(2000, "q"))
+        gen.write_fasta(ra, bc[2000:], gen.names_for(40000, "r"))
+        bip_argv = ["-t", "0.015", "-a", "resolve", "-g", "0.05", "-l", "500", "-s", str(ra), str(qa)]
+        d = Path(td) / "gbip"
+        d.mkdir()
+        cases.append((d, ["-q", *bip_argv[:-1], "-o", "out.csv", bip_argv[-1]]))
+        # the whole batch as tn93 commands: every job read, packed, computed in one tn93g_run_batch, certified and
+        # written by one tn93g --jobs process (the first of two runs pays the device's initialisation)
+        tn93g_jobs(cases, td)
+        got, jobs_wall = tn93g_jobs(cases, td)
+        line["steady_state"] = {"value": (n_pairs + bip_pairs) / jobs_wall, "wall_s": jobs_wall,
+                                "what": "tn93g --jobs over the 64 alignments and the -s block: read, pack, one "
+                                        "tn93g_run_batch, certify, format, write (one process, warm device)"}
+        if not a.no_cpu:
+            exe, kind = ref_exe()
+            if exe:
+                th = thread_counts()[0]
+                keep = []
+                w = run_reference(exe, argvs, th, keep)
+                wb = run_reference(exe, [bip_argv], th, keep)
+                line["cpu_baseline"] = {"value": n_pairs / w, "unit": "pairs/s", "cores": th, "kind": kind,
+                                        "variant": REF_VARIANT, "host": host_info(),
+                                        "sample": "the 64 alignments, one full command run each (read + compute + "
+                                                  "write)", "wall_s": w,
+                                        "bipartite": {"value": bip_pairs / wb, "wall_s": wb}}
+                line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+                line["steady_state_over_cpu"] = line["steady_state"]["value"] / ((n_pairs + bip_pairs) / (w + wb))
+                bad = [k for k, ((rc, o, so), want) in enumerate(zip(got, keep)) if rc != 0 or (o, so) != want]
+                line["parity"] = {"sha256_ok": not bad, "mismatched_jobs": bad,
+                                  "checked": "distance file + stdout of each of the 64 jobs and the -s block (tn93g "
+                                             "--jobs) vs the reference's"}
     print(json.dumps(line))
 
 
@@ -245,19 +393,24 @@ def bench_c4(a, T, gen):
     from swifttn93_amd import serve
     n_panel = 50000
     panel = gen.c1(n=n_panel, clusters=n_panel * 200 // 6700)
+    print(f"c4: generating a {a.serve_seconds:g} s query stream", file=sys.stderr, flush=True)
     stream = serve.query_stream(panel, seed=8, rate=20.0, seconds=a.serve_seconds)
     nq = sum(c.shape[0] for _, c in stream)
-    srv = serve.Server(panel, capacity=n_panel + nq, workers=4, hbm_budget=32 << 30)
+    srv = serve.Server(panel, capacity=n_panel + nq, workers=4, hbm_budget=32 << 30, merge_rows=a.merge_rows)
     # warm the worker contexts (their run buffers) with the last panel rows as if they had just arrived
     for e in srv.free:
         e.run(srv.set, kind=T.APPEND, n1=n_panel - 512, n2=n_panel, **srv.run_kw)
+    print(f"c4: serving {len(stream)} batches, {nq} queries", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     reqs = serve.serve(srv, stream)
     wall = time.perf_counter() - t0
     errors = [r.error for r in reqs if r.error]
     lat = np.array([r.latency for r in reqs]) * 1e3
     pairs = sum((r.n2 * (r.n2 - 1) - r.n1 * (r.n1 - 1)) // 2 for r in reqs)
-    busy = sum(r.done - r.started for r in reqs)
+    runs = {}
+    for r in reqs:
+        runs.setdefault((r.started, r.done), []).append(r)
+    busy = sum(d - s for s, d in runs)
     srv.close()
     line = {"metric": METRIC, "value": pairs / wall, "unit": "pairs/s", "n_gpus": 1, "steps": len(reqs), "warmup": 4,
             "ms_per_step": float(np.mean(lat)), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -265,13 +418,53 @@ def bench_c4(a, T, gen):
                                           "swifttn93_amd/serve.py)",
             "config": {"workload": f"serving: 50,000 x 1,300 device-resident panel, query batches of 1-512 at "
                                    f"20/s for {a.serve_seconds:g} s, each against the panel and all earlier queries, "
-                                   "-t 0.015 -a resolve -g 0.05 -l 500, 4 workers, 32 GiB budget",
-                       "pairs": pairs, "parallelism": "1 GPU"},
+                                   "-t 0.015 -a resolve -g 0.05 -l 500, 4 workers, 32 GiB budget, waiting batches "
+                                   f"merged up to {a.merge_rows} queries per run", "pairs": pairs,
+                       "parallelism": "1 GPU"},
             "roofline": None,
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                            "max": float(lat.max())},
-            "batches": len(reqs), "queries": nq, "errors": len(errors),
+            "batches": len(reqs), "queries": nq, "errors": len(errors), "runs": len(runs),
+            "merged_batches": sum(1 for r in reqs if r.group > 1),
             "pairs_per_busy_s": pairs / busy, "offered_pairs_per_s": pairs / a.serve_seconds}
+    if not a.no_cpu:
+        exe, kind = ref_exe()
+        if exe:
+            # parity at full size: the reference's triangle over everything received (panel, then the batches in
+            # arrival order); its links with a column among the queries must be exactly the served links
+            th = thread_counts()[0]
+            with tempfile.TemporaryDirectory() as td:
+                fa = Path(td) / "all.fa"
+                everything = np.concatenate([panel] + [c for _, c in stream])
+                gen.write_fasta(fa, everything)
+                out = Path(td) / "ref.csv"
+                t0 = time.perf_counter()
+                subprocess.run([str(exe), "-q", "-t", "0.015", "-a", "resolve", "-g", "0.05", "-l", "500", "-f",
+                                "csvn", "-o", str(out), str(fa)], check=True, capture_output=True, timeout=1200,
+                               env=dict(os.environ, OMP_NUM_THREADS=str(th)))
+                wref = time.perf_counter() - t0
+                want = set()
+                with open(out) as f:
+                    next(f)
+                    for ln in f:
+                        i, j, d = ln.rstrip("\n").split(",")
+                        if int(j) >= n_panel:
+                            want.add((int(i), int(j), d))
+            got = set()
+            for r in reqs:
+                if r.result is None:
+                    continue
+                for i, j, d in zip(r.result["link_row"], r.result["link_col"], r.result["link_dist"]):
+                    got.add((int(i), int(j), "%g" % d))
+            line["parity"] = {"sha256_ok": None, "served_links": len(got), "reference_links": len(want),
+                              "links_identical": got == want,
+                              "checked": "every served link (row, column, %g distance) vs the reference's all-pairs "
+                                         "triangle over the panel and every query in arrival order (-f csvn), "
+                                         "restricted to query columns"}
+            line["cpu_baseline"] = {"value": len(everything) * (len(everything) - 1) / 2 / wref, "unit": "pairs/s",
+                                    "cores": th, "kind": kind, "variant": REF_VARIANT, "host": host_info(),
+                                    "sample": "one full all-pairs run over the panel and every query (the parity "
+                                              "check above)", "wall_s": wref}
     print(json.dumps(line))
 
 
@@ -290,7 +483,9 @@ def main():
     ap.add_argument("--config", default="c1", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--mode", default="resolve", choices=["resolve", "average", "skip", "gapmm"],
                     help="c1: tn93's -a mode (resolve runs with -g 0.05)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0, help="c4: length of the query stream")
+    ap.add_argument("--serve-seconds", type=float, default=60.0, help="c4: length of the query stream")
+    ap.add_argument("--merge-rows", type=int, default=2048,
+                    help="c4: waiting batches merged into one run up to this many queries (0: no merging)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this engine runs on one GPU; --gpus {a.gpus} is not supported (sharding across GPUs is "
@@ -347,7 +542,7 @@ def main():
     # whose ambiguity sites are not visited)
     kname = ("k_pairs<1, true, false, %s>" % ("false" if a.mode == "skip" else "true")) if popc \
         else "k_pairs<2, true, false, false>"
-    traffic = profiled_traffic(kname) if a.mode == "resolve" else None  # the profile is of the resolve workload
+    traffic = profiled_traffic(kname) if a.mode == "resolve" and popc else None  # profiled: the resolve workload
     roofline = {"bound": "valu" if popc else "mfma", "achieved": achieved, "peak": peak, "unit": "TOPS",
                 "frac": achieved / peak, "traffic": traffic,
                 "kernel": f"{kname} ({'popcount' if popc else 'int8 MFMA'} counts + events + routing)",
@@ -361,6 +556,9 @@ def main():
         "data": "synthetic (seeded HIV-pol-shaped generator, swifttn93_amd/gen.py)",
         "config": {"workload": "HIV-pol-shaped single set, 6,700 x 1,300, " + " ".join(mode_flags(a.mode)[:-2]),
                    "pairs": n_pairs, "parallelism": "1 GPU"},
+        "value_is": "the engine with the packed set resident in HBM: tn93g_run (counts, events, replays, tail, "
+                    "histogram, row sums, links copied to the host); steady_state adds the reader, packing, "
+                    "certification, formatting and the file",
         "roofline": roofline,
         "gpu_ms_per_step": float(np.mean(gm)),
         "links": int(r["found_links"]), "replayed_pairs": int(r["n_replayed"]),
@@ -368,14 +566,51 @@ def main():
     }
     s.free()
     eng.close()
-    if not a.no_cpu:
-        with tempfile.TemporaryDirectory() as td:
-            fa = Path(td) / "c1.fa"
-            gen.write_fasta(fa, codes)
-            cb = cpu_baseline(fa, n_pairs, a.mode)
-        line["cpu_baseline"] = cb
-        if cb:
-            line["gpu_over_cpu"] = value / cb["value"]
+    modes = ["resolve", "average", "skip"] if a.mode == "resolve" else [a.mode]
+    with tempfile.TemporaryDirectory() as td:
+        fa = Path(td) / "c1.fa"
+        gen.write_fasta(fa, codes)
+        # the whole command, warm: the measured mode 6 times in one tn93g process (the first also pays the
+        # device's initialisation and is left out), then each parity mode once
+        cases = []
+        for k in range(6):
+            d = Path(td) / f"w{k}"
+            d.mkdir()
+            cases.append((d, ["-q", *mode_flags(a.mode), "-o", "out.csv", str(fa)]))
+        for m in modes:
+            d = Path(td) / f"p_{m}"
+            d.mkdir()
+            cases.append((d, ["-q", *mode_flags(m), "-o", "out.csv", str(fa)]))
+        res = tn93g_batch(cases, td)
+        warm = float(np.median([w for _, _, _, w in res[1:6]]))
+        cold = cold_cli([*mode_flags(a.mode), str(fa)], td)
+        line["steady_state"] = {"value": n_pairs / warm, "wall_s": warm, "cold_cli_wall_s": cold,
+                                "cold_cli_value": n_pairs / cold,
+                                "what": "tn93 command in a warm tn93g process: read FASTA, pack + upload, run, "
+                                        "certify, format, write the distance file (median of 5)"}
+        if not a.no_cpu:
+            cb, keep = cpu_baseline([[*mode_flags(a.mode), str(fa)]], n_pairs,
+                                    f"whole workload (6,700 x 1,300, {n_pairs} pairs, -a {a.mode})")
+            line["cpu_baseline"] = cb
+            if cb:
+                line["gpu_over_cpu"] = value / cb["value"]
+                line["steady_state_over_cpu"] = line["steady_state"]["value"] / cb["value"]
+                line["cold_cli_over_cpu"] = line["steady_state"]["cold_cli_value"] / cb["value"]
+                # parity: every mode's GPU command output against the reference's, at full size
+                exe, _ = ref_exe()
+                refs = {a.mode: keep[0]}
+                for m in modes:
+                    if m not in refs:
+                        kk = []
+                        run_reference(exe, [[*mode_flags(m), str(fa)]], cb["cores"], kk)
+                        refs[m] = kk[0]
+                checks = {m: res[6 + q][0] == 0 and (res[6 + q][1], res[6 + q][2]) == refs[m]
+                          for q, m in enumerate(modes)}
+                checks["timed_runs"] = all(rc == 0 and (o, so) == refs[a.mode] for rc, o, so, _ in res[:6])
+                line["parity"] = {"sha256_ok": all(checks.values()), "by_mode": checks,
+                                  "output_sha256": res[6][1],
+                                  "checked": "distance file + stdout of the GPU command vs the reference solver's, "
+                                             "same input and flags"}
     print(json.dumps(line))
 
 

@@ -44,7 +44,7 @@ class Result(ctypes.Structure):
 EXPORTS = ["tn93g_open", "tn93g_close", "tn93g_error", "tn93g_set_upload", "tn93g_set_free", "tn93g_run",
            "tn93g_result_free", "tn93g_version", "tn93g_run_batch", "tn93g_set_create", "tn93g_set_append",
            "tn93g_set_size", "tn93g_run_bytes", "tn93g_result_certify", "tn93g_run_noalloc"]
-NOALLOC_SCRATCH_BYTES = 256 << 20  # TN93G_NOALLOC_SCRATCH_BYTES
+NOALLOC_SCRATCH_BYTES = 16 << 20  # TN93G_NOALLOC_SCRATCH_BYTES
 
 _lib = None
 
@@ -135,12 +135,18 @@ class Engine:
                         flags, n2)
         r = ctypes.POINTER(Result)()
         self._check(self.lib.tn93g_run(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r)))
-        if certify and self.lib.tn93g_result_certify(r, 6) != 1:
-            self.lib.tn93g_result_free(r)
-            p.flags |= FLAG_EXACT
-            r = ctypes.POINTER(Result)()
-            self._check(self.lib.tn93g_run(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r)))
-        return _to_py(r, _ResultOwner(self.lib, r))
+        ok = None
+        if certify:
+            ok = self.lib.tn93g_result_certify(r, 6) == 1
+            if not ok:
+                self.lib.tn93g_result_free(r)
+                p.flags |= FLAG_EXACT
+                r = ctypes.POINTER(Result)()
+                self._check(self.lib.tn93g_run(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r)))
+                ok = self.lib.tn93g_result_certify(r, 6) == 1
+        d = _to_py(r, _ResultOwner(self.lib, r))
+        d["certified"] = ok  # None unless certify was asked for
+        return d
 
     def run_noalloc(self, seqset, link_capacity=1 << 16, **kw):
         """tn93g_run_noalloc: the run in the device's fixed scratch, its links written into arrays allocated here
@@ -267,7 +273,7 @@ def _to_py(rp, owner):
               _arr(owner, r.link_dist, n, np.float64), _arr(owner, r.link_dist_lo, n, np.float64),
               _arr(owner, r.link_dist_hi, n, np.float64)]
     return {
-        "exact": bool(r.exact), "certified": owner.lib.tn93g_result_certify(rp, 6) == 1,
+        "exact": bool(r.exact),
         "hist": np.frombuffer(r.hist, dtype=np.uint64).reshape(3, 200).copy(),
         "link_row": lk[0], "link_col": lk[1], "link_dist": lk[2], "link_lo": lk[3], "link_hi": lk[4],
         "mean_lo": list(r.mean_lo), "mean_hi": list(r.mean_hi), "weighted": list(r.weighted),

@@ -3377,6 +3377,22 @@ extern "C" {
 
 const char *tn93g_version(void) { return TN93G_VERSION; }
 
+// The streams and events of a batch's NBRANCH branches (created on a context's first batch).
+static int ensure_branches(tn93g_ctx *c)
+{
+  if (c->fork) return 0;
+  for (int b = 0; b < NBRANCH; b++) {
+    HIPCHK(c, hipStreamCreateWithFlags(&c->side[b], hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->join[b], hipEventDisableTiming));
+    if (b == 0) continue;  // branch 0's replay side stream exists from the start
+    HIPCHK(c, hipStreamCreateWithFlags(&c->rside[b], hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->rfork[b], hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->rjoin[b], hipEventDisableTiming));
+  }
+  HIPCHK(c, hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+  return 0;
+}
+
 int tn93g_open(int device, tn93g_ctx **out)
 {
   if (!out) return TN93G_E_ARG;
@@ -3391,17 +3407,10 @@ int tn93g_open(int device, tn93g_ctx **out)
   }
   for (auto &e : c->ev) (void)hipEventCreate(&e);
   if (hipDeviceGetAttribute(&c->wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess) c->wall_khz = 0;
-  for (int b = 0; b < NBRANCH; b++) {
-    if (hipStreamCreateWithFlags(&c->side[b], hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join[b], hipEventDisableTiming) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->rside[b], hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->rfork[b], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->rjoin[b], hipEventDisableTiming) != hipSuccess) {
-      tn93g_close(c);
-      return TN93G_E_NODEV;
-    }
-  }
-  if (hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess) {
+  // a single run needs branch 0's side stream (its tail beside its replay); the batch branches come on first use
+  if (hipStreamCreateWithFlags(&c->rside[0], hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->rfork[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->rjoin[0], hipEventDisableTiming) != hipSuccess) {
     tn93g_close(c);
     return TN93G_E_NODEV;
   }
@@ -3704,13 +3713,22 @@ void tn93g_result_free(tn93g_result *r)
 // rounding midpoint of the printed digits: then the scaled values lo 10^(prec-1-e) and hi 10^(prec-1-e) (e the
 // decimal exponent of lo) lie in [10^(prec-1), 10^prec) and round to the same integer with a margin of 1e-6, far
 // above the error of the scaling (a few ulps of 10^prec); anything else is decided by formatting both.
+static double pow10_tab(int k)  // 10^k for |k| <= 40 (the nearest doubles; their error is covered by the margin)
+{
+  static const std::array<double, 81> t = [] {
+    std::array<double, 81> a{};
+    for (int i = 0; i <= 80; i++) a[(size_t)i] = std::pow(10.0, (double)(i - 40));
+    return a;
+  }();
+  return t[(size_t)(k + 40)];
+}
+
 static bool same_printed(double lo, double hi, int prec)
 {
   if (lo == hi) return true;
-  if (lo > 0. && hi > lo && hi < 1e300 && lo > 1e-300) {
+  if (lo > 0. && hi > lo && hi < 1e20 && lo > 1e-20) {
     const int e = (int)std::floor(std::log10(lo));
-    const double p = std::pow(10.0, (double)(prec - 1 - e)), lo_d = std::pow(10.0, (double)(prec - 1)),
-                 hi_d = 10.0 * lo_d;
+    const double p = pow10_tab(prec - 1 - e), lo_d = pow10_tab(prec - 1), hi_d = 10.0 * lo_d;
     const double a = lo * p, b = hi * p;
     if (a >= lo_d * (1. + 1e-9) && b <= hi_d * (1. - 1e-9)) {
       const double fa = std::floor(a + 0.5), fb = std::floor(b + 0.5);
@@ -3968,6 +3986,7 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
   for (int k = 0; k < n; k++)
     if (!sets[k]) return TN93G_E_ARG;
   HIPCHK(c, hipSetDevice(c->device));
+  if (int rc = ensure_branches(c)) return rc;
   auto fail_all = [&](int rc) {
     for (int k = 0; k < n; k++) {
       tn93g_result_free(out[k]);

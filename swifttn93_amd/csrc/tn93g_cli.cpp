@@ -7,6 +7,7 @@
 // must be the same at both ends of the enclosure, otherwise the run is repeated with TN93G_FLAG_EXACT (logarithms
 // resolved with the host libm).
 #include <atomic>
+#include <chrono>
 #include <cctype>
 #include <cmath>
 #include <cstdarg>
@@ -287,9 +288,23 @@ void close_inputs(Args &A)
 
 // Arguments, reader and upload (swifttn93.cpp:2222-2306).  Returns -1 when the case is ready for its computation,
 // otherwise the exit code the process ends with.
+// TN93G_TIMES=1: the phases of a run (reader, device open, upload, computation, output) on stderr
+double now_s() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+struct Times {
+  double t0 = now_s(), read = 0, open = 0, upload = 0, run = 0, out = 0;
+};
+Times g_times;
+void mark(double &slot)
+{
+  const double t = now_s();
+  slot += t - g_times.t0;
+  g_times.t0 = t;
+}
+
 int prepare(Case &C, int argc, const char **argv)
 {
   Args &A = C.A;
+  g_times.t0 = now_s();
   parse_args(A, argc, argv);
   if (A.do_bootstrap || A.do_bootstrap_two_files || A.include_prob < 1.) {
     std::cerr << "swifttn93: -b, -r and -u (random resampling) are not supported" << std::endl;
@@ -319,6 +334,7 @@ int prepare(Case &C, int argc, const char **argv)
   }
   const bool two = C.two = A.input2 != nullptr;
   close_inputs(A);
+  mark(g_times.read);
   C.do_fst = two && A.do_fst;
   C.report_self = !two && A.report_self && !A.do_count;
   C.N = (unsigned long)S.N;
@@ -340,7 +356,9 @@ int prepare(Case &C, int argc, const char **argv)
     std::cerr << "tn93g: no usable GPU (error " << rc << ")" << std::endl;
     return 1;
   }
+  mark(g_times.open);
   rc = tn93g_set_upload(g_ctx, S.codes.data(), (int64_t)C.N, C.firstLen, S.counts.data(), &C.sg.s);
+  mark(g_times.upload);
   if (rc) {
     std::cerr << "tn93g: " << tn93g_error(g_ctx) << std::endl;
     return 1;
@@ -530,8 +548,15 @@ int run(int argc, const char **argv)
   Case C;
   int rc = prepare(C, argc, argv);
   if (rc >= 0) return rc;
+  g_times.t0 = now_s();
   if ((rc = settle(C))) return rc;
-  return output(C);
+  mark(g_times.run);
+  rc = output(C);
+  mark(g_times.out);
+  if (getenv("TN93G_TIMES"))
+    fprintf(stderr, "tn93g: read %.1f ms, device open %.1f ms, upload %.1f ms, run %.1f ms, output %.1f ms\n",
+            1e3 * g_times.read, 1e3 * g_times.open, 1e3 * g_times.upload, 1e3 * g_times.run, 1e3 * g_times.out);
+  return rc;
 }
 
 template <class F>
@@ -600,7 +625,12 @@ int batch(const char *file)
     if (f.empty() || chdir(f[0].c_str())) return 2;
     std::vector<const char *> av = case_argv(f);
     std::ofstream(f[0] + "/stdout.bin", std::ios::binary | std::ios::trunc);
+    // the case's wall time (reader, upload and packing, the run, certification, formatting, the file written) in
+    // DIR/wall.txt: the steady-state figure of a warm process
+    const auto t0 = std::chrono::steady_clock::now();
     captured(f[0], [&] { return run((int)av.size(), av.data()); });
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::ofstream(f[0] + "/wall.txt") << std::setprecision(9) << wall << "\n";
     if (chdir(cwd)) return 2;
   }
   return 0;

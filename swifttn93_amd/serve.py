@@ -166,7 +166,8 @@ def split_links(r, bounds):
 
 def query_stream(panel, seed=8, rate=20.0, seconds=60.0, max_batch=512):
     """Poisson arrivals at `rate` batches/s for `seconds`; batch sizes uniform in 1..max_batch; each query is a
-    panel sequence mutated at 0.5-2 % with 1 % ambiguity codes (the c1 generator's proportions)."""
+    panel sequence with each clean site substituted with probability 0.5-2 % (drawn per query), and 1 % ambiguity
+    codes (the c1 generator's proportions)."""
     rs = gen.Stream(seed)
     t, out = 0.0, []
     n, L = panel.shape
@@ -177,12 +178,11 @@ def query_stream(panel, seed=8, rate=20.0, seconds=60.0, max_batch=512):
         q = 1 + int(rs.integers(max_batch, 1)[0])
         src = rs.integers(n, q)
         codes = panel[src].copy()
-        for k in range(q):  # substitutions at clean sites only
-            m = gen._poisson(rs, L * (0.005 + 0.015 * rs.uniform(1)[0]))
-            pos = rs.integers(L, m)
-            shift = rs.integers(3, m).astype(np.uint8) + 1
-            ok = codes[k, pos] < 4
-            codes[k, pos[ok]] = (codes[k, pos[ok]] + shift[ok]) % 4
+        # substitutions at clean sites only, each site of query k with probability rate_k
+        sub = 0.005 + 0.015 * rs.uniform(q)
+        hit = (rs.uniform(q * L).reshape(q, L) < sub[:, None]) & (codes < 4)
+        shift = (rs.integers(3, int(hit.sum())) + 1).astype(np.uint8)
+        codes[hit] = (codes[hit] + shift) % 4
         w = list(gen._C1_WEIGHTS)
         w += [sum(w) * 0.005 / (1.0 - 0.005) / 4.0] * 4
         gen._ambiguities(rs, codes, 0.01, gen._C1_POOL, w)

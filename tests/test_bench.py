@@ -30,3 +30,24 @@ def test_reference_flags_follow_the_mode():
         f = b.mode_flags(m)
         assert f == ["-t", "0.015", "-a", m, "-l", "500", "-f", "csv"]
         assert "-g" not in f
+
+
+def test_cpu_baseline_record(tmp_path):
+    """The baseline times the reference on the affinity set's threads (and OMP_NUM_THREADS when it differs) and on 1
+    thread, names the build variant and the host, and keeps the sha256 of the reference's outputs for parity."""
+    import os
+
+    import pytest
+    b = _bench()
+    exe, kind = b.ref_exe()
+    if exe is None:
+        pytest.skip("neither the reference build nor the oracle is built")
+    sys.path.insert(0, str(REPO))
+    from swifttn93_amd import gen
+    fa = tmp_path / "x.fa"
+    gen.write_fasta(fa, gen.c1s(40, seed=3))
+    cb, keep = b.cpu_baseline([["-t", "0.05", "-l", "500", str(fa)]], 780, "test sample")
+    assert cb["cores"] in b.thread_counts() and b.thread_counts()[0] == len(os.sched_getaffinity(0))
+    assert cb["one_thread"]["cores"] == 1 and cb["one_thread"]["value"] > 0
+    assert cb["kind"] == kind and cb["variant"] and cb["host"]["nproc"] >= 1
+    assert len(keep) == 1 and len(keep[0][0]) == 64
