@@ -287,9 +287,16 @@ def bench_c3(a, T, gen):
                               resolve_fraction=0.05 if mode == "resolve" else 1.0, backend=T.BACKEND_AUTO,
                               flags=T.FLAG_AUTOTUNE)))
     n_pairs = sum(c.shape[0] * (c.shape[0] - 1) // 2 for c, _ in jobs_codes)
-    for _ in range(max(a.warmup, 1)):
+    # the -s bipartite block, 2,000 queries x 40,000 references, as one more job of the same batch
+    bc = gen.c3_bipartite()
+    bs = eng.upload(bc)
+    bkw = dict(mode=T.RESOLVE, kind=T.CROSS, n1=2000, threshold=0.015, min_overlap=500, resolve_fraction=0.05,
+               backend=T.BACKEND_AUTO, flags=T.FLAG_AUTOTUNE)
+    bip_pairs = 2000 * 40000
+    jobs.append((bs, bkw))
+    for _ in range(max(a.warmup, 1)):  # the first batch also tunes every job shape
         eng.run_batch(jobs)
-    # plain path: the same jobs one tn93g_run each
+    # plain path: the same jobs (the -s block included) one tn93g_run each
     t0 = time.perf_counter()
     pk = 0.0
     for _ in range(a.steps):
@@ -305,36 +312,38 @@ def bench_c3(a, T, gen):
     t_batch = (time.perf_counter() - t0) / a.steps
     bpk /= a.steps
     merged = [r["pair_kernel_ms"] > 0 for r in res]
-    value = n_pairs / t_batch
-    # the -s bipartite block
-    bc = gen.c3_bipartite()
-    bs = eng.upload(bc)
-    bkw = dict(mode=T.RESOLVE, kind=T.CROSS, n1=2000, threshold=0.015, min_overlap=500, resolve_fraction=0.05,
-               backend=T.BACKEND_AUTO, flags=T.FLAG_AUTOTUNE)
+    value = (n_pairs + bip_pairs) / t_batch
+    rb = res[-1]
+    # the -s block alone, for reference
     eng.run(bs, **bkw)
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        rb = eng.run(bs, **bkw)
+        eng.run(bs, **bkw)
     t_bip = (time.perf_counter() - t0) / a.steps
-    bip_pairs = 2000 * 40000
-    # roofline of the batch's dominant kernel, the pair launch shared by the jobs of one kernel variant: 22 int32 ops
-    # per word pair (count_word) over the merged jobs' pairs
+    import collections
+    shapes = collections.Counter(("mfma" if r["backend_used"] == T.BACKEND_MFMA else "popc") + f"x{r['tile_rows_used']}"
+                                 for r in res)
+    # roofline of the batch's dominant kernel, the pair launches shared by the jobs of one kernel variant and tile
+    # height (device clock around all of them): 22 int32 ops per word pair (count_word) over the merged jobs' pairs
     Lw = [c.shape[0] * (c.shape[0] - 1) // 2 * ((c.shape[1] + 31) // 32) for c, _ in jobs_codes]
+    Lw.append(bip_pairs * ((bc.shape[1] + 31) // 32))
     Lw_m = sum(w for w, m in zip(Lw, merged) if m)
     achieved = 22 * Lw_m / (bpk * 1e-3) / 1e12 if bpk > 0 else 0.0
     line = {"metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
             "ms_per_step": t_batch * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "int32+f64", "data": "synthetic (seeded generators, swifttn93_amd/gen.py c3, c3_bipartite)",
             "config": {"workload": "64 alignments, 50-3,000 sequences x 300-10,000 columns, -a resolve/average/skip/"
-                                   "gapmm cycling, -t 0.015, one batch (one graph launch)", "pairs": n_pairs,
-                       "parallelism": "1 GPU"},
+                                   "gapmm cycling, -t 0.015, and the -s block of 2,000 x 40,000 at 1,300 columns, "
+                                   "all in one batch (one graph launch; backends and tile heights autotuned per job "
+                                   "shape)", "pairs": n_pairs + bip_pairs, "parallelism": "1 GPU"},
             "roofline": {"bound": "valu", "achieved": achieved, "peak": VALU_INT32_TOPS, "unit": "TOPS",
                          "frac": achieved / VALU_INT32_TOPS, "traffic": None,
-                         "kernel": f"k_pairs_multi (one launch over the tiles of {sum(merged)} of the "
-                                   f"{len(merged)} jobs)", "kernel_ms": bpk,
+                         "kernel": f"k_pairs_multi (launches over the tiles of {sum(merged)} of the "
+                                   f"{len(merged)} jobs, one per kernel variant and tile height)", "kernel_ms": bpk,
                          "ops_per_word_pair": 22},
+            "tuned_shapes": dict(shapes),
             "plain_pair_kernel_ms": pk / a.steps,
-            "plain_pairs_per_s": n_pairs / t_plain, "plain_ms_per_step": t_plain * 1e3,
+            "plain_pairs_per_s": (n_pairs + bip_pairs) / t_plain, "plain_ms_per_step": t_plain * 1e3,
             "batch_over_plain": t_plain / t_batch,
             "bipartite": {"pairs": bip_pairs, "pairs_per_s": bip_pairs / t_bip, "ms": t_bip * 1e3,
                           "links": int(rb["found_links"])}}
@@ -342,6 +351,7 @@ def bench_c3(a, T, gen):
         st.free()
     bs.free()
     eng.close()
+    n_pairs_jobs = n_pairs
     with tempfile.TemporaryDirectory() as td:
         argvs, cases = [], []
         for k, (codes, mode) in enumerate(jobs_codes):
@@ -364,7 +374,7 @@ def bench_c3(a, T, gen):
         # written by one tn93g --jobs process (the first of two runs pays the device's initialisation)
         tn93g_jobs(cases, td)
         got, jobs_wall = tn93g_jobs(cases, td)
-        line["steady_state"] = {"value": (n_pairs + bip_pairs) / jobs_wall, "wall_s": jobs_wall,
+        line["steady_state"] = {"value": (n_pairs_jobs + bip_pairs) / jobs_wall, "wall_s": jobs_wall,
                                 "what": "tn93g --jobs over the 64 alignments and the -s block: read, pack, one "
                                         "tn93g_run_batch, certify, format, write (one process, warm device)"}
         if not a.no_cpu:
@@ -374,13 +384,14 @@ def bench_c3(a, T, gen):
                 keep = []
                 w = run_reference(exe, argvs, th, keep)
                 wb = run_reference(exe, [bip_argv], th, keep)
-                line["cpu_baseline"] = {"value": n_pairs / w, "unit": "pairs/s", "cores": th, "kind": kind,
-                                        "variant": REF_VARIANT, "host": host_info(),
-                                        "sample": "the 64 alignments, one full command run each (read + compute + "
-                                                  "write)", "wall_s": w,
+                line["cpu_baseline"] = {"value": (n_pairs + bip_pairs) / (w + wb), "unit": "pairs/s", "cores": th,
+                                        "kind": kind, "variant": REF_VARIANT, "host": host_info(),
+                                        "sample": "the 64 alignments and the -s block, one full command run each "
+                                                  "(read + compute + write)", "wall_s": w + wb,
+                                        "jobs": {"value": n_pairs / w, "wall_s": w},
                                         "bipartite": {"value": bip_pairs / wb, "wall_s": wb}}
                 line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
-                line["steady_state_over_cpu"] = line["steady_state"]["value"] / ((n_pairs + bip_pairs) / (w + wb))
+                line["steady_state_over_cpu"] = line["steady_state"]["value"] / line["cpu_baseline"]["value"]
                 bad = [k for k, ((rc, o, so), want) in enumerate(zip(got, keep)) if rc != 0 or (o, so) != want]
                 line["parity"] = {"sha256_ok": not bad, "mismatched_jobs": bad,
                                   "checked": "distance file + stdout of each of the 64 jobs and the -s block (tn93g "
@@ -484,6 +495,9 @@ def main():
     ap.add_argument("--mode", default="resolve", choices=["resolve", "average", "skip", "gapmm"],
                     help="c1: tn93's -a mode (resolve runs with -g 0.05)")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="c4: length of the query stream")
+    ap.add_argument("--tile-rows", type=int, default=0, choices=[0, 8, 16, 32],
+                    help="c1: rows of the pair kernel's tiles (0: the default, or the autotuner's with --autotune)")
+    ap.add_argument("--autotune", action="store_true", help="c1: let the autotuner pick the backend and tile rows")
     ap.add_argument("--merge-rows", type=int, default=2048,
                     help="c4: waiting batches merged into one run up to this many queries (0: no merging)")
     a = ap.parse_args()
@@ -518,7 +532,7 @@ def main():
     s = eng.upload(codes)
     tmode = {"resolve": T.RESOLVE, "average": T.AVERAGE, "skip": T.SKIP, "gapmm": T.GAPMM}[a.mode]
     kw = dict(mode=tmode, threshold=0.015, min_overlap=500, resolve_fraction=0.05 if a.mode == "resolve" else 1.0,
-              backend=backend)
+              backend=backend, tile_rows=a.tile_rows, flags=T.FLAG_AUTOTUNE if a.autotune else 0)
     for _ in range(a.warmup):
         r = eng.run(s, **kw)
     t0 = time.perf_counter()
@@ -532,7 +546,8 @@ def main():
     value = n_pairs * a.steps / dt
     pair_ms = float(np.mean(pk))
     words = (L + 31) // 32
-    popc = backend in (T.BACKEND_AUTO, T.BACKEND_POPC)
+    popc = r["backend_used"] == T.BACKEND_POPC
+    rows = r["tile_rows_used"]
     # popcount backend: 12 logic + 10 popcount-accumulate int32 operations per 32-bit word, against the VALU peak;
     # MFMA backend: the 4 x 4 one-hot product, 16 int8 multiply-adds = 32 operations per site, against the int8 peak
     ops_per_pair = 22 * words if popc else 32 * L
@@ -540,12 +555,13 @@ def main():
     achieved = ops_per_pair * n_pairs / (pair_ms * 1e-3) / 1e12
     # template arguments: backend (1 popcount, 2 MFMA), fast path, dense-N planes, marginal form (not in SKIP mode,
     # whose ambiguity sites are not visited)
-    kname = ("k_pairs<1, true, false, %s>" % ("false" if a.mode == "skip" else "true")) if popc \
-        else "k_pairs<2, true, false, false>"
+    kname = ("k_pairs<1, true, false, %s, %d>" % ("false" if a.mode == "skip" else "true", rows // 4)) if popc \
+        else "k_pairs<2, true, false, false, 4>"
     traffic = profiled_traffic(kname) if a.mode == "resolve" and popc else None  # profiled: the resolve workload
     roofline = {"bound": "valu" if popc else "mfma", "achieved": achieved, "peak": peak, "unit": "TOPS",
                 "frac": achieved / peak, "traffic": traffic,
-                "kernel": f"{kname} ({'popcount' if popc else 'int8 MFMA'} counts + events + routing)",
+                "kernel": f"{kname} ({'popcount' if popc else 'int8 MFMA'} counts + events + routing, tiles of "
+                          f"{rows} x 64)",
                 "kernel_ms": pair_ms, "ops_per_pair": ops_per_pair,
                 "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE, profiles/c1_popc_hbm_bytes.md",
                 "hbm_frac": None if traffic is None else traffic / (pair_ms * 1e-3) / HBM_BYTES_S}

@@ -54,7 +54,8 @@ extern "C" {
 /* run flags */
 #define TN93G_FLAG_EXACT 1u      /* resolve every uncertain log on the host (libm) instead of bracketing it   */
 #define TN93G_FLAG_ALL_SLOW 2u   /* route every pair through the site-order replay kernel (tests)            */
-#define TN93G_FLAG_AUTOTUNE 4u   /* with TN93G_BACKEND_AUTO: time the backends on the first job of each shape */
+#define TN93G_FLAG_AUTOTUNE 4u   /* time the pair kernel's free choices (TN93G_BACKEND_AUTO, tile_rows 0) on the
+                                    first job of each shape and keep the fastest for the context               */
 
 typedef struct tn93g_ctx tn93g_ctx;
 typedef struct tn93g_set tn93g_set;
@@ -95,6 +96,10 @@ typedef struct {
   double min_distance;     /* -w (default -1e-10)                                               */
   uint32_t flags;          /* TN93G_FLAG_*                                                      */
   int64_t n2;              /* APPEND: end of the appended range (0: the set's size)             */
+  int32_t tile_rows;       /* rows of the pair kernel's tiles of 64 columns: 8, 16 or 32; 0: the autotuner's
+                              choice for the job's shape (TN93G_FLAG_AUTOTUNE), else 16.  The MFMA backend and
+                              the exact path tile 16 rows.  The reference tiles 32 rows by blocks of 8 columns
+                              (swifttn93.cpp:2348, 2440); the results never depend on the shape                 */
 } tn93g_params;
 
 /* Result of one run.  All arrays are owned by the result.  Everything tn93 prints is here:
@@ -129,6 +134,8 @@ typedef struct {
                            the one pair-kernel launch shared by the jobs of the same kernel variant (all such launches
                            together, from the device clock), 0 for a job that ran its own                          */
   double tail_kernel_ms;/* device time of the FP64 tail kernel                            */
+  int32_t backend_used;  /* the pair kernel's backend (TN93G_BACKEND_POPC or _MFMA) and the rows of its tiles: the */
+  int32_t tile_rows_used;/* caller's or the autotuner's choice                                                    */
 } tn93g_result;
 
 /* Device memory (bytes) that tn93g_run of this set and these parameters works in, beyond the set itself: what a

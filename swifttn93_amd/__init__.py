@@ -26,7 +26,7 @@ class Params(ctypes.Structure):
                 ("subset_mask", ctypes.c_uint32), ("backend", ctypes.c_int32),
                 ("resolve_fraction", ctypes.c_double), ("min_overlap", ctypes.c_int64),
                 ("threshold", ctypes.c_double), ("min_distance", ctypes.c_double), ("flags", ctypes.c_uint32),
-                ("n2", ctypes.c_int64)]
+                ("n2", ctypes.c_int64), ("tile_rows", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
@@ -38,7 +38,8 @@ class Result(ctypes.Structure):
                 ("max_lo", ctypes.c_double * 3), ("max_hi", ctypes.c_double * 3), ("found_links", ctypes.c_int64),
                 ("skipped", ctypes.c_int64), ("n_replayed", ctypes.c_int64), ("n_uncertain", ctypes.c_int64),
                 ("n_host_exact", ctypes.c_int64), ("gpu_ms", ctypes.c_double), ("pair_kernel_ms", ctypes.c_double),
-                ("tail_kernel_ms", ctypes.c_double)]
+                ("tail_kernel_ms", ctypes.c_double), ("backend_used", ctypes.c_int32),
+                ("tile_rows_used", ctypes.c_int32)]
 
 
 EXPORTS = ["tn93g_open", "tn93g_close", "tn93g_error", "tn93g_set_upload", "tn93g_set_free", "tn93g_run",
@@ -84,9 +85,9 @@ def load():
 
 
 def make_params(mode=RESOLVE, kind=TRIANGLE, n1=0, subset_mask=0, backend=BACKEND_AUTO, resolve_fraction=1.0,
-                min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0, n2=0):
+                min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0, n2=0, tile_rows=0):
     return Params(mode, kind, n1, subset_mask, backend, resolve_fraction, min_overlap, threshold, min_distance, flags,
-                  n2)
+                  n2, tile_rows)
 
 
 class Engine:
@@ -127,12 +128,12 @@ class Engine:
 
     def run(self, seqset, mode=RESOLVE, kind=TRIANGLE, n1=0, subset_mask=0, backend=BACKEND_AUTO,
             resolve_fraction=1.0, min_overlap=100, threshold=0.015, min_distance=-1e-10, flags=0, n2=0,
-            certify=False):
+            certify=False, tile_rows=0):
         """One tn93g_run.  With certify, a result whose link distances do not print as tn93's would (%g at both
         ends of their enclosures, tn93g_result_certify) is replaced by the run again with FLAG_EXACT, so every
         returned link_dist prints as tn93 prints it."""
         p = make_params(mode, kind, n1, subset_mask, backend, resolve_fraction, min_overlap, threshold, min_distance,
-                        flags, n2)
+                        flags, n2, tile_rows)
         r = ctypes.POINTER(Result)()
         self._check(self.lib.tn93g_run(self.ctx, seqset.handle, ctypes.byref(p), ctypes.byref(r)))
         ok = None
@@ -280,4 +281,5 @@ def _to_py(rp, owner):
         "max_lo": list(r.max_lo), "max_hi": list(r.max_hi), "found_links": r.found_links, "skipped": r.skipped,
         "n_replayed": r.n_replayed, "n_uncertain": r.n_uncertain, "n_host_exact": r.n_host_exact,
         "gpu_ms": r.gpu_ms, "pair_kernel_ms": r.pair_kernel_ms, "tail_kernel_ms": r.tail_kernel_ms,
+        "backend_used": r.backend_used, "tile_rows_used": r.tile_rows_used,
     }

@@ -316,6 +316,7 @@ struct RunArgs {
                                // nf0 | nf1 << 16, nf2 | nf3 << 16, diag | AG << 16, CT
   int c16;                     // 32 * Ls < 2^16: every x16 count fits 16 bits
   int64_t gx;                  // column tiles
+  int ti;                      // rows of the pair kernel's tiles (8, 16 or 32)
 };
 
 struct ChunkHdr {
@@ -655,7 +656,9 @@ __global__ void k_evfill(const uint8_t *__restrict__ codes, const int32_t *__res
 }
 
 // ------------------------------------------------------------------------------------------ pair kernel
-// Tile: TI = 16 rows x 64 columns, 256 threads = 4 waves; lane = column j, each wave owns RPW = 4 rows (uniform i).
+// Tile: TI rows x 64 columns, 256 threads = 4 waves; lane = column j, each wave owns RP rows (uniform i), TI = 4 RP.
+// The pair kernel is instantiated for RP = 2, 4 and 8 (tiles of 8, 16 and 32 rows: the autotuner's shapes); RPW and
+// TI are the default shape, which the other kernels built on the same tiling (k_spec16) use.
 constexpr int RPW = 4, TI = 4 * RPW, TJ = 64, KW = 16;
 constexpr int EVCAP = 2048;  // events of a tile's rows and columns held in LDS
 
@@ -759,58 +762,59 @@ __device__ inline uint32_t quad_sum(uint32_t x)
 // padded by one element per word so that lanes storing different words of one sequence use different LDS banks.
 // MFMA backend: the operands of one 64-site block, and at the end the per-pair counts transposed into the
 // lane = column layout of the epilogue.
-template <int KWORDS, bool MFMA>
+template <int KWORDS, bool MFMA, int TI_>
 struct PairLds {
   union {
-    uint4 ops[(TI + TJ) / 4][64];  // one 64-site block: TI/4 row groups + 16 column groups, 1 KB each
-    uint32_t agg[7][TI][TJ];       // T, DG, TS, CT, B0, B1, BT per pair
+    uint4 ops[(TI_ + TJ) / 4][64];  // one 64-site block: TI/4 row groups + 16 column groups, 1 KB each
+    uint32_t agg[7][TI_][TJ];       // T, DG, TS, CT, B0, B1, BT per pair
   } u;
 };
-template <int KWORDS>
-struct PairLds<KWORDS, false> {
-  uint4 sI[KWORDS][TI + 1];
+template <int KWORDS, int TI_>
+struct PairLds<KWORDS, false, TI_> {
+  uint4 sI[KWORDS][TI_ + 1];
   uint4 sJ[KWORDS][TJ + 1];
 };
 
 // Staging of the planes of words [w0, w0 + kw) of the tile's sequences, in two steps so that the loads of the next
 // chunk are in flight while the current one is counted: SPT values per thread (consecutive threads: consecutive
 // words of one sequence), N-plane words too for dense-N sets.
-constexpr int SPT = (KW * (TI + TJ) + 255) / 256;
-template <bool NP>
+template <int TI_>
+constexpr int spt() { return (KW * (TI_ + TJ) + 255) / 256; }
+template <bool NP, int TI_>
 struct StageRegs {
-  uint4 v[SPT];
-  uint32_t nm[NP ? SPT : 1];
+  uint4 v[spt<TI_>()];
+  uint32_t nm[NP ? spt<TI_>() : 1];
 };
-template <bool NP>
+template <bool NP, int TI_>
 __device__ __attribute__((always_inline)) inline void stage_load(const RunArgs &A, int64_t i0, int64_t j0, int64_t w0,
-                                                                 int kw, int tid, StageRegs<NP> &r)
+                                                                 int kw, int tid, StageRegs<NP, TI_> &r)
 {
 #pragma unroll
-  for (int q = 0; q < SPT; q++) {
+  for (int q = 0; q < spt<TI_>(); q++) {
     const int k = tid + 256 * q;
     const int s = k / kw, w = k % kw;
-    const int64_t seq = s < TI ? i0 + s : j0 + (s - TI);
-    const bool ok = k < kw * (TI + TJ) && seq < A.n;
+    const int64_t seq = s < TI_ ? i0 + s : j0 + (s - TI_);
+    const bool ok = k < kw * (TI_ + TJ) && seq < A.n;
     r.v[q] = ok ? A.planes[seq * A.W + w0 + w] : make_uint4(0, 0, 0, 0);
     if constexpr (NP) r.nm[q] = ok ? A.nmask[seq * A.W + w0 + w] : 0u;
   }
 }
-template <bool NP, class LDS>
-__device__ __attribute__((always_inline)) inline void stage_store(LDS &lds, uint32_t (*sNI)[TI + 1],
+template <bool NP, int TI_, class LDS>
+__device__ __attribute__((always_inline)) inline void stage_store(LDS &lds, uint32_t (*sNI)[TI_ + 1],
                                                                   uint32_t (*sNJ)[TJ + 1], int kw, int tid,
-                                                                  const StageRegs<NP> &r)
+                                                                  const StageRegs<NP, TI_> &r)
 {
 #pragma unroll
-  for (int q = 0; q < SPT; q++) {
+  for (int q = 0; q < spt<TI_>(); q++) {
     const int k = tid + 256 * q;
-    if (k >= kw * (TI + TJ)) break;
+    if (k >= kw * (TI_ + TJ)) break;
     const int s = k / kw, w = k % kw;
-    if (s < TI) {
+    if (s < TI_) {
       lds.sI[w][s] = r.v[q];
       if constexpr (NP) sNI[w][s] = r.nm[q];
     } else {
-      lds.sJ[w][s - TI] = r.v[q];
-      if constexpr (NP) sNJ[w][s - TI] = r.nm[q];
+      lds.sJ[w][s - TI_] = r.v[q];
+      if constexpr (NP) sNJ[w][s - TI_] = r.nm[q];
     }
   }
 }
@@ -819,25 +823,26 @@ __device__ __attribute__((always_inline)) inline void stage_store(LDS &lds, uint
 // rows [er0, er0 + nr), columns [ec0, ec0 + nc).  They are taken in chunks that fit the LDS: chunk rows [r0, r1) and
 // columns [c0, c1) of those ranges (relative), row events first.  A chunk holds its event words in sEv (rows, then
 // columns) and, in the LDS of the counting buffers, the codes its events face: row event e the codes of the tile's
-// 64 columns at its site (g[64 e ...]), column event c the codes of the tile's 16 rows at its site
-// (g[64 R + 16 c ...], R = r1 - r0).  Almost every tile is one chunk.
+// 64 columns at its site (g[64 e ...]), column event c the codes of the tile's TI rows at its site
+// (g[64 R + TI c ...], R = r1 - r0; cb = TI bytes per column event).  Almost every tile is one chunk.
 constexpr int GCAP = 30 * 1024;
-template <bool MF>
+template <bool MF, int TI_>
 union TileLds {
-  PairLds<KW, MF> p;
+  PairLds<KW, MF, TI_> p;
   uint8_t g[GCAP];
 };
 
 struct TileEv {
   int er0, ec0, nr, nc;
   int r0, r1, c0, c1;
+  int cb;  // bytes of row codes per column event (the tile's rows)
   __device__ inline void next()
   {
     r0 = r1;
     c0 = c1;
     const int R = min(nr - r0, min(GCAP / 64, EVCAP));
     r1 = r0 + R;
-    c1 = c0 + min(nc - c0, min((GCAP - 64 * R) / 16, EVCAP - R));
+    c1 = c0 + min(nc - c0, min((GCAP - 64 * R) / cb, EVCAP - R));
   }
   __device__ inline void first()
   {
@@ -856,21 +861,56 @@ __device__ inline void chunk_events(const RunArgs &A, const TileEv &te, uint32_t
   for (int k = tid; k < C; k += 256) sEv[R + k] = A.ev[te.ec0 + te.c0 + k];
 }
 
-// The codes the chunk's events face, from the site-major codes into g (16 bytes per load, all loads of a thread in
-// flight together).
+// The codes the chunk's events face, from the site-major codes into g (16 bytes per load, 8 for the columns of a
+// tile of 8 rows; all loads of a thread in flight together).  i0 is a multiple of TI and j0 of 64, so every load is
+// aligned to its size.
+template <int TI_>
 __device__ __attribute__((always_inline)) inline void chunk_codes(const RunArgs &A, const TileEv &te,
                                                                    const uint32_t *sEv, uint8_t *g, int64_t i0,
                                                                    int64_t j0, int tid)
 {
   const size_t np = (size_t)A.npad;
   const int R = te.r1 - te.r0, C = te.c1 - te.c0;
-  const int nrow = 4 * R, tot = nrow + C;
+  const int nrow = 4 * R;
+  if constexpr (TI_ >= 16) {
+    constexpr int PC = TI_ / 16;  // 16-byte loads per column event
+    const int tot = nrow + PC * C;
 #pragma unroll 4
-  for (int k = tid; k < tot; k += 256) {
-    const uint8_t *src = k < nrow ? A.codesT + (size_t)(sEv[k >> 2] >> 5) * np + j0 + (k & 3) * 16
-                                  : A.codesT + (size_t)(sEv[R + k - nrow] >> 5) * np + i0;
-    *reinterpret_cast<uint4 *>(g + k * 16) = *reinterpret_cast<const uint4 *>(src);
+    for (int k = tid; k < tot; k += 256) {
+      const uint8_t *src;
+      if (k < nrow) {
+        src = A.codesT + (size_t)(sEv[k >> 2] >> 5) * np + j0 + (k & 3) * 16;
+      } else {
+        const int q = k - nrow, ce = q / PC, part = q % PC;
+        src = A.codesT + (size_t)(sEv[R + ce] >> 5) * np + i0 + part * 16;
+      }
+      *reinterpret_cast<uint4 *>(g + k * 16) = *reinterpret_cast<const uint4 *>(src);
+    }
+  } else {
+    static_assert(TI_ == 8, "tiles of 8, 16 or 32 rows");
+#pragma unroll 4
+    for (int k = tid; k < nrow; k += 256)
+      *reinterpret_cast<uint4 *>(g + k * 16) =
+          *reinterpret_cast<const uint4 *>(A.codesT + (size_t)(sEv[k >> 2] >> 5) * np + j0 + (k & 3) * 16);
+#pragma unroll 4
+    for (int c = tid; c < C; c += 256)
+      *reinterpret_cast<uint2 *>(g + nrow * 16 + c * 8) =
+          *reinterpret_cast<const uint2 *>(A.codesT + (size_t)(sEv[R + c] >> 5) * np + i0);
   }
+}
+
+// The codes of a wave's RP rows at a column event's site (one byte per row).
+template <int RP>
+using RowCodes = typename std::conditional<RP == 8, unsigned long long, uint32_t>::type;
+template <int RP>
+__device__ inline RowCodes<RP> row_codes(const uint8_t *p)
+{
+  if constexpr (RP == 2)
+    return *reinterpret_cast<const uint16_t *>(p);
+  else if constexpr (RP == 4)
+    return *reinterpret_cast<const uint32_t *>(p);
+  else
+    return *reinterpret_cast<const unsigned long long *>(p);
 }
 
 // Event tables in LDS, TABW entries per event code; code EVNONE is an all-zero row that pads partial batches.
@@ -902,21 +942,22 @@ __device__ inline uint4 keep_if(bool k, uint4 t) { return k ? t : make_uint4(0, 
 // word of the event's 16-byte record.  sRo[TI + 1]: the rows' event offsets relative to the tile's row list.  The
 // packed words are flushed every EVFLUSH events.  MASKED: only the (row, lane) pairs with use[a] set take the entries
 // (the others add zero), and rows or lanes with no such pair skip the loops.
-template <bool MASKED>
+template <bool MASKED, int RP>
 __device__ __attribute__((always_inline)) inline void chunk_apply(const RunArgs &A, const uint4 *sTabR,
                                                                   const uint4 *sTabC, const TileEv &te,
                                                                   const uint32_t *sEv, const uint8_t *g,
                                                                   const int *sRo, int64_t ib, int wave, int lane,
-                                                                  int pj, int pj_end, const bool (&use)[RPW],
-                                                                  EvAcc (&acc)[RPW], Tot (&tot)[RPW])
+                                                                  int pj, int pj_end, const bool (&use)[RP],
+                                                                  EvAcc (&acc)[RP], Tot (&tot)[RP])
 {
+  constexpr int TI_ = 4 * RP;
   const int R = te.r1 - te.r0;
 #pragma unroll
-  for (int a = 0; a < RPW; a++) {
+  for (int a = 0; a < RP; a++) {
     const int64_t i = ib + a;
     if (i >= A.r1) continue;  // uniform over the wave
     if (MASKED && !__ballot(use[a])) continue;
-    const int e0 = max(sRo[wave * RPW + a], te.r0) - te.r0, e1 = min(sRo[wave * RPW + a + 1], te.r1) - te.r0;
+    const int e0 = max(sRo[wave * RP + a], te.r0) - te.r0, e1 = min(sRo[wave * RP + a + 1], te.r1) - te.r0;
     for (int h0 = e0; h0 < e1; h0 += EVFLUSH) {
     const int h1 = min(e1, h0 + EVFLUSH);
     int e = h0;
@@ -940,45 +981,50 @@ __device__ __attribute__((always_inline)) inline void chunk_apply(const RunArgs 
     ev_flush(acc[a], tot[a]);
     }
   }
-  if (MASKED && !(use[0] || use[1] || use[2] || use[3])) return;
+  if constexpr (MASKED) {
+    bool any = false;
+#pragma unroll
+    for (int a = 0; a < RP; a++) any |= use[a];
+    if (!any) return;
+  }
   const int k0 = max(pj - te.ec0, te.c0) - te.c0, k1 = min(pj_end - te.ec0, te.c1) - te.c0;
   const uint32_t *evc = sEv + R;
-  const uint8_t *gc = g + R * 64 + wave * RPW;
+  const uint8_t *gc = g + R * 64 + wave * RP;
   for (int f0 = k0; f0 < k1; f0 += EVFLUSH) {
     const int f1 = min(k1, f0 + EVFLUSH);
     for (int k = f0; k < f1; k++) {
       const uint4 *t = sTabC + (evc[k] & 31u) * TABW;
-      const uint32_t r4 = *reinterpret_cast<const uint32_t *>(gc + k * 16);
+      const RowCodes<RP> rc = row_codes<RP>(gc + k * TI_);
 #pragma unroll
-      for (int a = 0; a < RPW; a++) {
-        const uint4 v = t[(r4 >> (8 * a)) & 255u];
+      for (int a = 0; a < RP; a++) {
+        const uint4 v = t[(uint32_t)(rc >> (8 * a)) & 255u];
         ev_add(acc[a], MASKED ? keep_if(use[a], v) : v);
       }
     }
 #pragma unroll
-    for (int a = 0; a < RPW; a++) ev_flush(acc[a], tot[a]);
+    for (int a = 0; a < RP; a++) ev_flush(acc[a], tot[a]);
   }
 }
 
 // All events of the tile, chunk by chunk.  The first chunk's event words and codes are in LDS already (ready); every
 // further chunk is loaded between barriers, so all threads of the block call this together.
-template <bool MASKED>
+template <bool MASKED, int RP>
 __device__ __attribute__((always_inline)) inline void tile_events(const RunArgs &A, const uint4 *sTabR,
                                                                   const uint4 *sTabC, TileEv &te, uint32_t *sEv,
                                                                   uint8_t *g, const int *sRo, int64_t i0, int64_t j0,
                                                                   int64_t ib, int tid, int pj, int pj_end,
-                                                                  const bool (&use)[RPW], EvAcc (&acc)[RPW],
-                                                                  Tot (&tot)[RPW])
+                                                                  const bool (&use)[RP], EvAcc (&acc)[RP],
+                                                                  Tot (&tot)[RP])
 {
   const int wave = tid >> 6, lane = tid & 63;
   for (;;) {
-    chunk_apply<MASKED>(A, sTabR, sTabC, te, sEv, g, sRo, ib, wave, lane, pj, pj_end, use, acc, tot);
+    chunk_apply<MASKED, RP>(A, sTabR, sTabC, te, sEv, g, sRo, ib, wave, lane, pj, pj_end, use, acc, tot);
     if (te.last()) return;
     te.next();
     __syncthreads();
     chunk_events(A, te, sEv, tid);
     __syncthreads();
-    chunk_codes(A, te, sEv, g, i0, j0, tid);
+    chunk_codes<4 * RP>(A, te, sEv, g, i0, j0, tid);
     __syncthreads();
   }
 }
@@ -1029,22 +1075,25 @@ struct TileRuns {
 // then its columns), less the clean sites of each facing a gap run of the other (gaps are not events outside GAPMM,
 // so no table entry sees them).  sRg[TI + 1]: the rows' run offsets relative to the tile's row runs; [h0, h1): the
 // column's runs in the set's list.
-__device__ __attribute__((always_inline)) inline void base_sums(const RunArgs &A, const uint4 *sTB, const TileRuns &tr, const int *sRg, int64_t ib,
-                                 int wave, int lane, int64_t j, int h0, int h1, uint4 (&S)[RPW])
+template <int RP>
+__device__ __attribute__((always_inline)) inline void base_sums(const RunArgs &A, const uint4 *sTB,
+                                                                const TileRuns &tr, const int *sRg, int64_t ib,
+                                                                int wave, int lane, int64_t j, int h0, int h1,
+                                                                uint4 (&S)[RP])
 {
   const int64_t W = A.W;
   const bool jin = j < A.n;
 #pragma unroll
-  for (int a = 0; a < RPW; a++) {
-    S[a] = sTB[TI + lane];
-    add4(S[a], sTB[wave * RPW + a]);
+  for (int a = 0; a < RP; a++) {
+    S[a] = sTB[4 * RP + lane];
+    add4(S[a], sTB[wave * RP + a]);
   }
   if (A.gapmm) return;
 #pragma unroll
-  for (int a = 0; a < RPW; a++) {
+  for (int a = 0; a < RP; a++) {
     const int64_t i = ib + a;
     if (i >= A.r1) continue;  // uniform over the wave
-    const int g0 = sRg[wave * RPW + a], g1 = sRg[wave * RPW + a + 1];
+    const int g0 = sRg[wave * RP + a], g1 = sRg[wave * RP + a + 1];
     for (int g = g0; g < g1; g++) {
       const uint2 r = tr.row(A, g);
       if (jin) {
@@ -1056,7 +1105,7 @@ __device__ __attribute__((always_inline)) inline void base_sums(const RunArgs &A
   for (int g = h0; g < h1; g++) {
     const uint2 r = tr.col(A, g);
 #pragma unroll
-    for (int a = 0; a < RPW; a++) {
+    for (int a = 0; a < RP; a++) {
       const int64_t i = ib + a;
       if (i >= A.r1) continue;
       sub4(S[a], plane_prefix(A.pre, A.planes, W, i, r.y));
@@ -1067,12 +1116,13 @@ __device__ __attribute__((always_inline)) inline void base_sums(const RunArgs &A
 
 // MARG: the base sums come from base_sums and tables whose entries facing a clean base subtract it (popcount
 // backend, every mode but SKIP, whose ambiguity sites are not visited).
-// One pair tile: rows A.r0 + by * TI .. + TI, columns (c0 rounded down to 64) + bx * TJ .. + TJ.
-template <int BACKEND, bool FAST, bool NP, bool MARG>
+// One pair tile: rows A.r0 + by * TI .. + TI, columns (c0 rounded down to 64) + bx * TJ .. + TJ; TI = 4 RP.
+template <int BACKEND, bool FAST, bool NP, bool MARG, int RP>
 __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &A, const int64_t bx, const int64_t by)
 {
+  constexpr int TI_ = 4 * RP;
   const int64_t c0a = A.c0 & ~(int64_t)63;  // column tiles start on a 64 boundary (whole 4-sequence groups)
-  const int64_t i0 = A.r0 + by * TI, j0 = c0a + bx * TJ;
+  const int64_t i0 = A.r0 + by * TI_, j0 = c0a + bx * TJ;
   if (A.triangle && j0 + TJ - 1 <= i0) return;  // no pair j > i in this tile
   constexpr bool MF = BACKEND == TN93G_BACKEND_MFMA;
   __shared__ unsigned lslow;
@@ -1082,36 +1132,36 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
     lslow = 0;
     bs.unc = bs.undecided = 0;
   }
-  __shared__ TileLds<MF> L;
-  PairLds<KW, MF> &lds = L.p;
-  __shared__ uint32_t sNI[NP ? KW : 1][TI + 1], sNJ[NP ? KW : 1][TJ + 1];
+  __shared__ TileLds<MF, TI_> L;
+  PairLds<KW, MF, TI_> &lds = L.p;
+  __shared__ uint32_t sNI[NP ? KW : 1][TI_ + 1], sNJ[NP ? KW : 1][TJ + 1];
   __shared__ uint4 sTabR[(NCODE + 1) * TABW], sTabC[(NCODE + 1) * TABW];
   __shared__ unsigned long long lhist[FAST ? 1 : 3 * 200];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  StageRegs<NP> stg;  // the first chunk's planes, loaded before everything else
+  StageRegs<NP, TI_> stg;  // the first chunk's planes, loaded before everything else
   if constexpr (!MF)
-    if (!A.all_slow) stage_load<NP>(A, i0, j0, 0, (int)min((int64_t)KW, A.W), tid, stg);
+    if (!A.all_slow) stage_load<NP, TI_>(A, i0, j0, 0, (int)min((int64_t)KW, A.W), tid, stg);
   const uint4 *tabs = A.evtab + (MARG ? 2 * NCODE * NCODE : 0);  // [main, AVERAGE], plain or folded
   load_event_tables(tabs, sTabR, sTabC, tid, A.gapmm, A.nplane);
   if constexpr (!FAST)
     for (int k = tid; k < 600; k += 256) lhist[k] = 0;
   const int64_t j = j0 + lane;
-  const int64_t ib = i0 + wave * RPW;
+  const int64_t ib = i0 + wave * RP;
   using CntT = typename std::conditional<MARG, CntM, Cnt>::type;
-  CntT cnt[RPW];
-  uint4 S[MARG ? RPW : 1];
-  Tot tot[RPW];
-  EvAcc acc[RPW];
-  NCnt ncn[NP ? RPW : 1];
+  CntT cnt[RP];
+  uint4 S[MARG ? RP : 1];
+  Tot tot[RP];
+  EvAcc acc[RP];
+  NCnt ncn[NP ? RP : 1];
 #pragma unroll
-  for (int a = 0; a < RPW; a++) {
+  for (int a = 0; a < RP; a++) {
     cnt[a] = CntT{};
     tot[a] = Tot{{0, 0, 0, 0}, 0, 0, 0, 0};
     acc[a] = EvAcc{0, 0, 0, 0, 0};
   }
   if constexpr (NP)
 #pragma unroll
-    for (int a = 0; a < RPW; a++) ncn[a] = NCnt{{0, 0, 0, 0}, {0, 0, 0, 0}, 0};
+    for (int a = 0; a < RP; a++) ncn[a] = NCnt{{0, 0, 0, 0}, {0, 0, 0, 0}, 0};
   const bool do_fast = !A.all_slow;
   const bool do_events = do_fast && !A.skip_events;
   // column event cursors (columns past the set have none)
@@ -1119,14 +1169,15 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
   // the tile's event lists, the rows' event offsets, and (marginal form) the tile's plane totals and gap runs into
   // LDS when they fit (synchronised by the first barrier of the counts below)
   __shared__ uint32_t sEv[EVCAP];
-  __shared__ int sRo[TI + 1], sRg[MARG ? TI + 1 : 1];
-  __shared__ uint4 sTB[MARG ? TI + TJ : 1];
+  __shared__ int sRo[TI_ + 1], sRg[MARG ? TI_ + 1 : 1];
+  __shared__ uint4 sTB[MARG ? TI_ + TJ : 1];
   __shared__ uint2 sGR[MARG ? GRCAP : 1];
   TileEv te;
+  te.cb = TI_;
   TileRuns tr{sGR, 0, 0, 0, false};
   int h0 = 0, h1 = 0;
   {
-    const int64_t rl = i0 < A.r1 ? min(i0 + (int64_t)TI, A.r1) : i0;
+    const int64_t rl = i0 < A.r1 ? min(i0 + (int64_t)TI_, A.r1) : i0;
     const int64_t cl = min(j0 + (int64_t)TJ, A.n);
     te.er0 = i0 < A.r1 ? A.ev_off[i0] : 0;
     te.nr = i0 < A.r1 ? A.ev_off[rl] - te.er0 : 0;
@@ -1135,12 +1186,12 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
     te.first();
     if (do_events) {
       chunk_events(A, te, sEv, tid);
-      if (tid <= TI) sRo[tid] = (i0 < A.r1 ? A.ev_off[min(i0 + tid, rl)] : 0) - te.er0;
+      if (tid <= TI_) sRo[tid] = (i0 < A.r1 ? A.ev_off[min(i0 + tid, rl)] : 0) - te.er0;
     }
     if constexpr (MARG) {
       if (do_fast) {
-        if (tid < TI + TJ) {
-          const int64_t q = tid < TI ? i0 + tid : j0 + (tid - TI);
+        if (tid < TI_ + TJ) {
+          const int64_t q = tid < TI_ ? i0 + tid : j0 + (tid - TI_);
           sTB[tid] = q < A.n ? A.tb[q] : make_uint4(0, 0, 0, 0);
         }
         if (j < A.n) {
@@ -1156,7 +1207,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
           for (int k = tid; k < tr.nrr; k += 256) sGR[k] = A.gruns[tr.gr0 + k];
           for (int k = tid; k < ncr; k += 256) sGR[tr.nrr + k] = A.gruns[tr.gc0 + k];
         }
-        if (tid <= TI) sRg[tid] = (i0 < A.r1 ? A.grun_off[min(i0 + tid, rl)] : 0) - tr.gr0;
+        if (tid <= TI_) sRg[tid] = (i0 < A.r1 ? A.grun_off[min(i0 + tid, rl)] : 0) - tr.gr0;
       }
     }
   }
@@ -1165,20 +1216,20 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
       for (int64_t w0 = 0; w0 < A.W; w0 += KW) {
         const int kw = (int)min((int64_t)KW, A.W - w0);
         __syncthreads();
-        stage_store<NP>(lds, sNI, sNJ, kw, tid, stg);
+        stage_store<NP, TI_>(lds, sNI, sNJ, kw, tid, stg);
         __syncthreads();
-        if (w0 + KW < A.W) stage_load<NP>(A, i0, j0, w0 + KW, (int)min((int64_t)KW, A.W - w0 - KW), tid, stg);
+        if (w0 + KW < A.W) stage_load<NP, TI_>(A, i0, j0, w0 + KW, (int)min((int64_t)KW, A.W - w0 - KW), tid, stg);
         for (int k = 0; k < kw; k++) {
           const uint4 pjv = lds.sJ[k][lane];
 #pragma unroll
-          for (int a = 0; a < RPW; a++) count_word(lds.sI[k][wave * RPW + a], pjv, cnt[a]);
+          for (int a = 0; a < RP; a++) count_word(lds.sI[k][wave * RP + a], pjv, cnt[a]);
           if constexpr (NP) {
             // the rows' N sites against the column (a uniform branch); the column's N sites against the rows are
             // counted per N run after the loop
             const uint32_t nj = sNJ[k][lane];
 #pragma unroll
-            for (int a = 0; a < RPW; a++) {
-              const uint32_t ni = sNI[k][wave * RPW + a];  // the same in every lane
+            for (int a = 0; a < RP; a++) {
+              const uint32_t ni = sNI[k][wave * RP + a];  // the same in every lane
               if (ni) {
                 ncn[a].r[0] += __popc(ni & pjv.x);
                 ncn[a].r[1] += __popc(ni & pjv.y);
@@ -1192,32 +1243,33 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
       }
   }
   if constexpr (MF) {
+    static_assert(RP == 4, "the MFMA backend tiles 16 rows");
     if (do_fast) {
-      // wave w: all TI/4 row groups x column groups 4w..4w+3; one v_mfma_i32_16x16x64_i8 per (row group, column
+      // wave w: all TI_/4 row groups x column groups 4w..4w+3; one v_mfma_i32_16x16x64_i8 per (row group, column
       // group) and 64-site block gives the 4x4 base table of 16 pairs in int32
-      v4i macc[TI / 4][4];
+      v4i macc[TI_ / 4][4];
 #pragma unroll
-      for (int ig = 0; ig < TI / 4; ig++)
+      for (int ig = 0; ig < TI_ / 4; ig++)
 #pragma unroll
         for (int jg = 0; jg < 4; jg++) macc[ig][jg] = v4i{0, 0, 0, 0};
       const uint4 *gI = reinterpret_cast<const uint4 *>(A.onehot) + (i0 >> 2) * A.W64 * 64;
       const uint4 *gJ = reinterpret_cast<const uint4 *>(A.onehot) + (j0 >> 2) * A.W64 * 64;
       for (int64_t blk = 0; blk < A.W64; blk++) {
         __syncthreads();
-        for (int k = tid; k < (TI + TJ) / 4 * 64; k += 256) {
+        for (int k = tid; k < (TI_ + TJ) / 4 * 64; k += 256) {
           const int g = k >> 6, q = k & 63;
-          lds.u.ops[g][q] = g < TI / 4 ? gI[((int64_t)g * A.W64 + blk) * 64 + q]
-                                       : gJ[((int64_t)(g - TI / 4) * A.W64 + blk) * 64 + q];
+          lds.u.ops[g][q] = g < TI_ / 4 ? gI[((int64_t)g * A.W64 + blk) * 64 + q]
+                                       : gJ[((int64_t)(g - TI_ / 4) * A.W64 + blk) * 64 + q];
         }
         __syncthreads();
         v4i bop[4];
 #pragma unroll
         for (int jg = 0; jg < 4; jg++) {
-          const uint4 v = lds.u.ops[TI / 4 + 4 * wave + jg][lane];
+          const uint4 v = lds.u.ops[TI_ / 4 + 4 * wave + jg][lane];
           bop[jg] = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
         }
 #pragma unroll
-        for (int ig = 0; ig < TI / 4; ig++) {
+        for (int ig = 0; ig < TI_ / 4; ig++) {
           const uint4 v = lds.u.ops[ig][lane];
           const v4i aop = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
 #pragma unroll
@@ -1230,7 +1282,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
       // with b = l%4; reduce over the quad into the seven counts of the popcount backend
       const int b = lane & 3;
 #pragma unroll
-      for (int ig = 0; ig < TI / 4; ig++)
+      for (int ig = 0; ig < TI_ / 4; ig++)
 #pragma unroll
         for (int jg = 0; jg < 4; jg++) {
           const v4i v = macc[ig][jg];
@@ -1251,17 +1303,17 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
         }
       __syncthreads();
 #pragma unroll
-      for (int a = 0; a < RPW; a++) {
-        const int il = wave * RPW + a;
+      for (int a = 0; a < RP; a++) {
+        const int il = wave * RP + a;
         cnt[a] = Cnt{lds.u.agg[0][il][lane], lds.u.agg[1][il][lane], lds.u.agg[2][il][lane], lds.u.agg[3][il][lane],
                      lds.u.agg[4][il][lane], lds.u.agg[5][il][lane], lds.u.agg[6][il][lane]};
       }
     }
   }
   __syncthreads();  // the event tables and lists are in LDS (also when no counts ran); the counting buffers are free
-  if (do_events) chunk_codes(A, te, sEv, L.g, i0, j0, tid);
+  if (do_events) chunk_codes<TI_>(A, te, sEv, L.g, i0, j0, tid);
   if constexpr (MARG)
-    if (do_fast) base_sums(A, sTB, tr, sRg, ib, wave, lane, j, h0, h1, S);
+    if (do_fast) base_sums<RP>(A, sTB, tr, sRg, ib, wave, lane, j, h0, h1, S);
   if constexpr (NP)
     if (do_fast && j < A.n) {
       // the column's N sites against the rows' planes: N comes in runs, so each run is a difference of the rows'
@@ -1270,7 +1322,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
       for (int q = A.nrun_off[j]; q < q1; q++) {
         const uint2 r = A.nruns[q];
 #pragma unroll
-        for (int a = 0; a < RPW; a++) {
+        for (int a = 0; a < RP; a++) {
           if (ib + a >= A.r1) continue;  // uniform over the wave
           const uint4 hi = plane_prefix(A.pre, A.planes, A.W, ib + a, r.y);
           const uint4 lo = plane_prefix(A.pre, A.planes, A.W, ib + a, r.x);
@@ -1282,20 +1334,23 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
       }
     }
   __syncthreads();
-  const bool all_rows[RPW] = {true, true, true, true};
-  if (do_events) tile_events<false>(A, sTabR, sTabC, te, sEv, L.g, sRo, i0, j0, ib, tid, pj, pj_end, all_rows, acc, tot);
+  bool all_rows[RP];
+#pragma unroll
+  for (int a = 0; a < RP; a++) all_rows[a] = true;
+  if (do_events) tile_events<false, RP>(A, sTabR, sTabC, te, sEv, L.g, sRo, i0, j0, ib, tid, pj, pj_end, all_rows, acc, tot);
   if constexpr (NP)
     if (do_events)
 #pragma unroll
-      for (int a = 0; a < RPW; a++) nplane_add(ncn[a], sTabR, tot[a], acc[a]);
+      for (int a = 0; a < RP; a++) nplane_add(ncn[a], sTabR, tot[a], acc[a]);
   // -g: pairs whose resolved ambiguities reach the fraction are computed again in AVERAGE (refkernel.cpp:742-750):
   // same clean/clean counts, the events through the AVERAGE table.  Order-dependent ones go to the replay instead.
-  bool avg[RPW] = {false, false, false, false};
-  bool avgr[RPW] = {false, false, false, false};  // order-dependent pairs whose -g test is decided for AVERAGE
+  bool avg[RP], avgr[RP];  // avgr: order-dependent pairs whose -g test is decided for AVERAGE
+#pragma unroll
+  for (int a = 0; a < RP; a++) avg[a] = avgr[a] = false;
   bool any_avg = false;
   if (A.avg_fallback && do_events) {
 #pragma unroll
-    for (int a = 0; a < RPW; a++) {
+    for (int a = 0; a < RP; a++) {
       const int64_t i = ib + a;
       const bool valid = i < A.r1 && j >= A.c0 && j < A.c1 && (!A.triangle || j > i);
       uint32_t nfs = tot[a].nf[0] + tot[a].nf[1] + tot[a].nf[2] + tot[a].nf[3];
@@ -1322,7 +1377,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
     load_event_tables(tabs + NCODE * NCODE, sTabR, sTabC, tid, A.gapmm, A.nplane);
     __syncthreads();
 #pragma unroll
-    for (int a = 0; a < RPW; a++)
+    for (int a = 0; a < RP; a++)
       if (avg[a]) {
         tot[a] = Tot{{0, 0, 0, 0}, 0, 0, 0, 0};
         acc[a] = EvAcc{0, 0, 0, 0, 0};
@@ -1331,18 +1386,18 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
       te.first();
       chunk_events(A, te, sEv, tid);
       __syncthreads();
-      chunk_codes(A, te, sEv, L.g, i0, j0, tid);
+      chunk_codes<TI_>(A, te, sEv, L.g, i0, j0, tid);
       __syncthreads();
     }
-    tile_events<true>(A, sTabR, sTabC, te, sEv, L.g, sRo, i0, j0, ib, tid, pj, pj_end, avg, acc, tot);
+    tile_events<true, RP>(A, sTabR, sTabC, te, sEv, L.g, sRo, i0, j0, ib, tid, pj, pj_end, avg, acc, tot);
     if constexpr (NP)
 #pragma unroll
-      for (int a = 0; a < RPW; a++)
+      for (int a = 0; a < RP; a++)
         if (avg[a]) nplane_add(ncn[a], sTabR, tot[a], acc[a]);
   }
   // totals x16 from the clean/clean counts, plus the events
 #pragma unroll
-  for (int a = 0; a < RPW; a++) {
+  for (int a = 0; a < RP; a++) {
     const CntT &c = cnt[a];
     uint32_t nf0, nf1, nf2, nf3;
     if constexpr (MARG) {
@@ -1370,7 +1425,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
   // test already chose AVERAGE) and appended after the loop.
   unsigned smask = 0;
 #pragma unroll
-  for (int a = 0; a < RPW; a++) {
+  for (int a = 0; a < RP; a++) {
     const int64_t i = ib + a;
     const bool valid = i < A.r1 && j >= A.c0 && j < A.c1 && (!A.triangle || j > i);
     int state = 0;  // 1: tail, 2: overlap too small
@@ -1386,7 +1441,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
         else
           state = 1;
       }
-      if (slow) smask |= (1u << a) | (avg[a] || avgr[a] ? 1u << (RPW + a) : 0u);
+      if (slow) smask |= (1u << a) | (avg[a] || avgr[a] ? 1u << (RP + a) : 0u);
     }
     if constexpr (FAST) {
       const unsigned long long need = __ballot(state == 1), skp = __ballot(state == 2);
@@ -1426,7 +1481,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
     }
   }
   // the tile's replay pairs: a slot range per thread in LDS, one range of the flat list per tile
-  const unsigned nmine = __popc(smask & ((1u << RPW) - 1));
+  const unsigned nmine = __popc(smask & ((1u << RP) - 1));
   const unsigned k0 = nmine ? atomicAdd(&lslow, nmine) : 0u;  // lslow <= TI * TJ
   flush_block(A, &bs, FAST ? nullptr : lhist, tid, 256);  // begins with a barrier: lslow is final
   if (tid == 0 && lslow) sbase = atomicAdd(A.slow_total, (unsigned long long)lslow);
@@ -1435,22 +1490,22 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
     unsigned long long *dst = A.slow_list + sbase + k0;
     unsigned q = 0;
 #pragma unroll
-    for (int a = 0; a < RPW; a++)
+    for (int a = 0; a < RP; a++)
       if (smask >> a & 1u)  // bit 63: the replay starts in AVERAGE
         dst[q++] = (unsigned long long)(ib + a) << 32 | (unsigned long long)j |
-                   (smask >> (RPW + a) & 1u ? 1ull << 63 : 0ull);
+                   (smask >> (RP + a) & 1u ? 1ull << 63 : 0ull);
   }
 }
 
-template <int BACKEND, bool FAST, bool NP, bool MARG>
+template <int BACKEND, bool FAST, bool NP, bool MARG, int RP>
 __global__ __launch_bounds__(256) void k_pairs(RunArgs A)
 {
-  pairs_tile<BACKEND, FAST, NP, MARG>(A, blockIdx.x, blockIdx.y);
+  pairs_tile<BACKEND, FAST, NP, MARG, RP>(A, blockIdx.x, blockIdx.y);
 }
 
 // The pair tiles of several jobs of one kernel variant in one launch (a batch): block b is tile b - toff[k] of job k,
 // toff[k] <= b < toff[k + 1], so small alignments fill the device together instead of one after another.
-template <int BACKEND, bool FAST, bool NP, bool MARG>
+template <int BACKEND, bool FAST, bool NP, bool MARG, int RP>
 __global__ __launch_bounds__(256) void k_pairs_multi(const RunArgs *__restrict__ jobs, const int64_t *__restrict__ toff,
                                                      int njobs)
 {
@@ -1465,7 +1520,7 @@ __global__ __launch_bounds__(256) void k_pairs_multi(const RunArgs *__restrict__
   }
   const RunArgs &A = jobs[lo];
   const int64_t t = b - toff[lo];
-  pairs_tile<BACKEND, FAST, NP, MARG>(A, t % A.gx, t / A.gx);
+  pairs_tile<BACKEND, FAST, NP, MARG, RP>(A, t % A.gx, t / A.gx);
 }
 
 // The FP64 tail of the fast path.  A block takes whole rows; its 4 waves take the row's chunks (64 columns each) in
@@ -1487,7 +1542,7 @@ __device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A
     RowPart rp = rowpart_zero();
     // the wave's chunks of the row, bx = wave, wave + 4, ...: the next chunk's header and (16-bit packed) inputs are
     // loaded before the current one is computed, so the loads of one chunk overlap the FP64 work of the previous
-    const int64_t tri = A.r0 + (r / TI) * TI;  // first row of the pair kernel's tile row
+    const int64_t tri = A.r0 + (r / A.ti) * A.ti;  // first row of the pair kernel's tile row
     auto live = [&](int64_t bx) { return bx < A.gx && !(A.triangle && c0a + bx * TJ + TJ - 1 <= tri); };
     int64_t bx = wave;
     while (bx < A.gx && !live(bx)) bx += 4;
@@ -2636,6 +2691,7 @@ struct Job {
   bool fast = false;
   bool sparse_gaps = true;  // the marginal form's gap-run corrections cost less than counting the base sums
   int64_t nrows = 0, ntiles = 0, gx = 0, gy = 0, npairs = 0;
+  int rp = 4;  // rows per wave of the pair kernel's tiles (tiles of 4 rp rows)
   size_t nr1 = 0;
   unsigned link_cap = 1, exact_cap = 1;
   uint64_t n_slow = 0, n_unc = 0, n_undecided = 0, n_linkrec = 0;
@@ -2818,7 +2874,13 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
   const size_t nr1 = J.nr1;
   const int64_t c0a = A.c0 & ~(int64_t)63;
   J.gx = std::max<int64_t>((A.c1 - c0a + TJ - 1) / TJ, 1);
-  J.gy = std::max<int64_t>((nrows + TI - 1) / TI, 1);
+  // tile rows: the caller's (8, 16, 32), else 16 until the autotuner or its earlier choice sets them
+  if (P->tile_rows != 0 && P->tile_rows != 8 && P->tile_rows != 16 && P->tile_rows != 32)
+    return c->fail(TN93G_E_ARG, "tile_rows must be 0, 8, 16 or 32");
+  if (P->tile_rows != 0 && P->tile_rows != 16 && P->backend == TN93G_BACKEND_MFMA)
+    return c->fail(TN93G_E_ARG, "the MFMA backend tiles 16 rows");
+  J.rp = P->tile_rows ? P->tile_rows / 4 : 4;
+  J.gy = std::max<int64_t>((nrows + 4 * J.rp - 1) / (4 * J.rp), 1);
   J.ntiles = J.gx * J.gy;
   const int64_t nslots = J.fast ? (int64_t)nr1 * J.gx : 1;
   if (J.gx >= (1ll << 31) || J.gy >= 65536) return c->fail(TN93G_E_ARG, "too many pair tiles in one run");
@@ -2869,6 +2931,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
   A.chdr = (ChunkHdr *)B->chdr.p;
   A.cdata = (uint32_t *)B->cdata.p;
   A.gx = J.gx;
+  A.ti = 4 * J.rp;
   // count tables of the special sequences for the replay: rows against the run's columns and, unless the run is a
   // triangle over one range (then a column's table is its row table), columns against the run's rows.  Long
   // alignments only (128 words or more): below that the replay's own per-pair count is cheaper than 16 gathers.
@@ -2931,38 +2994,83 @@ static int pairs_variant(const Job &J)
   return (J.fast ? 1 : 0) | (A.nplane ? 2 : 0) | (marg ? 4 : 0);
 }
 
-static void launch_pairs(int v, dim3 g, hipStream_t st, const RunArgs &A)
+// The popcount fast-path variants (1, 3, 5, 7) come in tiles of 8, 16 and 32 rows (rp 2, 4, 8); the exact-path
+// variants and the MFMA backend in tiles of 16.
+template <int RP>
+static void launch_pairs_rp(int v, dim3 g, hipStream_t st, const RunArgs &A)
+{
+  constexpr int PC = TN93G_BACKEND_POPC;
+  switch (v) {
+    case 1: hipLaunchKernelGGL((k_pairs<PC, true, false, false, RP>), g, dim3(256), 0, st, A); break;
+    case 3: hipLaunchKernelGGL((k_pairs<PC, true, true, false, RP>), g, dim3(256), 0, st, A); break;
+    case 5: hipLaunchKernelGGL((k_pairs<PC, true, false, true, RP>), g, dim3(256), 0, st, A); break;
+    case 7: hipLaunchKernelGGL((k_pairs<PC, true, true, true, RP>), g, dim3(256), 0, st, A); break;
+    default: break;
+  }
+}
+static void launch_pairs(int v, int rp, dim3 g, hipStream_t st, const RunArgs &A)
 {
   constexpr int PC = TN93G_BACKEND_POPC, MF = TN93G_BACKEND_MFMA;
+  if (rp == 2) return launch_pairs_rp<2>(v, g, st, A);
+  if (rp == 8) return launch_pairs_rp<8>(v, g, st, A);
   switch (v) {
-    case 0: hipLaunchKernelGGL((k_pairs<PC, false, false, false>), g, dim3(256), 0, st, A); break;
-    case 1: hipLaunchKernelGGL((k_pairs<PC, true, false, false>), g, dim3(256), 0, st, A); break;
-    case 2: hipLaunchKernelGGL((k_pairs<PC, false, true, false>), g, dim3(256), 0, st, A); break;
-    case 3: hipLaunchKernelGGL((k_pairs<PC, true, true, false>), g, dim3(256), 0, st, A); break;
-    case 4: hipLaunchKernelGGL((k_pairs<PC, false, false, true>), g, dim3(256), 0, st, A); break;
-    case 5: hipLaunchKernelGGL((k_pairs<PC, true, false, true>), g, dim3(256), 0, st, A); break;
-    case 6: hipLaunchKernelGGL((k_pairs<PC, false, true, true>), g, dim3(256), 0, st, A); break;
-    case 7: hipLaunchKernelGGL((k_pairs<PC, true, true, true>), g, dim3(256), 0, st, A); break;
-    case 8: hipLaunchKernelGGL((k_pairs<MF, true, false, false>), g, dim3(256), 0, st, A); break;
-    case 9: hipLaunchKernelGGL((k_pairs<MF, false, false, false>), g, dim3(256), 0, st, A); break;
+    case 0: hipLaunchKernelGGL((k_pairs<PC, false, false, false, 4>), g, dim3(256), 0, st, A); break;
+    case 1: hipLaunchKernelGGL((k_pairs<PC, true, false, false, 4>), g, dim3(256), 0, st, A); break;
+    case 2: hipLaunchKernelGGL((k_pairs<PC, false, true, false, 4>), g, dim3(256), 0, st, A); break;
+    case 3: hipLaunchKernelGGL((k_pairs<PC, true, true, false, 4>), g, dim3(256), 0, st, A); break;
+    case 4: hipLaunchKernelGGL((k_pairs<PC, false, false, true, 4>), g, dim3(256), 0, st, A); break;
+    case 5: hipLaunchKernelGGL((k_pairs<PC, true, false, true, 4>), g, dim3(256), 0, st, A); break;
+    case 6: hipLaunchKernelGGL((k_pairs<PC, false, true, true, 4>), g, dim3(256), 0, st, A); break;
+    case 7: hipLaunchKernelGGL((k_pairs<PC, true, true, true, 4>), g, dim3(256), 0, st, A); break;
+    case 8: hipLaunchKernelGGL((k_pairs<MF, true, false, false, 4>), g, dim3(256), 0, st, A); break;
+    case 9: hipLaunchKernelGGL((k_pairs<MF, false, false, false, 4>), g, dim3(256), 0, st, A); break;
     default: break;
   }
 }
 
-// One launch over the tiles of several jobs of fast-path variant v (a batch); false for other variants.
-static bool launch_pairs_multi(int v, unsigned nblocks, hipStream_t st, const RunArgs *jobs, const int64_t *toff,
-                               int nj)
+// One launch over the tiles of several jobs of fast-path variant v and tile rows 4 rp (a batch); false for other
+// variants.
+template <int RP>
+static bool launch_pairs_multi_rp(int v, const dim3 g, hipStream_t st, const RunArgs *jobs, const int64_t *toff,
+                                  int nj)
 {
   constexpr int PC = TN93G_BACKEND_POPC, MF = TN93G_BACKEND_MFMA;
-  const dim3 g(nblocks), b(256);
+  const dim3 b(256);
   switch (v) {
-    case 1: hipLaunchKernelGGL((k_pairs_multi<PC, true, false, false>), g, b, 0, st, jobs, toff, nj); return true;
-    case 3: hipLaunchKernelGGL((k_pairs_multi<PC, true, true, false>), g, b, 0, st, jobs, toff, nj); return true;
-    case 5: hipLaunchKernelGGL((k_pairs_multi<PC, true, false, true>), g, b, 0, st, jobs, toff, nj); return true;
-    case 7: hipLaunchKernelGGL((k_pairs_multi<PC, true, true, true>), g, b, 0, st, jobs, toff, nj); return true;
-    case 8: hipLaunchKernelGGL((k_pairs_multi<MF, true, false, false>), g, b, 0, st, jobs, toff, nj); return true;
+    case 1: hipLaunchKernelGGL((k_pairs_multi<PC, true, false, false, RP>), g, b, 0, st, jobs, toff, nj); return true;
+    case 3: hipLaunchKernelGGL((k_pairs_multi<PC, true, true, false, RP>), g, b, 0, st, jobs, toff, nj); return true;
+    case 5: hipLaunchKernelGGL((k_pairs_multi<PC, true, false, true, RP>), g, b, 0, st, jobs, toff, nj); return true;
+    case 7: hipLaunchKernelGGL((k_pairs_multi<PC, true, true, true, RP>), g, b, 0, st, jobs, toff, nj); return true;
+    case 8:
+      if constexpr (RP == 4) {
+        hipLaunchKernelGGL((k_pairs_multi<MF, true, false, false, 4>), g, b, 0, st, jobs, toff, nj);
+        return true;
+      }
+      return false;
     default: return false;
   }
+}
+static bool launch_pairs_multi(int v, int rp, unsigned nblocks, hipStream_t st, const RunArgs *jobs,
+                               const int64_t *toff, int nj)
+{
+  const dim3 g(nblocks);
+  if (rp == 2) return launch_pairs_multi_rp<2>(v, g, st, jobs, toff, nj);
+  if (rp == 8) return launch_pairs_multi_rp<8>(v, g, st, jobs, toff, nj);
+  return launch_pairs_multi_rp<4>(v, g, st, jobs, toff, nj);
+}
+
+// Sets a job's pair-kernel shape: backend and tile rows (4 rp), with the grid that follows.  The exact path and the
+// MFMA backend take tiles of 16 rows.
+static int job_set_shape(tn93g_ctx *c, Job &J, int backend, int rp)
+{
+  if (backend == TN93G_BACKEND_MFMA || !J.fast) rp = 4;
+  J.P.backend = backend;
+  J.rp = rp;
+  J.gy = std::max<int64_t>((J.nrows + 4 * rp - 1) / (4 * rp), 1);
+  J.ntiles = J.gx * J.gy;
+  J.A.ti = 4 * rp;
+  if (J.gy >= 65536) return c->fail(TN93G_E_ARG, "too many pair tiles in one run");
+  return 0;
 }
 
 // A job's kernels in three parts on stream `st`, with no host synchronisation (a batch captures them into one graph):
@@ -3011,7 +3119,7 @@ static void job_enqueue_pairs(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
 {
   if (timing) (void)hipEventRecord(c->ev[0], st);
   const int v = pairs_variant(J);
-  if (v >= 0) launch_pairs(v, dim3((unsigned)J.gx, (unsigned)J.gy), st, J.A);
+  if (v >= 0) launch_pairs(v, J.rp, dim3((unsigned)J.gx, (unsigned)J.gy), st, J.A);
   if (timing) (void)hipEventRecord(c->ev[1], st);
 }
 
@@ -3120,6 +3228,8 @@ static void job_finish_common(const Job &J, const unsigned long long *hrep, tn93
   R->n_replayed = (int64_t)J.n_slow;
   R->n_uncertain = (int64_t)J.n_unc;
   R->n_host_exact = J.n_exact;
+  R->backend_used = J.P.backend == TN93G_BACKEND_AUTO ? TN93G_BACKEND_POPC : J.P.backend;
+  R->tile_rows_used = 4 * J.rp;
   // self_v_self histogram entries (swifttn93.cpp:2406-2415)
   if (J.A.triangle)
     for (int64_t i = J.A.c0; i < J.A.n; i++) {
@@ -3766,47 +3876,69 @@ int tn93g_result_certify(const tn93g_result *r, int precision)
 
 
 // Autotuner.  A job's shape is its words per sequence, its size class (log2 of rows and of columns), its mode
-// class and its pair kind; the first job of a shape with TN93G_BACKEND_AUTO and TN93G_FLAG_AUTOTUNE runs its pair
-// kernel once with every variant (the counts are exact either way, so the results are the same) and the fastest
-// is kept for the context's lifetime.  Without the flag, AUTO takes the tuned choice when there is one, else the
-// popcount backend.
+// class and its pair kind.  The pair kernel's own shape has two choices: the backend (popcount or MFMA, when the
+// caller leaves it to TN93G_BACKEND_AUTO) and the tile rows (8, 16 or 32 rows by 64 columns, when tile_rows is 0).
+// The first job of a job shape with TN93G_FLAG_AUTOTUNE runs its kernels once with every candidate (the counts are
+// exact either way, so the results are the same) and the fastest pair kernel is kept for the context's lifetime.
+// Without the flag a job takes the tuned choice when there is one, else popcount in tiles of 16 rows.
 static uint64_t tune_shape(const Job &J)
 {
   auto lg = [](int64_t v) { return (uint64_t)(v > 0 ? 64 - __builtin_clzll((unsigned long long)v) : 0); };
   const uint64_t mode = J.A.gapmm ? 3 : J.A.all_slow ? 2 : J.A.skip_events ? 1 : 0;
-  return (uint64_t)J.A.W << 24 | lg(J.nrows) << 16 | lg(J.A.ncols) << 8 | mode << 2 | (uint64_t)J.P.kind;
+  const uint64_t free_dims = (J.P.backend == TN93G_BACKEND_AUTO ? 1u : 0u) | (J.P.tile_rows == 0 ? 2u : 0u);
+  return free_dims << 60 | (uint64_t)J.A.W << 24 | lg(J.nrows) << 16 | lg(J.A.ncols) << 8 | mode << 2 |
+         (uint64_t)J.P.kind;
+}
+
+// A tuned choice is backend | rp << 8.
+static int tuned_choice(const tn93g_ctx *c, const Job &J, int *backend, int *rp)
+{
+  *backend = J.P.backend == TN93G_BACKEND_AUTO ? TN93G_BACKEND_POPC : J.P.backend;
+  *rp = J.rp;
+  auto it = c->tuned.find(tune_shape(J));
+  if (it == c->tuned.end()) return 0;
+  *backend = it->second & 255;
+  *rp = it->second >> 8;
+  return 1;
 }
 
 static int autotune(tn93g_ctx *c, Job &J)
 {
-  if (J.P.backend != TN93G_BACKEND_AUTO) return 0;
-  const uint64_t key = tune_shape(J);
-  auto it = c->tuned.find(key);
-  if (it != c->tuned.end()) {
-    J.P.backend = it->second;
-    return 0;
+  const bool free_backend = J.P.backend == TN93G_BACKEND_AUTO, free_rows = J.P.tile_rows == 0;
+  const uint64_t key = tune_shape(J);  // before the trials set J.P.backend
+  int backend, rp;
+  const bool known = tuned_choice(c, J, &backend, &rp) != 0;
+  if (known || !(free_backend || free_rows) || !(J.P.flags & TN93G_FLAG_AUTOTUNE))
+    return job_set_shape(c, J, backend, rp);
+  // the candidates: popcount in every tile height allowed, MFMA in tiles of 16 (not for dense-N sets, whose pair
+  // kernel is popcount only); the exact path tiles 16 rows
+  std::vector<std::pair<int, int>> cand;
+  for (int b : {TN93G_BACKEND_POPC, TN93G_BACKEND_MFMA}) {
+    if (!free_backend && b != J.P.backend) continue;
+    if (b == TN93G_BACKEND_MFMA && J.A.nplane) continue;
+    for (int r : {2, 4, 8}) {
+      if (!free_rows && r != J.rp) continue;
+      if ((b == TN93G_BACKEND_MFMA || !J.fast) && r != 4) continue;
+      cand.push_back({b, r});
+    }
   }
-  if (!(J.P.flags & TN93G_FLAG_AUTOTUNE)) {
-    J.P.backend = TN93G_BACKEND_POPC;
-    return 0;
-  }
-  const int cand[2] = {TN93G_BACKEND_POPC, TN93G_BACKEND_MFMA};
+  if (cand.empty()) return job_set_shape(c, J, backend, rp);
   float best = 0;
-  int pick = TN93G_BACKEND_POPC;
-  for (int v : cand) {
-    J.P.backend = v;
+  std::pair<int, int> pick = cand[0];
+  for (size_t k = 0; k < cand.size(); k++) {
+    int rc = job_set_shape(c, J, cand[k].first, cand[k].second);
+    if (rc) return rc;
     job_enqueue(c, J, true, c->stream, 0);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    if (v == cand[0] || ms < best) {
+    if (k == 0 || ms < best) {
       best = ms;
-      pick = v;
+      pick = cand[k];
     }
   }
-  c->tuned[key] = pick;
-  J.P.backend = pick;
-  return 0;
+  c->tuned[key] = pick.first | pick.second << 8;
+  return job_set_shape(c, J, pick.first, pick.second);
 }
 
 int tn93g_run_bytes(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, int64_t *bytes)
@@ -3909,9 +4041,10 @@ int tn93g_run_noalloc(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, t
   ar.size = TN93G_NOALLOC_SCRATCH_BYTES;
   int rc = job_prepare(c, J, nullptr, &ar);
   if (rc) return rc;
-  if (J.P.backend == TN93G_BACKEND_AUTO) {  // the tuned choice of this shape when there is one (no trial runs here)
-    auto it = c->tuned.find(tune_shape(J));
-    J.P.backend = it != c->tuned.end() ? it->second : TN93G_BACKEND_POPC;
+  {  // the tuned choice of this shape when there is one (no trial runs here)
+    int backend, rp;
+    (void)tuned_choice(c, J, &backend, &rp);
+    if ((rc = job_set_shape(c, J, backend, rp))) return rc;
   }
   job_enqueue(c, J, true, c->stream, 0);
   HIPCHK(c, hipGetLastError());
@@ -4012,7 +4145,7 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
     // Pair kernels of jobs with the same fast-path variant run as one launch over all their tiles (small alignments
     // fill the device together): per group its RunArgs and tile offsets, in c->multi, uploaded before the graph.
     struct Group {
-      int v;
+      int v, rp;
       std::vector<size_t> jobs;
       size_t off_args, off_toff;
       int64_t tiles;
@@ -4028,14 +4161,15 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
     {
       std::map<int, std::vector<size_t>> byv;
       for (size_t t = 0; t < m; t++) {
-        const int v = pairs_variant(jobs[t]);
-        if (v == 1 || v == 3 || v == 5 || v == 7 || v == 8) byv[v].push_back(t);
+        const int v = pairs_variant(jobs[t]);  // grouped by variant and tile rows
+        if (v == 1 || v == 3 || v == 5 || v == 7 || v == 8) byv[v + 16 * jobs[t].rp].push_back(t);
       }
       size_t bytes = 0;
       for (auto &kv : byv) {
         if (kv.second.size() < 2) continue;
         Group G;
-        G.v = kv.first;
+        G.v = kv.first % 16;
+        G.rp = kv.first / 16;
         G.jobs = kv.second;
         G.off_args = bytes;
         bytes = (bytes + G.jobs.size() * sizeof(RunArgs) + 255) & ~(size_t)255;
@@ -4178,7 +4312,7 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
         if (!rc) hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, stamps);
         for (const Group &G : groups)
           if (!rc)
-            (void)launch_pairs_multi(G.v, (unsigned)G.tiles, c->stream,
+            (void)launch_pairs_multi(G.v, G.rp, (unsigned)G.tiles, c->stream,
                                      (const RunArgs *)((char *)c->multi.p + G.off_args),
                                      (const int64_t *)((char *)c->multi.p + G.off_toff), (int)G.jobs.size());
         if (!rc) hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, stamps + 1);

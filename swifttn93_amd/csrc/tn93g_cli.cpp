@@ -376,6 +376,7 @@ int prepare(Case &C, int argc, const char **argv)
   P.min_distance = A.min_distance;
   P.flags = C.do_fst ? TN93G_FLAG_EXACT : 0u;  // F_ST combines the means non-monotonically: resolve logs exactly
   if (getenv("TN93G_FLAGS")) P.flags |= (uint32_t)strtoul(getenv("TN93G_FLAGS"), nullptr, 0);
+  if (getenv("TN93G_TILE_ROWS")) P.tile_rows = (int32_t)atoi(getenv("TN93G_TILE_ROWS"));
   C.prec = (!A.quiet && A.output == stdout) ? 3 : 6;  // tn93's progress line leaves cerr at precision 3
   return -1;
 }

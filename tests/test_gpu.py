@@ -430,3 +430,53 @@ def test_event_chunks(engine):
         _same(fast, slow)
         _same(fast, mf)
     s.free()
+
+
+@pytest.mark.parametrize("case", ["c1s", "dense_n", "chunks", "edges", "cross_append"])
+def test_tile_shapes_agree(engine, case):
+    """The pair kernel in tiles of 8, 16 and 32 rows (the autotuner's shapes) gives the same outputs bit for bit, on
+    ambiguity-heavy sets (B/D/H/V: replays), dense N, event chunks that overflow the LDS, lengths around word
+    boundaries with gap runs at both ends, and -s / appended ranges; the 16-row shape also against exact mode."""
+    if case == "c1s":
+        codes, L_ov = gen.c1s(300, seed=171), 500
+    elif case == "dense_n":
+        codes, L_ov = gen.c2s(120, seed=172), 20000
+    elif case == "chunks":
+        rs = gen.Stream(173)
+        n, L = 100, 2000
+        codes = rs.integers(4, n * L).reshape(n, L).astype(np.uint8)
+        amb = rs.uniform(n * L).reshape(n, L) < 0.15
+        pool = np.array([5, 6, 7, 8, 9, 10, 11, 15], dtype=np.uint8)
+        codes[amb] = pool[rs.integers(len(pool), n * L).reshape(n, L)][amb]
+        L_ov = 100
+    else:
+        codes, L_ov = _edge_codes(174, 150, 97, False), 5
+    s = engine.upload(codes)
+    kinds = [dict()] if case != "cross_append" else [dict(kind=T.CROSS, n1=37), dict(kind=T.APPEND, n1=101)]
+    for kind in kinds:
+        for mode in (T.RESOLVE, T.AVERAGE, T.GAPMM, T.SUBSET):
+            kw = dict(mode=mode, threshold=1.0, min_overlap=L_ov, resolve_fraction=0.05 if mode == T.RESOLVE else 1.0,
+                      subset_mask=(1 << 5) | (1 << 6) | (1 << 15), backend=T.BACKEND_POPC, **kind)
+            print(f"{case} {kind} mode={mode}", flush=True)
+            base = engine.run(s, tile_rows=16, **kw)
+            for tr in (8, 32):
+                other = engine.run(s, tile_rows=tr, **kw)
+                _same(base, other)
+                assert (base["link_dist"] == other["link_dist"]).all()
+                assert base["n_replayed"] == other["n_replayed"]
+            exact = engine.run(s, tile_rows=16, flags=T.FLAG_EXACT, **kw)
+            _same(base, exact)
+    s.free()
+
+
+def test_autotuned_shapes(engine):
+    """With TN93G_FLAG_AUTOTUNE and tile_rows 0 the autotuner times the backends and tile heights on the first job
+    of a shape; the tuned job gives the untuned job's outputs."""
+    codes = gen.c1s(400, seed=175)
+    s = engine.upload(codes)
+    kw = dict(mode=T.RESOLVE, threshold=0.05, min_overlap=500, resolve_fraction=0.05)
+    want = engine.run(s, tile_rows=16, backend=T.BACKEND_POPC, **kw)
+    for _ in range(2):  # the first call tunes, the second takes the tuned choice
+        got = engine.run(s, backend=T.BACKEND_AUTO, flags=T.FLAG_AUTOTUNE, **kw)
+        _same(want, got)
+    s.free()

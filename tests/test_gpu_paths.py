@@ -117,6 +117,36 @@ def test_jobs_c3_vs_oracle(tmp_path):
     assert not bad, bad[:5]
 
 
+@pytest.mark.parametrize("rows", ["8", "32"])
+def test_tile_rows_vs_oracle(rows, tmp_path):
+    """The CLI with the pair kernel forced to tiles of 8 or 32 rows (TN93G_TILE_ROWS) against the oracle, every -a
+    mode, -t 0.015 and every pair written, a single file and -s."""
+    codes = gen.c1s(420, seed=181)
+    gen.write_fasta(tmp_path / "in.fa", codes)
+    gen.write_fasta(tmp_path / "ref.fa", codes[:150], gen.names_for(150, "r"))
+    cfgs = [("-a", "resolve", "-g", "0.05"), ("-a", "average"), ("-a", "skip"), ("-a", "gapmm"), ("-a", "RYN")]
+    cases, want = [], []
+    for ci, cfg in enumerate(cfgs):
+        for t in ("0.015", "1"):
+            for extra in ([], ["-s", str(tmp_path / "ref.fa")]):
+                d = tmp_path / f"k{ci}_{t}_{len(extra)}"
+                d.mkdir()
+                argv = ["-q", "-t", t, "-l", "500", *cfg, *extra]
+                p = subprocess.run([str(_oracle_cli()), *argv, "-o", str(d / "want.txt"), str(tmp_path / "in.fa")],
+                                   capture_output=True, check=True, timeout=600)
+                want.append(((d / "want.txt").read_bytes(), p.stdout))
+                cases.append((d, [*argv, "-o", "out.txt", str(tmp_path / "in.fa")]))
+    bf = tmp_path / "jobs.txt"
+    bf.write_text("".join(SEP.join([str(d)] + list(a)) + "\n" for d, a in cases))
+    p = subprocess.run([str(_cli()), "--batch", str(bf)], capture_output=True, timeout=600,
+                       env=dict(os.environ, TN93G_TILE_ROWS=rows))
+    assert p.returncode == 0, p.stderr[-2000:]
+    for (d, argv), (wo, ws) in zip(cases, want):
+        assert (d / "rc.txt").read_text().strip() == "0", argv
+        assert (d / "out.txt").read_bytes() == wo, argv
+        assert (d / "stdout.bin").read_bytes() == ws, argv
+
+
 def _exec(argv, cwd, stdin=None, exe=None):
     p = subprocess.run([str(exe or _cli()), *argv], cwd=cwd, stdin=stdin, capture_output=True, timeout=300)
     return p.returncode, p.stdout, p.stderr
