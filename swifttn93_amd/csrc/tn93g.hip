@@ -217,6 +217,7 @@ struct SetDev {
   uint2 *nruns;
   uint4 *pre;        // [n][W+1] plane counts before each word boundary
   uint4 *tb;         // [n] pre[s][W], the totals of sequence s
+  uint32_t *npre;    // [n][W+1] N sites before each word boundary
   int32_t *grun_off; // [n+1] gap runs
   uint2 *gruns;
   int ndense;        // N sites are not in ev: handled through nmask (pair kernel) and nruns (replay)
@@ -267,6 +268,7 @@ struct RunArgs {
   const uint2 *nruns;
   int nplane;              // N sites are counted from nmask / nruns, not in the event lists
   const uint4 *pre;        // [n][W+1] plane counts (clean, C|T, G|T, T) of the words before each word boundary
+  const uint32_t *npre;    // [n][W+1] the same for the N sites (nmask)
   const uint4 *tb;         // [n] the same over all words (the totals)
   const int32_t *grun_off; // [n+1] and the gap runs [start, end) of each sequence
   const uint2 *gruns;
@@ -600,28 +602,40 @@ __global__ void k_runfill(const uint8_t *__restrict__ codes, const int32_t *__re
 
 // pre[s][w] = plane counts (clean, C|T, G|T, T) of sequence s over its words [0, w), w = 0..W: one wave per sequence,
 // 64 word boundaries per step
-__global__ void k_prefix(const uint4 *__restrict__ planes, uint4 *__restrict__ pre, uint4 *__restrict__ tb, int64_t W,
-                         int64_t s0)
+__global__ void k_prefix(const uint4 *__restrict__ planes, const uint32_t *__restrict__ nmask, uint4 *__restrict__ pre,
+                         uint4 *__restrict__ tb, uint32_t *__restrict__ npre, int64_t W, int64_t s0)
 {
   const int64_t s = s0 + blockIdx.x;
   const int lane = threadIdx.x;
   uint4 carry = make_uint4(0, 0, 0, 0);
+  uint32_t ncarry = 0;
   for (int64_t w0 = 0; w0 <= W; w0 += 64) {
     const int64_t w = w0 + lane;
     uint4 v = make_uint4(0, 0, 0, 0);
+    uint32_t nv = 0;
     if (w > 0 && w <= W) {
       const uint4 q = planes[s * W + w - 1];
       v = make_uint4(__popc(q.x), __popc(q.y), __popc(q.z), __popc(q.w));
+      nv = __popc(nmask[s * W + w - 1]);
     }
     for (int o = 1; o < 64; o <<= 1) {  // inclusive scan over the wave
       const uint4 t = make_uint4(__shfl_up(v.x, o, 64), __shfl_up(v.y, o, 64), __shfl_up(v.z, o, 64),
                                  __shfl_up(v.w, o, 64));
-      if (lane >= o) add4(v, t);
+      const uint32_t tn = __shfl_up(nv, o, 64);
+      if (lane >= o) {
+        add4(v, t);
+        nv += tn;
+      }
     }
     add4(v, carry);
-    if (w <= W) pre[s * (W + 1) + w] = v;
+    nv += ncarry;
+    if (w <= W) {
+      pre[s * (W + 1) + w] = v;
+      npre[s * (W + 1) + w] = nv;
+    }
     if (w == W) tb[s] = v;
     carry = make_uint4(__shfl(v.x, 63, 64), __shfl(v.y, 63, 64), __shfl(v.z, 63, 64), __shfl(v.w, 63, 64));
+    ncarry = __shfl(nv, 63, 64);
   }
 }
 
@@ -715,6 +729,17 @@ __device__ inline uint4 plane_prefix(const uint4 *__restrict__ pre, const uint4 
     r.z += __popc(v.z & m);
     r.w += __popc(v.w & m);
   }
+  return r;
+}
+
+// N sites of sequence s over [0, p) (dense-N sets: nmask and its prefix counts).
+__device__ inline uint32_t n_prefix(const uint32_t *__restrict__ npre, const uint32_t *__restrict__ nmask, int64_t W,
+                                    int64_t s, uint32_t p)
+{
+  const int64_t w = p >> 5;
+  uint32_t r = npre[s * (W + 1) + w];
+  const uint32_t b = p & 31u;
+  if (b) r += __popc(nmask[s * W + w] & ((1u << b) - 1u));
   return r;
 }
 
@@ -1252,25 +1277,31 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
       for (int ig = 0; ig < TI_ / 4; ig++)
 #pragma unroll
         for (int jg = 0; jg < 4; jg++) macc[ig][jg] = v4i{0, 0, 0, 0};
+      // Operands of one 64-site block: the 4 row groups (1 KB each, shared by the waves) pass through LDS, double
+      // buffered (groups 4 buf .. 4 buf + 3 of the operand area), one group per wave's 64 lanes; each wave's own 4
+      // column groups go straight to registers.  The next block's operands are loaded while this one's MFMAs run, so
+      // one barrier per block remains.  The groups lie inside the one-hot array (npad rounded past every tile).
       const uint4 *gI = reinterpret_cast<const uint4 *>(A.onehot) + (i0 >> 2) * A.W64 * 64;
       const uint4 *gJ = reinterpret_cast<const uint4 *>(A.onehot) + (j0 >> 2) * A.W64 * 64;
+      uint4 ra = gI[(int64_t)wave * A.W64 * 64 + lane];
+      uint4 cb[4];
+#pragma unroll
+      for (int jg = 0; jg < 4; jg++) cb[jg] = gJ[(int64_t)(4 * wave + jg) * A.W64 * 64 + lane];
       for (int64_t blk = 0; blk < A.W64; blk++) {
-        __syncthreads();
-        for (int k = tid; k < (TI_ + TJ) / 4 * 64; k += 256) {
-          const int g = k >> 6, q = k & 63;
-          lds.u.ops[g][q] = g < TI_ / 4 ? gI[((int64_t)g * A.W64 + blk) * 64 + q]
-                                       : gJ[((int64_t)(g - TI_ / 4) * A.W64 + blk) * 64 + q];
-        }
-        __syncthreads();
+        const int buf = (int)(blk & 1);
+        lds.u.ops[4 * buf + wave][lane] = ra;
         v4i bop[4];
 #pragma unroll
-        for (int jg = 0; jg < 4; jg++) {
-          const uint4 v = lds.u.ops[TI_ / 4 + 4 * wave + jg][lane];
-          bop[jg] = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
+        for (int jg = 0; jg < 4; jg++) bop[jg] = v4i{(int)cb[jg].x, (int)cb[jg].y, (int)cb[jg].z, (int)cb[jg].w};
+        __syncthreads();
+        if (blk + 1 < A.W64) {
+          ra = gI[((int64_t)wave * A.W64 + blk + 1) * 64 + lane];
+#pragma unroll
+          for (int jg = 0; jg < 4; jg++) cb[jg] = gJ[((int64_t)(4 * wave + jg) * A.W64 + blk + 1) * 64 + lane];
         }
 #pragma unroll
         for (int ig = 0; ig < TI_ / 4; ig++) {
-          const uint4 v = lds.u.ops[ig][lane];
+          const uint4 v = lds.u.ops[4 * buf + ig][lane];
           const v4i aop = v4i{(int)v.x, (int)v.y, (int)v.z, (int)v.w};
 #pragma unroll
           for (int jg = 0; jg < 4; jg++)
@@ -1314,6 +1345,27 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
   if (do_events) chunk_codes<TI_>(A, te, sEv, L.g, i0, j0, tid);
   if constexpr (MARG)
     if (do_fast) base_sums<RP>(A, sTB, tr, sRg, ib, wave, lane, j, h0, h1, S);
+  if constexpr (NP && MF)
+    if (do_fast && j < A.n) {
+      // MFMA backend: the one-hot operands carry clean sites only, so the rows' N sites against the column (its
+      // planes and its N sites) come from the rows' N runs, each a difference of the column's prefix counts
+#pragma unroll
+      for (int a = 0; a < RP; a++) {
+        const int64_t i = ib + a;
+        if (i >= A.r1) continue;  // uniform over the wave
+        const int q1 = A.nrun_off[i + 1];
+        for (int q = A.nrun_off[i]; q < q1; q++) {
+          const uint2 r = A.nruns[q];
+          const uint4 hi = plane_prefix(A.pre, A.planes, A.W, j, r.y);
+          const uint4 lo = plane_prefix(A.pre, A.planes, A.W, j, r.x);
+          ncn[a].r[0] += hi.x - lo.x;
+          ncn[a].r[1] += hi.y - lo.y;
+          ncn[a].r[2] += hi.z - lo.z;
+          ncn[a].r[3] += hi.w - lo.w;
+          ncn[a].nn += n_prefix(A.npre, A.nmask, A.W, j, r.y) - n_prefix(A.npre, A.nmask, A.W, j, r.x);
+        }
+      }
+    }
   if constexpr (NP)
     if (do_fast && j < A.n) {
       // the column's N sites against the rows' planes: N comes in runs, so each run is a difference of the rows'
@@ -2777,6 +2829,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
   A.nruns = d.nruns;
   A.pre = d.pre;
   A.tb = d.tb;
+  A.npre = d.npre;
   A.grun_off = d.grun_off;
   A.gruns = d.gruns;
   // The marginal form of the pair kernel trades 13 of 24 operations per word pair for, per pair, two prefix
@@ -2983,13 +3036,12 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
 }
 
 // The pair-kernel variant of a job: 0-7 the popcount backend (bit 0 fast path, bit 1 dense-N planes, bit 2 marginal
-// form), 8 / 9 the MFMA backend (fast / exact path); -1 when the run has no pair.
+// form), 8 / 9 the MFMA backend (fast / exact path), 10 / 11 the same for dense-N sets; -1 when the run has no pair.
 static int pairs_variant(const Job &J)
 {
   const RunArgs &A = J.A;
   if (J.nrows <= 0 || A.ncols <= 0) return -1;
-  const bool mf = J.P.backend == TN93G_BACKEND_MFMA && !A.nplane;  // dense-N counting is popcount only
-  if (mf) return J.fast ? 8 : 9;
+  if (J.P.backend == TN93G_BACKEND_MFMA) return (J.fast ? 8 : 9) + (A.nplane ? 2 : 0);
   const bool marg = J.B->tab_fold && !A.skip_events && J.sparse_gaps;
   return (J.fast ? 1 : 0) | (A.nplane ? 2 : 0) | (marg ? 4 : 0);
 }
@@ -3024,6 +3076,8 @@ static void launch_pairs(int v, int rp, dim3 g, hipStream_t st, const RunArgs &A
     case 7: hipLaunchKernelGGL((k_pairs<PC, true, true, true, 4>), g, dim3(256), 0, st, A); break;
     case 8: hipLaunchKernelGGL((k_pairs<MF, true, false, false, 4>), g, dim3(256), 0, st, A); break;
     case 9: hipLaunchKernelGGL((k_pairs<MF, false, false, false, 4>), g, dim3(256), 0, st, A); break;
+    case 10: hipLaunchKernelGGL((k_pairs<MF, true, true, false, 4>), g, dim3(256), 0, st, A); break;
+    case 11: hipLaunchKernelGGL((k_pairs<MF, false, true, false, 4>), g, dim3(256), 0, st, A); break;
     default: break;
   }
 }
@@ -3042,8 +3096,12 @@ static bool launch_pairs_multi_rp(int v, const dim3 g, hipStream_t st, const Run
     case 5: hipLaunchKernelGGL((k_pairs_multi<PC, true, false, true, RP>), g, b, 0, st, jobs, toff, nj); return true;
     case 7: hipLaunchKernelGGL((k_pairs_multi<PC, true, true, true, RP>), g, b, 0, st, jobs, toff, nj); return true;
     case 8:
+    case 10:
       if constexpr (RP == 4) {
-        hipLaunchKernelGGL((k_pairs_multi<MF, true, false, false, 4>), g, b, 0, st, jobs, toff, nj);
+        if (v == 8)
+          hipLaunchKernelGGL((k_pairs_multi<MF, true, false, false, 4>), g, b, 0, st, jobs, toff, nj);
+        else
+          hipLaunchKernelGGL((k_pairs_multi<MF, true, true, false, 4>), g, b, 0, st, jobs, toff, nj);
         return true;
       }
       return false;
@@ -3597,6 +3655,7 @@ int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
   SCHK(hipMalloc(&d.grun_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.pre, (size_t)(nn * (d.W + 1)) * sizeof(uint4)));
   SCHK(hipMalloc(&d.tb, (size_t)nn * sizeof(uint4)));
+  SCHK(hipMalloc(&d.npre, (size_t)(nn * (d.W + 1)) * sizeof(uint32_t)));
   SCHK(hipMalloc(&d.nmask, (size_t)(nn * d.W) * sizeof(uint32_t)));
   SCHK(hipMalloc(&d.weights, (size_t)nn * sizeof(int64_t)));
   // padding reads as gaps (codes, site-major codes) and as nothing (planes, one-hot operands, event offsets)
@@ -3669,7 +3728,8 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
     const int64_t tot = n * d.W;
     hipLaunchKernelGGL(k_planes, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.planes,
                        d.nmask, n0, n1, L, d.Ls, d.W);
-    hipLaunchKernelGGL(k_prefix, dim3((unsigned)n), dim3(64), 0, c->stream, (const uint4 *)d.planes, d.pre, d.tb, d.W,
+    hipLaunchKernelGGL(k_prefix, dim3((unsigned)n), dim3(64), 0, c->stream, (const uint4 *)d.planes,
+                       (const uint32_t *)d.nmask, d.pre, d.tb, d.npre, d.W,
                        n0);
     const int64_t ob = n0 & ~(int64_t)3, oe = (n1 + 3) & ~(int64_t)3;
     const int64_t toh = (oe - ob) * d.W64 * 64;
@@ -3802,7 +3862,7 @@ void tn93g_set_free(tn93g_set *S)
   (void)hipSetDevice(S->ctx->device);
   (void)hipDeviceSynchronize();  // runs of any context may still read the set
   void *ps[] = {S->d.codes,   S->d.codesT, S->d.planes,   S->d.onehot, S->d.ev_off,   S->d.ev,    S->d.evg_off,
-                S->d.evg,     S->d.nrun_off, S->d.nruns, S->d.nmask,  S->d.grun_off, S->d.gruns, S->d.pre, S->d.tb,
+                S->d.evg,     S->d.nrun_off, S->d.nruns, S->d.nmask,  S->d.grun_off, S->d.gruns, S->d.pre, S->d.tb, S->d.npre,
                 S->d.weights};
   for (void *p : ps)
     if (p) (void)hipFree(p);
@@ -3910,12 +3970,10 @@ static int autotune(tn93g_ctx *c, Job &J)
   const bool known = tuned_choice(c, J, &backend, &rp) != 0;
   if (known || !(free_backend || free_rows) || !(J.P.flags & TN93G_FLAG_AUTOTUNE))
     return job_set_shape(c, J, backend, rp);
-  // the candidates: popcount in every tile height allowed, MFMA in tiles of 16 (not for dense-N sets, whose pair
-  // kernel is popcount only); the exact path tiles 16 rows
+  // the candidates: popcount in every tile height allowed, MFMA in tiles of 16; the exact path tiles 16 rows
   std::vector<std::pair<int, int>> cand;
   for (int b : {TN93G_BACKEND_POPC, TN93G_BACKEND_MFMA}) {
     if (!free_backend && b != J.P.backend) continue;
-    if (b == TN93G_BACKEND_MFMA && J.A.nplane) continue;
     for (int r : {2, 4, 8}) {
       if (!free_rows && r != J.rp) continue;
       if ((b == TN93G_BACKEND_MFMA || !J.fast) && r != 4) continue;
@@ -4162,7 +4220,7 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       std::map<int, std::vector<size_t>> byv;
       for (size_t t = 0; t < m; t++) {
         const int v = pairs_variant(jobs[t]);  // grouped by variant and tile rows
-        if (v == 1 || v == 3 || v == 5 || v == 7 || v == 8) byv[v + 16 * jobs[t].rp].push_back(t);
+        if (v == 1 || v == 3 || v == 5 || v == 7 || v == 8 || v == 10) byv[v + 16 * jobs[t].rp].push_back(t);
       }
       size_t bytes = 0;
       for (auto &kv : byv) {

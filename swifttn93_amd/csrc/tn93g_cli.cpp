@@ -377,6 +377,8 @@ int prepare(Case &C, int argc, const char **argv)
   P.flags = C.do_fst ? TN93G_FLAG_EXACT : 0u;  // F_ST combines the means non-monotonically: resolve logs exactly
   if (getenv("TN93G_FLAGS")) P.flags |= (uint32_t)strtoul(getenv("TN93G_FLAGS"), nullptr, 0);
   if (getenv("TN93G_TILE_ROWS")) P.tile_rows = (int32_t)atoi(getenv("TN93G_TILE_ROWS"));
+  if (const char *b = getenv("TN93G_BACKEND"))  // popc | mfma (diagnostics: the counts are exact either way)
+    P.backend = !strcmp(b, "mfma") ? TN93G_BACKEND_MFMA : !strcmp(b, "popc") ? TN93G_BACKEND_POPC : P.backend;
   C.prec = (!A.quiet && A.output == stdout) ? 3 : 6;  // tn93's progress line leaves cerr at precision 3
   return -1;
 }

@@ -239,8 +239,11 @@ C2_CONFIGS = [
 ]
 
 
-def test_long_alignment_c2s_vs_oracle(tmp_path):
-    """SARS-CoV-2-shaped long alignment (300 x 29,903, N-masked blocks of 100-2,000 columns) against the oracle."""
+@pytest.mark.parametrize("backend", ["popc", "mfma"])
+def test_long_alignment_c2s_vs_oracle(backend, tmp_path, monkeypatch):
+    """SARS-CoV-2-shaped long alignment (300 x 29,903, N-masked blocks of 100-2,000 columns) against the oracle,
+    on the popcount backend and on the int8 MFMA backend (clean/clean counts on the matrix unit, N from runs)."""
+    monkeypatch.setenv("TN93G_BACKEND", backend)
     codes = gen.c2s(300)
     fa = tmp_path / "in.fa"
     gen.write_fasta(fa, codes)
@@ -357,10 +360,16 @@ def test_dense_n_fast_equals_replay(engine, mode):
     fast = engine.run(s, **kw)
     slow = engine.run(s, flags=T.FLAG_ALL_SLOW, **kw)
     exact = engine.run(s, flags=T.FLAG_EXACT, **kw)
+    mf = engine.run(s, backend=T.BACKEND_MFMA, **kw)
+    mfx = engine.run(s, backend=T.BACKEND_MFMA, flags=T.FLAG_EXACT, **kw)
     _same(fast, slow)
     _same(fast, exact)
+    _same(fast, mf)
+    _same(exact, mfx)
+    assert mf["backend_used"] == T.BACKEND_MFMA
     assert slow["n_replayed"] == 160 * 159 // 2
     assert (fast["link_dist"] == slow["link_dist"]).all()
+    assert (fast["link_dist"] == mf["link_dist"]).all()
     s.free()
 
 

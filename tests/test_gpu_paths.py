@@ -326,3 +326,25 @@ def test_serving_full_panel_alone_vs_mixed():
             assert r.result["certified"] or r.result["exact"]
     finally:
         srv.close()
+
+
+def test_more_than_2_32_pairs_refused_cleanly(tmp_path):
+    """One run holds fewer than 2^32 pairs: a triangle of 92,683 sequences (2^32 + 60,209 pairs) is refused with
+    TN93G_E_ARG and a message before anything is allocated, through the C ABI and as the CLI's exit 1."""
+    n = 92683
+    codes = gen.Stream(191).integers(4, n * 32).reshape(n, 32).astype(np.uint8)  # random: distances saturate
+    codes[n - 5:] = codes[:5]  # five pairs at distance 0, all in the appended range below
+    eng = T.Engine(0)
+    try:
+        s = eng.upload(codes)
+        with pytest.raises(RuntimeError, match="2\\^32 pairs"):
+            eng.run(s, mode=T.RESOLVE, threshold=0.015, min_overlap=10)
+        ok = eng.run(s, kind=T.APPEND, n1=n - 1000, n2=n, mode=T.RESOLVE, threshold=0.015, min_overlap=10)
+        assert ok["skipped"] == 0 and ok["found_links"] >= 5  # the same set in a smaller range still runs
+        s.free()
+    finally:
+        eng.close()
+    fa = tmp_path / "big.fa"
+    gen.write_fasta(fa, codes)
+    rc, out, err = _exec(["-q", "-t", "0.015", "-l", "10", "-o", "out.csv", str(fa)], tmp_path)
+    assert rc == 1 and b"2^32 pairs" in err, err[-300:]
