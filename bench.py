@@ -533,13 +533,6 @@ def main():
     import numpy as np
     import swifttn93_amd as T
     from swifttn93_amd import gen
-    if a.config == "c2":
-        return bench_c2(a, T, gen)
-    if a.config == "c3":
-        return bench_c3(a, T, gen)
-    if a.config == "c4":
-        return bench_c4(a, T, gen)
-
     if a.copy_calibration:
         import torch
         x = torch.empty(a.copy_calibration << 18, dtype=torch.float32, device="cuda")
@@ -547,8 +540,16 @@ def main():
         for _ in range(3):
             y.copy_(x)
         torch.cuda.synchronize()
+        del x, y
         print(f"copy calibration: {a.copy_calibration} MiB read + {a.copy_calibration} MiB written per copy",
               file=sys.stderr)
+    if a.config == "c2":
+        return bench_c2(a, T, gen)
+    if a.config == "c3":
+        return bench_c3(a, T, gen)
+    if a.config == "c4":
+        return bench_c4(a, T, gen)
+
     codes = gen.c1()
     n, L = codes.shape
     n_pairs = n * (n - 1) // 2

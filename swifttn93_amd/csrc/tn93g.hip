@@ -2574,6 +2574,7 @@ struct RunBufs {
   std::vector<int32_t> h_spec_idx;  // host staging of spec_idx / spec_ids (kept until the next run of the slot)
   std::vector<int64_t> h_spec_ids;
   int64_t link_cap_hint = 0;  // link region size that sufficed last time
+  int64_t exact_hint = 0;     // host-exact list size that sufficed last time
   size_t link_spec = 4096;     // link records a single run reads back with its counters
   int tab_mode = -1;           // mode and subset of the tables in evtab/rtab (uploaded again only when they change)
   uint32_t tab_mask = 0;
@@ -2946,7 +2947,13 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
       need(B->chdr, (size_t)nslots * sizeof(ChunkHdr)) ||
       need(B->cdata, (size_t)nslots * 7 * 64 * sizeof(uint32_t)))  // slots are 7 x 64 words apart in both layouts
     return arena ? TN93G_E_ARG : TN93G_E_NOMEM;
-  J.exact_cap = A.exact ? (unsigned)std::min<int64_t>(J.npairs, 1ll << 31) : 1u;
+  // host-exact list: only the pairs with an uncertain logarithm are stored; it starts at 2^20 records (72 MB) or what
+  // sufficed last time, and a run that overflows it is repeated with the list it needed
+  // (TN93G_EXACT_LIST sets the starting size, for tests of the growth)
+  const char *es = getenv("TN93G_EXACT_LIST");
+  const int64_t start = es ? std::max<int64_t>(1, atoll(es)) : (1 << 20);
+  J.exact_cap = A.exact ? (unsigned)std::min<int64_t>({J.npairs, 1ll << 31, std::max<int64_t>(B->exact_hint, start)})
+                        : 1u;
   if (need(B->exact_list, (size_t)J.exact_cap * sizeof(ExactRec))) return arena ? TN93G_E_ARG : TN93G_E_NOMEM;
   J.link_cap = 1;
   if (J.fast) {
@@ -3437,7 +3444,7 @@ static int job_exact_result(tn93g_ctx *c, Job &J, tn93g_result *R)
   const size_t nr1 = J.nr1;
   const unsigned n_exact = J.n_exact;
   if (A.exact && n_exact) {
-    if (n_exact > J.exact_cap) return c->fail(TN93G_E_NOMEM, "host-exact list overflow");
+    if (n_exact > J.exact_cap) return c->fail(TN93G_E_NOMEM, "host-exact list overflow");  // tn93g_run grew it
     std::vector<ExactRec> ex(n_exact);
     HIPCHK(c, hipMemcpyAsync(ex.data(), B->exact_list.p, n_exact * sizeof(ExactRec), hipMemcpyDeviceToHost,
                              c->stream));
@@ -4046,6 +4053,10 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   }
   if (J.fast && J.max_region > J.link_cap) {  // more links than a region holds: grow the regions and run again
     J.B->link_cap_hint = (int64_t)J.max_region;
+    return tn93g_run(c, S, P, out);
+  }
+  if (J.A.exact && J.n_exact > J.exact_cap) {  // more uncertain pairs than the host-exact list holds: grow it
+    J.B->exact_hint = (int64_t)J.n_exact + J.n_exact / 4;
     return tn93g_run(c, S, P, out);
   }
   tn93g_result *R = (tn93g_result *)calloc(1, sizeof(tn93g_result));

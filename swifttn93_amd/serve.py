@@ -25,7 +25,7 @@ import time
 
 import numpy as np
 
-from . import APPEND, RESOLVE, Engine
+from . import APPEND, FLAG_EXACT, RESOLVE, Engine
 from . import gen
 
 
@@ -75,10 +75,18 @@ class Server:
         with self.lock:
             req.n1 = self.set.append(req.codes)
             req.n2 = req.n1 + req.codes.shape[0]
-            req.bytes = self.owner.run_bytes(self.set, kind=APPEND, n1=req.n1, n2=req.n2, **self.run_kw)
+            req.bytes = self.run_bytes(req.n1, req.n2)
             self.pending.append(req)
             self.lock.notify_all()
         return req
+
+    def run_bytes(self, n1, n2):
+        """Device memory an APPEND run over [n1, n2) may work in: the fast path's, or the exact path's when
+        certification sends it there (the larger of the two, tn93g_run_bytes)."""
+        kw = dict(self.run_kw, kind=APPEND, n1=n1, n2=n2)
+        fast = self.owner.run_bytes(self.set, **kw)
+        exact = self.owner.run_bytes(self.set, **dict(kw, flags=kw.get("flags", 0) | FLAG_EXACT))
+        return max(fast, exact)
 
     def _admissible(self):
         # a run larger than the whole budget is admitted alone
@@ -99,7 +107,7 @@ class Server:
                 while self.merge_rows and self.pending and rows + self.pending[0].n2 - self.pending[0].n1 <= \
                         self.merge_rows:
                     nxt = self.pending[0]
-                    b = self.owner.run_bytes(self.set, kind=APPEND, n1=group[0].n1, n2=nxt.n2, **self.run_kw)
+                    b = self.run_bytes(group[0].n1, nxt.n2)
                     if self.in_flight and self.in_flight + b > self.budget:
                         break
                     group.append(self.pending.popleft())

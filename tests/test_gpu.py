@@ -489,3 +489,20 @@ def test_autotuned_shapes(engine):
         got = engine.run(s, backend=T.BACKEND_AUTO, flags=T.FLAG_AUTOTUNE, **kw)
         _same(want, got)
     s.free()
+
+
+def test_exact_list_grows(monkeypatch):
+    """The host-exact list starts small (TN93G_EXACT_LIST) and a run with more uncertain pairs than it holds is
+    repeated with the list it needed: the result equals the fast path's."""
+    monkeypatch.setenv("TN93G_EXACT_LIST", "64")
+    e = T.Engine(0)  # a context of its own: no list size remembered from earlier runs
+    try:
+        codes = gen.c1s(300, seed=201)
+        s = e.upload(codes)
+        kw = dict(mode=T.RESOLVE, threshold=0.05, min_overlap=500, resolve_fraction=0.05)
+        exact = e.run(s, flags=T.FLAG_EXACT, **kw)
+        assert exact["exact"] and exact["n_host_exact"] > 64
+        _same(e.run(s, **kw), exact)
+        s.free()
+    finally:
+        e.close()
