@@ -161,7 +161,7 @@ void tn93g_result_free(tn93g_result *res);
  * receive the links.  The values are those tn93g_run gives (the same kernels; the mean's enclosure may differ in
  * its last bits, both contain tn93's value).
  * Fails with TN93G_E_ARG, writing no link, and tn93g_error names the reason, when: the run's buffers exceed the
- * scratch (a triangle of up to about 650 sequences of fewer than 2,048 columns fits); it has more links than
+ * scratch (a triangle of up to about 850 sequences of fewer than 2,048 columns fits); it has more links than
  * link_capacity or than the scratch's link regions hold; it is an exact-path run (TN93G_FLAG_EXACT, TN93G_FST); or an
  * output decision needs logarithms resolved on the host.  In each case tn93g_run computes it.  With
  * TN93G_BACKEND_AUTO the backend is the context's tuned choice for the shape, else popcount (no trial runs).

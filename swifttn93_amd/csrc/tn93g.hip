@@ -316,6 +316,7 @@ struct RunArgs {
   struct ChunkHdr *chdr;
   uint32_t *cdata;             // [slot][7][64]: nf0..nf3, diag, AG, CT (x16) per lane; with c16, [slot][4][64]:
                                // nf0 | nf1 << 16, nf2 | nf3 << 16, diag | AG << 16, CT
+  int64_t cstride;             // words from one slot to the next: 4 x 64 with c16, else 7 x 64
   int c16;                     // 32 * Ls < 2^16: every x16 count fits 16 bits
   int64_t gx;                  // column tiles
   int ti;                      // rows of the pair kernel's tiles (8, 16 or 32)
@@ -1501,7 +1502,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
         const size_t slot = (size_t)(i - A.r0) * (size_t)A.gx + (size_t)bx;
         if (lane == 0) A.chdr[slot] = ChunkHdr{need, skp};
         if (state == 1) {
-          uint32_t *d = A.cdata + slot * (7 * 64) + lane;
+          uint32_t *d = A.cdata + slot * A.cstride + lane;
           if (A.c16) {  // every value below 2^16
             d[0] = o.nf[0] | o.nf[1] << 16;
             d[64] = o.nf[2] | o.nf[3] << 16;
@@ -1604,7 +1605,7 @@ __device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A
       const int64_t k = r * A.gx + bx;
       h = A.chdr[k];
       if (A.c16) {
-        const uint32_t *d = A.cdata + (size_t)k * (7 * 64) + lane;
+        const uint32_t *d = A.cdata + (size_t)k * A.cstride + lane;
         p4[0] = d[0], p4[1] = d[64], p4[2] = d[128], p4[3] = d[192];
       }
     }
@@ -1617,7 +1618,7 @@ __device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A
         const int64_t k = r * A.gx + nb;
         hn = A.chdr[k];
         if (A.c16) {
-          const uint32_t *d = A.cdata + (size_t)k * (7 * 64) + lane;
+          const uint32_t *d = A.cdata + (size_t)k * A.cstride + lane;
           n4[0] = d[0], n4[1] = d[64], n4[2] = d[128], n4[3] = d[192];
         }
       }
@@ -1629,7 +1630,7 @@ __device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A
           q[0] = p4[0] & 0xffffu, q[1] = p4[0] >> 16, q[2] = p4[1] & 0xffffu, q[3] = p4[1] >> 16;
           q[4] = p4[2] & 0xffffu, q[5] = p4[2] >> 16, q[6] = p4[3];
         } else {
-          const uint32_t *d = A.cdata + (size_t)k * (7 * 64) + lane;
+          const uint32_t *d = A.cdata + (size_t)k * A.cstride + lane;
           for (int f = 0; f < 7; f++) q[f] = d[64 * f];
         }
         tn93g_tailin tin;
@@ -2879,6 +2880,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
   A.window = TN93G_LOG_WINDOW;
   A.min_overlap = P->min_overlap;
   A.c16 = 32 * A.Ls < 65536 ? 1 : 0;
+  A.cstride = (A.c16 ? 4 : 7) * 64;
   const int64_t npairs_box = nrows * A.ncols;
   J.npairs = A.triangle ? (d.n * (d.n - 1) - A.c0 * (A.c0 - 1)) / 2 : npairs_box;
   if (J.npairs >= (1ll << 32)) return c->fail(TN93G_E_ARG, "more than 2^32 pairs in one run");
@@ -2945,7 +2947,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
       need(B->counters, STRIPE_BYTES) ||
       need(B->rowout, nr1 * std::max<size_t>(15 * 8 + 16, sizeof(RowAcc))) ||
       need(B->chdr, (size_t)nslots * sizeof(ChunkHdr)) ||
-      need(B->cdata, (size_t)nslots * 7 * 64 * sizeof(uint32_t)))  // slots are 7 x 64 words apart in both layouts
+      need(B->cdata, (size_t)nslots * (A.c16 ? 4 : 7) * 64 * sizeof(uint32_t)))
     return arena ? TN93G_E_ARG : TN93G_E_NOMEM;
   // host-exact list: only the pairs with an uncertain logarithm are stored; it starts at 2^20 records (72 MB) or what
   // sufficed last time, and a run that overflows it is repeated with the list it needed
