@@ -136,6 +136,8 @@ typedef struct {
   double tail_kernel_ms;/* device time of the FP64 tail kernel                            */
   int32_t backend_used;  /* the pair kernel's backend (TN93G_BACKEND_POPC or _MFMA) and the rows of its tiles: the */
   int32_t tile_rows_used;/* caller's or the autotuner's choice                                                    */
+  int64_t folded_columns;/* constant columns (the same clean base in every sequence) that tn93g_set_upload left out
+                            of the device arrays, their sites added to every pair's counts (swifttn93.cpp:591-660) */
 } tn93g_result;
 
 /* Device memory (bytes) that tn93g_run of this set and these parameters works in, beyond the set itself: what a

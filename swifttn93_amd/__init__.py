@@ -39,7 +39,7 @@ class Result(ctypes.Structure):
                 ("skipped", ctypes.c_int64), ("n_replayed", ctypes.c_int64), ("n_uncertain", ctypes.c_int64),
                 ("n_host_exact", ctypes.c_int64), ("gpu_ms", ctypes.c_double), ("pair_kernel_ms", ctypes.c_double),
                 ("tail_kernel_ms", ctypes.c_double), ("backend_used", ctypes.c_int32),
-                ("tile_rows_used", ctypes.c_int32)]
+                ("tile_rows_used", ctypes.c_int32), ("folded_columns", ctypes.c_int64)]
 
 
 EXPORTS = ["tn93g_open", "tn93g_close", "tn93g_error", "tn93g_set_upload", "tn93g_set_free", "tn93g_run",
@@ -173,6 +173,7 @@ class Engine:
             "max_lo": list(r.max_lo), "max_hi": list(r.max_hi), "found_links": r.found_links, "skipped": r.skipped,
             "n_replayed": r.n_replayed, "n_uncertain": r.n_uncertain, "n_host_exact": r.n_host_exact,
             "gpu_ms": r.gpu_ms, "pair_kernel_ms": r.pair_kernel_ms, "tail_kernel_ms": r.tail_kernel_ms,
+            "folded_columns": r.folded_columns,
         }
 
     def run_batch(self, jobs):
@@ -281,5 +282,5 @@ def _to_py(rp, owner):
         "max_lo": list(r.max_lo), "max_hi": list(r.max_hi), "found_links": r.found_links, "skipped": r.skipped,
         "n_replayed": r.n_replayed, "n_uncertain": r.n_uncertain, "n_host_exact": r.n_host_exact,
         "gpu_ms": r.gpu_ms, "pair_kernel_ms": r.pair_kernel_ms, "tail_kernel_ms": r.tail_kernel_ms,
-        "backend_used": r.backend_used, "tile_rows_used": r.tile_rows_used,
+        "backend_used": r.backend_used, "tile_rows_used": r.tile_rows_used, "folded_columns": r.folded_columns,
     }

@@ -224,6 +224,10 @@ struct SetDev {
   int64_t *weights;
   int64_t total_events;
   int64_t total_gruns;  // gap runs of all sequences
+  // constant columns left out of every device array (the same clean base in every sequence): their number and
+  // their count per base, added to every pair's clean/clean diagonal
+  int64_t kt;
+  uint32_t kc[4];
 };
 
 struct ExactRec {
@@ -274,6 +278,8 @@ struct RunArgs {
   const uint2 *gruns;
   const int64_t *weights;
   int64_t n, Ls, npad, W, W64;
+  int64_t kt;                // constant columns folded out of the set (none of them is in W), kc[b] of base b:
+  uint32_t kc[4];            // each adds one A/A, C/C, G/G or T/T site to every pair
   // pair space: rows [r0, r1) x cols [c0, c1), triangle => only j > i
   int64_t r0, r1, c0, c1, ncols, n1;
   int triangle, fst, gapmm, skip_events, avg_fallback, exact, all_slow;
@@ -1411,6 +1417,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
         nfs += 16u * S[a].x;
       else
         nfs += 32u * cnt[a].T;
+      nfs += 32u * (uint32_t)A.kt;
       const double tng = (double)nfs * 0.03125;
       avg[a] = valid && !acc[a].fl && tng > (double)A.min_overlap && tng * A.rf <= (double)tot[a].amb;
       any_avg |= avg[a];
@@ -1419,7 +1426,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
       // additions); when tng is farther than that above the overlap and tng rf farther than that from amb, tn93's
       // -g test is decided: true => the replay starts in AVERAGE (margins doubled, the product's rounding included)
       if (valid && acc[a].fl && A.gcert) {
-        const double Lf = 32.0 * (double)A.W;
+        const double Lf = 32.0 * (double)A.W + (double)A.kt;
         const double E = (16.0 * Lf * Lf + 64.0 * Lf) * 0x1p-52;  // twice the bound on |tng_fp - tng|
         const double M = E * A.rf + 2.0 * tng * A.rf * 0x1p-52;
         avgr[a] = tng - E > (double)A.min_overlap && tng * A.rf + M < (double)tot[a].amb;
@@ -1465,11 +1472,11 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
       nf2 = c.B1 - c.BT;
       nf0 = 2 * c.T - c.B0 - c.B1 + c.BT;
     }
-    tot[a].nf[0] += 16 * nf0;
-    tot[a].nf[1] += 16 * nf1;
-    tot[a].nf[2] += 16 * nf2;
-    tot[a].nf[3] += 16 * nf3;
-    tot[a].dg += 16 * c.DG;
+    tot[a].nf[0] += 16 * nf0 + 32 * A.kc[0];
+    tot[a].nf[1] += 16 * nf1 + 32 * A.kc[1];
+    tot[a].nf[2] += 16 * nf2 + 32 * A.kc[2];
+    tot[a].nf[3] += 16 * nf3 + 32 * A.kc[3];
+    tot[a].dg += 16 * (c.DG + (uint32_t)A.kt);
     tot[a].ag += 16 * (c.TS - c.CT);
     tot[a].ct += 16 * c.CT;
   }
@@ -2118,6 +2125,8 @@ __global__ __launch_bounds__(RB, 4) void k_replay(RunArgs A)
       } else {
         cc_table(A, i, j, icc);
       }
+#pragma unroll
+      for (int b = 0; b < 4; b++) icc[5 * b] += A.kc[b];  // the folded constant columns
       ReplayState s;
       replay_reset(s);
       // the main mode first, unless the pair kernel's -g test (on exact, dyadic sums) already chose AVERAGE
@@ -2879,7 +2888,9 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
   A.mind = P->min_distance;
   A.window = TN93G_LOG_WINDOW;
   A.min_overlap = P->min_overlap;
-  A.c16 = 32 * A.Ls < 65536 ? 1 : 0;
+  A.kt = d.kt;
+  for (int b = 0; b < 4; b++) A.kc[b] = d.kc[b];
+  A.c16 = 32 * (A.Ls + A.kt) < 65536 ? 1 : 0;
   A.cstride = (A.c16 ? 4 : 7) * 64;
   const int64_t npairs_box = nrows * A.ncols;
   J.npairs = A.triangle ? (d.n * (d.n - 1) - A.c0 * (A.c0 - 1)) / 2 : npairs_box;
@@ -3296,6 +3307,7 @@ static void job_finish_common(const Job &J, const unsigned long long *hrep, tn93
   R->n_uncertain = (int64_t)J.n_unc;
   R->n_host_exact = J.n_exact;
   R->backend_used = J.P.backend == TN93G_BACKEND_AUTO ? TN93G_BACKEND_POPC : J.P.backend;
+  R->folded_columns = J.A.kt;
   R->tile_rows_used = 4 * J.rp;
   // self_v_self histogram entries (swifttn93.cpp:2406-2415)
   if (J.A.triangle)
@@ -3683,13 +3695,19 @@ int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
   return 0;
 }
 
-int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_t *weights)
+}  // extern "C"
+
+// Appends n sequences of srcL columns each; `cols` (the set's L entries, or null when srcL == L) names the columns
+// that are kept, in order.
+static int set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_t *weights, const int32_t *cols,
+                      int64_t srcL)
 {
   if (!S || n < 0 || (n > 0 && !codes)) return TN93G_E_ARG;
   tn93g_ctx *c = S->ctx;
   std::lock_guard<std::mutex> lk(S->mu);
   SetDev &d = S->d;
   const int64_t L = d.L, n0 = d.n;
+  if (!cols && d.kt) return c->fail(TN93G_E_ARG, "the set's constant columns were folded: it cannot grow");
   if (n0 + n > S->cap) return c->fail(TN93G_E_ARG, "set capacity exceeded");
   if (n == 0) return 0;
   // the codes checked, the B/D/H/V flags set and the gap-padded copy made in one pass, blocks of sequences on up to
@@ -3702,14 +3720,17 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
         1, std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16, bytes >> 22, n}));
     auto work = [&](int t) {
       for (int64_t q = n * t / nt; q < n * (t + 1) / nt && !bad.load(std::memory_order_relaxed); q++) {
-        const uint8_t *src = codes + q * L;
+        const uint8_t *src = codes + q * srcL;
         uint8_t *dst = &padded[(size_t)(q * d.Ls)];
         unsigned mx = 0, sp = 0;
+        if (cols)
+          for (int64_t k = 0; k < L; k++) dst[k] = src[cols[k]];
+        else
+          memcpy(dst, src, (size_t)L);
         for (int64_t k = 0; k < L; k++) {
-          const unsigned v = src[k];
+          const unsigned v = dst[k];
           mx |= v > (unsigned)GAPC ? 1u : 0u;
           sp |= (v - 11u) < 4u ? 1u : 0u;  // B, D, H, V: the only codes whose effects are not dyadic
-          dst[k] = (uint8_t)v;
         }
         memset(dst + L, GAPC, (size_t)(d.Ls - L));
         if (mx) bad = true;
@@ -3841,6 +3862,46 @@ int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_
   return 0;
 }
 
+// The constant columns of n sequences of L columns: cst[p] = 1 where every sequence holds the same clean base (the
+// columns swifttn93.cpp:591-660 leaves out of its counting).  Host threads take blocks of columns and leave a block
+// as soon as none of its columns can still be constant.
+static void constant_columns(const uint8_t *codes, int64_t n, int64_t L, std::vector<uint8_t> &cst)
+{
+  cst.assign((size_t)L, 0);
+  if (n <= 0) return;
+  for (int64_t p = 0; p < L; p++) cst[(size_t)p] = codes[p] < 4 ? 1 : 0;
+  const int64_t B = 4096, nb = (L + B - 1) / B;
+  const int nt = (int)std::max<int64_t>(
+      1, std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16, nb}));
+  std::atomic<int64_t> next{0};
+  auto work = [&] {
+    uint8_t *cs = cst.data();
+    for (int64_t b; (b = next.fetch_add(1)) < nb;) {
+      const int64_t p0 = b * B, p1 = std::min(L, p0 + B);
+      for (int64_t i = 1; i < n; i++) {
+        const uint8_t *r = codes + i * L;
+        unsigned any = 0;
+        for (int64_t p = p0; p < p1; p++) {
+          cs[p] &= (uint8_t)(r[p] == codes[p]);
+          any |= cs[p];
+        }
+        if (!any) break;
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; t++) pool.emplace_back(work);
+  work();
+  for (auto &th : pool) th.join();
+}
+
+extern "C" {
+
+int tn93g_set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64_t *weights)
+{
+  return set_append(S, codes, n, weights, nullptr, S ? S->d.L : 0);
+}
+
 int64_t tn93g_set_size(const tn93g_set *S)
 {
   if (!S) return 0;
@@ -3854,10 +3915,35 @@ int tn93g_set_upload(tn93g_ctx *c, const uint8_t *codes, int64_t n, int64_t L, c
   if (!c || !out || (n > 0 && !codes) || n < 0 || L <= 0 || n >= (1ll << 31) || L >= (1ll << 26))
     return TN93G_E_ARG;
   *out = nullptr;
+  // Constant columns add one clean/clean site of their base to every pair: when they are at least 1/16 of the
+  // columns they are left out of the device arrays and their counts added to each pair's totals.  TN93G_NOFOLD=1
+  // keeps every column (a diagnostic: the output is the same).
+  std::vector<int32_t> keep;
+  int64_t kt = 0;
+  uint32_t kc[4] = {0, 0, 0, 0};
+  const char *nf = getenv("TN93G_NOFOLD");
+  if (n > 1 && !(nf && *nf && strcmp(nf, "0"))) {
+    std::vector<uint8_t> cst;
+    constant_columns(codes, n, L, cst);
+    for (int64_t p = 0; p < L; p++)
+      if (cst[(size_t)p]) kt++;
+    if (kt < L && 16 * kt >= L) {
+      keep.reserve((size_t)(L - kt));
+      for (int64_t p = 0; p < L; p++)
+        if (cst[(size_t)p])
+          kc[codes[p]]++;
+        else
+          keep.push_back((int32_t)p);
+    } else {
+      kt = 0;
+    }
+  }
   tn93g_set *S = nullptr;
-  int rc = tn93g_set_create(c, L, n, &S);
+  int rc = tn93g_set_create(c, L - kt, n, &S);
   if (rc) return rc;
-  if ((rc = tn93g_set_append(S, codes, n, weights))) {
+  S->d.kt = kt;
+  for (int b = 0; b < 4; b++) S->d.kc[b] = kc[b];
+  if ((rc = set_append(S, codes, n, weights, kt ? keep.data() : nullptr, L))) {
     tn93g_set_free(S);
     return rc;
   }

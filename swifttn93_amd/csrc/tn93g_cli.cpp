@@ -430,9 +430,11 @@ int output(Case &C)
   const bool two = C.two, do_fst = C.do_fst, report_self = C.report_self;
   const int threads = tn93g_io::io_threads();
   if (getenv("TN93G_STATS"))
-    fprintf(stderr, "tn93g: gpu %.3f ms (pair kernel %.3f ms) links %lld replayed %lld uncertain %lld host-exact %lld\n",
+    fprintf(stderr,
+            "tn93g: gpu %.3f ms (pair kernel %.3f ms) links %lld replayed %lld uncertain %lld host-exact %lld "
+            "folded columns %lld\n",
             R->gpu_ms, R->pair_kernel_ms, (long long)R->n_links, (long long)R->n_replayed,
-            (long long)R->n_uncertain, (long long)R->n_host_exact);
+            (long long)R->n_uncertain, (long long)R->n_host_exact, (long long)R->folded_columns);
   // a full buffer for the distance file; stdout gets its own once, before anything is written to it
   static char obuf[1 << 16], sbuf[1 << 16];
   static bool stdout_buffered = false;

@@ -45,3 +45,21 @@ def hiv8766(tmp_path_factory):
                     for f in ["real_hiv_pol_dev.fa.xz", "real_hiv_pol_holdout.fa.xz"])
     p.write_bytes(data)
     return p
+
+
+@pytest.fixture(scope="session")
+def sars2(tmp_path_factory):
+    """The reference's real SARS-CoV-2 sets (500 and 459 genomes x 29,903 columns) as FASTA files, and their pins:
+    (dict name -> path, pins)."""
+    pins = json.loads((GOLDEN / "real" / "sars2_pins.json").read_text())
+    d = tmp_path_factory.mktemp("sars2")
+    paths = {}
+    for key, f in pins["files"].items():
+        paths[key] = d / f"{key}.fa"
+        paths[key].write_bytes(lzma.decompress((GOLDEN / "real" / f).read_bytes()))
+    return paths, pins["pins"]
+
+
+def sars2_argv(flags, paths):
+    """A pin's flags with {second} replaced by the second file's path."""
+    return [str(paths[f.strip("{}")]) if f.startswith("{") else f for f in flags]

@@ -12,12 +12,17 @@ Outputs (all data, no code from the reference):
   pair_vectors.json.xz per-pair inputs (regenerated from a seed by pair_inputs()) and the reference computeTN93
                        outputs (distance bits, histogram index), with and without gap descriptors.
   real/                the reference's real HIV-1 pol sets (data/real_hiv_pol_{dev,holdout}.fa.xz) and the ten
-                       published tn93 sha256 pins of application/results/network.jsonl.
+                       published tn93 sha256 pins of application/results/network.jsonl; its real SARS-CoV-2 sets
+                       (data/real_sars2_{dev,holdout}.fa.xz, 500 and 459 genomes x 29,903 columns, two thirds of
+                       them constant) and sars2_pins.json: the reference's sha256 of file and stdout for SARS2_FLAGS.
+
+``python3 tests/golden/make_fixtures.py sars2`` writes only the SARS-CoV-2 part.
 
 The fuzz distribution mirrors the reference harness's generator (gen_data.py:156-250) in our own code: sizes,
 alphabets (IUPAC, lowercase, U, ?, weird bytes), CRLF, wrapping, empty/duplicate/odd/copy-number names, unequal
 lengths, leading gaps, near-duplicates, random flags and a second file.
 """
+import hashlib
 import json
 import lzma
 import os
@@ -252,9 +257,46 @@ def make_pairs(n, seed):
     return {"seed": seed, "n": n, "results": res}
 
 
+# option sets of the SARS-CoV-2 pins: the reference's benchmark flags (-t 0.0002 -l 20000 -0), every pair written,
+# each -a mode, an overlap bound that skips pairs, and the two sets against each other (-s, and -s -m)
+SARS2_FLAGS = [
+    ["-t", "0.0002", "-l", "20000", "-0"],
+    ["-t", "1", "-a", "resolve", "-g", "0.05", "-l", "20000", "-f", "csvn"],
+    ["-t", "0.001", "-a", "average", "-l", "20000"],
+    ["-t", "0.001", "-a", "skip", "-l", "5000"],
+    ["-t", "0.001", "-a", "gapmm", "-l", "20000"],
+    ["-t", "0.0005", "-a", "resolve", "-g", "0.05", "-l", "29800"],
+    ["-t", "0.0005", "-a", "resolve", "-g", "0.05", "-l", "20000", "-s", "{second}"],
+    ["-t", "0.0005", "-a", "average", "-l", "20000", "-s", "{second}", "-m"],
+]
+
+
+def make_sars2():
+    real = HERE / "real"
+    real.mkdir(exist_ok=True)
+    names = ["real_sars2_dev.fa.xz", "real_sars2_holdout.fa.xz"]
+    for f in names:
+        shutil.copyfile(REFTREE / "data" / f, real / f)
+    files = {"in": lzma.decompress((real / names[0]).read_bytes()),
+             "second": lzma.decompress((real / names[1]).read_bytes())}
+    pins = []
+    with tempfile.TemporaryDirectory() as t:
+        for flags in SARS2_FLAGS:
+            rc, out, so = run_case(REF_BIN, files, flags, Path(t))
+            assert rc == 0 and out is not None, (flags, rc)
+            pins.append({"flags": flags, "ref_file_sha256": hashlib.sha256(out).hexdigest(),
+                         "ref_stdout_sha256": hashlib.sha256(so).hexdigest(), "file_bytes": len(out)})
+            print("sars2", " ".join(flags), len(out))
+    (real / "sars2_pins.json").write_text(json.dumps({"files": {"in": names[0], "second": names[1]},
+                                                      "pins": pins}, indent=1))
+
+
 def main():
     if not REF_BIN.exists() or not HARNESS.exists():
         sys.exit("build the reference first: make -C oracle ref")
+    if sys.argv[1:] == ["sars2"]:
+        make_sars2()
+        return
     seed = 20261203
     cases = make_fuzz(int(os.environ.get("N_FUZZ", "600")), seed)
     (HERE / "fuzz_cases.json.xz").write_bytes(lzma.compress(json.dumps({"seed": seed, "cases": cases}).encode()))
@@ -279,6 +321,7 @@ def main():
         "input_sha256": "70985c02b3ba52585c9ed30db147ca14984a9f5fd03babb3af235432062a0524",
         "flags": ["-q", "-0", "-a", "resolve", "-l", "500", "-g", "0.05", "-f", "csv"],
         "pins": pins}, indent=1))
+    make_sars2()
 
 
 if __name__ == "__main__":

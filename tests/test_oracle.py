@@ -147,3 +147,18 @@ def test_network_pins(oracle_bins, hiv8766, tmp_path, t_arg):
     assert out.stat().st_size == pin["file_bytes"]
     assert hashlib.sha256(out.read_bytes()).hexdigest() == pin["ref_file_sha256"]
     assert hashlib.sha256(p.stdout).hexdigest() == pin["ref_stdout_sha256"]
+
+
+@pytest.mark.parametrize("k", range(8))
+def test_sars2_pins(oracle_bins, sars2, tmp_path, k):
+    """The reference build's sha256 of file and JSON on the real SARS-CoV-2 set (two thirds of its columns are
+    constant), one option set each."""
+    from conftest import sars2_argv
+    paths, pins = sars2
+    pin = pins[k]
+    out = tmp_path / "out.csv"
+    argv = [str(oracle_bins[0]), "-q"] + sars2_argv(pin["flags"], paths) + ["-o", str(out), str(paths["in"])]
+    p = subprocess.run(argv, capture_output=True, check=True)
+    assert out.stat().st_size == pin["file_bytes"]
+    assert hashlib.sha256(out.read_bytes()).hexdigest() == pin["ref_file_sha256"]
+    assert hashlib.sha256(p.stdout).hexdigest() == pin["ref_stdout_sha256"]
