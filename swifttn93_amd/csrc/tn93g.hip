@@ -2086,6 +2086,71 @@ __device__ void rowacc_add_wave(const RunArgs &A, int64_t row, const RowPart &rp
 // lane of a wave runs the same iterations so the row parts can be combined across the wave.  The replay is latency
 // bound: a register budget for 4 waves per SIMD measured fastest on the MI355X (also with dense N: C2 470 -> 460 ms
 // against 3 waves), although the compiler then spills some registers to scratch.
+// The wave's 64 replay-list entries kb + lane (of `total`, from entry `beg` of slow_list) of run A, with A's replay
+// tables in `tabs`; histogram counts and certification counters go to the block's lhist / bs.  Every lane of the
+// wave calls it (the long-alignment count tables are built by the whole wave).
+template <bool DENSE>
+__device__ __attribute__((always_inline)) inline void replay_entries(const RunArgs &A, const ReplayEnt *tabs,
+                                                                      unsigned long long *lhist, BlockStats *bs,
+                                                                      uint32_t (*scc)[16], int64_t beg, int64_t total,
+                                                                      int64_t kb)
+{
+  const int lane = threadIdx.x & 63;
+  const int64_t k = kb + lane;
+  int64_t row = -1;
+  RowPart rp = rowpart_zero();
+  const bool coop = DENSE && A.W >= 128;  // uniform
+  unsigned long long pr = 0;
+  if (k < total) pr = A.slow_list[beg + k];
+  uint32_t icc[16];
+  const bool looked = k < total && cc_table_lookup(A, (int64_t)(pr >> 32 & 0x7fffffffu),
+                                                   (int64_t)(pr & 0xffffffffull), icc);
+  if (coop)
+    cc_table_wave(A, (int64_t)(pr >> 32 & 0x7fffffffu), (int64_t)(pr & 0xffffffffull), k < total && !looked, scc);
+  const bool avg_first = pr >> 63;
+  pr &= ~(1ull << 63);
+  if (k < total) {
+    const int64_t i = (int64_t)(pr >> 32), j = (int64_t)(pr & 0xffffffffull);
+    if (looked) {
+    } else if (coop) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) icc[q] = scc[lane][q];
+    } else {
+      cc_table(A, i, j, icc);
+    }
+#pragma unroll
+    for (int b = 0; b < 4; b++) icc[5 * b] += A.kc[b];  // the folded constant columns
+    ReplayState s;
+    replay_reset(s);
+    // the main mode first, unless the pair kernel's -g test (on exact, dyadic sums) already chose AVERAGE
+    const ReplayEnt *t0 = avg_first ? tabs + NCODE * NCODE : tabs;
+    if constexpr (DENSE)
+      replay_events_dense(A, i, j, t0, s);
+    else
+      replay_events(A, i, j, t0, s);  // GAPMM: the lists hold the gap sites too
+    tn93g_tailin tin;
+    double tng;
+    replay_merge(s, icc, tin, tng);
+    bool skip = tng <= (double)A.min_overlap;
+    if (!avg_first && !skip && A.avg_fallback &&
+        tng * A.rf <= (double)s.amb) {  // recompute in AVERAGE (refkernel.cpp:742-750)
+      replay_reset(s);
+      if constexpr (DENSE)
+        replay_events_dense(A, i, j, tabs + NCODE * NCODE, s);
+      else
+        replay_events(A, i, j, tabs + NCODE * NCODE, s);
+      replay_merge(s, icc, tin, tng);
+      skip = tng <= (double)A.min_overlap;
+    }
+    if (skip)
+      skip_pair(A, i, j, rp);
+    else
+      A.fast ? finish_pair<true>(A, i, j, tin, lhist, bs, rp) : finish_pair<false>(A, i, j, tin, lhist, bs, rp);
+    row = i;
+  }
+  if (A.fast) rowacc_add_wave(A, row, rp);
+}
+
 template <bool DENSE>
 __global__ __launch_bounds__(RB, 4) void k_replay(RunArgs A)
 {
@@ -2093,7 +2158,7 @@ __global__ __launch_bounds__(RB, 4) void k_replay(RunArgs A)
   __shared__ unsigned long long lhist[3 * 200];
   __shared__ BlockStats bs;
   __shared__ uint32_t scc[DENSE ? RB : 1][16];  // clean/clean tables built by the wave (long alignments)
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, wave = tid >> 6;
   const int64_t total = (int64_t)*A.slow_total;
   if ((int64_t)blockIdx.x * RB >= total) return;  // no pair for this block (uniform): a run without replays is cheap
   for (int k = tid; k < 2 * NCODE * NCODE; k += RB) tabs[k] = A.rtab[k];
@@ -2101,63 +2166,41 @@ __global__ __launch_bounds__(RB, 4) void k_replay(RunArgs A)
   if (tid == 0) bs.unc = bs.undecided = 0;
   __syncthreads();
   const int64_t nw = (int64_t)gridDim.x * (RB / 64);
-  for (int64_t kb = ((int64_t)blockIdx.x * (RB / 64) + wave) * 64; kb < total; kb += nw * 64) {
-    const int64_t k = kb + lane;
-    int64_t row = -1;
-    RowPart rp = rowpart_zero();
-    const bool coop = DENSE && A.W >= 128;  // uniform
-    unsigned long long pr = 0;
-    if (k < total) pr = A.slow_list[k];
-    uint32_t icc[16];
-    const bool looked = k < total && cc_table_lookup(A, (int64_t)(pr >> 32 & 0x7fffffffu),
-                                                     (int64_t)(pr & 0xffffffffull), icc);
-    if (coop)
-      cc_table_wave(A, (int64_t)(pr >> 32 & 0x7fffffffu), (int64_t)(pr & 0xffffffffull), k < total && !looked,
-                    scc + wave * 64);
-    const bool avg_first = pr >> 63;
-    pr &= ~(1ull << 63);
-    if (k < total) {
-      const int64_t i = (int64_t)(pr >> 32), j = (int64_t)(pr & 0xffffffffull);
-      if (looked) {
-      } else if (coop) {
-#pragma unroll
-        for (int q = 0; q < 16; q++) icc[q] = scc[wave * 64 + lane][q];
-      } else {
-        cc_table(A, i, j, icc);
-      }
-#pragma unroll
-      for (int b = 0; b < 4; b++) icc[5 * b] += A.kc[b];  // the folded constant columns
-      ReplayState s;
-      replay_reset(s);
-      // the main mode first, unless the pair kernel's -g test (on exact, dyadic sums) already chose AVERAGE
-      const ReplayEnt *t0 = avg_first ? tabs + NCODE * NCODE : tabs;
-      if constexpr (DENSE)
-        replay_events_dense(A, i, j, t0, s);
-      else
-        replay_events(A, i, j, t0, s);  // GAPMM: the lists hold the gap sites too
-      tn93g_tailin tin;
-      double tng;
-      replay_merge(s, icc, tin, tng);
-      bool skip = tng <= (double)A.min_overlap;
-      if (!avg_first && !skip && A.avg_fallback &&
-          tng * A.rf <= (double)s.amb) {  // recompute in AVERAGE (refkernel.cpp:742-750)
-        replay_reset(s);
-        if constexpr (DENSE)
-          replay_events_dense(A, i, j, tabs + NCODE * NCODE, s);
-        else
-          replay_events(A, i, j, tabs + NCODE * NCODE, s);
-        replay_merge(s, icc, tin, tng);
-        skip = tng <= (double)A.min_overlap;
-      }
-      if (skip)
-        skip_pair(A, i, j, rp);
-      else
-        A.fast ? finish_pair<true>(A, i, j, tin, lhist, &bs, rp) : finish_pair<false>(A, i, j, tin, lhist, &bs, rp);
-      row = i;
-    }
-    if (A.fast) rowacc_add_wave(A, row, rp);
-  }
+  for (int64_t kb = ((int64_t)blockIdx.x * (RB / 64) + wave) * 64; kb < total; kb += nw * 64)
+    replay_entries<DENSE>(A, tabs, lhist, &bs, scc + wave * 64, 0, total, kb);
   flush_block(A, &bs, lhist, tid, RB);
+}
+
+// The replays of several jobs (a batch) in one launch.  The jobs' lists are numbered one after another in units of
+// RB entries; block b takes the units u = b (mod gridDim.x), a job's units together: for each job with units for it,
+// the block loads that job's tables, replays its units and flushes its counters into that job's stripes.
+template <bool DENSE>
+__global__ __launch_bounds__(RB, 4) void k_replay_multi(const RunArgs *__restrict__ jobs,
+                                                        const int32_t *__restrict__ ids, int nj)
+{
+  __shared__ ReplayEnt tabs[2 * NCODE * NCODE];
+  __shared__ unsigned long long lhist[3 * 200];
+  __shared__ BlockStats bs;
+  __shared__ uint32_t scc[DENSE ? RB : 1][16];
+  const int tid = threadIdx.x, wave = tid >> 6;
+  const int64_t G = gridDim.x, b = blockIdx.x;
+  int64_t cum = 0;  // units of the jobs before q (uniform)
+  for (int q = 0; q < nj; q++) {
+    const RunArgs &A = jobs[ids[q]];
+    const int64_t total = (int64_t)*A.slow_total;
+    const int64_t units = (total + RB - 1) / RB;
+    const int64_t u0 = cum + (((b - cum) % G) + G) % G;  // the block's first unit at or after cum
+    cum += units;
+    if (u0 >= cum) continue;
+    __syncthreads();  // the previous job's tables and counters are no longer read
+    for (int k = tid; k < 2 * NCODE * NCODE; k += RB) tabs[k] = A.rtab[k];
+    for (int k = tid; k < 600; k += RB) lhist[k] = 0;
+    if (tid == 0) bs.unc = bs.undecided = 0;
+    __syncthreads();
+    for (int64_t u = u0; u < cum; u += G)
+      replay_entries<DENSE>(A, tabs, lhist, &bs, scc + wave * 64, 0, total, (u - (cum - units)) * RB + wave * 64);
+    flush_block(A, &bs, lhist, tid, RB);
+  }
 }
 
 // the links of the NSTRIPE regions, concatenated into `out` (one block per region)
@@ -4313,7 +4356,8 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
     std::vector<CopyRange> copies;
     std::vector<size_t> spec(m, 0), soff(m, 0);
     std::vector<size_t> post;  // the merged jobs, whose tails and link kernels also run as one launch each
-    size_t off_fills = 0, off_copies = 0, stage_bytes = 0, off_pargs = 0, off_tb = 0, off_ob = 0;
+    size_t off_fills = 0, off_copies = 0, stage_bytes = 0, off_pargs = 0, off_tb = 0, off_ob = 0, off_rid = 0;
+    std::vector<int32_t> rids[2];  // the merged jobs (indices into their RunArgs) whose replay is plain / dense
     int64_t tail_total = 0, order_total = 0;
     {
       std::map<int, std::vector<size_t>> byv;
@@ -4346,6 +4390,9 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       bytes = (bytes + (post.size() + 1) * 8 + 255) & ~(size_t)255;
       off_ob = bytes;
       bytes = (bytes + (post.size() + 1) * 8 + 255) & ~(size_t)255;
+      for (size_t q = 0; q < post.size(); q++) rids[jobs[post[q]].A.nplane ? 1 : 0].push_back((int32_t)q);
+      off_rid = bytes;
+      bytes = (bytes + (post.size() + 1) * 4 + 255) & ~(size_t)255;
       // every job's zero-started buffers, cleared by one launch before the branches fork
       for (size_t t = 0; t < m; t++) job_fill_ranges(jobs[t], fills);
       off_fills = bytes;
@@ -4403,6 +4450,9 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
         }
         tb[post.size()] = tail_total;
         ob[post.size()] = order_total;
+        int32_t *ri = (int32_t *)(host.data() + off_rid);
+        for (int d = 0; d < 2; d++)
+          for (int32_t q : rids[d]) *ri++ = q;
       }
       if (bytes) HIPCHK(c, hipMemcpyAsync(c->multi.p, host.data(), bytes, hipMemcpyHostToDevice, c->stream));
     }
@@ -4473,13 +4523,18 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
                                      (const RunArgs *)((char *)c->multi.p + G.off_args),
                                      (const int64_t *)((char *)c->multi.p + G.off_toff), (int)G.jobs.size());
         if (!rc) hipLaunchKernelGGL(k_stamp, dim3(1), dim3(64), 0, c->stream, stamps + 1);
-        // 3: the merged jobs' replays on their branches beside one tail launch over all of them on the main stream,
-        // then their link offsets, gather, ordering and histogram sums as one launch each
+        // 3: the merged jobs' replays as one launch per replay form (plain, dense N) on two branches, beside one
+        // tail launch over all of them on the main stream; then their link offsets, gather, ordering and histogram
+        // sums as one launch each
         if (!rc) rc = fork();
-        for (size_t t : order)
-          if (!rc && merged[t]) job_enqueue_replay(jobs[t], c->side[branch[t]]);
         const RunArgs *pa = (const RunArgs *)((char *)c->multi.p + off_pargs);
         const int np = (int)post.size();
+        const int32_t *rid = (const int32_t *)((char *)c->multi.p + off_rid);
+        if (!rc && !rids[0].empty())
+          hipLaunchKernelGGL(k_replay_multi<false>, dim3(2048), dim3(RB), 0, c->side[0], pa, rid, (int)rids[0].size());
+        if (!rc && !rids[1].empty())
+          hipLaunchKernelGGL(k_replay_multi<true>, dim3(3072), dim3(RB), 0, c->side[1], pa, rid + rids[0].size(),
+                             (int)rids[1].size());
         if (!rc && tail_total)
           hipLaunchKernelGGL(k_tail_multi, dim3((unsigned)tail_total), dim3(256), 0, c->stream, pa,
                              (const int64_t *)((char *)c->multi.p + off_tb), np);
