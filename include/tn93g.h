@@ -140,13 +140,17 @@ typedef struct {
                             of the device arrays, their sites added to every pair's counts (swifttn93.cpp:591-660) */
 } tn93g_result;
 
-/* Device memory (bytes) that tn93g_run of this set and these parameters works in, beyond the set itself: what a
- * caller admitting runs under an HBM budget reserves.  No counterpart in the reference. */
+/* Device memory (bytes) that tn93g_run of this set and these parameters works in, beyond the set itself (for a run
+ * computed in parts, its largest part's): what a caller admitting runs under an HBM budget reserves.  No counterpart
+ * in the reference. */
 int tn93g_run_bytes(tn93g_ctx *ctx, const tn93g_set *set, const tn93g_params *params, int64_t *bytes);
 
 /* Computes all pairs of `set` described by `params` (set must hold n1 + n2 sequences for CROSS/FST).
  * Replaces the OpenMP tile loop of swifttn93's main (swifttn93.cpp:2339-2864).  *out is allocated by the library
- * and released with tn93g_result_free. */
+ * and released with tn93g_result_free.  A run of 2^31 pairs or more (a triangle of 65,536 sequences and up), or of
+ * more than 2^25 (row, 64-column tile) slots, is computed in parts of consecutive rows (at most 2^30 pairs and 2^24
+ * slots each, about 25 GB of device memory) and returned as one result, the same as one run's; TN93G_FST runs are
+ * not split and hold fewer than 2^32 pairs.  Like the reference, there is no other limit on the number of pairs. */
 int tn93g_run(tn93g_ctx *ctx, const tn93g_set *set, const tn93g_params *params, tn93g_result **out);
 void tn93g_result_free(tn93g_result *res);
 

@@ -2802,7 +2802,19 @@ struct Job {
   unsigned link_cap = 1, exact_cap = 1;
   uint64_t n_slow = 0, n_unc = 0, n_undecided = 0, n_linkrec = 0;
   unsigned max_region = 0, n_exact = 0;
+  int64_t ra = -1, rb = -1;  // a part of a split run: its rows [ra, rb) (the whole run's rows when ra < 0)
 };
+
+// Pairs (i, j) with r0 <= i < r1, c0 <= j < c1 and j > i.
+static int64_t tri_pairs(int64_t r0, int64_t r1, int64_t c0, int64_t c1)
+{
+  int64_t p = 0;
+  const int64_t a = std::min(r1, c0);  // rows below c0 pair with every column
+  if (a > r0) p += (a - r0) * (c1 - c0);
+  const int64_t b0 = std::max(r0, c0), b1 = std::min(r1, c1);  // row i in [c0, c1) pairs with (i, c1)
+  if (b1 > b0) p += (b1 - b0) * (c1 - 1) - (b1 * (b1 - 1) / 2 - b0 * (b0 - 1) / 2);
+  return p;
+}
 
 // Checks the parameters, uploads the job's event tables and sizes its buffers: everything before its kernels.
 // The entry of the marginal form of the pair kernel: a site where exactly one code is clean and the other is an
@@ -2916,6 +2928,12 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
     A.c1 = d.n;
     A.triangle = 1;
   }
+  if (J.ra >= 0) {  // one part of a split run: its rows (a triangle's part needs only the columns from its first row)
+    if (J.ra < A.r0 || J.rb > A.r1 || J.ra >= J.rb) return c->fail(TN93G_E_ARG, "bad row range");
+    A.r0 = J.ra;
+    A.r1 = J.rb;
+    if (P->kind == TN93G_TRIANGLE) A.c0 = J.ra;
+  }
   A.fst = P->kind == TN93G_FST;
   A.n1 = (P->kind == TN93G_TRIANGLE || P->kind == TN93G_APPEND) ? d.n : P->n1;
   A.ncols = A.c1 - A.c0;
@@ -2936,7 +2954,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
   A.c16 = 32 * (A.Ls + A.kt) < 65536 ? 1 : 0;
   A.cstride = (A.c16 ? 4 : 7) * 64;
   const int64_t npairs_box = nrows * A.ncols;
-  J.npairs = A.triangle ? (d.n * (d.n - 1) - A.c0 * (A.c0 - 1)) / 2 : npairs_box;
+  J.npairs = A.triangle ? tri_pairs(A.r0, A.r1, A.c0, A.c1) : npairs_box;
   if (J.npairs >= (1ll << 32)) return c->fail(TN93G_E_ARG, "more than 2^32 pairs in one run");
   // tables: [4][18*18] packed event entries (main mode, AVERAGE; then both folded), [2][18*18] replay entries
   if (!dry_bytes && (B->tab_mode != P->mode || B->tab_mask != P->subset_mask || !B->evtab.p || !B->rtab.p)) {
@@ -3352,8 +3370,8 @@ static void job_finish_common(const Job &J, const unsigned long long *hrep, tn93
   R->backend_used = J.P.backend == TN93G_BACKEND_AUTO ? TN93G_BACKEND_POPC : J.P.backend;
   R->folded_columns = J.A.kt;
   R->tile_rows_used = 4 * J.rp;
-  // self_v_self histogram entries (swifttn93.cpp:2406-2415)
-  if (J.A.triangle)
+  // self_v_self histogram entries (swifttn93.cpp:2406-2415); a split run adds them once, for all its parts
+  if (J.A.triangle && J.ra < 0)
     for (int64_t i = J.A.c0; i < J.A.n; i++) {
       const int64_t wi = J.S->hw[(size_t)i];
       if (wi > 1) R->hist[(J.A.fst && i >= J.A.n1) ? 1 : 0][0] += (unsigned long long)((wi * (wi - 1)) >> 1);
@@ -3424,15 +3442,11 @@ static void free_links(int64_t *link_row)
 // bounds lo_r, hi_r up in double is again within gamma_{R+1} sum max(|lo_r|, |hi_r|) (their own rounding
 // included), so eR below covers both tn93's row-order sum and this one; mabs itself is inflated by K.
 struct RowMerge {
-  const Job &J;
   double mlo = 0, mhi = 0, mabs = 0, maxlo = 0., maxhi = 0.;
   long long wsum = 0, skip = 0, linkw = 0;
   static constexpr double cu = 0x1.00001p-53, K = 1.0 + 0x1p-20;
-  explicit RowMerge(const Job &j) : J(j) {}
-  void add(const RowAcc &q, int64_t r)  // row r0 + r of the run; rows must come in order
+  void add(const RowAcc &q, double n_r)  // one row, summed over n_r pairs; rows must come in order
   {
-    const int64_t i = J.A.r0 + r;
-    const double n_r = (double)(J.A.triangle ? (J.A.c1 - std::max(J.A.c0, i + 1)) : J.A.ncols);
     const double g = std::max(n_r, 1.0) * cu;
     const double e = (2.0 * g) * q.sabs * K;
     const double lo = q.slo - e, hi = q.shi + e;
@@ -3446,9 +3460,10 @@ struct RowMerge {
     skip += q.skip;
     linkw += q.linkw;
   }
-  void finish(tn93g_result *R) const
+  // nrows: the rows added; self_w: the weights of the self pairs (copy numbers above 1)
+  void finish(tn93g_result *R, int64_t nrows, long long self_w) const
   {
-    const double gR = ((double)std::max<int64_t>(J.nrows, 1) + 1.0) * cu;
+    const double gR = ((double)std::max<int64_t>(nrows, 1) + 1.0) * cu;
     const double eR = (2.0 * gR) * (mabs * K) * K + 0x1p-50 * mabs;
     R->mean_lo[0] = std::nextafter(mlo - eR, -INFINITY);
     R->mean_hi[0] = std::nextafter(mhi + eR, INFINITY);
@@ -3456,19 +3471,40 @@ struct RowMerge {
     R->max_hi[0] = maxhi;
     R->skipped = skip;
     R->found_links = linkw;
-    long long w_all = wsum;
-    if (J.A.triangle) {
-      for (int64_t i = J.A.c0; i < J.A.n; i++) {
-        const int64_t wi = J.S->hw[(size_t)i];
-        if (wi > 1) w_all += (wi * (wi - 1)) >> 1;
-      }
-    }
-    R->weighted[0] = (double)w_all;
+    R->weighted[0] = (double)(wsum + self_w);
   }
 };
 
+// Pairs of row r of a run.
+static double row_pairs(const RunArgs &A, int64_t r)
+{
+  const int64_t i = A.r0 + r;
+  return (double)(A.triangle ? std::max<int64_t>(A.c1 - std::max(A.c0, i + 1), 0) : A.ncols);
+}
+
+// The weights of the self pairs of sequences [c0, n) (a copy number w adds w (w - 1) / 2 pairs at distance 0).
+static long long self_weights(const tn93g_set *S, int64_t c0, int64_t n)
+{
+  long long w = 0;
+  for (int64_t i = c0; i < n; i++) {
+    const int64_t wi = S->hw[(size_t)i];
+    if (wi > 1) w += (wi * (wi - 1)) >> 1;
+  }
+  return w;
+}
+
+// The rows of the parts of a split run, gathered in row order: the enclosure merge, and tn93's own row-order sums
+// while every part is exact.
+struct RowSink {
+  RowMerge M;
+  int64_t rows = 0;
+  bool exact = true;
+  double mean[3] = {0., 0., 0.}, weighted[3] = {0., 0., 0.}, maxv[3] = {0., 0., 0.};
+};
+
 // The fast path's result from its read-back: links in tn93's order, the enclosures of the mean and the maximum.
-static int job_fast_result(const Job &J, const char *pin, const LinkRec *lk, tn93g_result *R)
+static int job_fast_result(const Job &J, const char *pin, const LinkRec *lk, tn93g_result *R,
+                           RowSink *sink = nullptr)
 {
   const FastLayout F = fast_layout(J, 0);
   const RowAcc *ra = (const RowAcc *)(pin + F.off_ra);
@@ -3483,9 +3519,15 @@ static int job_fast_result(const Job &J, const char *pin, const LinkRec *lk, tn9
     R->link_dist_lo[k] = q.lo;
     R->link_dist_hi[k] = q.hi;
   }
-  RowMerge M(J);
-  for (int64_t r = 0; r < J.nrows; r++) M.add(ra[(size_t)r], r);
-  M.finish(R);
+  if (sink) {  // a part of a split run: the rows go on to the whole run's merge
+    for (int64_t r = 0; r < J.nrows; r++) sink->M.add(ra[(size_t)r], row_pairs(J.A, r));
+    sink->rows += J.nrows;
+    sink->exact = false;
+  } else {
+    RowMerge M;
+    for (int64_t r = 0; r < J.nrows; r++) M.add(ra[(size_t)r], row_pairs(J.A, r));
+    M.finish(R, J.nrows, J.A.triangle ? self_weights(J.S, J.A.c0, J.A.n) : 0);
+  }
   unsigned long long hrep[600];
   memcpy(hrep, pin + F.off_h, sizeof hrep);
   job_finish_common(J, hrep, R);
@@ -3493,7 +3535,7 @@ static int job_fast_result(const Job &J, const char *pin, const LinkRec *lk, tn9
 }
 
 // The exact path after the run's kernels: host-exact tails patched in, rows summed in column order by k_rows.
-static int job_exact_result(tn93g_ctx *c, Job &J, tn93g_result *R)
+static int job_exact_result(tn93g_ctx *c, Job &J, tn93g_result *R, RowSink *sink = nullptr)
 {
   RunArgs &A = J.A;
   RunBufs *B = J.B;
@@ -3573,7 +3615,30 @@ static int job_exact_result(tn93g_ctx *c, Job &J, tn93g_result *R)
   HIPCHK(c, hipStreamSynchronize(c->stream));
   // row-order merge (swifttn93.cpp:2832-2848)
   for (int b = 0; b < 3; b++) R->mean_lo[b] = R->mean_hi[b] = R->weighted[b] = R->max_lo[b] = R->max_hi[b] = 0.;
-  for (int64_t r = 0; r < nrows; r++) {
+  if (sink) {  // a part of a split run (never F_ST: one histogram): rows into the whole run's merges
+    for (int64_t r = 0; r < nrows; r++) {
+      const size_t q = (size_t)r;
+      RowAcc a{};
+      a.slo = rows[q];
+      a.shi = rows[3 * nr1 + q];
+      a.sabs = 0.;  // tn93's own row sum (column order) when exact, else an enclosure of it
+      double mlo = rows[9 * nr1 + q], mhi = rows[12 * nr1 + q];
+      memcpy(&a.mlo, &mlo, 8);
+      memcpy(&a.mhi, &mhi, 8);
+      // k_rows counts the row's own self pairs into its weights; the merge adds them once for the whole run
+      const int64_t i = A.r0 + r, wi = J.S->hw[(size_t)i];
+      const double self = A.triangle && i >= A.c0 && wi > 1 ? (double)((wi * (wi - 1)) >> 1) : 0.;
+      a.wsum = (long long)(rows[6 * nr1 + q] - self);
+      memcpy(&a.skip, &rows[15 * nr1 + q], 8);
+      memcpy(&a.linkw, &rows[16 * nr1 + q], 8);
+      sink->M.add(a, row_pairs(A, r));
+      sink->mean[0] += a.slo;
+      sink->weighted[0] += rows[6 * nr1 + q] - self;
+      if (mlo > sink->maxv[0]) sink->maxv[0] = mlo;
+    }
+    sink->rows += nrows;
+  }
+  for (int64_t r = 0; r < nrows && !sink; r++) {
     for (int b = 0; b < 3; b++) {
       const double mlo = rows[9 * nr1 + b * nr1 + (size_t)r], mhi = rows[12 * nr1 + b * nr1 + (size_t)r];
       if (mlo > R->max_lo[b]) R->max_lo[b] = mlo;
@@ -4049,6 +4114,55 @@ static bool same_printed(double lo, double hi, int prec)
   return strcmp(x, y) == 0;
 }
 
+// Runs of 2^31 pairs or more, or of more than 2^25 (row, column tile) slots, are split into parts of consecutive
+// rows, each of at most 2^30 pairs and 2^24 slots (the device memory of a part stays below about 25 GB).  The parts
+// run one after the other; their links follow each other in tn93's row order, their histograms add, and their rows
+// enter one row-order merge.  F_ST runs are not split.
+constexpr int64_t SPLIT_PAIRS = 1ll << 31, PART_PAIRS = 1ll << 30, SPLIT_SLOTS = 1ll << 25, PART_SLOTS = 1ll << 24;
+
+// The run's rows [r0, r1) and columns [c0, c1) (triangle: only j > i), from the set's size and the parameters.
+static void run_space(const tn93g_set *S, const tn93g_params *P, int64_t *r0, int64_t *r1, int64_t *c0, int64_t *c1,
+                      bool *tri)
+{
+  const int64_t n = S->d.n;
+  *r0 = 0;
+  *r1 = P->kind == TN93G_CROSS ? P->n1 : n;
+  *c0 = P->kind == TN93G_CROSS || P->kind == TN93G_APPEND ? P->n1 : 0;
+  *c1 = n;
+  *tri = P->kind != TN93G_CROSS;
+}
+
+// The parts of a split run, or none when the run is done in one piece.
+static std::vector<std::pair<int64_t, int64_t>> split_parts(const tn93g_set *S, const tn93g_params *P)
+{
+  std::vector<std::pair<int64_t, int64_t>> parts;
+  if (P->kind == TN93G_FST || (P->kind != TN93G_TRIANGLE && (P->n1 < 0 || P->n1 > S->d.n))) return parts;
+  int64_t r0, r1, c0, c1;
+  bool tri;
+  run_space(S, P, &r0, &r1, &c0, &c1, &tri);
+  if (r1 <= r0 || c1 <= c0) return parts;
+  const int64_t gx = (c1 - (c0 & ~(int64_t)63) + TJ - 1) / TJ;
+  const int64_t pairs = tri ? tri_pairs(r0, r1, c0, c1) : (r1 - r0) * (c1 - c0);
+  const char *e = getenv("TN93G_PART_PAIRS");  // smaller parts (tests of the split at small sizes)
+  const int64_t part_pairs = e && *e ? std::max<int64_t>(atoll(e), 1) : PART_PAIRS;
+  if (pairs < SPLIT_PAIRS && (r1 - r0) * gx <= SPLIT_SLOTS && !(e && *e && pairs > part_pairs)) return parts;
+  const int64_t max_rows = std::max<int64_t>(PART_SLOTS / std::max<int64_t>(gx, 1) / 32 * 32, 32);
+  int64_t a = r0;
+  while (a < r1) {
+    int64_t b = a, p = 0;
+    while (b < r1 && b - a < max_rows) {  // rows in blocks of 32 while the part's pairs fit
+      const int64_t nb = std::min(b + 32, r1);
+      const int64_t q = tri ? tri_pairs(b, nb, c0, c1) : (nb - b) * (c1 - c0);
+      if (b > a && p + q > part_pairs) break;
+      p += q;
+      b = nb;
+    }
+    parts.push_back({a, b});
+    a = b;
+  }
+  return parts;
+}
+
 extern "C" {
 
 int tn93g_result_certify(const tn93g_result *r, int precision)
@@ -4140,26 +4254,39 @@ static int autotune(tn93g_ctx *c, Job &J)
 int tn93g_run_bytes(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, int64_t *bytes)
 {
   if (!c || !S || !P || !bytes) return TN93G_E_ARG;
-  Job J;
-  J.S = S;
-  J.P = *P;
-  J.B = c->slot(0);
-  size_t total = 0;
-  const int rc = job_prepare(c, J, &total);
-  if (rc) return rc;
-  *bytes = (int64_t)total;
+  // a split run works in the memory of its largest part
+  auto parts = split_parts(S, P);
+  if (parts.empty()) parts.push_back({-1, -1});
+  size_t most = 0;
+  for (const auto &pr : parts) {
+    Job J;
+    J.S = S;
+    J.P = *P;
+    J.B = c->slot(0);
+    J.ra = pr.first;
+    J.rb = pr.second;
+    size_t total = 0;
+    const int rc = job_prepare(c, J, &total);
+    if (rc) return rc;
+    most = std::max(most, total);
+  }
+  *bytes = (int64_t)most;
   return 0;
 }
 
-int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_result **out)
+}  // extern "C"
+
+// One run, or the part [ra, rb) of the rows of a split run (ra >= 0: its rows go to `sink`, and its result holds
+// its links, histogram and counters).
+static int run_one(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, int64_t ra, int64_t rb, RowSink *sink,
+                   tn93g_result **out)
 {
-  if (!c || !S || !P || !out) return TN93G_E_ARG;
-  *out = nullptr;
-  HIPCHK(c, hipSetDevice(c->device));
   Job J;
   J.S = S;
   J.P = *P;
   J.B = c->slot(0);
+  J.ra = ra;
+  J.rb = rb;
   int rc = job_prepare(c, J);
   if (rc) return rc;
   if ((rc = autotune(c, J))) return rc;
@@ -4180,15 +4307,15 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
     // an output decision could not be certified from the enclosures: run again resolving logs on the host
     tn93g_params Q = *P;
     Q.flags |= TN93G_FLAG_EXACT;
-    return tn93g_run(c, S, &Q, out);
+    return run_one(c, S, &Q, ra, rb, sink, out);
   }
   if (J.fast && J.max_region > J.link_cap) {  // more links than a region holds: grow the regions and run again
     J.B->link_cap_hint = (int64_t)J.max_region;
-    return tn93g_run(c, S, P, out);
+    return run_one(c, S, P, ra, rb, sink, out);
   }
   if (J.A.exact && J.n_exact > J.exact_cap) {  // more uncertain pairs than the host-exact list holds: grow it
     J.B->exact_hint = (int64_t)J.n_exact + J.n_exact / 4;
-    return tn93g_run(c, S, P, out);
+    return run_one(c, S, P, ra, rb, sink, out);
   }
   tn93g_result *R = (tn93g_result *)calloc(1, sizeof(tn93g_result));
   if (!R) return TN93G_E_NOMEM;
@@ -4204,9 +4331,9 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
         HIPCHK(c, hipStreamSynchronize(c->stream));
         lk = (const LinkRec *)c->pin_links.p;
       }
-      return job_fast_result(J, pf, lk, R);
+      return job_fast_result(J, pf, lk, R, sink);
     }
-    return job_exact_result(c, J, R);
+    return job_exact_result(c, J, R, sink);
   };
   if ((rc = finish())) {
     tn93g_result_free(R);
@@ -4215,6 +4342,88 @@ int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_res
   set_times(c, R, true);
   *out = R;
   return 0;
+}
+
+// A split run: the parts in row order, merged into one result as one run would give it.
+static int run_split(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P,
+                     const std::vector<std::pair<int64_t, int64_t>> &parts, tn93g_result **out)
+{
+  RowSink sink;
+  std::vector<tn93g_result *> res;
+  auto drop = [&](int rc) {
+    for (tn93g_result *r : res) tn93g_result_free(r);
+    return rc;
+  };
+  bool exact = true;
+  for (const auto &pr : parts) {
+    tn93g_result *r = nullptr;
+    const int rc = run_one(c, S, P, pr.first, pr.second, &sink, &r);
+    if (rc) return drop(rc);
+    res.push_back(r);
+    exact = exact && r->exact;
+  }
+  tn93g_result *R = (tn93g_result *)calloc(1, sizeof(tn93g_result));
+  if (!R) return drop(TN93G_E_NOMEM);
+  size_t nl = 0;
+  for (tn93g_result *r : res) nl += (size_t)r->n_links;
+  if (alloc_links(R, nl)) {
+    free(R);
+    return drop(TN93G_E_NOMEM);
+  }
+  size_t k = 0;
+  for (tn93g_result *r : res) {
+    const size_t m = (size_t)r->n_links;
+    if (m) {
+      memcpy(R->link_row + k, r->link_row, m * 8);
+      memcpy(R->link_col + k, r->link_col, m * 8);
+      memcpy(R->link_dist + k, r->link_dist, m * 8);
+      memcpy(R->link_dist_lo + k, r->link_dist_lo, m * 8);
+      memcpy(R->link_dist_hi + k, r->link_dist_hi, m * 8);
+    }
+    k += m;
+    for (int q = 0; q < 600; q++) (&R->hist[0][0])[q] += (&r->hist[0][0])[q];
+    R->n_replayed += r->n_replayed;
+    R->n_uncertain += r->n_uncertain;
+    R->n_host_exact += r->n_host_exact;
+    R->gpu_ms += r->gpu_ms;
+    R->pair_kernel_ms += r->pair_kernel_ms;
+    R->tail_kernel_ms += r->tail_kernel_ms;
+  }
+  R->backend_used = res[0]->backend_used;
+  R->tile_rows_used = res[0]->tile_rows_used;
+  R->folded_columns = res[0]->folded_columns;
+  drop(0);
+  int64_t r0, r1, c0, c1;
+  bool tri;
+  run_space(S, P, &r0, &r1, &c0, &c1, &tri);
+  const long long self_w = tri ? self_weights(S, c0, S->d.n) : 0;
+  if (tri)  // self_v_self histogram entries (swifttn93.cpp:2406-2415), once for the whole run
+    for (int64_t i = c0; i < S->d.n; i++) {
+      const int64_t wi = S->hw[(size_t)i];
+      if (wi > 1) R->hist[0][0] += (unsigned long long)((wi * (wi - 1)) >> 1);
+    }
+  R->nrows = (int32_t)(r1 - r0);
+  sink.M.finish(R, sink.rows, self_w);
+  if (exact && sink.exact) {  // every part exact: tn93's row-order sums themselves
+    R->mean_lo[0] = R->mean_hi[0] = sink.mean[0];
+    R->weighted[0] = sink.weighted[0] + (double)self_w;
+    R->max_lo[0] = R->max_hi[0] = sink.maxv[0];
+  }
+  R->exact = exact && sink.exact ? 1 : 0;
+  *out = R;
+  return 0;
+}
+
+extern "C" {
+
+int tn93g_run(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_result **out)
+{
+  if (!c || !S || !P || !out) return TN93G_E_ARG;
+  *out = nullptr;
+  HIPCHK(c, hipSetDevice(c->device));
+  const auto parts = split_parts(S, P);
+  if (!parts.empty()) return run_split(c, S, P, parts, out);
+  return run_one(c, S, P, -1, -1, nullptr, out);
 }
 
 int tn93g_run_noalloc(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, tn93g_result *R,
@@ -4293,15 +4502,15 @@ int tn93g_run_noalloc(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, t
   {
     constexpr int64_t CH = 1024;  // 64 KB of row accumulators per copy, merged in row order
     RowAcc buf[CH];
-    RowMerge M(J);
+    RowMerge M;
     for (int64_t r0 = 0; r0 < J.nrows; r0 += CH) {
       const int64_t m = std::min(CH, J.nrows - r0);
       HIPCHK(c, hipMemcpyAsync(buf, (const RowAcc *)B.rowout.p + r0, (size_t)m * sizeof(RowAcc),
                                hipMemcpyDeviceToHost, c->stream));
       HIPCHK(c, hipStreamSynchronize(c->stream));
-      for (int64_t r = 0; r < m; r++) M.add(buf[r], r0 + r);
+      for (int64_t r = 0; r < m; r++) M.add(buf[r], row_pairs(J.A, r0 + r));
     }
-    M.finish(R);
+    M.finish(R, J.nrows, J.A.triangle ? self_weights(J.S, J.A.c0, J.A.n) : 0);
   }
   unsigned long long hrep[600];
   HIPCHK(c, hipMemcpyAsync(hrep, B.hist.p, sizeof hrep, hipMemcpyDeviceToHost, c->stream));
@@ -4330,7 +4539,10 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
   // exact-path jobs (host work between kernels) run alone after the batch
   std::vector<int> idx, alone;
   for (int k = 0; k < n; k++)
-    ((params[k].flags & TN93G_FLAG_EXACT) || params[k].kind == TN93G_FST ? alone : idx).push_back(k);
+    ((params[k].flags & TN93G_FLAG_EXACT) || params[k].kind == TN93G_FST || !split_parts(sets[k], &params[k]).empty()
+         ? alone
+         : idx)
+        .push_back(k);
   const size_t m = idx.size();
   std::vector<Job> jobs(m);
   for (size_t t = 0; t < m; t++) {

@@ -170,8 +170,14 @@ def test_constant_columns_vs_oracle(workload, tmp_path):
 def _same(a, b):
     for k in ["hist", "link_row", "link_col", "link_dist", "link_lo", "link_hi"]:
         assert np.array_equal(np.asarray(a[k]), np.asarray(b[k])), k
-    for k in ["mean_lo", "mean_hi", "weighted", "max_lo", "max_hi", "found_links", "skipped", "exact"]:
+    for k in ["weighted", "max_lo", "max_hi", "found_links", "skipped", "exact"]:
         assert a[k] == b[k], (k, a[k], b[k])
+    # the fast path adds its row sums by atomics, in no fixed order: the mean's enclosure may differ in its last bits
+    # from run to run (exact results are tn93's own sums and must be equal)
+    lo, hi, lo1, hi1 = a["mean_lo"][0], a["mean_hi"][0], b["mean_lo"][0], b["mean_hi"][0]
+    assert lo <= hi1 and lo1 <= hi and abs(lo - lo1) <= 1e-12 * abs(lo1) and abs(hi - hi1) <= 1e-12 * abs(hi1)
+    if a["exact"]:
+        assert a["mean_lo"] == b["mean_lo"] and a["mean_hi"] == b["mean_hi"]
 
 
 def test_folded_backends_and_tiles():

@@ -328,23 +328,123 @@ def test_serving_full_panel_alone_vs_mixed():
         srv.close()
 
 
-def test_more_than_2_32_pairs_refused_cleanly(tmp_path):
-    """One run holds fewer than 2^32 pairs: a triangle of 92,683 sequences (2^32 + 60,209 pairs) is refused with
-    TN93G_E_ARG and a message before anything is allocated, through the C ABI and as the CLI's exit 1."""
-    n = 92683
-    codes = gen.Stream(191).integers(4, n * 32).reshape(n, 32).astype(np.uint8)  # random: distances saturate
-    codes[n - 5:] = codes[:5]  # five pairs at distance 0, all in the appended range below
+def _haplotypes(L, seed):
+    """Four haplotypes of L sites: h1 12 sites from h0, h2 30 other sites from h0, h3 one site from h2."""
+    rs = np.random.default_rng(seed)
+    h0 = rs.integers(0, 4, L).astype(np.uint8)
+    sites = rs.permutation(L)
+    h1, h2 = h0.copy(), h0.copy()
+    h1[sites[:12]] = (h1[sites[:12]] + 1) % 4
+    h2[sites[12:42]] = (h2[sites[12:42]] + 2) % 4
+    h3 = h2.copy()
+    h3[sites[42]] = (h3[sites[42]] + 1) % 4
+    return np.stack([h0, h1, h2, h3])
+
+
+def test_more_than_2_32_pairs():
+    """A triangle of 92,700 sequences (2^32 + 1,597,354 pairs) runs in parts.  The sequences are copies of four
+    haplotypes (92,000 + 600 + 50 + 50, shuffled), so the run equals the four haplotypes with those copy numbers:
+    the same histogram, link count and weight sum, an overlapping mean enclosure, and as links exactly the h2-h3
+    pairs, in row order, at the h2-h3 distance."""
+    L, m = 200, [92000, 600, 50, 50]
+    H = _haplotypes(L, 31)
+    lab = np.random.default_rng(32).permutation(np.repeat(np.arange(4), m))
+    n = lab.size
+    assert n * (n - 1) // 2 > 2 ** 32
+    codes = H[lab]
+    kw = dict(mode=T.RESOLVE, threshold=0.015, min_distance=1e-9, min_overlap=10)
     eng = T.Engine(0)
     try:
+        small = eng.upload(H, weights=np.array(m))
+        want = eng.run(small, **kw)
+        print("four haplotypes: links", want["found_links"], "hist nonzero", int((want["hist"] > 0).sum()), flush=True)
+        small.free()
         s = eng.upload(codes)
-        with pytest.raises(RuntimeError, match="2\\^32 pairs"):
-            eng.run(s, mode=T.RESOLVE, threshold=0.015, min_overlap=10)
-        ok = eng.run(s, kind=T.APPEND, n1=n - 1000, n2=n, mode=T.RESOLVE, threshold=0.015, min_overlap=10)
-        assert ok["skipped"] == 0 and ok["found_links"] >= 5  # the same set in a smaller range still runs
+        got = eng.run(s, **kw)
         s.free()
     finally:
         eng.close()
-    fa = tmp_path / "big.fa"
-    gen.write_fasta(fa, codes)
-    rc, out, err = _exec(["-q", "-t", "0.015", "-l", "10", "-o", "out.csv", str(fa)], tmp_path)
-    assert rc == 1 and b"2^32 pairs" in err, err[-300:]
+    assert np.array_equal(got["hist"], want["hist"])
+    assert got["found_links"] == want["found_links"] == 50 * 50
+    assert got["weighted"][0] == want["weighted"][0] == n * (n - 1) / 2
+    assert got["skipped"] == want["skipped"] == 0
+    assert got["mean_lo"][0] <= want["mean_hi"][0] and want["mean_lo"][0] <= got["mean_hi"][0]
+    i_idx, j_idx = [], []
+    pos2, pos3 = np.flatnonzero(lab == 2), np.flatnonzero(lab == 3)
+    for i in np.sort(np.concatenate([pos2, pos3])):
+        other = pos3 if lab[i] == 2 else pos2
+        js = other[other > i]
+        i_idx += [i] * js.size
+        j_idx += list(js)
+    assert np.array_equal(got["link_row"], np.array(i_idx)) and np.array_equal(got["link_col"], np.array(j_idx))
+    d23 = want["link_dist"][0]
+    assert (got["link_dist"] == d23).all() and len(want["link_dist"]) == 1
+
+
+SPLIT_KINDS = [
+    ("triangle", dict(mode=T.RESOLVE, resolve_fraction=0.05)),
+    ("triangle", dict(mode=T.AVERAGE)),
+    ("triangle", dict(mode=T.GAPMM)),
+    ("triangle", dict(mode=T.RESOLVE, resolve_fraction=0.05, flags=T.FLAG_EXACT)),
+    ("cross", dict(mode=T.RESOLVE, resolve_fraction=0.05)),
+    ("append", dict(mode=T.SKIP)),
+]
+
+
+@pytest.mark.parametrize("k", range(len(SPLIT_KINDS)))
+def test_split_equals_one_run(k):
+    """Runs cut into parts of at most 20,000 pairs (TN93G_PART_PAIRS) give the whole run's result field for field:
+    links, histogram, the mean's enclosure, maxima, counts; with copy numbers and replayed B/D/H/V pairs."""
+    kind, kw = SPLIT_KINDS[k]
+    codes = gen.c1s(600, seed=33)
+    w = np.random.default_rng(34).integers(1, 4, 600)
+    kw = dict(kw, threshold=0.03, min_overlap=300)
+    if kind == "cross":
+        kw.update(kind=T.CROSS, n1=250)
+    elif kind == "append":
+        kw.update(kind=T.APPEND, n1=420, n2=600)
+    eng = T.Engine(0)
+    try:
+        s = eng.upload(codes, weights=w)
+        whole = eng.run(s, **kw)
+        os.environ["TN93G_PART_PAIRS"] = "20000"
+        try:
+            parts = eng.run(s, **kw)
+        finally:
+            del os.environ["TN93G_PART_PAIRS"]
+        s.free()
+    finally:
+        eng.close()
+    print(kind, kw, "links", whole["found_links"], "replayed", whole["n_replayed"], "exact", whole["exact"], flush=True)
+    for f in ["hist", "link_row", "link_col", "link_dist", "link_lo", "link_hi"]:
+        assert np.array_equal(np.asarray(parts[f]), np.asarray(whole[f])), f
+    for f in ["weighted", "max_lo", "max_hi", "found_links", "skipped", "exact", "n_replayed"]:
+        assert parts[f] == whole[f], (f, parts[f], whole[f])
+    # the fast path's row sums are added by atomics, in no fixed order: the mean's enclosures may differ in their last
+    # bits from run to run, and must overlap and agree to far more digits than are printed
+    lo, hi, lo1, hi1 = parts["mean_lo"][0], parts["mean_hi"][0], whole["mean_lo"][0], whole["mean_hi"][0]
+    assert lo <= hi1 and lo1 <= hi and abs(lo - lo1) <= 1e-12 * abs(lo1) and abs(hi - hi1) <= 1e-12 * abs(hi1)
+    if whole["exact"]:
+        assert parts["mean_lo"] == whole["mean_lo"] and parts["mean_hi"] == whole["mean_hi"]
+    assert len(whole["link_row"]) > 0 and (whole["n_replayed"] > 0 or kw["mode"] == T.SKIP)
+
+
+def test_split_cli_vs_oracle(tmp_path):
+    """The command with its runs cut into parts of at most 5,000 pairs, byte for byte against the oracle: every
+    pair written, -s, and copy numbers."""
+    codes = gen.c1s(300, seed=35)
+    fa, fb = tmp_path / "in.fa", tmp_path / "ref.fa"
+    gen.write_fasta(fa, codes[:200], [f"s{i}:{1 + i % 3}" for i in range(200)])
+    gen.write_fasta(fb, codes[200:], [f"r{i}" for i in range(100)])
+    cases = [["-q", "-t", "1", "-f", "csvn", "-a", "resolve", "-g", "0.05", "-l", "500"],
+             ["-q", "-t", "0.03", "-a", "average", "-l", "500", "-s", str(fb)],
+             ["-q", "-t", "0.03", "-a", "gapmm", "-l", "500", "-0"]]
+    env = dict(os.environ, TN93G_PART_PAIRS="5000")
+    for k, argv in enumerate(cases):
+        want = subprocess.run([str(_oracle_cli()), *argv, "-o", str(tmp_path / f"w{k}"), str(fa)], capture_output=True,
+                              check=True)
+        p = subprocess.run([str(_cli()), *argv, "-o", str(tmp_path / f"g{k}"), str(fa)], capture_output=True,
+                           env=env, timeout=300)
+        assert p.returncode == 0, p.stderr[-500:]
+        assert (tmp_path / f"g{k}").read_bytes() == (tmp_path / f"w{k}").read_bytes(), argv
+        assert p.stdout == want.stdout, (argv, p.stdout[-300:], want.stdout[-300:])
