@@ -6,7 +6,7 @@ columns (swifttn93_amd/gen.py c1), options -t 0.015 -a resolve -g 0.05 -l 500, 2
 resident in HBM before the timed region; a step is one tn93g_run: clean/clean counts, ambiguity events,
 site-order replays, FP64 tail, histogram, per-row sums in tn93's order and the link list copied back to the host.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--backend popc|mfma] [--no-cpu] [--config c1|c2|c3|c4]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--backend popc|mfma] [--no-cpu] [--config c1|c2|c3|c4|sars2]
                     [--mode resolve|average|skip|gapmm]
 
 --mode runs the same 6,700 x 1,300 set with another -a mode (-g 0.05 only with resolve; the default line is
@@ -15,7 +15,8 @@ resolve).
 Other workloads (--config): c2, the SARS-CoV-2-shaped long alignment (20,000 x 29,903, N-masked blocks,
 -t 0.0005 -a resolve -g 0.05 -l 25000); c3, 64 uneven alignments with mixed -a modes run as one batch (one graph
 launch) beside the same jobs run one by one, and the 2,000 x 40,000 -s bipartite block; c4, serving: a 50,000 x
-1,300 device-resident panel answering a Poisson stream of query batches (p50/p99 latency, pairs/s).
+1,300 device-resident panel answering a Poisson stream of query batches (p50/p99 latency, pairs/s); sars2, the
+reference's real SARS-CoV-2 dev set (500 x 29,903, two thirds of the columns constant), folded and not.
 
 Prints one JSON line.  Beside `value` (the engine with the set resident) it carries:
   steady_state     the whole tn93 command in a warm process (tn93g's --batch driver, the same invocation repeated):
@@ -274,6 +275,86 @@ def bench_c2(a, T, gen):
     print(json.dumps(line))
 
 
+def fasta_codes(path):
+    """(names, codes) of an aligned FASTA file, codes as ValidChars indices (upper or lower case)."""
+    import numpy as np
+    lut = np.full(256, 255, dtype=np.uint8)
+    for k, ch in enumerate("ACGTURYSWKMBDHVN?-"):
+        lut[ord(ch)] = lut[ord(ch.lower())] = k
+    names, rows = [], []
+    for block in Path(path).read_text().split(">")[1:]:
+        lines = block.splitlines()
+        names.append(lines[0])
+        rows.append(lut[np.frombuffer("".join(lines[1:]).encode(), dtype=np.uint8)])
+    return names, np.stack(rows)
+
+
+def bench_sars2(a, T, gen):
+    """The reference's real SARS-CoV-2 dev set (data/real_sars2_dev.fa.xz, kept under tests/golden/real/): 500
+    genomes x 29,903 columns, of which 20,673 hold one clean base in every genome and are folded out of the device
+    arrays, with the reference's benchmark flags -t 0.0002 -l 20000 -0; the engine resident folded and with every
+    column kept, the whole command warm and cold, and the reference beside it."""
+    import lzma
+    import numpy as np
+    src = REPO / "tests" / "golden" / "real" / "real_sars2_dev.fa.xz"
+    flags = ["-t", "0.0002", "-l", "20000", "-0"]
+    with tempfile.TemporaryDirectory() as td:
+        fa = Path(td) / "sars2_dev.fa"
+        fa.write_bytes(lzma.decompress(src.read_bytes()))
+        _, codes = fasta_codes(fa)
+        n, L = codes.shape
+        n_pairs = n * (n - 1) // 2
+        eng = T.Engine(0)
+        kw = dict(mode=T.RESOLVE, threshold=0.0002, min_overlap=20000, resolve_fraction=1.0)
+        runs = {}
+        for name, nofold in [("folded", "0"), ("all_columns", "1")]:
+            os.environ["TN93G_NOFOLD"] = nofold
+            try:
+                s = eng.upload(codes)
+            finally:
+                del os.environ["TN93G_NOFOLD"]
+            for _ in range(max(a.warmup, 1)):
+                r = eng.run(s, **kw)
+            steps = max(a.steps, 20)
+            pk, t0 = [], time.perf_counter()
+            for _ in range(steps):
+                r = eng.run(s, **kw)
+                pk.append(r["pair_kernel_ms"])
+            dt = (time.perf_counter() - t0) / steps
+            runs[name] = {"ms_per_step": dt * 1e3, "pairs_per_s": n_pairs / dt, "pair_kernel_ms": float(np.mean(pk)),
+                          "folded_columns": int(r["folded_columns"]), "links": int(r["found_links"])}
+            s.free()
+        eng.close()
+        value = runs["folded"]["pairs_per_s"]
+        line = {"metric": METRIC, "value": value, "unit": "pairs/s", "n_gpus": 1, "steps": max(a.steps, 20),
+                "warmup": a.warmup, "ms_per_step": runs["folded"]["ms_per_step"], "higher_is_better": True,
+                "scaling": "weak", "vs_baseline": None, "dtype": "int32+f64",
+                "data": "real (the reference's SARS-CoV-2 dev set, 500 GenBank genomes aligned to MN908947)",
+                "config": {"workload": "real SARS-CoV-2 dev set, 500 x 29,903, -t 0.0002 -l 20000 -0",
+                           "pairs": n_pairs, "parallelism": "1 GPU"},
+                "folding": runs}
+        cases = []
+        for k in range(6):  # the first case also pays the device's initialisation
+            d = Path(td) / f"e{k}"
+            d.mkdir()
+            cases.append((d, ["-q", *flags, "-o", "out.csv", str(fa)]))
+        res = tn93g_batch(cases, td)
+        warm = float(np.median([w for _, _, _, w in res[1:]]))
+        line["steady_state"] = {"value": n_pairs / warm, "wall_s": warm,
+                                "cold_cli_wall_s": min(cold_cli([*flags, str(fa)], td) for _ in range(3))}
+        if not a.no_cpu:
+            base, keep = cpu_baseline([[*flags, str(fa)]], n_pairs, "the whole workload (one command)")
+            if base:
+                line["cpu_baseline"] = base
+                line["gpu_over_cpu"] = value / base["value"]
+                line["steady_state_over_cpu"] = line["steady_state"]["value"] / base["value"]
+                line["cold_cli_over_cpu"] = base["wall_s"] / line["steady_state"]["cold_cli_wall_s"]
+                ok = all(rc == 0 and (o, so) == keep[0] for rc, o, so, _ in res)
+                line["parity"] = {"sha256_ok": ok, "output_sha256": res[0][1],
+                                  "checked": "distance file + stdout of every GPU command vs the reference's"}
+    print(json.dumps(line))
+
+
 def bench_c3(a, T, gen):
     import numpy as np
     jobs_codes = gen.c3()
@@ -516,7 +597,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (torch), the reference for the HBM byte counters")
-    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c1", choices=["c1", "c2", "c3", "c4", "sars2"])
     ap.add_argument("--mode", default="resolve", choices=["resolve", "average", "skip", "gapmm"],
                     help="c1: tn93's -a mode (resolve runs with -g 0.05)")
     ap.add_argument("--serve-seconds", type=float, default=60.0, help="c4: length of the query stream")
@@ -549,6 +630,8 @@ def main():
         return bench_c3(a, T, gen)
     if a.config == "c4":
         return bench_c4(a, T, gen)
+    if a.config == "sars2":
+        return bench_sars2(a, T, gen)
 
     codes = gen.c1()
     n, L = codes.shape

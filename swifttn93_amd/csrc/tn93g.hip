@@ -574,17 +574,6 @@ __device__ inline bool is_event(unsigned code, int mode)
 
 // Runs of one code (N, or the gap) in each sequence: run k = [start, end) of consecutive sites holding it, in site
 // order.
-__global__ void k_runcount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls,
-                           int64_t s0, int code)
-{
-  const int64_t s = s0 + blockIdx.x;
-  const uint8_t x = (uint8_t)code;
-  int c = 0;
-  for (int64_t p = threadIdx.x; p < L; p += blockDim.x)
-    c += (codes[s * Ls + p] == x && (p == 0 || codes[s * Ls + p - 1] != x)) ? 1 : 0;
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
-  if (threadIdx.x == 0) cnt[blockIdx.x] = c;
-}
 __global__ void k_runfill(const uint8_t *__restrict__ codes, const int32_t *__restrict__ off, uint2 *__restrict__ runs,
                           int64_t L, int64_t Ls, int64_t s0, int code)
 {
@@ -646,14 +635,27 @@ __global__ void k_prefix(const uint4 *__restrict__ planes, const uint32_t *__res
   }
 }
 
-__global__ void k_evcount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls,
-                          int gaps, int64_t s0)
+// Every list's length for sequences s0 + blockIdx.x (one wave each) in one pass over the codes: cnt[k * n + s - s0]
+// for k = 0: EV_AMB events, 1: EV_NO_N events, 2: EV_GAPS events, 3: N runs, 4: gap runs.
+__global__ void k_listcount(const uint8_t *__restrict__ codes, int32_t *__restrict__ cnt, int64_t L, int64_t Ls,
+                            int64_t s0, int64_t n)
 {
   const int64_t s = s0 + blockIdx.x;
-  int c = 0;
-  for (int64_t p = threadIdx.x; p < L; p += blockDim.x) c += is_event(codes[s * Ls + p], gaps) ? 1 : 0;
-  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
-  if (threadIdx.x == 0) cnt[blockIdx.x] = c;
+  const uint8_t *q = codes + s * Ls;
+  int c[5] = {0, 0, 0, 0, 0};
+  for (int64_t p = threadIdx.x; p < L; p += blockDim.x) {
+    const unsigned v = q[p], u = p ? q[p - 1] : 255u;
+    c[0] += is_event(v, EV_AMB) ? 1 : 0;
+    c[1] += is_event(v, EV_NO_N) ? 1 : 0;
+    c[2] += is_event(v, EV_GAPS) ? 1 : 0;
+    c[3] += v == (unsigned)NC && u != (unsigned)NC ? 1 : 0;
+    c[4] += v == (unsigned)GAPC && u != (unsigned)GAPC ? 1 : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    for (int o = 32; o > 0; o >>= 1) c[k] += __shfl_down(c[k], o, 64);
+    if (threadIdx.x == 0) cnt[k * n + blockIdx.x] = c[k];
+  }
 }
 
 // events in site order: one wave per sequence
@@ -2600,6 +2602,8 @@ struct tn93g_set {
   std::vector<int64_t> hw;                            // copy numbers (host), cap entries
   std::vector<uint8_t> spec;                          // 1: the sequence has a B, D, H or V code (host), cap entries
   std::vector<void *> retired;                        // replaced event arrays
+  void *listcnt = nullptr;                            // the list lengths of an append (5 x n int32), grown on demand
+  size_t listcnt_bytes = 0;
   std::mutex mu;
 };
 
@@ -3819,13 +3823,14 @@ static int set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64
   if (n0 + n > S->cap) return c->fail(TN93G_E_ARG, "set capacity exceeded");
   if (n == 0) return 0;
   // the codes checked, the B/D/H/V flags set and the gap-padded copy made in one pass, blocks of sequences on up to
-  // 16 host threads (one below 4 MB)
+  // 16 host threads (one per 1 MB copied, one per 256 KB gathered from kept columns)
   std::vector<uint8_t> padded((size_t)(n * d.Ls));
   std::atomic<bool> bad{false};
   {
     const int64_t bytes = n * L;
     const int nt = (int)std::max<int64_t>(
-        1, std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16, bytes >> 22, n}));
+        1, std::min<int64_t>({(int64_t)std::max(1u, std::thread::hardware_concurrency()), 16,
+                              cols ? bytes >> 18 : bytes >> 20, n}));
     auto work = [&](int t) {
       for (int64_t q = n * t / nt; q < n * (t + 1) / nt && !bad.load(std::memory_order_relaxed); q++) {
         const uint8_t *src = codes + q * srcL;
@@ -3874,29 +3879,34 @@ static int set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64
     hipLaunchKernelGGL(k_onehot, dim3((unsigned)((toh + 255) / 256)), dim3(256), 0, c->stream, d.codes, d.onehot, n1,
                        L, d.Ls, d.W64, ob, oe);
   }
-  int32_t *cnt_d = nullptr, *cnt2_d = nullptr;
-  HIPCHK(c, hipMalloc(&cnt_d, (size_t)n * sizeof(int32_t)));
-  HIPCHK(c, hipMalloc(&cnt2_d, (size_t)n * sizeof(int32_t)));
+  // every list's lengths in one kernel and one read-back; then per list the offsets, the array (grown when needed)
+  // and the fill kernel, with one synchronisation at the end
   int rc = 0;
+  if (S->listcnt_bytes < (size_t)(5 * n) * sizeof(int32_t)) {
+    if (S->listcnt) (void)hipFree(S->listcnt);
+    S->listcnt = nullptr;
+    S->listcnt_bytes = 0;
+    if (hipMalloc(&S->listcnt, (size_t)(5 * n) * sizeof(int32_t)) != hipSuccess)
+      return c->fail(TN93G_E_NOMEM, "list count allocation failed");
+    S->listcnt_bytes = (size_t)(5 * n) * sizeof(int32_t);
+  }
+  int32_t *cnt_d = (int32_t *)S->listcnt;
+  hipLaunchKernelGGL(k_listcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, n0, n);
+  std::vector<int32_t> cnt((size_t)(5 * n));
+  HIPCHK(c, hipMemcpyAsync(cnt.data(), cnt_d, cnt.size() * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
   if (!S->decided) {
     // dense N: on average 32 or more N sites per sequence of the first batch; then N stays out of the ambiguity
     // event lists for the life of the set
-    hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, (int)EV_AMB,
-                       n0);
-    hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt2_d, L, d.Ls, (int)EV_NO_N,
-                       n0);
-    std::vector<int32_t> a((size_t)n), b((size_t)n);
-    if (hipMemcpyAsync(a.data(), cnt_d, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-        hipMemcpyAsync(b.data(), cnt2_d, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess)
-      rc = c->fail(TN93G_E_HIP, "event count read-back failed");
     int64_t nsites = 0;
-    for (int64_t k = 0; k < n; k++) nsites += a[(size_t)k] - b[(size_t)k];
+    for (int64_t k = 0; k < n; k++) nsites += cnt[(size_t)k] - cnt[(size_t)(n + k)];
     d.ndense = nsites >= 32 * n ? 1 : 0;
     S->decided = true;
   }
   // lists: 0 = ambiguity events (without N when dense), 1 = with gap sites (GAPMM), 2 = N runs (dense N only),
   // 3 = gap runs
+  std::vector<int32_t> offs((size_t)(4 * n), 0);  // kept until the synchronisation below (pageable source)
+  int64_t totals[4] = {S->ev_total[0], S->ev_total[1], S->ev_total[2], S->ev_total[3]};
   for (int li = 0; li < 4 && !rc; li++) {
     if (li == 2 && !d.ndense) continue;
     int32_t *eoff = li == 0 ? d.ev_off : li == 1 ? d.evg_off : li == 2 ? d.nrun_off : d.grun_off;
@@ -3904,21 +3914,12 @@ static int set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64
     void *arr = li == 0 ? (void *)d.ev : li == 1 ? (void *)d.evg : li == 2 ? (void *)d.nruns : (void *)d.gruns;
     const int mode = li == 1 ? EV_GAPS : d.ndense ? EV_NO_N : EV_AMB;
     const int rcode = li == 2 ? NC : GAPC;
-    if (li < 2)
-      hipLaunchKernelGGL(k_evcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, mode, n0);
-    else
-      hipLaunchKernelGGL(k_runcount, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, cnt_d, L, d.Ls, n0, rcode);
-    std::vector<int32_t> cnt((size_t)n, 0), off((size_t)n, 0);
-    if (hipMemcpyAsync(cnt.data(), cnt_d, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream) !=
-            hipSuccess ||
-        hipStreamSynchronize(c->stream) != hipSuccess) {
-      rc = c->fail(TN93G_E_HIP, "event count read-back failed");
-      break;
-    }
-    int64_t total = S->ev_total[li];
+    const int32_t *lc = cnt.data() + (size_t)n * (li == 0 ? (d.ndense ? 1 : 0) : li == 1 ? 2 : li == 2 ? 3 : 4);
+    int32_t *off = offs.data() + (size_t)li * n;
+    int64_t total = totals[li];
     for (int64_t k = 0; k < n; k++) {
-      total += cnt[(size_t)k];
-      off[(size_t)k] = (int32_t)total;  // offsets of sequences n0 + 1 + k
+      total += lc[(size_t)k];
+      off[(size_t)k] = (int32_t)std::min<int64_t>(total, INT32_MAX);  // offsets of sequences n0 + 1 + k
     }
     if (total >= (1ll << 31)) {
       rc = c->fail(TN93G_E_ARG, "too many ambiguity events");
@@ -3942,7 +3943,7 @@ static int set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64
       else d.gruns = (uint2 *)na;
       S->ev_cap[li] = want;
     }
-    if (hipMemcpyAsync(eoff + n0 + 1, off.data(), (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream) !=
+    if (hipMemcpyAsync(eoff + n0 + 1, off, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, c->stream) !=
         hipSuccess) {
       rc = c->fail(TN93G_E_HIP, "event offsets upload failed");
       break;
@@ -3953,14 +3954,11 @@ static int set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64
     else
       hipLaunchKernelGGL(k_runfill, dim3((unsigned)n), dim3(64), 0, c->stream, d.codes, eoff, (uint2 *)arr, L, d.Ls,
                          n0, rcode);
-    if (hipStreamSynchronize(c->stream) != hipSuccess) {  // `off` is pageable host memory
-      rc = c->fail(TN93G_E_HIP, "event packing failed");
-      break;
-    }
-    S->ev_total[li] = total;
+    totals[li] = total;
   }
-  (void)hipFree(cnt2_d);
-  (void)hipFree(cnt_d);
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !rc) rc = c->fail(TN93G_E_HIP, "event packing failed");
+  if (!rc)
+    for (int li = 0; li < 4; li++) S->ev_total[li] = totals[li];
   if (rc) return rc;
   HIPCHK(c, hipGetLastError());
   for (int64_t s = 0; s < n; s++) S->hw[(size_t)(n0 + s)] = wts[(size_t)s];
@@ -4070,6 +4068,7 @@ void tn93g_set_free(tn93g_set *S)
   for (void *p : ps)
     if (p) (void)hipFree(p);
   for (void *p : S->retired) (void)hipFree(p);
+  if (S->listcnt) (void)hipFree(S->listcnt);
   delete S;
 }
 
