@@ -2692,6 +2692,7 @@ struct tn93g_ctx {
   std::vector<RunBufs *> slots;
   HostBuf pin;  // staging of the read-backs (counters, links, row accumulators, histogram)
   HostBuf pin_links;  // the links of a single run that produced more than its read-back expected
+  HostBuf pin_up;     // staging of appended sequences (sets owned by this context)
   // the instantiated graph of the last batch, launched again when a batch enqueues exactly the same work
   hipGraphExec_t gexec = nullptr;
   std::vector<unsigned char> gkey;
@@ -3742,6 +3743,7 @@ void tn93g_close(tn93g_ctx *c)
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->pin.p) (void)hipHostFree(c->pin.p);
   if (c->pin_links.p) (void)hipHostFree(c->pin_links.p);
+  if (c->pin_up.p) (void)hipHostFree(c->pin_up.p);
   if (c->multi.p) (void)hipFree(c->multi.p);
   if (c->stage.p) (void)hipFree(c->stage.p);
   (void)hipStreamDestroy(c->stream);
@@ -3824,7 +3826,10 @@ static int set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64
   if (n == 0) return 0;
   // the codes checked, the B/D/H/V flags set and the gap-padded copy made in one pass, blocks of sequences on up to
   // 16 host threads (one per 1 MB copied, one per 256 KB gathered from kept columns)
-  std::vector<uint8_t> padded((size_t)(n * d.Ls));
+  // staged in the context's pinned buffer (one copy at full PCIe rate; an append ends synchronised, so the buffer is
+  // free again when the next append of a set of this context starts)
+  if (ensure_pinned(c, c->pin_up, (size_t)(n * d.Ls))) return TN93G_E_NOMEM;
+  uint8_t *padded = (uint8_t *)c->pin_up.p;
   std::atomic<bool> bad{false};
   {
     const int64_t bytes = n * L;
@@ -3860,7 +3865,7 @@ static int set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64
   std::vector<int64_t> wts((size_t)n, 1);
   if (weights)
     for (int64_t s = 0; s < n; s++) wts[(size_t)s] = weights[s];
-  HIPCHK(c, hipMemcpyAsync(d.codes + n0 * d.Ls, padded.data(), padded.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(d.codes + n0 * d.Ls, padded, (size_t)(n * d.Ls), hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipMemcpyAsync(d.weights + n0, wts.data(), wts.size() * sizeof(int64_t), hipMemcpyHostToDevice,
                            c->stream));
   const int64_t n1 = n0 + n;
