@@ -455,10 +455,11 @@ int output(Case &C)
   const long upper = cross ? (long)n1 : (long)N;
   const int64_t nl = R->n_links;
   if (A.format != F_HYPHY && (emit || report_self)) {
-    // rows in pieces of about 64K links, formatted on all threads and written in row order
+    // rows in pieces of 1K-64K links (about two per thread), formatted on all threads and written in row order
     std::vector<long> rb;
     std::vector<int64_t> kb;
-    tn93g_io::split_rows(R->link_row, nl, upper, 1 << 16, rb, kb);
+    const int64_t piece = std::max<int64_t>(1024, std::min<int64_t>(1 << 16, nl / (2 * (int64_t)threads)));
+    tn93g_io::split_rows(R->link_row, nl, upper, piece, rb, kb);
     tn93g_io::emit_pieces(A.output, threads, rb.size() - 1, (size_t)threads * 4, [&](size_t p, std::string &out) {
       char t[64];
       int64_t k = kb[p];
