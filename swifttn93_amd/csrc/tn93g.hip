@@ -689,13 +689,20 @@ struct Cnt {
   uint32_t T, DG, TS, CT, B0, B1, BT;
 };
 
+// v_bitop3_b32 (gfx950): any bitwise function of three operands in one instruction; the truth table is the function
+// evaluated on 0xf0, 0xcc, 0xaa (first, second, third operand).
+constexpr unsigned BOP_A_NB_NC = 0x10;  // a & ~b & ~c
+constexpr unsigned BOP_A_NB_C = 0x20;   // a & ~b & c
+
+// Per word pair: M = both clean; x, y = the C|T and G|T bits differ.  Same base: M & ~x & ~y; a transition (A<->G
+// or C<->T): M & ~x & y, a C<->T one when the C|T bit is set.
 __device__ inline void count_word(const uint4 pi, const uint4 pj, Cnt &c)
 {
   const uint32_t M = pi.x & pj.x;
   const uint32_t x = pi.y ^ pj.y, y = pi.z ^ pj.z;
   c.T += __popc(M);
-  c.DG += __popc(M & ~(x | y));
-  const uint32_t tr = M & y & ~x;
+  c.DG += __popc(__builtin_amdgcn_bitop3_b32(M, x, y, BOP_A_NB_NC));
+  const uint32_t tr = __builtin_amdgcn_bitop3_b32(M, x, y, BOP_A_NB_C);
   c.TS += __popc(tr);
   c.CT += __popc(tr & pi.y);
   c.B0 += __popc(pj.x & pi.y) + __popc(pi.x & pj.y);
@@ -707,7 +714,7 @@ __device__ inline void count_word(const uint4 pi, const uint4 pj, Cnt &c)
 // clean are the totals of both sequences less their clean sites facing a non-clean site of the other sequence, and
 // those sites are all visited anyway: ambiguity events (their table entries subtract the clean base), N runs of
 // dense-N sets (the N-plane counts, through the same entries) and gap runs (prefix counts).  11 operations per word
-// pair instead of 24.
+// pair instead of 24 (9 with v_bitop3_b32).
 struct CntM {
   uint32_t DG, TS, CT;
 };
@@ -716,8 +723,8 @@ __device__ inline void count_word(const uint4 pi, const uint4 pj, CntM &c)
 {
   const uint32_t M = pi.x & pj.x;
   const uint32_t x = pi.y ^ pj.y, y = pi.z ^ pj.z;
-  c.DG += __popc(M & ~(x | y));
-  const uint32_t tr = M & y & ~x;
+  c.DG += __popc(__builtin_amdgcn_bitop3_b32(M, x, y, BOP_A_NB_NC));
+  const uint32_t tr = __builtin_amdgcn_bitop3_b32(M, x, y, BOP_A_NB_C);
   c.TS += __popc(tr);
   c.CT += __popc(tr & pi.y);
 }
@@ -810,8 +817,11 @@ struct PairLds<KWORDS, false, TI_> {
 };
 
 // Staging of the planes of words [w0, w0 + kw) of the tile's sequences, in two steps so that the loads of the next
-// chunk are in flight while the current one is counted: SPT values per thread (consecutive threads: consecutive
-// words of one sequence), N-plane words too for dense-N sets.
+// chunk are in flight while the current one is counted: SPT values per thread, N-plane words too for dense-N sets.
+// Every sequence has KW slots whatever kw is: thread t takes word t % KW of sequences t / KW + 16 q, so the slot of
+// a thread is fixed for the whole tile (no division by kw) and consecutive threads read consecutive words of one
+// sequence; slots at or past kw load nothing and are never read.
+static_assert(KW == 16, "staging slots: 16 words per sequence, 16 sequences per 256 threads");
 template <int TI_>
 constexpr int spt() { return (KW * (TI_ + TJ) + 255) / 256; }
 template <bool NP, int TI_>
@@ -823,12 +833,12 @@ template <bool NP, int TI_>
 __device__ __attribute__((always_inline)) inline void stage_load(const RunArgs &A, int64_t i0, int64_t j0, int64_t w0,
                                                                  int kw, int tid, StageRegs<NP, TI_> &r)
 {
+  const int w = tid & (KW - 1);
 #pragma unroll
   for (int q = 0; q < spt<TI_>(); q++) {
-    const int k = tid + 256 * q;
-    const int s = k / kw, w = k % kw;
+    const int s = (tid >> 4) + 16 * q;
     const int64_t seq = s < TI_ ? i0 + s : j0 + (s - TI_);
-    const bool ok = k < kw * (TI_ + TJ) && seq < A.n;
+    const bool ok = s < TI_ + TJ && w < kw && seq < A.n;
     r.v[q] = ok ? A.planes[seq * A.W + w0 + w] : make_uint4(0, 0, 0, 0);
     if constexpr (NP) r.nm[q] = ok ? A.nmask[seq * A.W + w0 + w] : 0u;
   }
@@ -838,11 +848,12 @@ __device__ __attribute__((always_inline)) inline void stage_store(LDS &lds, uint
                                                                   uint32_t (*sNJ)[TJ + 1], int kw, int tid,
                                                                   const StageRegs<NP, TI_> &r)
 {
+  (void)kw;
+  const int w = tid & (KW - 1);
 #pragma unroll
   for (int q = 0; q < spt<TI_>(); q++) {
-    const int k = tid + 256 * q;
-    if (k >= kw * (TI_ + TJ)) break;
-    const int s = k / kw, w = k % kw;
+    const int s = (tid >> 4) + 16 * q;
+    if (s >= TI_ + TJ) break;
     if (s < TI_) {
       lds.sI[w][s] = r.v[q];
       if constexpr (NP) sNI[w][s] = r.nm[q];
