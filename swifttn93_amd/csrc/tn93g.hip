@@ -277,6 +277,7 @@ struct RunArgs {
   const int32_t *grun_off; // [n+1] and the gap runs [start, end) of each sequence
   const uint2 *gruns;
   const int64_t *weights;
+  int unitw;                 // every copy number of the set is 1: pair weights are 1 without loading them
   int64_t n, Ls, npad, W, W64;
   int64_t kt;                // constant columns folded out of the set (none of them is in W), kc[b] of base b:
   uint32_t kc[4];            // each adds one A/A, C/C, G/G or T/T site to every pair
@@ -390,7 +391,7 @@ __device__ void finish_pair(const RunArgs &A, int64_t i, int64_t j, const tn93g_
     }
     return;
   }
-  const int64_t w = A.weights[i] * A.weights[j];
+  const int64_t w = A.unitw ? 1 : A.weights[i] * A.weights[j];
   const int bin = fst_bin(A, i, j);
   bool decided = true;
   // histogram bin: floor(200 d) must agree over the enclosure (or clamp to 199 on the same side)
@@ -447,7 +448,7 @@ __device__ void skip_pair(const RunArgs &A, int64_t i, int64_t j, RowPart &rp)
     rp.skip += 1;
     if (link) {
       append_link(A, i, j, -1.0, -1.0, -1.0);
-      rp.linkw += A.weights[i] * A.weights[j];
+      rp.linkw += A.unitw ? 1 : A.weights[i] * A.weights[j];
     }
     return;
   }
@@ -1653,14 +1654,17 @@ __device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A
           const uint32_t *d = A.cdata + (size_t)k * A.cstride + lane;
           for (int f = 0; f < 7; f++) q[f] = d[64 * f];
         }
+        // the x16 counts as they are: the tail only uses ratios of its inputs (1 / sum, then products with it), and
+        // scaling every input by 16 scales those intermediates by exact powers of two, so every rounded value after
+        // them, and the distance, are the same bits as with the counts divided by 16
         tn93g_tailin tin;
-        tin.nf[0] = (double)q[0] * 0.0625;
-        tin.nf[1] = (double)q[1] * 0.0625;
-        tin.nf[2] = (double)q[2] * 0.0625;
-        tin.nf[3] = (double)q[3] * 0.0625;
-        tin.diag = (double)q[4] * 0.0625;
-        tin.ag = (double)q[5] * 0.0625;
-        tin.ct = (double)q[6] * 0.0625;
+        tin.nf[0] = (double)q[0];
+        tin.nf[1] = (double)q[1];
+        tin.nf[2] = (double)q[2];
+        tin.nf[3] = (double)q[3];
+        tin.diag = (double)q[4];
+        tin.ag = (double)q[5];
+        tin.ct = (double)q[6];
         finish_pair<true>(A, i, j, tin, lhist, &bs, rp);
       } else if (h.skip >> lane & 1ull) {
         skip_pair(A, i, j, rp);
@@ -2068,30 +2072,66 @@ __device__ void replay_events_dense(const RunArgs &A, int64_t i, int64_t j, cons
   }
 }
 
-// Adds each lane's row part to its row's accumulator, one set of atomics per distinct row of the wave: the lanes of
-// a wave mostly share a few rows, and per-lane atomics on one address serialise.  All 64 lanes must call it; lanes
-// without a part pass row < 0.
-__device__ void rowacc_add_wave(const RunArgs &A, int64_t row, const RowPart &rp)
+// A wave's LDS in the replay: 64 row slots (the wave's row parts, summed per row) or, for long alignments, the 64
+// lanes' clean/clean tables (built by the wave before the walk, read before any slot is used).
+struct RowSlot {
+  double slo, shi, sabs;
+  unsigned long long mlo, mhi;  // bit patterns of max thisD (non-negative doubles order like their bits); 0: none
+  unsigned long long wsum, skip, linkw;
+};
+union ReplayLds {
+  uint32_t scc[64][16];
+  RowSlot slot[64];
+};
+
+// Adds each lane's row part to its row's accumulator, one set of global atomics per distinct row of the wave (lanes
+// of one row hitting one address with global atomics serialise in the L2): a ballot loop gives every distinct row a
+// slot of the wave's LDS, the lanes add their parts into their slots with LDS atomics, and one lane per slot sends
+// the slot on.  All 64 lanes must call it; lanes without a part pass row < 0.
+__device__ void rowacc_add_wave(const RunArgs &A, int64_t row, const RowPart &rp, RowSlot *slot)
 {
   const int lane = threadIdx.x & 63;
-  bool pend = row >= 0;
+  bool pend = row >= 0, lead = false;
+  int my = 0, ns = 0;
   for (;;) {
     const unsigned long long m = __ballot(pend);
     if (!m) break;
     const int leader = __ffsll((long long)m) - 1;
-    const int64_t r = __shfl(row, leader, 64);
-    const bool mine = pend && row == r;
-    RowPart q;
-    q.slo = wave_sum_d(mine ? rp.slo : 0.);
-    q.shi = wave_sum_d(mine ? rp.shi : 0.);
-    q.sabs = wave_sum_d(mine ? rp.sabs : 0.);
-    q.mlo = wave_max_d(mine ? rp.mlo : -1.);
-    q.mhi = wave_max_d(mine ? rp.mhi : -1.);
-    q.wsum = wave_sum_l(mine ? rp.wsum : 0);
-    q.skip = wave_sum_l(mine ? rp.skip : 0);
-    q.linkw = wave_sum_l(mine ? rp.linkw : 0);
-    if (lane == leader) rowacc_add(A, r, q);
-    if (mine) pend = false;
+    const int64_t r = (int64_t)((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(row >> 32), leader) << 32 |
+                                (uint32_t)__builtin_amdgcn_readlane((int)row, leader));
+    if (pend && row == r) {
+      my = ns;
+      lead = lane == leader;
+      pend = false;
+    }
+    ns++;
+  }
+  if (lead) slot[my] = RowSlot{0., 0., 0., 0ull, 0ull, 0ull, 0ull, 0ull};
+  __builtin_amdgcn_wave_barrier();  // LDS operations of one wave complete in order
+  if (row >= 0) {
+    RowSlot *q = slot + my;
+    if (rp.slo != 0.) atomicAdd(&q->slo, rp.slo);
+    if (rp.shi != 0.) atomicAdd(&q->shi, rp.shi);
+    if (rp.sabs != 0.) atomicAdd(&q->sabs, rp.sabs);
+    if (rp.mlo > 0.) atomicMax(&q->mlo, (unsigned long long)tn93g_bits(rp.mlo));
+    if (rp.mhi > 0.) atomicMax(&q->mhi, (unsigned long long)tn93g_bits(rp.mhi));
+    if (rp.wsum) atomicAdd(&q->wsum, (unsigned long long)rp.wsum);
+    if (rp.skip) atomicAdd(&q->skip, (unsigned long long)rp.skip);
+    if (rp.linkw) atomicAdd(&q->linkw, (unsigned long long)rp.linkw);
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (lead) {
+    const RowSlot v = slot[my];
+    RowPart t;
+    t.slo = v.slo;
+    t.shi = v.shi;
+    t.sabs = v.sabs;
+    t.mlo = v.mlo ? tn93g_from_bits(v.mlo) : -1.;
+    t.mhi = v.mhi ? tn93g_from_bits(v.mhi) : -1.;
+    t.wsum = (long long)v.wsum;
+    t.skip = (long long)v.skip;
+    t.linkw = (long long)v.linkw;
+    rowacc_add(A, row, t);
   }
 }
 
@@ -2105,7 +2145,7 @@ __device__ void rowacc_add_wave(const RunArgs &A, int64_t row, const RowPart &rp
 template <bool DENSE>
 __device__ __attribute__((always_inline)) inline void replay_entries(const RunArgs &A, const ReplayEnt *tabs,
                                                                       unsigned long long *lhist, BlockStats *bs,
-                                                                      uint32_t (*scc)[16], int64_t beg, int64_t total,
+                                                                      ReplayLds *wl, int64_t beg, int64_t total,
                                                                       int64_t kb)
 {
   const int lane = threadIdx.x & 63;
@@ -2119,7 +2159,7 @@ __device__ __attribute__((always_inline)) inline void replay_entries(const RunAr
   const bool looked = k < total && cc_table_lookup(A, (int64_t)(pr >> 32 & 0x7fffffffu),
                                                    (int64_t)(pr & 0xffffffffull), icc);
   if (coop)
-    cc_table_wave(A, (int64_t)(pr >> 32 & 0x7fffffffu), (int64_t)(pr & 0xffffffffull), k < total && !looked, scc);
+    cc_table_wave(A, (int64_t)(pr >> 32 & 0x7fffffffu), (int64_t)(pr & 0xffffffffull), k < total && !looked, wl->scc);
   const bool avg_first = pr >> 63;
   pr &= ~(1ull << 63);
   if (k < total) {
@@ -2127,7 +2167,7 @@ __device__ __attribute__((always_inline)) inline void replay_entries(const RunAr
     if (looked) {
     } else if (coop) {
 #pragma unroll
-      for (int q = 0; q < 16; q++) icc[q] = scc[lane][q];
+      for (int q = 0; q < 16; q++) icc[q] = wl->scc[lane][q];
     } else {
       cc_table(A, i, j, icc);
     }
@@ -2161,7 +2201,7 @@ __device__ __attribute__((always_inline)) inline void replay_entries(const RunAr
       A.fast ? finish_pair<true>(A, i, j, tin, lhist, bs, rp) : finish_pair<false>(A, i, j, tin, lhist, bs, rp);
     row = i;
   }
-  if (A.fast) rowacc_add_wave(A, row, rp);
+  if (A.fast) rowacc_add_wave(A, row, rp, wl->slot);
 }
 
 template <bool DENSE>
@@ -2170,7 +2210,7 @@ __global__ __launch_bounds__(RB, 4) void k_replay(RunArgs A)
   __shared__ ReplayEnt tabs[2 * NCODE * NCODE];
   __shared__ unsigned long long lhist[3 * 200];
   __shared__ BlockStats bs;
-  __shared__ uint32_t scc[DENSE ? RB : 1][16];  // clean/clean tables built by the wave (long alignments)
+  __shared__ ReplayLds wl[RB / 64];
   const int tid = threadIdx.x, wave = tid >> 6;
   const int64_t total = (int64_t)*A.slow_total;
   if ((int64_t)blockIdx.x * RB >= total) return;  // no pair for this block (uniform): a run without replays is cheap
@@ -2180,7 +2220,7 @@ __global__ __launch_bounds__(RB, 4) void k_replay(RunArgs A)
   __syncthreads();
   const int64_t nw = (int64_t)gridDim.x * (RB / 64);
   for (int64_t kb = ((int64_t)blockIdx.x * (RB / 64) + wave) * 64; kb < total; kb += nw * 64)
-    replay_entries<DENSE>(A, tabs, lhist, &bs, scc + wave * 64, 0, total, kb);
+    replay_entries<DENSE>(A, tabs, lhist, &bs, wl + wave, 0, total, kb);
   flush_block(A, &bs, lhist, tid, RB);
 }
 
@@ -2194,7 +2234,7 @@ __global__ __launch_bounds__(RB, 4) void k_replay_multi(const RunArgs *__restric
   __shared__ ReplayEnt tabs[2 * NCODE * NCODE];
   __shared__ unsigned long long lhist[3 * 200];
   __shared__ BlockStats bs;
-  __shared__ uint32_t scc[DENSE ? RB : 1][16];
+  __shared__ ReplayLds wl[RB / 64];
   const int tid = threadIdx.x, wave = tid >> 6;
   const int64_t G = gridDim.x, b = blockIdx.x;
   int64_t cum = 0;  // units of the jobs before q (uniform)
@@ -2211,7 +2251,7 @@ __global__ __launch_bounds__(RB, 4) void k_replay_multi(const RunArgs *__restric
     if (tid == 0) bs.unc = bs.undecided = 0;
     __syncthreads();
     for (int64_t u = u0; u < cum; u += G)
-      replay_entries<DENSE>(A, tabs, lhist, &bs, scc + wave * 64, 0, total, (u - (cum - units)) * RB + wave * 64);
+      replay_entries<DENSE>(A, tabs, lhist, &bs, wl + wave, 0, total, (u - (cum - units)) * RB + wave * 64);
     flush_block(A, &bs, lhist, tid, RB);
   }
 }
@@ -2611,6 +2651,7 @@ struct tn93g_set {
   int64_t ev_total[4] = {0, 0, 0, 0}, ev_cap[4] = {0, 0, 0, 0};  // entries used / allocated: ev, evg, nruns, gruns
   bool decided = false;                                  // ndense fixed by the first append
   std::vector<int64_t> hw;                            // copy numbers (host), cap entries
+  int64_t nonunit = 0;                                // sequences whose copy number is not 1
   std::vector<uint8_t> spec;                          // 1: the sequence has a B, D, H or V code (host), cap entries
   std::vector<void *> retired;                        // replaced event arrays
   void *listcnt = nullptr;                            // the list lengths of an append (5 x n int32), grown on demand
@@ -2884,9 +2925,11 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
   tn93g_set *S = const_cast<tn93g_set *>(J.S);
   const tn93g_params *P = &J.P;
   SetDev d;
+  int64_t nonunit;
   {
     std::lock_guard<std::mutex> lk(S->mu);
     d = S->d;
+    nonunit = S->nonunit;  // counts later appends too: only ever conservative
   }
   RunBufs *B = J.B;
   if (P->mode < 0 || P->mode > 4 || P->kind < 0 || P->kind > 3) return c->fail(TN93G_E_ARG, "bad mode or kind");
@@ -2920,6 +2963,7 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
   // events and cost nothing extra.
   J.sparse_gaps = P->mode == TN93G_GAPMM || 210 * d.total_gruns < d.W * std::max<int64_t>(d.n, 1);
   A.weights = d.weights;
+  A.unitw = nonunit == 0 ? 1 : 0;
   A.n = d.n;
   A.Ls = d.Ls;
   A.npad = d.npad;
@@ -3977,7 +4021,10 @@ static int set_append(tn93g_set *S, const uint8_t *codes, int64_t n, const int64
     for (int li = 0; li < 4; li++) S->ev_total[li] = totals[li];
   if (rc) return rc;
   HIPCHK(c, hipGetLastError());
-  for (int64_t s = 0; s < n; s++) S->hw[(size_t)(n0 + s)] = wts[(size_t)s];
+  for (int64_t s = 0; s < n; s++) {
+    S->hw[(size_t)(n0 + s)] = wts[(size_t)s];
+    S->nonunit += wts[(size_t)s] != 1 ? 1 : 0;
+  }
   d.total_events = S->ev_total[0];
   d.total_gruns = S->ev_total[3];
   d.n = n1;  // published: runs that start from now on see the new sequences
