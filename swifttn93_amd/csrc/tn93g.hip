@@ -377,11 +377,13 @@ __device__ inline void flush_block(const RunArgs &A, BlockStats *bs, unsigned lo
 // swifttn93.cpp:2547-2603).  Fast path: the pair's share of its row's reductions in rp; otherwise D/U for k_rows.
 // FASTLOG (fast path only): the tail with the enclosed fast logarithm, every output certified from [dlo, dhi];
 // otherwise the double-double logarithm, exact unless a log lies near a rounding midpoint.
+// ltab: the logarithm table, or a copy of it in the block's LDS.
 template <bool FASTLOG>
 __device__ void finish_pair(const RunArgs &A, int64_t i, int64_t j, const tn93g_tailin &tin,
-                            unsigned long long *lhist, BlockStats *bs, RowPart &rp)
+                            unsigned long long *lhist, BlockStats *bs, RowPart &rp,
+                            const double (*ltab)[4] = tn93g_logtab)
 {
-  tn93g_tailout t = FASTLOG ? tn93g_tail_fast(tin) : tn93g_tail(tin, A.window);
+  tn93g_tailout t = FASTLOG ? tn93g_tail_fast(tin, ltab) : tn93g_tail(tin, A.window);
   if (A.exact && t.unc) {
     unsigned k = atomicAdd(A.exact_cnt, 1u);
     if (k < A.exact_cap) {
@@ -1605,8 +1607,10 @@ __device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A
   __shared__ unsigned long long lhist[3 * 200];
   __shared__ BlockStats bs;
   __shared__ RowPart wpart[4];
+  __shared__ double sLog[TN93G_LOGTAB_N + 1][4];  // the logarithm's table: LDS look-ups instead of memory ones
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   for (int k = tid; k < 600; k += 256) lhist[k] = 0;
+  for (int k = tid; k < (TN93G_LOGTAB_N + 1) * 4; k += 256) sLog[k >> 2][k & 3] = tn93g_logtab[k >> 2][k & 3];
   if (tid == 0) bs.unc = bs.undecided = 0;
   __syncthreads();
   const int64_t nrows = A.r1 - A.r0;
@@ -1665,7 +1669,7 @@ __device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A
         tin.diag = (double)q[4];
         tin.ag = (double)q[5];
         tin.ct = (double)q[6];
-        finish_pair<true>(A, i, j, tin, lhist, &bs, rp);
+        finish_pair<true>(A, i, j, tin, lhist, &bs, rp, sLog);
       } else if (h.skip >> lane & 1ull) {
         skip_pair(A, i, j, rp);
       }

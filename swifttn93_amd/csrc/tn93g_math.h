@@ -160,8 +160,8 @@ TN93G_FN double tn93g_log1p_small(double f)
   return f + (f * f) * q;
 }
 
-// x > 0, finite.
-TN93G_FN tn93g_logenc tn93g_log_fast(double x)
+// x > 0, finite.  `tab` is tn93g_logtab or a copy of it (in LDS, where the table look-ups are short).
+TN93G_FN tn93g_logenc tn93g_log_fast(double x, const double (*tab)[4] = tn93g_logtab)
 {
   double v;
   const double f = x - 1.0;
@@ -179,12 +179,12 @@ TN93G_FN tn93g_logenc tn93g_log_fast(double x)
     const double m = tn93g_from_bits((u & 0x000FFFFFFFFFFFFFULL) | 0x3FF0000000000000ULL);  // [1, 2)
     const int j = (int)((m - 1.0) * 256.0 + 0.5);                                         // 0..256
     const double F = 1.0 + (double)j * (1.0 / 256.0);
-    const double r = (m - F) * tn93g_logtab[j][2];  // m - F exact
+    const double r = (m - F) * tab[j][2];  // m - F exact
     const double l1 = tn93g_log1p_small(r);
     const double ed = (double)e;
     const double a0 = ed * TN93G_LN2_HI;
-    const double c = (TN93G_FMA(ed, TN93G_LN2_HI, -a0) + ed * TN93G_LN2_LO) + tn93g_logtab[j][1];
-    const double S = a0 + tn93g_logtab[j][0];
+    const double c = (TN93G_FMA(ed, TN93G_LN2_HI, -a0) + ed * TN93G_LN2_LO) + tab[j][1];
+    const double S = a0 + tab[j][0];
     v = S + (l1 + c);
   }
   tn93g_logenc R;
@@ -274,26 +274,65 @@ TN93G_FN tn93g_tailout tn93g_tail(const tn93g_tailin &in, double window)
   return o;
 }
 
+// Quotients by one divisor that share its reciprocal.  On the GPU an IEEE division n / d compiles to v_div_scale_f64
+// (for d, then for n), v_rcp_f64 of the scaled d, two Newton steps on that reciprocal r (e = 1 - d r, r += r e),
+// q = n r, the remainder n - d q by one FMA, v_div_fmas_f64 (q + rem r, rescaled when a scale was applied) and
+// v_div_fixup_f64 (zero, infinite or NaN operands, overflow).  The scales are applied only when an operand is near
+// the ends of the double range (the numerator below 2^-969, the quotient within 2^768 of overflow, a subnormal
+// denominator or reciprocal), and the fixup changes nothing for finite non-zero d and a finite normal quotient or a
+// zero numerator (then both give +0 for n = +0, d > 0).  The divisions of tn93's tail have d between 2^-64 and 2^32
+// and n zero or between 2^-128 and 2^33, so the same steps without scale and fixup give the same bits, and r is
+// computed once per divisor.  The host build divides.
+struct tn93g_recip {
+  double d, r;
+};
+TN93G_FN tn93g_recip tn93g_recip_of(double d)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  double r = __builtin_amdgcn_rcp(d);
+  double e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-d, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  return tn93g_recip{d, r};
+#else
+  return tn93g_recip{d, 0.0};
+#endif
+}
+TN93G_FN double tn93g_div(double n, const tn93g_recip &R)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double q = n * R.r;
+  const double e = __builtin_fma(-R.d, q, n);
+  return __builtin_fma(e, R.r, q);
+#else
+  return n / R.d;
+#endif
+}
+
 // tn93g_tail with tn93g_log_fast: the same operations; d from the logs' estimates, [dlo, dhi] from their
 // enclosures (d is non-increasing in each log whose coefficient is positive, and every IEEE operation rounds
 // monotonically), so [dlo, dhi] holds tn93's distance.  unc is set whenever the enclosure is not a single value.
-TN93G_FN tn93g_tailout tn93g_tail_fast(const tn93g_tailin &in)
+TN93G_FN tn93g_tailout tn93g_tail_fast(const tn93g_tailin &in, const double (*tab)[4] = tn93g_logtab)
 {
   tn93g_tailout o;
   const double *nucFreq = in.nf;
   double auxd, fY, fR, K1 = 0, K2 = 0, K3 = 0, AG, CT, ti, tv, tng2, nucF[4];
   bool useK2P = false;
-  auxd = 1. / (nucFreq[0] + nucFreq[1] + nucFreq[2] + nucFreq[3]);
+  auxd = tn93g_div(1., tn93g_recip_of(nucFreq[0] + nucFreq[1] + nucFreq[2] + nucFreq[3]));
   tng2 = 2. * auxd;
   for (int k = 0; k < 4; k++) nucF[k] = nucFreq[k] * auxd;
   fR = nucF[0] + nucF[2];
   fY = nucF[1] + nucF[3];
-  if (nucFreq[0] == 0 || nucFreq[1] == 0 || nucFreq[2] == 0 || nucFreq[3] == 0) {
+  const bool k2p = nucFreq[0] == 0 || nucFreq[1] == 0 || nucFreq[2] == 0 || nucFreq[3] == 0;
+  // fR, fY > 0 outside K2P; in K2P they are not divided by
+  const tn93g_recip rR = tn93g_recip_of(k2p ? 1. : fR), rY = tn93g_recip_of(k2p ? 1. : fY);
+  if (k2p) {
     useK2P = true;
   } else {
-    K1 = 2. * nucF[0] * nucF[2] / fR;
-    K2 = 2. * nucF[1] * nucF[3] / fY;
-    K3 = 2. * (fR * fY - nucF[0] * nucF[2] * fY / fR - nucF[1] * nucF[3] * fR / fY);
+    K1 = tn93g_div(2. * nucF[0] * nucF[2], rR);
+    K2 = tn93g_div(2. * nucF[1] * nucF[3], rY);
+    K3 = 2. * (fR * fY - tn93g_div(nucF[0] * nucF[2] * fY, rR) - tn93g_div(nucF[1] * nucF[3] * fR, rY));
   }
   AG = in.ag * tng2;
   CT = in.ct * tng2;
@@ -303,18 +342,20 @@ TN93G_FN tn93g_tailout tn93g_tail_fast(const tn93g_tailin &in)
     AG = 1. - 2. * ti - tv;
     CT = 1. - 2. * tv;
     if (AG > 0. && CT > 0.) {
-      const tn93g_logenc la = tn93g_log_fast(AG), lc = tn93g_log_fast(CT);
+      const tn93g_logenc la = tn93g_log_fast(AG, tab), lc = tn93g_log_fast(CT, tab);
       o.d = -0.5 * la.v - 0.25 * lc.v;
       o.dlo = -0.5 * la.hi - 0.25 * lc.hi;
       o.dhi = -0.5 * la.lo - 0.25 * lc.lo;
     } else
       o.d = o.dlo = o.dhi = 1000.0;
   } else {
-    AG = 1. - AG / K1 - 0.5 * tv / fR;
-    CT = 1. - CT / K2 - 0.5 * tv / fY;
-    tv = 1. - 0.5 * tv / fY / fR;
+    const double h = 0.5 * tv;
+    const double hY = tn93g_div(h, rY);
+    AG = 1. - tn93g_div(AG, tn93g_recip_of(K1)) - tn93g_div(h, rR);
+    CT = 1. - tn93g_div(CT, tn93g_recip_of(K2)) - hY;
+    tv = 1. - tn93g_div(hY, rR);
     if (AG > 0. && CT > 0. && tv > 0) {
-      const tn93g_logenc la = tn93g_log_fast(AG), lc = tn93g_log_fast(CT), lt = tn93g_log_fast(tv);
+      const tn93g_logenc la = tn93g_log_fast(AG, tab), lc = tn93g_log_fast(CT, tab), lt = tn93g_log_fast(tv, tab);
       o.d = -K1 * la.v - K2 * lc.v - K3 * lt.v;
       o.dlo = -K1 * la.hi - K2 * lc.hi - K3 * (K3 > 0 ? lt.hi : lt.lo);
       o.dhi = -K1 * la.lo - K2 * lc.lo - K3 * (K3 > 0 ? lt.lo : lt.hi);
