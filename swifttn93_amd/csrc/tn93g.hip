@@ -764,8 +764,11 @@ __device__ inline uint32_t n_prefix(const uint32_t *__restrict__ npre, const uin
 
 // event accumulators: packed 16-bit fields plus 32-bit totals (flushed before a 16-bit field can overflow)
 constexpr int EVFLUSH = 256;  // entries summed in the packed words between flushes
+// The four packed words are added as two 64-bit words (one v_lshl_add_u64 each): no field overflows between flushes,
+// so no carry crosses from one field (or word) into the next.
 struct EvAcc {
-  uint32_t px, py, pz, pw, fl;  // fl: non-zero once an order-dependent entry was added
+  unsigned long long pxy, pzw;  // px | py << 32, pz | pw << 32
+  uint32_t fl;                  // non-zero once an order-dependent entry was added
 };
 struct Tot {
   uint32_t nf[4], dg, ag, ct, amb;
@@ -773,23 +776,23 @@ struct Tot {
 
 __device__ inline void ev_add(EvAcc &e, const uint4 t)
 {
-  e.px += t.x;
-  e.py += t.y;
-  e.pz += t.z;
-  e.pw += t.w;
+  e.pxy += (unsigned long long)t.x | (unsigned long long)t.y << 32;
+  e.pzw += (unsigned long long)t.z | (unsigned long long)t.w << 32;
 }
 __device__ inline void ev_flush(EvAcc &e, Tot &o)
 {
-  o.nf[0] += e.px & 0xffffu;
-  o.nf[1] += e.px >> 16;
-  o.nf[2] += e.py & 0xffffu;
-  o.nf[3] += e.py >> 16;
-  o.dg += e.pz & 0xffffu;
-  o.ag += e.pz >> 16;
-  o.ct += e.pw & 0x3fffu;
-  o.amb += (e.pw >> 14) & 0x1ffu;
-  e.fl |= e.pw >> 23;
-  e.px = e.py = e.pz = e.pw = 0;
+  const uint32_t px = (uint32_t)e.pxy, py = (uint32_t)(e.pxy >> 32), pz = (uint32_t)e.pzw,
+                 pw = (uint32_t)(e.pzw >> 32);
+  o.nf[0] += px & 0xffffu;
+  o.nf[1] += px >> 16;
+  o.nf[2] += py & 0xffffu;
+  o.nf[3] += py >> 16;
+  o.dg += pz & 0xffffu;
+  o.ag += pz >> 16;
+  o.ct += pw & 0x3fffu;
+  o.amb += (pw >> 14) & 0x1ffu;
+  e.fl |= pw >> 23;
+  e.pxy = e.pzw = 0;
 }
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -1205,7 +1208,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
   for (int a = 0; a < RP; a++) {
     cnt[a] = CntT{};
     tot[a] = Tot{{0, 0, 0, 0}, 0, 0, 0, 0};
-    acc[a] = EvAcc{0, 0, 0, 0, 0};
+    acc[a] = EvAcc{0, 0, 0};
   }
   if constexpr (NP)
 #pragma unroll
@@ -1456,7 +1459,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
     for (int a = 0; a < RP; a++)
       if (avg[a]) {
         tot[a] = Tot{{0, 0, 0, 0}, 0, 0, 0, 0};
-        acc[a] = EvAcc{0, 0, 0, 0, 0};
+        acc[a] = EvAcc{0, 0, 0};
       }
     if (!te.single()) {  // the tile's first chunk again
       te.first();
@@ -1602,61 +1605,63 @@ __global__ __launch_bounds__(256) void k_pairs_multi(const RunArgs *__restrict__
 // The FP64 tail of the fast path.  A block takes whole rows; its 4 waves take the row's chunks (64 columns each) in
 // turn, at high occupancy so the long dependent chains of divisions and logarithms of many pairs overlap.  Each lane
 // keeps its share of the row's reductions over the chunks; the block combines them into one set of atomics per row.
-__device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A, int64_t rbeg, int64_t rstep)
+// C16: the chunks hold 16-bit packed inputs (4 words per lane), else 7 words.  A chunk's header and inputs are loaded
+// one chunk ahead and no other load sits between, so the wait for them never covers the next chunk's loads.
+struct TailLds {
+  unsigned long long lhist[3 * 200];
+  BlockStats bs;
+  RowPart wpart[4];
+  double log[TN93G_LOGTAB_N + 1][4];  // the logarithm's table: LDS look-ups instead of memory ones
+};
+template <bool C16>
+__device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A, int64_t rbeg, int64_t rstep,
+                                                                TailLds &S)
 {
-  __shared__ unsigned long long lhist[3 * 200];
-  __shared__ BlockStats bs;
-  __shared__ RowPart wpart[4];
-  __shared__ double sLog[TN93G_LOGTAB_N + 1][4];  // the logarithm's table: LDS look-ups instead of memory ones
+  constexpr int NQ = C16 ? 4 : 7;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  for (int k = tid; k < 600; k += 256) lhist[k] = 0;
-  for (int k = tid; k < (TN93G_LOGTAB_N + 1) * 4; k += 256) sLog[k >> 2][k & 3] = tn93g_logtab[k >> 2][k & 3];
-  if (tid == 0) bs.unc = bs.undecided = 0;
+  for (int k = tid; k < 600; k += 256) S.lhist[k] = 0;
+  for (int k = tid; k < (TN93G_LOGTAB_N + 1) * 4; k += 256) S.log[k >> 2][k & 3] = tn93g_logtab[k >> 2][k & 3];
+  if (tid == 0) S.bs.unc = S.bs.undecided = 0;
   __syncthreads();
   const int64_t nrows = A.r1 - A.r0;
   const int64_t c0a = A.c0 & ~(int64_t)63;
   for (int64_t r = rbeg; r < nrows; r += rstep) {
     const int64_t i = A.r0 + r;
     RowPart rp = rowpart_zero();
-    // the wave's chunks of the row, bx = wave, wave + 4, ...: the next chunk's header and (16-bit packed) inputs are
-    // loaded before the current one is computed, so the loads of one chunk overlap the FP64 work of the previous
+    // the wave's chunks of the row, bx = wave, wave + 4, ...
     const int64_t tri = A.r0 + (r / A.ti) * A.ti;  // first row of the pair kernel's tile row
     auto live = [&](int64_t bx) { return bx < A.gx && !(A.triangle && c0a + bx * TJ + TJ - 1 <= tri); };
+    auto load = [&](int64_t bx, ChunkHdr &h, uint32_t (&v)[NQ]) {
+      const int64_t k = r * A.gx + bx;
+      h = A.chdr[k];
+      const uint32_t *d = A.cdata + (size_t)k * A.cstride + lane;
+#pragma unroll
+      for (int f = 0; f < NQ; f++) v[f] = d[64 * f];
+    };
     int64_t bx = wave;
     while (bx < A.gx && !live(bx)) bx += 4;
     ChunkHdr h{0, 0};
-    uint32_t p4[4] = {0, 0, 0, 0};
-    if (bx < A.gx) {
-      const int64_t k = r * A.gx + bx;
-      h = A.chdr[k];
-      if (A.c16) {
-        const uint32_t *d = A.cdata + (size_t)k * A.cstride + lane;
-        p4[0] = d[0], p4[1] = d[64], p4[2] = d[128], p4[3] = d[192];
-      }
-    }
+    uint32_t p[NQ];
+#pragma unroll
+    for (int f = 0; f < NQ; f++) p[f] = 0;
+    if (bx < A.gx) load(bx, h, p);
     while (bx < A.gx) {
       int64_t nb = bx + 4;
       while (nb < A.gx && !live(nb)) nb += 4;
       ChunkHdr hn{0, 0};
-      uint32_t n4[4] = {0, 0, 0, 0};
-      if (nb < A.gx) {
-        const int64_t k = r * A.gx + nb;
-        hn = A.chdr[k];
-        if (A.c16) {
-          const uint32_t *d = A.cdata + (size_t)k * A.cstride + lane;
-          n4[0] = d[0], n4[1] = d[64], n4[2] = d[128], n4[3] = d[192];
-        }
-      }
-      const int64_t k = r * A.gx + bx;
+      uint32_t n[NQ];
+#pragma unroll
+      for (int f = 0; f < NQ; f++) n[f] = 0;
+      if (nb < A.gx) load(nb, hn, n);
       const int64_t j = c0a + bx * TJ + lane;
       if (h.need >> lane & 1ull) {
         uint32_t q[7];
-        if (A.c16) {
-          q[0] = p4[0] & 0xffffu, q[1] = p4[0] >> 16, q[2] = p4[1] & 0xffffu, q[3] = p4[1] >> 16;
-          q[4] = p4[2] & 0xffffu, q[5] = p4[2] >> 16, q[6] = p4[3];
+        if constexpr (C16) {
+          q[0] = p[0] & 0xffffu, q[1] = p[0] >> 16, q[2] = p[1] & 0xffffu, q[3] = p[1] >> 16;
+          q[4] = p[2] & 0xffffu, q[5] = p[2] >> 16, q[6] = p[3];
         } else {
-          const uint32_t *d = A.cdata + (size_t)k * A.cstride + lane;
-          for (int f = 0; f < 7; f++) q[f] = d[64 * f];
+#pragma unroll
+          for (int f = 0; f < 7; f++) q[f] = p[f];
         }
         // the x16 counts as they are: the tail only uses ratios of its inputs (1 / sum, then products with it), and
         // scaling every input by 16 scales those intermediates by exact powers of two, so every rounded value after
@@ -1669,14 +1674,14 @@ __device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A
         tin.diag = (double)q[4];
         tin.ag = (double)q[5];
         tin.ct = (double)q[6];
-        finish_pair<true>(A, i, j, tin, lhist, &bs, rp, sLog);
+        finish_pair<true>(A, i, j, tin, S.lhist, &S.bs, rp, S.log);
       } else if (h.skip >> lane & 1ull) {
         skip_pair(A, i, j, rp);
       }
       bx = nb;
       h = hn;
 #pragma unroll
-      for (int f = 0; f < 4; f++) p4[f] = n4[f];
+      for (int f = 0; f < NQ; f++) p[f] = n[f];
     }
     RowPart q;
     q.slo = wave_sum_d(rp.slo);
@@ -1687,28 +1692,33 @@ __device__ __attribute__((always_inline)) inline void tail_rows(const RunArgs &A
     q.wsum = wave_sum_l(rp.wsum);
     q.skip = wave_sum_l(rp.skip);
     q.linkw = wave_sum_l(rp.linkw);
-    if (lane == 0) wpart[wave] = q;
+    if (lane == 0) S.wpart[wave] = q;
     __syncthreads();
     if (tid == 0) {
-      RowPart s = wpart[0];
+      RowPart s = S.wpart[0];
       for (int w = 1; w < 4; w++) {
-        s.slo += wpart[w].slo;
-        s.shi += wpart[w].shi;
-        s.sabs += wpart[w].sabs;
-        s.mlo = fmax(s.mlo, wpart[w].mlo);
-        s.mhi = fmax(s.mhi, wpart[w].mhi);
-        s.wsum += wpart[w].wsum;
-        s.skip += wpart[w].skip;
-        s.linkw += wpart[w].linkw;
+        s.slo += S.wpart[w].slo;
+        s.shi += S.wpart[w].shi;
+        s.sabs += S.wpart[w].sabs;
+        s.mlo = fmax(s.mlo, S.wpart[w].mlo);
+        s.mhi = fmax(s.mhi, S.wpart[w].mhi);
+        s.wsum += S.wpart[w].wsum;
+        s.skip += S.wpart[w].skip;
+        s.linkw += S.wpart[w].linkw;
       }
       rowacc_add(A, i, s);
     }
     __syncthreads();
   }
-  flush_block(A, &bs, lhist, tid, 256);
+  flush_block(A, &S.bs, S.lhist, tid, 256);
 }
 
-__global__ __launch_bounds__(256) void k_tail(RunArgs A) { tail_rows(A, blockIdx.x, gridDim.x); }
+template <bool C16>
+__global__ __launch_bounds__(256, 4) void k_tail(RunArgs A)
+{
+  __shared__ TailLds S;
+  tail_rows<C16>(A, blockIdx.x, gridDim.x, S);
+}
 
 // The block of flat index b in a list of jobs: the largest k with off[k] <= b (uniform over the block).
 __device__ inline int job_of(const int64_t *__restrict__ off, int nj, int64_t b)
@@ -1726,11 +1736,15 @@ __device__ inline int job_of(const int64_t *__restrict__ off, int nj, int64_t b)
 
 // The tails of several jobs (a batch) in one launch: job k owns blocks [boff[k], boff[k + 1]), which stride over
 // its rows as k_tail's grid does.
-__global__ __launch_bounds__(256) void k_tail_multi(const RunArgs *__restrict__ jobs, const int64_t *__restrict__ boff,
+__global__ __launch_bounds__(256, 4) void k_tail_multi(const RunArgs *__restrict__ jobs, const int64_t *__restrict__ boff,
                                                     int nj)
 {
+  __shared__ TailLds S;
   const int k = job_of(boff, nj, blockIdx.x);
-  tail_rows(jobs[k], (int64_t)blockIdx.x - boff[k], boff[k + 1] - boff[k]);
+  if (jobs[k].c16)
+    tail_rows<true>(jobs[k], (int64_t)blockIdx.x - boff[k], boff[k + 1] - boff[k], S);
+  else
+    tail_rows<false>(jobs[k], (int64_t)blockIdx.x - boff[k], boff[k + 1] - boff[k], S);
 }
 
 // ------------------------------------------------------------------------------------------ replay kernel
@@ -1936,35 +1950,52 @@ __device__ inline bool cc_table_lookup(const RunArgs &A, int64_t i, int64_t j, u
   return false;
 }
 
+// One sequence's event list read in site order for the replay, with the partner's code at each site.  The loads run
+// ahead of their use: the entry two places on and the partner's code one place on are in flight while the current
+// site is applied, so no step waits for a load issued in the same step.  Past the end the entries read NONE, whose
+// site (NONE >> 5) lies after every site of an alignment (L < 2^26).
+struct EvStream {
+  static constexpr uint32_t NONE = 0xffffffffu;
+  uint32_t w0, w1, w2;  // entries k, k + 1, k + 2 (site << 5 | code)
+  uint32_t c0, c1;      // the partner's codes at the sites of w0 and w1
+  int k, e;
+  __device__ inline void init(const uint32_t *ev, int k_, int e_, const uint8_t *partner)
+  {
+    k = k_;
+    e = e_;
+    w0 = k < e ? ev[k] : NONE;
+    w1 = k + 1 < e ? ev[k + 1] : NONE;
+    w2 = k + 2 < e ? ev[k + 2] : NONE;
+    c0 = w0 != NONE ? partner[w0 >> 5] : 0u;
+    c1 = w1 != NONE ? partner[w1 >> 5] : 0u;
+  }
+  __device__ inline void next(const uint32_t *ev, const uint8_t *partner)
+  {
+    w0 = w1;
+    c0 = c1;
+    w1 = w2;
+    c1 = w1 != NONE ? partner[w1 >> 5] : 0u;
+    k++;
+    w2 = k + 2 < e ? ev[k + 2] : NONE;
+  }
+};
+
 // The sites where either sequence is ambiguous (under GAPMM also gapped: its lists hold the gap sites), in
-// increasing site order: the two sorted event lists merged; a site in one list only takes its partner's code from
-// the codes array.
+// increasing site order: the two sorted event lists merged.  A site in one list only takes the partner's code from
+// the codes array; at a site in both lists that array holds the partner's own event code, so either list's entry
+// gives both codes.
 __device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const ReplayEnt *tab, ReplayState &s)
 {
-  int a = A.ev_off[i], ae = A.ev_off[i + 1], b = A.ev_off[j], be = A.ev_off[j + 1];
   const uint8_t *si = A.codes + i * A.Ls, *sj = A.codes + j * A.Ls;
-  uint32_t ua = a < ae ? A.ev[a] : 0xFFFFFFFFu, ub = b < be ? A.ev[b] : 0xFFFFFFFFu;
-  while (a < ae || b < be) {
-    const uint32_t pa = ua >> 5, pb = ub >> 5;
-    unsigned c1, c2;
-    if (pa == pb) {
-      c1 = ua & 31u;
-      c2 = ub & 31u;
-      a++;
-      b++;
-      ua = a < ae ? A.ev[a] : 0xFFFFFFFFu;
-      ub = b < be ? A.ev[b] : 0xFFFFFFFFu;
-    } else if (pa < pb) {
-      c1 = ua & 31u;
-      c2 = sj[pa];
-      a++;
-      ua = a < ae ? A.ev[a] : 0xFFFFFFFFu;
-    } else {
-      c1 = si[pb];
-      c2 = ub & 31u;
-      b++;
-      ub = b < be ? A.ev[b] : 0xFFFFFFFFu;
-    }
+  EvStream a, b;
+  a.init(A.ev, A.ev_off[i], A.ev_off[i + 1], sj);
+  b.init(A.ev, A.ev_off[j], A.ev_off[j + 1], si);
+  while (a.w0 != EvStream::NONE || b.w0 != EvStream::NONE) {
+    const uint32_t pa = a.w0 >> 5, pb = b.w0 >> 5;
+    const bool ta = pa <= pb, tb = pb <= pa;
+    const unsigned c1 = ta ? (a.w0 & 31u) : b.c0, c2 = ta ? a.c0 : (b.w0 & 31u);
+    if (ta) a.next(A.ev, sj);
+    if (tb) b.next(A.ev, si);
     replay_apply(s, tab[c1 * NCODE + c2]);
   }
 }
@@ -3352,7 +3383,12 @@ static void job_enqueue_post(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, 
   (void)hipEventRecord(c->rfork[branch], st);
   (void)hipStreamWaitEvent(rs, c->rfork[branch], 0);
   job_enqueue_replay(J, st);
-  if (tail_blocks(J)) hipLaunchKernelGGL(k_tail, dim3(tail_blocks(J)), dim3(256), 0, rs, A);
+  if (tail_blocks(J)) {
+    if (A.c16)
+      hipLaunchKernelGGL(k_tail<true>, dim3(tail_blocks(J)), dim3(256), 0, rs, A);
+    else
+      hipLaunchKernelGGL(k_tail<false>, dim3(tail_blocks(J)), dim3(256), 0, rs, A);
+  }
   if (timing) (void)hipEventRecord(c->ev[3], rs);
   (void)hipEventRecord(c->rjoin[branch], rs);
   (void)hipStreamWaitEvent(st, c->rjoin[branch], 0);
