@@ -377,6 +377,22 @@ __device__ inline void flush_block(const RunArgs &A, BlockStats *bs, unsigned lo
 // swifttn93.cpp:2547-2603).  Fast path: the pair's share of its row's reductions in rp; otherwise D/U for k_rows.
 // FASTLOG (fast path only): the tail with the enclosed fast logarithm, every output certified from [dlo, dhi];
 // otherwise the double-double logarithm, exact unless a log lies near a rounding midpoint.
+// Adds w to bin idx of the block's histogram.  With unit weights, the lanes of the wave that fall into the bin of the
+// first active lane add their count with one LDS atomic: distances of closely related sequences all fall into the
+// first bins, and 64 atomics on one LDS address serialise.
+__device__ inline void hist_add(unsigned long long *lhist, int idx, int64_t w, bool unit)
+{
+  if (unit) {
+    const int lead = __builtin_amdgcn_readfirstlane(idx);
+    const unsigned long long m = __ballot(idx == lead);
+    if (idx == lead) {
+      if ((int)(threadIdx.x & 63) == __ffsll((long long)m) - 1) atomicAdd(&lhist[lead], (unsigned long long)__popcll(m));
+      return;
+    }
+  }
+  atomicAdd(&lhist[idx], (unsigned long long)w);
+}
+
 // ltab: the logarithm table, or a copy of it in the block's LDS.
 template <bool FASTLOG>
 __device__ void finish_pair(const RunArgs &A, int64_t i, int64_t j, const tn93g_tailin &tin,
@@ -399,7 +415,7 @@ __device__ void finish_pair(const RunArgs &A, int64_t i, int64_t j, const tn93g_
   // histogram bin: floor(200 d) must agree over the enclosure (or clamp to 199 on the same side)
   const double flo = floor(t.dlo * 200.0), fhi = floor(t.dhi * 200.0);
   if (flo != fhi && !(flo >= 200.0 && fhi >= 200.0) && !(flo < 0.0 && fhi < 0.0)) decided = false;
-  atomicAdd(&lhist[bin * 200 + tn93g_hist_index(t.d)], (unsigned long long)w);
+  hist_add(lhist, bin * 200 + tn93g_hist_index(t.d), w, A.unitw);
   const double x = t.d <= 0. ? 0. : t.d;
   const double xlo = t.dlo <= 0. ? 0. : t.dlo, xhi = t.dhi <= 0. ? 0. : t.dhi;
   const bool cert_in = xlo >= A.mind && xhi <= A.thr;
@@ -2000,45 +2016,26 @@ __device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const Repl
   }
 }
 
-// m additions of v (a power of two, the value of every N-site effect) to x >= 0, one after the other in FP64 as
-// tn93's site loop makes them: exact while the sum stays below the next power of two above x (v is a multiple of
-// the spacing of doubles there), and rounded once, to nearest even, at the addition that crosses it; so the run is
-// applied as one exact addition up to each crossing plus the crossing addition itself.
-__device__ __noinline__ double add_repeat_crossing(double x, double v, uint32_t m)
+// m additions of v to a cell (tn93g_add_repeat); the part past a multiple of v is a call: inlined at every cell it
+// spills the replay's registers.
+__device__ __noinline__ double add_repeat_slow(double x, double v, double inv, uint32_t m)
 {
-  while (m) {
-    if (x == 0.0) {
-      x = v;
-      m--;
-      continue;
-    }
-    const uint64_t e = (tn93g_bits(x) >> 52) & 0x7ffu;
-    const double B = tn93g_from_bits((e + 1) << 52);  // the power of two above x
-    const double steps = ceil((B - x) / v);           // additions that reach B; B - x and the quotient are exact
-    if ((double)m < steps) return x + (double)m * v;  // stays below B: exact
-    const uint32_t t = (uint32_t)steps - 1;
-    x = x + (double)t * v;  // exact, < B
-    x = x + v;              // the crossing addition, rounded like tn93's
-    m -= t + 1;
-  }
-  return x;
+  return tn93g_add_repeat(x, v, inv, m);
 }
-
-// add_repeat(x, v, m) given inv = 1 / v (exact: v is a power of two, so x * inv is x / v exactly).  A multiple of v
-// stays one: every partial sum is exact (the cells hold at most 2^26 sites, far below 2^53 v).
 __device__ inline double add_repeat(double x, double v, double inv, uint32_t m)
 {
-  const double y = x * inv;
-  if (floor(y) == y) return x + (double)m * v;
-  return add_repeat_crossing(x, v, m);
+  if (floor(x * inv) == x * inv) return x + (double)m * v;  // a multiple of v: every partial sum exact
+  return add_repeat_slow(x, v, inv, m);
 }
+
+// 1 / v for a normal power of two v = 2^e: the exponent field 2046 - (e + 1023)
+__device__ inline double pow2_inv(double v) { return tn93g_from_bits((2046ull - ((tn93g_bits(v) >> 52) & 0x7ffu)) << 52); }
 
 __device__ inline void replay_apply_rep(ReplayState &s, const ReplayEnt &E, uint32_t m)
 {
   if (!m) return;
   if (E.mask) {
-    // 1 / v for a normal power of two v = 2^e: the exponent field 2046 - (e + 1023)
-    const double inv = tn93g_from_bits((2046ull - ((tn93g_bits(E.v) >> 52) & 0x7ffu)) << 52);
+    const double inv = pow2_inv(E.v);
 #pragma unroll
     for (int k = 0; k < 16; k++)
       if (E.mask >> k & 1u) s.fc[k] = add_repeat(s.fc[k], E.v, inv, m);
@@ -2046,6 +2043,28 @@ __device__ inline void replay_apply_rep(ReplayState &s, const ReplayEnt &E, uint
 #pragma unroll
   for (int d = 0; d < 4; d++) s.dgi[d] += E.icd == d ? m : 0u;
   s.amb += E.amb * m;
+}
+
+// A segment of sites where one sequence is in an N run and the other holds cnt[x] sites of clean base x.  by_col:
+// the row sequence is in N, and entry (N, x) marks cells of column x only; else the column sequence is, and (x, N)
+// marks cells of row x only (checked when the tables are built).  So each cell takes at most one entry's run, cell
+// (a, b) the run of (N, b) or of (a, N), and the 16 cells are one pass.
+__device__ inline void replay_apply_seg(ReplayState &s, const ReplayEnt *tab, const uint32_t (&cnt)[4], bool by_col)
+{
+#pragma unroll
+  for (int k = 0; k < 16; k++) {
+    const int x = by_col ? (k & 3) : (k >> 2);
+    const ReplayEnt &E = by_col ? tab[NC * NCODE + x] : tab[x * NCODE + NC];
+    const uint32_t m = (E.mask >> k & 1u) ? cnt[x] : 0u;
+    if (m) s.fc[k] = add_repeat(s.fc[k], E.v, pow2_inv(E.v), m);
+  }
+#pragma unroll
+  for (int x = 0; x < 4; x++) {
+    const ReplayEnt &E = by_col ? tab[NC * NCODE + x] : tab[x * NCODE + NC];
+#pragma unroll
+    for (int d = 0; d < 4; d++) s.dgi[d] += E.icd == d ? cnt[x] : 0u;
+    s.amb += E.amb * cnt[x];
+  }
 }
 
 // Counts of A, C, G, T of sequence q over sites [p, e): differences of its plane prefix counts (clean, C|T, G|T, T)
@@ -2087,8 +2106,7 @@ __device__ void replay_events_dense(const RunArgs &A, int64_t i, int64_t j, cons
       } else {
         uint32_t cnt[4];
         base_counts(A, ni ? j : i, p, q, cnt);
-#pragma unroll
-        for (int x = 0; x < 4; x++) replay_apply_rep(s, ni ? tab[NC * NCODE + x] : tab[x * NCODE + NC], cnt[x]);
+        replay_apply_seg(s, tab, cnt, ni);
       }
     }
     p = q;
@@ -3074,6 +3092,15 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
           r.icd = (int16_t)F.icd;
           r.amb = (uint16_t)F.amb;
         }
+      }
+    // the replay applies a run of N sites facing clean bases cell by cell: (N, x) may mark cells of column x only,
+    // (x, N) cells of row x only (true of tn93's handler in every mode)
+    for (int t = 0; t < 2; t++)
+      for (int x = 0; x < 4; x++) {
+        const uint32_t mc = rt[(size_t)(t * NCODE * NCODE + NC * NCODE + x)].mask;
+        const uint32_t mr = rt[(size_t)(t * NCODE * NCODE + x * NCODE + NC)].mask;
+        if ((mc & ~(0x1111u << x)) || (mr & ~(0xfu << (4 * x))))
+          return c->fail(TN93G_E_ARG, "replay table: an N effect outside its row or column");
       }
     int rc;
     if ((rc = need(B->evtab, et.size() * sizeof(uint4))) || (rc = need(B->rtab, rt.size() * sizeof(ReplayEnt))))

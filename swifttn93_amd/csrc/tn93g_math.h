@@ -195,6 +195,33 @@ TN93G_FN tn93g_logenc tn93g_log_fast(double x, const double (*tab)[4] = tn93g_lo
   return R;
 }
 
+// m additions of v (a power of two, the value of every N-site effect of tn93's replay) to x >= 0, one after the
+// other in FP64 as tn93's site loop makes them, given inv = 1 / v (exact).  Closed form: with v <= x, write
+// x = S + f, S the largest multiple of v not above x, 0 <= f < v, f a multiple of g = the spacing of doubles at x.
+// Every addition is exact (v is a multiple of the spacing: the sums stay below 2^27 and v >= 1/16) except the one
+// that reaches a power of two 2^E: that partial sum is exactly 2^E + f (S only takes multiples of v, 2^E is one,
+// and f < v), which rounds f to the doubled spacing, ties to even (2^E itself is an even multiple of it).  So the
+// result is S + m v plus f rounded once for each power of two passed, in order; the powers passed are those above x
+// up to S + m v, because S + m v + f stays below the next one.  The roundings run on F = f / g in integers: rounding
+// to 2^t g looks at bit t - 1 of F (the lower bits are zero by then), and a set bit is a tie, broken towards a clear
+// bit t.  With x < v the first addition can pass several powers of two at once; it is made as tn93 makes it.
+TN93G_FN double tn93g_add_repeat(double x, double v, double inv, uint32_t m)
+{
+  if (!m || floor(x * inv) == x * inv) return x + (double)m * v;  // a multiple of v: every partial sum exact
+  if (x < v) {
+    x = x + v;
+    if (!--m || floor(x * inv) == x * inv) return x + (double)m * v;
+  }
+  const double S = floor(x * inv) * v, f = x - S;  // both exact
+  const double Sm = S + (double)m * v;           // exact: a multiple of v below 2^27
+  const int e0 = (int)((tn93g_bits(x) >> 52) & 0x7ffu), e1 = (int)((tn93g_bits(Sm) >> 52) & 0x7ffu);
+  const double g = tn93g_from_bits((uint64_t)(e0 - 52) << 52);                       // spacing at x (x >= v: normal)
+  uint64_t F = (uint64_t)(f * tn93g_from_bits((uint64_t)(2046 - (e0 - 52)) << 52));  // f / g, exact
+  for (int t = 1; t <= e1 - e0; t++)
+    if (F >> (t - 1) & 1ull) F = (F >> t & 1ull) ? F + (1ull << (t - 1)) : F - (1ull << (t - 1));
+  return Sm + (double)F * g;  // exact: F g is a multiple of the spacing at the result
+}
+
 // Inputs of the tail: tn93's nucFreq[4] and float_counts sums after the merge (refkernel.cpp:714-729).
 struct tn93g_tailin {
   double nf[4];
