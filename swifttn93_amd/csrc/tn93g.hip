@@ -834,7 +834,7 @@ struct PairLds {
 };
 template <int KWORDS, int TI_>
 struct PairLds<KWORDS, false, TI_> {
-  uint4 sI[KWORDS][TI_ + 1];
+  uint4 sI[KWORDS][TI_ + 1];  // the rows (unless they are read with scalar loads)
   uint4 sJ[KWORDS][TJ + 1];
 };
 
@@ -842,49 +842,57 @@ struct PairLds<KWORDS, false, TI_> {
 // chunk are in flight while the current one is counted: SPT values per thread, N-plane words too for dense-N sets.
 // Every sequence has KW slots whatever kw is: thread t takes word t % KW of sequences t / KW + 16 q, so the slot of
 // a thread is fixed for the whole tile (no division by kw) and consecutive threads read consecutive words of one
-// sequence; slots at or past kw load nothing and are never read.
+// sequence; slots at or past kw load nothing and are never read.  SR: only the 64 columns are staged; the rows' words
+// are the same in every lane of a wave and the counting loop reads them with scalar loads, which keeps them off the
+// LDS: the LDS read bandwidth bounds the loop when every lane reads a row word (and its N word) per word and row.
 static_assert(KW == 16, "staging slots: 16 words per sequence, 16 sequences per 256 threads");
-template <int TI_>
-constexpr int spt() { return (KW * (TI_ + TJ) + 255) / 256; }
-template <bool NP, int TI_>
+template <int TI_, bool SR>
+constexpr int spt() { return (KW * ((SR ? 0 : TI_) + TJ) + 255) / 256; }
+template <bool NP, int TI_, bool SR>
 struct StageRegs {
-  uint4 v[spt<TI_>()];
-  uint32_t nm[NP ? spt<TI_>() : 1];
+  uint4 v[spt<TI_, SR>()];
+  uint32_t nm[NP ? spt<TI_, SR>() : 1];
 };
-template <bool NP, int TI_>
+template <bool NP, int TI_, bool SR>
 __device__ __attribute__((always_inline)) inline void stage_load(const RunArgs &A, int64_t i0, int64_t j0, int64_t w0,
-                                                                 int kw, int tid, StageRegs<NP, TI_> &r)
+                                                                 int kw, int tid, StageRegs<NP, TI_, SR> &r)
 {
+  constexpr int R = SR ? 0 : TI_;
   const int w = tid & (KW - 1);
 #pragma unroll
-  for (int q = 0; q < spt<TI_>(); q++) {
+  for (int q = 0; q < spt<TI_, SR>(); q++) {
     const int s = (tid >> 4) + 16 * q;
-    const int64_t seq = s < TI_ ? i0 + s : j0 + (s - TI_);
-    const bool ok = s < TI_ + TJ && w < kw && seq < A.n;
+    const int64_t seq = s < R ? i0 + s : j0 + (s - R);
+    const bool ok = s < R + TJ && w < kw && seq < A.n;
     r.v[q] = ok ? A.planes[seq * A.W + w0 + w] : make_uint4(0, 0, 0, 0);
     if constexpr (NP) r.nm[q] = ok ? A.nmask[seq * A.W + w0 + w] : 0u;
   }
 }
-template <bool NP, int TI_, class LDS>
+template <bool NP, int TI_, bool SR, class LDS>
 __device__ __attribute__((always_inline)) inline void stage_store(LDS &lds, uint32_t (*sNI)[TI_ + 1],
-                                                                  uint32_t (*sNJ)[TJ + 1], int kw, int tid,
-                                                                  const StageRegs<NP, TI_> &r)
+                                                                  uint32_t (*sNJ)[TJ + 1], int tid,
+                                                                  const StageRegs<NP, TI_, SR> &r)
 {
-  (void)kw;
+  constexpr int R = SR ? 0 : TI_;
   const int w = tid & (KW - 1);
 #pragma unroll
-  for (int q = 0; q < spt<TI_>(); q++) {
+  for (int q = 0; q < spt<TI_, SR>(); q++) {
     const int s = (tid >> 4) + 16 * q;
-    if (s >= TI_ + TJ) break;
-    if (s < TI_) {
+    if (s >= R + TJ) break;
+    if (s < R) {
       lds.sI[w][s] = r.v[q];
       if constexpr (NP) sNI[w][s] = r.nm[q];
     } else {
-      lds.sJ[w][s - TI_] = r.v[q];
-      if constexpr (NP) sNJ[w][s - TI_] = r.nm[q];
+      lds.sJ[w][s - R] = r.v[q];
+      if constexpr (NP) sNJ[w][s - R] = r.nm[q];
     }
   }
 }
+
+// Scalar (SMEM) views of the planes and N masks: loads through them with an address that is the same in every lane
+// go to SGPRs, which the VALU reads as operands.  The planes do not change during a run.
+typedef __attribute__((address_space(4))) const uint32_t suint32;
+__device__ inline uint4 sload4(suint32 *p) { return make_uint4(p[0], p[1], p[2], p[3]); }
 
 // Events of a tile.  The rows' and the columns' event lists are each one contiguous range of the set's CSR array:
 // rows [er0, er0 + nr), columns [ec0, ec0 + nc).  They are taken in chunks that fit the LDS: chunk rows [r0, r1) and
@@ -1192,6 +1200,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
   const int64_t i0 = A.r0 + by * TI_, j0 = c0a + bx * TJ;
   if (A.triangle && j0 + TJ - 1 <= i0) return;  // no pair j > i in this tile
   constexpr bool MF = BACKEND == TN93G_BACKEND_MFMA;
+  constexpr bool SR = NP;  // dense-N sets (long N-masked alignments): the rows' words through scalar loads
   __shared__ unsigned lslow;
   __shared__ unsigned long long sbase;
   __shared__ BlockStats bs;
@@ -1201,13 +1210,13 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
   }
   __shared__ TileLds<MF, TI_> L;
   PairLds<KW, MF, TI_> &lds = L.p;
-  __shared__ uint32_t sNI[NP ? KW : 1][TI_ + 1], sNJ[NP ? KW : 1][TJ + 1];
+  __shared__ uint32_t sNI[NP && !SR ? KW : 1][TI_ + 1], sNJ[NP ? KW : 1][TJ + 1];
   __shared__ uint4 sTabR[(NCODE + 1) * TABW], sTabC[(NCODE + 1) * TABW];
   __shared__ unsigned long long lhist[FAST ? 1 : 3 * 200];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  StageRegs<NP, TI_> stg;  // the first chunk's planes, loaded before everything else
+  StageRegs<NP, TI_, SR> stg;  // the first chunk's planes, loaded before everything else
   if constexpr (!MF)
-    if (!A.all_slow) stage_load<NP, TI_>(A, i0, j0, 0, (int)min((int64_t)KW, A.W), tid, stg);
+    if (!A.all_slow) stage_load<NP, TI_, SR>(A, i0, j0, 0, (int)min((int64_t)KW, A.W), tid, stg);
   const uint4 *tabs = A.evtab + (MARG ? 2 * NCODE * NCODE : 0);  // [main, AVERAGE], plain or folded
   load_event_tables(tabs, sTabR, sTabC, tid, A.gapmm, A.nplane);
   if constexpr (!FAST)
@@ -1279,35 +1288,67 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
     }
   }
   if constexpr (!MF) {
-    if (do_fast)
+    if (do_fast) {
+      // SR: the wave's rows through scalar loads, RG words at a time (words past the row's end read the next row or
+      // the arrays' padding and are not used; row indices past the set read its last sequence, whose counts are
+      // never used)
+      constexpr int RG = 2;
+      const int64_t ibs = i0 + (int64_t)__builtin_amdgcn_readfirstlane(wave) * RP;
+      suint32 *rowp[SR ? RP : 1];
+      suint32 *rown[SR && NP ? RP : 1];
+      if constexpr (SR)
+#pragma unroll
+        for (int a = 0; a < RP; a++) {
+          const int64_t ia = min(ibs + a, A.n - 1);
+          rowp[a] = (suint32 *)(A.planes + ia * A.W);
+          if constexpr (NP) rown[a] = (suint32 *)(A.nmask + ia * A.W);
+        }
       for (int64_t w0 = 0; w0 < A.W; w0 += KW) {
         const int kw = (int)min((int64_t)KW, A.W - w0);
         __syncthreads();
-        stage_store<NP, TI_>(lds, sNI, sNJ, kw, tid, stg);
+        stage_store<NP, TI_, SR>(lds, sNI, sNJ, tid, stg);
         __syncthreads();
-        if (w0 + KW < A.W) stage_load<NP, TI_>(A, i0, j0, w0 + KW, (int)min((int64_t)KW, A.W - w0 - KW), tid, stg);
-        for (int k = 0; k < kw; k++) {
-          const uint4 pjv = lds.sJ[k][lane];
+        if (w0 + KW < A.W)
+          stage_load<NP, TI_, SR>(A, i0, j0, w0 + KW, (int)min((int64_t)KW, A.W - w0 - KW), tid, stg);
+        for (int k0 = 0; k0 < kw; k0 += RG) {
+          uint4 rw[RP][RG];
+          uint32_t rn[RP][RG];
 #pragma unroll
-          for (int a = 0; a < RP; a++) count_word(lds.sI[k][wave * RP + a], pjv, cnt[a]);
-          if constexpr (NP) {
-            // the rows' N sites against the column (a uniform branch); the column's N sites against the rows are
-            // counted per N run after the loop
-            const uint32_t nj = sNJ[k][lane];
+          for (int a = 0; a < RP; a++)
 #pragma unroll
-            for (int a = 0; a < RP; a++) {
-              const uint32_t ni = sNI[k][wave * RP + a];  // the same in every lane
-              if (ni) {
-                ncn[a].r[0] += __popc(ni & pjv.x);
-                ncn[a].r[1] += __popc(ni & pjv.y);
-                ncn[a].r[2] += __popc(ni & pjv.z);
-                ncn[a].r[3] += __popc(ni & pjv.w);
-                ncn[a].nn += __popc(ni & nj);
+            for (int q = 0; q < RG; q++) {
+              if constexpr (SR) {
+                rw[a][q] = sload4(rowp[a] + 4 * (w0 + k0 + q));
+                if constexpr (NP) rn[a][q] = rown[a][w0 + k0 + q];
+              }
+            }
+#pragma unroll
+          for (int q = 0; q < RG; q++) {
+            const int k = k0 + q;
+            if (k >= kw) break;
+            const uint4 pjv = lds.sJ[k][lane];
+#pragma unroll
+            for (int a = 0; a < RP; a++) count_word(SR ? rw[a][q] : lds.sI[k][wave * RP + a], pjv, cnt[a]);
+            if constexpr (NP) {
+              // the rows' N sites against the column (a uniform branch); the column's N sites against the rows are
+              // counted per N run after the loop
+              const uint32_t nj = sNJ[k][lane];
+#pragma unroll
+              for (int a = 0; a < RP; a++) {
+                const uint32_t ni = SR ? rn[a][q] : sNI[k][wave * RP + a];  // the same in every lane
+                if (ni) {
+                  ncn[a].r[0] += __popc(ni & pjv.x);
+                  ncn[a].r[1] += __popc(ni & pjv.y);
+                  ncn[a].r[2] += __popc(ni & pjv.z);
+                  ncn[a].r[3] += __popc(ni & pjv.w);
+                  ncn[a].nn += __popc(ni & nj);
+                }
               }
             }
           }
         }
       }
+    }
   }
   if constexpr (MF) {
     static_assert(RP == 4, "the MFMA backend tiles 16 rows");
@@ -3904,7 +3945,7 @@ int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
   } while (0)
   SCHK(hipMalloc(&d.codes, (size_t)(nn * d.Ls)));
   SCHK(hipMalloc(&d.codesT, (size_t)(d.Ls * d.npad)));
-  SCHK(hipMalloc(&d.planes, (size_t)(nn * d.W) * sizeof(uint4)));
+  SCHK(hipMalloc(&d.planes, (size_t)(nn * d.W + 4) * sizeof(uint4)));  // + 4: the pair kernel's 4-word row loads
   SCHK(hipMalloc(&d.onehot, (size_t)(d.npad * d.W64 * 256)));
   SCHK(hipMalloc(&d.ev_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.evg_off, (size_t)(nn + 1) * sizeof(int32_t)));
@@ -3913,18 +3954,18 @@ int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
   SCHK(hipMalloc(&d.pre, (size_t)(nn * (d.W + 1)) * sizeof(uint4)));
   SCHK(hipMalloc(&d.tb, (size_t)nn * sizeof(uint4)));
   SCHK(hipMalloc(&d.npre, (size_t)(nn * (d.W + 1)) * sizeof(uint32_t)));
-  SCHK(hipMalloc(&d.nmask, (size_t)(nn * d.W) * sizeof(uint32_t)));
+  SCHK(hipMalloc(&d.nmask, (size_t)(nn * d.W + 4) * sizeof(uint32_t)));
   SCHK(hipMalloc(&d.weights, (size_t)nn * sizeof(int64_t)));
   // padding reads as gaps (codes, site-major codes) and as nothing (planes, one-hot operands, event offsets)
   SCHK(hipMemsetAsync(d.codes, GAPC, (size_t)(nn * d.Ls), c->stream));
   SCHK(hipMemsetAsync(d.codesT, GAPC, (size_t)(d.Ls * d.npad), c->stream));
-  SCHK(hipMemsetAsync(d.planes, 0, (size_t)(nn * d.W) * sizeof(uint4), c->stream));
+  SCHK(hipMemsetAsync(d.planes, 0, (size_t)(nn * d.W + 4) * sizeof(uint4), c->stream));
   SCHK(hipMemsetAsync(d.onehot, 0, (size_t)(d.npad * d.W64 * 256), c->stream));
   SCHK(hipMemsetAsync(d.ev_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
   SCHK(hipMemsetAsync(d.evg_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
   SCHK(hipMemsetAsync(d.nrun_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
   SCHK(hipMemsetAsync(d.grun_off, 0, (size_t)(nn + 1) * sizeof(int32_t), c->stream));
-  SCHK(hipMemsetAsync(d.nmask, 0, (size_t)(nn * d.W) * sizeof(uint32_t), c->stream));
+  SCHK(hipMemsetAsync(d.nmask, 0, (size_t)(nn * d.W + 4) * sizeof(uint32_t), c->stream));
   SCHK(hipStreamSynchronize(c->stream));
 #undef SCHK
   *out = S;
