@@ -1927,15 +1927,24 @@ __device__ void cc_table_wave(const RunArgs &A, int64_t i, int64_t j, bool valid
 
 // Clean/clean 4x4 count tables of `ns` special sequences ids[0..ns) against the sequences [x0, x0 + nx):
 // T[(s * 16 + 4a + b) * nx + x - x0] = sites where sequence ids[s] has clean base a and sequence x clean base b.  A
-// block takes 16 special sequences x 64 others (the pair kernel's tile), their planes staged through LDS KW words at
-// a time; a wave's lanes are the 64 others, each with the 16 counters of the wave's 4 special sequences.
+// block takes 16 special sequences x 64 others (the pair kernel's tile); a wave's lanes are the 64 others, each with
+// the 16 counters of the wave's 4 special sequences.  The others' planes are staged through LDS KW words at a time;
+// the special sequences' words are the same in every lane of a wave and are read with scalar loads, two words at a
+// time (words past a sequence's end read the next one or the planes' padding and are not used).
 __global__ __launch_bounds__(256) void k_spec16(RunArgs A, const int64_t *__restrict__ ids, int ns, int64_t x0,
                                                 int64_t nx, uint32_t *__restrict__ T)
 {
-  __shared__ uint4 sI[KW][TI + 1], sJ[KW][TJ + 1];
+  __shared__ uint4 sJ[KW][TJ + 1];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const int sb = blockIdx.y * TI;
   const int64_t xb = (int64_t)blockIdx.x * TJ;
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  suint32 *rowp[RPW];
+#pragma unroll
+  for (int a = 0; a < RPW; a++) {
+    const int sp = min(sb + wv * RPW + a, ns - 1);  // past the list: its last sequence (counts not stored)
+    rowp[a] = (suint32 *)(A.planes + ids[sp] * A.W);
+  }
   uint32_t c[RPW][16];
 #pragma unroll
   for (int a = 0; a < RPW; a++)
@@ -1944,33 +1953,33 @@ __global__ __launch_bounds__(256) void k_spec16(RunArgs A, const int64_t *__rest
   for (int64_t w0 = 0; w0 < A.W; w0 += KW) {
     const int kw = (int)min((int64_t)KW, A.W - w0);
     __syncthreads();
-    for (int k = tid; k < kw * (TI + TJ); k += 256) {
-      const int q = k / kw, w = k % kw;
-      int64_t seq = -1;
-      if (q < TI) {
-        if (sb + q < ns) seq = ids[sb + q];
-      } else if (xb + (q - TI) < nx) {
-        seq = x0 + xb + (q - TI);
-      }
-      const uint4 v = seq >= 0 ? A.planes[seq * A.W + w0 + w] : make_uint4(0, 0, 0, 0);
-      if (q < TI)
-        sI[w][q] = v;
-      else
-        sJ[w][q - TI] = v;
+    for (int k = tid; k < KW * TJ; k += 256) {
+      const int q = k >> 4, w = k & (KW - 1);
+      const int64_t seq = x0 + xb + q;
+      sJ[w][q] = w < kw && xb + q < nx ? A.planes[seq * A.W + w0 + w] : make_uint4(0, 0, 0, 0);
     }
     __syncthreads();
-    for (int k = 0; k < kw; k++) {
-      const uint4 b = sJ[k][lane];
-      const uint32_t bj[4] = {~(b.y | b.z), b.y & ~b.z, b.z & ~b.y, b.w};
+    for (int k0 = 0; k0 < kw; k0 += 2) {
+      uint4 rw[RPW][2];
 #pragma unroll
-      for (int a = 0; a < RPW; a++) {
-        const uint4 p = sI[k][wave * RPW + a];
-        const uint32_t M = p.x & b.x;
-        const uint32_t ai[4] = {M & ~(p.y | p.z), M & p.y & ~p.z, M & p.z & ~p.y, M & p.w};
+      for (int a = 0; a < RPW; a++)
 #pragma unroll
-        for (int x = 0; x < 4; x++)
+        for (int q = 0; q < 2; q++) rw[a][q] = sload4(rowp[a] + 4 * (w0 + k0 + q));
 #pragma unroll
-          for (int y = 0; y < 4; y++) c[a][4 * x + y] += __popc(ai[x] & bj[y]);
+      for (int q = 0; q < 2; q++) {
+        if (k0 + q >= kw) break;
+        const uint4 b = sJ[k0 + q][lane];
+        const uint32_t bj[4] = {~(b.y | b.z), b.y & ~b.z, b.z & ~b.y, b.w};
+#pragma unroll
+        for (int a = 0; a < RPW; a++) {
+          const uint4 p = rw[a][q];
+          const uint32_t M = p.x & b.x;
+          const uint32_t ai[4] = {M & ~(p.y | p.z), M & p.y & ~p.z, M & p.z & ~p.y, M & p.w};
+#pragma unroll
+          for (int x = 0; x < 4; x++)
+#pragma unroll
+            for (int y = 0; y < 4; y++) c[a][4 * x + y] += __popc(ai[x] & bj[y]);
+        }
       }
     }
   }
