@@ -1027,30 +1027,36 @@ __device__ __attribute__((always_inline)) inline void chunk_apply(const RunArgs 
 {
   constexpr int TI_ = 4 * RP;
   const int R = te.r1 - te.r0;
+  // Row events are the same in every lane: their bounds and words go to SGPRs (the words by scalar loads from the
+  // set's event list, which the chunk's sEv copies), so the loop and the table row are scalar work.
+  const int wv = __builtin_amdgcn_readfirstlane(wave);
+  suint32 *evs = (suint32 *)(A.ev + te.er0 + te.r0);
 #pragma unroll
   for (int a = 0; a < RP; a++) {
     const int64_t i = ib + a;
     if (i >= A.r1) continue;  // uniform over the wave
     if (MASKED && !__ballot(use[a])) continue;
-    const int e0 = max(sRo[wave * RP + a], te.r0) - te.r0, e1 = min(sRo[wave * RP + a + 1], te.r1) - te.r0;
+    const int e0 = __builtin_amdgcn_readfirstlane(max(sRo[wv * RP + a], te.r0) - te.r0);
+    const int e1 = __builtin_amdgcn_readfirstlane(min(sRo[wv * RP + a + 1], te.r1) - te.r0);
     for (int h0 = e0; h0 < e1; h0 += EVFLUSH) {
     const int h1 = min(e1, h0 + EVFLUSH);
     int e = h0;
     for (; e + EVB <= h1; e += EVB) {
       uint32_t c[EVB], x[EVB];
+      const uint8_t *gx = g + e * 64 + lane;
 #pragma unroll
       for (int k = 0; k < EVB; k++) {
-        c[k] = sEv[e + k] & 31u;
-        x[k] = g[(e + k) * 64 + lane];
+        c[k] = (evs[e + k] & 31u) * TABW;
+        x[k] = gx[k * 64];
       }
 #pragma unroll
       for (int k = 0; k < EVB; k++) {
-        const uint4 t = sTabR[c[k] * TABW + x[k]];
+        const uint4 t = sTabR[c[k] + x[k]];
         ev_add(acc[a], MASKED ? keep_if(use[a], t) : t);
       }
     }
     for (; e < h1; e++) {
-      const uint4 t = sTabR[(sEv[e] & 31u) * TABW + g[e * 64 + lane]];
+      const uint4 t = sTabR[(evs[e] & 31u) * TABW + g[e * 64 + lane]];
       ev_add(acc[a], MASKED ? keep_if(use[a], t) : t);
     }
     ev_flush(acc[a], tot[a]);
@@ -2074,7 +2080,8 @@ __device__ __noinline__ double add_repeat_slow(double x, double v, double inv, u
 }
 __device__ inline double add_repeat(double x, double v, double inv, uint32_t m)
 {
-  if (floor(x * inv) == x * inv) return x + (double)m * v;  // a multiple of v: every partial sum exact
+  double y;
+  if (tn93g_add_repeat_exact(x, v, inv, m, &y)) return y;
   return add_repeat_slow(x, v, inv, m);
 }
 

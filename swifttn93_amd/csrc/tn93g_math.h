@@ -195,6 +195,16 @@ TN93G_FN tn93g_logenc tn93g_log_fast(double x, const double (*tab)[4] = tn93g_lo
   return R;
 }
 
+// The cases in which every partial sum is exact, so that the m additions are one: x a multiple of v (the partial
+// sums are multiples of v below 2^27), or x + m v in the binade of x (x >= 0 and v are multiples of the spacing g
+// there, so every partial sum up to x + m v < 2^(e+1) is a multiple of g in that binade; a sum that reaches the next
+// binade rounds to it or above, which the test sees).  True with *y the result.
+TN93G_FN bool tn93g_add_repeat_exact(double x, double v, double inv, uint32_t m, double *y)
+{
+  *y = x + (double)m * v;
+  return (tn93g_bits(*y) >> 52) == (tn93g_bits(x) >> 52) || floor(x * inv) == x * inv;
+}
+
 // m additions of v (a power of two, the value of every N-site effect of tn93's replay) to x >= 0, one after the
 // other in FP64 as tn93's site loop makes them, given inv = 1 / v (exact).  Closed form: with v <= x, write
 // x = S + f, S the largest multiple of v not above x, 0 <= f < v, f a multiple of g = the spacing of doubles at x.
@@ -207,7 +217,8 @@ TN93G_FN tn93g_logenc tn93g_log_fast(double x, const double (*tab)[4] = tn93g_lo
 // bit t.  With x < v the first addition can pass several powers of two at once; it is made as tn93 makes it.
 TN93G_FN double tn93g_add_repeat(double x, double v, double inv, uint32_t m)
 {
-  if (!m || floor(x * inv) == x * inv) return x + (double)m * v;  // a multiple of v: every partial sum exact
+  double y;
+  if (tn93g_add_repeat_exact(x, v, inv, m, &y)) return y;
   if (x < v) {
     x = x + v;
     if (!--m || floor(x * inv) == x * inv) return x + (double)m * v;
