@@ -10,7 +10,8 @@
 //                                                                                (clean/clean counts, MFMA backend)
 //   ev_off/ev   [n+1] i32, u32 (site << 5 | code): ambiguity events in site order, gaps excluded
 //   evg_off/evg the same with gap sites (-a gapmm);   nrun_off/nruns  N runs [first, end) of dense-N sets;
-//   grun_off/gruns gap runs;   pre [n][W+1] uint4 plane counts before each word boundary, tb [n] their totals
+//   grun_off/gruns gap runs;   pre [n][W+1] 32-byte records: uint4 plane counts before each word boundary and
+//   the planes of that word (one cache line per prefix count), tb [n] the totals
 // Per run: either the fast-path row accumulators and unordered links (distances certified in the kernel), or D
 // [rows][cols] f64 + U [rows][cols] u16 (ulps that bound an uncertain logarithm) for the exact path; the histogram
 // in per-stripe replicas, the flat replay list and the host-exact list; for long alignments the 16-cell count
@@ -203,6 +204,12 @@ static bool event_entry(const SiteEffect &E, uint32_t out[4], int a = -1, int b 
 }
 
 // ------------------------------------------------------------------------------------------ device structures
+// Plane counts of a sequence before word boundary w (clean, C|T, G|T, T over sites [0, 32 w)) beside the planes of
+// word w itself, so that the counts before any site are one 32-byte record (one cache line).
+struct PrefixRec {
+  uint4 pre, word;
+};
+
 struct SetDev {
   int64_t n, L, Ls, npad, W, W64;
   uint8_t *codes, *codesT;
@@ -215,7 +222,7 @@ struct SetDev {
   uint32_t *nmask;   // [n][W] N sites
   int32_t *nrun_off; // [n+1] N runs, for sets with dense N (ndense)
   uint2 *nruns;
-  uint4 *pre;        // [n][W+1] plane counts before each word boundary
+  PrefixRec *pre;    // [n][W+1] plane counts before each word boundary, and the word (zero at W)
   uint4 *tb;         // [n] pre[s][W], the totals of sequence s
   uint32_t *npre;    // [n][W+1] N sites before each word boundary
   int32_t *grun_off; // [n+1] gap runs
@@ -271,7 +278,7 @@ struct RunArgs {
   const int32_t *nrun_off; // [n+1] and the N runs [start, end) of each sequence (dense-N sets)
   const uint2 *nruns;
   int nplane;              // N sites are counted from nmask / nruns, not in the event lists
-  const uint4 *pre;        // [n][W+1] plane counts (clean, C|T, G|T, T) of the words before each word boundary
+  const PrefixRec *pre;    // [n][W+1] plane counts (clean, C|T, G|T, T) before each word boundary, and that word
   const uint32_t *npre;    // [n][W+1] the same for the N sites (nmask)
   const uint4 *tb;         // [n] the same over all words (the totals)
   const int32_t *grun_off; // [n+1] and the gap runs [start, end) of each sequence
@@ -617,8 +624,9 @@ __global__ void k_runfill(const uint8_t *__restrict__ codes, const int32_t *__re
 
 // pre[s][w] = plane counts (clean, C|T, G|T, T) of sequence s over its words [0, w), w = 0..W: one wave per sequence,
 // 64 word boundaries per step
-__global__ void k_prefix(const uint4 *__restrict__ planes, const uint32_t *__restrict__ nmask, uint4 *__restrict__ pre,
-                         uint4 *__restrict__ tb, uint32_t *__restrict__ npre, int64_t W, int64_t s0)
+__global__ void k_prefix(const uint4 *__restrict__ planes, const uint32_t *__restrict__ nmask,
+                         PrefixRec *__restrict__ pre, uint4 *__restrict__ tb, uint32_t *__restrict__ npre, int64_t W,
+                         int64_t s0)
 {
   const int64_t s = s0 + blockIdx.x;
   const int lane = threadIdx.x;
@@ -645,7 +653,7 @@ __global__ void k_prefix(const uint4 *__restrict__ planes, const uint32_t *__res
     add4(v, carry);
     nv += ncarry;
     if (w <= W) {
-      pre[s * (W + 1) + w] = v;
+      pre[s * (W + 1) + w] = PrefixRec{v, w < W ? planes[s * W + w] : make_uint4(0, 0, 0, 0)};
       npre[s * (W + 1) + w] = nv;
     }
     if (w == W) tb[s] = v;
@@ -750,14 +758,13 @@ __device__ inline void count_word(const uint4 pi, const uint4 pj, CntM &c)
 
 // Plane counts (clean, C|T, G|T, T) of sequence s over the sites [0, p): the prefix at the word boundary plus the
 // bits of the partial word.
-__device__ inline uint4 plane_prefix(const uint4 *__restrict__ pre, const uint4 *__restrict__ planes, int64_t W,
-                                     int64_t s, uint32_t p)
+__device__ inline uint4 plane_prefix(const PrefixRec *__restrict__ pre, int64_t W, int64_t s, uint32_t p)
 {
-  const int64_t w = p >> 5;
-  uint4 r = pre[s * (W + 1) + w];
+  const PrefixRec e = pre[s * (W + 1) + (p >> 5)];  // both halves: one cache line
+  uint4 r = e.pre;
   const uint32_t b = p & 31u;
   if (b) {
-    const uint4 v = planes[s * W + w];
+    const uint4 v = e.word;
     const uint32_t m = (1u << b) - 1u;
     r.x += __popc(v.x & m);
     r.y += __popc(v.y & m);
@@ -1178,8 +1185,8 @@ __device__ __attribute__((always_inline)) inline void base_sums(const RunArgs &A
     for (int g = g0; g < g1; g++) {
       const uint2 r = tr.row(A, g);
       if (jin) {
-        sub4(S[a], plane_prefix(A.pre, A.planes, W, j, r.y));
-        add4(S[a], plane_prefix(A.pre, A.planes, W, j, r.x));
+        sub4(S[a], plane_prefix(A.pre, W, j, r.y));
+        add4(S[a], plane_prefix(A.pre, W, j, r.x));
       }
     }
   }
@@ -1189,8 +1196,8 @@ __device__ __attribute__((always_inline)) inline void base_sums(const RunArgs &A
     for (int a = 0; a < RP; a++) {
       const int64_t i = ib + a;
       if (i >= A.r1) continue;
-      sub4(S[a], plane_prefix(A.pre, A.planes, W, i, r.y));
-      add4(S[a], plane_prefix(A.pre, A.planes, W, i, r.x));
+      sub4(S[a], plane_prefix(A.pre, W, i, r.y));
+      add4(S[a], plane_prefix(A.pre, W, i, r.x));
     }
   }
 }
@@ -1445,8 +1452,8 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
         const int q1 = A.nrun_off[i + 1];
         for (int q = A.nrun_off[i]; q < q1; q++) {
           const uint2 r = A.nruns[q];
-          const uint4 hi = plane_prefix(A.pre, A.planes, A.W, j, r.y);
-          const uint4 lo = plane_prefix(A.pre, A.planes, A.W, j, r.x);
+          const uint4 hi = plane_prefix(A.pre, A.W, j, r.y);
+          const uint4 lo = plane_prefix(A.pre, A.W, j, r.x);
           ncn[a].r[0] += hi.x - lo.x;
           ncn[a].r[1] += hi.y - lo.y;
           ncn[a].r[2] += hi.z - lo.z;
@@ -1465,8 +1472,8 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
 #pragma unroll
         for (int a = 0; a < RP; a++) {
           if (ib + a >= A.r1) continue;  // uniform over the wave
-          const uint4 hi = plane_prefix(A.pre, A.planes, A.W, ib + a, r.y);
-          const uint4 lo = plane_prefix(A.pre, A.planes, A.W, ib + a, r.x);
+          const uint4 hi = plane_prefix(A.pre, A.W, ib + a, r.y);
+          const uint4 lo = plane_prefix(A.pre, A.W, ib + a, r.x);
           ncn[a].c[0] += hi.x - lo.x;
           ncn[a].c[1] += hi.y - lo.y;
           ncn[a].c[2] += hi.z - lo.z;
@@ -2127,7 +2134,7 @@ __device__ inline void replay_apply_seg(ReplayState &s, const ReplayEnt *tab, co
 // Counts of A, C, G, T of sequence q over sites [p, e): differences of its plane prefix counts (clean, C|T, G|T, T)
 __device__ inline void base_counts(const RunArgs &A, int64_t q, uint32_t p, uint32_t e, uint32_t (&n)[4])
 {
-  const uint4 hi = plane_prefix(A.pre, A.planes, A.W, q, e), lo = plane_prefix(A.pre, A.planes, A.W, q, p);
+  const uint4 hi = plane_prefix(A.pre, A.W, q, e), lo = plane_prefix(A.pre, A.W, q, p);
   const uint32_t c = hi.x - lo.x, b0 = hi.y - lo.y, b1 = hi.z - lo.z, t = hi.w - lo.w;
   n[0] = c - b0 - b1 + t;
   n[1] = b0 - t;
@@ -3967,7 +3974,7 @@ int tn93g_set_create(tn93g_ctx *c, int64_t L, int64_t capacity, tn93g_set **out)
   SCHK(hipMalloc(&d.evg_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.nrun_off, (size_t)(nn + 1) * sizeof(int32_t)));
   SCHK(hipMalloc(&d.grun_off, (size_t)(nn + 1) * sizeof(int32_t)));
-  SCHK(hipMalloc(&d.pre, (size_t)(nn * (d.W + 1)) * sizeof(uint4)));
+  SCHK(hipMalloc(&d.pre, (size_t)(nn * (d.W + 1)) * sizeof(PrefixRec)));
   SCHK(hipMalloc(&d.tb, (size_t)nn * sizeof(uint4)));
   SCHK(hipMalloc(&d.npre, (size_t)(nn * (d.W + 1)) * sizeof(uint32_t)));
   SCHK(hipMalloc(&d.nmask, (size_t)(nn * d.W + 4) * sizeof(uint32_t)));
