@@ -2088,7 +2088,7 @@ __device__ __noinline__ double add_repeat_slow(double x, double v, double inv, u
 __device__ inline double add_repeat(double x, double v, double inv, uint32_t m)
 {
   double y;
-  if (tn93g_add_repeat_exact(x, v, inv, m, &y)) return y;
+  if (tn93g_add_repeat_exact(x, v, inv, m, &y) || (x >= v && tn93g_add_repeat_two(x, v, inv, m, &y))) return y;
   return add_repeat_slow(x, v, inv, m);
 }
 

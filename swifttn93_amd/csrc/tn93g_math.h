@@ -205,6 +205,18 @@ TN93G_FN bool tn93g_add_repeat_exact(double x, double v, double inv, uint32_t m,
   return (tn93g_bits(*y) >> 52) == (tn93g_bits(x) >> 52) || floor(x * inv) == x * inv;
 }
 
+// With v <= x (S, f and Sm = S + m v as in tn93g_add_repeat below) and one or two powers of two passed, the one or
+// two roundings of f to the doubled spacings give what a single rounding of f to the final spacing 2^k g gives (k <=
+// 2; from three on, ties rounded down first can meet a tie rounded up later), and Sm is an even multiple of that
+// spacing (v >= 1/16 > 2^3 g), so the FP64 sum Sm + f, rounded to nearest even, is the result.  True with *y it.
+TN93G_FN bool tn93g_add_repeat_two(double x, double v, double inv, uint32_t m, double *y)
+{
+  const double S = floor(x * inv) * v, Sm = S + (double)m * v;  // both exact
+  if ((int)(tn93g_bits(Sm) >> 52) - (int)(tn93g_bits(x) >> 52) > 2) return false;
+  *y = Sm + (x - S);
+  return true;
+}
+
 // m additions of v (a power of two, the value of every N-site effect of tn93's replay) to x >= 0, one after the
 // other in FP64 as tn93's site loop makes them, given inv = 1 / v (exact).  Closed form: with v <= x, write
 // x = S + f, S the largest multiple of v not above x, 0 <= f < v, f a multiple of g = the spacing of doubles at x.
@@ -223,6 +235,7 @@ TN93G_FN double tn93g_add_repeat(double x, double v, double inv, uint32_t m)
     x = x + v;
     if (!--m || floor(x * inv) == x * inv) return x + (double)m * v;
   }
+  if (tn93g_add_repeat_two(x, v, inv, m, &y)) return y;
   const double S = floor(x * inv) * v, f = x - S;  // both exact
   const double Sm = S + (double)m * v;           // exact: a multiple of v below 2^27
   const int e0 = (int)((tn93g_bits(x) >> 52) & 0x7ffu), e1 = (int)((tn93g_bits(Sm) >> 52) & 0x7ffu);

@@ -23,12 +23,20 @@ def test_add_repeat_equals_sequential_additions(addrep, seed):
     assert out.returncode == 0
 
 
-def test_planted_mistake_is_caught(tmp_path):
-    """A closed form that rounds the ties away from zero instead of to even must fail the check."""
+MISTAKES = [
+    # ties rounded away from zero instead of to even
+    ("F = (F >> t & 1ull) ? F + (1ull << (t - 1)) : F - (1ull << (t - 1));", "F = F + (1ull << (t - 1));"),
+    # one rounding taken for three powers of two passed (it is right for one or two only)
+    ("(int)(tn93g_bits(x) >> 52) > 2) return false;", "(int)(tn93g_bits(x) >> 52) > 3) return false;"),
+]
+
+
+@pytest.mark.parametrize("k", range(len(MISTAKES)))
+def test_planted_mistake_is_caught(tmp_path, k):
+    """A closed form with a planted mistake must fail the check."""
     src = (REPO / "tests" / "native" / "addrep.cpp").read_text()
     hdr = (REPO / "swifttn93_amd" / "csrc" / "tn93g_math.h").read_text()
-    bad = hdr.replace("F = (F >> t & 1ull) ? F + (1ull << (t - 1)) : F - (1ull << (t - 1));",
-                      "F = F + (1ull << (t - 1));")
+    bad = hdr.replace(*MISTAKES[k])
     assert bad != hdr
     (tmp_path / "tn93g_math.h").write_text(bad)
     (tmp_path / "tn93g_logtab.h").write_text((REPO / "swifttn93_amd" / "csrc" / "tn93g_logtab.h").read_text())
