@@ -4875,7 +4875,10 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
       const unsigned char *b = (const unsigned char *)dims;
       key.insert(key.end(), b, b + sizeof dims);
     }
-    if (!c->gexec || key != c->gkey) {
+    // TN93G_NO_GRAPH=1 issues the same launches directly on the streams instead of through a captured graph
+    // (profilers that collect hardware counters per kernel do not follow graph launches)
+    const bool eager = getenv("TN93G_NO_GRAPH") != nullptr;
+    if (eager || !c->gexec || key != c->gkey) {
       if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
       c->gexec = nullptr;
       c->gkey.clear();
@@ -4905,7 +4908,10 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
         }
         return 0;
       };
-      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      if (eager)
+        HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+      else
+        HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       // 0: every job's fills; 1: each job's tables on its branch; 2: the merged pair kernels on the main stream, the
       // other jobs' whole sequence on their branches; 3: the merged jobs' replays, tails and link kernels
       hipLaunchKernelGGL(k_fill_ranges, dim3(64, (unsigned)std::min<size_t>(fills.size(), 65535)), dim3(256), 0,
@@ -4958,23 +4964,27 @@ int tn93g_run_batch(tn93g_ctx *c, int n, const tn93g_set *const *sets, const tn9
                            c->stream, (const CopyRange *)((char *)c->multi.p + off_copies), (int)copies.size());
       if (rc) {
         hipGraph_t dropped = nullptr;
-        (void)hipStreamEndCapture(c->stream, &dropped);
+        if (!eager) (void)hipStreamEndCapture(c->stream, &dropped);
         if (dropped) (void)hipGraphDestroy(dropped);
         return fail_all(rc);
       }
-      const hipError_t e1 = hipStreamEndCapture(c->stream, &g);
-      if (e1 != hipSuccess)
-        return fail_all(c->fail(TN93G_E_HIP, fmt("hipStreamEndCapture: %s", hipGetErrorString(e1))));
-      const hipError_t e2 = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e2 != hipSuccess) {
-        c->gexec = nullptr;
-        return fail_all(c->fail(TN93G_E_HIP, fmt("hipGraphInstantiate: %s", hipGetErrorString(e2))));
+      if (!eager) {
+        const hipError_t e1 = hipStreamEndCapture(c->stream, &g);
+        if (e1 != hipSuccess)
+          return fail_all(c->fail(TN93G_E_HIP, fmt("hipStreamEndCapture: %s", hipGetErrorString(e1))));
+        const hipError_t e2 = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e2 != hipSuccess) {
+          c->gexec = nullptr;
+          return fail_all(c->fail(TN93G_E_HIP, fmt("hipGraphInstantiate: %s", hipGetErrorString(e2))));
+        }
+        c->gkey = key;
       }
-      c->gkey = key;
     }
-    HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    if (!eager) {
+      HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+      HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    }
     HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
     // one transfer of the staged read-back of every job
     if (ensure_pinned(c, c->pin, std::max<size_t>(stage_bytes, 1))) return fail_all(TN93G_E_NOMEM);

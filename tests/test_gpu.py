@@ -297,6 +297,15 @@ def test_batch_equals_single(engine):
             print(f"batch launch {rep}, job {k}", flush=True)
             _same_summary(w, g)
         engine.run(sets[1], **jobs[1][1])  # eager work between the launches of the cached graph
+    # the same launches issued directly on the streams (TN93G_NO_GRAPH: the profilers' counter passes)
+    os.environ["TN93G_NO_GRAPH"] = "1"
+    try:
+        got = engine.run_batch(jobs)
+    finally:
+        del os.environ["TN93G_NO_GRAPH"]
+    for k, (w, g) in enumerate(zip(want, got)):
+        print(f"batch without a graph, job {k}", flush=True)
+        _same_summary(w, g)
     for st in sets:
         st.free()
 
