@@ -2079,8 +2079,9 @@ __device__ void replay_events(const RunArgs &A, int64_t i, int64_t j, const Repl
   }
 }
 
-// m additions of v to a cell (tn93g_add_repeat); the part past a multiple of v is a call: inlined at every cell it
-// spills the replay's registers.
+// m additions of v to a cell (tn93g_add_repeat): the cases that come to one FP64 addition (every partial sum exact,
+// or one or two powers of two passed) inline, the rest (three or more powers passed, or x < v) a call: the crossing
+// loop inlined at every cell spills the replay's registers.
 __device__ __noinline__ double add_repeat_slow(double x, double v, double inv, uint32_t m)
 {
   return tn93g_add_repeat(x, v, inv, m);
