@@ -30,7 +30,7 @@ def build(verbose=True):
     LIB.parent.mkdir(exist_ok=True)
     CLI.parent.mkdir(exist_ok=True)
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, CSRC / "tn93g.hip"])
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, CSRC / "tn93g.hip", "-lrocblas"])
     os.replace(tmp, LIB)
     ctmp = CLI.with_name("tn93g.tmp")
     _run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-pthread", "-o", ctmp, CSRC / "tn93g_cli.cpp",

@@ -30,9 +30,11 @@
 // a pair-kernel variant as one launch each (the *_multi kernels; each block finds its job from a table of offsets),
 // and clears and reads back every job's buffers with one k_fill_ranges and one k_copy_ranges launch.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 
 #include <algorithm>
 #include <array>
+#include <iostream>
 #include <map>
 #include <mutex>
 #include <atomic>
@@ -309,8 +311,8 @@ struct RunArgs {
   // pair of the fast path has one) against the other side of the run, for the replay.  Null when not built.
   const int32_t *spec_row;  // [nrows]: rank of row r0 + r among the special rows, or -1
   const int32_t *spec_col;  // [ncols]: rank of column c0 + c among the special columns, or -1
-  const uint32_t *tab_row;  // [nsr][16][ncols]: cell 4a + b counts row base a against column base b
-  const uint32_t *tab_col;  // [nsc][16][nrows]: cell 4a + b counts column base a against row base b
+  const uint32_t *tab_row;  // [nsr][4][ncols][4]: [s][a][c][b] counts special row s's base a against column c's base b
+  const uint32_t *tab_col;  // [nsc][4][nrows][4]: [s][a][r][b] counts special column s's base a against row r's base b
   const int64_t *spec_rows; // [nsr] the special rows (set indices), then [nsc] the special columns
   int nsr, nsc;
   ExactRec *exact_list;
@@ -2003,8 +2005,37 @@ __global__ __launch_bounds__(256) void k_spec16(RunArgs A, const int64_t *__rest
       const int sp = sb + wave * RPW + a;
       if (sp < ns)
 #pragma unroll
-        for (int k = 0; k < 16; k++) T[((size_t)sp * 16 + k) * (size_t)nx + (size_t)xl] = c[a][k];
+        for (int x = 0; x < 4; x++)
+          *reinterpret_cast<uint4 *>(T + (((size_t)sp * 4 + x) * (size_t)nx + (size_t)xl) * 4) =
+              make_uint4(c[a][4 * x], c[a][4 * x + 1], c[a][4 * x + 2], c[a][4 * x + 3]);
     }
+}
+
+// One-hot rows of sequences for the special-sequence tables as an int8 GEMM: out[(q * 4 + b) * Kp + p] = 1 where
+// sequence seq(q) (ids[q], or x0 + q without ids) holds the clean base b at site p, else 0 (ambiguity codes, N, gaps
+// and the padding of the codes rows past L).  One thread per 16 sites of one sequence.
+__global__ __launch_bounds__(256) void k_onehot_rows(const uint8_t *__restrict__ codes, int64_t Ls,
+                                                     const int64_t *__restrict__ ids, int64_t x0, int64_t nq,
+                                                     int8_t *__restrict__ out)
+{
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x, chunks = Ls / 16;
+  if (t >= nq * chunks) return;
+  const int64_t q = t / chunks, ch = t - q * chunks;
+  const int64_t seq = ids ? ids[q] : x0 + q;
+  const uint4 v = *reinterpret_cast<const uint4 *>(codes + seq * Ls + ch * 16);
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (uint32_t b = 0; b < 4; b++) {
+    uint32_t o[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      uint32_t r = 0;
+#pragma unroll
+      for (int y = 0; y < 4; y++) r |= ((w[k] >> (8 * y) & 255u) == b ? 1u : 0u) << (8 * y);
+      o[k] = r;
+    }
+    *reinterpret_cast<uint4 *>(out + (q * 4 + b) * Ls + ch * 16) = make_uint4(o[0], o[1], o[2], o[3]);
+  }
 }
 
 // The clean/clean table of pair (i, j) from the special-sequence tables when i or j is special.
@@ -2012,18 +2043,24 @@ __device__ inline bool cc_table_lookup(const RunArgs &A, int64_t i, int64_t j, u
 {
   if (!A.spec_row) return false;
   const int32_t si = A.spec_row[i - A.r0];
-  if (si >= 0) {
-    const uint32_t *t = A.tab_row + (size_t)si * 16 * (size_t)A.ncols + (size_t)(j - A.c0);
+  if (si >= 0) {  // icc[4a + b] = [si][a][j][b]
+    const uint32_t *t = A.tab_row + ((size_t)si * 4 * (size_t)A.ncols + (size_t)(j - A.c0)) * 4;
 #pragma unroll
-    for (int k = 0; k < 16; k++) icc[k] = t[(size_t)k * (size_t)A.ncols];
+    for (int a = 0; a < 4; a++) {
+      const uint4 v = *reinterpret_cast<const uint4 *>(t + (size_t)a * (size_t)A.ncols * 4);
+      icc[4 * a] = v.x, icc[4 * a + 1] = v.y, icc[4 * a + 2] = v.z, icc[4 * a + 3] = v.w;
+    }
     return true;
   }
   const int32_t sj = A.spec_col[j - A.c0];
-  if (sj >= 0) {
+  if (sj >= 0) {  // icc[4a + b] = [sj][b][i][a] (the special sequence is the column)
     const int64_t nr = A.r1 - A.r0;
-    const uint32_t *t = A.tab_col + (size_t)sj * 16 * (size_t)nr + (size_t)(i - A.r0);
+    const uint32_t *t = A.tab_col + ((size_t)sj * 4 * (size_t)nr + (size_t)(i - A.r0)) * 4;
 #pragma unroll
-    for (int k = 0; k < 16; k++) icc[k] = t[(size_t)(4 * (k & 3) + (k >> 2)) * (size_t)nr];  // transposed
+    for (int b = 0; b < 4; b++) {
+      const uint4 v = *reinterpret_cast<const uint4 *>(t + (size_t)b * (size_t)nr * 4);
+      icc[b] = v.x, icc[4 + b] = v.y, icc[8 + b] = v.z, icc[12 + b] = v.w;
+    }
     return true;
   }
   return false;
@@ -2858,6 +2895,9 @@ struct tn93g_ctx {
   hipStream_t rside[NBRANCH] = {};
   hipEvent_t rfork[NBRANCH] = {}, rjoin[NBRANCH] = {};
   hipEvent_t ev[5];  // pair kernel start, its end, run end, tail end, run start
+  rocblas_handle blas = nullptr;  // single runs' special-sequence tables as int8 GEMMs (made on first use)
+  void *gemm_ops = nullptr;       // their one-hot operands
+  size_t gemm_ops_n = 0;
   std::string err;
   std::vector<RunBufs *> slots;
   HostBuf pin;  // staging of the read-backs (counters, links, row accumulators, histogram)
@@ -3422,8 +3462,81 @@ static void job_fill_ranges(const Job &J, std::vector<FillRange> &out)
   for (int k = 0; k < nf; k++) out.push_back(L.r[k]);
 }
 
-// fills: false when a batch clears every job's buffers with one k_fill_ranges launch
-static void job_enqueue_pre(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, bool fills = true)
+// The special-sequence tables of a single run as int8 GEMMs on the matrix cores (rocBLAS): table [s][a][x][b] is
+// entry (4 x + b, 4 s + a) of the column-major product X^T Y, X the one-hot rows of the other side's sequences and Y
+// those of the special ones ([4 q + base][site], site-contiguous, so both operands are K-major: a "TN" product), with
+// int32 accumulation: exact counts of the clean/clean sites.  False (nothing enqueued) when the operands cannot be
+// allocated or the library cannot be initialised: the caller then runs k_spec16.
+static bool spec_gemm_one(tn93g_ctx *c, const RunArgs &A, const int64_t *ids, int64_t ns, int64_t x0, int64_t nx,
+                          uint32_t *T, hipStream_t st)
+{
+  const int64_t Ls = A.Ls;  // a multiple of 16; sites from L on are gaps, zero in every one-hot row
+  int8_t *X = (int8_t *)c->gemm_ops, *Y = X + 4 * nx * Ls;
+  hipLaunchKernelGGL(k_onehot_rows, dim3((unsigned)((nx * (Ls / 16) + 255) / 256)), dim3(256), 0, st, A.codes, Ls,
+                     (const int64_t *)nullptr, x0, nx, X);
+  hipLaunchKernelGGL(k_onehot_rows, dim3((unsigned)((ns * (Ls / 16) + 255) / 256)), dim3(256), 0, st, A.codes, Ls,
+                     ids, (int64_t)0, ns, Y);
+  const int32_t one = 1, zero = 0;
+  return rocblas_gemm_ex(c->blas, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)(4 * nx),
+                         (rocblas_int)(4 * ns), (rocblas_int)Ls, &one, X, rocblas_datatype_i8_r, (rocblas_int)Ls, Y,
+                         rocblas_datatype_i8_r, (rocblas_int)Ls, &zero, T, rocblas_datatype_i32_r,
+                         (rocblas_int)(4 * nx), T, rocblas_datatype_i32_r, (rocblas_int)(4 * nx),
+                         rocblas_datatype_i32_r, rocblas_gemm_algo_standard, 0, 0) == rocblas_status_success;
+}
+// The standard streams' formatting (flags, precision, width, fill), saved on construction and put back on
+// destruction: rocBLAS changes the formatting of std::cout when it is set up, and the caller's output (tn93's
+// JSON, written with the default formatting) must not see that.
+struct StreamFormatGuard {
+  std::ios *s[3] = {&std::cout, &std::cerr, &std::clog};
+  std::ios::fmtflags f[3];
+  std::streamsize p[3], w[3];
+  char fill[3];
+  StreamFormatGuard()
+  {
+    for (int k = 0; k < 3; k++) f[k] = s[k]->flags(), p[k] = s[k]->precision(), w[k] = s[k]->width(), fill[k] = s[k]->fill();
+  }
+  ~StreamFormatGuard()
+  {
+    for (int k = 0; k < 3; k++) s[k]->flags(f[k]), s[k]->precision(p[k]), s[k]->width(w[k]), s[k]->fill(fill[k]);
+  }
+};
+
+static bool spec_gemm(tn93g_ctx *c, const Job &J, hipStream_t st)
+{
+  const StreamFormatGuard keep_format;
+  const RunArgs &A = J.A;
+  const bool cols = A.nsc > 0;  // a triangle over one range has none: its column tables are its row tables
+  const int64_t rows_side = 4 * (A.ncols + A.nsr) * A.Ls, cols_side = cols ? 4 * (J.nrows + A.nsc) * A.Ls : 0;
+  // GEMM dimensions are 32-bit: 4 x (sequences) and the sites
+  if (4 * std::max(A.ncols, J.nrows) >= (1ll << 31) || A.Ls >= (1ll << 31)) return false;
+  const size_t need_bytes = (size_t)std::max(rows_side, cols_side);
+  if (c->gemm_ops_n < need_bytes) {
+    if (c->gemm_ops) (void)hipFree(c->gemm_ops);
+    c->gemm_ops = nullptr;
+    c->gemm_ops_n = 0;
+    if (hipMalloc(&c->gemm_ops, need_bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      c->gemm_ops = nullptr;
+      return false;
+    }
+    c->gemm_ops_n = need_bytes;
+  }
+  if (!c->blas && rocblas_create_handle(&c->blas) != rocblas_status_success) {
+    c->blas = nullptr;
+    return false;
+  }
+  if (rocblas_set_stream(c->blas, st) != rocblas_status_success) return false;
+  if (A.nsr && !spec_gemm_one(c, A, A.spec_rows, A.nsr, A.c0, A.ncols, const_cast<uint32_t *>(A.tab_row), st))
+    return false;
+  // the second product reuses the operand buffer after the first on the same stream
+  if (cols && !spec_gemm_one(c, A, A.spec_rows + A.nsr, A.nsc, A.r0, J.nrows, const_cast<uint32_t *>(A.tab_col), st))
+    return false;
+  return true;
+}
+
+// fills: false when a batch clears every job's buffers with one k_fill_ranges launch; blas: a single run (not
+// captured into a batch graph), whose special-sequence tables may come from rocBLAS
+static void job_enqueue_pre(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, bool fills = true, bool blas = false)
 {
   RunArgs &A = J.A;
   const int64_t nrows = J.nrows;
@@ -3433,6 +3546,7 @@ static void job_enqueue_pre(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, b
     const int nf = job_fill_list(J, L);
     hipLaunchKernelGGL(k_fill_list, dim3(64, (unsigned)nf), dim3(256), 0, st, L);
   }
+  if ((A.nsr || A.nsc) && blas && !getenv("TN93G_NO_BLAS") && spec_gemm(c, J, st)) return;
   if (A.nsr)
     hipLaunchKernelGGL(k_spec16, dim3((unsigned)((A.ncols + TJ - 1) / TJ), (unsigned)((A.nsr + TI - 1) / TI)),
                        dim3(256), 0, st, A, A.spec_rows, A.nsr, A.c0, A.ncols, const_cast<uint32_t *>(A.tab_row));
@@ -3492,9 +3606,10 @@ static void job_enqueue_post(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, 
   hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, st, A);
 }
 
-static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int branch, bool fills = true)
+static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int branch, bool fills = true,
+                        bool blas = false)
 {
-  job_enqueue_pre(c, J, timing, st, fills);
+  job_enqueue_pre(c, J, timing, st, fills, blas);
   job_enqueue_pairs(c, J, timing, st);
   job_enqueue_post(c, J, timing, st, branch);
 }
@@ -3928,6 +4043,8 @@ void tn93g_close(tn93g_ctx *c)
     if (c->rjoin[b]) (void)hipEventDestroy(c->rjoin[b]);
   }
   if (c->fork) (void)hipEventDestroy(c->fork);
+  if (c->blas) (void)rocblas_destroy_handle(c->blas);
+  if (c->gemm_ops) (void)hipFree(c->gemm_ops);
   if (c->pin.p) (void)hipHostFree(c->pin.p);
   if (c->pin_links.p) (void)hipHostFree(c->pin_links.p);
   if (c->pin_up.p) (void)hipHostFree(c->pin_up.p);
@@ -4432,7 +4549,7 @@ static int autotune(tn93g_ctx *c, Job &J)
   for (size_t k = 0; k < cand.size(); k++) {
     int rc = job_set_shape(c, J, cand[k].first, cand[k].second);
     if (rc) return rc;
-    job_enqueue(c, J, true, c->stream, 0);
+    job_enqueue(c, J, true, c->stream, 0, true, true);
     HIPCHK(c, hipStreamSynchronize(c->stream));
     float ms = 0;
     HIPCHK(c, hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
@@ -4484,7 +4601,7 @@ static int run_one(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, int6
   int rc = job_prepare(c, J);
   if (rc) return rc;
   if ((rc = autotune(c, J))) return rc;
-  job_enqueue(c, J, true, c->stream, 0);
+  job_enqueue(c, J, true, c->stream, 0, true, true);
   HIPCHK(c, hipGetLastError());
   // one synchronisation: the counters, and for the fast path the row accumulators, the histogram and as many link
   // records as the last run of this slot produced (plus a margin); more links take a second copy

@@ -271,6 +271,42 @@ def _same_summary(r1, r2):
     assert "%.6g" % r1["mean_lo"][0] == "%.6g" % r2["mean_lo"][0]
 
 
+@pytest.mark.parametrize("kind", ["triangle", "cross", "cross_cols"])
+def test_gemm_tables_equal_kernel_tables(kind):
+    """A single run's special-sequence tables as int8 GEMMs (rocBLAS) against the same tables from k_spec16
+    (TN93G_NO_BLAS=1), the GEMM run first in a fresh context (its tables are not left over from another run):
+    triangle, -s with special sequences on both sides, and -s with special sequences among the columns only."""
+    codes = gen.c2s(200, seed=5)
+    kw = dict(mode=T.RESOLVE, threshold=0.05, min_overlap=5000, resolve_fraction=0.05)
+    if kind == "cross":
+        kw.update(kind=T.CROSS, n1=80)
+    elif kind == "cross_cols":  # the rows are the last 40 sequences, none of them special
+        spec = [k for k in range(codes.shape[0]) if any(c in (11, 12, 13, 14) for c in codes[k].tolist())]
+        plain = [k for k in range(codes.shape[0]) if k not in set(spec)]
+        order = plain[-40:] + [k for k in range(codes.shape[0]) if k not in set(plain[-40:])]
+        codes = codes[order]
+        kw.update(kind=T.CROSS, n1=40)
+    eng = T.Engine(0)
+    try:
+        s = eng.upload(codes)
+        got = eng.run(s, **kw)
+        os.environ["TN93G_NO_BLAS"] = "1"
+        try:
+            want = eng.run(s, **kw)
+        finally:
+            del os.environ["TN93G_NO_BLAS"]
+        s.free()
+    finally:
+        eng.close()
+    print(kind, "replayed", want["n_replayed"], flush=True)
+    assert want["n_replayed"] > 0
+    for f in ["hist", "link_row", "link_col", "link_dist"]:
+        assert np.array_equal(np.asarray(got[f]), np.asarray(want[f])), f
+    for f in ["weighted", "max_lo", "max_hi", "found_links", "skipped", "n_replayed"]:
+        assert got[f] == want[f], (f, got[f], want[f])
+    assert abs(got["mean_lo"][0] - want["mean_lo"][0]) <= 1e-12 * abs(want["mean_lo"][0])
+
+
 def test_batch_equals_single(engine):
     """tn93g_run_batch (one graph launch for all jobs) gives every job its single-run result.  The cached graph is
     launched again with eager runs in between, and every launch is checked."""
