@@ -2896,6 +2896,7 @@ struct tn93g_ctx {
   hipEvent_t rfork[NBRANCH] = {}, rjoin[NBRANCH] = {};
   hipEvent_t ev[5];  // pair kernel start, its end, run end, tail end, run start
   rocblas_handle blas = nullptr;  // single runs' special-sequence tables as int8 GEMMs (made on first use)
+  int64_t table_runs = 0;         // single runs that built special-sequence tables
   void *gemm_ops = nullptr;       // their one-hot operands
   size_t gemm_ops_n = 0;
   std::string err;
@@ -3534,6 +3535,20 @@ static bool spec_gemm(tn93g_ctx *c, const Job &J, hipStream_t st)
   return true;
 }
 
+// Whether a single run's tables come from rocBLAS: setting the library up costs about 0.15 s once per process, so a
+// context takes the GEMM from its second run with tables on (a process that runs many), or at once when k_spec16
+// would take longer than that (about 1e12 word pairs per second).  TN93G_TABLES=blas|kernel forces one (tests).
+static bool use_gemm_tables(tn93g_ctx *c, const Job &J)
+{
+  const char *e = getenv("TN93G_TABLES");
+  if (e && !strcmp(e, "kernel")) return false;
+  const bool first = c->table_runs++ == 0;
+  if (e && !strcmp(e, "blas")) return true;
+  const RunArgs &A = J.A;
+  const double word_pairs = ((double)A.nsr * (double)A.ncols + (double)A.nsc * (double)J.nrows) * (double)A.W;
+  return c->blas || !first || word_pairs > 1.5e11;
+}
+
 // fills: false when a batch clears every job's buffers with one k_fill_ranges launch; blas: a single run (not
 // captured into a batch graph), whose special-sequence tables may come from rocBLAS
 static void job_enqueue_pre(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, bool fills = true, bool blas = false)
@@ -3546,7 +3561,7 @@ static void job_enqueue_pre(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, b
     const int nf = job_fill_list(J, L);
     hipLaunchKernelGGL(k_fill_list, dim3(64, (unsigned)nf), dim3(256), 0, st, L);
   }
-  if ((A.nsr || A.nsc) && blas && !getenv("TN93G_NO_BLAS") && spec_gemm(c, J, st)) return;
+  if ((A.nsr || A.nsc) && blas && use_gemm_tables(c, J) && spec_gemm(c, J, st)) return;
   if (A.nsr)
     hipLaunchKernelGGL(k_spec16, dim3((unsigned)((A.ncols + TJ - 1) / TJ), (unsigned)((A.nsr + TI - 1) / TI)),
                        dim3(256), 0, st, A, A.spec_rows, A.nsr, A.c0, A.ncols, const_cast<uint32_t *>(A.tab_row));

@@ -273,9 +273,9 @@ def _same_summary(r1, r2):
 
 @pytest.mark.parametrize("kind", ["triangle", "cross", "cross_cols"])
 def test_gemm_tables_equal_kernel_tables(kind):
-    """A single run's special-sequence tables as int8 GEMMs (rocBLAS) against the same tables from k_spec16
-    (TN93G_NO_BLAS=1), the GEMM run first in a fresh context (its tables are not left over from another run):
-    triangle, -s with special sequences on both sides, and -s with special sequences among the columns only."""
+    """A single run's special-sequence tables as int8 GEMMs (rocBLAS, TN93G_TABLES=blas) against the same tables from
+    k_spec16 (TN93G_TABLES=kernel), the GEMM run first in a fresh context (its tables are not left over from another
+    run): triangle, -s with special sequences on both sides, and -s with special sequences among the columns only."""
     codes = gen.c2s(200, seed=5)
     kw = dict(mode=T.RESOLVE, threshold=0.05, min_overlap=5000, resolve_fraction=0.05)
     if kind == "cross":
@@ -289,12 +289,14 @@ def test_gemm_tables_equal_kernel_tables(kind):
     eng = T.Engine(0)
     try:
         s = eng.upload(codes)
-        got = eng.run(s, **kw)
-        os.environ["TN93G_NO_BLAS"] = "1"
-        try:
-            want = eng.run(s, **kw)
-        finally:
-            del os.environ["TN93G_NO_BLAS"]
+        res = {}
+        for how in ["blas", "kernel"]:
+            os.environ["TN93G_TABLES"] = how
+            try:
+                res[how] = eng.run(s, **kw)
+            finally:
+                del os.environ["TN93G_TABLES"]
+        got, want = res["blas"], res["kernel"]
         s.free()
     finally:
         eng.close()
