@@ -50,6 +50,8 @@ extern "C" {
 #define TN93G_BACKEND_AUTO 0
 #define TN93G_BACKEND_POPC 1 /* VALU popcount over bit planes */
 #define TN93G_BACKEND_MFMA 2 /* int8 one-hot contraction on MFMA, int32 accumulation */
+#define TN93G_BACKEND_GEMM 3 /* reported only (backend_used): the clean/clean counts of a long dense-N alignment as
+                                int8 GEMMs (rocBLAS) on the matrix cores, chosen by the library for single runs */
 
 /* run flags */
 #define TN93G_FLAG_EXACT 1u      /* resolve every uncertain log on the host (libm) instead of bracketing it   */

@@ -16,6 +16,7 @@ CLI_PATH = PKG / "bin" / "tn93g"
 RESOLVE, AVERAGE, SKIP, GAPMM, SUBSET = 0, 1, 2, 3, 4
 TRIANGLE, CROSS, FST, APPEND = 0, 1, 2, 3
 BACKEND_AUTO, BACKEND_POPC, BACKEND_MFMA = 0, 1, 2
+BACKEND_GEMM = 3  # reported in backend_used only: the counts of a long dense-N alignment as rocBLAS int8 GEMMs
 FLAG_EXACT, FLAG_ALL_SLOW, FLAG_AUTOTUNE = 1, 2, 4
 
 MODES = {"resolve": RESOLVE, "average": AVERAGE, "skip": SKIP, "gapmm": GAPMM}

@@ -335,6 +335,10 @@ struct RunArgs {
   int64_t cstride;             // words from one slot to the next: 4 x 64 with c16, else 7 x 64
   int c16;                     // 32 * Ls < 2^16: every x16 count fits 16 bits
   int64_t gx;                  // column tiles
+  // clean/clean counts from int8 GEMMs (the GEMM-count variant of the pair kernel): four [nrows][ncols] products
+  // gcs apart, element (i - r0) ncols + (j - c0): [A|G].[A|G], [C|T].[C|T], R.R and Y.Y over the sites
+  const uint32_t *gcnt;
+  int64_t gcs;
   int ti;                      // rows of the pair kernel's tiles (8, 16 or 32)
 };
 
@@ -1003,6 +1007,7 @@ __device__ inline RowCodes<RP> row_codes(const uint8_t *p)
 //   sTabC[c][x]  a column event of code c facing the row's code x, zero where x is ambiguous (that site is counted
 //                as the row's event).
 constexpr int TABW = NCODE + 1, EVNONE = NCODE, EVB = 4;
+constexpr int BACKEND_GEMMC = 3;  // the pair kernel's clean/clean counts from int8 GEMMs (rocBLAS)
 
 // Fills the LDS event tables from one [NCODE][NCODE] table of packed entries (entry [row code][column code]).
 // With `gaps` (GAPMM), gap sites are events too, so a column event facing a row gap is the row's event.  With
@@ -1215,6 +1220,8 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
   const int64_t i0 = A.r0 + by * TI_, j0 = c0a + bx * TJ;
   if (A.triangle && j0 + TJ - 1 <= i0) return;  // no pair j > i in this tile
   constexpr bool MF = BACKEND == TN93G_BACKEND_MFMA;
+  constexpr bool GC = BACKEND == BACKEND_GEMMC;  // clean/clean counts read from the GEMM products
+  static_assert(!GC || (MARG && NP && FAST && RP == 4), "the GEMM-count variant: fast dense-N marginal form");
   constexpr bool SR = NP;  // dense-N sets (long N-masked alignments): the rows' words through scalar loads
   __shared__ unsigned lslow;
   __shared__ unsigned long long sbase;
@@ -1230,7 +1237,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
   __shared__ unsigned long long lhist[FAST ? 1 : 3 * 200];
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   StageRegs<NP, TI_, SR> stg;  // the first chunk's planes, loaded before everything else
-  if constexpr (!MF)
+  if constexpr (!MF && !GC)
     if (!A.all_slow) stage_load<NP, TI_, SR>(A, i0, j0, 0, (int)min((int64_t)KW, A.W), tid, stg);
   const uint4 *tabs = A.evtab + (MARG ? 2 * NCODE * NCODE : 0);  // [main, AVERAGE], plain or folded
   load_event_tables(tabs, sTabR, sTabC, tid, A.gapmm, A.nplane);
@@ -1302,7 +1309,22 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
       }
     }
   }
-  if constexpr (!MF) {
+  if constexpr (GC) {
+    // DG = AA + CC + GG + TT, TS = (RR + YY) - DG, CT = YY - (CC + TT): the marginal form's three counts
+    // (a triangle's products hold the pairs j > i only)
+    if (do_fast && j >= A.c0 && j < A.c1)
+#pragma unroll
+      for (int a = 0; a < RP; a++) {
+        const int64_t i = ib + a;
+        if (i >= A.r1 || (A.triangle && j <= i)) continue;
+        const size_t o = (size_t)(i - A.r0) * (size_t)A.ncols + (size_t)(j - A.c0);
+        const uint32_t p1 = A.gcnt[o], p2 = A.gcnt[o + A.gcs], p3 = A.gcnt[o + 2 * A.gcs], p4 = A.gcnt[o + 3 * A.gcs];
+        cnt[a].DG = p1 + p2;
+        cnt[a].TS = p3 + p4 - (p1 + p2);
+        cnt[a].CT = p4 - p2;
+      }
+  }
+  if constexpr (!MF && !GC) {
     if (do_fast) {
       // SR: the wave's rows through scalar loads, RG words at a time (words past the row's end read the next row or
       // the arrays' padding and are not used; row indices past the set read its last sequence, whose counts are
@@ -1443,7 +1465,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
   if (do_events) chunk_codes<TI_>(A, te, sEv, L.g, i0, j0, tid);
   if constexpr (MARG)
     if (do_fast) base_sums<RP>(A, sTB, tr, sRg, ib, wave, lane, j, h0, h1, S);
-  if constexpr (NP && MF)
+  if constexpr (NP && (MF || GC))
     if (do_fast && j < A.n) {
       // MFMA backend: the one-hot operands carry clean sites only, so the rows' N sites against the column (its
       // planes and its N sites) come from the rows' N runs, each a difference of the column's prefix counts
@@ -2009,6 +2031,41 @@ __global__ __launch_bounds__(256) void k_spec16(RunArgs A, const int64_t *__rest
           *reinterpret_cast<uint4 *>(T + (((size_t)sp * 4 + x) * (size_t)nx + (size_t)xl) * 4) =
               make_uint4(c[a][4 * x], c[a][4 * x + 1], c[a][4 * x + 2], c[a][4 * x + 3]);
     }
+}
+
+// Operands of the GEMM counts for sequences s0 .. s0 + nq (one thread per 16 sites of one sequence): X1 row q =
+// [A indicator | G indicator] (2 Ls bytes), X2 = [C | T], X3 = purine (A or G), X4 = pyrimidine (C or T); 0 at
+// every other code.  X1.X1 = AA + GG, X2.X2 = CC + TT, X3.X3 = RR, X4.X4 = YY over the clean/clean sites.
+__global__ __launch_bounds__(256) void k_onehot_counts(const uint8_t *__restrict__ codes, int64_t Ls, int64_t s0,
+                                                       int64_t nq, int8_t *__restrict__ X1, int8_t *__restrict__ X2,
+                                                       int8_t *__restrict__ X3, int8_t *__restrict__ X4)
+{
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x, chunks = Ls / 16;
+  if (t >= nq * chunks) return;
+  const int64_t q = t / chunks, ch = t - q * chunks;
+  const uint4 v = *reinterpret_cast<const uint4 *>(codes + (s0 + q) * Ls + ch * 16);
+  const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+  uint32_t ind[4][4];  // [base][word]: byte y of word k is 1 where site 4k + y holds that base
+#pragma unroll
+  for (uint32_t b = 0; b < 4; b++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      uint32_t r = 0;
+#pragma unroll
+      for (int y = 0; y < 4; y++) r |= ((w[k] >> (8 * y) & 255u) == b ? 1u : 0u) << (8 * y);
+      ind[b][k] = r;
+    }
+  auto put = [&](int8_t *row, const uint32_t(&x)[4]) {
+    *reinterpret_cast<uint4 *>(row + ch * 16) = make_uint4(x[0], x[1], x[2], x[3]);
+  };
+  const uint32_t pur[4] = {ind[0][0] | ind[2][0], ind[0][1] | ind[2][1], ind[0][2] | ind[2][2], ind[0][3] | ind[2][3]};
+  const uint32_t pyr[4] = {ind[1][0] | ind[3][0], ind[1][1] | ind[3][1], ind[1][2] | ind[3][2], ind[1][3] | ind[3][3]};
+  put(X1 + q * 2 * Ls, ind[0]);
+  put(X1 + q * 2 * Ls + Ls, ind[2]);
+  put(X2 + q * 2 * Ls, ind[1]);
+  put(X2 + q * 2 * Ls + Ls, ind[3]);
+  put(X3 + q * Ls, pur);
+  put(X4 + q * Ls, pyr);
 }
 
 // One-hot rows of sequences for the special-sequence tables as an int8 GEMM: out[(q * 4 + b) * Kp + p] = 1 where
@@ -2897,6 +2954,9 @@ struct tn93g_ctx {
   hipEvent_t ev[5];  // pair kernel start, its end, run end, tail end, run start
   rocblas_handle blas = nullptr;  // single runs' special-sequence tables as int8 GEMMs (made on first use)
   int64_t table_runs = 0;         // single runs that built special-sequence tables
+  int64_t count_runs = 0;         // single runs that could take the GEMM counts
+  void *gemm_out = nullptr;       // the GEMM counts' four products
+  size_t gemm_out_n = 0;
   void *gemm_ops = nullptr;       // their one-hot operands
   size_t gemm_ops_n = 0;
   std::string err;
@@ -3019,6 +3079,7 @@ struct Job {
   uint64_t n_slow = 0, n_unc = 0, n_undecided = 0, n_linkrec = 0;
   unsigned max_region = 0, n_exact = 0;
   int64_t ra = -1, rb = -1;  // a part of a split run: its rows [ra, rb) (the whole run's rows when ra < 0)
+  bool gemm_counts = false;  // the pair kernel reads its clean/clean counts from int8 GEMMs (a single run)
 };
 
 // Pairs (i, j) with r0 <= i < r1, c0 <= j < c1 and j > i.
@@ -3351,6 +3412,7 @@ static int pairs_variant(const Job &J)
   const RunArgs &A = J.A;
   if (J.nrows <= 0 || A.ncols <= 0) return -1;
   if (J.P.backend == TN93G_BACKEND_MFMA) return (J.fast ? 8 : 9) + (A.nplane ? 2 : 0);
+  if (J.gemm_counts) return 12;
   const bool marg = J.B->tab_fold && !A.skip_events && J.sparse_gaps;
   return (J.fast ? 1 : 0) | (A.nplane ? 2 : 0) | (marg ? 4 : 0);
 }
@@ -3387,6 +3449,7 @@ static void launch_pairs(int v, int rp, dim3 g, hipStream_t st, const RunArgs &A
     case 9: hipLaunchKernelGGL((k_pairs<MF, false, false, false, 4>), g, dim3(256), 0, st, A); break;
     case 10: hipLaunchKernelGGL((k_pairs<MF, true, true, false, 4>), g, dim3(256), 0, st, A); break;
     case 11: hipLaunchKernelGGL((k_pairs<MF, false, true, false, 4>), g, dim3(256), 0, st, A); break;
+    case 12: hipLaunchKernelGGL((k_pairs<BACKEND_GEMMC, true, true, true, 4>), g, dim3(256), 0, st, A); break;
     default: break;
   }
 }
@@ -3535,6 +3598,95 @@ static bool spec_gemm(tn93g_ctx *c, const Job &J, hipStream_t st)
   return true;
 }
 
+// The pair kernel's clean/clean counts as four int8 GEMMs (rocBLAS) over one-hot operands of the run's sequences
+// (k_onehot_counts): product k is [ncols][nrows]-shaped, entry (j - c0) + (i - r0) ncols the dot product of column
+// j's and row i's operand rows, so that a wave's row reads its 64 columns' counts contiguously.  The rows and columns
+// of a run lie in [lo, hi), whose operands are built once.  A triangle's products are computed in blocks of rows, each
+// against the columns from the block's first row on.  False, with nothing of the counts enqueued, when memory or rocBLAS is not available: the
+// pair kernel then counts with popcount.
+static bool gemm_counts(tn93g_ctx *c, Job &J, hipStream_t st)
+{
+  const StreamFormatGuard keep_format;
+  RunArgs &A = J.A;
+  const int64_t lo = std::min(A.r0, A.c0), hi = std::max(A.r1, A.c1), nq = hi - lo, Ls = A.Ls;
+  if (2 * Ls >= (1ll << 31) || A.ncols >= (1ll << 31) || J.nrows >= (1ll << 31)) return false;
+  const size_t ops = (size_t)nq * 6 * (size_t)Ls, out = (size_t)A.ncols * (size_t)J.nrows * 16;
+  auto grow = [](void *&p, size_t &n, size_t want) {
+    if (n >= want) return true;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (hipMalloc(&p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      p = nullptr;
+      return false;
+    }
+    n = want;
+    return true;
+  };
+  if (!grow(c->gemm_ops, c->gemm_ops_n, ops) || !grow(c->gemm_out, c->gemm_out_n, out)) return false;
+  if (!c->blas && rocblas_create_handle(&c->blas) != rocblas_status_success) {
+    c->blas = nullptr;
+    return false;
+  }
+  if (rocblas_set_stream(c->blas, st) != rocblas_status_success) return false;
+  int8_t *X[4];
+  X[0] = (int8_t *)c->gemm_ops;
+  X[1] = X[0] + nq * 2 * Ls;
+  X[2] = X[1] + nq * 2 * Ls;
+  X[3] = X[2] + nq * Ls;
+  const int64_t K[4] = {2 * Ls, 2 * Ls, Ls, Ls};
+  hipLaunchKernelGGL(k_onehot_counts, dim3((unsigned)((nq * (Ls / 16) + 255) / 256)), dim3(256), 0, st, A.codes, Ls,
+                     lo, nq, X[0], X[1], X[2], X[3]);
+  uint32_t *C = (uint32_t *)c->gemm_out;
+  const int64_t gcs = A.ncols * J.nrows;
+  const int32_t one = 1, zero = 0;
+  // a triangle in blocks of rows, each against the columns from its first row on: (nb + 1) / 2nb of the square
+  int nb = A.triangle ? (int)std::min<int64_t>(std::max<int64_t>(J.nrows / 4096, 1), 8) : 1;
+  if (const char *e = getenv("TN93G_GEMM_BLOCKS")) nb = std::max(1, atoi(e));
+  for (int b = 0; b < nb; b++) {
+    const int64_t i0 = A.r0 + J.nrows * b / nb, i1 = A.r0 + J.nrows * (b + 1) / nb;
+    const int64_t cs = A.triangle ? std::min(std::max(A.c0, i0), A.c1) : A.c0;
+    if (i1 <= i0 || cs >= A.c1) continue;
+    for (int k = 0; k < 4; k++)
+      if (rocblas_gemm_ex(c->blas, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)(A.c1 - cs),
+                          (rocblas_int)(i1 - i0), (rocblas_int)K[k], &one, X[k] + (cs - lo) * K[k],
+                          rocblas_datatype_i8_r, (rocblas_int)K[k], X[k] + (i0 - lo) * K[k], rocblas_datatype_i8_r,
+                          (rocblas_int)K[k], &zero, C + k * gcs + (i0 - A.r0) * A.ncols + (cs - A.c0),
+                          rocblas_datatype_i32_r, (rocblas_int)A.ncols,
+                          C + k * gcs + (i0 - A.r0) * A.ncols + (cs - A.c0), rocblas_datatype_i32_r,
+                          (rocblas_int)A.ncols, rocblas_datatype_i32_r, rocblas_gemm_algo_standard, 0,
+                          0) != rocblas_status_success)
+        return false;
+  }
+  A.gcnt = C;
+  A.gcs = gcs;
+  return true;
+}
+
+// Whether a single run's pair kernel takes the GEMM counts: the fast dense-N marginal form of long alignments (the
+// counts of 32 sites cost the popcount form 9 VALU operations per pair, 7e-15 s per pair and site; the six one-hot
+// units of the products 12 int8 operations, ~5e-15 s at the ~2.4 POPS rocBLAS reaches on these shapes, with a
+// triangle's blocks ~0.6 of the square), within 32 GB of operands and products.  From a context's second such run
+// on (rocBLAS's set-up costs ~0.15 s per process, more than the products save on one 20,000 x 30,000-site triangle),
+// or at once past 1e13 word pairs, unless the caller asked for a backend.  TN93G_COUNTS=gemm|popc forces one.
+static bool use_gemm_counts(tn93g_ctx *c, const Job &J, int asked)
+{
+  const char *e = getenv("TN93G_COUNTS");
+  if (e && !strcmp(e, "popc")) return false;
+  const bool force = e && !strcmp(e, "gemm");
+  const RunArgs &A = J.A;
+  if (J.gemm_counts || pairs_variant(J) != 7 || A.all_slow) return false;
+  if (asked != TN93G_BACKEND_AUTO && !force) return false;  // a backend asked for by name is kept
+  const int64_t lo = std::min(A.r0, A.c0), hi = std::max(A.r1, A.c1);
+  const double bytes = (double)(hi - lo) * 6.0 * (double)A.Ls + 16.0 * (double)A.ncols * (double)J.nrows;
+  if (bytes > 32e9) return false;
+  if (force) return true;
+  if (A.W < 256) return false;
+  const bool first = c->count_runs++ == 0;
+  return c->blas || !first || (double)J.npairs * (double)A.W > 1e13;
+}
+
 // Whether a single run's tables come from rocBLAS: setting the library up costs about 0.15 s once per process, so a
 // context takes the GEMM from its second run with tables on (a process that runs many), or at once when k_spec16
 // would take longer than that (about 1e12 word pairs per second).  TN93G_TABLES=blas|kernel forces one (tests).
@@ -3573,6 +3725,7 @@ static void job_enqueue_pre(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, b
 static void job_enqueue_pairs(tn93g_ctx *c, Job &J, bool timing, hipStream_t st)
 {
   if (timing) (void)hipEventRecord(c->ev[0], st);
+  if (J.gemm_counts && !gemm_counts(c, J, st)) J.gemm_counts = false;  // (the products are part of the pair time)
   const int v = pairs_variant(J);
   if (v >= 0) launch_pairs(v, J.rp, dim3((unsigned)J.gx, (unsigned)J.gy), st, J.A);
   if (timing) (void)hipEventRecord(c->ev[1], st);
@@ -3689,7 +3842,9 @@ static void job_finish_common(const Job &J, const unsigned long long *hrep, tn93
   R->n_replayed = (int64_t)J.n_slow;
   R->n_uncertain = (int64_t)J.n_unc;
   R->n_host_exact = J.n_exact;
-  R->backend_used = J.P.backend == TN93G_BACKEND_AUTO ? TN93G_BACKEND_POPC : J.P.backend;
+  R->backend_used = J.gemm_counts                         ? TN93G_BACKEND_GEMM
+                    : J.P.backend == TN93G_BACKEND_AUTO ? TN93G_BACKEND_POPC
+                                                        : J.P.backend;
   R->folded_columns = J.A.kt;
   R->tile_rows_used = 4 * J.rp;
   // self_v_self histogram entries (swifttn93.cpp:2406-2415); a split run adds them once, for all its parts
@@ -4060,6 +4215,7 @@ void tn93g_close(tn93g_ctx *c)
   if (c->fork) (void)hipEventDestroy(c->fork);
   if (c->blas) (void)rocblas_destroy_handle(c->blas);
   if (c->gemm_ops) (void)hipFree(c->gemm_ops);
+  if (c->gemm_out) (void)hipFree(c->gemm_out);
   if (c->pin.p) (void)hipHostFree(c->pin.p);
   if (c->pin_links.p) (void)hipHostFree(c->pin_links.p);
   if (c->pin_up.p) (void)hipHostFree(c->pin_up.p);
@@ -4616,6 +4772,10 @@ static int run_one(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, int6
   int rc = job_prepare(c, J);
   if (rc) return rc;
   if ((rc = autotune(c, J))) return rc;
+  if (use_gemm_counts(c, J, P->backend)) {
+    if ((rc = job_set_shape(c, J, TN93G_BACKEND_POPC, 4))) return rc;
+    J.gemm_counts = true;
+  }
   job_enqueue(c, J, true, c->stream, 0, true, true);
   HIPCHK(c, hipGetLastError());
   // one synchronisation: the counters, and for the fast path the row accumulators, the histogram and as many link

@@ -239,11 +239,15 @@ C2_CONFIGS = [
 ]
 
 
-@pytest.mark.parametrize("backend", ["popc", "mfma"])
+@pytest.mark.parametrize("backend", ["popc", "mfma", "gemm"])
 def test_long_alignment_c2s_vs_oracle(backend, tmp_path, monkeypatch):
     """SARS-CoV-2-shaped long alignment (300 x 29,903, N-masked blocks of 100-2,000 columns) against the oracle,
-    on the popcount backend and on the int8 MFMA backend (clean/clean counts on the matrix unit, N from runs)."""
-    monkeypatch.setenv("TN93G_BACKEND", backend)
+    on the popcount backend, on the int8 MFMA backend (clean/clean counts on the matrix unit, N from runs) and with
+    the clean/clean counts as rocBLAS int8 GEMMs (TN93G_COUNTS=gemm)."""
+    if backend == "gemm":
+        monkeypatch.setenv("TN93G_COUNTS", "gemm")
+    else:
+        monkeypatch.setenv("TN93G_BACKEND", backend)
     codes = gen.c2s(300)
     fa = tmp_path / "in.fa"
     gen.write_fasta(fa, codes)
@@ -307,6 +311,43 @@ def test_gemm_tables_equal_kernel_tables(kind):
     for f in ["weighted", "max_lo", "max_hi", "found_links", "skipped", "n_replayed"]:
         assert got[f] == want[f], (f, got[f], want[f])
     assert abs(got["mean_lo"][0] - want["mean_lo"][0]) <= 1e-12 * abs(want["mean_lo"][0])
+
+
+@pytest.mark.parametrize("kind", ["triangle", "cross", "cross_wide"])
+@pytest.mark.parametrize("mode", ["resolve", "average"])
+def test_gemm_counts_equal_popcount(kind, mode):
+    """A single run's clean/clean counts as rocBLAS int8 GEMMs (TN93G_COUNTS=gemm) against the popcount pair kernel
+    (TN93G_COUNTS=popc) on a dense-N long alignment, the GEMM run first in a fresh context: triangle (its products in
+    three blocks of rows), -s with fewer rows than columns, and -s with more rows than columns (the products' operands
+    span both ranges)."""
+    codes = gen.c2s(180, seed=7)
+    kw = dict(mode=T.RESOLVE if mode == "resolve" else T.AVERAGE, threshold=0.05 if mode == "resolve" else 1.0,
+              min_overlap=5000, resolve_fraction=0.05 if mode == "resolve" else 1.0)
+    if kind == "cross":
+        kw.update(kind=T.CROSS, n1=50)
+    elif kind == "cross_wide":
+        kw.update(kind=T.CROSS, n1=130)
+    eng = T.Engine(0)
+    try:
+        s = eng.upload(codes)
+        res = {}
+        for how in ["gemm", "popc"]:
+            os.environ["TN93G_COUNTS"] = how
+            os.environ["TN93G_GEMM_BLOCKS"] = "3"  # a triangle's products in three blocks of rows
+            try:
+                res[how] = eng.run(s, **kw)
+            finally:
+                del os.environ["TN93G_COUNTS"], os.environ["TN93G_GEMM_BLOCKS"]
+        s.free()
+    finally:
+        eng.close()
+    got, want = res["gemm"], res["popc"]
+    assert got["backend_used"] == T.BACKEND_GEMM and want["backend_used"] == T.BACKEND_POPC
+    _same_summary(got, want)
+    for f in ["hist", "link_row", "link_col", "link_dist"]:
+        assert np.array_equal(np.asarray(got[f]), np.asarray(want[f])), f
+    print(kind, mode, "links", len(want["link_dist"]), flush=True)
+    assert len(want["link_dist"]) > 0
 
 
 def test_batch_equals_single(engine):
