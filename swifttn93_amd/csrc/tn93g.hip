@@ -294,6 +294,7 @@ struct RunArgs {
   int64_t r0, r1, c0, c1, ncols, n1;
   int triangle, fst, gapmm, skip_events, avg_fallback, exact, all_slow;
   int gcert;                   // the event tables' nf fields count the sites of order-dependent pairs exactly
+  double gE;                   // with gcert: twice the bound on |tng_fp - tng| below, (16 L^2 + 64 L) 2^-52
   double rf, thr, mind, window;
   int64_t min_overlap;
   const uint4 *evtab;          // [4][18*18] packed event entries: main mode, AVERAGE (the -g fallback), and the same
@@ -1539,8 +1540,7 @@ __device__ __attribute__((always_inline)) inline void pairs_tile(const RunArgs &
       // additions); when tng is farther than that above the overlap and tng rf farther than that from amb, tn93's
       // -g test is decided: true => the replay starts in AVERAGE (margins doubled, the product's rounding included)
       if (valid && acc[a].fl && A.gcert) {
-        const double Lf = 32.0 * (double)A.W + (double)A.kt;
-        const double E = (16.0 * Lf * Lf + 64.0 * Lf) * 0x1p-52;  // twice the bound on |tng_fp - tng|
+        const double E = A.gE;  // twice the bound on |tng_fp - tng|
         const double M = E * A.rf + 2.0 * tng * A.rf * 0x1p-52;
         avgr[a] = tng - E > (double)A.min_overlap && tng * A.rf + M < (double)tot[a].amb;
       }
@@ -2616,6 +2616,19 @@ __global__ void k_copy_ranges(const CopyRange *__restrict__ r, int nr)
   }
 }
 
+// A single run's read-back (at most 4 ranges, passed by value) written by the kernel straight into the caller's
+// pinned host buffer (its device mapping): one launch after the run's kernels instead of a copy per range, each of which
+// costs a switch to the copy engine and back (~10 us).  Grid row y copies range y.
+struct CopyList {
+  CopyRange r[4];
+};
+__global__ __launch_bounds__(256) void k_copy_list(CopyList L)
+{
+  const CopyRange g = L.r[blockIdx.y];
+  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < g.n; q += (int64_t)gridDim.x * blockDim.x)
+    g.dst[q] = g.src[q];
+}
+
 // The device's constant-rate wall clock into *t (one lane): stamps inside a captured graph, where events time nothing.
 __global__ void k_stamp(unsigned long long *t)
 {
@@ -3281,6 +3294,10 @@ static int job_prepare(tn93g_ctx *c, Job &J, size_t *dry_bytes = nullptr, Arena 
     B->tab_gcert = gcert;
   }
   A.gcert = B->tab_gcert ? 1 : 0;
+  {
+    const double Lf = 32.0 * (double)A.W + (double)A.kt;
+    A.gE = (16.0 * Lf * Lf + 64.0 * Lf) * 0x1p-52;
+  }
   A.evtab = (const uint4 *)B->evtab.p;
   A.rtab = (const ReplayEnt *)B->rtab.p;
   // buffers
@@ -3830,6 +3847,33 @@ static int job_enqueue_fast_readback(tn93g_ctx *c, Job &J, char *pin, size_t nl)
   HIPCHK(c, hipMemcpyAsync(pin + F.off_h, J.B->hist.p, 600 * 8, hipMemcpyDeviceToHost, c->stream));
   if (nl) HIPCHK(c, hipMemcpyAsync(pin + F.off_lk, J.B->links.p, nl * sizeof(LinkRec), hipMemcpyDeviceToHost,
                                    c->stream));
+  return 0;
+}
+
+// A single run's counters and, for the fast path, its row accumulators, histogram and first nl link records, read
+// back by one k_copy_list launch into the pinned buffer's device mapping; the copies of job_enqueue_counters and
+// job_enqueue_fast_readback when the buffer has no device mapping.
+static int job_enqueue_readback(tn93g_ctx *c, Job &J, char *pin, size_t nl)
+{
+  char *d = nullptr;
+  if (hipHostGetDevicePointer((void **)&d, pin, 0) != hipSuccess || !d) {
+    (void)hipGetLastError();
+    int rc = job_enqueue_counters(c, J, pin);
+    if (!rc && J.fast) rc = job_enqueue_fast_readback(c, J, pin + COUNTER_STRIDE, nl);
+    return rc;
+  }
+  CopyList L{};
+  int n = 0;
+  L.r[n++] = CopyRange{(const uint2 *)J.B->counters.p, (uint2 *)d, (int64_t)(STRIPE_BYTES / 8)};
+  if (J.fast) {
+    const FastLayout F = fast_layout(J, nl);
+    char *f = d + COUNTER_STRIDE;
+    L.r[n++] = CopyRange{(const uint2 *)J.B->rowout.p, (uint2 *)(f + F.off_ra), (int64_t)(J.nr1 * sizeof(RowAcc) / 8)};
+    L.r[n++] = CopyRange{(const uint2 *)J.B->hist.p, (uint2 *)(f + F.off_h), 600};
+    if (nl) L.r[n++] = CopyRange{(const uint2 *)J.B->links.p, (uint2 *)(f + F.off_lk), (int64_t)(nl * sizeof(LinkRec) / 8)};
+  }
+  hipLaunchKernelGGL(k_copy_list, dim3(64, (unsigned)n), dim3(256), 0, c->stream, L);
+  HIPCHK(c, hipGetLastError());
   return 0;
 }
 
@@ -4784,8 +4828,7 @@ static int run_one(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, int6
   const size_t fbytes = J.fast ? fast_layout(J, spec).bytes : 0;
   if (ensure_pinned(c, c->pin, COUNTER_STRIDE + fbytes)) return TN93G_E_NOMEM;
   char *pin = (char *)c->pin.p;
-  if ((rc = job_enqueue_counters(c, J, pin))) return rc;
-  if (J.fast && (rc = job_enqueue_fast_readback(c, J, pin + COUNTER_STRIDE, spec))) return rc;
+  if ((rc = job_enqueue_readback(c, J, pin, spec))) return rc;
   HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   job_parse_counters(J, pin);
