@@ -2619,14 +2619,31 @@ __global__ void k_copy_ranges(const CopyRange *__restrict__ r, int nr)
 // A single run's read-back (at most 4 ranges, passed by value) written by the kernel straight into the caller's
 // pinned host buffer (its device mapping): one launch after the run's kernels instead of a copy per range, each of which
 // costs a switch to the copy engine and back (~10 us).  Grid row y copies range y.
+// With nlk > 0, the last grid row writes the first nlk link records straight into a result's five link arrays (a
+// pinned block's device mapping).
 struct CopyList {
   CopyRange r[4];
+  const LinkRec *lk;
+  int64_t *row, *col;
+  double *d, *lo, *hi;
+  int64_t nlk;
 };
-__global__ __launch_bounds__(256) void k_copy_list(CopyList L)
+__global__ __launch_bounds__(256) void k_copy_list(CopyList L, int nr)
 {
-  const CopyRange g = L.r[blockIdx.y];
-  for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < g.n; q += (int64_t)gridDim.x * blockDim.x)
-    g.dst[q] = g.src[q];
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, step = (int64_t)gridDim.x * blockDim.x;
+  if ((int)blockIdx.y < nr) {
+    const CopyRange g = L.r[blockIdx.y];
+    for (int64_t q = t0; q < g.n; q += step) g.dst[q] = g.src[q];
+    return;
+  }
+  for (int64_t q = t0; q < L.nlk; q += step) {
+    const LinkRec r = L.lk[q];
+    L.row[q] = r.i;
+    L.col[q] = r.j;
+    L.d[q] = r.d;
+    L.lo[q] = r.lo;
+    L.hi[q] = r.hi;
+  }
 }
 
 // The device's constant-rate wall clock into *t (one lane): stamps inside a captured graph, where events time nothing.
@@ -3853,11 +3870,14 @@ static int job_enqueue_fast_readback(tn93g_ctx *c, Job &J, char *pin, size_t nl)
 // A single run's counters and, for the fast path, its row accumulators, histogram and first nl link records, read
 // back by one k_copy_list launch into the pinned buffer's device mapping; the copies of job_enqueue_counters and
 // job_enqueue_fast_readback when the buffer has no device mapping.
-static int job_enqueue_readback(tn93g_ctx *c, Job &J, char *pin, size_t nl)
+// With R (a result whose link block is pinned, holding at least nl links), the links go straight into R's arrays
+// instead of the pinned buffer.
+static int job_enqueue_readback(tn93g_ctx *c, Job &J, char *pin, size_t nl, tn93g_result *R = nullptr)
 {
   char *d = nullptr;
   if (hipHostGetDevicePointer((void **)&d, pin, 0) != hipSuccess || !d) {
     (void)hipGetLastError();
+    if (R) return TN93G_E_HIP;  // (the caller checked R's mapping first)
     int rc = job_enqueue_counters(c, J, pin);
     if (!rc && J.fast) rc = job_enqueue_fast_readback(c, J, pin + COUNTER_STRIDE, nl);
     return rc;
@@ -3870,9 +3890,22 @@ static int job_enqueue_readback(tn93g_ctx *c, Job &J, char *pin, size_t nl)
     char *f = d + COUNTER_STRIDE;
     L.r[n++] = CopyRange{(const uint2 *)J.B->rowout.p, (uint2 *)(f + F.off_ra), (int64_t)(J.nr1 * sizeof(RowAcc) / 8)};
     L.r[n++] = CopyRange{(const uint2 *)J.B->hist.p, (uint2 *)(f + F.off_h), 600};
-    if (nl) L.r[n++] = CopyRange{(const uint2 *)J.B->links.p, (uint2 *)(f + F.off_lk), (int64_t)(nl * sizeof(LinkRec) / 8)};
+    if (nl && R) {
+      int64_t *row = nullptr;
+      HIPCHK(c, hipHostGetDevicePointer((void **)&row, R->link_row, 0));
+      const int64_t cap = R->link_col - R->link_row;  // the block's capacity (arrays of cap entries, in order)
+      L.lk = (const LinkRec *)J.B->links.p;
+      L.row = row;
+      L.col = row + cap;
+      L.d = (double *)(row + 2 * cap);
+      L.lo = (double *)(row + 3 * cap);
+      L.hi = (double *)(row + 4 * cap);
+      L.nlk = (int64_t)nl;
+    } else if (nl) {
+      L.r[n++] = CopyRange{(const uint2 *)J.B->links.p, (uint2 *)(f + F.off_lk), (int64_t)(nl * sizeof(LinkRec) / 8)};
+    }
   }
-  hipLaunchKernelGGL(k_copy_list, dim3(64, (unsigned)n), dim3(256), 0, c->stream, L);
+  hipLaunchKernelGGL(k_copy_list, dim3(64, (unsigned)n + (L.nlk ? 1u : 0u)), dim3(256), 0, c->stream, L, n);
   HIPCHK(c, hipGetLastError());
   return 0;
 }
@@ -3899,34 +3932,45 @@ static void job_finish_common(const Job &J, const unsigned long long *hrep, tn93
     }
 }
 
-// The five link arrays of a result are one block, with its capacity in a 64-byte header.  Freed blocks are kept
-// (at most LINK_CACHE of them) and handed out again, so a caller running the same workload repeatedly does not map
-// and fault in fresh pages for every result.
+// The five link arrays of a result are one block, with its capacity and kind (0 host memory, 1 pinned) in a 64-byte
+// header.  Freed blocks are kept (at most LINK_CACHE of each kind) and handed out again, so a caller running the same
+// workload repeatedly does not map and fault in fresh pages for every result.  Single fast-path runs take pinned
+// blocks, into which the read-back kernel writes the links directly (no copy on the host).
 constexpr int LINK_CACHE = 4;
 static std::mutex g_link_mu;
-static std::vector<void *> g_link_cache;
+static std::vector<void *> g_link_cache[2];
 
-static int alloc_links(tn93g_result *R, size_t nl)
+static int alloc_links(tn93g_result *R, size_t nl, size_t cap_want = 0, int kind = 0)
 {
   R->n_links = (int64_t)nl;
-  const size_t cap = std::max<size_t>(nl, 1), bytes = 64 + 5 * cap * 8;
+  const size_t cap = std::max<size_t>({nl, cap_want, 1}), bytes = 64 + 5 * cap * 8;
   char *blk = nullptr;
   {
     std::lock_guard<std::mutex> lk(g_link_mu);
-    for (size_t k = 0; k < g_link_cache.size(); k++) {
+    auto &cache = g_link_cache[kind];
+    for (size_t k = 0; k < cache.size(); k++) {
       size_t have;
-      memcpy(&have, g_link_cache[k], sizeof have);
+      memcpy(&have, cache[k], sizeof have);
       if (have >= cap && have <= 2 * cap + 4096) {
-        blk = (char *)g_link_cache[k];
-        g_link_cache.erase(g_link_cache.begin() + (long)k);
+        blk = (char *)cache[k];
+        cache.erase(cache.begin() + (long)k);
         break;
       }
     }
   }
   if (!blk) {
-    blk = (char *)malloc(bytes);
+    if (kind) {
+      if (hipHostMalloc((void **)&blk, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        blk = nullptr;
+      }
+    } else {
+      blk = (char *)malloc(bytes);
+    }
     if (!blk) return TN93G_E_NOMEM;
+    const size_t k64 = (size_t)kind;
     memcpy(blk, &cap, sizeof cap);
+    memcpy(blk + 8, &k64, sizeof k64);
   }
   size_t have;
   memcpy(&have, blk, sizeof have);
@@ -3943,14 +3987,19 @@ static void free_links(int64_t *link_row)
 {
   if (!link_row) return;
   char *blk = (char *)link_row - 64;
+  size_t kind;
+  memcpy(&kind, blk + 8, sizeof kind);
   {
     std::lock_guard<std::mutex> lk(g_link_mu);
-    if (g_link_cache.size() < LINK_CACHE) {
-      g_link_cache.push_back(blk);
+    if (g_link_cache[kind].size() < LINK_CACHE) {
+      g_link_cache[kind].push_back(blk);
       return;
     }
   }
-  free(blk);
+  if (kind)
+    (void)hipHostFree(blk);
+  else
+    free(blk);
 }
 
 // Enclosure of tn93's mean (swifttn93.cpp:2594-2602, 2839-2848), the maximum and the integer sums, from the fast
@@ -4030,9 +4079,13 @@ static int job_fast_result(const Job &J, const char *pin, const LinkRec *lk, tn9
   const FastLayout F = fast_layout(J, 0);
   const RowAcc *ra = (const RowAcc *)(pin + F.off_ra);
   const size_t nl = J.n_linkrec;
-  // the records arrive in tn93's emission order (rows ascending, columns ascending: k_link_order)
-  if (alloc_links(R, nl)) return TN93G_E_NOMEM;
-  for (size_t k = 0; k < nl; k++) {
+  // the records arrive in tn93's emission order (rows ascending, columns ascending: k_link_order); lk null: the
+  // read-back kernel wrote them into R's arrays already
+  if (!lk)
+    R->n_links = (int64_t)nl;
+  else if (alloc_links(R, nl))
+    return TN93G_E_NOMEM;
+  for (size_t k = 0; lk && k < nl; k++) {
     const LinkRec &q = lk[k];
     R->link_row[k] = q.i;
     R->link_col[k] = q.j;
@@ -4828,31 +4881,60 @@ static int run_one(tn93g_ctx *c, const tn93g_set *S, const tn93g_params *P, int6
   const size_t fbytes = J.fast ? fast_layout(J, spec).bytes : 0;
   if (ensure_pinned(c, c->pin, COUNTER_STRIDE + fbytes)) return TN93G_E_NOMEM;
   char *pin = (char *)c->pin.p;
-  if ((rc = job_enqueue_readback(c, J, pin, spec))) return rc;
-  HIPCHK(c, hipEventRecord(c->ev[2], c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // a single fast run's links go straight into its result's arrays: a pinned link block of spec entries
+  tn93g_result *R = (tn93g_result *)calloc(1, sizeof(tn93g_result));
+  if (!R) return TN93G_E_NOMEM;
+  bool direct = false;
+  if (J.fast && spec && !sink && alloc_links(R, 0, spec, 1) == 0) {
+    void *dv = nullptr;
+    direct = hipHostGetDevicePointer(&dv, R->link_row, 0) == hipSuccess && dv;
+    if (!direct) {
+      (void)hipGetLastError();
+      free_links(R->link_row);
+      R->link_row = R->link_col = nullptr;
+      R->link_dist = R->link_dist_lo = R->link_dist_hi = nullptr;
+    }
+  }
+  auto retry = [&](const tn93g_params *Q) {
+    tn93g_result_free(R);
+    return run_one(c, S, Q, ra, rb, sink, out);
+  };
+  if ((rc = job_enqueue_readback(c, J, pin, spec, direct ? R : nullptr))) {
+    tn93g_result_free(R);
+    return rc;
+  }
+  hipError_t e = hipEventRecord(c->ev[2], c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    tn93g_result_free(R);
+    return c->fail(TN93G_E_HIP, fmt("run read-back: %s", hipGetErrorString(e)));
+  }
   job_parse_counters(J, pin);
   if (J.n_undecided && !J.A.exact) {
     // an output decision could not be certified from the enclosures: run again resolving logs on the host
     tn93g_params Q = *P;
     Q.flags |= TN93G_FLAG_EXACT;
-    return run_one(c, S, &Q, ra, rb, sink, out);
+    return retry(&Q);
   }
   if (J.fast && J.max_region > J.link_cap) {  // more links than a region holds: grow the regions and run again
     J.B->link_cap_hint = (int64_t)J.max_region;
-    return run_one(c, S, P, ra, rb, sink, out);
+    return retry(P);
   }
   if (J.A.exact && J.n_exact > J.exact_cap) {  // more uncertain pairs than the host-exact list holds: grow it
     J.B->exact_hint = (int64_t)J.n_exact + J.n_exact / 4;
-    return run_one(c, S, P, ra, rb, sink, out);
+    return retry(P);
   }
-  tn93g_result *R = (tn93g_result *)calloc(1, sizeof(tn93g_result));
-  if (!R) return TN93G_E_NOMEM;
   auto finish = [&]() -> int {
     if (J.fast) {
       const char *pf = pin + COUNTER_STRIDE;
       const LinkRec *lk = (const LinkRec *)(pf + fast_layout(J, 0).off_lk);
       J.B->link_spec = std::max<size_t>(4096, J.n_linkrec + J.n_linkrec / 4);
+      if (direct && J.n_linkrec <= spec) return job_fast_result(J, pf, nullptr, R, sink);
+      if (direct) {  // the pinned block is too small: the links are copied into a block of their size below
+        free_links(R->link_row);
+        R->link_row = R->link_col = nullptr;
+        R->link_dist = R->link_dist_lo = R->link_dist_hi = nullptr;
+      }
       if (J.n_linkrec > spec) {  // more links than copied: all of them again, into a buffer of their own
         if (ensure_pinned(c, c->pin_links, J.n_linkrec * sizeof(LinkRec))) return TN93G_E_NOMEM;
         HIPCHK(c, hipMemcpyAsync(c->pin_links.p, J.B->links.p, J.n_linkrec * sizeof(LinkRec),
