@@ -3885,10 +3885,13 @@ static int job_enqueue_readback(tn93g_ctx *c, Job &J, char *pin, size_t nl, tn93
   CopyList L{};
   int n = 0;
   L.r[n++] = CopyRange{(const uint2 *)J.B->counters.p, (uint2 *)d, (int64_t)(STRIPE_BYTES / 8)};
+  bool big_rows = false;  // row sums of more than 4 MB (large APPEND runs) go by the copy engine, which moves them faster
   if (J.fast) {
     const FastLayout F = fast_layout(J, nl);
     char *f = d + COUNTER_STRIDE;
-    L.r[n++] = CopyRange{(const uint2 *)J.B->rowout.p, (uint2 *)(f + F.off_ra), (int64_t)(J.nr1 * sizeof(RowAcc) / 8)};
+    big_rows = J.nr1 * sizeof(RowAcc) > ((size_t)4 << 20);
+    if (!big_rows)
+      L.r[n++] = CopyRange{(const uint2 *)J.B->rowout.p, (uint2 *)(f + F.off_ra), (int64_t)(J.nr1 * sizeof(RowAcc) / 8)};
     L.r[n++] = CopyRange{(const uint2 *)J.B->hist.p, (uint2 *)(f + F.off_h), 600};
     if (nl && R) {
       int64_t *row = nullptr;
@@ -3907,6 +3910,9 @@ static int job_enqueue_readback(tn93g_ctx *c, Job &J, char *pin, size_t nl, tn93
   }
   hipLaunchKernelGGL(k_copy_list, dim3(64, (unsigned)n + (L.nlk ? 1u : 0u)), dim3(256), 0, c->stream, L, n);
   HIPCHK(c, hipGetLastError());
+  if (big_rows)
+    HIPCHK(c, hipMemcpyAsync(pin + COUNTER_STRIDE + fast_layout(J, nl).off_ra, J.B->rowout.p, J.nr1 * sizeof(RowAcc),
+                             hipMemcpyDeviceToHost, c->stream));
   return 0;
 }
 
@@ -3933,41 +3939,52 @@ static void job_finish_common(const Job &J, const unsigned long long *hrep, tn93
 }
 
 // The five link arrays of a result are one block, with its capacity and kind (0 host memory, 1 pinned) in a 64-byte
-// header.  Freed blocks are kept (at most LINK_CACHE of each kind) and handed out again, so a caller running the same
+// header.  Freed blocks are kept (at most LINK_CACHE host blocks) and handed out again, so a caller running the same
 // workload repeatedly does not map and fault in fresh pages for every result.  Single fast-path runs take pinned
-// blocks, into which the read-back kernel writes the links directly (no copy on the host).
-constexpr int LINK_CACHE = 4;
+// blocks, into which the read-back kernel writes the links directly (no copy on the host): at most PIN_BLOCKS of them
+// exist, in use or kept, each with half again the capacity first asked for, and none is unpinned (pinning and
+// unpinning cost far more than the copy they save; unpinning may wait for the device); a run that finds none free
+// and large enough takes a host block and the copy.
+constexpr int LINK_CACHE = 4, PIN_BLOCKS = 8;
 static std::mutex g_link_mu;
 static std::vector<void *> g_link_cache[2];
+static int g_pin_blocks = 0;
 
 static int alloc_links(tn93g_result *R, size_t nl, size_t cap_want = 0, int kind = 0)
 {
   R->n_links = (int64_t)nl;
-  const size_t cap = std::max<size_t>({nl, cap_want, 1}), bytes = 64 + 5 * cap * 8;
+  size_t cap = std::max<size_t>({nl, cap_want, 1});
   char *blk = nullptr;
+  bool fresh = false;
   {
     std::lock_guard<std::mutex> lk(g_link_mu);
     auto &cache = g_link_cache[kind];
     for (size_t k = 0; k < cache.size(); k++) {
       size_t have;
       memcpy(&have, cache[k], sizeof have);
-      if (have >= cap && have <= 2 * cap + 4096) {
+      if (have >= cap && (kind || have <= 2 * cap + 4096)) {
         blk = (char *)cache[k];
         cache.erase(cache.begin() + (long)k);
         break;
       }
     }
+    if (!blk && kind) {
+      if (g_pin_blocks >= PIN_BLOCKS) return TN93G_E_NOMEM;  // every one in use or too small: a host block
+      cap += cap / 2;  // room for the next runs' growth
+      if (hipHostMalloc((void **)&blk, 64 + 5 * cap * 8) != hipSuccess) {
+        (void)hipGetLastError();
+        return TN93G_E_NOMEM;
+      }
+      g_pin_blocks++;
+      fresh = true;
+    }
   }
   if (!blk) {
-    if (kind) {
-      if (hipHostMalloc((void **)&blk, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        blk = nullptr;
-      }
-    } else {
-      blk = (char *)malloc(bytes);
-    }
+    blk = (char *)malloc(64 + 5 * cap * 8);
     if (!blk) return TN93G_E_NOMEM;
+    fresh = true;
+  }
+  if (fresh) {
     const size_t k64 = (size_t)kind;
     memcpy(blk, &cap, sizeof cap);
     memcpy(blk + 8, &k64, sizeof k64);
@@ -3991,15 +4008,12 @@ static void free_links(int64_t *link_row)
   memcpy(&kind, blk + 8, sizeof kind);
   {
     std::lock_guard<std::mutex> lk(g_link_mu);
-    if (g_link_cache[kind].size() < LINK_CACHE) {
+    if (kind || g_link_cache[0].size() < LINK_CACHE) {  // pinned blocks are all kept (at most PIN_BLOCKS)
       g_link_cache[kind].push_back(blk);
       return;
     }
   }
-  if (kind)
-    (void)hipHostFree(blk);
-  else
-    free(blk);
+  free(blk);
 }
 
 // Enclosure of tn93's mean (swifttn93.cpp:2594-2602, 2839-2848), the maximum and the integer sums, from the fast
