@@ -350,6 +350,35 @@ def test_gemm_counts_equal_popcount(kind, mode):
     assert len(want["link_dist"]) > 0
 
 
+def test_link_blocks_beyond_the_pinned_pool():
+    """Single fast-path runs write their links straight into pinned link blocks of the library (at most 8): twelve
+    results kept alive at once (the last ones on host blocks and the host's copy), then freed and the run repeated on
+    the kept blocks, and a run with more links than the previous one read back (its block too small): every result's
+    links and summary equal the first's, in a fresh context."""
+    codes = gen.c0(seed=31, N=700, L=900)
+    kw = dict(mode=T.RESOLVE, threshold=0.05, min_overlap=300, resolve_fraction=0.05)
+    eng = T.Engine(0)
+    try:
+        s = eng.upload(codes)
+        first = eng.run(s, **kw)
+        assert len(first["link_dist"]) > 4096  # more than a fresh context reads back with its counters
+        alive = [eng.run(s, **kw) for _ in range(12)]
+        for r in alive:
+            _same_summary(r, first)
+            assert np.array_equal(r["link_dist"], first["link_dist"])
+        del alive
+        again = [eng.run(s, **kw) for _ in range(3)]
+        for r in again:
+            _same_summary(r, first)
+        wide = eng.run(s, **dict(kw, threshold=0.2))  # more links than the blocks were sized for
+        want = eng.run(s, **dict(kw, threshold=0.2, flags=T.FLAG_EXACT))
+        assert len(wide["link_dist"]) > len(first["link_dist"])
+        _same(wide, want)
+        s.free()
+    finally:
+        eng.close()
+
+
 def test_batch_equals_single(engine):
     """tn93g_run_batch (one graph launch for all jobs) gives every job its single-run result.  The cached graph is
     launched again with eager runs in between, and every launch is checked."""
