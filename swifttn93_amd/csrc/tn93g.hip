@@ -3787,9 +3787,11 @@ static void job_enqueue_post(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, 
   const int64_t nrows = J.nrows;
   // the replay is enqueued first so that its long, latency-bound waves are resident before the tail's blocks fill
   // the machine; the tail runs on the side stream and is joined back before the link gather
+  // (with timing, the pair kernel's end event, just recorded on st, is the fork, and the tail's end event the join:
+  // each event record is a marker in the queue, a few microseconds of the device's time)
   hipStream_t rs = c->rside[branch];
-  (void)hipEventRecord(c->rfork[branch], st);
-  (void)hipStreamWaitEvent(rs, c->rfork[branch], 0);
+  if (!timing) (void)hipEventRecord(c->rfork[branch], st);
+  (void)hipStreamWaitEvent(rs, timing ? c->ev[1] : c->rfork[branch], 0);
   job_enqueue_replay(J, st);
   if (tail_blocks(J)) {
     if (A.c16)
@@ -3797,9 +3799,9 @@ static void job_enqueue_post(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, 
     else
       hipLaunchKernelGGL(k_tail<false>, dim3(tail_blocks(J)), dim3(256), 0, rs, A);
   }
-  if (timing) (void)hipEventRecord(c->ev[3], rs);
-  (void)hipEventRecord(c->rjoin[branch], rs);
-  (void)hipStreamWaitEvent(st, c->rjoin[branch], 0);
+  hipEvent_t join = timing ? c->ev[3] : c->rjoin[branch];
+  (void)hipEventRecord(join, rs);
+  (void)hipStreamWaitEvent(st, join, 0);
   if (J.fast) {
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const int32_t *)B->row_links.p, A.link_off, nrows);
     hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, st, A);
