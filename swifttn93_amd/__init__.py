@@ -43,6 +43,10 @@ class Result(ctypes.Structure):
                 ("tile_rows_used", ctypes.c_int32), ("folded_columns", ctypes.c_int64)]
 
 
+_LINK_PTR_OFFSET = Result.link_row.offset  # link_row .. link_dist_hi: five consecutive pointers
+assert all(getattr(Result, f).offset == _LINK_PTR_OFFSET + 8 * k
+           for k, f in enumerate(["link_row", "link_col", "link_dist", "link_dist_lo", "link_dist_hi"]))
+
 EXPORTS = ["tn93g_open", "tn93g_close", "tn93g_error", "tn93g_set_upload", "tn93g_set_free", "tn93g_run",
            "tn93g_result_free", "tn93g_version", "tn93g_run_batch", "tn93g_set_create", "tn93g_set_append",
            "tn93g_set_size", "tn93g_run_bytes", "tn93g_result_certify", "tn93g_run_noalloc"]
@@ -264,6 +268,21 @@ def _copy(ptr, n, ctype, dt):
     return np.frombuffer((ctype * n).from_address(ctypes.cast(ptr, ctypes.c_void_p).value), dtype=dt).copy()
 
 
+def _link_views(owner, r, n):
+    """The five link arrays as views of the result's link block: one array over the block, sliced, when the arrays lie
+    in the library's layout (five arrays of one capacity, one after the other), else one view per array."""
+    a = (ctypes.c_size_t * 5).from_buffer(r, _LINK_PTR_OFFSET)  # the five pointers as integers
+    cap8 = a[1] - a[0]
+    if cap8 >= 8 * n and all(a[k] - a[0] == k * cap8 for k in range(5)):
+        cap = cap8 // 8
+        blk = np.asarray(_View(owner, a[0], 5 * cap, "<i8"))
+        f = blk.view(np.float64)
+        return [blk[:n], blk[cap:cap + n], f[2 * cap:2 * cap + n], f[3 * cap:3 * cap + n], f[4 * cap:4 * cap + n]]
+    return [_arr(owner, r.link_row, n, np.int64), _arr(owner, r.link_col, n, np.int64),
+            _arr(owner, r.link_dist, n, np.float64), _arr(owner, r.link_dist_lo, n, np.float64),
+            _arr(owner, r.link_dist_hi, n, np.float64)]
+
+
 def _to_py(rp, owner):
     r = rp.contents
     n = r.n_links
@@ -272,9 +291,7 @@ def _to_py(rp, owner):
               _copy(r.link_dist, n, ctypes.c_double, np.float64), _copy(r.link_dist_lo, n, ctypes.c_double, np.float64),
               _copy(r.link_dist_hi, n, ctypes.c_double, np.float64)]
     else:
-        lk = [_arr(owner, r.link_row, n, np.int64), _arr(owner, r.link_col, n, np.int64),
-              _arr(owner, r.link_dist, n, np.float64), _arr(owner, r.link_dist_lo, n, np.float64),
-              _arr(owner, r.link_dist_hi, n, np.float64)]
+        lk = _link_views(owner, r, n)
     return {
         "exact": bool(r.exact),
         "hist": np.frombuffer(r.hist, dtype=np.uint64).reshape(3, 200).copy(),
