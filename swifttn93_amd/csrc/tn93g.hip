@@ -2683,6 +2683,16 @@ __device__ inline void hist_sum(const RunArgs &A, int k)
   A.hist[k] = v;
 }
 __global__ void k_hist_sum(RunArgs A) { hist_sum(A, blockIdx.x * blockDim.x + threadIdx.x); }
+// A single fast run's link order and histogram sum in one launch: blocks [0, nob) order the links as k_link_order,
+// the three after them sum the histogram replicas as k_hist_sum (both follow the tail; neither reads the other's data).
+__global__ __launch_bounds__(256) void k_link_order_hist(RunArgs A, unsigned nob)
+{
+  if (blockIdx.x >= nob) {
+    hist_sum(A, (int)(blockIdx.x - nob) * 256 + (int)threadIdx.x);
+    return;
+  }
+  link_order(A, A.links_flat, A.link_off, A.lslot, A.links, blockIdx.x, nob);
+}
 // the same for several jobs: grid row y is job y
 __global__ void k_hist_sum_multi(const RunArgs *__restrict__ jobs)
 {
@@ -3805,9 +3815,10 @@ static void job_enqueue_post(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, 
   if (J.fast) {
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, st, (const int32_t *)B->row_links.p, A.link_off, nrows);
     hipLaunchKernelGGL(k_link_gather, dim3(NSTRIPE), dim3(256), 0, st, A);
-    hipLaunchKernelGGL(k_link_order, dim3(order_blocks(J)), dim3(256), 0, st, A);
+    hipLaunchKernelGGL(k_link_order_hist, dim3(order_blocks(J) + 3), dim3(256), 0, st, A, order_blocks(J));
+  } else {
+    hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, st, A);
   }
-  hipLaunchKernelGGL(k_hist_sum, dim3(3), dim3(256), 0, st, A);
 }
 
 static void job_enqueue(tn93g_ctx *c, Job &J, bool timing, hipStream_t st, int branch, bool fills = true,
